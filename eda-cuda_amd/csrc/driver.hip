@@ -1,0 +1,1947 @@
+// SPDX-License-Identifier: Apache-2.0
+// The decomposition driver and the C ABI (include/eda_hip.h).
+//
+// A call runs its elements in chunks; a chunk runs the method's modes in lockstep on one HIP stream
+// (reference cpp/src/method/driver.cpp:393-700). Field work is device kernels: the cube filter,
+// selections, extrema, components, the orientation and the element's operators are in their own
+// files; the pointwise stages of the threshold, the scale field, the continuation weight, G1, the
+// sifts and the commit are device lambdas here (each() of common.hpp). The per-element control of a
+// mode (the window rule, G2's rung machine, the growth floor, the ladder, the sift guard's verdict,
+// the commit's bookkeeping) runs on the host between synchronizations, in the reference's order,
+// from per-element statistics the kernels reduce on the device.
+//
+// Per-element arrays that kernels read (windows, thresholds, run flags) are uploaded through a
+// pinned staging area into a device parameter arena, reset after every synchronization.
+#include "common.hpp"
+#include "kernels.hpp"
+#include "../../include/eda_hip.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <set>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace eda {
+
+void throw_hip(hipError_t e, const char* expr, const char* file, int line) {
+  char buf[512];
+  std::snprintf(buf, sizeof buf, "eda: HIP error %d (%s) in %s at %s:%d", static_cast<int>(e), hipGetErrorName(e), expr,
+                file, line);
+  if (e == hipErrorOutOfMemory) throw OutOfDeviceMemory(buf);
+  throw HipError(e, buf);
+}
+
+MethodCtl method_ctl(const Geom& g, bool f32) {
+  MethodCtl c;
+  std::memset(&c, 0, sizeof c);
+  c.n_max = 1;
+  for (int a = 0; a < g.dim; ++a) c.n_max = std::max(c.n_max, g.n[a]);
+  c.count = g.count;
+  c.sifts = 3;
+  c.kappa = 1.0;
+  c.beta = 2.0;
+  c.c_lo = 1.5;
+  c.c_hi = 2.0;
+  c.reset = 2.0;
+  c.q = 4.0;
+  c.span = 1.0;
+  c.growth = 2.0;
+  c.growth_from = 2;
+  c.sig_theta = 0.3;
+  c.sig_rho = 2.0;
+  c.sig_noise = 1.0;
+  c.sig_passes = 2;
+  c.absent = 3.0;
+  c.absent_jump = 2.0;
+  c.absent_reach = 1.0;
+  c.absent_lo = 1.2;
+  c.t0 = 0.2;
+  c.t1 = 0.5;
+  c.u = 2;
+  c.s = 1;
+  c.anorm_floor = 0.1;
+  c.anorm_smooth = 1;
+  c.ramp_k0 = 0.3;
+  c.ramp_k1 = 0.6;
+  c.fine_beta = 2.0;
+  c.fine_theta = 1.0;
+  c.fine_jump = 2.0;
+  c.fine_lo = 1.5;
+  c.fine_pop = 0.25;
+  c.fine_core = 2.0;
+  c.split = 1.0;
+  c.tiny = f32 ? static_cast<double>(FLT_MIN) : 1e-300;
+  return c;
+}
+
+namespace {
+
+constexpr double kRungRatio = 1.4142135623730951;
+constexpr int kMaxG2Rungs = 32;
+
+// ------------------------------------------------------------------------------------------------
+// Device memory.
+struct DevMem {
+  void* p = nullptr;
+  DevMem() = default;
+  explicit DevMem(size_t bytes) {
+    if (bytes) {
+      const hipError_t e = hipMalloc(&p, bytes);
+      if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        throw OutOfDeviceMemory("eda: the device memory does not hold the call's workspace (" +
+                                std::to_string(bytes >> 20) + " MiB more)");
+      }
+      EDA_HIP(e);
+    }
+  }
+  DevMem(const DevMem&) = delete;
+  DevMem& operator=(const DevMem&) = delete;
+  DevMem(DevMem&& o) noexcept : p(o.p) { o.p = nullptr; }
+  DevMem& operator=(DevMem&& o) noexcept {
+    std::swap(p, o.p);
+    return *this;
+  }
+  ~DevMem() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+// A bump allocator over one device allocation.
+struct Carve {
+  char* base;
+  size_t off = 0;
+  template <class V>
+  V* take(size_t n) {
+    off = (off + 255) / 256 * 256;
+    V* p = reinterpret_cast<V*>(base + off);
+    off += n * sizeof(V);
+    return p;
+  }
+};
+
+// The parameter arena: host values copied through pinned memory into device memory.
+struct Uploader {
+  char* host = nullptr;
+  DevMem dev;
+  size_t cap = 0, off = 0;
+  hipStream_t s = nullptr;
+  void init(size_t bytes, hipStream_t stream) {
+    cap = bytes;
+    s = stream;
+    EDA_HIP(hipHostMalloc(reinterpret_cast<void**>(&host), cap, hipHostMallocDefault));
+    dev = DevMem(cap);
+  }
+  ~Uploader() {
+    if (host) (void)hipHostFree(host);
+  }
+  template <class V>
+  V* put(const V* data, size_t n) {
+    const size_t bytes = n * sizeof(V);
+    const size_t a = (off + 255) / 256 * 256;
+    if (a + bytes > cap) throw std::logic_error("eda: the parameter arena is full");
+    std::memcpy(host + a, data, bytes);
+    char* d = static_cast<char*>(dev.p) + a;
+    EDA_HIP(hipMemcpyAsync(d, host + a, bytes, hipMemcpyHostToDevice, s));
+    off = a + bytes;
+    return reinterpret_cast<V*>(d);
+  }
+  template <class V>
+  V* put(const std::vector<V>& v) {
+    return put(v.data(), v.size());
+  }
+  void reset() { off = 0; }  // only after the stream is synchronized and no upload is read again
+  void reserve(size_t bytes) {
+    if (bytes <= cap) return;
+    if (host) EDA_HIP(hipHostFree(host));
+    host = nullptr;
+    init(bytes, s);
+  }
+};
+
+// The per-(thread, device) context: the stream and the parameter arena.
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Uploader up;
+  explicit Context(int dev) : device(dev) {
+    EDA_HIP(hipSetDevice(dev));
+    EDA_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    up.init(size_t{4} << 20, stream);
+  }
+  ~Context() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+Context& context(int device) {
+  thread_local std::map<int, std::unique_ptr<Context>> ctxs;
+  auto it = ctxs.find(device);
+  if (it == ctxs.end()) it = ctxs.emplace(device, std::make_unique<Context>(device)).first;
+  EDA_HIP(hipSetDevice(device));
+  return *it->second;
+}
+
+// ------------------------------------------------------------------------------------------------
+// G2's candidate rungs and confirmation machine (reference driver.cpp:59-116).
+struct G2Res {
+  int outcome = 0;  // 0 none, 1 accepted, 2 stop
+  int32_t s = 0;
+  int64_t n_p = 0;
+  double p = 0.0;
+};
+struct RungMachine {
+  std::vector<int32_t> rungs;
+  int32_t i = 0;
+  bool pending = false, done = true, accepted = false;
+  int32_t acc_index = -1;
+  G2Res cand{};
+  int32_t next_rung() const {
+    if (done) return 0;
+    if (i < static_cast<int32_t>(rungs.size())) return rungs[static_cast<size_t>(i)];
+    return odd_window(static_cast<double>(rungs.back()) * kRungRatio + 1.0);
+  }
+  void feed(const G2Res& o) {
+    if (pending) {
+      if (o.outcome == 1) {
+        done = accepted = true;
+        acc_index = i - 1;
+        return;
+      }
+      if (o.outcome == 2) {
+        done = true;
+        return;
+      }
+      pending = false;
+      ++i;
+    } else {
+      if (o.outcome == 2) {
+        done = true;
+        return;
+      }
+      if (o.outcome == 1) {
+        pending = true;
+        cand = o;
+        cand.s = rungs[static_cast<size_t>(i)];
+      }
+      ++i;
+    }
+    if (i >= static_cast<int32_t>(rungs.size()) && !pending) done = true;
+  }
+};
+RungMachine make_rung_machine(int32_t w, int32_t w_prev, double jump) {
+  RungMachine m;
+  const int32_t s0 = std::max(3, w_prev + 2);
+  for (int32_t j = 0;; ++j) {
+    const int32_t sv = odd_window(static_cast<double>(s0) * std::pow(kRungRatio, static_cast<double>(j)));
+    if (static_cast<double>(sv) * jump > static_cast<double>(w)) break;
+    if (m.rungs.empty() || sv != m.rungs.back()) m.rungs.push_back(sv);
+    if (static_cast<int32_t>(m.rungs.size()) > kMaxG2Rungs) throw std::logic_error("eda: too many G2 rungs");
+  }
+  m.done = m.rungs.empty();
+  return m;
+}
+
+// The rung ladder (reference element.cu:343-370): in double with the C library's log and pow.
+std::vector<int32_t> rung_ladder(double d_min, double d_max, int32_t w, int32_t n_max) {
+  if (!(std::isfinite(d_min) && std::isfinite(d_max))) throw std::logic_error("eda: rung_ladder: non-finite d");
+  volatile double qv = kRungRatio;
+  const double q = qv;
+  const double lq = std::log(q);
+  const double wd = static_cast<double>(w);
+  const double jl = std::floor(std::log((d_min > 1.0 ? d_min : 1.0) / wd) / lq + 1e-9);
+  const double jh = std::ceil(std::log((d_max > 1.0 ? d_max : 1.0) / wd) / lq - 1e-9);
+  if (!(jh - jl <= 4096.0)) throw std::logic_error("eda: rung_ladder: the ladder spans too many rungs");
+  const int64_t cap = 2 * static_cast<int64_t>(n_max) + 1;
+  std::set<int64_t> sides;
+  for (int64_t j = static_cast<int64_t>(jl); j <= static_cast<int64_t>(jh); ++j) {
+    const double v = wd * std::pow(q, static_cast<double>(j));
+    const int64_t side = v >= static_cast<double>(cap) + 1.0 ? cap : std::min<int64_t>(odd_window(v), cap);
+    sides.insert(side);
+  }
+  if (sides.size() > static_cast<size_t>(kMaxRungs)) throw std::logic_error("eda: the ladder exceeds 32 rungs");
+  return std::vector<int32_t>(sides.begin(), sides.end());
+}
+
+int32_t mode_bound(const MethodCtl& c) {  // reference driver.cpp:128-140
+  int32_t w_prev = 1, count = 0;
+  for (int32_t k = 0;; ++k) {
+    int64_t w = std::max(3, w_prev + 2);
+    if (c.growth > 1.0 && k >= c.growth_from) w = std::max<int64_t>(w, odd_window(c.growth * static_cast<double>(w_prev)));
+    if (w > c.n_max) break;
+    ++count;
+    w_prev = static_cast<int32_t>(w);
+  }
+  return count;
+}
+
+template <class T>
+double tie_tol(double tau, double amax) {
+  return static_cast<double>(static_cast<T>(tau) * static_cast<T>(amax));
+}
+
+// ------------------------------------------------------------------------------------------------
+// The rows of a call.
+struct Run {
+  int device = 0;
+  int dtype = 0;
+  int dim = 1;
+  int64_t n[3] = {1, 1, 1};
+  int64_t batch = 0, count = 0;
+  int32_t cap = 0;
+  int64_t chunk = 0;
+  DevMem rows;  // (batch, cap, count) in the compute dtype
+  std::vector<int32_t> n_imfs, error, windows, sifts;
+  std::vector<double> spacing, energy;
+  int32_t P = 1, K = 0;
+};
+
+// Per-element host state of a chunk.
+struct Rec {
+  int status = 0;  // 0 active, 1 done, 2 error
+  int32_t err = 0;
+  int32_t k = 0, w_prev = 1, w = 0, W = 0, p_odd = 0, kexp = 0, n_imfs = 0, n_sifts = 0;
+  bool fine = false, sifting = false;
+  double amax = 0.0, tol = 0.0, r_inf = 0.0, sigma_x = 0.0, spacing = 0.0, jump = 0.0, g3 = 1.0, wc = 0.0;
+  int32_t Wd = 0, Wl = 0;
+};
+
+// One sift's update with the guard maxima (reference sift.cu:160-199): hn = h - a mh, and per
+// element the integer maxima of the bits of |r - hn| and |hn| (non-negative doubles order as
+// integers), one atomic per wave after a wave64 reduction over all lanes.
+template <class T>
+__global__ void __launch_bounds__(kThreads) k_sift_step(int64_t count, const int32_t* __restrict__ on,
+                                                        const T* __restrict__ a, const T* __restrict__ Mn,
+                                                        const T* __restrict__ h, const T* __restrict__ r,
+                                                        T* __restrict__ hn, uint64_t* __restrict__ gmax) {
+  const int64_t b = blockIdx.y;
+  if (on[b] == 0) return;
+  uint64_t mr = 0, mh = 0;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(kThreads) + threadIdx.x; i < count;
+       i += static_cast<int64_t>(gridDim.x) * kThreads) {
+    const int64_t at = b * count + i;
+    const T m = a[at] * Mn[at];
+    const T xv = h[at] - m;
+    hn[at] = xv;
+    const uint64_t br = abs_bits(r[at] - xv);
+    const uint64_t bh = abs_bits(xv);
+    mr = br > mr ? br : mr;
+    mh = bh > mh ? bh : mh;
+  }
+  mr = wave_max_u64(mr);
+  mh = wave_max_u64(mh);
+  if ((threadIdx.x & 63) == 0) {
+    if (mr) atomicMax(reinterpret_cast<unsigned long long*>(&gmax[2 * b]), static_cast<unsigned long long>(mr));
+    if (mh) atomicMax(reinterpret_cast<unsigned long long*>(&gmax[2 * b + 1]), static_cast<unsigned long long>(mh));
+  }
+}
+
+template <class T>
+class Chunk {
+ public:
+  Chunk(Context& ctx, const Geom& g, const MethodCtl& c, double tau, bool check_finite, int32_t max_imf, T* rows,
+        int32_t cap)
+      : ctx_(ctx), g_(g), c_(c), s_(ctx.stream), tau_(tau), check_finite_(check_finite), max_imf_(max_imf),
+        rows_(rows), cap_(cap), B_(g.batch) {
+    const int64_t F = g.batch * g.count;
+    const int D = g.dim;
+    const int K = D * (D + 1) / 2;
+    nfields_ = 6 + D + 12 + D + 1 + (D >= 2 ? 3 + K : 0);
+    const size_t fbytes = static_cast<size_t>(F) * sizeof(T);
+    const size_t small = 64 * 1024 + static_cast<size_t>(B_) * (1024 + sizeof(ElemDev) + kMaxRungs * 4);
+    const size_t sel = std::max(noise_ws_bytes(g), select_ws_bytes(g));
+    const size_t bytes = (nfields_ + 1) * (fbytes + 256) + 8 * (static_cast<size_t>(F) + 256) +
+                         static_cast<size_t>(F) * 4 + 256 + small + sel + elem_stats_ws_bytes(g) + 4096;
+    mem_ = DevMem(bytes);
+    Carve cv{static_cast<char*>(mem_.p)};
+    r_ = cv.take<T>(F);
+    L_ = cv.take<T>(F);
+    Lp_ = cv.take<T>(F);
+    h_ = cv.take<T>(F);
+    t_ = cv.take<T>(F);
+    o_ = cv.take<T>(static_cast<size_t>(F) * D);
+    for (int i = 0; i < 12; ++i) X_[i] = cv.take<T>(F);
+    f1_ = cv.take<T>(F);
+    nvec_ = cv.take<T>(static_cast<size_t>(F) * D);
+    coh_ = cv.take<T>(F);
+    if (D >= 2) owork_ = cv.take<T>(static_cast<size_t>(F) * (3 + K));
+    flags_ = cv.take<uint8_t>(F);
+    ff_ = cv.take<uint8_t>(F);
+    P_[0] = cv.take<uint8_t>(F);
+    P_[1] = cv.take<uint8_t>(F);
+    Pacc_ = cv.take<uint8_t>(F);
+    A_ = cv.take<uint8_t>(F);
+    lo_ = cv.take<uint8_t>(F);
+    aw_ = cv.take<uint8_t>(F);
+    labels_ = cv.take<int32_t>(F);
+    sel_ws_ = cv.take<char>(sel);
+    stats_ws_ = cv.take<char>(elem_stats_ws_bytes(g));
+    d_sigma_n_ = cv.take<double>(B_);
+    d_e_med_ = cv.take<double>(B_);
+    d_wc_ = cv.take<double>(B_);
+    d_p_ = cv.take<double>(B_);
+    d_energy_ = cv.take<double>(B_);
+    d_amax_ = cv.take<uint64_t>(B_);
+    d_nonfin_cnt_ = cv.take<uint64_t>(B_);
+    d_ext_ = cv.take<uint64_t>(3 * B_);
+    d_np_ = cv.take<int64_t>(B_);
+    d_nonfinite_ = cv.take<int32_t>(B_);
+    d_any_ = cv.take<int32_t>(2 * B_);
+    d_dminmax_ = cv.take<uint64_t>(2 * B_);
+    d_bits_ = cv.take<uint32_t>(2 * B_);
+    d_gmax_ = cv.take<uint64_t>(2 * B_);
+    d_el_ = cv.take<ElemDev>(B_);
+    d_rung_win_ = cv.take<int32_t>(static_cast<size_t>(kMaxRungs) * B_);
+    if (cv.off > bytes) throw std::logic_error("eda: the workspace layout overflows");
+    rec_.assign(static_cast<size_t>(B_), Rec{});
+    ctx_.up.reserve((size_t{16} << 20) + static_cast<size_t>(B_) * 16384);
+  }
+
+  // Copies the input (device pointer with element strides, or contiguous host memory) into r.
+  void start(const void* x, bool on_device, const int64_t* xs) {
+    ctx_.up.reset();
+    const int64_t F = B_ * g_.count;
+    const Geom g = g_;
+    T* r = r_;
+    if (!on_device) {
+      EDA_HIP(hipMemcpyAsync(r, x, static_cast<size_t>(F) * sizeof(T), hipMemcpyHostToDevice, s_));
+    } else {
+      const T* xp = static_cast<const T*>(x);
+      const int64_t s0 = xs[0], s1 = xs[1], s2 = xs[2], s3 = xs[3];
+      each(s_, g, nullptr, [=] __device__(int64_t b, int64_t i) {
+        int64_t off = b * s0;
+        const int64_t c0 = (i / g.st[0]) % g.n[0];
+        off += c0 * s1;
+        if (g.dim >= 2) off += ((i / g.st[1]) % g.n[1]) * s2;
+        if (g.dim >= 3) off += ((i / g.st[2]) % g.n[2]) * s3;
+        r[b * g.count + i] = xp[off];
+      });
+    }
+    // non-finite samples and max |x|; the exact power-of-two prescale (reference mode.cu:43-58,271-282)
+    enqueue_elem_stats<T>(s_, g_, r_, nullptr, nullptr, d_amax_, d_nonfin_cnt_, stats_ws_);
+    std::vector<uint64_t> amax(B_), nf(B_);
+    d2h(amax.data(), d_amax_, B_);
+    d2h(nf.data(), d_nonfin_cnt_, B_);
+    sync();
+    std::vector<int32_t> kexp(B_, 0);
+    for (int64_t b = 0; b < B_; ++b) {
+      const double a = bits_d(amax[b]);
+      kexp[b] = (a > 0.0 && std::isfinite(a)) ? std::ilogb(a) : 0;
+      rec_[b].kexp = kexp[b];
+      if (nf[b] != 0 && check_finite_) {
+        rec_[b].status = 2;
+        rec_[b].err = 1;
+      }
+    }
+    const int32_t* dk = ctx_.up.put(kexp);
+    each(s_, g, nullptr, [=] __device__(int64_t b, int64_t i) {
+      const int32_t k = dk[b];
+      if (k == 0) return;
+      const int64_t at = b * g.count + i;
+      if constexpr (std::is_same_v<T, float>) r[at] = static_cast<float>(ldexp(static_cast<double>(r[at]), -k));
+      else r[at] = ldexp(r[at], -k);
+    });
+    enqueue_elem_stats<T>(s_, g_, r_, nullptr, nullptr, d_amax_, nullptr, stats_ws_);
+    d2h(amax.data(), d_amax_, B_);
+    sync();
+    for (int64_t b = 0; b < B_; ++b) {
+      rec_[b].amax = bits_d(amax[b]);
+      rec_[b].tol = tie_tol<T>(tau_, rec_[b].amax);
+    }
+    // sigma_x: the largest per-axis noise level of the prescaled input
+    const int32_t* on = upload_on([](const Rec& e) { return e.status == 0; });
+    enqueue_noise_sigma<T>(s_, g_, r_, true, d_sigma_n_, on, sel_ws_);
+    std::vector<double> sx(B_);
+    d2h(sx.data(), d_sigma_n_, B_);
+    sync();
+    for (int64_t b = 0; b < B_; ++b) rec_[b].sigma_x = rec_[b].status == 0 ? sx[b] : 0.0;
+  }
+
+  // One mode of every active element (reference driver.cpp:423-689); false when none is active.
+  bool mode();
+
+  // The residue rows and the unscale (reference field_ops.cu:157-174, mode.cu:284-302).
+  void finish(Run& run, int64_t b0) {
+    ctx_.up.reset();
+    const Geom g = g_;
+    std::vector<int32_t> row_of(B_), kx(B_);
+    for (int64_t b = 0; b < B_; ++b) {
+      row_of[b] = rec_[b].status == 2 ? -1 : rec_[b].n_imfs;
+      kx[b] = rec_[b].kexp;
+    }
+    const int32_t* drow = ctx_.up.put(row_of);
+    const int32_t* dk = ctx_.up.put(kx);
+    T* rows = rows_;
+    T* r = r_;
+    const int64_t cap = cap_;
+    int32_t* bad = d_nonfinite_;
+    EDA_HIP(hipMemsetAsync(bad, 0, sizeof(int32_t) * B_, s_));
+    each(s_, g, nullptr, [=] __device__(int64_t b, int64_t i) {
+      const int32_t nr = drow[b];
+      if (nr < 0) return;
+      rows[(b * cap + nr) * g.count + i] = r[b * g.count + i];
+    });
+    const int64_t per = cap * g.count;
+    const int64_t Bc = B_;
+    flat(s_, Bc * per, [=] __device__(int64_t idx) {
+      const int64_t b = idx / per;
+      const int32_t k = dk[b];
+      const int32_t nr = drow[b];
+      if (k == 0 || nr < 0) return;
+      if ((idx - b * per) / g.count > nr) return;
+      T v;
+      if constexpr (std::is_same_v<T, float>) v = static_cast<float>(ldexp(static_cast<double>(rows[idx]), k));
+      else v = ldexp(rows[idx], k);
+      rows[idx] = v;
+      if (!isfinite(v)) bad[b] = 1;
+    });
+    std::vector<int32_t> hb(B_);
+    d2h(hb.data(), bad, B_);
+    sync();
+    for (int64_t b = 0; b < B_; ++b) {
+      Rec& e = rec_[b];
+      const int64_t ib = b0 + b;
+      int32_t err = 0;
+      if (e.status == 2) err = e.err ? e.err : 255;
+      else if (hb[b]) err = 2;
+      else if (e.status != 1) err = 255;
+      run.error[ib] = err;
+      run.n_imfs[ib] = err ? 0 : e.n_imfs;
+      const int32_t hcap = cap_ - 1;
+      for (int32_t j = 0; j < hcap; ++j) {
+        const size_t h = static_cast<size_t>(ib) * hcap + j;
+        const bool keep = err == 0 && j < e.n_imfs;
+        run.windows[h] = keep ? hist_w_[b][j] : 0;
+        run.sifts[h] = keep ? hist_s_[b][j] : 0;
+        run.spacing[h] = keep ? hist_sp_[b][j] : 0.0;
+        run.energy[h] = keep ? std::ldexp(hist_e_[b][j], 2 * e.kexp) : 0.0;
+      }
+    }
+  }
+
+  void alloc_history() {
+    hist_w_.assign(B_, std::vector<int32_t>(cap_ - 1, 0));
+    hist_s_.assign(B_, std::vector<int32_t>(cap_ - 1, 0));
+    hist_sp_.assign(B_, std::vector<double>(cap_ - 1, 0.0));
+    hist_e_.assign(B_, std::vector<double>(cap_ - 1, 0.0));
+  }
+
+ private:
+  template <class V>
+  void d2h(V* dst, const V* src, int64_t n) {
+    EDA_HIP(hipMemcpyAsync(dst, src, sizeof(V) * n, hipMemcpyDeviceToHost, s_));
+  }
+  void sync() { EDA_HIP(hipStreamSynchronize(s_)); }
+  template <class P>
+  const int32_t* upload_on(P pred) {
+    std::vector<int32_t> on(B_);
+    for (int64_t b = 0; b < B_; ++b) on[b] = pred(rec_[b]) ? 1 : 0;
+    return ctx_.up.put(on);
+  }
+  // A window array of the elements that satisfy pred (0 elsewhere) and its maximum.
+  template <class W, class P>
+  const int32_t* upload_win(W winf, P pred, int32_t* maxw) {
+    std::vector<int32_t> w(B_, 0);
+    int32_t m = 0;
+    for (int64_t b = 0; b < B_; ++b) {
+      if (!pred(rec_[b])) continue;
+      w[b] = winf(rec_[b]);
+      m = std::max(m, w[b]);
+    }
+    *maxw = m;
+    return ctx_.up.put(w);
+  }
+  template <class F, class P>
+  const double* upload_f64(F f, P pred) {
+    std::vector<double> v(B_, 0.0);
+    for (int64_t b = 0; b < B_; ++b)
+      if (pred(rec_[b])) v[b] = f(rec_[b]);
+    return ctx_.up.put(v);
+  }
+
+  void tau_field(const int32_t* on, const int32_t* dWd, int32_t mWd, const int32_t* dWl, int32_t mWl,
+                 const double* tol, const double* floor2, T* tau, T* tmp);
+  void raw_window(const uint8_t* flags, const int32_t* win, int32_t maxw, const int32_t* on, T* om, T* tmp);
+  void envelope_mean(const T* h, T* m, T* u, T* l) {
+    const Geom g = g_;
+    enqueue_element_op<T>(s_, g_, 2, d_el_, lo_, t_, o_, d_rung_win_, rung_max_.data(), K_max_, h, u, l, f0(), f1());
+    T* mid = X_[3];
+    const int32_t* on = on_op_;
+    each(s_, g, on, [=] __device__(int64_t b, int64_t i) {
+      const int64_t at = b * g.count + i;
+      mid[at] = T(0.5) * (u[at] + l[at]);
+    });
+    enqueue_element_op<T>(s_, g_, 3, d_el_, lo_, t_, o_, d_rung_win_, rung_max_.data(), K_max_, mid, m, nullptr, f0(),
+                          f1());
+  }
+  T* f0() { return X_[4]; }
+  T* f1() { return f1_; }
+
+  Context& ctx_;
+  Geom g_;
+  MethodCtl c_;
+  hipStream_t s_;
+  double tau_;
+  bool check_finite_;
+  int32_t max_imf_;
+  T* rows_;
+  int32_t cap_;
+  int64_t B_;
+  int nfields_ = 0;
+  DevMem mem_;
+  T *r_, *L_, *Lp_, *h_, *t_, *o_, *nvec_, *coh_, *owork_ = nullptr;
+  T* X_[12];
+  T* f1_ = nullptr;
+  uint8_t *flags_, *ff_, *P_[2], *Pacc_, *A_, *lo_, *aw_;
+  int32_t* labels_;
+  char *sel_ws_, *stats_ws_;
+  double *d_sigma_n_, *d_e_med_, *d_wc_, *d_p_, *d_energy_;
+  uint64_t *d_amax_, *d_nonfin_cnt_, *d_ext_, *d_dminmax_, *d_gmax_;
+  int64_t* d_np_;
+  int32_t *d_nonfinite_, *d_any_;
+  uint32_t* d_bits_;
+  ElemDev* d_el_;
+  int32_t* d_rung_win_;
+  const int32_t* on_op_ = nullptr;
+  std::vector<int32_t> rw_;
+  std::vector<int32_t> rung_max_;
+  int K_max_ = 0;
+  std::vector<Rec> rec_;
+  std::vector<std::vector<int32_t>> hist_w_, hist_s_;
+  std::vector<std::vector<double>> hist_sp_, hist_e_;
+};
+
+// The threshold field (reference stats_fields.cu:26-69,375-389) into tau; nonfinite flags into
+// d_nonfinite_ (the caller zeroes them).
+template <class T>
+void Chunk<T>::tau_field(const int32_t* on, const int32_t* dWd, int32_t mWd, const int32_t* dWl, int32_t mWl,
+                         const double* tol, const double* floor2, T* tau, T* tmp) {
+  const Geom g = g_;
+  const T* r = r_;
+  enqueue_cube<T>(s_, g_, CubeOp::kBox, r_, tmp, nullptr, dWd, mWd);
+  each(s_, g, on, [=] __device__(int64_t b, int64_t i) {
+    const int64_t at = b * g.count + i;
+    const T det = r[at] - tmp[at];
+    tau[at] = det * det;
+  });
+  enqueue_cube<T>(s_, g_, CubeOp::kBox, tau, tau, nullptr, dWl, mWl);
+  const double theta = c_.sig_theta, k_noise = c_.sig_noise;
+  const double* sigma_n = d_sigma_n_;
+  int32_t* nonfinite = d_nonfinite_;
+  each(s_, g, on, [=] __device__(int64_t b, int64_t i) {
+    const int64_t at = b * g.count + i;
+    const T be = tau[at];
+    if (!(isfinite(be) && isfinite(sigma_n[b]))) nonfinite[b] = 1;
+    const T lev = sqrt(be > T(0) ? be : T(0));
+    const double f = k_noise * sigma_n[b];
+    const double fl = f > 0.0 ? f : 0.0;
+    const double ld = static_cast<double>(lev);
+    double t = theta * (ld > fl ? ld : fl);
+    const double tl = tol[b];
+    t = t > tl ? t : tl;
+    if (floor2 != nullptr) t = t > floor2[b] ? t : floor2[b];
+    tau[at] = static_cast<T>(t);
+  });
+}
+
+// The raw local window (reference stats_fields.cu:334-369,425-440).
+template <class T>
+void Chunk<T>::raw_window(const uint8_t* flags, const int32_t* win, int32_t maxw, const int32_t* on, T* om, T* tmp) {
+  const Geom g = g_;
+  const T two_kappa = static_cast<T>(2.0 * c_.kappa);
+  const T tiny = static_cast<T>(c_.tiny);
+  for (int a = 0; a < g.dim; ++a) {
+    each(s_, g, on, [=] __device__(int64_t b, int64_t i) {
+      const int64_t at = b * g.count + i;
+      tmp[at] = ((flags[at] >> a) & 1u) ? T(1) : T(0);
+    });
+    enqueue_cube<T>(s_, g_, CubeOp::kBox, tmp, tmp, nullptr, win, maxw);
+    const bool last = a == g.dim - 1;
+    each(s_, g, on, [=] __device__(int64_t b, int64_t i) {
+      const int64_t at = b * g.count + i;
+      const T sq = tmp[at] * tmp[at];
+      T acc = a == 0 ? sq : om[at] + sq;
+      if (last) {
+        const T dens = sqrt(acc);
+        acc = dens > T(0) ? two_kappa / (dens > tiny ? dens : tiny) : inf_t<T>();
+      }
+      om[at] = acc;
+    });
+  }
+}
+
+template <class T>
+bool Chunk<T>::mode() {
+  const Geom g = g_;
+  const MethodCtl c = c_;
+  const int D = g.dim;
+  const int64_t B = B_;
+  auto active = [](const Rec& e) { return e.status == 0; };
+  bool any = false;
+  for (const Rec& e : rec_) any = any || e.status == 0;
+  if (!any) return false;
+  ctx_.up.reset();  // the previous mode ended with a synchronization
+
+  // step 0: mode start (reference mode.cu:95-136); sigma_n
+  for (Rec& e : rec_) {
+    if (e.status != 0) continue;
+    e.r_inf = e.amax;
+    e.fine = false;
+    e.jump = c.absent_jump;
+    e.g3 = 1.0;
+    e.wc = INFINITY;
+    e.p_odd = 0;
+    e.n_sifts = 0;
+    e.sifting = true;
+    const int32_t w0 = std::max(3, e.w_prev + 2);
+    e.Wd = odd_window(c.sig_rho * static_cast<double>(w0));
+    e.Wl = odd_window(std::max(c.beta, c.sig_rho) * static_cast<double>(w0));
+  }
+  const int32_t* on = upload_on(active);
+  enqueue_noise_sigma<T>(s_, g_, r_, false, d_sigma_n_, on, sel_ws_);
+  T* tau = X_[0];
+  T* tmp = X_[1];
+  // steps 1-2: the window passes (reference mode.cu:138-187)
+  for (int32_t pass = 0; pass < c.sig_passes; ++pass) {
+    on = upload_on(active);
+    int32_t mWd, mWl;
+    const int32_t* dWd = upload_win([](const Rec& e) { return e.Wd; }, active, &mWd);
+    const int32_t* dWl = upload_win([](const Rec& e) { return e.Wl; }, active, &mWl);
+    const double* dtol = upload_f64([](const Rec& e) { return e.tol; }, active);
+    EDA_HIP(hipMemsetAsync(d_nonfinite_, 0, sizeof(int32_t) * B, s_));
+    EDA_HIP(hipMemsetAsync(d_ext_, 0, sizeof(uint64_t) * 3 * B, s_));
+    tau_field(on, dWd, mWd, dWl, mWl, dtol, nullptr, tau, tmp);
+    enqueue_sig_flags<T>(s_, g_, r_, tau, flags_, d_ext_, on, tmp);
+    std::vector<uint64_t> ext(3 * B);
+    std::vector<int32_t> nf(B);
+    d2h(ext.data(), d_ext_, 3 * B);
+    d2h(nf.data(), d_nonfinite_, B);
+    sync();
+    for (int64_t b = 0; b < B; ++b) {
+      Rec& e = rec_[b];
+      if (e.status != 0) continue;
+      if (nf[b]) {
+        e.status = 2;
+        e.err = 2;
+        continue;
+      }
+      const uint64_t N = static_cast<uint64_t>(g.count);
+      uint64_t S2 = 0;
+      bool enough = false;
+      for (int a = 0; a < D; ++a) {
+        const uint64_t ea = ext[3 * b + a];
+        S2 += ea * ea;
+        if (ea * static_cast<uint64_t>(g.n[a]) >= 3 * N) enough = true;
+      }
+      if (!enough) {
+        e.status = 1;
+        continue;
+      }
+      const double sp = static_cast<double>(N) / std::sqrt(static_cast<double>(S2));
+      const double v = (2.0 * c.kappa) * sp;
+      if (!(v < 1073741824.0)) {
+        e.status = 1;
+        continue;
+      }
+      int32_t w = 2 * static_cast<int32_t>(std::floor(std::max(v, 1.0) / 2.0)) + 1;
+      w = std::max(w, 3);
+      w = std::max(w, e.w_prev + 2);
+      if (w > c.n_max) {
+        e.status = 1;
+        continue;
+      }
+      e.spacing = sp;
+      e.w = w;
+      e.Wd = odd_window(c.sig_rho * static_cast<double>(w));
+      e.Wl = odd_window(std::max(c.beta, c.sig_rho) * static_cast<double>(w));
+    }
+  }
+  any = false;
+  for (const Rec& e : rec_) any = any || e.status == 0;
+  if (!any) return false;
+
+  // step 3: G2 (reference driver.cpp:445-509)
+  std::vector<RungMachine> mach(static_cast<size_t>(B));
+  bool any_g2 = false;
+  for (int64_t b = 0; b < B; ++b) {
+    const Rec& e = rec_[b];
+    if (e.status != 0) continue;
+    const int32_t s0 = std::max(3, e.w_prev + 2);
+    if (c.fine_jump * static_cast<double>(s0) <= static_cast<double>(e.w)) {
+      mach[b] = make_rung_machine(e.w, e.w_prev, c.fine_jump);
+      any_g2 = any_g2 || !mach[b].done;
+    }
+  }
+  const uint8_t* ffp = flags_;
+  if (any_g2) {
+    auto in_g2 = [&](int64_t b) { return rec_[b].status == 0 && !mach[b].done; };
+    std::vector<int32_t> ong(B);
+    for (int64_t b = 0; b < B; ++b) ong[b] = in_g2(b) ? 1 : 0;
+    std::vector<int32_t> g2nf(B, 0);
+    if (c.fine_theta > 0) {
+      const int32_t* don = ctx_.up.put(ong);
+      std::vector<int32_t> wd(B, 0), wl(B, 0);
+      std::vector<double> tol(B, 0.0), fl2(B, 0.0);
+      int32_t mWd = 0, mWl = 0;
+      for (int64_t b = 0; b < B; ++b) {
+        if (!ong[b]) continue;
+        wd[b] = rec_[b].Wd;
+        wl[b] = rec_[b].Wl;
+        mWd = std::max(mWd, wd[b]);
+        mWl = std::max(mWl, wl[b]);
+        tol[b] = rec_[b].tol;
+        fl2[b] = c.fine_theta * rec_[b].sigma_x;
+      }
+      const int32_t* dWd = ctx_.up.put(wd);
+      const int32_t* dWl = ctx_.up.put(wl);
+      const double* dtol = ctx_.up.put(tol);
+      const double* dfl2 = ctx_.up.put(fl2);
+      EDA_HIP(hipMemsetAsync(d_nonfinite_, 0, sizeof(int32_t) * B, s_));
+      EDA_HIP(hipMemsetAsync(d_ext_, 0, sizeof(uint64_t) * 3 * B, s_));
+      tau_field(don, dWd, mWd, dWl, mWl, dtol, dfl2, tau, tmp);
+      enqueue_sig_flags<T>(s_, g_, r_, tau, ff_, d_ext_, don, tmp);
+      d2h(g2nf.data(), d_nonfinite_, B);
+      ffp = ff_;
+    }
+    T* om = X_[2];
+    T* t1 = X_[3];
+    for (int32_t step = 0;; ++step) {
+      std::vector<int32_t> sr(B, 0);
+      bool anyr = false;
+      for (int64_t b = 0; b < B; ++b) {
+        if (rec_[b].status != 0 || mach[b].done) continue;
+        sr[b] = mach[b].next_rung();
+        anyr = true;
+      }
+      if (!anyr) break;
+      if (step > kMaxG2Rungs + 1) throw std::logic_error("eda: the G2 rung loop did not end");
+      uint8_t* P = P_[step & 1];
+      // one rung test per element (reference stats_rungs.cu:16-87,126-159)
+      std::vector<int32_t> wb(B, 0), wcore(B, 0), wr(B, 0);
+      int32_t mwb = 0, mwc = 0, mwr = 0;
+      std::vector<double> lo_s(B, 0.0);
+      for (int64_t b = 0; b < B; ++b) {
+        if (!sr[b]) continue;
+        const double sd = static_cast<double>(sr[b]);
+        wb[b] = odd_window(c.fine_beta * sd);
+        wcore[b] = odd_window(c.fine_core * sd);
+        wr[b] = odd_window((2.0 * c.absent_reach) * static_cast<double>(wb[b]));
+        mwb = std::max(mwb, wb[b]);
+        mwc = std::max(mwc, wcore[b]);
+        mwr = std::max(mwr, wr[b]);
+        lo_s[b] = c.fine_lo * sd;
+      }
+      const int32_t* don = ctx_.up.put(sr);
+      const int32_t* dwb = ctx_.up.put(wb);
+      const int32_t* dwc = ctx_.up.put(wcore);
+      const int32_t* dwr = ctx_.up.put(wr);
+      const double* dlo = ctx_.up.put(lo_s);
+      raw_window(ffp, dwb, mwb, don, om, tmp);
+      each(s_, g, don, [=] __device__(int64_t b, int64_t i) {
+        const int64_t at = b * g.count + i;
+        const bool in = static_cast<double>(om[at]) < dlo[b];
+        P[at] = in ? 1 : 0;
+        t1[at] = in ? T(1) : T(0);
+      });
+      enqueue_cube<T>(s_, g_, CubeOp::kMin, t1, t1, nullptr, dwc, mwc);
+      int32_t* any_core = d_any_;
+      int32_t* any_jump = d_any_ + B;
+      EDA_HIP(hipMemsetAsync(d_any_, 0, sizeof(int32_t) * 2 * B, s_));
+      each(s_, g, don, [=] __device__(int64_t b, int64_t i) {
+        const int64_t at = b * g.count + i;
+        if (t1[at] > T(0.5)) any_core[b] = 1;
+        t1[at] = (P[at] & 1u) ? T(1) : T(0);
+      });
+      Selection<T> sel{};
+      sel.src = SelSrc::kField;
+      sel.field = om;
+      sel.mask = P;
+      sel.bits = 1;
+      enqueue_select<T>(s_, g_, sel, 1, c.fine_pop, d_p_, d_np_, don, sel_ws_);
+      enqueue_cube<T>(s_, g_, CubeOp::kMax, t1, t1, nullptr, dwr, mwr);
+      const double reset = c.reset;
+      const double* dp = d_p_;
+      each(s_, g, don, [=] __device__(int64_t b, int64_t i) {
+        const int64_t at = b * g.count + i;
+        if (static_cast<double>(om[at]) > reset * dp[b] && t1[at] > T(0.5)) any_jump[b] = 1;
+      });
+      std::vector<double> hp(B);
+      std::vector<int64_t> hnp(B);
+      std::vector<int32_t> hany(2 * B);
+      d2h(hp.data(), d_p_, B);
+      d2h(hnp.data(), d_np_, B);
+      d2h(hany.data(), d_any_, 2 * B);
+      sync();
+      for (int64_t b = 0; b < B; ++b) {
+        if (!sr[b]) continue;
+        G2Res res;
+        res.s = sr[b];
+        res.n_p = hnp[b];
+        res.p = hp[b];
+        if (res.n_p == 0 || !hany[b]) res.outcome = 0;
+        else if (c.fine_jump * res.p > static_cast<double>(rec_[b].w)) res.outcome = 2;
+        else res.outcome = hany[B + b] ? 1 : 0;
+        mach[b].feed(res);
+      }
+    }
+    bool any_acc = false;
+    for (int64_t b = 0; b < B; ++b) {
+      if (rec_[b].status != 0 || !mach[b].accepted) continue;
+      Rec& e = rec_[b];
+      e.fine = true;
+      e.p_odd = odd_window(mach[b].cand.p);
+      EDA_HIP(hipMemcpyAsync(Pacc_ + b * g.count, P_[mach[b].acc_index & 1] + b * g.count, g.count,
+                             hipMemcpyDeviceToDevice, s_));
+      any_acc = true;
+    }
+    if (any_acc) {
+      // G3's coarse scale (reference stats_rungs.cu:90-121,161-187)
+      std::vector<int32_t> wc(B, 0), wrr(B, 0), fon(B, 0);
+      int32_t mwc = 0, mwr = 0;
+      for (int64_t b = 0; b < B; ++b) {
+        const Rec& e = rec_[b];
+        if (e.status != 0 || !e.fine) continue;
+        fon[b] = 1;
+        wc[b] = odd_window(c.beta * static_cast<double>(e.w));
+        const int32_t wf = odd_window(c.fine_beta * static_cast<double>(e.p_odd));
+        wrr[b] = odd_window((2.0 * c.absent_reach) * static_cast<double>(wf));
+        mwc = std::max(mwc, wc[b]);
+        mwr = std::max(mwr, wrr[b]);
+      }
+      const int32_t* don = ctx_.up.put(fon);
+      const int32_t* dwc = ctx_.up.put(wc);
+      const int32_t* dwr = ctx_.up.put(wrr);
+      raw_window(ffp, dwc, mwc, don, om, tmp);
+      const uint8_t* Pa = Pacc_;
+      each(s_, g, don, [=] __device__(int64_t b, int64_t i) {
+        const int64_t at = b * g.count + i;
+        t1[at] = (Pa[at] & 1u) ? T(1) : T(0);
+      });
+      enqueue_cube<T>(s_, g_, CubeOp::kMax, t1, t1, nullptr, dwr, mwr);
+      uint8_t* selm = aw_;
+      each(s_, g, don, [=] __device__(int64_t b, int64_t i) {
+        const int64_t at = b * g.count + i;
+        selm[at] = (t1[at] < T(0.5) && om[at] < inf_t<T>()) ? 1 : 0;
+      });
+      Selection<T> sel{};
+      sel.src = SelSrc::kField;
+      sel.field = om;
+      sel.mask = selm;
+      sel.bits = 1;
+      enqueue_select<T>(s_, g_, sel, 0, 0.5, d_wc_, d_np_, don, sel_ws_);
+      std::vector<double> hwc(B);
+      std::vector<int64_t> hn(B);
+      d2h(hwc.data(), d_wc_, B);
+      d2h(hn.data(), d_np_, B);
+      sync();
+      for (int64_t b = 0; b < B; ++b)
+        if (fon[b]) rec_[b].wc = hn[b] == 0 ? INFINITY : hwc[b];
+    } else {
+      sync();
+    }
+    for (int64_t b = 0; b < B; ++b)
+      if (ong[b] && g2nf[b] && rec_[b].status == 0) {
+        rec_[b].status = 2;
+        rec_[b].err = 2;
+      }
+  }
+
+  // step 4: the window (reference mode.cu:189-223)
+  for (Rec& e : rec_) {
+    if (e.status != 0) continue;
+    int32_t w = e.w;
+    if (e.fine) {
+      w = std::max(std::max(e.p_odd, e.w_prev + 2), 3);
+      e.jump = 1.0;
+    }
+    if (c.growth > 1.0 && e.k >= c.growth_from) {
+      w = std::max(w, odd_window(c.growth * static_cast<double>(e.w_prev)));
+      if (w > c.n_max) {
+        e.status = 1;
+        continue;
+      }
+    }
+    e.w = w;
+    e.W = odd_window(c.beta * static_cast<double>(w));
+    if (e.fine && std::isfinite(e.wc)) {
+      const double q = std::sqrt(e.wc / static_cast<double>(e.p_odd));
+      double f = (c.split * q) / 2.0;
+      e.g3 = f < 1.0 ? f : 1.0;
+    }
+  }
+  any = false;
+  for (const Rec& e : rec_) any = any || e.status == 0;
+  if (!any) return false;
+
+  // step 5: the raw window, the scale field, the floor, G3's scaling (reference stats_scale.cu)
+  on = upload_on(active);
+  int32_t mW;
+  const int32_t* dW = upload_win([](const Rec& e) { return e.W; }, active, &mW);
+  T* om = X_[0];
+  raw_window(flags_, dW, mW, on, om, tmp);
+  {
+    std::vector<int32_t> wv(B, 0), wn(B, 0);
+    std::vector<double> jump(B, 1.0);
+    int32_t mwn = 0;
+    for (int64_t b = 0; b < B; ++b) {
+      const Rec& e = rec_[b];
+      if (e.status != 0) continue;
+      wv[b] = e.w;
+      jump[b] = e.jump;
+      wn[b] = odd_window((2.0 * c.absent_reach) * static_cast<double>(e.W));
+      mwn = std::max(mwn, wn[b]);
+    }
+    const int32_t* dw = ctx_.up.put(wv);
+    const int32_t* dwn = ctx_.up.put(wn);
+    const double* djump = ctx_.up.put(jump);
+    T* fine = X_[2];
+    uint8_t* A = A_;
+    uint8_t* aw = aw_;
+    const double reset = c.reset, lo = c.absent_lo;
+    each(s_, g, on, [=] __device__(int64_t b, int64_t i) {
+      const int64_t at = b * g.count + i;
+      const double w = static_cast<double>(dw[b]);
+      const double o = static_cast<double>(om[at]);
+      const bool a0 = o > reset * w;
+      const bool awv = o > lo * w;
+      A[at] = a0 ? 4 : 0;
+      aw[at] = awv ? 1 : 0;
+      fine[at] = o < (lo * w) / djump[b] ? T(1) : T(0);
+    });
+    enqueue_cube<T>(s_, g_, CubeOp::kMax, fine, fine, nullptr, dwn, mwn);  // near
+    enqueue_ccl(s_, g_, aw_, 1, labels_, on);
+    uint8_t* marks = P_[0];  // free after G2 (its accepted population lives in Pacc)
+    const int32_t* labels = labels_;
+    each(s_, g, on, [=] __device__(int64_t b, int64_t i) { marks[b * g.count + i] = 0; });
+    each(s_, g, on, [=] __device__(int64_t b, int64_t i) {
+      const int64_t at = b * g.count + i;
+      if (aw[at] && fine[at] > T(0.5)) marks[b * g.count + labels[at]] = 1;
+    });
+    T* L = L_;
+    const double absent = c.absent, reset2 = c.reset, c_lo = c.c_lo, c_hi = c.c_hi;
+    each(s_, g, on, [=] __device__(int64_t b, int64_t i) {
+      const int64_t at = b * g.count + i;
+      const int32_t lab = labels[at];
+      const bool a = (A[at] & 4u) && lab >= 0 && marks[b * g.count + lab];
+      A[at] = a ? 1 : 0;
+      const double w = static_cast<double>(dw[b]);
+      double v;
+      if (a) {
+        v = absent;
+      } else {
+        const double o = static_cast<double>(om[at]);
+        const double ow = o > reset2 * w ? w : o;
+        const double lo_ = w / c_lo;
+        const double hi = c_hi * w;
+        v = ow > lo_ ? ow : lo_;
+        v = v < hi ? v : hi;
+      }
+      L[at] = log(static_cast<T>(v));
+    });
+    enqueue_cube<T>(s_, g_, CubeOp::kBox, L_, L_, nullptr, dW, mW);
+    each(s_, g, on, [=] __device__(int64_t b, int64_t i) {
+      const int64_t at = b * g.count + i;
+      L[at] = exp(L[at]);
+    });
+    // the per-voxel floor for modes k >= 1 (reference stats_scale.cu:169-201,249-268)
+    bool any_floor = false, any_fine = false;
+    for (const Rec& e : rec_) {
+      if (e.status != 0) continue;
+      any_floor = any_floor || e.k >= 1;
+      any_fine = any_fine || e.fine;
+    }
+    if (any_floor) {
+      const int32_t* fon = upload_on([](const Rec& e) { return e.status == 0 && e.k >= 1; });
+      int32_t mWf;
+      const int32_t* dWf = upload_win([](const Rec& e) { return e.W; }, [](const Rec& e) { return e.status == 0 && e.k >= 1; }, &mWf);
+      std::vector<int32_t> kk(B, 0);
+      for (int64_t b = 0; b < B; ++b) kk[b] = rec_[b].k;
+      const int32_t* dk = ctx_.up.put(kk);
+      T* chi = X_[1];
+      each(s_, g, fon, [=] __device__(int64_t b, int64_t i) {
+        const int64_t at = b * g.count + i;
+        chi[at] = (A[at] & 1u) ? T(1) : T(0);
+      });
+      enqueue_cube<T>(s_, g_, CubeOp::kBox, chi, chi, nullptr, dWf, mWf);
+      const T log_absent = static_cast<T>(std::log(c.absent));
+      const double growth = c.growth;
+      const int32_t growth_from = c.growth_from;
+      const T* Lp = Lp_;
+      each(s_, g, fon, [=] __device__(int64_t b, int64_t i) {
+        const int64_t at = b * g.count + i;
+        const T lp = Lp[at];
+        const bool grow = growth > 1.0 && dk[b] >= growth_from;
+        T fl = grow ? static_cast<T>(growth) * lp : lp + T(2);
+        const T cap = static_cast<T>(c_hi * static_cast<double>(dw[b]));
+        fl = fl < cap ? fl : cap;
+        const T x = chi[at];
+        const T e = (T(1) - x) * log(fl) + x * log_absent;
+        const T f = exp(e);
+        const T l = L[at];
+        L[at] = l >= f ? l : f;
+      });
+    }
+    if (any_fine) {
+      const int32_t* fon = upload_on([](const Rec& e) { return e.status == 0 && e.fine; });
+      const double* dg3 = upload_f64([](const Rec& e) { return e.g3; }, [](const Rec& e) { return e.status == 0 && e.fine; });
+      each(s_, g, fon, [=] __device__(int64_t b, int64_t i) {
+        const int64_t at = b * g.count + i;
+        const T v = static_cast<T>(dg3[b]) * L[at];
+        L[at] = v >= T(3) ? v : T(3);
+      });
+    }
+  }
+
+  // step 6: the orientation and the element build (reference driver.cpp:540-569; element.cu:166-216)
+  if (D >= 2) {
+    std::vector<int32_t> wv(B, 0);
+    for (int64_t b = 0; b < B; ++b)
+      if (rec_[b].status == 0) wv[b] = rec_[b].w;
+    const int32_t* dw = ctx_.up.put(wv);
+    EDA_HIP(hipMemsetAsync(d_nonfinite_, 0, sizeof(int32_t) * B, s_));
+    enqueue_orientation<T>(s_, ctx_.device, g_, r_, dw, on, nvec_, coh_, owork_, d_nonfinite_);
+  }
+  {
+    std::vector<uint64_t> init(2 * B);
+    for (int64_t b = 0; b < B; ++b) {
+      init[2 * b] = 0x7ff0000000000000ULL;
+      init[2 * b + 1] = 0;
+    }
+    const uint64_t* dinit = ctx_.up.put(init);
+    EDA_HIP(hipMemcpyAsync(d_dminmax_, dinit, sizeof(uint64_t) * 2 * B, hipMemcpyDeviceToDevice, s_));
+  }
+  enqueue_element_build<T>(s_, g_, c_, L_, nvec_, coh_, o_, on, d_dminmax_);
+  std::vector<uint64_t> dmm(2 * B);
+  std::vector<int32_t> onf(B, 0);
+  d2h(dmm.data(), d_dminmax_, 2 * B);
+  if (D >= 2) d2h(onf.data(), d_nonfinite_, B);
+  sync();
+  // step 7: the ladder, the weights, the rung windows
+  std::vector<ElemDev> el(B);
+  for (int64_t b = 0; b < B; ++b) {
+    Rec& e = rec_[b];
+    std::memset(&el[b], 0, sizeof(ElemDev));
+    if (e.status != 0) continue;
+    if (onf[b]) {
+      e.status = 2;
+      e.err = 2;
+      continue;
+    }
+    const std::vector<int32_t> lad = rung_ladder(bits_d(dmm[2 * b]), bits_d(dmm[2 * b + 1]), e.w, c.n_max);
+    el[b].n_rungs = static_cast<int32_t>(lad.size());
+    for (size_t j = 0; j < lad.size(); ++j) el[b].rung_w[j] = lad[j];
+  }
+  any = false;
+  for (const Rec& e : rec_) any = any || e.status == 0;
+  if (!any) return false;
+  on = upload_on(active);
+  {
+    const ElemDev* del = ctx_.up.put(el);
+    EDA_HIP(hipMemcpyAsync(d_el_, del, sizeof(ElemDev) * B, hipMemcpyDeviceToDevice, s_));
+  }
+  EDA_HIP(hipMemsetAsync(d_bits_, 0, sizeof(uint32_t) * 2 * B, s_));
+  enqueue_element_weights<T>(s_, g_, c_, L_, coh_, d_el_, lo_, t_, on, d_bits_);
+  std::vector<uint32_t> bits(2 * B);
+  d2h(bits.data(), d_bits_, 2 * B);
+  sync();
+  K_max_ = 0;
+  for (int64_t b = 0; b < B; ++b) {
+    if (rec_[b].status != 0) continue;
+    el[b].used = bits[2 * b];
+    el[b].pts = bits[2 * b + 1];
+    K_max_ = std::max(K_max_, el[b].n_rungs);
+  }
+  std::vector<int32_t> rw(static_cast<size_t>(K_max_) * B, 0);
+  rung_max_.assign(static_cast<size_t>(K_max_), 0);
+  for (int j = 0; j < K_max_; ++j)
+    for (int64_t b = 0; b < B; ++b) {
+      const ElemDev& e = el[b];
+      if (rec_[b].status == 0 && j < e.n_rungs && ((e.used >> j) & 1u)) {
+        rw[j * B + b] = e.rung_w[j];
+        rung_max_[j] = std::max(rung_max_[j], e.rung_w[j]);
+      }
+    }
+  {
+    const ElemDev* del = ctx_.up.put(el);
+    EDA_HIP(hipMemcpyAsync(d_el_, del, sizeof(ElemDev) * B, hipMemcpyDeviceToDevice, s_));
+    if (K_max_ > 0) {
+      const int32_t* drw = ctx_.up.put(rw);
+      EDA_HIP(hipMemcpyAsync(d_rung_win_, drw, sizeof(int32_t) * rw.size(), hipMemcpyDeviceToDevice, s_));
+    }
+  }
+  rw_ = rw;
+  on = upload_on(active);
+  on_op_ = on;
+
+  // step 8: sift 1 and the continuation weight (reference driver.cpp:606-638; sift.cu:55-85)
+  T* const* X = X_;
+  T* r = r_;
+  T* h = h_;
+  envelope_mean(r_, X[0], X[1], X[2]);
+  {
+    T* m = X[0];
+    each(s_, g, on, [=] __device__(int64_t b, int64_t i) {
+      const int64_t at = b * g.count + i;
+      h[at] = r[at] - m[at];
+    });
+  }
+  for (Rec& e : rec_)
+    if (e.status == 0) e.n_sifts = 1;
+  T* a = X[8];
+  const double* dtol = upload_f64([](const Rec& e) { return e.tol; }, active);
+  if (c.sifts > 1) {
+    T* R = X[5];
+    T* u = X[1];
+    T* l = X[2];
+    each(s_, g, on, [=] __device__(int64_t b, int64_t i) {
+      const int64_t at = b * g.count + i;
+      const T dv = u[at] - l[at];
+      R[at] = dv >= T(0) ? dv : T(0);
+    });
+    T* U = R;
+    T* pp[2] = {X[6], X[7]};
+    for (int32_t it = 0; it < c.u; ++it) {
+      T* dst = pp[it & 1];
+      enqueue_element_op<T>(s_, g_, 0, d_el_, lo_, t_, o_, d_rung_win_, rung_max_.data(), K_max_, U, dst, nullptr,
+                            f0(), f1());
+      U = dst;
+    }
+    T* gw = U == X[6] ? X[7] : X[6];
+    const T t0 = static_cast<T>(c.t0);
+    const double dd = c.t1 - c.t0;
+    const T den = static_cast<T>(dd >= 1e-12 ? dd : 1e-12);
+    const T* Uc = U;
+    each(s_, g, on, [=] __device__(int64_t b, int64_t i) {
+      const int64_t at = b * g.count + i;
+      const T Uv = Uc[at];
+      T rho = T(0);
+      if (static_cast<double>(Uv) > dtol[b]) rho = clip_np<T>(R[at] / Uv, T(0), T(1));
+      gw[at] = clip_np<T>((rho - t0) / den, T(0), T(1));
+    });
+    T* cur = gw;
+    for (int32_t it = 0; it < c.s; ++it) {
+      T* dst = a;
+      if (it + 1 < c.s) dst = cur == X[5] ? X[6] : X[5];
+      enqueue_element_op<T>(s_, g_, 3, d_el_, lo_, t_, o_, d_rung_win_, rung_max_.data(), K_max_, cur, dst, nullptr,
+                            f0(), f1());
+      cur = dst;
+    }
+    each(s_, g, on, [=] __device__(int64_t b, int64_t i) {
+      const int64_t at = b * g.count + i;
+      a[at] = clip_np<T>(a[at], T(0), T(1));
+    });
+  }
+
+  // step 9: sifts 2..S with G1 (reference driver.cpp:641-680; sift.cu:88-232)
+  for (int32_t j = 1; j < c.sifts; ++j) {
+    auto sifting = [](const Rec& e) { return e.status == 0 && e.sifting; };
+    bool anys = false;
+    for (const Rec& e : rec_) anys = anys || sifting(e);
+    if (!anys) break;
+    const int32_t* son = upload_on(sifting);
+    on_op_ = son;
+    if (j > 1) {
+      // elements that stopped sifting drop out of the operators' cubes and gathers
+      std::vector<int32_t> rw = rw_;
+      for (int jj = 0; jj < K_max_; ++jj)
+        for (int64_t b = 0; b < B; ++b)
+          if (!sifting(rec_[b])) rw[jj * B + b] = 0;
+      const int32_t* drw = ctx_.up.put(rw);
+      EDA_HIP(hipMemcpyAsync(d_rung_win_, drw, sizeof(int32_t) * rw.size(), hipMemcpyDeviceToDevice, s_));
+    }
+    T* plain = X[0];
+    T* up = X[1];
+    T* lo = X[2];
+    envelope_mean(h, plain, up, lo);
+    T* hd = X[5];
+    each(s_, g, son, [=] __device__(int64_t b, int64_t i) {
+      const int64_t at = b * g.count + i;
+      hd[at] = T(0.5) * (up[at] - lo[at]);
+    });
+    T* e0 = X[6];
+    enqueue_element_op<T>(s_, g_, 3, d_el_, lo_, t_, o_, d_rung_win_, rung_max_.data(), K_max_, hd, e0, nullptr, f0(),
+                          f1());
+    int32_t mWs;
+    const int32_t* dWs = upload_win([](const Rec& e) { return e.W; }, sifting, &mWs);
+    const double* dtol = upload_f64([](const Rec& e) { return e.tol; }, sifting);
+    T* emax = X[5];
+    T* emin = X[7];
+    enqueue_cube<T>(s_, g_, CubeOp::kMaxMin, e0, emax, emin, dWs, mWs);
+    {
+      const T k0 = static_cast<T>(c.ramp_k0);
+      const double dd = c.ramp_k1 - c.ramp_k0;
+      const T kden = static_cast<T>(dd >= 1e-12 ? dd : 1e-12);
+      const double tiny = c.tiny;
+      each(s_, g, son, [=] __device__(int64_t b, int64_t i) {
+        const int64_t at = b * g.count + i;
+        const double tl = dtol[b];
+        const T fl = static_cast<T>(tl >= tiny ? tl : tiny);
+        const T hi = emax[at], lw = emin[at];
+        const T sum = hi + lw;
+        const T den = sum >= fl ? sum : fl;
+        const T kap = (hi - lw) / den;
+        emax[at] = clip_np<T>((kap - k0) / kden, T(0), T(1));
+      });
+    }
+    T* gk = X[7];
+    enqueue_cube<T>(s_, g_, CubeOp::kBox, emax, gk, nullptr, dWs, mWs);
+    each(s_, g, son, [=] __device__(int64_t b, int64_t i) {
+      const int64_t at = b * g.count + i;
+      gk[at] = clip_np<T>(gk[at], T(0), T(1));
+    });
+    T* e = X[9];
+    if (c.anorm_smooth <= 0) {
+      EDA_HIP(hipMemcpyAsync(e, e0, sizeof(T) * B * g.count, hipMemcpyDeviceToDevice, s_));
+    }
+    for (int32_t it = 0; it < c.anorm_smooth; ++it) enqueue_cube<T>(s_, g_, CubeOp::kBox, it ? e : e0, e, nullptr, dWs, mWs);
+    {
+      Selection<T> sel{};
+      sel.src = SelSrc::kField;
+      sel.field = e;
+      enqueue_select<T>(s_, g_, sel, 0, 0.5, d_e_med_, nullptr, son, sel_ws_);
+      const double* emed = d_e_med_;
+      const double floor_k = c.anorm_floor, tiny = c.tiny;
+      each(s_, g, son, [=] __device__(int64_t b, int64_t i) {
+        const int64_t at = b * g.count + i;
+        double thr = floor_k * emed[b];
+        const double tl = dtol[b];
+        thr = thr >= tl ? thr : tl;
+        thr = thr >= tiny ? thr : tiny;
+        const T ev = e[at];
+        e[at] = static_cast<double>(ev) >= thr ? ev : static_cast<T>(thr);
+      });
+    }
+    T* q = X[6];
+    each(s_, g, son, [=] __device__(int64_t b, int64_t i) {
+      const int64_t at = b * g.count + i;
+      q[at] = h[at] / e[at];
+    });
+    T* Mn = X[5];
+    envelope_mean(q, Mn, X[10], X[11]);
+    each(s_, g, son, [=] __device__(int64_t b, int64_t i) {
+      const int64_t at = b * g.count + i;
+      const T wv = gk[at];
+      const T mA = clip_np<T>(e[at] * Mn[at], lo[at], up[at]);
+      Mn[at] = wv * mA + (T(1) - wv) * plain[at];
+    });
+    // the sift with the sup-norm guard: hn = h - a mh; accepted iff max|r - hn| <= r_inf and
+    // max|hn| <= r_inf (reference sift.cu:160-232)
+    T* hn = X[6];
+    uint64_t* gmax = d_gmax_;
+    EDA_HIP(hipMemsetAsync(gmax, 0, sizeof(uint64_t) * 2 * B, s_));
+    {
+      int64_t gx = (g.count + kThreads - 1) / kThreads;
+      gx = gx > 1024 ? 1024 : gx;
+      hipLaunchKernelGGL(k_sift_step<T>, dim3(static_cast<unsigned>(gx), static_cast<unsigned>(B)), dim3(kThreads), 0,
+                         s_, g.count, son, a, Mn, h, r, hn, gmax);
+      EDA_LAUNCH("k_sift_step");
+    }
+    const double* drinf = upload_f64([](const Rec& e) { return e.r_inf; }, sifting);
+    each(s_, g, son, [=] __device__(int64_t b, int64_t i) {
+      const bool ok = !(bits_d(gmax[2 * b]) > drinf[b]) && !(bits_d(gmax[2 * b + 1]) > drinf[b]);
+      if (ok) {
+        const int64_t at = b * g.count + i;
+        h[at] = hn[at];
+      }
+    });
+    std::vector<uint64_t> gm(2 * B);
+    d2h(gm.data(), gmax, 2 * B);
+    sync();
+    for (int64_t b = 0; b < B; ++b) {
+      Rec& er = rec_[b];
+      if (!sifting(er)) continue;
+      const bool ok = !(bits_d(gm[2 * b]) > er.r_inf) && !(bits_d(gm[2 * b + 1]) > er.r_inf);
+      if (ok) ++er.n_sifts;
+      else er.sifting = false;
+    }
+  }
+
+  // step 10: the commit (reference mode.cu:306-348,420-430; field_ops.cu:103-154)
+  {
+    on = upload_on(active);
+    std::vector<int32_t> row(B, 0);
+    for (int64_t b = 0; b < B; ++b) row[b] = rec_[b].n_imfs;
+    const int32_t* drow = ctx_.up.put(row);
+    T* rows = rows_;
+    const int64_t cap = cap_;
+    each(s_, g, on, [=] __device__(int64_t b, int64_t i) {
+      const int64_t at = b * g.count + i;
+      const T hv = h[at];
+      rows[(b * cap + drow[b]) * g.count + i] = hv;
+      r[at] = r[at] - hv;
+    });
+    enqueue_elem_stats<T>(s_, g_, h_, on, d_energy_, nullptr, nullptr, stats_ws_);
+    std::vector<double> en(B);
+    d2h(en.data(), d_energy_, B);
+    enqueue_elem_stats<T>(s_, g_, r_, on, nullptr, d_amax_, nullptr, stats_ws_);
+    std::vector<uint64_t> am(B);
+    d2h(am.data(), d_amax_, B);
+    sync();
+    for (int64_t b = 0; b < B; ++b) {
+      Rec& e = rec_[b];
+      if (e.status != 0) continue;
+      const int32_t k = e.n_imfs;
+      hist_w_[b][k] = e.w;
+      hist_e_[b][k] = en[b];
+      hist_sp_[b][k] = e.spacing;
+      hist_s_[b][k] = e.n_sifts;
+      e.n_imfs = k + 1;
+      e.amax = bits_d(am[b]);
+      e.tol = tie_tol<T>(tau_, e.amax);
+      if (max_imf_ > 0 && e.n_imfs >= max_imf_) e.status = 1;
+      if (e.n_imfs + 1 >= cap_) throw std::logic_error("eda: the output rows are full (mode bound violated)");
+      e.w_prev = e.w;
+      e.k += 1;
+      if (!std::isfinite(e.amax)) {
+        e.status = 2;
+        e.err = 2;
+      }
+    }
+  }
+  std::swap(L_, Lp_);
+  any = false;
+  for (const Rec& e : rec_) any = any || e.status == 0;
+  return any;
+}
+
+// ------------------------------------------------------------------------------------------------
+template <class T>
+void run_call(const eda_array_desc& x, const eda_params& p, Run& run) {
+  Context& ctx = context(run.device);
+  if (p.stream) {
+    hipEvent_t ev;
+    EDA_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    EDA_HIP(hipEventRecord(ev, reinterpret_cast<hipStream_t>(p.stream)));
+    EDA_HIP(hipStreamWaitEvent(ctx.stream, ev, 0));
+    EDA_HIP(hipEventDestroy(ev));
+  }
+  const Geom gall = make_geom(x.dim, x.batch, x.n);
+  MethodCtl c = method_ctl(gall, std::is_same_v<T, float>);
+  c.sifts = p.sifts;
+  const int32_t bound = mode_bound(c);
+  const int32_t modes_max = p.max_imf > 0 ? std::min(bound, p.max_imf) : bound;
+  run.cap = modes_max + 2;
+  const double tau = p.tie_tolerance >= 0 ? p.tie_tolerance : (std::is_same_v<T, float> ? std::ldexp(1.0, -14) : std::ldexp(1.0, -30));
+  const int64_t B = x.batch;
+  run.rows = DevMem(static_cast<size_t>(B) * run.cap * gall.count * sizeof(T));
+  const size_t he = static_cast<size_t>(B) * (run.cap - 1);
+  run.windows.assign(he, 0);
+  run.sifts.assign(he, 0);
+  run.spacing.assign(he, 0.0);
+  run.energy.assign(he, 0.0);
+  run.n_imfs.assign(B, 0);
+  run.error.assign(B, 0);
+  // the chunk: bounded by max_chunk and by the free device memory (halving)
+  int64_t Bc = B;
+  if (p.max_chunk > 0) Bc = std::min<int64_t>(Bc, p.max_chunk);
+  size_t free_b = 0, total_b = 0;
+  EDA_HIP(hipMemGetInfo(&free_b, &total_b));
+  const size_t per_elem = static_cast<size_t>(gall.count) * (sizeof(T) * 40 + 16) + 4096;
+  while (Bc > 1 && static_cast<size_t>(Bc) * per_elem + (size_t{512} << 20) > free_b) Bc = (Bc + 1) / 2;
+  run.chunk = Bc;
+  for (int64_t b0 = 0; b0 < B; b0 += Bc) {
+    const int64_t nb = std::min(Bc, B - b0);
+    const Geom g = make_geom(x.dim, nb, x.n);
+    T* rows = static_cast<T*>(run.rows.p) + static_cast<size_t>(b0) * run.cap * g.count;
+    Chunk<T> ch(ctx, g, c, tau, p.check_finite != 0, p.max_imf, rows, run.cap);
+    ch.alloc_history();
+    if (x.device < 0) {
+      ch.start(static_cast<const T*>(x.data) + b0 * g.count, false, nullptr);
+    } else {
+      const int64_t xs[4] = {x.batch_stride, x.strides[0], x.strides[1], x.strides[2]};
+      ch.start(static_cast<const T*>(x.data) + b0 * x.batch_stride, true, xs);
+    }
+    while (ch.mode()) {
+    }
+    ch.finish(run, b0);
+  }
+  EDA_HIP(hipStreamSynchronize(ctx.stream));
+  int32_t P = 0, K = 0;
+  bool any_err = false;
+  for (int64_t b = 0; b < B; ++b) {
+    if (run.error[b]) any_err = true;
+    else P = std::max(P, run.n_imfs[b] + 1);
+    K = std::max(K, run.n_imfs[b]);
+  }
+  if (any_err) P = std::max(P, 1);
+  run.P = P;
+  run.K = K;
+}
+
+// ------------------------------------------------------------------------------------------------
+// The C ABI plumbing.
+void set_err(char* err, size_t len, const std::string& msg) {
+  if (err == nullptr || len == 0) return;
+  std::snprintf(err, len, "%s", msg.c_str());
+}
+template <class F>
+int guarded(char* err, size_t len, F&& f) {
+  try {
+    f();
+    return EDA_OK;
+  } catch (const OutOfDeviceMemory& e) {
+    set_err(err, len, e.what());
+    return EDA_ENOMEM;
+  } catch (const std::bad_alloc&) {
+    set_err(err, len, "eda: host memory exhausted");
+    return EDA_ENOMEM;
+  } catch (const HipError& e) {
+    set_err(err, len, e.what());
+    return EDA_EHIP;
+  } catch (const std::invalid_argument& e) {
+    set_err(err, len, e.what());
+    return EDA_EINVAL;
+  } catch (const std::exception& e) {
+    set_err(err, len, e.what());
+    return EDA_EINTERNAL;
+  }
+}
+
+void validate(const eda_array_desc& x, const eda_params& p) {
+  if (x.dim < 1 || x.dim > 3) throw std::invalid_argument("eda: D must be 1, 2 or 3");
+  if (x.dtype != EDA_F32 && x.dtype != EDA_F64) throw std::invalid_argument("eda: the input must be float32 or float64");
+  const int64_t lim = x.dtype == EDA_F32 ? 8192 : 4096;
+  for (int a = 0; a < x.dim; ++a) {
+    if (x.n[a] < 1) throw std::invalid_argument("eda: every spatial extent must be >= 1");
+    if (x.n[a] > lim) {
+      throw std::invalid_argument("eda: axis extent " + std::to_string(x.n[a]) + " exceeds the limit " +
+                                  std::to_string(lim) + " for " + (x.dtype == EDA_F32 ? "float32" : "float64"));
+    }
+  }
+  for (int a = 0; a < x.dim; ++a) {
+    if (x.n[a] < 2) {
+      if (x.dim == 1) {
+        throw std::invalid_argument(
+            "eda: a 1D signal needs at least 2 samples, got 1 (one sample has no sample differences, so no noise "
+            "level; it is its own residue)");
+      }
+      throw std::invalid_argument("eda: every spatial axis of a " + std::to_string(x.dim) +
+                                  "D field needs extent >= 2, but spatial axis " + std::to_string(a) +
+                                  " has extent 1; drop that axis (squeeze it, or leave it out of axes=) to decompose "
+                                  "the field in " + std::to_string(x.dim - 1) + "D");
+    }
+  }
+  int64_t count = 1;
+  for (int a = 0; a < x.dim; ++a) count *= x.n[a];
+  if (count > (int64_t{1} << 31)) throw std::invalid_argument("eda: an element has more than 2^31 samples");
+  if (x.batch < 0) throw std::invalid_argument("eda: the batch must be >= 0");
+  if (p.sifts < 1) throw std::invalid_argument("eda: sifts must be >= 1");
+  if (p.max_chunk < 0) throw std::invalid_argument("eda: max_chunk must be >= 0");
+  if (p.orientation) throw std::invalid_argument("eda: return_orientation is not supported by this build");
+  if (p.tie_tolerance >= 0 && !std::isfinite(p.tie_tolerance))
+    throw std::invalid_argument("eda: tie_tolerance must be a finite value >= 0");
+  if (x.batch > 0 && x.data == nullptr) throw std::invalid_argument("eda: the input has no data pointer");
+  if (x.device >= 0 && reinterpret_cast<uintptr_t>(x.data) % (x.dtype == EDA_F32 ? 4 : 8) != 0)
+    throw std::invalid_argument("eda: the device pointer is not aligned to its item size");
+}
+
+template <class T>
+void collect_rows(const Run& run, void* dst, int32_t dst_device) {
+  Context& ctx = context(run.device);
+  const int64_t B = run.batch, count = run.count;
+  const int32_t P = run.P, cap = run.cap;
+  if (B == 0 || P == 0) return;
+  const size_t elems = static_cast<size_t>(B) * P * count;
+  DevMem tmp;
+  T* out;
+  if (dst_device >= 0) {
+    out = static_cast<T*>(dst);
+  } else {
+    tmp = DevMem(elems * sizeof(T));
+    out = static_cast<T*>(tmp.p);
+  }
+  const int32_t* dn = ctx.up.put(run.n_imfs);
+  const int32_t* de = ctx.up.put(run.error);
+  const T* src = static_cast<const T*>(run.rows.p);
+  flat(ctx.stream, static_cast<int64_t>(elems), [=] __device__(int64_t idx) {
+    const int64_t i = idx % count;
+    const int64_t bk = idx / count;
+    const int64_t k = bk % P;
+    const int64_t b = bk / P;
+    T v;
+    if (de[b] != 0) v = static_cast<T>(__longlong_as_double(0x7ff8000000000000LL));
+    else if (k <= dn[b]) v = src[(b * cap + k) * count + i];
+    else v = T(0);
+    out[idx] = v;
+  });
+  if (dst_device < 0) EDA_HIP(hipMemcpyAsync(dst, out, elems * sizeof(T), hipMemcpyDeviceToHost, ctx.stream));
+  EDA_HIP(hipStreamSynchronize(ctx.stream));
+  ctx.up.reset();
+}
+
+int current_device_of(const eda_array_desc& x) {
+  if (x.device >= 0) return x.device;
+  int d = 0;
+  EDA_HIP(hipGetDevice(&d));
+  return d;
+}
+
+template <class T>
+struct PrimGeom {
+  Geom g;
+  PrimGeom(int32_t dim, int64_t batch, const int64_t* n) {
+    if (dim < 1 || dim > 3 || batch < 1) throw std::invalid_argument("eda: prim: bad geometry");
+    g = make_geom(dim, batch, n);
+  }
+};
+
+}  // namespace
+}  // namespace eda
+
+// ================================================================================================
+using namespace eda;
+
+extern "C" {
+
+int eda_decompose(const eda_array_desc* x, const eda_params* p, eda_result* out, char* err, size_t errlen) {
+  return guarded(err, errlen, [&] {
+    if (x == nullptr || p == nullptr || out == nullptr) throw std::invalid_argument("eda: null argument");
+    std::memset(out, 0, sizeof *out);
+    validate(*x, *p);
+    auto run = std::make_unique<Run>();
+    run->device = current_device_of(*x);
+    run->dtype = x->dtype;
+    run->dim = x->dim;
+    run->batch = x->batch;
+    run->count = 1;
+    for (int a = 0; a < x->dim; ++a) {
+      run->n[a] = x->n[a];
+      run->count *= x->n[a];
+    }
+    if (x->batch > 0) {
+      EDA_HIP(hipSetDevice(run->device));
+      if (x->dtype == EDA_F32) run_call<float>(*x, *p, *run);
+      else run_call<double>(*x, *p, *run);
+    } else {
+      run->P = 1;
+    }
+    out->batch = run->batch;
+    out->count = run->count;
+    out->rows = run->P;
+    out->K = run->K;
+    out->dtype = run->dtype;
+    out->device = run->device;
+    out->chunk = run->chunk;
+    const int64_t B = run->batch;
+    const int32_t K = run->K;
+    const int32_t hc = std::max(run->cap - 1, 0);
+    out->n_imfs = static_cast<int32_t*>(std::calloc(std::max<int64_t>(B, 1), sizeof(int32_t)));
+    out->error = static_cast<int32_t*>(std::calloc(std::max<int64_t>(B, 1), sizeof(int32_t)));
+    const size_t nk = static_cast<size_t>(std::max<int64_t>(B * K, 1));
+    out->windows = static_cast<int32_t*>(std::calloc(nk, sizeof(int32_t)));
+    out->sifts = static_cast<int32_t*>(std::calloc(nk, sizeof(int32_t)));
+    out->spacing = static_cast<double*>(std::calloc(nk, sizeof(double)));
+    out->energy = static_cast<double*>(std::calloc(nk, sizeof(double)));
+    if (!out->n_imfs || !out->error || !out->windows || !out->sifts || !out->spacing || !out->energy) throw std::bad_alloc();
+    for (int64_t b = 0; b < B; ++b) {
+      out->n_imfs[b] = run->n_imfs[b];
+      out->error[b] = run->error[b];
+      for (int32_t k = 0; k < K && k < hc; ++k) {
+        out->windows[b * K + k] = run->windows[b * hc + k];
+        out->sifts[b * K + k] = run->sifts[b * hc + k];
+        out->spacing[b * K + k] = run->spacing[b * hc + k];
+        out->energy[b * K + k] = run->energy[b * hc + k];
+      }
+    }
+    out->handle = run.release();
+  });
+}
+
+int eda_collect(const eda_result* r, void* dst, int32_t dst_device, char* err, size_t errlen) {
+  return guarded(err, errlen, [&] {
+    if (r == nullptr || r->handle == nullptr) throw std::invalid_argument("eda: collect: no result");
+    const Run& run = *static_cast<const Run*>(r->handle);
+    if (run.batch == 0) return;
+    if (dst == nullptr) throw std::invalid_argument("eda: collect: null destination");
+    if (run.dtype == EDA_F32) collect_rows<float>(run, dst, dst_device);
+    else collect_rows<double>(run, dst, dst_device);
+  });
+}
+
+void eda_release(eda_result* r) {
+  if (r == nullptr) return;
+  delete static_cast<Run*>(r->handle);
+  std::free(r->n_imfs);
+  std::free(r->error);
+  std::free(r->windows);
+  std::free(r->sifts);
+  std::free(r->spacing);
+  std::free(r->energy);
+  std::memset(r, 0, sizeof *r);
+}
+
+int eda_build_info(char* buf, size_t len) {
+  std::snprintf(buf, len,
+                "{\"version\": \"0.1.0\", \"backend\": \"hip\", \"architectures\": [\"gfx950\"], "
+                "\"hip_version\": %d, \"compiler\": \"%s\"}",
+                HIP_VERSION, __VERSION__);
+  return EDA_OK;
+}
+
+int eda_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+int eda_device_info(int32_t device, char* buf, size_t len) {
+  return guarded(buf, len, [&] {
+    hipDeviceProp_t pr;
+    EDA_HIP(hipGetDeviceProperties(&pr, device));
+    std::snprintf(buf, len,
+                  "{\"name\": \"%s\", \"gcn_arch\": \"%s\", \"multiprocessors\": %d, \"total_memory\": %zu, "
+                  "\"lds_per_block\": %zu, \"warp_size\": %d}",
+                  pr.name, pr.gcnArchName, pr.multiProcessorCount, pr.totalGlobalMem, pr.sharedMemPerBlock, pr.warpSize);
+  });
+}
+
+int eda_pointer_device(const void* ptr, int32_t* device) {
+  return guarded(nullptr, 0, [&] {
+    hipPointerAttribute_t attr;
+    const hipError_t e = hipPointerGetAttributes(&attr, ptr);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      *device = -1;
+      return;
+    }
+    *device = attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged ? attr.device : -1;
+  });
+}
+
+}  // extern "C"
+
+// ---- primitives ---------------------------------------------------------------------------------
+namespace {
+template <class F>
+int prim(int32_t dtype, char* err, size_t errlen, F&& f) {
+  return guarded(err, errlen, [&] {
+    if (dtype != EDA_F32 && dtype != EDA_F64) throw std::invalid_argument("eda: dtype must be 0 or 1");
+    int dev = 0;
+    EDA_HIP(hipGetDevice(&dev));
+    Context& ctx = context(dev);
+    if (dtype == EDA_F32) f(float{}, ctx);
+    else f(double{}, ctx);
+    EDA_HIP(hipStreamSynchronize(ctx.stream));
+    ctx.up.reset();
+  });
+}
+}  // namespace
+
+extern "C" {
+
+int eda_prim_cube(int32_t dtype, int32_t dim, int64_t batch, const int64_t* n, int32_t op, const int32_t* win,
+                  const void* in, void* out0, void* out1, char* err, size_t errlen) {
+  return prim(dtype, err, errlen, [&](auto tv, Context& ctx) {
+    using T = decltype(tv);
+    const Geom g = PrimGeom<T>(dim, batch, n).g;
+    if (op < 0 || op > 3) throw std::invalid_argument("eda: cube op must be 0..3");
+    std::vector<int32_t> w(win, win + batch);
+    int32_t mw = 0;
+    for (int32_t v : w) {
+      if (v != 0 && (v < 1 || v % 2 == 0)) throw std::invalid_argument("eda: windows must be odd (or 0)");
+      mw = std::max(mw, v);
+    }
+    const int32_t* dw = ctx.up.put(w);
+    enqueue_cube<T>(ctx.stream, g, static_cast<CubeOp>(op), static_cast<const T*>(in), static_cast<T*>(out0),
+                    static_cast<T*>(out1), dw, mw);
+  });
+}
+
+int eda_prim_stats(int32_t dtype, int32_t dim, int64_t batch, const int64_t* n, const void* f, double* energy,
+                   double* amax, uint64_t* nonfinite, char* err, size_t errlen) {
+  return prim(dtype, err, errlen, [&](auto tv, Context& ctx) {
+    using T = decltype(tv);
+    const Geom g = PrimGeom<T>(dim, batch, n).g;
+    DevMem ws(elem_stats_ws_bytes(g) + 3 * 8 * batch + 1024);
+    double* de = static_cast<double*>(ws.p);
+    uint64_t* da = reinterpret_cast<uint64_t*>(de + batch);
+    uint64_t* dn = da + batch;
+    char* rows = reinterpret_cast<char*>(dn + batch);
+    enqueue_elem_stats<T>(ctx.stream, g, static_cast<const T*>(f), nullptr, de, da, dn, rows);
+    std::vector<uint64_t> a(batch);
+    EDA_HIP(hipMemcpyAsync(energy, de, 8 * batch, hipMemcpyDeviceToHost, ctx.stream));
+    EDA_HIP(hipMemcpyAsync(a.data(), da, 8 * batch, hipMemcpyDeviceToHost, ctx.stream));
+    EDA_HIP(hipMemcpyAsync(nonfinite, dn, 8 * batch, hipMemcpyDeviceToHost, ctx.stream));
+    EDA_HIP(hipStreamSynchronize(ctx.stream));
+    for (int64_t b = 0; b < batch; ++b) amax[b] = bits_d(a[b]);
+  });
+}
+
+int eda_prim_select(int32_t dtype, int32_t dim, int64_t batch, const int64_t* n, const void* field,
+                    const uint8_t* mask, int32_t bits, int32_t mode, double q, double* out, int64_t* count, char* err,
+                    size_t errlen) {
+  return prim(dtype, err, errlen, [&](auto tv, Context& ctx) {
+    using T = decltype(tv);
+    const Geom g = PrimGeom<T>(dim, batch, n).g;
+    if (mode != 0 && !(q >= 0.0 && q <= 1.0)) throw std::invalid_argument("eda: quantile: q must lie in [0, 1]");
+    DevMem ws(select_ws_bytes(g) + 16 * batch + 1024);
+    double* dout = reinterpret_cast<double*>(static_cast<char*>(ws.p) + select_ws_bytes(g));
+    int64_t* dcnt = reinterpret_cast<int64_t*>(dout + batch);
+    Selection<T> sel{};
+    sel.src = SelSrc::kField;
+    sel.field = static_cast<const T*>(field);
+    sel.mask = mask;
+    sel.bits = static_cast<uint8_t>(bits);
+    enqueue_select<T>(ctx.stream, g, sel, mode, q, dout, dcnt, nullptr, ws.p);
+    EDA_HIP(hipMemcpyAsync(out, dout, 8 * batch, hipMemcpyDeviceToHost, ctx.stream));
+    if (count) EDA_HIP(hipMemcpyAsync(count, dcnt, 8 * batch, hipMemcpyDeviceToHost, ctx.stream));
+    EDA_HIP(hipStreamSynchronize(ctx.stream));
+  });
+}
+
+int eda_prim_noise(int32_t dtype, int32_t dim, int64_t batch, const int64_t* n, const void* r, int32_t per_axis,
+                   double* out, char* err, size_t errlen) {
+  return prim(dtype, err, errlen, [&](auto tv, Context& ctx) {
+    using T = decltype(tv);
+    const Geom g = PrimGeom<T>(dim, batch, n).g;
+    DevMem ws(noise_ws_bytes(g) + 8 * batch + 1024);
+    double* dout = reinterpret_cast<double*>(static_cast<char*>(ws.p) + noise_ws_bytes(g));
+    enqueue_noise_sigma<T>(ctx.stream, g, static_cast<const T*>(r), per_axis != 0, dout, nullptr, ws.p);
+    EDA_HIP(hipMemcpyAsync(out, dout, 8 * batch, hipMemcpyDeviceToHost, ctx.stream));
+    EDA_HIP(hipStreamSynchronize(ctx.stream));
+  });
+}
+
+int eda_prim_tau(int32_t dtype, int32_t dim, int64_t batch, const int64_t* n, const void* r, const int32_t* Wd,
+                 const int32_t* Wl, const double* sigma_n, const double* tol, void* tau, int32_t* nonfinite, char* err,
+                 size_t errlen) {
+  return prim(dtype, err, errlen, [&](auto tv, Context& ctx) {
+    using T = decltype(tv);
+    const Geom g = PrimGeom<T>(dim, batch, n).g;
+    const MethodCtl c = method_ctl(g, std::is_same_v<T, float>);
+    DevMem tmp(static_cast<size_t>(g.batch * g.count) * sizeof(T) + 4 * batch + 1024);
+    T* dtmp = static_cast<T*>(tmp.p);
+    int32_t* dnf = reinterpret_cast<int32_t*>(dtmp + g.batch * g.count);
+    std::vector<int32_t> wd(Wd, Wd + batch), wl(Wl, Wl + batch);
+    const int32_t mWd = *std::max_element(wd.begin(), wd.end()), mWl = *std::max_element(wl.begin(), wl.end());
+    const int32_t* dWd = ctx.up.put(wd);
+    const int32_t* dWl = ctx.up.put(wl);
+    const double* dsig = ctx.up.put(sigma_n, batch);
+    const double* dtol = ctx.up.put(tol, batch);
+    EDA_HIP(hipMemsetAsync(dnf, 0, 4 * batch, ctx.stream));
+    const T* rp = static_cast<const T*>(r);
+    T* tp = static_cast<T*>(tau);
+    enqueue_cube<T>(ctx.stream, g, CubeOp::kBox, rp, dtmp, nullptr, dWd, mWd);
+    each(ctx.stream, g, nullptr, [=] __device__(int64_t b, int64_t i) {
+      const int64_t at = b * g.count + i;
+      const T det = rp[at] - dtmp[at];
+      tp[at] = det * det;
+    });
+    enqueue_cube<T>(ctx.stream, g, CubeOp::kBox, tp, tp, nullptr, dWl, mWl);
+    const double theta = c.sig_theta, k_noise = c.sig_noise;
+    each(ctx.stream, g, nullptr, [=] __device__(int64_t b, int64_t i) {
+      const int64_t at = b * g.count + i;
+      const T be = tp[at];
+      if (!(isfinite(be) && isfinite(dsig[b]))) dnf[b] = 1;
+      const T lev = sqrt(be > T(0) ? be : T(0));
+      const double f = k_noise * dsig[b];
+      const double fl = f > 0.0 ? f : 0.0;
+      const double ld = static_cast<double>(lev);
+      double t = theta * (ld > fl ? ld : fl);
+      t = t > dtol[b] ? t : dtol[b];
+      tp[at] = static_cast<T>(t);
+    });
+    EDA_HIP(hipMemcpyAsync(nonfinite, dnf, 4 * batch, hipMemcpyDeviceToHost, ctx.stream));
+    EDA_HIP(hipStreamSynchronize(ctx.stream));
+  });
+}
+
+int eda_prim_flags(int32_t dtype, int32_t dim, int64_t batch, const int64_t* n, const void* r, const void* tau,
+                   uint8_t* flags, uint64_t* ext, char* err, size_t errlen) {
+  return prim(dtype, err, errlen, [&](auto tv, Context& ctx) {
+    using T = decltype(tv);
+    const Geom g = PrimGeom<T>(dim, batch, n).g;
+    DevMem tmp(static_cast<size_t>(g.batch * g.count) * sizeof(T) + 24 * batch + 1024);
+    T* scratch = static_cast<T*>(tmp.p);
+    uint64_t* dext = reinterpret_cast<uint64_t*>(scratch + g.batch * g.count);
+    EDA_HIP(hipMemsetAsync(dext, 0, 24 * batch, ctx.stream));
+    enqueue_sig_flags<T>(ctx.stream, g, static_cast<const T*>(r), static_cast<const T*>(tau), flags, dext, nullptr,
+                         scratch);
+    EDA_HIP(hipMemcpyAsync(ext, dext, 24 * batch, hipMemcpyDeviceToHost, ctx.stream));
+    EDA_HIP(hipStreamSynchronize(ctx.stream));
+  });
+}
+
+int eda_prim_raw_window(int32_t dtype, int32_t dim, int64_t batch, const int64_t* n, const uint8_t* flags,
+                        const int32_t* win, void* om, char* err, size_t errlen) {
+  return prim(dtype, err, errlen, [&](auto tv, Context& ctx) {
+    using T = decltype(tv);
+    const Geom g = PrimGeom<T>(dim, batch, n).g;
+    const MethodCtl c = method_ctl(g, std::is_same_v<T, float>);
+    DevMem tmp(static_cast<size_t>(g.batch * g.count) * sizeof(T));
+    T* dtmp = static_cast<T*>(tmp.p);
+    std::vector<int32_t> w(win, win + batch);
+    const int32_t mw = *std::max_element(w.begin(), w.end());
+    const int32_t* dw = ctx.up.put(w);
+    T* o = static_cast<T*>(om);
+    const T two_kappa = static_cast<T>(2.0 * c.kappa);
+    const T tiny = static_cast<T>(c.tiny);
+    for (int a = 0; a < g.dim; ++a) {
+      each(ctx.stream, g, nullptr, [=] __device__(int64_t b, int64_t i) {
+        const int64_t at = b * g.count + i;
+        dtmp[at] = ((flags[at] >> a) & 1u) ? T(1) : T(0);
+      });
+      enqueue_cube<T>(ctx.stream, g, CubeOp::kBox, dtmp, dtmp, nullptr, dw, mw);
+      const bool last = a == g.dim - 1;
+      each(ctx.stream, g, nullptr, [=] __device__(int64_t b, int64_t i) {
+        const int64_t at = b * g.count + i;
+        const T sq = dtmp[at] * dtmp[at];
+        T acc = a == 0 ? sq : o[at] + sq;
+        if (last) {
+          const T dens = sqrt(acc);
+          acc = dens > T(0) ? two_kappa / (dens > tiny ? dens : tiny) : inf_t<T>();
+        }
+        o[at] = acc;
+      });
+    }
+  });
+}
+
+int eda_prim_ccl(int32_t dim, int64_t batch, const int64_t* n, const uint8_t* mask, int32_t bits, int32_t* labels,
+                 char* err, size_t errlen) {
+  return prim(EDA_F32, err, errlen, [&](auto, Context& ctx) {
+    const Geom g = PrimGeom<float>(dim, batch, n).g;
+    enqueue_ccl(ctx.stream, g, mask, static_cast<uint8_t>(bits), labels, nullptr);
+  });
+}
+
+int eda_prim_dct(int32_t dtype, int32_t dim, int64_t batch, const int64_t* n, const void* in, void* out,
+                 int32_t inverse, int32_t odd, char* err, size_t errlen) {
+  return prim(dtype, err, errlen, [&](auto tv, Context& ctx) {
+    using T = decltype(tv);
+    const Geom g = PrimGeom<T>(dim, batch, n).g;
+    DevMem tmp(static_cast<size_t>(g.batch * g.count) * sizeof(T));
+    if (in == out) throw std::invalid_argument("eda: dct: in and out must differ");
+    enqueue_dct<T>(ctx.stream, ctx.device, g, static_cast<const T*>(in), static_cast<T*>(out), static_cast<T*>(tmp.p),
+                   inverse, odd);
+  });
+}
+
+int eda_prim_orientation(int32_t dtype, int32_t dim, int64_t batch, const int64_t* n, const void* r, const int32_t* w,
+                         void* nvec, void* coh, char* err, size_t errlen) {
+  return prim(dtype, err, errlen, [&](auto tv, Context& ctx) {
+    using T = decltype(tv);
+    if (dim < 2) throw std::invalid_argument("eda: orientation needs 2 or 3 axes");
+    const Geom g = PrimGeom<T>(dim, batch, n).g;
+    const int K = dim * (dim + 1) / 2;
+    DevMem work(static_cast<size_t>(g.batch * g.count) * sizeof(T) * (3 + K) + 4 * batch + 1024);
+    int32_t* dnf = reinterpret_cast<int32_t*>(static_cast<char*>(work.p) +
+                                              static_cast<size_t>(g.batch * g.count) * sizeof(T) * (3 + K));
+    std::vector<int32_t> wv(w, w + batch);
+    const int32_t* dw = ctx.up.put(wv);
+    enqueue_orientation<T>(ctx.stream, ctx.device, g, static_cast<const T*>(r), dw, nullptr, static_cast<T*>(nvec),
+                           static_cast<T*>(coh), static_cast<T*>(work.p), dnf);
+  });
+}
+
+}  // extern "C"

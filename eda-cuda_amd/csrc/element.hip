@@ -1,0 +1,308 @@
+// SPDX-License-Identifier: Apache-2.0
+// The variable structuring element of a mode and its operators (reference
+// cpp/src/method/element.hpp:1-29, element.cuh:53-75, element.cu:166-303, element_op.cuh:1-263):
+//   d = L ((1 - c) + c / q) the cube side, e = 0.5 max(L (1 + (span - 1) c) - d, 0) the segment
+//   half length, o = e n the segment half vector (D >= 2); the hat weights in log d over the rung
+//   ladder (lo = the lower rung, t = the weight of rung lo + 1); per rung j, the cube of side
+//   rung_w[j] and a gather: acc = f(x), or over the 5 points x + (i / 2) o, i = -2..2 read by
+//   D-linear interpolation of the half-sample reflected periodic extension when the rung carries
+//   segment points (pts bit j); max / min, or the left-to-right sum divided by 5; the voxel's first
+//   rung with a positive weight stores weight * acc, the second adds it. Point coordinates are
+//   formed in double for both dtypes.
+#include "common.hpp"
+#include "kernels.hpp"
+
+#include <cmath>
+
+namespace eda {
+namespace {
+
+template <class T>
+struct DE {
+  T d, e;
+};
+template <class T>
+__device__ __forceinline__ DE<T> element_de(T L, T c, T q, T span_m1) {
+  const T f = (T(1) - c) + c / q;
+  DE<T> r;
+  r.d = L * f;
+  const T len = L * (T(1) + span_m1 * c);
+  const T diff = len - r.d;
+  r.e = T(0.5) * (diff > T(0) ? diff : T(0));
+  return r;
+}
+
+dim3 elem_grid(const Geom& g) {
+  int64_t bx = (g.count + kThreads - 1) / kThreads;
+  bx = bx < 1 ? 1 : (bx > 1024 ? 1024 : bx);
+  return dim3(static_cast<unsigned>(bx), static_cast<unsigned>(g.batch));
+}
+
+template <class T>
+__global__ void __launch_bounds__(kThreads) k_build(Geom g, T q, T span_m1, const T* __restrict__ L,
+                                                    const T* __restrict__ nvec, const T* __restrict__ coh,
+                                                    T* __restrict__ o, const int32_t* __restrict__ on,
+                                                    uint64_t* __restrict__ dminmax) {
+  __shared__ uint64_t smin[kThreads / 64], smax[kThreads / 64];
+  const int64_t b = blockIdx.y;
+  if (on != nullptr && on[b] == 0) return;
+  const int64_t F = g.batch * g.count;
+  uint64_t tmin = 0x7ff0000000000000ULL, tmax = 0;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(kThreads) + threadIdx.x; i < g.count;
+       i += static_cast<int64_t>(gridDim.x) * kThreads) {
+    const int64_t at = b * g.count + i;
+    const T c = coh != nullptr ? coh[at] : T(0);
+    const DE<T> de = element_de<T>(L[at], c, q, span_m1);
+    if (g.dim >= 2)
+      for (int a = 0; a < g.dim; ++a) o[a * F + at] = de.e * nvec[a * F + at];
+    const uint64_t bits = dbits(static_cast<double>(de.d));
+    tmin = bits < tmin ? bits : tmin;
+    tmax = bits > tmax ? bits : tmax;
+  }
+  tmin = wave_min_u64(tmin);
+  tmax = wave_max_u64(tmax);
+  if ((threadIdx.x & 63) == 0) {
+    smin[threadIdx.x >> 6] = tmin;
+    smax[threadIdx.x >> 6] = tmax;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kThreads / 64; ++w) {
+      tmin = smin[w] < tmin ? smin[w] : tmin;
+      tmax = smax[w] > tmax ? smax[w] : tmax;
+    }
+    tmin = smin[0] < tmin ? smin[0] : tmin;
+    tmax = smax[0] > tmax ? smax[0] : tmax;
+    atomicMin(reinterpret_cast<unsigned long long*>(&dminmax[2 * b]), static_cast<unsigned long long>(tmin));
+    atomicMax(reinterpret_cast<unsigned long long*>(&dminmax[2 * b + 1]), static_cast<unsigned long long>(tmax));
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(kThreads) k_weights(Geom g, T q, T span_m1, const T* __restrict__ L,
+                                                      const T* __restrict__ coh, const ElemDev* __restrict__ el,
+                                                      uint8_t* __restrict__ lo, T* __restrict__ t,
+                                                      const int32_t* __restrict__ on, uint32_t* __restrict__ bits_out) {
+  __shared__ double s_lw[kMaxRungs];
+  __shared__ uint32_t s_used, s_pts;
+  const int64_t b = blockIdx.y;
+  if (on != nullptr && on[b] == 0) return;
+  const int K = el[b].n_rungs;
+  // log(rung side) with the device log, the same function as the voxels' log d: a d clipped to a
+  // rung gets a weight of exactly 0 or 1
+  if (threadIdx.x < kMaxRungs) s_lw[threadIdx.x] = threadIdx.x < static_cast<unsigned>(K) ? log(static_cast<double>(el[b].rung_w[threadIdx.x])) : 0.0;
+  if (threadIdx.x == 0) {
+    s_used = 0u;
+    s_pts = 0u;
+  }
+  __syncthreads();
+  const double w_lo = static_cast<double>(el[b].rung_w[0]);
+  const double w_hi = static_cast<double>(el[b].rung_w[K > 0 ? K - 1 : 0]);
+  uint32_t used = 0u, pts = 0u;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(kThreads) + threadIdx.x; i < g.count;
+       i += static_cast<int64_t>(gridDim.x) * kThreads) {
+    const int64_t at = b * g.count + i;
+    const T c = coh != nullptr ? coh[at] : T(0);
+    const DE<T> de = element_de<T>(L[at], c, q, span_m1);
+    int idx = 0;
+    T tt = T(0);
+    if (K > 1) {
+      double dc = static_cast<double>(de.d);
+      dc = dc > w_lo ? dc : w_lo;
+      dc = dc < w_hi ? dc : w_hi;
+      const double lg = log(dc);
+      int cnt = 0;
+      for (int j = 0; j < K; ++j) cnt += s_lw[j] <= lg ? 1 : 0;
+      idx = cnt - 1;
+      idx = idx < 0 ? 0 : (idx > K - 2 ? K - 2 : idx);
+      tt = static_cast<T>((lg - s_lw[idx]) / (s_lw[idx + 1] - s_lw[idx]));
+    }
+    lo[at] = static_cast<uint8_t>(idx);
+    t[at] = tt;
+    const T wl = T(1) - tt;
+    uint32_t bits = 0u;
+    if (wl > T(0)) bits |= 1u << idx;
+    if (tt > T(0)) bits |= 1u << (idx + 1);
+    used |= bits;
+    if (de.e > T(0.25)) pts |= bits;
+  }
+  used = wave_or_u32(used);
+  pts = wave_or_u32(pts);
+  if ((threadIdx.x & 63) == 0) {
+    if (used) atomicOr(&s_used, used);
+    if (pts) atomicOr(&s_pts, pts);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (s_used) atomicOr(&bits_out[2 * b], s_used);
+    if (s_pts) atomicOr(&bits_out[2 * b + 1], s_pts);
+  }
+}
+
+// D-linear interpolation at y of the half-sample reflected periodic extension of f (one element).
+template <class T, int D>
+__device__ __forceinline__ T interp(const Geom& g, const T* __restrict__ f, const double* y) {
+  int64_t off0[D], off1[D];
+  T w0[D], w1[D];
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    const double fl = floor(y[a]);
+    const double tt = y[a] - fl;
+    const int64_t i0 = static_cast<int64_t>(fl);
+    const T t1 = static_cast<T>(tt);
+    w1[a] = t1;
+    w0[a] = T(1) - t1;
+    off0[a] = reflect_any(i0, g.n[a]) * g.st[a];
+    off1[a] = reflect_any(i0 + 1, g.n[a]) * g.st[a];
+  }
+  T acc = T(0);
+#pragma unroll
+  for (int corner = 0; corner < (1 << D); ++corner) {
+    int64_t off = 0;
+#pragma unroll
+    for (int a = 0; a < D; ++a) off += ((corner >> (D - 1 - a)) & 1) ? off1[a] : off0[a];
+    T term = f[off];
+#pragma unroll
+    for (int a = 0; a < D; ++a) term = term * (((corner >> (D - 1 - a)) & 1) ? w1[a] : w0[a]);
+    acc = acc + term;
+  }
+  return acc;
+}
+
+// One rung's gather. OP 0 max, 2 max + min, 3 box.
+template <class T, int D, int OP>
+__global__ void __launch_bounds__(kThreads) k_gather(Geom g, int j, const int32_t* __restrict__ rung_win,
+                                                     const ElemDev* __restrict__ el, const uint8_t* __restrict__ lo,
+                                                     const T* __restrict__ t, const T* __restrict__ o,
+                                                     const T* __restrict__ f0, const T* __restrict__ f1,
+                                                     T* __restrict__ out0, T* __restrict__ out1) {
+  const int64_t b = blockIdx.y;
+  if (rung_win[static_cast<int64_t>(j) * g.batch + b] == 0) return;
+  const bool pts = D >= 2 && ((el[b].pts >> j) & 1u) != 0u;
+  const int64_t base = b * g.count;
+  const int64_t F = g.batch * g.count;
+  const T* e0 = f0 + base;
+  const T* e1 = OP == 2 ? f1 + base : nullptr;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(kThreads) + threadIdx.x; i < g.count;
+       i += static_cast<int64_t>(gridDim.x) * kThreads) {
+    const int64_t at = base + i;
+    const int l = lo[at];
+    const T tt = t[at];
+    const T wl = T(1) - tt;
+    T wgt;
+    bool add;
+    if (l == j && wl > T(0)) {
+      wgt = wl;
+      add = false;
+    } else if (l + 1 == j && tt > T(0)) {
+      wgt = tt;
+      add = wl > T(0);
+    } else {
+      continue;
+    }
+    T acc0 = e0[i];
+    T acc1 = OP == 2 ? e1[i] : T(0);
+    if (pts) {
+      int32_t x[kMaxDim];
+#pragma unroll
+      for (int a = 0; a < D; ++a) x[a] = static_cast<int32_t>((i / g.st[a]) % g.n[a]);
+      double ov[D];
+#pragma unroll
+      for (int a = 0; a < D; ++a) ov[a] = static_cast<double>(o[a * F + at]);
+#pragma unroll
+      for (int k = 1; k <= 2; ++k) {
+        const double sc = static_cast<double>(k) / 2.0;
+#pragma unroll
+        for (int sg = 0; sg < 2; ++sg) {
+          double y[D];
+#pragma unroll
+          for (int a = 0; a < D; ++a) {
+            const double off = sc * ov[a];
+            const double xa = static_cast<double>(x[a]);
+            y[a] = sg == 0 ? xa + off : xa - off;
+          }
+          if constexpr (OP == 2) {
+            acc0 = max_c(acc0, interp<T, D>(g, e0, y));
+            acc1 = min_c(acc1, interp<T, D>(g, e1, y));
+          } else if constexpr (OP == 0) {
+            acc0 = max_c(acc0, interp<T, D>(g, e0, y));
+          } else {
+            acc0 = acc0 + interp<T, D>(g, e0, y);
+          }
+        }
+      }
+      if constexpr (OP == 3) acc0 = acc0 / T(5);
+    }
+    const T v0 = wgt * acc0;
+    out0[at] = add ? out0[at] + v0 : v0;
+    if constexpr (OP == 2) {
+      const T v1 = wgt * acc1;
+      out1[at] = add ? out1[at] + v1 : v1;
+    }
+  }
+}
+
+template <class T, int OP>
+void launch_gather(hipStream_t s, const Geom& g, int j, const int32_t* rung_win, const ElemDev* el, const uint8_t* lo,
+                   const T* t, const T* o, const T* f0, const T* f1, T* out0, T* out1) {
+  const dim3 grid = elem_grid(g);
+  if (g.dim == 1) hipLaunchKernelGGL((k_gather<T, 1, OP>), grid, dim3(kThreads), 0, s, g, j, rung_win, el, lo, t, o, f0, f1, out0, out1);
+  else if (g.dim == 2) hipLaunchKernelGGL((k_gather<T, 2, OP>), grid, dim3(kThreads), 0, s, g, j, rung_win, el, lo, t, o, f0, f1, out0, out1);
+  else hipLaunchKernelGGL((k_gather<T, 3, OP>), grid, dim3(kThreads), 0, s, g, j, rung_win, el, lo, t, o, f0, f1, out0, out1);
+  EDA_LAUNCH("k_gather");
+}
+
+}  // namespace
+
+template <class T>
+void enqueue_element_build(hipStream_t s, const Geom& g, const MethodCtl& c, const T* L, const T* nvec, const T* coh,
+                           T* o, const int32_t* on, uint64_t* dminmax) {
+  hipLaunchKernelGGL(k_build<T>, elem_grid(g), dim3(kThreads), 0, s, g, static_cast<T>(c.q),
+                     static_cast<T>(c.span - 1.0), L, nvec, g.dim >= 2 ? coh : nullptr, o, on, dminmax);
+  EDA_LAUNCH("k_build");
+}
+
+template <class T>
+void enqueue_element_weights(hipStream_t s, const Geom& g, const MethodCtl& c, const T* L, const T* coh,
+                             const ElemDev* el, uint8_t* lo, T* t, const int32_t* on, uint32_t* bits) {
+  hipLaunchKernelGGL(k_weights<T>, elem_grid(g), dim3(kThreads), 0, s, g, static_cast<T>(c.q),
+                     static_cast<T>(c.span - 1.0), L, g.dim >= 2 ? coh : nullptr, el, lo, t, on, bits);
+  EDA_LAUNCH("k_weights");
+}
+
+template <class T>
+void enqueue_element_op(hipStream_t s, const Geom& g, int op, const ElemDev* el, const uint8_t* lo, const T* t,
+                        const T* o, const int32_t* rung_win, const int32_t* rung_max, int K_max, const T* in, T* out0,
+                        T* out1, T* f0, T* f1) {
+  for (int j = 0; j < K_max; ++j) {
+    if (rung_max[j] < 1) continue;
+    const int32_t* win = rung_win + static_cast<int64_t>(j) * g.batch;
+    if (op == 3) {
+      enqueue_cube<T>(s, g, CubeOp::kBox, in, f0, nullptr, win, rung_max[j]);
+      launch_gather<T, 3>(s, g, j, rung_win, el, lo, t, o, f0, nullptr, out0, nullptr);
+    } else if (op == 2) {
+      enqueue_cube<T>(s, g, CubeOp::kMaxMin, in, f0, f1, win, rung_max[j]);
+      launch_gather<T, 2>(s, g, j, rung_win, el, lo, t, o, f0, f1, out0, out1);
+    } else {
+      enqueue_cube<T>(s, g, CubeOp::kMax, in, f0, nullptr, win, rung_max[j]);
+      launch_gather<T, 0>(s, g, j, rung_win, el, lo, t, o, f0, nullptr, out0, nullptr);
+    }
+  }
+}
+
+template void enqueue_element_build<float>(hipStream_t, const Geom&, const MethodCtl&, const float*, const float*,
+                                           const float*, float*, const int32_t*, uint64_t*);
+template void enqueue_element_build<double>(hipStream_t, const Geom&, const MethodCtl&, const double*, const double*,
+                                            const double*, double*, const int32_t*, uint64_t*);
+template void enqueue_element_weights<float>(hipStream_t, const Geom&, const MethodCtl&, const float*, const float*,
+                                             const ElemDev*, uint8_t*, float*, const int32_t*, uint32_t*);
+template void enqueue_element_weights<double>(hipStream_t, const Geom&, const MethodCtl&, const double*, const double*,
+                                              const ElemDev*, uint8_t*, double*, const int32_t*, uint32_t*);
+template void enqueue_element_op<float>(hipStream_t, const Geom&, int, const ElemDev*, const uint8_t*, const float*,
+                                        const float*, const int32_t*, const int32_t*, int, const float*, float*, float*,
+                                        float*, float*);
+template void enqueue_element_op<double>(hipStream_t, const Geom&, int, const ElemDev*, const uint8_t*, const double*,
+                                         const double*, const int32_t*, const int32_t*, int, const double*, double*,
+                                         double*, double*, double*);
+
+}  // namespace eda

@@ -1,0 +1,134 @@
+// SPDX-License-Identifier: Apache-2.0
+// The internal interface between the host driver (driver.hip) and the kernel files. Every function
+// enqueues work on the stream s and never synchronizes. Per-element arrays are device arrays [B];
+// a window 0 (or on[b] == 0) skips element b, whose outputs are left untouched.
+#pragma once
+
+#include "common.hpp"
+
+#include <cstdint>
+
+namespace eda {
+
+// Per-call constants of the method (reference cpp/src/method/common.cpp:14-63).
+struct MethodCtl {
+  int32_t n_max;
+  int64_t count;
+  int32_t sifts, u, s, sig_passes, growth_from, anorm_smooth;
+  double kappa, beta, c_lo, c_hi, reset, q, span, growth;
+  double sig_theta, sig_rho, sig_noise;
+  double absent, absent_jump, absent_reach, absent_lo;
+  double t0, t1, anorm_floor, ramp_k0, ramp_k1;
+  double fine_beta, fine_theta, fine_jump, fine_lo, fine_pop, fine_core, split;
+  double tiny;
+};
+MethodCtl method_ctl(const Geom& g, bool f32);
+
+// ---- the cube filter (vhgw.hip) ------------------------------------------------------------------
+enum class CubeOp : int8_t { kMax = 0, kMin = 1, kMaxMin = 2, kBox = 3 };
+// win [B]: one odd side per element (0: skipped); max_w: the largest side of the launch (host).
+template <class T>
+void enqueue_cube(hipStream_t s, const Geom& g, CubeOp op, const T* in, T* out0, T* out1, const int32_t* win,
+                  int32_t max_w);
+int32_t vhgw_task_slots(int32_t n, int32_t max_w);
+
+// ---- reductions (reduce.hip) ---------------------------------------------------------------------
+// Per element: energy[b] = the canonical pairwise sum of squares (double; nullptr: none),
+// amax_bits[b] = the largest bit pattern of |v| as a double, nonfinite[b] = the count of inf / NaN
+// samples (nullptr: none). rows_ws: a device scratch of (B * rows) doubles + (B * rows) u64.
+template <class T>
+void enqueue_elem_stats(hipStream_t s, const Geom& g, const T* f, const int32_t* on, double* energy,
+                        uint64_t* amax_bits, uint64_t* nonfinite, void* rows_ws);
+size_t elem_stats_ws_bytes(const Geom& g);
+
+// ---- exact selections (select.hip) ---------------------------------------------------------------
+// The value set of element b: kField: field[i] where mask == nullptr or (mask[i] & bits) != 0;
+// kDiff: forward differences along the axes of the bit mask `axes`, computed in T; optionally
+// |value - center[b]| in double. Values compare as doubles.
+enum class SelSrc : int8_t { kField = 0, kDiff = 1 };
+template <class T>
+struct Selection {
+  SelSrc src;
+  const T* field;
+  const uint8_t* mask;
+  uint8_t bits;
+  uint8_t axes;
+  const double* center;
+};
+// mode 0: median (numpy.median); 1: quantile q (numpy 'linear'). out[b]: NaN for an empty set;
+// count[b] (nullptr: none): the size of the set.
+template <class T>
+void enqueue_select(hipStream_t s, const Geom& g, const Selection<T>& sel, int mode, double q, double* out,
+                    int64_t* count, const int32_t* on, void* ws);
+size_t select_ws_bytes(const Geom& g);
+// Noise levels: (1.4826 MAD) / sqrt 2 of the forward differences, pooled over the axes or the
+// largest per axis. ws: select_ws_bytes + 3 [B] doubles.
+template <class T>
+void enqueue_noise_sigma(hipStream_t s, const Geom& g, const T* r, bool per_axis, double* out, const int32_t* on,
+                         void* ws);
+size_t noise_ws_bytes(const Geom& g);
+
+// ---- significant extrema (flags.hip) -------------------------------------------------------------
+// Clears flags of the running elements, then along every axis with n >= 3 sets bit a at the
+// significant peaks of r and -r and adds their count into ext[3 b + a] (the caller zeroes ext).
+// scratch: one field of T.
+template <class T>
+void enqueue_sig_flags(hipStream_t s, const Geom& g, const T* r, const T* tau, uint8_t* flags, uint64_t* ext,
+                       const int32_t* on, T* scratch);
+
+// ---- components (ccl.hip) ------------------------------------------------------------------------
+// Face-connected components of {mask & bits}: labels[i] = the smallest in-element index of the
+// component, -1 outside.
+void enqueue_ccl(hipStream_t s, const Geom& g, const uint8_t* mask, uint8_t bits, int32_t* labels,
+                 const int32_t* on);
+
+// ---- orientation (orient.hip) --------------------------------------------------------------------
+// The dense DCT matrices of one axis length, on the device (built once per (n, dtype)).
+template <class T>
+struct AxisMats {
+  int32_t n = 0;
+  T* fwd = nullptr;   // fwd[k * n + x] = c_k cos(pi k (2x+1) / (2n))
+  T* icos = nullptr;  // icos[x * n + k] = c_k cos(...)
+  T* isin = nullptr;  // isin[x * n + k] = c_k sin(...)
+};
+template <class T>
+const AxisMats<T>& axis_mats(int device, int32_t n);
+// n (D fields, component-major [a][B][count]) and coh of every element with on[b]; w [B]: the mode
+// windows (the band's centre period). work: (2 + D + D (D + 1) / 2) fields of T; nonfinite[b] = 1
+// where the tensor is not finite.
+template <class T>
+void enqueue_orientation(hipStream_t s, int device, const Geom& g, const T* r, const int32_t* w, const int32_t* on,
+                         T* nvec, T* coh, T* work, int32_t* nonfinite);
+// The orthonormal DCT-II of every element (inverse = 0) or its inverse with the sin basis on the
+// axes of the bit mask odd (inverse = 1), through the same GEMMs; tmp: one field.
+template <class T>
+void enqueue_dct(hipStream_t s, int device, const Geom& g, const T* in, T* out, T* tmp, int inverse, int odd);
+// n, c of the symmetric tensors in t (K = D (D + 1) / 2 fields, pair order) of the running elements.
+template <class T>
+void enqueue_eigen(hipStream_t s, const Geom& g, const T* t, const int32_t* on, T* nvec, T* coh, int32_t* nonfinite);
+
+// ---- the element (element.hip) -------------------------------------------------------------------
+// Per-element data of the structuring element (device, [B]).
+struct ElemDev {
+  int32_t n_rungs;
+  uint32_t used, pts;
+  int32_t rung_w[kMaxRungs];
+};
+// d, e, o (D >= 2) of every voxel and the per-element bits of min d / max d (u64 [2 B], the caller
+// sets +inf / 0 first).
+template <class T>
+void enqueue_element_build(hipStream_t s, const Geom& g, const MethodCtl& c, const T* L, const T* nvec, const T* coh,
+                           T* o, const int32_t* on, uint64_t* dminmax);
+// lo, t of every voxel and the rung bits used / pts (u32 [2 B], atomicOr; the caller zeroes them).
+template <class T>
+void enqueue_element_weights(hipStream_t s, const Geom& g, const MethodCtl& c, const T* L, const T* coh,
+                             const ElemDev* el, uint8_t* lo, T* t, const int32_t* on, uint32_t* bits);
+// The operators over the rungs: op 0 Dil (max), 2 Dil + Ero (max into out0, min into out1), 3 Avg.
+// rung_win [K_max * B]: rung j of element b at rung_win[j B + b] (0: unused); rung_max [K_max]:
+// the largest side of rung j over the batch (host). f0, f1: cube scratch fields.
+template <class T>
+void enqueue_element_op(hipStream_t s, const Geom& g, int op, const ElemDev* el, const uint8_t* lo, const T* t,
+                        const T* o, const int32_t* rung_win, const int32_t* rung_max, int K_max, const T* in, T* out0,
+                        T* out1, T* f0, T* f1);
+
+}  // namespace eda

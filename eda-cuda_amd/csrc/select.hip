@@ -1,0 +1,348 @@
+// SPDX-License-Identifier: Apache-2.0
+// Exact order statistics of per-element value sets, and the noise levels (reference
+// cpp/src/method/stats.cu:1-347, stats_common.cuh:44-171).
+//
+// A selection finds the order statistics of ranks r0 <= r1 of every element's value set by six
+// most-significant-digit radix passes (11, 11, 11, 11, 11, 9 bits) over the 64-bit order keys of
+// the values as doubles (-0 counts as +0). Per pass a histogram kernel counts the digit of every
+// value whose higher digits equal a rank's known prefix (LDS uint32 bins, flushed into the global
+// 64-bit bins by integer atomics, so the counts never depend on the order of the atomics), and a
+// pick kernel (one workgroup per element) finds the bin holding the rank. The median is the mean
+// of the two middle values (numpy.median); the quantile follows numpy.quantile(method='linear'):
+// virtual index (n - 1) q, the lerp that is exact at its end. An empty set gives NaN, count 0.
+#include "common.hpp"
+#include "kernels.hpp"
+
+#include <cmath>
+
+namespace eda {
+namespace {
+
+constexpr int kBits = 11;
+constexpr int kBins = 1 << kBits;
+constexpr int kPasses = 6;
+
+struct alignas(16) SelState {
+  uint64_t prefix[2];
+  uint64_t rank[2];
+  int64_t count;
+  double gamma;
+  int32_t valid;
+  int32_t same;
+};
+
+__host__ __device__ __forceinline__ uint64_t order_key(double d) {
+  uint64_t u = dbits(d);
+  if ((u << 1) == 0) u = 0;  // -0 -> +0
+  return (u >> 63) ? ~u : (u | 0x8000000000000000ULL);
+}
+__device__ __forceinline__ double key_value(uint64_t k) {
+  const uint64_t u = (k >> 63) ? (k & 0x7fffffffffffffffULL) : ~k;
+  return bits_d(u);
+}
+
+template <class T>
+struct SelArgs {
+  Selection<T> sel;
+  Geom g;
+  int32_t naxes;
+  int32_t ax[kMaxDim];
+  int64_t items;
+};
+
+template <class T>
+__device__ __forceinline__ bool sel_value(const SelArgs<T>& a, int64_t b, int64_t t, double* out) {
+  double val;
+  if (a.sel.src == SelSrc::kField) {
+    const int64_t i = b * a.g.count + t;
+    if (a.sel.mask != nullptr && (a.sel.mask[i] & a.sel.bits) == 0) return false;
+    val = static_cast<double>(a.sel.field[i]);
+  } else {
+    const int64_t k = t / a.g.count;
+    const int64_t v = t - k * a.g.count;
+    const int axis = a.ax[k];
+    const int64_t st = a.g.st[axis];
+    if ((v / st) % a.g.n[axis] >= a.g.n[axis] - 1) return false;
+    const int64_t i = b * a.g.count + v;
+    const T d = a.sel.field[i + st] - a.sel.field[i];  // the difference in T
+    val = static_cast<double>(d);
+  }
+  if (a.sel.center != nullptr) val = fabs(val - a.sel.center[b]);
+  *out = val;
+  return true;
+}
+
+__global__ void k_sel_init(int64_t B, SelState* st, uint64_t* hist, const int32_t* on) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= B * 2 * kBins) return;
+  const int64_t b = i / (2 * kBins);
+  if (on != nullptr && on[b] == 0) return;
+  hist[i] = 0;
+  if (i % (2 * kBins) == 0) st[b] = SelState{};
+}
+
+template <class T>
+__global__ void __launch_bounds__(kThreads) k_sel_hist(SelArgs<T> a, const SelState* st, uint64_t* hist, int32_t nblk,
+                                                       int32_t shift, int32_t width, int32_t first, const int32_t* on) {
+  __shared__ uint32_t h[2][kBins];
+  const int64_t b = blockIdx.x / nblk;
+  const int64_t chunk = blockIdx.x % nblk;
+  if (on != nullptr && on[b] == 0) return;
+  const SelState S = st[b];
+  if (!first && !S.valid) return;
+  const int nr = (first || S.same) ? 1 : 2;
+  for (int i = threadIdx.x; i < 2 * kBins; i += kThreads) h[i / kBins][i % kBins] = 0u;
+  __syncthreads();
+  const int hi = shift + width;
+  const uint64_t dmask = (1ULL << width) - 1ULL;
+  const int64_t stride = static_cast<int64_t>(nblk) * kThreads;
+  for (int64_t t = chunk * kThreads + threadIdx.x; t < a.items; t += stride) {
+    double v;
+    if (!sel_value(a, b, t, &v)) continue;
+    const uint64_t k = order_key(v);
+    const uint32_t bin = static_cast<uint32_t>((k >> shift) & dmask);
+    if (first) {
+      atomicAdd(&h[0][bin], 1u);
+    } else {
+      if ((k >> hi) == (S.prefix[0] >> hi)) atomicAdd(&h[0][bin], 1u);
+      if (nr == 2 && (k >> hi) == (S.prefix[1] >> hi)) atomicAdd(&h[1][bin], 1u);
+    }
+  }
+  __syncthreads();
+  const int nb = 1 << width;
+  for (int i = threadIdx.x; i < nr * nb; i += kThreads) {
+    const int r = i / nb, bin = i % nb;
+    const uint32_t c = h[r][bin];
+    if (c) atomicAdd(reinterpret_cast<unsigned long long*>(&hist[(b * 2 + r) * kBins + bin]),
+                     static_cast<unsigned long long>(c));
+  }
+}
+
+// NumPy's linear quantile index: (n - 1) q; an index >= n - 1 takes the last value (gamma v + 1),
+// an index < 0 the first.
+__device__ __forceinline__ void quantile_index(int64_t n, double q, int64_t* i, int64_t* j, double* gamma) {
+  const double v = static_cast<double>(n - 1) * q;
+  if (v >= static_cast<double>(n - 1)) {
+    *i = *j = n - 1;
+    *gamma = v + 1.0;
+  } else if (v < 0.0) {
+    *i = *j = 0;
+    *gamma = v;
+  } else {
+    const double f = floor(v);
+    *i = static_cast<int64_t>(f);
+    *j = *i + 1;
+    *gamma = v - f;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) k_sel_pick(SelState* st, uint64_t* hist, int32_t shift, int32_t width,
+                                                       int32_t first, int32_t mode, double q, const int32_t* on) {
+  __shared__ uint64_t part[kThreads];
+  __shared__ uint64_t base[kThreads];
+  __shared__ SelState S;
+  __shared__ int32_t found_bin;
+  __shared__ uint64_t found_rank;
+  const int64_t b = blockIdx.x;
+  if (on != nullptr && on[b] == 0) return;
+  const int tid = threadIdx.x;
+  if (tid == 0) S = st[b];
+  __syncthreads();
+  if (!first && !S.valid) return;
+  const int nb = 1 << width;
+  const int per = nb / kThreads;
+  uint64_t* h0 = hist + (b * 2) * kBins;
+  if (first) {
+    uint64_t sum = 0;
+    for (int j = 0; j < per; ++j) sum += h0[tid * per + j];
+    part[tid] = sum;
+    __syncthreads();
+    if (tid == 0) {
+      uint64_t n = 0;
+      for (int t = 0; t < kThreads; ++t) n += part[t];
+      S.count = static_cast<int64_t>(n);
+      S.valid = n > 0;
+      S.prefix[0] = S.prefix[1] = 0;
+      S.same = 1;
+      if (n > 0) {
+        if (mode == 0) {
+          S.rank[0] = (n - 1) / 2;
+          S.rank[1] = n / 2;
+        } else {
+          int64_t i = 0, j = 0;
+          double gm = 0.0;
+          quantile_index(static_cast<int64_t>(n), q, &i, &j, &gm);
+          S.gamma = gm;
+          S.rank[0] = static_cast<uint64_t>(i);
+          S.rank[1] = static_cast<uint64_t>(j);
+        }
+      }
+    }
+    __syncthreads();
+    if (!S.valid) {
+      if (tid == 0) st[b] = S;
+      return;
+    }
+  }
+  for (int r = 0; r < 2; ++r) {
+    const uint64_t* hr = hist + (b * 2 + (S.same ? 0 : r)) * kBins;
+    uint64_t sum = 0;
+    for (int j = 0; j < per; ++j) sum += hr[tid * per + j];
+    part[tid] = sum;
+    __syncthreads();
+    if (tid == 0) {
+      uint64_t acc = 0;
+      for (int t = 0; t < kThreads; ++t) {
+        base[t] = acc;
+        acc += part[t];
+      }
+    }
+    __syncthreads();
+    const uint64_t rank = S.rank[r];
+    if (rank >= base[tid] && rank < base[tid] + part[tid]) {
+      uint64_t acc = base[tid];
+      for (int j = 0; j < per; ++j) {
+        const uint64_t c = hr[tid * per + j];
+        if (rank < acc + c) {
+          found_bin = tid * per + j;
+          found_rank = rank - acc;
+          break;
+        }
+        acc += c;
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      S.prefix[r] |= static_cast<uint64_t>(found_bin) << shift;
+      S.rank[r] = found_rank;
+    }
+    __syncthreads();
+  }
+  for (int i = tid; i < 2 * kBins; i += kThreads) h0[i] = 0;
+  if (tid == 0) {
+    S.same = S.prefix[0] == S.prefix[1];
+    st[b] = S;
+  }
+}
+
+__global__ void k_sel_final(int64_t B, const SelState* st, int32_t mode, double* out, int64_t* count,
+                            const int32_t* on) {
+  const int64_t b = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b >= B || (on != nullptr && on[b] == 0)) return;
+  const SelState S = st[b];
+  if (count != nullptr) count[b] = S.valid ? S.count : 0;
+  if (!S.valid) {
+    out[b] = __longlong_as_double(0x7ff8000000000000LL);
+    return;
+  }
+  const double a = key_value(S.prefix[0]);
+  const double c = key_value(S.prefix[1]);
+  if (mode == 0) {
+    out[b] = S.count % 2 ? a : (a + c) / 2.0;
+  } else {
+    const double diff = c - a;
+    out[b] = S.gamma >= 0.5 ? c - diff * (1.0 - S.gamma) : a + diff * S.gamma;
+  }
+}
+
+size_t align_up(size_t v) { return (v + 255) / 256 * 256; }
+
+}  // namespace
+
+size_t select_ws_bytes(const Geom& g) {
+  return align_up(static_cast<size_t>(g.batch) * sizeof(SelState)) +
+         align_up(static_cast<size_t>(g.batch) * 2 * kBins * sizeof(uint64_t));
+}
+size_t noise_ws_bytes(const Geom& g) { return select_ws_bytes(g) + 3 * align_up(static_cast<size_t>(g.batch) * 8); }
+
+template <class T>
+void enqueue_select(hipStream_t s, const Geom& g, const Selection<T>& sel, int mode, double q, double* out,
+                    int64_t* count, const int32_t* on, void* ws) {
+  SelArgs<T> a{};
+  a.sel = sel;
+  a.g = g;
+  if (sel.src == SelSrc::kField) {
+    a.naxes = 1;
+    a.items = g.count;
+  } else {
+    for (int ax = 0; ax < g.dim; ++ax)
+      if (sel.axes & (1u << ax)) a.ax[a.naxes++] = ax;
+    a.items = static_cast<int64_t>(a.naxes) * g.count;
+  }
+  const int64_t B = g.batch;
+  SelState* st = static_cast<SelState*>(ws);
+  uint64_t* hist = reinterpret_cast<uint64_t*>(static_cast<char*>(ws) + align_up(static_cast<size_t>(B) * sizeof(SelState)));
+  const int64_t ni = B * 2 * kBins;
+  hipLaunchKernelGGL(k_sel_init, dim3(static_cast<unsigned>((ni + kThreads - 1) / kThreads)), dim3(kThreads), 0, s, B,
+                     st, hist, on);
+  EDA_LAUNCH("k_sel_init");
+  int64_t nblk = (a.items + int64_t{kThreads} * 32 - 1) / (int64_t{kThreads} * 32);
+  nblk = nblk < 1 ? 1 : (nblk > 512 ? 512 : nblk);
+  if (a.items > 0) {
+    for (int p = 0; p < kPasses; ++p) {
+      const int top = 64 - kBits * p;
+      const int shift = top - kBits > 0 ? top - kBits : 0;
+      const int width = top - shift;
+      hipLaunchKernelGGL(k_sel_hist<T>, dim3(static_cast<unsigned>(nblk * B)), dim3(kThreads), 0, s, a, st, hist,
+                         static_cast<int32_t>(nblk), shift, width, p == 0 ? 1 : 0, on);
+      EDA_LAUNCH("k_sel_hist");
+      hipLaunchKernelGGL(k_sel_pick, dim3(static_cast<unsigned>(B)), dim3(kThreads), 0, s, st, hist, shift, width,
+                         p == 0 ? 1 : 0, mode, q, on);
+      EDA_LAUNCH("k_sel_pick");
+    }
+  }
+  hipLaunchKernelGGL(k_sel_final, dim3(static_cast<unsigned>((B + kThreads - 1) / kThreads)), dim3(kThreads), 0, s, B,
+                     st, mode, out, count, on);
+  EDA_LAUNCH("k_sel_final");
+}
+
+// sigma = (1.4826 MAD(d)) / sqrt 2 (reference stats.cu:262-334): pooled over the axes, or the
+// largest over the axes (the first axis wins ties).
+template <class T>
+void enqueue_noise_sigma(hipStream_t s, const Geom& g, const T* r, bool per_axis, double* out, const int32_t* on,
+                         void* ws) {
+  const int64_t B = g.batch;
+  char* extra = static_cast<char*>(ws) + select_ws_bytes(g);
+  double* center = reinterpret_cast<double*>(extra);
+  double* mad = reinterpret_cast<double*>(extra + align_up(static_cast<size_t>(B) * 8));
+  double* acc = reinterpret_cast<double*>(extra + 2 * align_up(static_cast<size_t>(B) * 8));
+  const double sqrt2 = std::sqrt(2.0);
+  Selection<T> sel{};
+  sel.src = SelSrc::kDiff;
+  sel.field = r;
+  const auto level = [&](uint8_t axes, int op) {
+    sel.axes = axes;
+    sel.center = nullptr;
+    enqueue_select<T>(s, g, sel, 0, 0.5, center, nullptr, on, ws);
+    sel.center = center;
+    enqueue_select<T>(s, g, sel, 0, 0.5, mad, nullptr, on, ws);
+    flat(s, B, [=] __device__(int64_t b) {
+      if (on != nullptr && on[b] == 0) return;
+      const double v = (1.4826 * mad[b]) / sqrt2;
+      if (op == 0) out[b] = v;
+      else if (v > acc[b]) acc[b] = v;
+    });
+  };
+  if (!per_axis) {
+    level(static_cast<uint8_t>((1u << g.dim) - 1u), 0);
+    return;
+  }
+  flat(s, B, [=] __device__(int64_t b) { acc[b] = 0.0; });
+  for (int a = 0; a < g.dim; ++a) {
+    if (g.n[a] < 2) continue;
+    level(static_cast<uint8_t>(1u << a), 1);
+  }
+  flat(s, B, [=] __device__(int64_t b) {
+    if (on == nullptr || on[b] != 0) out[b] = acc[b];
+  });
+}
+
+template void enqueue_select<float>(hipStream_t, const Geom&, const Selection<float>&, int, double, double*, int64_t*,
+                                    const int32_t*, void*);
+template void enqueue_select<double>(hipStream_t, const Geom&, const Selection<double>&, int, double, double*, int64_t*,
+                                     const int32_t*, void*);
+template void enqueue_noise_sigma<float>(hipStream_t, const Geom&, const float*, bool, double*, const int32_t*, void*);
+template void enqueue_noise_sigma<double>(hipStream_t, const Geom&, const double*, bool, double*, const int32_t*,
+                                          void*);
+
+}  // namespace eda

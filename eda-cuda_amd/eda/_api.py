@@ -1,0 +1,400 @@
+# SPDX-License-Identifier: Apache-2.0
+"""``eda.decompose``: argument rules, layout, errors and the result types, over the C ABI of
+libeda_hip.so. The rules, the result layout and the error policy follow the reference's
+src/eda/_api.py:184-353 (``decompose``), :433-526 (argument checks), :539-607 (layout, info,
+errors), so that the call is a drop-in.
+
+Host input (NumPy, or anything ``numpy.asarray`` takes) returns NumPy arrays. ROCm device input (a
+PyTorch-ROCm tensor, or any object with ``__dlpack__`` on a ROCm device, which PyTorch imports)
+returns a PyTorch tensor on the input's device, complete on return."""
+
+from __future__ import annotations
+
+import ctypes
+import math
+import operator
+from collections.abc import Sequence
+from dataclasses import dataclass
+from typing import Any
+
+import numpy as np
+
+from . import _native
+from ._errors import DecompositionError
+
+ERROR_NON_FINITE = 1
+ERROR_NON_FINITE_RESULT = 2
+ERROR_INTERNAL = 255
+_ERROR_MESSAGES = {
+    ERROR_NON_FINITE: "the input has inf or NaN; check_finite=True",
+    ERROR_NON_FINITE_RESULT: (
+        "a non-finite intermediate or output row: the decomposition overflows the dtype's range, "
+        "or the input has inf or NaN with check_finite=False"
+    ),
+}
+_FLOAT_DTYPES = ("float32", "float64")
+_ERRORS = ("raise", "status")
+_INT32_MAX = 2**31 - 1
+_KDL_ROCM = 10
+
+
+@dataclass(frozen=True, eq=False)
+class DecompositionInfo:
+    """Per-call information of ``eda.decompose(..., return_info=True)`` (the reference's
+    src/eda/_api.py:35-61): ``n_imfs``, ``error`` have the batch shape; ``windows``, ``spacing``,
+    ``sifts``, ``energy`` the batch shape plus K (the largest IMF count), 0 beyond an element's
+    ``n_imfs`` and for a failed element; ``dtype`` the compute dtype; ``axes`` the spatial axes."""
+
+    n_imfs: np.ndarray
+    error: np.ndarray
+    windows: np.ndarray
+    spacing: np.ndarray
+    sifts: np.ndarray
+    energy: np.ndarray
+    dtype: str
+    axes: tuple[int, ...]
+
+
+# ----------------------------------------------------------------------------------------------
+# Arguments (reference src/eda/_api.py:433-526)
+
+
+def _max_imf(value: Any) -> int:
+    if isinstance(value, (bool, np.bool_)):
+        raise TypeError("eda: max_imf must be an integer, not bool")
+    try:
+        number = operator.index(value)
+    except TypeError:
+        raise TypeError(f"eda: max_imf must be an integer, got {type(value).__name__}") from None
+    return max(-_INT32_MAX - 1, min(number, _INT32_MAX))
+
+
+def _sifts(value: Any) -> int:
+    if isinstance(value, (bool, np.bool_)):
+        raise TypeError("eda: sifts must be an integer, not bool")
+    try:
+        number = operator.index(value)
+    except TypeError:
+        raise TypeError(f"eda: sifts must be an integer, got {type(value).__name__}") from None
+    if not 1 <= number <= _INT32_MAX:
+        raise ValueError(f"eda: sifts must be an integer from 1 to {_INT32_MAX}, got {number}")
+    return number
+
+
+def _tie_tolerance(value: Any) -> float | None:
+    if value is None:
+        return None
+    if isinstance(value, (bool, np.bool_)) or not isinstance(value, (int, float, np.integer, np.floating)):
+        raise TypeError(f"eda: tie_tolerance must be a real number or None, got {type(value).__name__}")
+    try:
+        number = float(value)
+    except OverflowError:
+        number = math.inf
+    if not (math.isfinite(number) and number >= 0):
+        raise ValueError(f"eda: tie_tolerance must be a finite number >= 0, got {value}")
+    return number
+
+
+def _flag(value: Any, name: str) -> bool:
+    if not isinstance(value, (bool, np.bool_)):
+        raise TypeError(f"eda: {name} must be True or False, got {type(value).__name__}")
+    return bool(value)
+
+
+def _dtype_name(dtype: Any) -> str:
+    try:
+        return np.dtype(dtype).name
+    except TypeError:  # a framework dtype such as torch.float32
+        return str(dtype).removeprefix("torch.")
+
+
+def _axis(value: Any) -> int:
+    if isinstance(value, (bool, np.bool_)):
+        raise TypeError("bool")
+    return operator.index(value)
+
+
+def _normalize_axes(axes: Any, ndim: int) -> tuple[int, ...]:
+    if ndim < 1:
+        raise ValueError("eda: expected an array with at least one dimension")
+    if axes is None:
+        if ndim > 3:
+            raise ValueError(
+                f"eda: axes=None takes every axis and needs x.ndim <= 3, got {ndim}: name the "
+                "spatial axes (1 to 3) with axes="
+            )
+        return tuple(range(ndim))
+    if isinstance(axes, (int, np.integer)) and not isinstance(axes, bool):
+        axes = (axes,)
+    try:
+        listed = [_axis(a) for a in axes]
+    except TypeError:
+        raise TypeError(f"eda: axes must be an int, a sequence of ints or None, got {type(axes).__name__}") from None
+    out = []
+    for a in listed:
+        if not -ndim <= a < ndim:
+            raise np.exceptions.AxisError(a, ndim, "eda")
+        out.append(a % ndim)
+    if len(set(out)) != len(out):
+        raise ValueError(f"eda: axes must be unique, got {tuple(listed)}")
+    if not 1 <= len(out) <= 3:
+        raise ValueError(f"eda: 1 to 3 spatial axes, got {len(out)}")
+    return tuple(out)
+
+
+def _restore_order(ndim: int, batch_axes: tuple[int, ...], axes: tuple[int, ...], extra: int = 1) -> list[int]:
+    """The permutation taking a computed ``(*batch, *E, *spatial)`` array to the public layout
+    (reference src/eda/_api.py:539-555)."""
+    nb = len(batch_axes)
+
+    def computed_position(i: int) -> int:
+        return nb + extra + axes.index(i) if i in axes else batch_axes.index(i)
+
+    m = min(axes)
+    return (
+        [computed_position(f) for f in range(m)]
+        + [nb + j for j in range(extra)]
+        + [computed_position(f) for f in range(m, ndim)]
+    )
+
+
+def _error_summary(codes: np.ndarray, indices: np.ndarray, total: int) -> str:
+    parts = []
+    for code in sorted({int(c) for c in codes}):
+        where = indices[codes == code]
+        shown = ", ".join(str(int(i)) for i in where[:8]) + (", ..." if where.size > 8 else "")
+        text = _ERROR_MESSAGES.get(code, f"error code {code}")
+        parts.append(f"code {code} ({text}) at batch index {shown}")
+    return f"eda: the decomposition failed for {indices.size} of {total} element(s): " + "; ".join(parts)
+
+
+# ----------------------------------------------------------------------------------------------
+# Input intake
+
+
+def _torch():
+    try:
+        import torch  # noqa: PLC0415
+
+        return torch
+    except Exception:  # pragma: no cover - torch is optional for host input
+        return None
+
+
+def _is_device_input(x: Any) -> bool:
+    torch = _torch()
+    if torch is not None and isinstance(x, torch.Tensor):
+        return x.device.type == "cuda"
+    dev = getattr(x, "__dlpack_device__", None)
+    if dev is not None:
+        try:
+            return int(dev()[0]) == _KDL_ROCM
+        except Exception:
+            return False
+    return False
+
+
+def _device_tensor(x: Any):
+    torch = _torch()
+    if torch is None:
+        raise TypeError("eda: device input needs PyTorch-ROCm (the DLPack import of ROCm arrays)")
+    if isinstance(x, torch.Tensor):
+        return x
+    return torch.from_dlpack(x)
+
+
+def _call_native(desc: _native.ArrayDesc, max_imf: int, tie: float | None, sifts: int, check_finite: bool,
+                 stream: int) -> _native.Result:
+    lib = _native.core()
+    p = _native.Params()
+    p.max_imf = max_imf
+    p.sifts = sifts
+    p.check_finite = 1 if check_finite else 0
+    p.orientation = 0
+    p.tie_tolerance = -1.0 if tie is None else tie
+    p.max_chunk = 0
+    p.stream = stream
+    res = _native.Result()
+    err = _native.errbuf()
+    _native.check(lib.eda_decompose(ctypes.byref(desc), ctypes.byref(p), ctypes.byref(res), err, len(err)), err)
+    return res
+
+
+def _info_of(res: _native.Result, batch_shape: tuple[int, ...], dtype: str, axes: tuple[int, ...]) -> DecompositionInfo:
+    B, K = int(res.batch), int(res.K)
+
+    def arr(ptr, n, np_dtype):
+        if n == 0:
+            return np.zeros(0, np_dtype)
+        return np.ctypeslib.as_array(ptr, shape=(n,)).astype(np_dtype, copy=True)
+
+    n_imfs = arr(res.n_imfs, B, np.int32).reshape(batch_shape)
+    error = arr(res.error, B, np.int32).reshape(batch_shape)
+    shape_k = batch_shape + (K,)
+    return DecompositionInfo(
+        n_imfs=n_imfs,
+        error=error,
+        windows=arr(res.windows, B * K, np.int32).reshape(shape_k),
+        spacing=arr(res.spacing, B * K, np.float64).reshape(shape_k),
+        sifts=arr(res.sifts, B * K, np.int32).reshape(shape_k),
+        energy=arr(res.energy, B * K, np.float64).reshape(shape_k),
+        dtype=dtype,
+        axes=axes,
+    )
+
+
+def decompose(
+    x: Any,
+    axes: int | Sequence[int] | None = None,
+    max_imf: int = -1,
+    *,
+    dtype: Any = None,
+    tie_tolerance: float | None = None,
+    sifts: int = 3,
+    check_finite: bool = True,
+    errors: str = "raise",
+    stream: Any = None,
+    return_info: bool = False,
+    return_orientation: bool = False,
+) -> Any:
+    """Decompose every 1-D, 2-D or 3-D field of ``x`` into intrinsic mode functions (IMFs) and a
+    residue, on an AMD Instinct GPU (gfx950).
+
+    The arguments, the result layout ``x.shape[:m] + (P,) + x.shape[m:]`` with ``m = min(axes)``,
+    the information and the error policy are those of the reference's ``eda.decompose``: each batch
+    element is decomposed on its own; along the mode axis each element holds its IMFs from fine to
+    coarse, then its residue, then zero rows; ``P`` is one more than the largest IMF count over the
+    elements without an error. Every spatial extent must be at least 2 and at most 8192 (float32)
+    or 4096 (float64). ``return_orientation=True`` is not supported by this build and raises
+    ``ValueError``.
+
+    Host input returns NumPy arrays; a ROCm device array (a PyTorch-ROCm tensor, or a DLPack
+    producer on a ROCm device) returns a PyTorch tensor on the same device. The call waits for the
+    current PyTorch stream of the device (or ``stream``, a ``torch.cuda.Stream`` or a raw HIP
+    stream handle) before it reads device input, and its result is complete on return."""
+    max_imf = _max_imf(max_imf)
+    tie = _tie_tolerance(tie_tolerance)
+    sifts = _sifts(sifts)
+    check_finite = _flag(check_finite, "check_finite")
+    return_info = _flag(return_info, "return_info")
+    return_orientation = _flag(return_orientation, "return_orientation")
+    if not isinstance(errors, str) or errors not in _ERRORS:
+        raise ValueError(f"eda: errors={errors!r} is not one of 'raise', 'status'")
+
+    on_device = _is_device_input(x)
+    if on_device:
+        data = _device_tensor(x)
+        src = str(data.dtype).removeprefix("torch.")
+        shape = tuple(data.shape)
+        kind_ok = not (data.is_complex())
+    else:
+        data = np.asarray(x)
+        src = data.dtype.name
+        shape = data.shape
+        kind_ok = data.dtype.kind in "biuf"
+    if not kind_ok:
+        raise TypeError(f"eda: arrays of dtype {src} are not supported (real bool, integer and float arrays only)")
+    target = src if dtype is None else _dtype_name(dtype)
+    if target not in _FLOAT_DTYPES:
+        if dtype is None:
+            raise TypeError(
+                f"eda: expected a float32 or float64 array, got dtype {src} (pass dtype='float32' "
+                "or dtype='float64' to cast it)"
+            )
+        raise TypeError(f"eda: dtype= must be float32 or float64, got {target}")
+    ndim = len(shape)
+    axes = _normalize_axes(axes, ndim)
+    if return_orientation:
+        if len(axes) < 2:
+            raise ValueError(
+                "eda: return_orientation=True needs 2 or 3 spatial axes, got 1 (the method has no "
+                "orientation in 1-D)"
+            )
+        raise ValueError("eda: return_orientation=True is not supported by this build")
+    batch_axes = tuple(i for i in range(ndim) if i not in axes)
+    spatial = tuple(shape[a] for a in axes)
+    batch_shape = tuple(shape[i] for i in batch_axes)
+    B = math.prod(batch_shape)
+    out_final = _restore_order(ndim, batch_axes, axes)
+    order = list(batch_axes) + list(axes)
+    code = 0 if target == "float32" else 1
+
+    desc = _native.ArrayDesc()
+    desc.dtype = code
+    desc.dim = len(axes)
+    desc.batch = B
+    for a, n in enumerate(spatial):
+        desc.n[a] = n
+    keep_alive = None
+    stream_handle = 0
+    if on_device:
+        torch = _torch()
+        tdt = torch.float32 if code == 0 else torch.float64
+        t = data if data.dtype == tdt else data.to(tdt)
+        tv = t.permute(order) if order != list(range(ndim)) else t
+        # read in place when the batch axes merge into one stride (reference
+        # cpp/src/decompose.cpp:44-63); else a contiguous copy on the device
+        try:
+            flat = tv.view((B,) + spatial) if B > 0 else tv.reshape((0,) + spatial)
+        except RuntimeError:
+            flat = tv.contiguous().view((B,) + spatial)
+        keep_alive = flat
+        device = flat.device.index if flat.device.index is not None else torch.cuda.current_device()
+        desc.data = flat.data_ptr() if flat.numel() else None
+        desc.device = device
+        desc.batch_stride = flat.stride(0) if B > 0 else 0
+        for a in range(len(axes)):
+            desc.strides[a] = flat.stride(1 + a)
+        if stream is None:
+            stream_handle = int(torch.cuda.current_stream(device).cuda_stream)
+        elif isinstance(stream, int):
+            stream_handle = stream
+        else:
+            stream_handle = int(getattr(stream, "cuda_stream"))
+    else:
+        xc = data if data.dtype == np.dtype(target) else data.astype(target)
+        xt = np.ascontiguousarray(xc.transpose(order)) if order != list(range(ndim)) else np.ascontiguousarray(xc)
+        xb = xt.reshape((B,) + spatial)
+        keep_alive = xb
+        desc.data = xb.ctypes.data if xb.size else None
+        desc.device = -1
+
+    res = _call_native(desc, max_imf, tie, sifts, check_finite, stream_handle)
+    try:
+        info = _info_of(res, batch_shape, target, axes)
+        codes = info.error.reshape(-1)
+        internal = np.flatnonzero(codes == ERROR_INTERNAL)
+        if internal.size:
+            raise RuntimeError(
+                "eda: an internal invariant failed (error code 255) for batch index "
+                f"{', '.join(str(int(i)) for i in internal[:8])}: this is a bug in eda"
+            )
+        failed = np.flatnonzero(codes != 0)
+        if failed.size and errors == "raise":
+            raise DecompositionError(
+                _error_summary(codes[failed], failed, codes.size), indices=failed, codes=codes[failed], info=info
+            )
+        P = int(res.rows)
+        lib = _native.core()
+        err = _native.errbuf()
+        if on_device:
+            torch = _torch()
+            out = torch.empty((B, P) + spatial, dtype=torch.float32 if code == 0 else torch.float64,
+                              device=keep_alive.device)
+            if out.numel():
+                _native.check(lib.eda_collect(ctypes.byref(res), ctypes.c_void_p(out.data_ptr()), int(desc.device),
+                                              err, len(err)), err)
+            y = out.reshape(batch_shape + (P,) + spatial).permute(out_final)
+        else:
+            out = np.empty((B, P) + spatial, dtype=np.dtype(target))
+            if out.size:
+                _native.check(lib.eda_collect(ctypes.byref(res), out.ctypes.data_as(ctypes.c_void_p), -1, err,
+                                              len(err)), err)
+            computed = out.reshape(batch_shape + (P,) + spatial)
+            y = computed.transpose(out_final) if out_final != list(range(ndim + 1)) else computed
+    finally:
+        _native.core().eda_release(ctypes.byref(res))
+    del keep_alive
+    if return_info:
+        return y, info
+    return y

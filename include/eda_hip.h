@@ -1,0 +1,157 @@
+/* SPDX-License-Identifier: Apache-2.0
+ *
+ * The C ABI of the MI355X (gfx950) decomposition library libeda_hip.so: plain pointers and sizes,
+ * no C++ or framework types. It is the drop-in boundary of `eda.decompose`: the reference binds the
+ * same calls through its nanobind module `eda._core`; the Python package eda (ctypes) and any other
+ * FFI bind these entry points instead.
+ *
+ * Conventions: every entry point returns EDA_OK or one of the error codes below and writes a message
+ * starting with "eda:" into err (err may be NULL). Device pointers are HIP device pointers of the
+ * device named in the call; host pointers are ordinary memory. Calls from several threads are
+ * supported: each (thread, device) pair has its own stream and workspace.
+ */
+#ifndef EDA_HIP_H
+#define EDA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Return codes (reference cpp/python/module.cpp:28-51 maps its C++ exceptions onto the same
+ * Python classes): */
+#define EDA_OK 0
+#define EDA_EINVAL 1    /* an invalid argument: Python ValueError */
+#define EDA_ENOMEM 2    /* host or device memory exhausted: Python MemoryError */
+#define EDA_EHIP 3      /* a HIP runtime call or a kernel launch failed: Python eda.CudaError */
+#define EDA_EINTERNAL 4 /* an internal invariant failed: Python RuntimeError */
+
+/* dtype codes */
+#define EDA_F32 0
+#define EDA_F64 1
+
+/* The input of a call: `batch` elements of `dim` (1..3) spatial axes. Host input (device = -1) is
+ * C-contiguous (batch, n[0], .., n[dim-1]); device input is read in place with the given element
+ * strides (any sign), the batch axes merged into one stride. */
+typedef struct eda_array_desc {
+  const void* data;
+  int32_t device; /* -1: host memory; else the HIP device that holds data */
+  int32_t dtype;  /* EDA_F32 or EDA_F64: the compute dtype */
+  int32_t dim;
+  int32_t reserved;
+  int64_t batch;
+  int64_t n[3];
+  int64_t batch_stride; /* in elements (device input) */
+  int64_t strides[3];   /* in elements (device input) */
+} eda_array_desc;
+
+/* The parameters of a call (reference cpp/src/decompose.hpp:69-76, src/eda/_api.py:184-353). */
+typedef struct eda_params {
+  int32_t max_imf;      /* <= 0: no limit */
+  int32_t sifts;        /* >= 1 */
+  int32_t check_finite; /* 1: an element with inf / NaN input fails with error 1 */
+  int32_t orientation;  /* must be 0 (return_orientation is not built) */
+  double tie_tolerance; /* < 0: the dtype's default (2^-14 float32, 2^-30 float64) */
+  int64_t max_chunk;    /* > 0: at most this many elements per chunk; 0: by free memory */
+  uint64_t stream;      /* a hipStream_t the call waits for before it reads device input (0: none) */
+} eda_params;
+
+/* The result of eda_decompose. The rows stay in a library-owned device buffer until eda_collect
+ * copies them out; eda_release frees everything. Info arrays are host arrays owned by the result:
+ * n_imfs, error [batch]; windows, sifts, spacing, energy [batch * K] (K = max n_imfs; entries
+ * beyond an element's n_imfs and all entries of a failed element are 0). */
+typedef struct eda_result {
+  void* handle;
+  int64_t batch;
+  int64_t count;  /* samples per element */
+  int32_t rows;   /* P = max n_imfs + 1 over the elements without an error (>= 1 if any failed) */
+  int32_t K;
+  int32_t dtype;
+  int32_t device;
+  int64_t chunk;  /* elements per chunk */
+  int32_t* n_imfs;
+  int32_t* error; /* 0, 1 (non-finite input), 2 (non-finite intermediate or row), 255 (internal) */
+  int32_t* windows;
+  int32_t* sifts;
+  double* spacing;
+  double* energy;
+} eda_result;
+
+/* Decompose every element of x: modes (IMFs, fine to coarse) and the residue. Replaces the
+ * reference's `eda._core.decompose_host` (cpp/python/decompose.cpp:325-384) and
+ * `eda._core.decompose_device` (cpp/python/decompose.cpp:452-509), which both run
+ * eda::decompose (cpp/src/decompose.hpp:106). */
+int eda_decompose(const eda_array_desc* x, const eda_params* p, eda_result* out, char* err, size_t errlen);
+
+/* Copies rows [0, P) of every element into dst, C-contiguous (batch, P, *spatial) in the compute
+ * dtype: rows beyond an element's n_imfs are 0, the rows of a failed element NaN. dst_device -1:
+ * host memory (synchronous); else device memory of that device, complete on return. Replaces
+ * eda::Run::collect (cpp/src/decompose.hpp:100-101). */
+int eda_collect(const eda_result* r, void* dst, int32_t dst_device, char* err, size_t errlen);
+
+/* Frees the rows and the info arrays of a result (idempotent). */
+void eda_release(eda_result* r);
+
+/* Replaces eda._core.build_info / device_count / device_info / pointer_device
+ * (cpp/python/module.cpp:64-142): */
+int eda_build_info(char* buf, size_t len);                   /* a JSON object */
+int eda_device_count(void);                                  /* 0 without a driver or device */
+int eda_device_info(int32_t device, char* buf, size_t len);  /* a JSON object */
+int eda_pointer_device(const void* ptr, int32_t* device);    /* -1 for host memory */
+
+/* ---- Primitive entry points (device pointers; each call synchronizes before it returns). They
+ * expose the method's stages one by one, so that each can be checked against its closed form and
+ * against the CPU oracle; the decomposition itself does not call them. Fields are C-contiguous
+ * (batch, n[0], .., n[dim-1]). ---- */
+
+/* The separable cube filter of every element with side win[b] (host array; 0 skips the element):
+ * op 0 max, 1 min, 2 max into out0 and min into out1, 3 box mean (reference
+ * cpp/src/method/common.cpp:65-136, enqueue_cube). */
+int eda_prim_cube(int32_t dtype, int32_t dim, int64_t batch, const int64_t* n, int32_t op, const int32_t* win,
+                  const void* in, void* out0, void* out1, char* err, size_t errlen);
+/* Per element: the canonical energy, max |v| and the count of non-finite samples (host outputs;
+ * reference cpp/src/field/field_ops.cu:52-83). */
+int eda_prim_stats(int32_t dtype, int32_t dim, int64_t batch, const int64_t* n, const void* f, double* energy,
+                   double* amax, uint64_t* nonfinite, char* err, size_t errlen);
+/* The median (mode 0) or the linear quantile q (mode 1) of every element's values where
+ * mask & bits != 0 (mask NULL: all), as numpy; out, count: host [batch] (reference
+ * cpp/src/method/stats.cu:261-295). */
+int eda_prim_select(int32_t dtype, int32_t dim, int64_t batch, const int64_t* n, const void* field,
+                    const uint8_t* mask, int32_t bits, int32_t mode, double q, double* out, int64_t* count, char* err,
+                    size_t errlen);
+/* The noise level (1.4826 MAD / sqrt 2 of the forward differences), pooled or the largest per axis
+ * (reference cpp/src/method/stats.cu:299-334); out: host [batch]. */
+int eda_prim_noise(int32_t dtype, int32_t dim, int64_t batch, const int64_t* n, const void* r, int32_t per_axis,
+                   double* out, char* err, size_t errlen);
+/* The threshold field with boxes Wd, Wl and per-element sigma_n, tol (host arrays); nonfinite:
+ * host [batch] (reference cpp/src/method/stats_fields.cu:375-389). */
+int eda_prim_tau(int32_t dtype, int32_t dim, int64_t batch, const int64_t* n, const void* r, const int32_t* Wd,
+                 const int32_t* Wl, const double* sigma_n, const double* tol, void* tau, int32_t* nonfinite, char* err,
+                 size_t errlen);
+/* Significant extrema flags (bit a per axis) and their counts ext: host [batch * 3] (reference
+ * cpp/src/method/stats_fields.cu:392-422). */
+int eda_prim_flags(int32_t dtype, int32_t dim, int64_t batch, const int64_t* n, const void* r, const void* tau,
+                   uint8_t* flags, uint64_t* ext, char* err, size_t errlen);
+/* The raw local window of the flags with box win[b] (host) (reference stats_fields.cu:425-440). */
+int eda_prim_raw_window(int32_t dtype, int32_t dim, int64_t batch, const int64_t* n, const uint8_t* flags,
+                        const int32_t* win, void* om, char* err, size_t errlen);
+/* Face-connected components of mask & bits: labels = the smallest in-element index of the
+ * component, -1 outside (reference cpp/src/method/stats_scale.cu:208-220). */
+int eda_prim_ccl(int32_t dim, int64_t batch, const int64_t* n, const uint8_t* mask, int32_t bits, int32_t* labels,
+                 char* err, size_t errlen);
+/* The orthonormal N-D DCT-II (inverse 0) or its inverse with the sin basis on the axes of the bit
+ * mask odd (inverse 1) of every element (the reference's DctPlans, cpp/src/field/dct.hpp:35-85). */
+int eda_prim_dct(int32_t dtype, int32_t dim, int64_t batch, const int64_t* n, const void* in, void* out,
+                 int32_t inverse, int32_t odd, char* err, size_t errlen);
+/* The orientation n (dim fields, [a][batch][count]) and the coherence of every element in the band
+ * of window w[b] (host) (reference cpp/src/method/orientation.cu:771-778). */
+int eda_prim_orientation(int32_t dtype, int32_t dim, int64_t batch, const int64_t* n, const void* r, const int32_t* w,
+                         void* nvec, void* coh, char* err, size_t errlen);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EDA_HIP_H */

@@ -28,7 +28,7 @@
 //   ccl                       scipy.ndimage.label after canonical relabelling.
 //   dct / riesz               scipy.fft.dctn / idctn(norm='ortho') and DCT-basis plane waves.
 //   eigen                     numpy.linalg.eigh.
-//   rung_ladder               the KAT cpp/tests/rungs_table.inc (72 rows).
+//   rung_ladder               the KAT cpp/tests/rungs_table.inc (66 rows).
 //   interp (gather)           scipy.ndimage.map_coordinates(order=1) on the reflected extension.
 //   decompose (T = double)    the golden corpus tests/data/golden (9 FP64 cases: decisions exact,
 //                              rows within the corpus's own bound).

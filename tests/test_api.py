@@ -1,0 +1,83 @@
+# SPDX-License-Identifier: Apache-2.0
+"""The argument rules of eda.decompose (CPU: every rule raises before any GPU work), restated from
+the reference's tests/python/test_api.py:100-210 and src/eda/_api.py:433-526, and the layout
+permutation of the result."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import eda
+from eda._api import _normalize_axes, _restore_order
+
+X = np.sin(np.arange(64.0))
+
+
+@pytest.mark.parametrize("bad", [True, 1.5, "3", None])
+def test_max_imf_must_be_an_integer(bad):
+    with pytest.raises(TypeError):
+        eda.decompose(X, max_imf=bad)
+
+
+@pytest.mark.parametrize("bad,exc", [(0, ValueError), (-1, ValueError), (True, TypeError), (2.0, TypeError)])
+def test_sifts_rules(bad, exc):
+    with pytest.raises(exc):
+        eda.decompose(X, sifts=bad)
+
+
+@pytest.mark.parametrize("bad,exc", [(-1.0, ValueError), (float("inf"), ValueError), (float("nan"), ValueError),
+                                     (True, TypeError), ("1e-3", TypeError)])
+def test_tie_tolerance_rules(bad, exc):
+    with pytest.raises(exc):
+        eda.decompose(X, tie_tolerance=bad)
+
+
+def test_flags_must_be_bool():
+    for name in ("check_finite", "return_info", "return_orientation"):
+        with pytest.raises(TypeError):
+            eda.decompose(X, **{name: 1})
+
+
+def test_errors_policy_name():
+    with pytest.raises(ValueError):
+        eda.decompose(X, errors="ignore")
+
+
+def test_dtype_rules():
+    with pytest.raises(TypeError):
+        eda.decompose(np.arange(64))  # int64 without dtype=
+    with pytest.raises(TypeError):
+        eda.decompose(X, dtype="float16")
+    with pytest.raises(TypeError):
+        eda.decompose(X.astype(np.complex128))
+
+
+def test_axes_rules():
+    assert _normalize_axes(None, 2) == (0, 1)
+    assert _normalize_axes(-1, 3) == (2,)
+    assert _normalize_axes((2, 0), 3) == (2, 0)
+    with pytest.raises(ValueError):
+        _normalize_axes(None, 4)
+    with pytest.raises(ValueError):
+        _normalize_axes((0, 0), 2)
+    with pytest.raises(np.exceptions.AxisError):
+        _normalize_axes(3, 2)
+    with pytest.raises(ValueError):
+        _normalize_axes((0, 1, 2, 3), 4)
+    with pytest.raises(TypeError):
+        _normalize_axes((True,), 2)
+
+
+def test_return_orientation_needs_2_axes_and_is_not_built():
+    with pytest.raises(ValueError, match="2 or 3 spatial axes"):
+        eda.decompose(X, return_orientation=True)
+    with pytest.raises(ValueError, match="not supported"):
+        eda.decompose(np.zeros((8, 8)), return_orientation=True)
+
+
+def test_restore_order_inserts_the_mode_axis_before_the_first_spatial_axis():
+    # x of shape (b0, s0, b1, s1), axes (1, 3): computed (b0, b1, P, s0, s1) -> (b0, P, s0, b1, s1)
+    perm = _restore_order(4, (0, 2), (1, 3))
+    comp = np.zeros((2, 3, 7, 5, 6))
+    assert comp.transpose(perm).shape == (2, 7, 5, 3, 6)
