@@ -1,0 +1,188 @@
+# SPDX-License-Identifier: Apache-2.0
+"""Benchmark of the decomposition hot path on one MI355X: eda.decompose of 64 float32 images of
+512 x 512 (rings, warped stripes, two Gaussian blobs and noise; per-image seeds), the input already
+resident in HBM as a PyTorch-ROCm tensor, default parameters.
+
+A step is one eda.decompose call over the whole batch (every mode of every image, rows collected
+into a device tensor). W untimed warm-up steps, then K timed steps bracketed by a device
+synchronization. Prints one JSON line on stdout (progress goes to stderr):
+
+  value      Msamples/s = 64 * 512 * 512 / (seconds per step)
+  roofline   the dominant kernel, the cube filter's vHGW pass (k_vhgw): algorithmic bytes per
+             launch (its input field(s) read once, output field(s) written once, over the running
+             elements) / its average launch duration, both measured live with HIP events on the
+             library's stream over the timed steps, against 8 TB/s; `traffic` the HBM bytes per
+             launch from rocprofv3 FETCH_SIZE + WRITE_SIZE (profiles/), null when not measured
+  cpu_baseline  the oracle (a C++ restatement of the method, FP64, one image per host thread)
+             on a bounded sample of the same workload, on this host's cores
+
+Usage: python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--no-cpu]
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+for _p in (ROOT / "eda-cuda_amd", ROOT / "oracle", ROOT / "tests"):
+    if str(_p) not in sys.path:
+        sys.path.insert(0, str(_p))
+
+METRIC = "Msamples/s decomposed (64×512² FP32), fwd and fwd+bwd, % of HBM roofline"
+HBM_PEAK_GBS = 8000.0
+B, N = 64, 512
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles():
+    """HBM bytes per k_vhgw launch measured by rocprofv3 (profiles/vhgw_pass_64x512x512_f32.json),
+    or None."""
+    p = ROOT / "profiles" / "vhgw_pass_64x512x512_f32.json"
+    if not p.exists():
+        return None
+    try:
+        return json.loads(p.read_text()).get("traffic_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def cpu_baseline():
+    import numpy as np
+
+    import oracle
+    import workloads
+
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    threads = max(1, min(16, cores))
+    x = workloads.rings_stripes_blobs(threads, N).astype(np.float64)
+    log(f"bench: cpu baseline: the FP64 oracle on {threads} image(s) of {N}x{N}, {threads} thread(s)")
+    t0 = time.perf_counter()
+    oracle.decompose(x, dim=2, threads=threads)
+    dt = time.perf_counter() - t0
+    return {
+        "value": round(threads * N * N / dt / 1e6, 6),
+        "unit": "Msamples/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{threads} of the 64 images ({N}x{N}, float64 oracle, one image per thread), {dt:.1f} s",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(
+            "bench.py: this build measures one GPU (a single-GPU library; elements shard across GPUs "
+            "with no exchange, see DESIGN.md); --gpus must be 1",
+            file=sys.stderr,
+        )
+        return 2
+
+    import numpy as np
+    import torch
+
+    import eda
+    import workloads
+    from eda import _native
+
+    assert torch.cuda.is_available(), "bench: no GPU"
+    lib = _native.core()
+    log("bench: generating 64 x 512 x 512 float32 inputs")
+    x = torch.from_numpy(workloads.rings_stripes_blobs(B, N, np.float32)).to("cuda:0")
+    torch.cuda.synchronize()
+
+    info = None
+    for i in range(args.warmup):
+        t = time.perf_counter()
+        y, info = eda.decompose(x, axes=(1, 2), return_info=True)
+        torch.cuda.synchronize()
+        log(f"bench: warm-up {i}: {time.perf_counter() - t:.3f} s, n_imfs {sorted(set(info.n_imfs.tolist()))}")
+        del y
+
+    err = _native.errbuf()
+    lib.eda_timer_enable(1)
+    _native.check(lib.eda_timer_read(ctypes.byref(ctypes.c_double()), ctypes.byref(ctypes.c_int64()),
+                                     ctypes.byref(ctypes.c_double()), err, len(err)), err)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        y, info = eda.decompose(x, axes=(1, 2), return_info=True)
+        del y
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / max(args.steps, 1)
+    ms = ctypes.c_double()
+    launches = ctypes.c_int64()
+    nbytes = ctypes.c_double()
+    _native.check(lib.eda_timer_read(ctypes.byref(ms), ctypes.byref(launches), ctypes.byref(nbytes), err, len(err)),
+                  err)
+    lib.eda_timer_enable(0)
+    log(f"bench: {args.steps} steps, {dt * 1e3:.1f} ms per step; k_vhgw: {launches.value} launches, "
+        f"{ms.value:.1f} ms in total")
+
+    value = B * N * N / dt / 1e6
+    per_launch_bytes = nbytes.value / max(launches.value, 1)
+    avg_s = ms.value * 1e-3 / max(launches.value, 1)
+    achieved = per_launch_bytes / avg_s / 1e9 if avg_s > 0 else 0.0
+    roofline = {
+        "bound": "hbm",
+        "kernel": "k_vhgw",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic_from_profiles(),
+        "algorithmic_bytes_per_launch": round(per_launch_bytes),
+        "avg_launch_us": round(avg_s * 1e6, 2),
+        "launches_per_step": launches.value // max(args.steps, 1),
+        "share_of_step": round(ms.value * 1e-3 / max(args.steps, 1) / dt, 4),
+    }
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "eda.decompose fwd, 64 x 512 x 512 float32 images (rings + warped stripes + "
+                        "2 Gaussian blobs + N(0, 0.05^2) noise, seed = image index), default parameters "
+                        "(sifts 3), input resident in HBM",
+            "batch": B,
+            "shape": [N, N],
+            "pass": "fwd",
+            "n_imfs": sorted(set(info.n_imfs.tolist())) if info is not None else None,
+        },
+        "roofline": roofline,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -27,6 +27,7 @@
 #include <set>
 #include <string>
 #include <thread>
+#include <atomic>
 #include <vector>
 
 namespace eda {
@@ -37,6 +38,90 @@ void throw_hip(hipError_t e, const char* expr, const char* file, int line) {
                 file, line);
   if (e == hipErrorOutOfMemory) throw OutOfDeviceMemory(buf);
   throw HipError(e, buf);
+}
+
+// ---- the pass timer and the host mirrors of window arrays ----------------------------------------
+namespace {
+struct TimerState {
+  std::mutex mu;
+  std::atomic<bool> on{false};
+  std::vector<hipEvent_t> pool;
+  struct Span {
+    hipEvent_t a, b;
+    double bytes;
+  };
+  std::vector<Span> spans;
+  hipEvent_t take() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    EDA_HIP(hipEventCreate(&e));
+    return e;
+  }
+};
+TimerState& timer_state() {
+  static TimerState t;
+  return t;
+}
+struct Mirror {
+  const char* dev;
+  const char* host;
+  size_t bytes;
+};
+thread_local std::vector<Mirror> g_mirrors;
+}  // namespace
+
+void timer_set(bool on) { timer_state().on = on; }
+bool timer_on() { return timer_state().on.load(std::memory_order_relaxed); }
+void timer_note(hipStream_t s, hipEvent_t* start) {
+  TimerState& t = timer_state();
+  std::lock_guard<std::mutex> lock(t.mu);
+  *start = t.take();
+  EDA_HIP(hipEventRecord(*start, s));
+}
+void timer_done(hipStream_t s, hipEvent_t start, double bytes) {
+  TimerState& t = timer_state();
+  std::lock_guard<std::mutex> lock(t.mu);
+  hipEvent_t stop = t.take();
+  EDA_HIP(hipEventRecord(stop, s));
+  t.spans.push_back({start, stop, bytes});
+}
+void timer_read(double* ms, int64_t* launches, double* bytes) {
+  TimerState& t = timer_state();
+  std::lock_guard<std::mutex> lock(t.mu);
+  double sum = 0.0, by = 0.0;
+  for (const auto& sp : t.spans) {
+    EDA_HIP(hipEventSynchronize(sp.b));
+    float e = 0.f;
+    EDA_HIP(hipEventElapsedTime(&e, sp.a, sp.b));
+    sum += e;
+    by += sp.bytes;
+    t.pool.push_back(sp.a);
+    t.pool.push_back(sp.b);
+  }
+  *ms = sum;
+  *launches = static_cast<int64_t>(t.spans.size());
+  *bytes = by;
+  t.spans.clear();
+}
+void mirror_note(const void* dev, const void* host, size_t bytes) {
+  g_mirrors.push_back({static_cast<const char*>(dev), static_cast<const char*>(host), bytes});
+}
+void mirror_clear() { g_mirrors.clear(); }
+int64_t mirror_active(const int32_t* win, int64_t n) {
+  const char* p = reinterpret_cast<const char*>(win);
+  for (auto it = g_mirrors.rbegin(); it != g_mirrors.rend(); ++it) {
+    if (p >= it->dev && p + n * sizeof(int32_t) <= it->dev + it->bytes) {
+      const int32_t* h = reinterpret_cast<const int32_t*>(it->host + (p - it->dev));
+      int64_t c = 0;
+      for (int64_t b = 0; b < n; ++b) c += h[b] != 0 ? 1 : 0;
+      return c;
+    }
+  }
+  return -1;
 }
 
 MethodCtl method_ctl(const Geom& g, bool f32) {
@@ -152,13 +237,17 @@ struct Uploader {
     char* d = static_cast<char*>(dev.p) + a;
     EDA_HIP(hipMemcpyAsync(d, host + a, bytes, hipMemcpyHostToDevice, s));
     off = a + bytes;
+    if (timer_on()) mirror_note(d, host + a, bytes);
     return reinterpret_cast<V*>(d);
   }
   template <class V>
   V* put(const std::vector<V>& v) {
     return put(v.data(), v.size());
   }
-  void reset() { off = 0; }  // only after the stream is synchronized and no upload is read again
+  void reset() {  // only after the stream is synchronized and no upload is read again
+    off = 0;
+    mirror_clear();
+  }
   void reserve(size_t bytes) {
     if (bytes <= cap) return;
     if (host) EDA_HIP(hipHostFree(host));
@@ -610,7 +699,7 @@ class Chunk {
   ElemDev* d_el_;
   int32_t* d_rung_win_;
   const int32_t* on_op_ = nullptr;
-  std::vector<int32_t> rw_;
+  std::vector<int32_t> rw_, rw_cur_;
   std::vector<int32_t> rung_max_;
   int K_max_ = 0;
   std::vector<Rec> rec_;
@@ -1177,6 +1266,8 @@ bool Chunk<T>::mode() {
     }
   }
   rw_ = rw;
+  rw_cur_ = rw;
+  if (timer_on() && !rw_cur_.empty()) mirror_note(d_rung_win_, rw_cur_.data(), rw_cur_.size() * sizeof(int32_t));
   on = upload_on(active);
   on_op_ = on;
 
@@ -1255,6 +1346,8 @@ bool Chunk<T>::mode() {
           if (!sifting(rec_[b])) rw[jj * B + b] = 0;
       const int32_t* drw = ctx_.up.put(rw);
       EDA_HIP(hipMemcpyAsync(d_rung_win_, drw, sizeof(int32_t) * rw.size(), hipMemcpyDeviceToDevice, s_));
+      rw_cur_ = rw;
+      if (timer_on() && !rw_cur_.empty()) mirror_note(d_rung_win_, rw_cur_.data(), rw_cur_.size() * sizeof(int32_t));
     }
     T* plain = X[0];
     T* up = X[1];
@@ -1716,6 +1809,15 @@ int eda_pointer_device(const void* ptr, int32_t* device) {
     }
     *device = attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged ? attr.device : -1;
   });
+}
+
+int eda_timer_enable(int32_t on) {
+  timer_set(on != 0);
+  return EDA_OK;
+}
+
+int eda_timer_read(double* ms, int64_t* launches, double* bytes, char* err, size_t errlen) {
+  return guarded(err, errlen, [&] { timer_read(ms, launches, bytes); });
 }
 
 }  // extern "C"

@@ -24,6 +24,22 @@ struct MethodCtl {
 };
 MethodCtl method_ctl(const Geom& g, bool f32);
 
+// ---- the pass timer (the benchmark's live roofline figure) ---------------------------------------
+// While enabled, every launch of the cube filter's pass kernel (k_vhgw) is bracketed by two HIP
+// events on its stream, and its algorithmic bytes are counted: the pass's input field(s) read once
+// and output field(s) written once, over the elements whose window is non-zero. The count of those
+// elements comes from the host mirror of the window array (no device read-back).
+void timer_set(bool on);
+bool timer_on();
+void timer_note(hipStream_t s, hipEvent_t* start);                      // before a launch
+void timer_done(hipStream_t s, hipEvent_t start, double bytes);         // after it
+void timer_read(double* ms, int64_t* launches, double* bytes);          // synchronizes, sums, clears
+// Host mirrors of device window arrays (registered by the driver's uploads): the active count of
+// win[0 .. n) for the timer, or -1 when the array is not mirrored.
+void mirror_note(const void* dev, const void* host, size_t bytes);
+void mirror_clear();
+int64_t mirror_active(const int32_t* win, int64_t n);
+
 // ---- the cube filter (vhgw.hip) ------------------------------------------------------------------
 enum class CubeOp : int8_t { kMax = 0, kMin = 1, kMaxMin = 2, kBox = 3 };
 // win [B]: one odd side per element (0: skipped); max_w: the largest side of the launch (host).

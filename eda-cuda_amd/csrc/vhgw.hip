@@ -339,6 +339,9 @@ void enqueue_vhgw(hipStream_t s, const Geom& g, int axis, int nf, bool same_inpu
   const int64_t wgs = g.batch * a.groups;
   const bool sum = ops[0] == kVSum;
   const size_t lds = lds_bytes<T>(n, P, a.tmax);
+  hipEvent_t t0 = nullptr;
+  const bool timed = timer_on();
+  if (timed) timer_note(s, &t0);
   if (sum) {
     auto k = k_vhgw<T, T>;
     if (lds > 65536) EDA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
@@ -349,6 +352,12 @@ void enqueue_vhgw(hipStream_t s, const Geom& g, int axis, int nf, bool same_inpu
     hipLaunchKernelGGL(k, dim3(static_cast<unsigned>(wgs)), dim3(kThreads), lds, s, a, p);
   }
   EDA_LAUNCH("k_vhgw");
+  if (timed) {
+    int64_t active = mirror_active(win, g.batch);
+    if (active < 0) active = g.batch;
+    const int fields = (nf == 2 && !same_input ? 2 : 1) + nf;
+    timer_done(s, t0, static_cast<double>(fields) * active * g.count * sizeof(T));
+  }
 }
 
 // The cube (reference cpp/src/method/common.cpp:65-136): box runs axis 0 first, the last pass

@@ -101,6 +101,15 @@ int eda_device_count(void);                                  /* 0 without a driv
 int eda_device_info(int32_t device, char* buf, size_t len);  /* a JSON object */
 int eda_pointer_device(const void* ptr, int32_t* device);    /* -1 for host memory */
 
+/* Kernel timing, for the live roofline figure of a benchmark: while enabled, every launch of the
+ * cube filter's pass kernel is bracketed by HIP events on the stream it runs on, and its algorithmic
+ * bytes (input fields read once, output fields written once, over the running elements) are
+ * counted. eda_timer_read waits for the recorded launches, returns the total kernel time in ms, the
+ * launch count and the bytes, and clears them. No counterpart in the reference (it has no timing
+ * in code). */
+int eda_timer_enable(int32_t on);
+int eda_timer_read(double* ms, int64_t* launches, double* bytes, char* err, size_t errlen);
+
 /* ---- Primitive entry points (device pointers; each call synchronizes before it returns). They
  * expose the method's stages one by one, so that each can be checked against its closed form and
  * against the CPU oracle; the decomposition itself does not call them. Fields are C-contiguous

@@ -73,7 +73,8 @@ def noise(x: np.ndarray, per_axis: bool):
     B = x.shape[0]
     sh = x.shape[1:]
     out = np.zeros(B)
-    call("eda_prim_noise", code(x.dtype), len(sh), B, shape_arr(sh), ptr(dev(x)), int(per_axis),
+    xt = dev(x)  # kept alive across the call (a temporary's block can be handed out again)
+    call("eda_prim_noise", code(x.dtype), len(sh), B, shape_arr(sh), ptr(xt), int(per_axis),
          out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
     return out
 
@@ -84,7 +85,8 @@ def stats(x: np.ndarray):
     e = np.zeros(B)
     a = np.zeros(B)
     nf = np.zeros(B, np.uint64)
-    call("eda_prim_stats", code(x.dtype), len(sh), B, shape_arr(sh), ptr(dev(x)),
+    xt = dev(x)
+    call("eda_prim_stats", code(x.dtype), len(sh), B, shape_arr(sh), ptr(xt),
          e.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
          nf.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
     return e, a, nf
@@ -110,7 +112,9 @@ def flags(x: np.ndarray, t: np.ndarray):
     sh = x.shape[1:]
     fl = torch.empty(x.shape, dtype=torch.uint8, device="cuda")
     ext = np.zeros(3 * B, np.uint64)
-    call("eda_prim_flags", code(x.dtype), len(sh), B, shape_arr(sh), ptr(dev(x)), ptr(dev(t.astype(x.dtype))), ptr(fl),
+    xt = dev(x)
+    tt = dev(t.astype(x.dtype))
+    call("eda_prim_flags", code(x.dtype), len(sh), B, shape_arr(sh), ptr(xt), ptr(tt), ptr(fl),
          ext.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
     return fl.cpu().numpy(), ext.reshape(B, 3)
 
@@ -120,7 +124,8 @@ def raw_window(fl: np.ndarray, wins, dtype):
     B = fl.shape[0]
     sh = fl.shape[1:]
     om = torch.empty(fl.shape, dtype=torch.float32 if code(dtype) == 0 else torch.float64, device="cuda")
-    call("eda_prim_raw_window", code(dtype), len(sh), B, shape_arr(sh), ptr(dev(fl.astype(np.uint8))),
+    ft = dev(fl.astype(np.uint8))
+    call("eda_prim_raw_window", code(dtype), len(sh), B, shape_arr(sh), ptr(ft),
          (ctypes.c_int32 * B)(*wins), ptr(om))
     return om.cpu().numpy()
 
@@ -130,7 +135,8 @@ def ccl(mask: np.ndarray):
     B = mask.shape[0]
     sh = mask.shape[1:]
     lab = torch.empty(mask.shape, dtype=torch.int32, device="cuda")
-    call("eda_prim_ccl", len(sh), B, shape_arr(sh), ptr(dev(mask.astype(np.uint8))), 1, ptr(lab))
+    mt = dev(mask.astype(np.uint8))
+    call("eda_prim_ccl", len(sh), B, shape_arr(sh), ptr(mt), 1, ptr(lab))
     return lab.cpu().numpy()
 
 
