@@ -424,9 +424,19 @@ __global__ void __launch_bounds__(kThreads) k_sift_step(int64_t count, const int
     mr = br > mr ? br : mr;
     mh = bh > mh ? bh : mh;
   }
+  __shared__ uint64_t red[2][kThreads / 64];
   mr = wave_max_u64(mr);
   mh = wave_max_u64(mh);
   if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = mr;
+    red[1][threadIdx.x >> 6] = mh;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kThreads / 64; ++w) {
+      mr = red[0][w] > mr ? red[0][w] : mr;
+      mh = red[1][w] > mh ? red[1][w] : mh;
+    }
     if (mr) atomicMax(reinterpret_cast<unsigned long long*>(&gmax[2 * b]), static_cast<unsigned long long>(mr));
     if (mh) atomicMax(reinterpret_cast<unsigned long long*>(&gmax[2 * b + 1]), static_cast<unsigned long long>(mh));
   }
@@ -444,7 +454,8 @@ class Chunk {
     const int K = D * (D + 1) / 2;
     nfields_ = 6 + D + 12 + D + 1 + (D >= 2 ? 3 + K : 0);
     const size_t fbytes = static_cast<size_t>(F) * sizeof(T);
-    const size_t small = 64 * 1024 + static_cast<size_t>(B_) * (1024 + sizeof(ElemDev) + kMaxRungs * 4);
+    const size_t small = 64 * 1024 + static_cast<size_t>(B_) * (1024 + sizeof(ElemDev) + kMaxRungs * 4) +
+                         static_cast<size_t>(orient_scratch_doubles(B_)) * sizeof(double);
     const size_t sel = std::max(noise_ws_bytes(g), select_ws_bytes(g));
     const size_t bytes = (nfields_ + 1) * (fbytes + 256) + 8 * (static_cast<size_t>(F) + 256) +
                          static_cast<size_t>(F) * 4 + 256 + small + sel + elem_stats_ws_bytes(g) + 4096;
@@ -488,6 +499,7 @@ class Chunk {
     d_gmax_ = cv.take<uint64_t>(2 * B_);
     d_el_ = cv.take<ElemDev>(B_);
     d_rung_win_ = cv.take<int32_t>(static_cast<size_t>(kMaxRungs) * B_);
+    d_oscr_ = cv.take<double>(static_cast<size_t>(orient_scratch_doubles(B_)));
     if (cv.off > bytes) throw std::logic_error("eda: the workspace layout overflows");
     rec_.assign(static_cast<size_t>(B_), Rec{});
     ctx_.up.reserve((size_t{16} << 20) + static_cast<size_t>(B_) * 16384);
@@ -660,18 +672,29 @@ class Chunk {
   void raw_window(const uint8_t* flags, const int32_t* win, int32_t maxw, const int32_t* on, T* om, T* tmp);
   void envelope_mean(const T* h, T* m, T* u, T* l) {
     const Geom g = g_;
-    enqueue_element_op<T>(s_, g_, 2, d_el_, lo_, t_, o_, d_rung_win_, rung_max_.data(), K_max_, h, u, l, f0(), f1());
+    elem_op(2, h, u, l);
     T* mid = X_[3];
     const int32_t* on = on_op_;
     each(s_, g, on, [=] __device__(int64_t b, int64_t i) {
       const int64_t at = b * g.count + i;
       mid[at] = T(0.5) * (u[at] + l[at]);
     });
-    enqueue_element_op<T>(s_, g_, 3, d_el_, lo_, t_, o_, d_rung_win_, rung_max_.data(), K_max_, mid, m, nullptr, f0(),
-                          f1());
+    elem_op(3, mid, m, nullptr);
   }
-  T* f0() { return X_[4]; }
-  T* f1() { return f1_; }
+  // An operator of the element over every rung (the rung scratch grows to the mode's ladder).
+  void elem_op(int op, const T* in, T* out0, T* out1) {
+    const size_t need = static_cast<size_t>(K_max_) * static_cast<size_t>(B_ * g_.count);
+    if (need > rung_cap_) {
+      EDA_HIP(hipStreamSynchronize(s_));
+      rung_mem_ = DevMem();
+      rung_mem_ = DevMem(2 * need * sizeof(T));
+      rung_cap_ = need;
+    }
+    T* r0 = static_cast<T*>(rung_mem_.p);
+    T* r1 = r0 + rung_cap_;
+    enqueue_element_op<T>(s_, g_, op, d_el_, lo_, t_, o_, d_rung_win_, rung_max_.data(), K_max_, in, out0, out1, r0, r1,
+                          on_op_);
+  }
 
   Context& ctx_;
   Geom g_;
@@ -688,6 +711,8 @@ class Chunk {
   T *r_, *L_, *Lp_, *h_, *t_, *o_, *nvec_, *coh_, *owork_ = nullptr;
   T* X_[12];
   T* f1_ = nullptr;
+  DevMem rung_mem_;
+  size_t rung_cap_ = 0;
   uint8_t *flags_, *ff_, *P_[2], *Pacc_, *A_, *lo_, *aw_;
   int32_t* labels_;
   char *sel_ws_, *stats_ws_;
@@ -698,6 +723,7 @@ class Chunk {
   uint32_t* d_bits_;
   ElemDev* d_el_;
   int32_t* d_rung_win_;
+  double* d_oscr_;
   const int32_t* on_op_ = nullptr;
   std::vector<int32_t> rw_, rw_cur_;
   std::vector<int32_t> rung_max_;
@@ -1195,7 +1221,7 @@ bool Chunk<T>::mode() {
       if (rec_[b].status == 0) wv[b] = rec_[b].w;
     const int32_t* dw = ctx_.up.put(wv);
     EDA_HIP(hipMemsetAsync(d_nonfinite_, 0, sizeof(int32_t) * B, s_));
-    enqueue_orientation<T>(s_, ctx_.device, g_, r_, dw, on, nvec_, coh_, owork_, d_nonfinite_);
+    enqueue_orientation<T>(s_, ctx_.device, g_, r_, dw, on, nvec_, coh_, owork_, d_nonfinite_, d_oscr_);
   }
   {
     std::vector<uint64_t> init(2 * B);
@@ -1300,8 +1326,7 @@ bool Chunk<T>::mode() {
     T* pp[2] = {X[6], X[7]};
     for (int32_t it = 0; it < c.u; ++it) {
       T* dst = pp[it & 1];
-      enqueue_element_op<T>(s_, g_, 0, d_el_, lo_, t_, o_, d_rung_win_, rung_max_.data(), K_max_, U, dst, nullptr,
-                            f0(), f1());
+      elem_op(0, U, dst, nullptr);
       U = dst;
     }
     T* gw = U == X[6] ? X[7] : X[6];
@@ -1320,8 +1345,7 @@ bool Chunk<T>::mode() {
     for (int32_t it = 0; it < c.s; ++it) {
       T* dst = a;
       if (it + 1 < c.s) dst = cur == X[5] ? X[6] : X[5];
-      enqueue_element_op<T>(s_, g_, 3, d_el_, lo_, t_, o_, d_rung_win_, rung_max_.data(), K_max_, cur, dst, nullptr,
-                            f0(), f1());
+      elem_op(3, cur, dst, nullptr);
       cur = dst;
     }
     each(s_, g, on, [=] __device__(int64_t b, int64_t i) {
@@ -1359,8 +1383,7 @@ bool Chunk<T>::mode() {
       hd[at] = T(0.5) * (up[at] - lo[at]);
     });
     T* e0 = X[6];
-    enqueue_element_op<T>(s_, g_, 3, d_el_, lo_, t_, o_, d_rung_win_, rung_max_.data(), K_max_, hd, e0, nullptr, f0(),
-                          f1());
+    elem_op(3, hd, e0, nullptr);
     int32_t mWs;
     const int32_t* dWs = upload_win([](const Rec& e) { return e.W; }, sifting, &mWs);
     const double* dtol = upload_f64([](const Rec& e) { return e.tol; }, sifting);
@@ -1431,7 +1454,7 @@ bool Chunk<T>::mode() {
     EDA_HIP(hipMemsetAsync(gmax, 0, sizeof(uint64_t) * 2 * B, s_));
     {
       int64_t gx = (g.count + kThreads - 1) / kThreads;
-      gx = gx > 1024 ? 1024 : gx;
+      gx = gx > 64 ? 64 : gx;  // 64 workgroups per element: 64 atomics per element and maximum
       hipLaunchKernelGGL(k_sift_step<T>, dim3(static_cast<unsigned>(gx), static_cast<unsigned>(B)), dim3(kThreads), 0,
                          s_, g.count, son, a, Mn, h, r, hn, gmax);
       EDA_LAUNCH("k_sift_step");
@@ -2036,13 +2059,15 @@ int eda_prim_orientation(int32_t dtype, int32_t dim, int64_t batch, const int64_
     if (dim < 2) throw std::invalid_argument("eda: orientation needs 2 or 3 axes");
     const Geom g = PrimGeom<T>(dim, batch, n).g;
     const int K = dim * (dim + 1) / 2;
-    DevMem work(static_cast<size_t>(g.batch * g.count) * sizeof(T) * (3 + K) + 4 * batch + 1024);
-    int32_t* dnf = reinterpret_cast<int32_t*>(static_cast<char*>(work.p) +
-                                              static_cast<size_t>(g.batch * g.count) * sizeof(T) * (3 + K));
+    const size_t fbytes = static_cast<size_t>(g.batch * g.count) * sizeof(T) * (3 + K);
+    const size_t dbytes = static_cast<size_t>(orient_scratch_doubles(batch)) * sizeof(double);
+    DevMem work(fbytes + dbytes + 4 * batch + 1024);
+    double* dscr = reinterpret_cast<double*>(static_cast<char*>(work.p) + (fbytes + 255) / 256 * 256);
+    int32_t* dnf = reinterpret_cast<int32_t*>(dscr + orient_scratch_doubles(batch));
     std::vector<int32_t> wv(w, w + batch);
     const int32_t* dw = ctx.up.put(wv);
     enqueue_orientation<T>(ctx.stream, ctx.device, g, static_cast<const T*>(r), dw, nullptr, static_cast<T*>(nvec),
-                           static_cast<T*>(coh), static_cast<T*>(work.p), dnf);
+                           static_cast<T*>(coh), static_cast<T*>(work.p), dnf, dscr);
   });
 }
 

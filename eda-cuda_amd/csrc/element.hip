@@ -12,6 +12,7 @@
 #include "common.hpp"
 #include "kernels.hpp"
 
+#include <algorithm>
 #include <cmath>
 
 namespace eda {
@@ -32,9 +33,13 @@ __device__ __forceinline__ DE<T> element_de(T L, T c, T q, T span_m1) {
   return r;
 }
 
-dim3 elem_grid(const Geom& g) {
+// Workgroups per element: enough to fill the device for a batch, few enough that the per-element
+// integer atomics of the build and the weights (one per workgroup) do not contend.
+dim3 elem_grid(const Geom& g, int64_t cap = 1024) {
   int64_t bx = (g.count + kThreads - 1) / kThreads;
-  bx = bx < 1 ? 1 : (bx > 1024 ? 1024 : bx);
+  const int64_t want = std::max<int64_t>(16, (int64_t{8192} + g.batch - 1) / g.batch);
+  bx = std::min(bx, std::min(cap, want));
+  bx = bx < 1 ? 1 : bx;
   return dim3(static_cast<unsigned>(bx), static_cast<unsigned>(g.batch));
 }
 
@@ -169,87 +174,96 @@ __device__ __forceinline__ T interp(const Geom& g, const T* __restrict__ f, cons
   return acc;
 }
 
-// One rung's gather. OP 0 max, 2 max + min, 3 box.
+// The gather of every rung in one pass (OP 0 max, 2 max + min, 3 box): each voxel reads its lower
+// rung lo, its weight t and (when one of its two rungs carries segment points) o once, and the cubes
+// of its two rungs from the rung fields r0 (r1) + j F. The arithmetic is the per-rung one in rung
+// order: the lower rung's weighted value stores (wl = 1 - t > 0), the upper rung's adds (t > 0).
 template <class T, int D, int OP>
-__global__ void __launch_bounds__(kThreads) k_gather(Geom g, int j, const int32_t* __restrict__ rung_win,
-                                                     const ElemDev* __restrict__ el, const uint8_t* __restrict__ lo,
-                                                     const T* __restrict__ t, const T* __restrict__ o,
-                                                     const T* __restrict__ f0, const T* __restrict__ f1,
-                                                     T* __restrict__ out0, T* __restrict__ out1) {
+__device__ __forceinline__ void rung_acc(const Geom& g, int64_t i, int64_t at, int64_t F, const T* __restrict__ f0,
+                                         const T* __restrict__ f1, bool pts, const T* __restrict__ o, T* a0, T* a1) {
+  T acc0 = f0[i];
+  T acc1 = OP == 2 ? f1[i] : T(0);
+  if (pts) {
+    int32_t x[kMaxDim];
+#pragma unroll
+    for (int a = 0; a < D; ++a) x[a] = static_cast<int32_t>((i / g.st[a]) % g.n[a]);
+    double ov[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) ov[a] = static_cast<double>(o[a * F + at]);
+#pragma unroll
+    for (int k = 1; k <= 2; ++k) {
+      const double sc = static_cast<double>(k) / 2.0;
+#pragma unroll
+      for (int sg = 0; sg < 2; ++sg) {
+        double y[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+          const double off = sc * ov[a];
+          const double xa = static_cast<double>(x[a]);
+          y[a] = sg == 0 ? xa + off : xa - off;
+        }
+        if constexpr (OP == 2) {
+          acc0 = max_c(acc0, interp<T, D>(g, f0, y));
+          acc1 = min_c(acc1, interp<T, D>(g, f1, y));
+        } else if constexpr (OP == 0) {
+          acc0 = max_c(acc0, interp<T, D>(g, f0, y));
+        } else {
+          acc0 = acc0 + interp<T, D>(g, f0, y);
+        }
+      }
+    }
+    if constexpr (OP == 3) acc0 = acc0 / T(5);
+  }
+  *a0 = acc0;
+  *a1 = acc1;
+}
+
+template <class T, int D, int OP>
+__global__ void __launch_bounds__(kThreads) k_gather_all(Geom g, const ElemDev* __restrict__ el,
+                                                         const uint8_t* __restrict__ lo, const T* __restrict__ t,
+                                                         const T* __restrict__ o, const T* __restrict__ r0,
+                                                         const T* __restrict__ r1, T* __restrict__ out0,
+                                                         T* __restrict__ out1, const int32_t* __restrict__ on) {
   const int64_t b = blockIdx.y;
-  if (rung_win[static_cast<int64_t>(j) * g.batch + b] == 0) return;
-  const bool pts = D >= 2 && ((el[b].pts >> j) & 1u) != 0u;
+  if (on != nullptr && on[b] == 0) return;
+  const uint32_t pts = D >= 2 ? el[b].pts : 0u;
   const int64_t base = b * g.count;
   const int64_t F = g.batch * g.count;
-  const T* e0 = f0 + base;
-  const T* e1 = OP == 2 ? f1 + base : nullptr;
   for (int64_t i = blockIdx.x * static_cast<int64_t>(kThreads) + threadIdx.x; i < g.count;
        i += static_cast<int64_t>(gridDim.x) * kThreads) {
     const int64_t at = base + i;
     const int l = lo[at];
     const T tt = t[at];
     const T wl = T(1) - tt;
-    T wgt;
-    bool add;
-    if (l == j && wl > T(0)) {
-      wgt = wl;
-      add = false;
-    } else if (l + 1 == j && tt > T(0)) {
-      wgt = tt;
-      add = wl > T(0);
-    } else {
-      continue;
+    T v0 = T(0), v1 = T(0);
+    if (wl > T(0)) {
+      T a0, a1;
+      rung_acc<T, D, OP>(g, i, at, F, r0 + l * F + base, OP == 2 ? r1 + l * F + base : nullptr,
+                         ((pts >> l) & 1u) != 0u, o, &a0, &a1);
+      v0 = wl * a0;
+      v1 = wl * a1;
     }
-    T acc0 = e0[i];
-    T acc1 = OP == 2 ? e1[i] : T(0);
-    if (pts) {
-      int32_t x[kMaxDim];
-#pragma unroll
-      for (int a = 0; a < D; ++a) x[a] = static_cast<int32_t>((i / g.st[a]) % g.n[a]);
-      double ov[D];
-#pragma unroll
-      for (int a = 0; a < D; ++a) ov[a] = static_cast<double>(o[a * F + at]);
-#pragma unroll
-      for (int k = 1; k <= 2; ++k) {
-        const double sc = static_cast<double>(k) / 2.0;
-#pragma unroll
-        for (int sg = 0; sg < 2; ++sg) {
-          double y[D];
-#pragma unroll
-          for (int a = 0; a < D; ++a) {
-            const double off = sc * ov[a];
-            const double xa = static_cast<double>(x[a]);
-            y[a] = sg == 0 ? xa + off : xa - off;
-          }
-          if constexpr (OP == 2) {
-            acc0 = max_c(acc0, interp<T, D>(g, e0, y));
-            acc1 = min_c(acc1, interp<T, D>(g, e1, y));
-          } else if constexpr (OP == 0) {
-            acc0 = max_c(acc0, interp<T, D>(g, e0, y));
-          } else {
-            acc0 = acc0 + interp<T, D>(g, e0, y);
-          }
-        }
-      }
-      if constexpr (OP == 3) acc0 = acc0 / T(5);
+    if (tt > T(0)) {
+      T a0, a1;
+      rung_acc<T, D, OP>(g, i, at, F, r0 + (l + 1) * F + base, OP == 2 ? r1 + (l + 1) * F + base : nullptr,
+                         ((pts >> (l + 1)) & 1u) != 0u, o, &a0, &a1);
+      const T u0 = tt * a0, u1 = tt * a1;
+      v0 = wl > T(0) ? v0 + u0 : u0;
+      v1 = wl > T(0) ? v1 + u1 : u1;
     }
-    const T v0 = wgt * acc0;
-    out0[at] = add ? out0[at] + v0 : v0;
-    if constexpr (OP == 2) {
-      const T v1 = wgt * acc1;
-      out1[at] = add ? out1[at] + v1 : v1;
-    }
+    out0[at] = v0;
+    if constexpr (OP == 2) out1[at] = v1;
   }
 }
 
 template <class T, int OP>
-void launch_gather(hipStream_t s, const Geom& g, int j, const int32_t* rung_win, const ElemDev* el, const uint8_t* lo,
-                   const T* t, const T* o, const T* f0, const T* f1, T* out0, T* out1) {
+void launch_gather_all(hipStream_t s, const Geom& g, const ElemDev* el, const uint8_t* lo, const T* t, const T* o,
+                       const T* r0, const T* r1, T* out0, T* out1, const int32_t* on) {
   const dim3 grid = elem_grid(g);
-  if (g.dim == 1) hipLaunchKernelGGL((k_gather<T, 1, OP>), grid, dim3(kThreads), 0, s, g, j, rung_win, el, lo, t, o, f0, f1, out0, out1);
-  else if (g.dim == 2) hipLaunchKernelGGL((k_gather<T, 2, OP>), grid, dim3(kThreads), 0, s, g, j, rung_win, el, lo, t, o, f0, f1, out0, out1);
-  else hipLaunchKernelGGL((k_gather<T, 3, OP>), grid, dim3(kThreads), 0, s, g, j, rung_win, el, lo, t, o, f0, f1, out0, out1);
-  EDA_LAUNCH("k_gather");
+  if (g.dim == 1) hipLaunchKernelGGL((k_gather_all<T, 1, OP>), grid, dim3(kThreads), 0, s, g, el, lo, t, o, r0, r1, out0, out1, on);
+  else if (g.dim == 2) hipLaunchKernelGGL((k_gather_all<T, 2, OP>), grid, dim3(kThreads), 0, s, g, el, lo, t, o, r0, r1, out0, out1, on);
+  else hipLaunchKernelGGL((k_gather_all<T, 3, OP>), grid, dim3(kThreads), 0, s, g, el, lo, t, o, r0, r1, out0, out1, on);
+  EDA_LAUNCH("k_gather_all");
 }
 
 }  // namespace
@@ -257,7 +271,7 @@ void launch_gather(hipStream_t s, const Geom& g, int j, const int32_t* rung_win,
 template <class T>
 void enqueue_element_build(hipStream_t s, const Geom& g, const MethodCtl& c, const T* L, const T* nvec, const T* coh,
                            T* o, const int32_t* on, uint64_t* dminmax) {
-  hipLaunchKernelGGL(k_build<T>, elem_grid(g), dim3(kThreads), 0, s, g, static_cast<T>(c.q),
+  hipLaunchKernelGGL(k_build<T>, elem_grid(g, 64), dim3(kThreads), 0, s, g, static_cast<T>(c.q),
                      static_cast<T>(c.span - 1.0), L, nvec, g.dim >= 2 ? coh : nullptr, o, on, dminmax);
   EDA_LAUNCH("k_build");
 }
@@ -265,7 +279,7 @@ void enqueue_element_build(hipStream_t s, const Geom& g, const MethodCtl& c, con
 template <class T>
 void enqueue_element_weights(hipStream_t s, const Geom& g, const MethodCtl& c, const T* L, const T* coh,
                              const ElemDev* el, uint8_t* lo, T* t, const int32_t* on, uint32_t* bits) {
-  hipLaunchKernelGGL(k_weights<T>, elem_grid(g), dim3(kThreads), 0, s, g, static_cast<T>(c.q),
+  hipLaunchKernelGGL(k_weights<T>, elem_grid(g, 64), dim3(kThreads), 0, s, g, static_cast<T>(c.q),
                      static_cast<T>(c.span - 1.0), L, g.dim >= 2 ? coh : nullptr, el, lo, t, on, bits);
   EDA_LAUNCH("k_weights");
 }
@@ -273,20 +287,20 @@ void enqueue_element_weights(hipStream_t s, const Geom& g, const MethodCtl& c, c
 template <class T>
 void enqueue_element_op(hipStream_t s, const Geom& g, int op, const ElemDev* el, const uint8_t* lo, const T* t,
                         const T* o, const int32_t* rung_win, const int32_t* rung_max, int K_max, const T* in, T* out0,
-                        T* out1, T* f0, T* f1) {
-  for (int j = 0; j < K_max; ++j) {
-    if (rung_max[j] < 1) continue;
-    const int32_t* win = rung_win + static_cast<int64_t>(j) * g.batch;
-    if (op == 3) {
-      enqueue_cube<T>(s, g, CubeOp::kBox, in, f0, nullptr, win, rung_max[j]);
-      launch_gather<T, 3>(s, g, j, rung_win, el, lo, t, o, f0, nullptr, out0, nullptr);
-    } else if (op == 2) {
-      enqueue_cube<T>(s, g, CubeOp::kMaxMin, in, f0, f1, win, rung_max[j]);
-      launch_gather<T, 2>(s, g, j, rung_win, el, lo, t, o, f0, f1, out0, out1);
-    } else {
-      enqueue_cube<T>(s, g, CubeOp::kMax, in, f0, nullptr, win, rung_max[j]);
-      launch_gather<T, 0>(s, g, j, rung_win, el, lo, t, o, f0, nullptr, out0, nullptr);
-    }
+                        T* out1, T* rungs0, T* rungs1, const int32_t* on) {
+  if (K_max < 1) return;
+  int32_t mw = 0;
+  for (int j = 0; j < K_max; ++j) mw = std::max(mw, rung_max[j]);
+  const int64_t F = g.batch * g.count;
+  if (op == 3) {
+    enqueue_cube_rungs<T>(s, g, CubeOp::kBox, in, rungs0, nullptr, rung_win, K_max, F, mw);
+    launch_gather_all<T, 3>(s, g, el, lo, t, o, rungs0, nullptr, out0, nullptr, on);
+  } else if (op == 2) {
+    enqueue_cube_rungs<T>(s, g, CubeOp::kMaxMin, in, rungs0, rungs1, rung_win, K_max, F, mw);
+    launch_gather_all<T, 2>(s, g, el, lo, t, o, rungs0, rungs1, out0, out1, on);
+  } else {
+    enqueue_cube_rungs<T>(s, g, CubeOp::kMax, in, rungs0, nullptr, rung_win, K_max, F, mw);
+    launch_gather_all<T, 0>(s, g, el, lo, t, o, rungs0, nullptr, out0, nullptr, on);
   }
 }
 
@@ -300,9 +314,9 @@ template void enqueue_element_weights<double>(hipStream_t, const Geom&, const Me
                                               const ElemDev*, uint8_t*, double*, const int32_t*, uint32_t*);
 template void enqueue_element_op<float>(hipStream_t, const Geom&, int, const ElemDev*, const uint8_t*, const float*,
                                         const float*, const int32_t*, const int32_t*, int, const float*, float*, float*,
-                                        float*, float*);
+                                        float*, float*, const int32_t*);
 template void enqueue_element_op<double>(hipStream_t, const Geom&, int, const ElemDev*, const uint8_t*, const double*,
                                          const double*, const int32_t*, const int32_t*, int, const double*, double*,
-                                         double*, double*, double*);
+                                         double*, double*, double*, const int32_t*);
 
 }  // namespace eda

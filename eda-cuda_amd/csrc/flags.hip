@@ -16,6 +16,8 @@
 #include "common.hpp"
 #include "kernels.hpp"
 
+#include <algorithm>
+
 namespace eda {
 namespace {
 
@@ -116,18 +118,17 @@ __global__ void __launch_bounds__(kThreads) k_flags(FlagLines L, int64_t batch, 
                                                     const T* __restrict__ tau, uint8_t* __restrict__ flags,
                                                     uint64_t* __restrict__ ext, const int32_t* __restrict__ on,
                                                     const T* __restrict__ sum) {
-  const int64_t total = batch * L.count;
-  const int64_t id = blockIdx.x * static_cast<int64_t>(kThreads) + threadIdx.x;
-  const int64_t idc = id < total ? id : total - 1;
-  const int64_t b = idc / L.count;
-  const int64_t e = idc - b * L.count;
-  const int64_t j = e % L.inner;
-  const int64_t t = e / L.inner;
-  const int64_t o = t / L.n;
-  const int32_t s = static_cast<int32_t>(t - o * L.n);
+  const int64_t b = blockIdx.y;
+  if (on != nullptr && on[b] == 0) return;
   const int32_t n = L.n;
-  bool flagged = false;
-  if (id < total && s >= 1 && s <= n - 2 && (on == nullptr || on[b] != 0)) {
+  uint64_t count = 0;
+  for (int64_t e = blockIdx.x * static_cast<int64_t>(kThreads) + threadIdx.x; e < L.count;
+       e += static_cast<int64_t>(gridDim.x) * kThreads) {
+    const int64_t j = e % L.inner;
+    const int64_t t = e / L.inner;
+    const int64_t o = t / L.n;
+    const int32_t s = static_cast<int32_t>(t - o * L.n);
+    if (s < 1 || s > n - 2) continue;
     const int64_t base = b * L.count + o * static_cast<int64_t>(n) * L.inner + j;
     const T* x = r + base;
     const int64_t st = L.step;
@@ -151,21 +152,21 @@ __global__ void __launch_bounds__(kThreads) k_flags(FlagLines L, int64_t batch, 
                                         side_deep<false, 1>(x, st, n, L.nb, p, cur, tp, bmin, bmax, sst);
           if (sig) {
             flags[base + p * st] |= static_cast<uint8_t>(1u << L.axis);
-            flagged = true;
+            ++count;
           }
         }
       }
     }
   }
-  // the count: one atomic per wave when its samples are of one element, else per lane
-  const int64_t b0 = __shfl(b, 0, 64);
-  const bool uniform = __all(b == b0);
-  if (uniform) {
-    const uint64_t c = wave_sum_u64(flagged ? 1u : 0u);
-    if ((threadIdx.x & 63) == 0 && c) atomicAdd(reinterpret_cast<unsigned long long*>(&ext[3 * b0 + L.axis]),
-                                                static_cast<unsigned long long>(c));
-  } else if (flagged) {
-    atomicAdd(reinterpret_cast<unsigned long long*>(&ext[3 * b + L.axis]), 1ULL);
+  // the count: a wave64 sum, a workgroup sum, one integer atomic per workgroup
+  __shared__ uint64_t red[kThreads / 64];
+  count = wave_sum_u64(count);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = count;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t c = 0;
+    for (int w = 0; w < kThreads / 64; ++w) c += red[w];
+    if (c) atomicAdd(reinterpret_cast<unsigned long long*>(&ext[3 * b + L.axis]), static_cast<unsigned long long>(c));
   }
 }
 
@@ -193,9 +194,11 @@ void enqueue_sig_flags(hipStream_t s, const Geom& g, const T* r, const T* tau, u
                          0, s, L, g.batch, r, scratch, on);
       EDA_LAUNCH("k_flag_blocks");
     }
-    const int64_t ids = g.batch * g.count;
-    hipLaunchKernelGGL(k_flags<T>, dim3(static_cast<unsigned>((ids + kThreads - 1) / kThreads)), dim3(kThreads), 0, s,
-                       L, g.batch, r, tau, flags, ext, on, scratch);
+    // (workgroups per element, elements): at most 128 workgroups, so 128 atomics, per element
+    int64_t gx = (g.count + kThreads - 1) / kThreads;
+    gx = std::min<int64_t>(gx, std::max<int64_t>(16, std::min<int64_t>(128, 16384 / std::max<int64_t>(g.batch, 1))));
+    hipLaunchKernelGGL(k_flags<T>, dim3(static_cast<unsigned>(gx), static_cast<unsigned>(g.batch)), dim3(kThreads), 0,
+                       s, L, g.batch, r, tau, flags, ext, on, scratch);
     EDA_LAUNCH("k_flags");
   }
 }

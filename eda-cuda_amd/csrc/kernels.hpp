@@ -46,6 +46,11 @@ enum class CubeOp : int8_t { kMax = 0, kMin = 1, kMaxMin = 2, kBox = 3 };
 template <class T>
 void enqueue_cube(hipStream_t s, const Geom& g, CubeOp op, const T* in, T* out0, T* out1, const int32_t* win,
                   int32_t max_w);
+// The cubes of `rungs` window arrays (win + j B) of one input in one launch per pass, rung j into
+// out0 + j stride (out1 + j stride); each rung's outputs are bitwise those of enqueue_cube.
+template <class T>
+void enqueue_cube_rungs(hipStream_t s, const Geom& g, CubeOp op, const T* in, T* out0, T* out1, const int32_t* win,
+                        int32_t rungs, int64_t stride, int32_t max_w);
 int32_t vhgw_task_slots(int32_t n, int32_t max_w);
 
 // ---- reductions (reduce.hip) ---------------------------------------------------------------------
@@ -110,11 +115,12 @@ struct AxisMats {
 template <class T>
 const AxisMats<T>& axis_mats(int device, int32_t n);
 // n (D fields, component-major [a][B][count]) and coh of every element with on[b]; w [B]: the mode
-// windows (the band's centre period). work: (2 + D + D (D + 1) / 2) fields of T; nonfinite[b] = 1
-// where the tensor is not finite.
+// windows (the band's centre period). work: (3 + D (D + 1) / 2) fields of T; nonfinite[b] = 1
+// where the tensor is not finite; dscratch: orient_scratch_doubles(B) doubles.
 template <class T>
 void enqueue_orientation(hipStream_t s, int device, const Geom& g, const T* r, const int32_t* w, const int32_t* on,
-                         T* nvec, T* coh, T* work, int32_t* nonfinite);
+                         T* nvec, T* coh, T* work, int32_t* nonfinite, double* dscratch);
+inline int64_t orient_scratch_doubles(int64_t batch) { return batch * 65; }
 // The orthonormal DCT-II of every element (inverse = 0) or its inverse with the sin basis on the
 // axes of the bit mask odd (inverse = 1), through the same GEMMs; tmp: one field.
 template <class T>
@@ -141,10 +147,12 @@ void enqueue_element_weights(hipStream_t s, const Geom& g, const MethodCtl& c, c
                              const ElemDev* el, uint8_t* lo, T* t, const int32_t* on, uint32_t* bits);
 // The operators over the rungs: op 0 Dil (max), 2 Dil + Ero (max into out0, min into out1), 3 Avg.
 // rung_win [K_max * B]: rung j of element b at rung_win[j B + b] (0: unused); rung_max [K_max]:
-// the largest side of rung j over the batch (host). f0, f1: cube scratch fields.
+// the largest side of rung j over the batch (host). The cubes of every rung run in one launch per
+// pass into rungs0 (rungs1: the min cubes of op 2), K_max fields each; then one gather over all
+// rungs. on [B]: the elements that run (the others' outputs are left untouched).
 template <class T>
 void enqueue_element_op(hipStream_t s, const Geom& g, int op, const ElemDev* el, const uint8_t* lo, const T* t,
                         const T* o, const int32_t* rung_win, const int32_t* rung_max, int K_max, const T* in, T* out0,
-                        T* out1, T* f0, T* f1);
+                        T* out1, T* rungs0, T* rungs1, const int32_t* on);
 
 }  // namespace eda

@@ -313,20 +313,38 @@ void eigen_pass(hipStream_t s, const Geom& g, const T* t, const int32_t* on, T* 
 // The mean of every running element in double: 256 strided partial sums and a fixed tree (the
 // value depends on the element's samples only).
 template <class T>
-__global__ void __launch_bounds__(kThreads) k_mean(const T* __restrict__ r, int64_t count,
-                                                   const int32_t* __restrict__ on, double* __restrict__ mean) {
+__global__ void __launch_bounds__(kThreads) k_mean_part(const T* __restrict__ r, int64_t count,
+                                                        const int32_t* __restrict__ on, double* __restrict__ part) {
   __shared__ double sh[kThreads];
-  const int64_t b = blockIdx.x;
+  const int64_t b = blockIdx.y;
   if (on != nullptr && on[b] == 0) return;
   double acc = 0.0;
-  for (int64_t i = threadIdx.x; i < count; i += kThreads) acc = acc + static_cast<double>(r[b * count + i]);
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(kThreads) + threadIdx.x; i < count;
+       i += static_cast<int64_t>(gridDim.x) * kThreads)
+    acc = acc + static_cast<double>(r[b * count + i]);
   sh[threadIdx.x] = acc;
   __syncthreads();
   for (int st = kThreads / 2; st > 0; st >>= 1) {
     if (threadIdx.x < st) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + st];
     __syncthreads();
   }
-  if (threadIdx.x == 0) mean[b] = sh[0] / static_cast<double>(count);
+  if (threadIdx.x == 0) part[b * gridDim.x + blockIdx.x] = sh[0];
+}
+constexpr int kMeanParts = 64;
+// The mean of every running element in double: kMeanParts workgroups of 256 strided partial sums
+// with a fixed tree each, then their sum in order (the value depends on the element's samples only).
+template <class T>
+void enqueue_mean(hipStream_t s, const Geom& g, const T* r, const int32_t* on, double* mean, double* part) {
+  hipLaunchKernelGGL(k_mean_part<T>, dim3(kMeanParts, static_cast<unsigned>(g.batch)), dim3(kThreads), 0, s, r, g.count,
+                     on, part);
+  EDA_LAUNCH("k_mean_part");
+  const int64_t count = g.count;
+  flat(s, g.batch, [=] __device__(int64_t b) {
+    if (on != nullptr && on[b] == 0) return;
+    double v = 0.0;
+    for (int k = 0; k < kMeanParts; ++k) v = v + part[b * kMeanParts + k];
+    mean[b] = v / static_cast<double>(count);
+  });
 }
 
 // The DCT index k of sample i, the frequencies u_a = k_a (1 / (2 n_a)) and rad = |u| (summed from 0
@@ -385,7 +403,7 @@ void enqueue_eigen(hipStream_t s, const Geom& g, const T* t, const int32_t* on, 
 
 template <class T>
 void enqueue_orientation(hipStream_t s, int device, const Geom& g, const T* r, const int32_t* w, const int32_t* on,
-                         T* nvec, T* coh, T* work, int32_t* nonfinite) {
+                         T* nvec, T* coh, T* work, int32_t* nonfinite, double* dscratch) {
   const int D = g.dim;
   if (D < 2) return;
   const int64_t F = g.batch * g.count;
@@ -395,9 +413,8 @@ void enqueue_orientation(hipStream_t s, int device, const Geom& g, const T* r, c
   T* T1 = work + 2 * F;
   T* R2 = work + 3 * F;
   T* R1 = nvec;  // n is written last, over R1
-  double* mean = reinterpret_cast<double*>(X);  // X is written only after the mean is used
-  hipLaunchKernelGGL(k_mean<T>, dim3(static_cast<unsigned>(g.batch)), dim3(kThreads), 0, s, r, g.count, on, mean);
-  EDA_LAUNCH("k_mean");
+  double* mean = dscratch;
+  enqueue_mean<T>(s, g, r, on, mean, mean + g.batch);
   each(s, g, on, [=] __device__(int64_t b, int64_t i) {
     const int64_t at = b * g.count + i;
     Y[at] = static_cast<T>(static_cast<double>(r[at]) - mean[b]);
@@ -476,8 +493,8 @@ template void enqueue_dct<double>(hipStream_t, int, const Geom&, const double*, 
 template void enqueue_eigen<float>(hipStream_t, const Geom&, const float*, const int32_t*, float*, float*, int32_t*);
 template void enqueue_eigen<double>(hipStream_t, const Geom&, const double*, const int32_t*, double*, double*, int32_t*);
 template void enqueue_orientation<float>(hipStream_t, int, const Geom&, const float*, const int32_t*, const int32_t*,
-                                         float*, float*, float*, int32_t*);
+                                         float*, float*, float*, int32_t*, double*);
 template void enqueue_orientation<double>(hipStream_t, int, const Geom&, const double*, const int32_t*, const int32_t*,
-                                          double*, double*, double*, int32_t*);
+                                          double*, double*, double*, int32_t*, double*);
 
 }  // namespace eda

@@ -78,11 +78,14 @@ struct VPass {
   int32_t scale;        // multiply by fl_T(1 / prod of the clamped sides) (the last box pass)
 };
 
+// gridDim.y passes (rungs) in one launch: rung j reads win + j B, in + j in_stride and writes
+// out + j out_stride.
 template <class T>
 struct VPtrs {
   const T* in[2];
   T* out[2];
   const int32_t* win;
+  int64_t in_stride, out_stride;
 };
 
 // One direction of one stage on every line of the workgroup. DIR 1: H (the reversed line), whose
@@ -198,7 +201,8 @@ __global__ void __launch_bounds__(kThreads) k_vhgw(VPass a, VPtrs<T> ptr) {
   const int64_t wg = blockIdx.x;
   const int64_t b = wg / a.groups;
   const int64_t grp = wg - b * a.groups;
-  const int32_t w = ptr.win[b];
+  const int64_t rung = blockIdx.y;
+  const int32_t w = ptr.win[rung * a.g.batch + b];
   if (w < 1) return;  // uniform: the element does not run
   const int32_t n = a.n;
   const int32_t ls = line_stride(n);
@@ -234,7 +238,7 @@ __global__ void __launch_bounds__(kThreads) k_vhgw(VPass a, VPtrs<T> ptr) {
   for (int f = 0; f < a.nf; ++f) {
     const int op = a.ops[f];
     if (f == 0 || !a.same_input) {
-      const T* src = ptr.in[f];
+      const T* src = ptr.in[f] + rung * ptr.in_stride;
       for (int32_t idx = threadIdx.x; idx < total_pos; idx += kThreads) {
         int32_t p, i;
         int64_t at;
@@ -261,7 +265,7 @@ __global__ void __launch_bounds__(kThreads) k_vhgw(VPass a, VPtrs<T> ptr) {
       if (op == kVMax) run_stage<S, kVMax>(V, R, tot, lines, ls, a.tmax, n, wi);
       else run_stage<S, kVMin>(V, R, tot, lines, ls, a.tmax, n, wi);
     }
-    T* dst = ptr.out[f];
+    T* dst = ptr.out[f] + rung * ptr.out_stride;
     for (int32_t idx = threadIdx.x; idx < total_pos; idx += kThreads) {
       int32_t p, i;
       int64_t at;
@@ -304,8 +308,10 @@ int32_t vhgw_task_slots(int32_t n, int32_t max_w) {
 
 template <class T>
 void enqueue_vhgw(hipStream_t s, const Geom& g, int axis, int nf, bool same_input, const int* ops, bool scale,
-                  const T* const* in, T* const* out, const int32_t* win, int32_t max_w) {
-  if (max_w < 1) return;
+                  const T* const* in, T* const* out, const int32_t* win, int32_t max_w, int32_t rungs,
+                  int64_t in_stride, int64_t out_stride) {
+  if (max_w < 1 || rungs < 1) return;
+  if (rungs > 65535) throw std::invalid_argument("eda: too many rungs for one launch");
   VPass a{};
   a.g = g;
   a.axis = axis;
@@ -336,35 +342,40 @@ void enqueue_vhgw(hipStream_t s, const Geom& g, int axis, int nf, bool same_inpu
   p.out[0] = out[0];
   p.out[1] = nf > 1 ? out[1] : out[0];
   p.win = win;
+  p.in_stride = in_stride;
+  p.out_stride = out_stride;
   const int64_t wgs = g.batch * a.groups;
   const bool sum = ops[0] == kVSum;
   const size_t lds = lds_bytes<T>(n, P, a.tmax);
   hipEvent_t t0 = nullptr;
   const bool timed = timer_on();
   if (timed) timer_note(s, &t0);
+  const dim3 grid(static_cast<unsigned>(wgs), static_cast<unsigned>(rungs));
   if (sum) {
     auto k = k_vhgw<T, T>;
     if (lds > 65536) EDA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-    hipLaunchKernelGGL(k, dim3(static_cast<unsigned>(wgs)), dim3(kThreads), lds, s, a, p);
+    hipLaunchKernelGGL(k, grid, dim3(kThreads), lds, s, a, p);
   } else {
     auto k = k_vhgw<T, okey_t<T>>;
     if (lds > 65536) EDA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-    hipLaunchKernelGGL(k, dim3(static_cast<unsigned>(wgs)), dim3(kThreads), lds, s, a, p);
+    hipLaunchKernelGGL(k, grid, dim3(kThreads), lds, s, a, p);
   }
   EDA_LAUNCH("k_vhgw");
   if (timed) {
-    int64_t active = mirror_active(win, g.batch);
-    if (active < 0) active = g.batch;
+    int64_t active = mirror_active(win, g.batch * rungs);
+    if (active < 0) active = g.batch * rungs;
     const int fields = (nf == 2 && !same_input ? 2 : 1) + nf;
     timer_done(s, t0, static_cast<double>(fields) * active * g.count * sizeof(T));
   }
 }
 
 // The cube (reference cpp/src/method/common.cpp:65-136): box runs axis 0 first, the last pass
-// scaled; max / min run the last axis first; maxmin stages the input once for both outputs.
+// scaled; max / min run the last axis first; maxmin stages the input once for both outputs. With
+// rungs > 1 every pass runs all rungs in one launch: rung j filters in with the windows win + j B
+// into out0 + j stride (out1 + j stride).
 template <class T>
-void enqueue_cube(hipStream_t s, const Geom& g, CubeOp op, const T* in, T* out0, T* out1, const int32_t* win,
-                  int32_t max_w) {
+void enqueue_cube_rungs(hipStream_t s, const Geom& g, CubeOp op, const T* in, T* out0, T* out1, const int32_t* win,
+                        int32_t rungs, int64_t stride, int32_t max_w) {
   if (max_w < 1) return;
   const int D = g.dim;
   if (op == CubeOp::kBox) {
@@ -372,28 +383,39 @@ void enqueue_cube(hipStream_t s, const Geom& g, CubeOp op, const T* in, T* out0,
       const int ops[1] = {kVSum};
       const T* src[1] = {a == 0 ? in : out0};
       T* dst[1] = {out0};
-      enqueue_vhgw<T>(s, g, a, 1, true, ops, a == D - 1, src, dst, win, max_w);
+      enqueue_vhgw<T>(s, g, a, 1, true, ops, a == D - 1, src, dst, win, max_w, rungs, a == 0 ? 0 : stride, stride);
     }
     return;
   }
   for (int a = D - 1; a >= 0; --a) {
     const bool first = a == D - 1;
+    const int64_t ins = first ? 0 : stride;
     if (op == CubeOp::kMaxMin) {
       const int ops[2] = {kVMax, kVMin};
       const T* src[2] = {first ? in : out0, first ? in : out1};
       T* dst[2] = {out0, out1};
-      enqueue_vhgw<T>(s, g, a, 2, first, ops, false, src, dst, win, max_w);
+      enqueue_vhgw<T>(s, g, a, 2, first, ops, false, src, dst, win, max_w, rungs, ins, stride);
     } else {
       const int ops[1] = {op == CubeOp::kMax ? kVMax : kVMin};
       const T* src[1] = {first ? in : out0};
       T* dst[1] = {out0};
-      enqueue_vhgw<T>(s, g, a, 1, true, ops, false, src, dst, win, max_w);
+      enqueue_vhgw<T>(s, g, a, 1, true, ops, false, src, dst, win, max_w, rungs, ins, stride);
     }
   }
+}
+
+template <class T>
+void enqueue_cube(hipStream_t s, const Geom& g, CubeOp op, const T* in, T* out0, T* out1, const int32_t* win,
+                  int32_t max_w) {
+  enqueue_cube_rungs<T>(s, g, op, in, out0, out1, win, 1, 0, max_w);
 }
 
 template void enqueue_cube<float>(hipStream_t, const Geom&, CubeOp, const float*, float*, float*, const int32_t*, int32_t);
 template void enqueue_cube<double>(hipStream_t, const Geom&, CubeOp, const double*, double*, double*, const int32_t*,
                                    int32_t);
+template void enqueue_cube_rungs<float>(hipStream_t, const Geom&, CubeOp, const float*, float*, float*, const int32_t*,
+                                        int32_t, int64_t, int32_t);
+template void enqueue_cube_rungs<double>(hipStream_t, const Geom&, CubeOp, const double*, double*, double*,
+                                         const int32_t*, int32_t, int64_t, int32_t);
 
 }  // namespace eda
