@@ -1,0 +1,205 @@
+# SPDX-License-Identifier: Apache-2.0
+"""The reference's native-module interface, served by libeda_hip.so.
+
+The reference's Python package calls its nanobind module ``eda._core`` (signatures in
+src/eda/_core.pyi:9-27; implementation cpp/python/decompose.cpp:325-509, module.cpp:64-142). This
+module exposes the functions of that interface that the decomposition path uses, with the same
+names, argument order, argument meaning and results, over the C ABI of include/eda_hip.h, so that
+the reference's ``_native.core()`` can return it in place of ``eda._core`` on a ROCm machine:
+
+  decompose_host(x, max_imf, tie_tolerance, sifts, check_finite, orientation, max_chunk)
+      -> (rows (B, P, *spatial) NumPy array, info dict)
+  decompose_device(ptr, device, dtype, target, batch_shape, batch_strides, shape, strides, fences,
+                   max_imf, tie_tolerance, sifts, check_finite, orientation, max_chunk)
+      -> (owner, out_ptr, info dict); rows C-contiguous (B, P, *spatial) on ``device``
+  build_info(), device_count(), device_info(device=0), pointer_device(ptr)
+
+info holds ``n_imfs``, ``error`` [B] int32; ``windows``, ``sifts`` [B, K] int32; ``spacing``,
+``energy`` [B, K] float64; ``chunk`` int. Errors raise the reference's classes: ValueError,
+MemoryError, ``eda.CudaError``, RuntimeError. Not served: ``orientation=True`` (ValueError),
+device input whose dtype differs from ``target`` or whose batch axes do not merge into one stride
+(ValueError), and the monogenic and DLPack helpers. The device path allocates its output through
+PyTorch-ROCm, whose tensor is the ``owner``."""
+
+from __future__ import annotations
+
+import ctypes
+import json
+import math
+from collections.abc import Sequence
+
+import numpy as np
+
+from . import _native
+
+_CODES = {"float32": 0, "float64": 1}
+
+
+def build_info() -> dict:
+    buf = ctypes.create_string_buffer(4096)
+    _native.core().eda_build_info(buf, len(buf))
+    return json.loads(buf.value.decode())
+
+
+def device_count() -> int:
+    return int(_native.core().eda_device_count())
+
+
+def device_info(device: int = 0) -> dict:
+    buf = ctypes.create_string_buffer(4096)
+    err = _native.errbuf()
+    rc = _native.core().eda_device_info(int(device), buf, len(buf))
+    if rc != _native.EDA_OK:
+        err.value = buf.value
+        _native.check(rc, err)
+    return json.loads(buf.value.decode())
+
+
+def pointer_device(ptr: int) -> int:
+    dev = ctypes.c_int32(-1)
+    rc = _native.core().eda_pointer_device(ctypes.c_void_p(int(ptr)), ctypes.byref(dev))
+    if rc != _native.EDA_OK:
+        raise ValueError(f"eda: {ptr:#x} is not a pointer the HIP runtime knows")
+    return int(dev.value)
+
+
+def _params(max_imf, tie_tolerance, sifts, check_finite, orientation, max_chunk, stream=0) -> _native.Params:
+    if orientation:
+        raise ValueError("eda: orientation output is not supported by this build")
+    if int(sifts) < 1:
+        raise ValueError(f"eda: sifts must be at least 1, got {sifts}")
+    if int(max_chunk) < 0:
+        raise ValueError(f"eda: max_chunk must be >= 0, got {max_chunk}")
+    p = _native.Params()
+    p.max_imf = int(max_imf)
+    p.sifts = int(sifts)
+    p.check_finite = 1 if check_finite else 0
+    p.orientation = 0
+    p.tie_tolerance = -1.0 if tie_tolerance is None else float(tie_tolerance)
+    p.max_chunk = int(max_chunk)
+    p.stream = int(stream)
+    return p
+
+
+def _info(res: _native.Result) -> dict:
+    B, K = int(res.batch), int(res.K)
+
+    def arr(ptr, n, dt, shape):
+        if n == 0:
+            return np.zeros(shape, dt)
+        return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dt, copy=True).reshape(shape)
+
+    return {
+        "n_imfs": arr(res.n_imfs, B, np.int32, (B,)),
+        "error": arr(res.error, B, np.int32, (B,)),
+        "windows": arr(res.windows, B * K, np.int32, (B, K)),
+        "sifts": arr(res.sifts, B * K, np.int32, (B, K)),
+        "spacing": arr(res.spacing, B * K, np.float64, (B, K)),
+        "energy": arr(res.energy, B * K, np.float64, (B, K)),
+        "chunk": int(res.chunk),
+    }
+
+
+def _run(desc: _native.ArrayDesc, p: _native.Params) -> _native.Result:
+    res = _native.Result()
+    err = _native.errbuf()
+    _native.check(_native.core().eda_decompose(ctypes.byref(desc), ctypes.byref(p), ctypes.byref(res), err,
+                                               len(err)), err)
+    return res
+
+
+def decompose_host(x, max_imf: int, tie_tolerance: float | None, sifts: int, check_finite: bool,
+                   orientation: bool, max_chunk: int) -> tuple:
+    """The C-contiguous host array x (B, *spatial), float32 or float64: (rows (B, P, *spatial),
+    info) (reference cpp/python/decompose.cpp:325-384)."""
+    x = np.asarray(x)
+    if x.dtype.name not in _CODES:
+        raise TypeError(f"eda: decompose_host needs float32 or float64, got {x.dtype}")
+    if not x.flags.c_contiguous:
+        raise ValueError("eda: decompose_host needs a C-contiguous array")
+    if not 2 <= x.ndim <= 4:
+        raise ValueError(f"eda: expected (B, *spatial) with 1 to 3 spatial axes, got {x.ndim} axes")
+    p = _params(max_imf, tie_tolerance, sifts, check_finite, orientation, max_chunk)
+    desc = _native.ArrayDesc()
+    desc.dtype = _CODES[x.dtype.name]
+    desc.dim = x.ndim - 1
+    desc.batch = x.shape[0]
+    for a, n in enumerate(x.shape[1:]):
+        desc.n[a] = n
+    desc.device = -1
+    desc.data = x.ctypes.data if x.size else None
+    res = _run(desc, p)
+    try:
+        info = _info(res)
+        rows = np.empty((x.shape[0], int(res.rows)) + x.shape[1:], dtype=x.dtype)
+        if rows.size:
+            err = _native.errbuf()
+            _native.check(_native.core().eda_collect(ctypes.byref(res), rows.ctypes.data_as(ctypes.c_void_p), -1,
+                                                     err, len(err)), err)
+    finally:
+        _native.core().eda_release(ctypes.byref(res))
+    return rows, info
+
+
+def _merged_batch_stride(batch_shape: Sequence[int], batch_strides: Sequence[int]) -> int:
+    """The one stride of the batch axes, when they merge (reference cpp/src/decompose.cpp:44-63)."""
+    axes = [(int(n), int(s)) for n, s in zip(batch_shape, batch_strides) if int(n) != 1]
+    if not axes:
+        return 0
+    stride = axes[-1][1]
+    span = axes[-1][0]
+    for n, s in reversed(axes[:-1]):
+        if s != stride * span:
+            raise ValueError("eda: the batch axes of device input must merge into one stride in this build")
+        span *= n
+    return stride
+
+
+def decompose_device(ptr: int, device: int, dtype: str, target: str, batch_shape: Sequence[int],
+                     batch_strides: Sequence[int], shape: Sequence[int], strides: Sequence[int],
+                     fences: Sequence[int], max_imf: int, tie_tolerance: float | None, sifts: int,
+                     check_finite: bool, orientation: bool, max_chunk: int) -> tuple:
+    """A ROCm device array (strides in elements, any sign) after waiting for the fence streams:
+    (owner, out_ptr, info), the rows C-contiguous (B, P, *spatial) in ``target`` on ``device``,
+    complete on return (reference cpp/python/decompose.cpp:452-509)."""
+    import torch  # noqa: PLC0415 - the device path allocates its output through PyTorch-ROCm
+
+    if target not in _CODES:
+        raise ValueError(f"eda: target must be float32 or float64, got {target}")
+    if dtype != target:
+        raise ValueError(f"eda: device input of dtype {dtype} converted to {target} is not supported by this build")
+    if not 1 <= len(shape) <= 3 or len(strides) != len(shape):
+        raise ValueError("eda: expected 1 to 3 spatial axes with one stride each")
+    B = math.prod(int(n) for n in batch_shape)
+    fences = [int(f) for f in fences]
+    stream = 0
+    if fences:
+        stream = fences[0]
+        if len(fences) > 1 or stream in (1, 2):
+            # legacy / per-thread default streams and extra fences: wait for the whole device
+            torch.cuda.synchronize(int(device))
+            stream = 0
+    p = _params(max_imf, tie_tolerance, sifts, check_finite, orientation, max_chunk, stream)
+    desc = _native.ArrayDesc()
+    desc.dtype = _CODES[target]
+    desc.dim = len(shape)
+    desc.batch = B
+    for a, (n, s) in enumerate(zip(shape, strides)):
+        desc.n[a] = int(n)
+        desc.strides[a] = int(s)
+    desc.batch_stride = _merged_batch_stride(batch_shape, batch_strides)
+    desc.device = int(device)
+    desc.data = int(ptr) if B and math.prod(int(n) for n in shape) else None
+    res = _run(desc, p)
+    try:
+        info = _info(res)
+        tdt = torch.float32 if target == "float32" else torch.float64
+        owner = torch.empty((B, int(res.rows)) + tuple(int(n) for n in shape), dtype=tdt,
+                            device=torch.device("cuda", int(device)))
+        if owner.numel():
+            err = _native.errbuf()
+            _native.check(_native.core().eda_collect(ctypes.byref(res), ctypes.c_void_p(owner.data_ptr()),
+                                                     int(device), err, len(err)), err)
+    finally:
+        _native.core().eda_release(ctypes.byref(res))
+    return owner, owner.data_ptr(), info

@@ -81,3 +81,61 @@ def test_restore_order_inserts_the_mode_axis_before_the_first_spatial_axis():
     perm = _restore_order(4, (0, 2), (1, 3))
     comp = np.zeros((2, 3, 7, 5, 6))
     assert comp.transpose(perm).shape == (2, 7, 5, 3, 6)
+
+
+# ----------------------------------------------------------------------------------------------
+# The reference's native-module interface (src/eda/_core.pyi:9-27) served over the C ABI
+
+_CORE_SIGNATURES = {
+    "build_info": [],
+    "device_count": [],
+    "device_info": ["device"],
+    "pointer_device": ["ptr"],
+    "decompose_host": ["x", "max_imf", "tie_tolerance", "sifts", "check_finite", "orientation", "max_chunk"],
+    "decompose_device": ["ptr", "device", "dtype", "target", "batch_shape", "batch_strides", "shape", "strides",
+                         "fences", "max_imf", "tie_tolerance", "sifts", "check_finite", "orientation", "max_chunk"],
+}
+
+
+@pytest.mark.parametrize("name", sorted(_CORE_SIGNATURES))
+def test_core_hip_has_the_reference_interface(name):
+    import inspect
+
+    from eda import _core_hip
+
+    fn = getattr(_core_hip, name)
+    assert list(inspect.signature(fn).parameters) == _CORE_SIGNATURES[name]
+
+
+def test_core_hip_argument_rules_before_any_gpu_work():
+    from eda import _core_hip
+
+    x = np.zeros((2, 16), np.float32)
+    with pytest.raises(ValueError):
+        _core_hip.decompose_host(x, -1, None, 3, True, True, 0)  # orientation
+    with pytest.raises(ValueError):
+        _core_hip.decompose_host(x, -1, None, 0, True, False, 0)  # sifts
+    with pytest.raises(TypeError):
+        _core_hip.decompose_host(x.astype(np.int32), -1, None, 3, True, False, 0)
+    with pytest.raises(ValueError):
+        _core_hip.decompose_host(np.zeros((2, 4, 4), np.float32)[:, ::2], -1, None, 3, True, False, 0)
+
+
+@pytest.mark.parametrize("shape,strides,want", [
+    ((4,), (100,), 100),
+    ((2, 3), (300, 100), 100),
+    ((2, 1, 3), (300, 7, 100), 100),
+    ((), (), 0),
+    ((3, 2), (-200, -100), -100),
+])
+def test_core_hip_merged_batch_stride(shape, strides, want):
+    from eda._core_hip import _merged_batch_stride
+
+    assert _merged_batch_stride(shape, strides) == want
+
+
+def test_core_hip_batch_axes_that_do_not_merge():
+    from eda._core_hip import _merged_batch_stride
+
+    with pytest.raises(ValueError):
+        _merged_batch_stride((2, 3), (100, 300))

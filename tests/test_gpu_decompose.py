@@ -136,6 +136,34 @@ def test_layout_and_batch_invariance():
         assert np.array_equal(yb[: int(ib.n_imfs) + 1], y[: int(ib.n_imfs) + 1, :, :, b])
 
 
+def test_reference_core_interface_matches_decompose():
+    """eda._core_hip serves the reference's native-module calls (src/eda/_api.py:310-320 calls them
+    with max_chunk 0): the same rows and info as eda.decompose, host and device, bitwise."""
+    from eda import _core_hip
+
+    torch = torch_cuda()
+    x = workloads.rings_stripes_blobs(3, 64, np.float32)
+    y, info = eda.decompose(x, axes=(1, 2), return_info=True)
+    rows, raw = _core_hip.decompose_host(x, -1, None, 3, True, False, 0)
+    assert np.array_equal(rows, y)
+    for k in ("n_imfs", "error", "windows", "sifts", "spacing", "energy"):
+        assert np.array_equal(raw[k], getattr(info, k)), k
+    assert raw["chunk"] == 3
+    # a chunk limit changes nothing but the chunk
+    rows1, raw1 = _core_hip.decompose_host(x, -1, None, 3, True, False, 1)
+    assert np.array_equal(rows1, y) and raw1["chunk"] == 1
+    # device input: (3, 64, 64) viewed through a transposed (64, 64, 3) tensor
+    xt = torch.from_numpy(np.ascontiguousarray(np.moveaxis(x, 0, 2))).to("cuda")
+    owner, ptr, rawd = _core_hip.decompose_device(
+        xt.data_ptr(), 0, "float32", "float32", (3,), (1,), (64, 64), (64 * 3, 3),
+        [int(torch.cuda.current_stream().cuda_stream)], -1, None, 3, True, False, 0)
+    assert ptr == owner.data_ptr() and tuple(owner.shape) == y.shape
+    assert np.array_equal(owner.cpu().numpy(), y)
+    assert np.array_equal(rawd["windows"], info.windows)
+    assert _core_hip.pointer_device(ptr) == 0
+    assert _core_hip.device_count() >= 1 and "gfx950" in _core_hip.device_info(0)["gcn_arch"]
+
+
 def test_full_size_benchmark_batch_properties():
     """64 x 512 x 512 float32 on the device: reconstruction, batch invariance of a sample of
     elements, and one element against the FP64 oracle."""
