@@ -56,6 +56,74 @@ def traffic_from_profiles():
         return None
 
 
+COPY_PROBE_BYTES = 1 << 30  # the known-size copy of --copy-probe: 1 GiB read + 1 GiB written
+
+
+def _counter_rows(directory):
+    """(kernel name, counter name, value) of every counter_collection.csv under directory."""
+    import csv
+
+    for path in sorted(Path(directory).rglob("*counter_collection.csv")):
+        with open(path, newline="") as f:
+            for row in csv.DictReader(f):
+                yield row.get("Kernel_Name", ""), row.get("Counter_Name", ""), float(row.get("Counter_Value", 0.0))
+
+
+def pmc_summary(fetch_dir, write_dir, out_json):
+    """HBM traffic per k_vhgw launch from two rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE,
+    KiB per dispatch) of `bench.py --steps 1 --warmup 0 --no-cpu --copy-probe`, each scaled by
+    what the counter reads for the probe's copy kernel (torch.neg over 1 GiB: 1 GiB read, 1 GiB
+    written)."""
+    def averages(directory, counter):
+        vh, cp = [], []
+        for name, cname, value in _counter_rows(directory):
+            if cname != counter:
+                continue
+            if "k_vhgw" in name:
+                vh.append(value)
+            elif "neg" in name:
+                cp.append(value)
+        if not vh or not cp:
+            raise SystemExit(f"bench: no {counter} rows for k_vhgw and the copy probe under {directory}")
+        return sum(vh) / len(vh), max(cp), len(vh)
+
+    f_vh, f_cp, n_f = averages(fetch_dir, "FETCH_SIZE")
+    w_vh, w_cp, n_w = averages(write_dir, "WRITE_SIZE")
+    f_scale = COPY_PROBE_BYTES / (f_cp * 1024.0)
+    w_scale = COPY_PROBE_BYTES / (w_cp * 1024.0)
+    traffic = f_vh * 1024.0 * f_scale + w_vh * 1024.0 * w_scale
+    res = {
+        "kernel": "k_vhgw",
+        "workload": "eda.decompose fwd, 64 x 512 x 512 float32 (one call)",
+        "launches": n_f,
+        "fetch_kib_per_launch": round(f_vh, 1),
+        "write_kib_per_launch": round(w_vh, 1),
+        "copy_probe_fetch_kib": round(f_cp, 1),
+        "copy_probe_write_kib": round(w_cp, 1),
+        "fetch_scale": round(f_scale, 4),
+        "write_scale": round(w_scale, 4),
+        "traffic_bytes_per_launch": round(traffic),
+    }
+    Path(out_json).write_text(json.dumps(res, indent=1) + "\n")
+    print(json.dumps(res))
+    return 0
+
+
+def copy_probe(torch, log_fn):
+    """One elementwise copy kernel of known size (torch.neg over 1 GiB of float32) for scaling the
+    HBM byte counters."""
+    n = COPY_PROBE_BYTES // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    torch.neg(a, out=b)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    log_fn(f"bench: copy probe: {2 * COPY_PROBE_BYTES / dt / 1e9:.0f} GB/s (torch.neg over 1 GiB, wall clock)")
+    del a, b
+
+
 def cpu_baseline():
     import numpy as np
 
@@ -88,7 +156,13 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run one copy kernel of known size (for scaling rocprofv3 byte counters)")
+    ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "OUT_JSON"),
+                    help="summarize two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.pmc_summary:
+        return pmc_summary(*args.pmc_summary)
     if args.gpus != 1:
         print(
             "bench.py: measures one GPU only (the library runs on one GPU per call; batch elements "
@@ -137,6 +211,9 @@ def main() -> int:
     lib.eda_timer_enable(0)
     log(f"bench: {args.steps} steps, {dt * 1e3:.1f} ms per step; k_vhgw: {launches.value} launches, "
         f"{ms.value:.1f} ms in total")
+
+    if args.copy_probe:
+        copy_probe(torch, log)
 
     # the host-input rate (numpy in, numpy out: PCIe transfers included); reported on stderr only
     xh = x.cpu().numpy()
