@@ -51,7 +51,6 @@ void enqueue_cube(hipStream_t s, const Geom& g, CubeOp op, const T* in, T* out0,
 template <class T>
 void enqueue_cube_rungs(hipStream_t s, const Geom& g, CubeOp op, const T* in, T* out0, T* out1, const int32_t* win,
                         int32_t rungs, int64_t stride, int32_t max_w);
-int32_t vhgw_task_slots(int32_t n, int32_t max_w);
 
 // ---- reductions (reduce.hip) ---------------------------------------------------------------------
 // Per element: energy[b] = the canonical pairwise sum of squares (double; nullptr: none),

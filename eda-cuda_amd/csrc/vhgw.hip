@@ -3,50 +3,309 @@
 // van Herk / Gil-Werman passes (reference: cpp/src/field/vhgw.cu:1-36 for the canonical order,
 // cpp/src/method/common.cpp:65-136 for the cube; host twin cpp/tests/twin.hpp:74-133).
 //
-// Canonical arithmetic of one line of n samples and window w: r = min((w-1)/2, n-1), wa = 2r+1;
-// the extended line e in [-r, n+r) reads the line through half-sample reflection; it is cut into
-// segments of wa samples starting at o + k wa, o = (n (wa+1)/2) mod wa. The prefix fold G of a
-// segment runs in 32-sample chunks: a sequential fold inside a chunk and a sequential carry over the
-// chunk totals, G = carry op local (chunk 0: local). The suffix fold H is G of the reversed line.
-// y(x) = op(H(x-r), G(x+r)); an aligned window (both ends in one segment) of a sum takes
-// 0.5 (H + G). Max and min are exact, so they fold integer order keys; sums depend only on (line,
-// n, w), never on the launch shape.
+// One line of n samples, window w: r = min((w-1)/2, n-1), wa = 2r+1; the extended line e in
+// [-r, n+r) reads the line through half-sample reflection.
 //
-// gfx950 mapping. A workgroup of 256 threads (4 waves of 64) stages P lines of one element in LDS
-// (V: the input line, R: the H folds and then the output), P = 4096 / n lines for the contiguous
-// axis and P adjacent columns for the strided axes, so that every global load and store is a
-// coalesced run of the row. The folds run as chunk tasks (line, segment, chunk): phase A folds
-// each chunk (32 sequential LDS operations) and records its total, phase B turns the totals of a
-// segment into carries by an in-place exclusive scan, phase C applies them. Line storage uses the
-// XOR swizzle i ^ ((i >> 5) & 31) (float) / & 15 (8-byte) and a line stride of round_up(n, 32) + 1,
-// so that chunk walks (32 lanes 32 samples apart) and column loads hit distinct LDS banks.
+// Max and min are exact, so their result does not depend on how the line is cut: with r <= n-1
+// every reflected sample of a window is also inside the clipped window [max(0, x-r), min(n-1, x+r)],
+// so y(x) is the max (min) of the clipped window. Here the line is cut into blocks [jL, jL+L),
+// L = wa, aligned at 0: H = the suffix folds and G = the prefix folds inside the blocks, and
+// y(x) = op(H(x-r), G(x+r)) except at the clipped ends (x < r: G(x+r); a window inside the last,
+// truncated block: H(x-r)). The folds compare integer order keys.
+//
+// The box sum keeps the reference's canonical order, which fixes its bits: segments of wa samples
+// starting at o + k wa, o = (n (wa+1)/2) mod wa; the prefix fold G of a segment runs in 32-sample
+// chunks (a sequential fold inside a chunk, a sequential carry over the chunk totals, G = carry +
+// local); the suffix fold H is G of the reversed line; y(x) = H(x-r) + G(x+r), times 0.5 when x+r
+// ends a segment (the window is the segment).
+//
+// gfx950 mapping. A workgroup of W wave64s stages P lines in LDS with coalesced global accesses (P
+// consecutive rows of the contiguous axis, or P adjacent columns of a strided axis: 64-byte runs),
+// one pad word per 32 samples. Each wave then owns one line at a time and works in registers:
+// max/min as per-lane runs of KT consecutive samples, block resets from a bitmask, and a segmented
+// scan across the lanes (DPP row shifts inside 16-lane rows, readlane across rows, only as deep as
+// a block is long); the box sum as one chunk per lane, folded in registers, with the chunk carries
+// passed lane to lane by DPP wave shifts. G and H go to the wave's own LDS rows, the output back
+// over the staged line, which the workgroup writes out coalesced. Workgroup ids are remapped so
+// that consecutive groups of one element run on one XCD and share its L2.
 #include "common.hpp"
 #include "kernels.hpp"
 
 #include <algorithm>
+#include <limits>
 
 namespace eda {
 namespace {
 
 constexpr int kVMax = 0, kVMin = 1, kVSum = 2;
 
-template <class S>
-__device__ __forceinline__ int32_t swz(int32_t i) {
-  constexpr int32_t m = sizeof(S) == 4 ? 31 : 15;
-  return i ^ ((i >> 5) & m);
+// LDS slot of sample i of a staged line: one pad word per 32 samples, so that lanes 32 samples
+// apart (the chunk walks), lanes KT samples apart (the register runs) and the columns of a staged
+// tile fall in distinct banks.
+__host__ __device__ __forceinline__ int32_t pidx(int32_t i) { return i + (i >> 5); }
+
+// i / L and i mod L for 0 <= i < 2^24 by a float reciprocal and one correction.
+__device__ __forceinline__ int32_t div_small(int32_t i, int32_t L, float inv) {
+  int32_t q = static_cast<int32_t>(static_cast<float>(i) * inv);
+  const int32_t m = i - q * L;
+  q += m < 0 ? -1 : (m >= L ? 1 : 0);
+  return q;
 }
-__host__ __device__ __forceinline__ int32_t line_stride(int32_t n) { return (n + 31) / 32 * 32 + 1; }
+__device__ __forceinline__ int32_t mod_small(int32_t i, int32_t L, float inv) {
+  int32_t m = i - static_cast<int32_t>(static_cast<float>(i) * inv) * L;
+  m += m < 0 ? L : (m >= L ? -L : 0);
+  return m;
+}
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---- cross-lane moves -----------------------------------------------------------------------------
+// DPP (gfx9 encodings): row_shr:d gives lane i the value of lane i-d of its 16-lane row, row_shl:d
+// that of lane i+d; wave_shr:1 / wave_shl:1 shift by one lane across the wave. Lanes whose source
+// lies outside the row (wave) keep `old`.
+constexpr int kRowShl = 0x100, kRowShr = 0x110, kWaveShl1 = 0x130, kWaveShr1 = 0x138;
+
+template <int CTRL>
+__device__ __forceinline__ int32_t dpp32(int32_t old, int32_t v) {
+  return __builtin_amdgcn_update_dpp(old, v, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL, class S>
+__device__ __forceinline__ S dpp(S old, S v) {
+  if constexpr (sizeof(S) == 4) {
+    return __builtin_bit_cast(S, dpp32<CTRL>(__builtin_bit_cast(int32_t, old), __builtin_bit_cast(int32_t, v)));
+  } else {
+    const uint64_t o = __builtin_bit_cast(uint64_t, old), x = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = static_cast<uint32_t>(
+        dpp32<CTRL>(static_cast<int32_t>(static_cast<uint32_t>(o)), static_cast<int32_t>(static_cast<uint32_t>(x))));
+    const uint32_t hi = static_cast<uint32_t>(dpp32<CTRL>(static_cast<int32_t>(static_cast<uint32_t>(o >> 32)),
+                                                          static_cast<int32_t>(static_cast<uint32_t>(x >> 32))));
+    return __builtin_bit_cast(S, (static_cast<uint64_t>(hi) << 32) | lo);
+  }
+}
+template <class S>
+__device__ __forceinline__ S readlane(S v, int l) {
+  if constexpr (sizeof(S) == 4) {
+    return __builtin_bit_cast(S, __builtin_amdgcn_readlane(__builtin_bit_cast(int32_t, v), l));
+  } else {
+    const uint64_t x = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int32_t>(static_cast<uint32_t>(x)), l));
+    const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int32_t>(static_cast<uint32_t>(x >> 32)), l));
+    return __builtin_bit_cast(S, (static_cast<uint64_t>(hi) << 32) | lo);
+  }
+}
 
 template <int OP, class S>
-__device__ __forceinline__ S fold_op(S a, S b) {
+__device__ __forceinline__ S mm_op(S a, S b) {
   if constexpr (OP == kVMax) return a > b ? a : b;
-  else if constexpr (OP == kVMin) return a < b ? a : b;
-  else return a + b;
+  else return a < b ? a : b;
+}
+template <int OP, class S>
+__device__ __forceinline__ S mm_ident() {
+  if constexpr (OP == kVMax) return std::numeric_limits<S>::lowest();
+  else return std::numeric_limits<S>::max();
 }
 
-// The per-window constants of a line (all lines of a workgroup share the element and its window).
+// One doubling step of a segmented scan: (a, f) takes the lane D back (UP) or ahead (!UP) in its
+// row unless f is set.
+template <bool UP, int D, int OP, class S>
+__device__ __forceinline__ void seg_step(S& a, int& f, S id) {
+  constexpr int ctrl = UP ? kRowShr + D : kRowShl + D;
+  const S a2 = dpp<ctrl>(id, a);
+  const int f2 = dpp32<ctrl>(0, f);
+  if (!f) {
+    a = mm_op<OP>(a2, a);
+    f = f2;
+  }
+}
+// Inclusive segmented scan over the wave: a lane's (a, f) becomes the fold of a over the lanes from
+// the nearest lane with f = 1 to itself, in lane order (UP) or reverse lane order. depth: the
+// doubling steps inside a row (chains of up to 2^depth - 1 lanes back); then the rows in order.
+template <bool UP, int OP, class S>
+__device__ __forceinline__ void seg_scan(S& a, int& f, int depth, S id) {
+  if (depth > 0) seg_step<UP, 1, OP>(a, f, id);
+  if (depth > 1) seg_step<UP, 2, OP>(a, f, id);
+  if (depth > 2) seg_step<UP, 4, OP>(a, f, id);
+  if (depth > 3) seg_step<UP, 8, OP>(a, f, id);
+  const int row = (threadIdx.x & 63) >> 4;
+#pragma unroll
+  for (int k = 1; k < 4; ++k) {
+    const int rr = UP ? k : 3 - k;                 // the row that takes its neighbour's value
+    const int src = UP ? 16 * rr - 1 : 16 * rr + 16;  // the neighbour row's last (first) lane
+    const S as = readlane(a, src);
+    const int fs = readlane(f, src);
+    if (row == rr && !f) {
+      a = mm_op<OP>(as, a);
+      f = fs;
+    }
+  }
+}
+
+// ---- max / min of one line (one wave) -----------------------------------------------------------------
+// V: the staged keys, valid up to the tile end (samples beyond n repeat x[n-1]); Hb, Gb: the
+// wave's rows; Y: the output slots (over V).
+// The per-line constants of the block folds: block length L = 2r+1, the block starts of a run of KT
+// samples as a bitmask (the multiples of L, shifted to the run's first start), and the segmented
+// scan's depth: a chain of lanes without a start is at most (L - 1) / KT long.
+struct MmCtl {
+  int32_t n, r, L;
+  float inv;
+  uint32_t pat;
+  int32_t chain;
+  int depth;
+};
+template <int KT>
+__device__ __forceinline__ MmCtl mm_ctl(int32_t n, int32_t r) {
+  MmCtl c;
+  c.n = n;
+  c.r = r;
+  c.L = 2 * r + 1;
+  c.inv = 1.0f / static_cast<float>(c.L);
+  c.pat = 0;
+  for (int32_t j = 0; j < KT; j += c.L) c.pat |= 1u << j;
+  c.chain = (c.L - 1) / KT;
+  c.depth = c.chain == 0 ? 0 : c.chain < 2 ? 1 : c.chain < 4 ? 2 : c.chain < 8 ? 3 : 4;
+  return c;
+}
+template <int KT>
+__device__ __forceinline__ uint32_t mm_starts(const MmCtl& c, int32_t i0) {
+  constexpr uint32_t kMask = KT >= 32 ? 0xffffffffu : ((1u << KT) - 1u);
+  const int32_t m0 = mod_small(i0, c.L, c.inv);
+  const int32_t k0 = m0 == 0 ? 0 : c.L - m0;
+  return k0 < KT ? (c.pat << k0) & kMask : 0u;
+}
+
+// H (suffix folds inside the blocks) of one tile whose lane run starts at i0, the keys in v; the
+// tile to the right is folded in through carry (its H at its first sample). Writes Hb[pidx(i0 + k)]
+// and returns this tile's H at its first sample.
+template <int OP, class S, int KT>
+__device__ __forceinline__ S mm_tile_h(const S (&v0)[KT], int32_t i0, S carry, S* __restrict__ Hb, const MmCtl& c) {
+  const S id = mm_ident<OP, S>();
+  const uint32_t starts = mm_starts<KT>(c, i0);
+  const uint32_t ends = (starts >> 1) | (mod_small(i0 + KT, c.L, c.inv) == 0 ? 1u << (KT - 1) : 0u);
+  S v[KT];
+  S h = id;
+#pragma unroll
+  for (int k = KT - 1; k >= 0; --k) {
+    h = (ends >> k) & 1u ? v0[k] : mm_op<OP>(v0[k], h);
+    v[k] = h;
+  }
+  S a = v[0];
+  int f = ends != 0;
+  if (c.chain > 0) seg_scan<false, OP>(a, f, c.depth, id);
+  if (!f) a = mm_op<OP>(a, carry);
+  const S nxt = dpp<kWaveShl1>(carry, a);
+  const int kl = ends ? 31 - __clz(static_cast<int>(ends)) : -1;
+  S* dst = Hb + pidx(i0);
+#pragma unroll
+  for (int k = 0; k < KT; ++k) dst[k] = k > kl ? mm_op<OP>(v[k], nxt) : v[k];
+  return readlane(a, 0);
+}
+// G (prefix folds inside the blocks) of one tile; carry: the tile to the left's G at its last
+// sample. Writes Gb[pidx(i0 + k)] and returns this tile's G at its last sample.
+template <int OP, class S, int KT>
+__device__ __forceinline__ S mm_tile_g(const S (&v0)[KT], int32_t i0, S carry, S* __restrict__ Gb, const MmCtl& c) {
+  const S id = mm_ident<OP, S>();
+  const uint32_t starts = mm_starts<KT>(c, i0);
+  S v[KT];
+  S g = id;
+#pragma unroll
+  for (int k = 0; k < KT; ++k) {
+    g = (starts >> k) & 1u ? v0[k] : mm_op<OP>(g, v0[k]);
+    v[k] = g;
+  }
+  S a = v[KT - 1];
+  int f = starts != 0;
+  if (c.chain > 0) seg_scan<true, OP>(a, f, c.depth, id);
+  if (!f) a = mm_op<OP>(carry, a);
+  const S prv = dpp<kWaveShr1>(carry, a);
+  const int kf = starts ? __ffs(static_cast<int>(starts)) - 1 : KT;
+  S* dst = Gb + pidx(i0);
+#pragma unroll
+  for (int k = 0; k < KT; ++k) dst[k] = k < kf ? mm_op<OP>(prv, v[k]) : v[k];
+  return readlane(a, 63);
+}
+// y(x) for the lane's x = lane + 64 j < n from H and G: emit(x, y).
+template <int OP, class S, class Emit>
+__device__ __forceinline__ void mm_out(const S* __restrict__ Hb, const S* __restrict__ Gb, const MmCtl& c, Emit emit) {
+  const int lane = threadIdx.x & 63;
+  const int32_t n = c.n, r = c.r;
+  const int32_t last = div_small(n - 1, c.L, c.inv) * c.L;  // the last block's start
+  for (int32_t x0 = 0; x0 < n; x0 += 64) {
+    const int32_t x = x0 + lane;
+    if (x0 >= r && x0 + 63 + r <= n - 1) {  // every window of the 64 lies inside the line
+      emit(x, mm_op<OP>(Hb[pidx(x - r)], Gb[pidx(x + r)]));
+    } else if (x < n) {
+      const int32_t lo = x - r > 0 ? x - r : 0;
+      const int32_t hi = x + r < n - 1 ? x + r : n - 1;
+      const S h = Hb[pidx(lo)];
+      const S g = Gb[pidx(hi)];
+      S y;
+      if (x < r) y = g;
+      else if (x + r > n - 1 && lo >= last) y = h;
+      else y = mm_op<OP>(h, g);
+      emit(x, y);
+    }
+  }
+}
+
+// mm_out into registers: y[j] = y(64 j + lane) (junk where that is n or more).
+template <int OP, class S, int KT>
+__device__ __forceinline__ void mm_out_regs(const S* __restrict__ Hb, const S* __restrict__ Gb, const MmCtl& c,
+                                            S (&y)[KT]) {
+  const int lane = threadIdx.x & 63;
+  const int32_t n = c.n, r = c.r;
+  const int32_t last = div_small(n - 1, c.L, c.inv) * c.L;
+#pragma unroll
+  for (int j = 0; j < KT; ++j) {
+    const int32_t x = 64 * j + lane;
+    const int32_t lo = x - r > 0 ? x - r : 0;
+    const int32_t hi = x + r < n - 1 ? x + r : n - 1;
+    const S h = Hb[pidx(min(lo, n - 1))];
+    const S g = Gb[pidx(hi)];
+    if (x < r) y[j] = g;
+    else if (x + r > n - 1 && lo >= last) y[j] = h;
+    else y[j] = mm_op<OP>(h, g);
+  }
+}
+
+// A staged line of any length (tiles of 64 KT samples): V valid up to the tile end (samples beyond
+// n repeat x[n-1]); Y: the output slots (over V).
+template <int OP, class S, int KT>
+__device__ void mm_line(const S* __restrict__ V, S* __restrict__ Hb, S* __restrict__ Gb, S* Y, int32_t n,
+                        int32_t r) {
+  const int lane = threadIdx.x & 63;
+  const MmCtl c = mm_ctl<KT>(n, r);
+  const S id = mm_ident<OP, S>();
+  constexpr int32_t kTile = 64 * KT;
+  const int32_t ntiles = (n + kTile - 1) / kTile;
+  S carry = id;
+  for (int32_t t = ntiles - 1; t >= 0; --t) {
+    const int32_t i0 = t * kTile + lane * KT;
+    S v[KT];
+#pragma unroll
+    for (int k = 0; k < KT; ++k) v[k] = V[pidx(i0) + k];
+    carry = mm_tile_h<OP, S, KT>(v, i0, carry, Hb, c);
+  }
+  carry = id;
+  for (int32_t t = 0; t < ntiles; ++t) {
+    const int32_t i0 = t * kTile + lane * KT;
+    S v[KT];
+#pragma unroll
+    for (int k = 0; k < KT; ++k) v[k] = V[pidx(i0) + k];
+    carry = mm_tile_g<OP, S, KT>(v, i0, carry, Gb, c);
+  }
+  wave_sync();
+  mm_out<OP, S>(Hb, Gb, c, [&](int32_t x, S y) { Y[pidx(x)] = y; });
+  wave_sync();
+}
+
+// ---- the canonical box sum of one line (one wave) --------------------------------------------------------
 struct WinInfo {
-  int32_t r, wa, o, k_lo, nsegs, cs, tasks;
+  int32_t r, wa, o, k_lo, nsegs, cs;
 };
 __host__ __device__ __forceinline__ WinInfo win_info(int32_t n, int32_t w) {
   WinInfo v;
@@ -58,24 +317,114 @@ __host__ __device__ __forceinline__ WinInfo win_info(int32_t n, int32_t w) {
   const int32_t k_hi = static_cast<int32_t>(floor_div(static_cast<int64_t>(n) - 1 + v.r - v.o, v.wa));
   v.nsegs = k_hi - v.k_lo + 1;
   v.cs = (v.wa + 31) / 32;
-  v.tasks = v.nsegs * v.cs;
   return v;
 }
 
+// DIR 0: G of the line into R at slot x = e - r; DIR 1: G of the reversed line (H) into R at slot
+// x = e + r of the forward position e. One chunk per lane, CH >= the longest chunk. Samples past a
+// chunk's end add -0, which leaves every fold's bits unchanged; stores that belong to no output go
+// to the junk slot.
+template <int DIR, int CH, class T>
+__device__ void sum_dir(const T* __restrict__ V, T* __restrict__ R, int32_t junk, int32_t n, const WinInfo& wi) {
+  const int lane = threadIdx.x & 63;
+  const int32_t r = wi.r, wa = wi.wa;
+  const int32_t total = wi.nsegs * wi.cs;
+  const float inv_cs = 1.0f / static_cast<float>(wi.cs);
+  const int iters = wi.cs - 1 < 64 ? wi.cs - 1 : 64;
+  const T nz = -T(0);
+  const int32_t e_max = n - 1 + r;
+  T carry = nz;
+  for (int32_t t0 = 0; t0 < total; t0 += 64) {
+    const int32_t t = t0 + lane;
+    const int32_t kk = div_small(t, wi.cs, inv_cs);
+    const int32_t q = t - kk * wi.cs;
+    const int32_t s = wi.o + (wi.k_lo + kk) * wa;
+    const int32_t e_last = min(s + wa - 1, e_max);
+    const int32_t c0 = s + 32 * q;
+    const int32_t len = t < total ? min(32, e_last - c0 + 1) : 0;
+    const int32_t c1 = min(c0 + (len > 0 ? len - 1 : 0), e_max);  // the last sample read
+    T v[CH];
+    T acc = nz;
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+      const int32_t e = min(c0 + j, c1);
+      // half-sample reflection of e in [-r, n + r) (r <= n - 1): ~e below 0, 2n - 1 - e above n - 1
+      const int32_t ef = e ^ (e >> 31);
+      int32_t i = min(ef, 2 * n - 1 - ef);
+      if (DIR) i = n - 1 - i;
+      const T xv = V[pidx(i)];
+      const T add = j < len ? xv : nz;
+      acc = j == 0 ? add : acc + add;
+      v[j] = acc;
+    }
+    // the carries: out = the fold of the segment up to this chunk's end (chunk 0: its total);
+    // cin = the previous chunk's out; lane 0 continues the previous round's lane 63
+    T out = acc, cin = nz;
+    for (int it = 0; it < iters; ++it) {
+      const T prev = dpp<kWaveShr1>(carry, out);
+      if (q > 0) {
+        cin = prev;
+        out = prev + acc;
+      }
+    }
+    carry = readlane(out, 63);
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+      const int32_t e = c0 + j;
+      int32_t x = e - r;
+      if (DIR) x = n - 1 - x;
+      const int32_t slot = j < len && e >= r ? pidx(x) : junk;
+      R[slot] = q > 0 ? cin + v[j] : v[j];
+    }
+  }
+}
+
+// The box sum of the line staged in V (one wave): emit(x, y) for the lane's x = lane + 64 j < n.
+template <int CH, class T, class Emit>
+__device__ void sum_line_ch(const T* __restrict__ V, T* __restrict__ Hb, T* __restrict__ Gb, int32_t junk, int32_t n,
+                            const WinInfo& wi, Emit emit) {
+  sum_dir<1, CH, T>(V, Hb, junk, n, wi);
+  sum_dir<0, CH, T>(V, Gb, junk, n, wi);
+  wave_sync();
+  const int lane = threadIdx.x & 63;
+  const int32_t wa = wi.wa;
+  const int32_t d = 64 % wa;
+  int32_t m = static_cast<int32_t>(floor_mod(static_cast<int64_t>(lane) + wi.r - wi.o, wa));
+  for (int32_t x = lane; x < n; x += 64) {
+    const T sum = Hb[pidx(x)] + Gb[pidx(x)];
+    emit(x, m == wa - 1 ? sum * T(0.5) : sum);
+    m += d;
+    m -= m >= wa ? wa : 0;
+  }
+}
+
+template <class T, class Emit>
+__device__ void sum_line(const T* V, T* Hb, T* Gb, int32_t junk, int32_t n, const WinInfo& wi, Emit emit) {
+  const int32_t longest = wi.wa < 32 ? wi.wa : 32;
+  if (longest <= 4) sum_line_ch<4, T>(V, Hb, Gb, junk, n, wi, emit);
+  else if (longest <= 8) sum_line_ch<8, T>(V, Hb, Gb, junk, n, wi, emit);
+  else if (longest <= 16) sum_line_ch<16, T>(V, Hb, Gb, junk, n, wi, emit);
+  else sum_line_ch<32, T>(V, Hb, Gb, junk, n, wi, emit);
+}
+
+// ---- the pass kernel ----------------------------------------------------------------------------------------
 struct VPass {
   Geom g;
-  int32_t axis;
-  int32_t n;            // line length
-  int64_t step;         // sample stride along the line
-  int64_t inner;        // adjacent lines (1 on the contiguous axis)
-  int64_t outer;        // line groups per element along the other axes
-  int32_t P;            // lines per workgroup
-  int64_t groups;       // workgroups per element
-  int32_t tmax;         // chunk-task slots per line (LDS)
-  int32_t nf;           // fields: 1, or 2 (max then min, same input, or two inputs)
-  int32_t same_input;   // nf == 2 from one input (staged once)
+  int32_t n;        // line length
+  int64_t step;     // sample stride along the line
+  int64_t inner;    // adjacent lines (1 on the contiguous axis)
+  int64_t outer;    // rows (contiguous axis) or column blocks per element
+  int32_t P;        // lines per workgroup (a power of two on a strided axis)
+  int32_t lpad;     // LDS slots per line
+  int32_t span;     // samples per line the max / min tiles cover (n rounded up to the tile)
+  int32_t flat;     // rows with pidx(p n + i) = p lpad + pidx(i)
+  int64_t groups;   // workgroups per element
+  int32_t nf;       // output fields: 1, or 2 (ops[0] into out[0], ops[1] into out[1])
   int32_t ops[2];
-  int32_t scale;        // multiply by fl_T(1 / prod of the clamped sides) (the last box pass)
+  int32_t scale;    // multiply by fl_T(1 / prod of the clamped sides) (the last box pass)
+  int32_t xcd;      // remap workgroup ids: consecutive groups on one XCD
+  int32_t vec;      // strided axis: 16-byte global accesses over 16 / sizeof(T) adjacent columns
+  int32_t wrows;    // LDS rows per wave: 1 (max / min of one tile: H in place), 2 (H and G)
 };
 
 // gridDim.y passes (rungs) in one launch: rung j reads win + j B, in + j in_stride and writes
@@ -88,132 +437,34 @@ struct VPtrs {
   int64_t in_stride, out_stride;
 };
 
-// One direction of one stage on every line of the workgroup. DIR 1: H (the reversed line), whose
-// local folds and then carried folds go into R at the output slots; DIR 0: G, which combines with
-// R's H into the stage output in R.
-template <class S, int OP, int DIR>
-__device__ void stage_dir(const S* __restrict__ V, S* __restrict__ R, S* __restrict__ tot, int P, int32_t ls,
-                          int32_t tmax, int32_t n, const WinInfo& wi) {
-  const int tid = threadIdx.x;
-  const int32_t r = wi.r, wa = wi.wa;
-  const int32_t total = P * wi.tasks;
-  const bool chain = wi.cs > 1;
-  const auto load = [&](const S* Vl, int32_t e) {
-    int32_t i = reflect_index(e, n);
-    if (DIR) i = n - 1 - i;
-    return Vl[swz<S>(i)];
-  };
-  const auto slot = [&](int32_t e) {
-    int32_t x = e - r;
-    if (DIR) x = n - 1 - x;
-    return swz<S>(x);
-  };
-  // phase A: chunk folds (H: local values into R at the output slots) and the chunk totals
-  if (DIR == 1 || chain) {
-    for (int32_t t = tid; t < total; t += kThreads) {
-      const int32_t p = t / wi.tasks, tl = t - p * wi.tasks;
-      const int32_t k = wi.k_lo + tl / wi.cs, q = tl % wi.cs;
-      const int32_t s = wi.o + k * wa;
-      const int32_t e_last = min(s + wa - 1, n - 1 + r);
-      const int32_t c0 = s + 32 * q;
-      const int32_t c1 = min(c0 + 31, e_last);
-      if (c0 > c1) continue;
-      const S* Vl = V + p * ls;
-      S* Rl = R + p * ls;
-      S acc = load(Vl, c0);
-      if (DIR == 1 && c0 >= r) Rl[slot(c0)] = acc;
-      for (int32_t e = c0 + 1; e <= c1; ++e) {
-        acc = fold_op<OP>(acc, load(Vl, e));
-        if (DIR == 1 && e >= r) Rl[slot(e)] = acc;
-      }
-      if (chain) tot[p * tmax + tl] = acc;
-    }
-    __syncthreads();
-  }
-  // phase B: per segment, the exclusive scan of the chunk totals (in place: slot q holds the carry
-  // of chunk q, the fold of the totals of chunks 0 .. q-1)
-  if (chain) {
-    for (int32_t t = tid; t < P * wi.nsegs; t += kThreads) {
-      const int32_t p = t / wi.nsegs, kk = t - p * wi.nsegs;
-      const int32_t s = wi.o + (wi.k_lo + kk) * wa;
-      const int32_t e_last = min(s + wa - 1, n - 1 + r);
-      S* tl = tot + p * tmax + kk * wi.cs;
-      S run = tl[0];
-      for (int32_t q = 1; q < wi.cs && s + 32 * q <= e_last; ++q) {
-        const S v = tl[q];
-        tl[q] = run;
-        run = fold_op<OP>(run, v);
-      }
-    }
-    __syncthreads();
-  }
-  // phase C
-  for (int32_t t = tid; t < total; t += kThreads) {
-    const int32_t p = t / wi.tasks, tl = t - p * wi.tasks;
-    const int32_t k = wi.k_lo + tl / wi.cs, q = tl % wi.cs;
-    const int32_t s = wi.o + k * wa;
-    const int32_t e_last = min(s + wa - 1, n - 1 + r);
-    const int32_t c0 = s + 32 * q;
-    const int32_t c1 = min(c0 + 31, e_last);
-    if (c0 > c1) continue;
-    S* Rl = R + p * ls;
-    const S carry = q > 0 ? tot[p * tmax + tl] : S(0);
-    if constexpr (DIR == 1) {
-      if (q == 0) continue;
-      for (int32_t e = max(c0, r); e <= c1; ++e) {
-        const int32_t sl = slot(e);
-        Rl[sl] = fold_op<OP>(carry, Rl[sl]);
-      }
-    } else {
-      const S* Vl = V + p * ls;
-      const int32_t e_al = s + wa - 1;
-      S loc = S(0);
-      for (int32_t e = c0; e <= c1; ++e) {
-        const S xv = load(Vl, e);
-        loc = e == c0 ? xv : fold_op<OP>(loc, xv);
-        if (e >= r) {
-          const S gv = q > 0 ? fold_op<OP>(carry, loc) : loc;
-          const int32_t sl = slot(e);
-          const S h = Rl[sl];
-          if constexpr (OP == kVSum) {
-            const S sum = h + gv;
-            Rl[sl] = e == e_al ? sum * S(0.5) : sum;
-          } else {
-            Rl[sl] = fold_op<OP>(h, gv);
-          }
-        }
-      }
-    }
-  }
-  __syncthreads();
-}
-
-template <class S, int OP>
-__device__ void run_stage(const S* V, S* R, S* tot, int P, int32_t ls, int32_t tmax, int32_t n, const WinInfo& wi) {
-  stage_dir<S, OP, 1>(V, R, tot, P, ls, tmax, n, wi);
-  stage_dir<S, OP, 0>(V, R, tot, P, ls, tmax, n, wi);
-}
-
-// S: the LDS type (T for sums; the integer order key of T for max / min).
-template <class T, class S>
-__global__ void __launch_bounds__(kThreads) k_vhgw(VPass a, VPtrs<T> ptr) {
+// S: the LDS type (T for the box sum; the integer order key of T for max / min). KT: samples per
+// lane of a max / min tile.
+template <class T, class S, int KT>
+__global__ void __launch_bounds__(512) k_vhgw(VPass a, VPtrs<T> ptr) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int64_t wg = blockIdx.x;
+  int64_t wg = blockIdx.x;
+  if (a.xcd) {
+    const int64_t per = gridDim.x >> 3;
+    wg = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  }
   const int64_t b = wg / a.groups;
   const int64_t grp = wg - b * a.groups;
   const int64_t rung = blockIdx.y;
   const int32_t w = ptr.win[rung * a.g.batch + b];
   if (w < 1) return;  // uniform: the element does not run
   const int32_t n = a.n;
-  const int32_t ls = line_stride(n);
-  const WinInfo wi = win_info(n, w);
+  const int nthr = blockDim.x;
+  const int W = nthr >> 6;
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  // V: P staged lines; per wave wrows rows of lpad + 1 slots (the last one: the junk slot), H then
+  // G (one row: G only, the line's H goes over its staged samples)
   S* V = reinterpret_cast<S*>(smem);
-  S* R = V + a.P * ls;
-  S* tot = R + a.P * ls;
-  // the lines of this workgroup: P consecutive rows (contiguous axis) or P adjacent columns
+  S* Hb = V + a.P * a.lpad + wave * a.wrows * (a.lpad + 1);
+  S* Gb = a.wrows == 2 ? Hb + a.lpad + 1 : Hb;
+  const bool contiguous = a.inner == 1;
   int64_t base0;
   int32_t lines;
-  const bool contiguous = a.inner == 1;
   if (contiguous) {
     const int64_t l0 = grp * a.P;
     lines = static_cast<int32_t>(min<int64_t>(a.P, a.outer - l0));
@@ -225,7 +476,9 @@ __global__ void __launch_bounds__(kThreads) k_vhgw(VPass a, VPtrs<T> ptr) {
     lines = static_cast<int32_t>(min<int64_t>(a.P, a.inner - j0));
     base0 = b * a.g.count + o * static_cast<int64_t>(n) * a.inner + j0;
   }
-  const int32_t total_pos = lines * n;
+  const int32_t total = lines * n;
+  const float inv_n = 1.0f / static_cast<float>(n);
+  const int pshift = __ffs(a.P) - 1;  // strided axis: P is a power of two
   T fac = T(1);
   if (a.scale) {
     double prod = 1.0;
@@ -235,76 +488,410 @@ __global__ void __launch_bounds__(kThreads) k_vhgw(VPass a, VPtrs<T> ptr) {
     }
     fac = static_cast<T>(1.0 / prod);
   }
+  const int32_t half = (w - 1) / 2;
+  const int32_t r = half < n - 1 ? half : n - 1;
   for (int f = 0; f < a.nf; ++f) {
-    const int op = a.ops[f];
-    if (f == 0 || !a.same_input) {
-      const T* src = ptr.in[f] + rung * ptr.in_stride;
-      for (int32_t idx = threadIdx.x; idx < total_pos; idx += kThreads) {
-        int32_t p, i;
-        int64_t at;
-        if (contiguous) {
-          p = idx / n;
-          i = idx - p * n;
-          at = base0 + static_cast<int64_t>(p) * n + i;
+    const T* src = ptr.in[f] + rung * ptr.in_stride;
+    const auto put = [&](int32_t slot, T v) {
+      if constexpr (std::is_same_v<S, T>) V[slot] = v;
+      else V[slot] = okey(v);
+    };
+    if (a.flat) {
+      for (int32_t idx = threadIdx.x; idx < total; idx += nthr) put(pidx(idx), src[base0 + idx]);
+    } else if (contiguous) {
+      for (int32_t idx = threadIdx.x; idx < total; idx += nthr) {
+        const int32_t p = div_small(idx, n, inv_n);
+        const int32_t i = idx - p * n;
+        put(p * a.lpad + pidx(i), src[base0 + idx]);
+      }
+    } else if (a.vec && lines == a.P) {
+      // 16-byte loads: tpr threads per row, each VW adjacent columns
+      constexpr int VW = 16 / static_cast<int>(sizeof(T));
+      using V16 = std::conditional_t<sizeof(T) == 4, float4, double2>;
+      const int tpr = a.P / VW;
+      const int cg = threadIdx.x % tpr;
+      for (int32_t i = threadIdx.x / tpr; i < n; i += nthr / tpr) {
+        const V16 q = *reinterpret_cast<const V16*>(src + base0 + static_cast<int64_t>(i) * a.step + cg * VW);
+        if constexpr (VW == 4) {
+          put((cg * 4 + 0) * a.lpad + pidx(i), q.x);
+          put((cg * 4 + 1) * a.lpad + pidx(i), q.y);
+          put((cg * 4 + 2) * a.lpad + pidx(i), q.z);
+          put((cg * 4 + 3) * a.lpad + pidx(i), q.w);
         } else {
-          i = idx / lines;
-          p = idx - i * lines;
-          at = base0 + static_cast<int64_t>(i) * a.step + p;
+          put((cg * 2 + 0) * a.lpad + pidx(i), q.x);
+          put((cg * 2 + 1) * a.lpad + pidx(i), q.y);
         }
-        const T v = src[at];
-        S sv;
-        if constexpr (std::is_same_v<S, T>) sv = v;
-        else sv = okey(v);
-        V[p * ls + swz<S>(i)] = sv;
       }
-      __syncthreads();
-    }
-    if constexpr (std::is_same_v<S, T>) {
-      run_stage<S, kVSum>(V, R, tot, lines, ls, a.tmax, n, wi);
     } else {
-      if (op == kVMax) run_stage<S, kVMax>(V, R, tot, lines, ls, a.tmax, n, wi);
-      else run_stage<S, kVMin>(V, R, tot, lines, ls, a.tmax, n, wi);
+      const int32_t p = threadIdx.x & (a.P - 1);
+      if (p < lines)
+        for (int32_t i = threadIdx.x >> pshift; i < n; i += nthr >> pshift)
+          put(p * a.lpad + pidx(i), src[base0 + static_cast<int64_t>(i) * a.step + p]);
     }
-    T* dst = ptr.out[f] + rung * ptr.out_stride;
-    for (int32_t idx = threadIdx.x; idx < total_pos; idx += kThreads) {
-      int32_t p, i;
-      int64_t at;
-      if (contiguous) {
-        p = idx / n;
-        i = idx - p * n;
-        at = base0 + static_cast<int64_t>(p) * n + i;
+    __syncthreads();
+    for (int p = wave; p < lines; p += W) {
+      S* Vl = V + p * a.lpad;
+      if constexpr (std::is_same_v<S, T>) {
+        const WinInfo wi = win_info(n, w);
+        sum_line<T>(Vl, Hb, Gb, a.lpad, n, wi, [&](int32_t x, T y) { Vl[pidx(x)] = y; });
+        wave_sync();
       } else {
-        i = idx / lines;
-        p = idx - i * lines;
-        at = base0 + static_cast<int64_t>(i) * a.step + p;
+        // the tiles read up to the span: repeat x[n-1] there
+        const S tail = Vl[pidx(n - 1)];
+        for (int32_t i = n + lane; i < a.span; i += 64) Vl[pidx(i)] = tail;
+        wave_sync();
+        if (a.wrows == 1) {
+          // one tile: the line in registers, H over the staged samples, G in the wave's row, the
+          // outputs in registers until every lane has read H and G
+          const int32_t i0 = lane * KT;
+          S v[KT];
+#pragma unroll
+          for (int k = 0; k < KT; ++k) v[k] = Vl[pidx(i0) + k];
+          const MmCtl c = mm_ctl<KT>(n, r);
+          S y[KT];
+          if (a.ops[f] == kVMax) {
+            mm_tile_h<kVMax, S, KT>(v, i0, mm_ident<kVMax, S>(), Vl, c);
+            mm_tile_g<kVMax, S, KT>(v, i0, mm_ident<kVMax, S>(), Gb, c);
+            wave_sync();
+            mm_out_regs<kVMax, S, KT>(Vl, Gb, c, y);
+          } else {
+            mm_tile_h<kVMin, S, KT>(v, i0, mm_ident<kVMin, S>(), Vl, c);
+            mm_tile_g<kVMin, S, KT>(v, i0, mm_ident<kVMin, S>(), Gb, c);
+            wave_sync();
+            mm_out_regs<kVMin, S, KT>(Vl, Gb, c, y);
+          }
+          wave_sync();
+#pragma unroll
+          for (int j = 0; j < KT; ++j)
+            if (64 * j + lane < n) Vl[pidx(64 * j + lane)] = y[j];
+          wave_sync();
+        } else if (a.ops[f] == kVMax) {
+          mm_line<kVMax, S, KT>(Vl, Hb, Gb, Vl, n, r);
+        } else {
+          mm_line<kVMin, S, KT>(Vl, Hb, Gb, Vl, n, r);
+        }
       }
-      const S sv = R[p * ls + swz<S>(i)];
+    }
+    __syncthreads();
+    T* dst = ptr.out[f] + rung * ptr.out_stride;
+    const auto get = [&](int32_t slot) {
       T y;
-      if constexpr (std::is_same_v<S, T>) y = sv;
-      else y = okey_inv(sv);
-      if (a.scale) y = y * fac;
-      dst[at] = y;
+      if constexpr (std::is_same_v<S, T>) y = V[slot];
+      else y = okey_inv(V[slot]);
+      return a.scale ? y * fac : y;
+    };
+    if (a.flat) {
+      for (int32_t idx = threadIdx.x; idx < total; idx += nthr) dst[base0 + idx] = get(pidx(idx));
+    } else if (contiguous) {
+      for (int32_t idx = threadIdx.x; idx < total; idx += nthr) {
+        const int32_t p = div_small(idx, n, inv_n);
+        const int32_t i = idx - p * n;
+        dst[base0 + idx] = get(p * a.lpad + pidx(i));
+      }
+    } else if (a.vec && lines == a.P) {
+      constexpr int VW = 16 / static_cast<int>(sizeof(T));
+      using V16 = std::conditional_t<sizeof(T) == 4, float4, double2>;
+      const int tpr = a.P / VW;
+      const int cg = threadIdx.x % tpr;
+      for (int32_t i = threadIdx.x / tpr; i < n; i += nthr / tpr) {
+        V16 q;
+        if constexpr (VW == 4) {
+          q.x = get((cg * 4 + 0) * a.lpad + pidx(i));
+          q.y = get((cg * 4 + 1) * a.lpad + pidx(i));
+          q.z = get((cg * 4 + 2) * a.lpad + pidx(i));
+          q.w = get((cg * 4 + 3) * a.lpad + pidx(i));
+        } else {
+          q.x = get((cg * 2 + 0) * a.lpad + pidx(i));
+          q.y = get((cg * 2 + 1) * a.lpad + pidx(i));
+        }
+        *reinterpret_cast<V16*>(dst + base0 + static_cast<int64_t>(i) * a.step + cg * VW) = q;
+      }
+    } else {
+      const int32_t p = threadIdx.x & (a.P - 1);
+      if (p < lines)
+        for (int32_t i = threadIdx.x >> pshift; i < n; i += nthr >> pshift)
+          dst[base0 + static_cast<int64_t>(i) * a.step + p] = get(p * a.lpad + pidx(i));
     }
     __syncthreads();
   }
 }
 
+// ---- the streaming pass of the contiguous axis (lines of at most 64 * 32 samples) ------------------------
+// One wave per line, grid-stride over the (rung, element, line) items; each lane holds KT consecutive
+// samples of the line in registers, loaded with 16-byte loads where the line allows, and the next
+// item's samples are loaded while the current one is folded. Max / min fold the registers directly
+// (G and H through the wave's two LDS rows); the box sum stages the line in a third row for its
+// chunk walks. Outputs go straight to global memory, lanes on consecutive samples.
+struct RPass {
+  Geom g;
+  int32_t n;       // line length
+  int64_t lines;   // lines per element
+  int64_t items;   // rungs * batch * lines
+  int32_t nf;      // output fields: 1, or 2 (ops[0] into out[0], ops[1] into out[1])
+  int32_t two_in;  // nf == 2 with two inputs (else field 1 folds the input of field 0)
+  int32_t ops[2];
+  int32_t scale;   // multiply by fl_T(1 / prod of the clamped sides) (the last box pass)
+  int32_t vec;     // n % 4 == 0: 16-byte loads
+};
+
+template <class T, int KT>
+struct LineRegs {
+  T v[2][KT];
+  T tail[2];  // x[n-1]
+};
+
+template <class T, class S, int KT>
+__global__ void __launch_bounds__(256) k_vhgw_rows(RPass a, VPtrs<T> ptr) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr bool kSum = std::is_same_v<S, T>;
+  constexpr int32_t kSpan = 64 * KT;
+  constexpr int32_t kSlots = kSpan + kSpan / 32 + 1;  // pidx(kSpan - 1) + 1, and the junk slot
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int W = blockDim.x >> 6;
+  S* Hb = reinterpret_cast<S*>(smem) + wave * (kSum ? 3 : 2) * kSlots;
+  S* Gb = Hb + kSlots;
+  T* Vb = reinterpret_cast<T*>(Gb + kSlots);  // the box sum's staged line
+  const int32_t n = a.n;
+  const int64_t B = a.g.batch;
+  const int32_t i0 = lane * KT;
+  const auto window = [&](int64_t it) { return ptr.win[it / a.lines]; };
+  const auto load_field = [&](const T* src, T(&v)[KT], T& tail) {
+    if constexpr (!kSum) tail = src[n - 1];
+    if constexpr (KT >= 4) {
+      if (a.vec) {
+        using V4 = std::conditional_t<sizeof(T) == 4, float4, double4>;
+#pragma unroll
+        for (int c = 0; c < KT / 4; ++c) {
+          V4 q{};
+          if (i0 + 4 * c < n) q = *reinterpret_cast<const V4*>(src + i0 + 4 * c);
+          v[4 * c] = q.x;
+          v[4 * c + 1] = q.y;
+          v[4 * c + 2] = q.z;
+          v[4 * c + 3] = q.w;
+        }
+        return;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < KT; ++k) v[k] = i0 + k < n ? src[i0 + k] : T(0);
+  };
+  const auto load = [&](int64_t it, LineRegs<T, KT>& R) {
+    const int64_t rb = it / a.lines;
+    const int64_t line = it - rb * a.lines;
+    const int64_t rung = rb / B;
+    const int64_t b = rb - rung * B;
+    const int64_t off = rung * ptr.in_stride + b * a.g.count + line * n;
+    load_field(ptr.in[0] + off, R.v[0], R.tail[0]);
+    if (a.two_in) load_field(ptr.in[1] + off, R.v[1], R.tail[1]);
+  };
+  const int64_t nw = static_cast<int64_t>(gridDim.x) * W;
+  // the windows run two items ahead of the samples, so that a load never waits for its window
+  int64_t it = static_cast<int64_t>(blockIdx.x) * W + wave;
+  int32_t w_cur = it < a.items ? window(it) : 0;
+  int32_t w_nxt = it + nw < a.items ? window(it + nw) : 0;
+  LineRegs<T, KT> cur{};
+  if (w_cur >= 1) load(it, cur);
+  while (it < a.items) {
+    const int64_t it2 = it + nw;
+    const int32_t w_far = it2 + nw < a.items ? window(it2 + nw) : 0;
+    LineRegs<T, KT> nxt{};
+    if (w_nxt >= 1) load(it2, nxt);
+    if (w_cur >= 1) {
+      const int64_t rb = it / a.lines;
+      const int64_t line = it - rb * a.lines;
+      const int64_t rung = rb / B;
+      const int64_t b = rb - rung * B;
+      const int32_t w = w_cur;
+      const int32_t half = (w - 1) / 2;
+      const int32_t r = half < n - 1 ? half : n - 1;
+      T fac = T(1);
+      if (a.scale) {
+        double prod = 1.0;
+        for (int ax = 0; ax < a.g.dim; ++ax) prod *= static_cast<double>(2 * min(half, a.g.n[ax] - 1) + 1);
+        fac = static_cast<T>(1.0 / prod);
+      }
+      for (int f = 0; f < a.nf; ++f) {
+        T* dst = ptr.out[f] + rung * ptr.out_stride + b * a.g.count + line * n;
+        // the field's samples, selected without indexing the register arrays at run time
+        const bool second = a.two_in && f == 1;
+        T x[KT];
+#pragma unroll
+        for (int k = 0; k < KT; ++k) x[k] = second ? cur.v[1][k] : cur.v[0][k];
+        if constexpr (kSum) {
+#pragma unroll
+          for (int k = 0; k < KT; ++k)
+            if (i0 + k < n) Vb[pidx(i0) + k] = x[k];
+          wave_sync();
+          const WinInfo wi = win_info(n, w);
+          sum_line<T>(Vb, Hb, Gb, kSlots - 1, n, wi, [&](int32_t xo, T y) { dst[xo] = a.scale ? y * fac : y; });
+        } else {
+          // keys; samples past the line's end repeat x[n-1]
+          const T tail = second ? cur.tail[1] : cur.tail[0];
+          S v[KT];
+#pragma unroll
+          for (int k = 0; k < KT; ++k) v[k] = okey(i0 + k < n ? x[k] : tail);
+          const MmCtl c = mm_ctl<KT>(n, r);
+          const auto emit = [&](int32_t xo, S y) { dst[xo] = okey_inv(y); };
+          if (a.ops[f] == kVMax) {
+            mm_tile_h<kVMax, S, KT>(v, i0, mm_ident<kVMax, S>(), Hb, c);
+            mm_tile_g<kVMax, S, KT>(v, i0, mm_ident<kVMax, S>(), Gb, c);
+            wave_sync();
+            mm_out<kVMax, S>(Hb, Gb, c, emit);
+          } else {
+            mm_tile_h<kVMin, S, KT>(v, i0, mm_ident<kVMin, S>(), Hb, c);
+            mm_tile_g<kVMin, S, KT>(v, i0, mm_ident<kVMin, S>(), Gb, c);
+            wave_sync();
+            mm_out<kVMin, S>(Hb, Gb, c, emit);
+          }
+        }
+        wave_sync();
+      }
+    }
+    cur = nxt;
+    w_cur = w_nxt;
+    w_nxt = w_far;
+    it = it2;
+  }
+}
+
+#ifndef EDA_VHGW_ROWS
+#define EDA_VHGW_ROWS 16  // rows per workgroup on the contiguous axis
+#endif
+#ifndef EDA_VHGW_LDS
+#define EDA_VHGW_LDS 65536  // the LDS a workgroup aims at (bytes)
+#endif
+
+int kt_for(int32_t n) {
+  const int32_t per = (n + 63) / 64;
+  return per <= 1 ? 1 : per <= 2 ? 2 : per <= 4 ? 4 : per <= 8 ? 8 : 16;
+}
+
+// The launch shape of one pass: P lines, W waves, LDS slots per line, the tile span.
+struct Shape {
+  int32_t P, W, lpad, span, flat, wrows;
+  size_t lds;
+};
+Shape pass_shape(const VPass& a, size_t s, bool sum) {
+  const int32_t n = a.n;
+  const bool contiguous = a.inner == 1;
+  Shape sh{};
+  const int32_t tile = 64 * kt_for(n);
+  sh.span = sum ? n : (n + tile - 1) / tile * tile;
+  const int32_t need = pidx(sh.span - 1) + 1;
+  if (contiguous && (n & 31) == 0 && sh.span == n) {
+    sh.lpad = n + n / 32;  // rows in one run: pidx(p n + i) = p lpad + pidx(i)
+    sh.flat = 1;
+  } else {
+    sh.lpad = need | 1;
+    sh.flat = 0;
+  }
+  if (contiguous) {
+    sh.P = static_cast<int32_t>(std::min<int64_t>(a.outer, EDA_VHGW_ROWS));
+  } else {
+    int32_t P = static_cast<int32_t>(64 / s);  // 64-byte runs
+    while (P > 1 && P / 2 >= a.inner) P /= 2;
+    sh.P = P;
+  }
+  // max / min of a line that fits one tile keep H over the staged line: one LDS row per wave, and
+  // up to 8 waves; otherwise two rows and 4 waves
+  sh.wrows = !sum && sh.span == tile ? 1 : 2;
+  sh.W = std::min(sh.wrows == 1 ? 8 : 4, sh.P);
+  const auto bytes = [&](int32_t P, int32_t W) {
+    return (static_cast<size_t>(P) * sh.lpad + static_cast<size_t>(sh.wrows * W) * (sh.lpad + 1)) * s;
+  };
+  while (bytes(sh.P, sh.W) > EDA_VHGW_LDS && sh.P > 1) {
+    sh.P /= 2;
+    sh.W = std::min(sh.W, sh.P);
+  }
+  sh.lds = bytes(sh.P, sh.W);
+  if (sh.lds > 160 * 1024) throw std::invalid_argument("eda: the line is too long for the cube filter");
+  return sh;
+}
+
+template <class T, class S>
+void launch_kt(int kt, dim3 grid, dim3 block, size_t lds, hipStream_t s, const VPass& a, const VPtrs<T>& p) {
+  const auto go = [&](auto k) {
+    if (lds > 65536)
+      EDA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  static_cast<int>(lds)));
+    hipLaunchKernelGGL(k, grid, block, lds, s, a, p);
+  };
+  if constexpr (std::is_same_v<S, T>) {
+    go(k_vhgw<T, S, 1>);
+  } else {
+    switch (kt) {
+      case 1: go(k_vhgw<T, S, 1>); break;
+      case 2: go(k_vhgw<T, S, 2>); break;
+      case 4: go(k_vhgw<T, S, 4>); break;
+      case 8: go(k_vhgw<T, S, 8>); break;
+      default: go(k_vhgw<T, S, 16>); break;
+    }
+  }
+}
+
+template <class T, class S, int KT>
+void go_rows(hipStream_t s, int64_t items, const RPass& r, const VPtrs<T>& p) {
+  constexpr bool sum = std::is_same_v<S, T>;
+  constexpr size_t slots = 64 * KT + 2 * KT + 1;
+  const size_t per_wave = (sum ? 3 : 2) * slots * sizeof(S);
+  int W = 4;
+  while (W > 1 && W * per_wave > 64 * 1024) W /= 2;
+  const size_t lds = W * per_wave;
+  const int64_t wgs = std::max<int64_t>(1, std::min<int64_t>((items + W - 1) / W, 8192 / W));
+  auto k = k_vhgw_rows<T, S, KT>;
+  if (lds > 65536)
+    EDA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                static_cast<int>(lds)));
+  hipLaunchKernelGGL(k, dim3(static_cast<unsigned>(wgs)), dim3(64 * W), lds, s, r, p);
+}
+
+// The streaming pass serves a contiguous axis with n <= 2048.
+bool rows_qualify(const Geom& g, int axis) {
+  const bool contiguous = axis == g.dim - 1 || g.st[axis] == 1;
+  return contiguous && g.n[axis] <= 2048;
+}
 template <class T>
-size_t lds_bytes(int32_t n, int P, int32_t tmax) {
-  const size_t s = sizeof(T);
-  return (2 * static_cast<size_t>(P) * line_stride(n) + static_cast<size_t>(P) * tmax) * s;
+void launch_rows(hipStream_t s, const Geom& g, int axis, int nf, bool same_input, const int* ops, bool scale,
+                 const VPtrs<T>& p, int32_t rungs) {
+  const int32_t n = g.n[axis];
+  RPass r{};
+  r.g = g;
+  r.n = n;
+  r.lines = g.count / n;
+  r.items = static_cast<int64_t>(rungs) * g.batch * r.lines;
+  r.nf = nf;
+  r.two_in = nf == 2 && !same_input ? 1 : 0;
+  r.ops[0] = ops[0];
+  r.ops[1] = nf > 1 ? ops[1] : ops[0];
+  r.scale = scale ? 1 : 0;
+  const auto al16 = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; };
+  r.vec = n % 4 == 0 && al16(p.in[0]) && al16(p.in[1]) && (p.in_stride * sizeof(T)) % 16 == 0 ? 1 : 0;
+  const int32_t per = (n + 63) / 64;
+  const int kt = per <= 1 ? 1 : per <= 2 ? 2 : per <= 4 ? 4 : per <= 8 ? 8 : per <= 16 ? 16 : 32;
+  if (ops[0] == kVSum) {
+    switch (kt) {
+      case 1: go_rows<T, T, 1>(s, r.items, r, p); break;
+      case 2: go_rows<T, T, 2>(s, r.items, r, p); break;
+      case 4: go_rows<T, T, 4>(s, r.items, r, p); break;
+      case 8: go_rows<T, T, 8>(s, r.items, r, p); break;
+      case 16: go_rows<T, T, 16>(s, r.items, r, p); break;
+      default: go_rows<T, T, 32>(s, r.items, r, p); break;
+    }
+  } else {
+    using S = okey_t<T>;
+    switch (kt) {
+      case 1: go_rows<T, S, 1>(s, r.items, r, p); break;
+      case 2: go_rows<T, S, 2>(s, r.items, r, p); break;
+      case 4: go_rows<T, S, 4>(s, r.items, r, p); break;
+      case 8: go_rows<T, S, 8>(s, r.items, r, p); break;
+      case 16: go_rows<T, S, 16>(s, r.items, r, p); break;
+      default: go_rows<T, S, 32>(s, r.items, r, p); break;
+    }
+  }
 }
 
 }  // namespace
-
-// The chunk-task slots per line for the windows of a launch.
-int32_t vhgw_task_slots(int32_t n, int32_t max_w) {
-  // tasks(w) is not monotone in w: take the largest over every odd w up to max_w, by its closed form
-  int32_t best = 1;
-  const int32_t wmax = std::min<int32_t>(max_w, 2 * n - 1);
-  for (int32_t w = 33; w <= wmax; w += 2) best = std::max(best, win_info(n, w).tasks);
-  return best;
-}
 
 template <class T>
 void enqueue_vhgw(hipStream_t s, const Geom& g, int axis, int nf, bool same_input, const int* ops, bool scale,
@@ -312,30 +899,6 @@ void enqueue_vhgw(hipStream_t s, const Geom& g, int axis, int nf, bool same_inpu
                   int64_t in_stride, int64_t out_stride) {
   if (max_w < 1 || rungs < 1) return;
   if (rungs > 65535) throw std::invalid_argument("eda: too many rungs for one launch");
-  VPass a{};
-  a.g = g;
-  a.axis = axis;
-  a.n = g.n[axis];
-  a.step = g.st[axis];
-  a.inner = axis == g.dim - 1 ? 1 : g.st[axis];
-  a.outer = g.count / (static_cast<int64_t>(a.n) * a.inner);  // rows (contiguous axis) or column groups
-  const int32_t n = a.n;
-  int P;
-  if (a.inner == 1) {
-    P = std::max(1, std::min(16, 4096 / n));
-    a.groups = (a.outer + P - 1) / P;
-  } else {
-    P = static_cast<int>(std::min<int64_t>(a.inner, std::max(1, std::min(8, 8192 / n))));
-    const int64_t gpo = (a.inner + P - 1) / P;
-    a.groups = a.outer * gpo;
-  }
-  a.P = P;
-  a.tmax = vhgw_task_slots(n, max_w);
-  a.nf = nf;
-  a.same_input = same_input ? 1 : 0;
-  a.ops[0] = ops[0];
-  a.ops[1] = nf > 1 ? ops[1] : ops[0];
-  a.scale = scale ? 1 : 0;
   VPtrs<T> p{};
   p.in[0] = in[0];
   p.in[1] = nf > 1 ? in[1] : in[0];
@@ -344,23 +907,54 @@ void enqueue_vhgw(hipStream_t s, const Geom& g, int axis, int nf, bool same_inpu
   p.win = win;
   p.in_stride = in_stride;
   p.out_stride = out_stride;
-  const int64_t wgs = g.batch * a.groups;
   const bool sum = ops[0] == kVSum;
-  const size_t lds = lds_bytes<T>(n, P, a.tmax);
   hipEvent_t t0 = nullptr;
   const bool timed = timer_on();
-  if (timed) timer_note(s, &t0);
-  const dim3 grid(static_cast<unsigned>(wgs), static_cast<unsigned>(rungs));
-  if (sum) {
-    auto k = k_vhgw<T, T>;
-    if (lds > 65536) EDA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-    hipLaunchKernelGGL(k, grid, dim3(kThreads), lds, s, a, p);
+  if (rows_qualify(g, axis)) {
+    if (timed) timer_note(s, &t0);
+    launch_rows<T>(s, g, axis, nf, same_input, ops, scale, p, rungs);
+    EDA_LAUNCH("k_vhgw_rows");
   } else {
-    auto k = k_vhgw<T, okey_t<T>>;
-    if (lds > 65536) EDA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-    hipLaunchKernelGGL(k, grid, dim3(kThreads), lds, s, a, p);
+    VPass a{};
+    a.g = g;
+    a.n = g.n[axis];
+    a.step = g.st[axis];
+    a.inner = axis == g.dim - 1 ? 1 : g.st[axis];
+    a.outer = g.count / (static_cast<int64_t>(a.n) * a.inner);
+    const Shape sh = pass_shape(a, sizeof(T), sum);
+    a.P = sh.P;
+    a.lpad = sh.lpad;
+    a.span = sh.span;
+    a.flat = sh.flat;
+    a.wrows = sh.wrows;
+    if (a.inner == 1) {
+      a.groups = (a.outer + a.P - 1) / a.P;
+    } else {
+      const int64_t gpo = (a.inner + a.P - 1) / a.P;
+      a.groups = a.outer * gpo;
+    }
+    a.nf = nf;
+    a.ops[0] = ops[0];
+    a.ops[1] = nf > 1 ? ops[1] : ops[0];
+    a.scale = scale ? 1 : 0;
+    const int64_t wgs = g.batch * a.groups;
+    if (wgs > 0x7fffffff) throw std::invalid_argument("eda: too many workgroups for one pass");
+    a.xcd = (wgs & 7) == 0 ? 1 : 0;
+    {
+      const auto al16 = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; };
+      constexpr int64_t VW = 16 / static_cast<int64_t>(sizeof(T));
+      a.vec = a.inner > 1 && a.P % VW == 0 && a.inner % VW == 0 && al16(p.in[0]) && al16(p.in[1]) &&
+                      al16(p.out[0]) && al16(p.out[1]) && (p.in_stride % VW) == 0 && (p.out_stride % VW) == 0
+                  ? 1
+                  : 0;
+    }
+    if (timed) timer_note(s, &t0);
+    const dim3 grid(static_cast<unsigned>(wgs), static_cast<unsigned>(rungs));
+    const dim3 block(static_cast<unsigned>(64 * sh.W));
+    if (sum) launch_kt<T, T>(1, grid, block, sh.lds, s, a, p);
+    else launch_kt<T, okey_t<T>>(kt_for(a.n), grid, block, sh.lds, s, a, p);
+    EDA_LAUNCH("k_vhgw");
   }
-  EDA_LAUNCH("k_vhgw");
   if (timed) {
     int64_t active = mirror_active(win, g.batch * rungs);
     if (active < 0) active = g.batch * rungs;
@@ -370,7 +964,7 @@ void enqueue_vhgw(hipStream_t s, const Geom& g, int axis, int nf, bool same_inpu
 }
 
 // The cube (reference cpp/src/method/common.cpp:65-136): box runs axis 0 first, the last pass
-// scaled; max / min run the last axis first; maxmin stages the input once for both outputs. With
+// scaled; max / min run the last axis first; maxmin reads the input for both outputs. With
 // rungs > 1 every pass runs all rungs in one launch: rung j filters in with the windows win + j B
 // into out0 + j stride (out1 + j stride).
 template <class T>
