@@ -19,15 +19,18 @@
 // local); the suffix fold H is G of the reversed line; y(x) = H(x-r) + G(x+r), times 0.5 when x+r
 // ends a segment (the window is the segment).
 //
-// gfx950 mapping. A workgroup of W wave64s stages P lines in LDS with coalesced global accesses (P
-// consecutive rows of the contiguous axis, or P adjacent columns of a strided axis: 64-byte runs),
-// one pad word per 32 samples. Each wave then owns one line at a time and works in registers:
-// max/min as per-lane runs of KT consecutive samples, block resets from a bitmask, and a segmented
-// scan across the lanes (DPP row shifts inside 16-lane rows, readlane across rows, only as deep as
-// a block is long); the box sum as one chunk per lane, folded in registers, with the chunk carries
-// passed lane to lane by DPP wave shifts. G and H go to the wave's own LDS rows, the output back
-// over the staged line, which the workgroup writes out coalesced. Workgroup ids are remapped so
-// that consecutive groups of one element run on one XCD and share its L2.
+// gfx950 mapping. Max / min keep each lane's run of KT consecutive samples in registers, with block
+// resets from a bitmask and a segmented scan across the lanes (DPP row shifts inside 16-lane rows,
+// readlane across rows, only as deep as a block is long); G and H meet in LDS for the output.
+// Along the contiguous axis (n <= 2048) this is a streaming kernel, one wave per line, loading the
+// next line while it folds the current one (k_vhgw_rows). Otherwise (strided axes, long lines, and
+// every box sum) a workgroup of W wave64s stages P lines in LDS with coalesced global accesses (P
+// consecutive rows, or P adjacent columns: 16-byte loads over 64-byte runs), one pad word per 32
+// samples, and writes the outputs back the same way (k_vhgw). The box sum runs one 32-sample chunk
+// per lane, with the chunk tasks of all the lines a wave owns packed into its 64-lane rounds and the
+// chunk carries passed lane to lane by DPP wave shifts; the G pass adds H and writes y in place.
+// Workgroup ids are remapped so that consecutive groups of one element run on one XCD and share its
+// L2.
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -320,29 +323,40 @@ __host__ __device__ __forceinline__ WinInfo win_info(int32_t n, int32_t w) {
   return v;
 }
 
-// DIR 0: G of the line into R at slot x = e - r; DIR 1: G of the reversed line (H) into R at slot
-// x = e + r of the forward position e. One chunk per lane, CH >= the longest chunk. Samples past a
-// chunk's end add -0, which leaves every fold's bits unchanged; stores that belong to no output go
-// to the junk slot.
+// The box sums of the lines one wave owns, their chunk tasks packed into 64-lane rounds (a chain of
+// carries never leaves the wave). Line li < nl of the wave is staged at V + p lpad, p = p0 + li
+// pstep, its output row at R + p lpad. DIR 1 writes H (the G of the reversed line) at slot x = e + r
+// of the forward position e; DIR 0 computes G at e = x + r and writes y = H + G over it, halved
+// where e ends a segment. One chunk per lane, CH >= the longest chunk; samples past a chunk's end
+// add -0, which leaves every fold's bits unchanged; lanes with no output touch the junk slot.
 template <int DIR, int CH, class T>
-__device__ void sum_dir(const T* __restrict__ V, T* __restrict__ R, int32_t junk, int32_t n, const WinInfo& wi) {
+__device__ void sum_dir(const T* __restrict__ V, T* __restrict__ R, int32_t lpad, int32_t junk, int32_t p0,
+                        int32_t pstep, int32_t nl, int32_t n, const WinInfo& wi) {
   const int lane = threadIdx.x & 63;
   const int32_t r = wi.r, wa = wi.wa;
-  const int32_t total = wi.nsegs * wi.cs;
+  const int32_t per = wi.nsegs * wi.cs;  // tasks per line
+  const int32_t total = nl * per;
   const float inv_cs = 1.0f / static_cast<float>(wi.cs);
+  const float inv_per = 1.0f / static_cast<float>(per);
   const int iters = wi.cs - 1 < 64 ? wi.cs - 1 : 64;
   const T nz = -T(0);
   const int32_t e_max = n - 1 + r;
   T carry = nz;
   for (int32_t t0 = 0; t0 < total; t0 += 64) {
-    const int32_t t = t0 + lane;
+    const int32_t u = t0 + lane;
+    const int32_t li = min(div_small(u, per, inv_per), nl - 1);
+    const int32_t t = u - li * per;
     const int32_t kk = div_small(t, wi.cs, inv_cs);
     const int32_t q = t - kk * wi.cs;
     const int32_t s = wi.o + (wi.k_lo + kk) * wa;
     const int32_t e_last = min(s + wa - 1, e_max);
+    const int32_t e_al = s + wa - 1;  // the segment's end
     const int32_t c0 = s + 32 * q;
-    const int32_t len = t < total ? min(32, e_last - c0 + 1) : 0;
+    const int32_t len = u < total ? min(32, e_last - c0 + 1) : 0;
     const int32_t c1 = min(c0 + (len > 0 ? len - 1 : 0), e_max);  // the last sample read
+    const int32_t p = p0 + li * pstep;
+    const T* Vl = V + p * lpad;
+    T* Rl = R + p * lpad;
     T v[CH];
     T acc = nz;
 #pragma unroll
@@ -352,7 +366,7 @@ __device__ void sum_dir(const T* __restrict__ V, T* __restrict__ R, int32_t junk
       const int32_t ef = e ^ (e >> 31);
       int32_t i = min(ef, 2 * n - 1 - ef);
       if (DIR) i = n - 1 - i;
-      const T xv = V[pidx(i)];
+      const T xv = Vl[pidx(i)];
       const T add = j < len ? xv : nz;
       acc = j == 0 ? add : acc + add;
       v[j] = acc;
@@ -373,38 +387,36 @@ __device__ void sum_dir(const T* __restrict__ V, T* __restrict__ R, int32_t junk
       const int32_t e = c0 + j;
       int32_t x = e - r;
       if (DIR) x = n - 1 - x;
-      const int32_t slot = j < len && e >= r ? pidx(x) : junk;
-      R[slot] = q > 0 ? cin + v[j] : v[j];
+      const bool valid = j < len && e >= r;
+      T* at = valid ? Rl + pidx(x) : R + junk;
+      const T gv = q > 0 ? cin + v[j] : v[j];
+      if constexpr (DIR == 1) {
+        *at = gv;
+      } else {
+        const T sum = *at + gv;
+        *at = e == e_al ? sum * T(0.5) : sum;
+      }
     }
   }
 }
 
-// The box sum of the line staged in V (one wave): emit(x, y) for the lane's x = lane + 64 j < n.
-template <int CH, class T, class Emit>
-__device__ void sum_line_ch(const T* __restrict__ V, T* __restrict__ Hb, T* __restrict__ Gb, int32_t junk, int32_t n,
-                            const WinInfo& wi, Emit emit) {
-  sum_dir<1, CH, T>(V, Hb, junk, n, wi);
-  sum_dir<0, CH, T>(V, Gb, junk, n, wi);
+template <int CH, class T>
+__device__ void sum_lines_ch(const T* V, T* R, int32_t lpad, int32_t junk, int32_t p0, int32_t pstep, int32_t nl,
+                             int32_t n, const WinInfo& wi) {
+  sum_dir<1, CH, T>(V, R, lpad, junk, p0, pstep, nl, n, wi);
   wave_sync();
-  const int lane = threadIdx.x & 63;
-  const int32_t wa = wi.wa;
-  const int32_t d = 64 % wa;
-  int32_t m = static_cast<int32_t>(floor_mod(static_cast<int64_t>(lane) + wi.r - wi.o, wa));
-  for (int32_t x = lane; x < n; x += 64) {
-    const T sum = Hb[pidx(x)] + Gb[pidx(x)];
-    emit(x, m == wa - 1 ? sum * T(0.5) : sum);
-    m += d;
-    m -= m >= wa ? wa : 0;
-  }
+  sum_dir<0, CH, T>(V, R, lpad, junk, p0, pstep, nl, n, wi);
+  wave_sync();
 }
 
-template <class T, class Emit>
-__device__ void sum_line(const T* V, T* Hb, T* Gb, int32_t junk, int32_t n, const WinInfo& wi, Emit emit) {
+template <class T>
+__device__ void sum_lines(const T* V, T* R, int32_t lpad, int32_t junk, int32_t p0, int32_t pstep, int32_t nl,
+                          int32_t n, const WinInfo& wi) {
   const int32_t longest = wi.wa < 32 ? wi.wa : 32;
-  if (longest <= 4) sum_line_ch<4, T>(V, Hb, Gb, junk, n, wi, emit);
-  else if (longest <= 8) sum_line_ch<8, T>(V, Hb, Gb, junk, n, wi, emit);
-  else if (longest <= 16) sum_line_ch<16, T>(V, Hb, Gb, junk, n, wi, emit);
-  else sum_line_ch<32, T>(V, Hb, Gb, junk, n, wi, emit);
+  if (longest <= 4) sum_lines_ch<4, T>(V, R, lpad, junk, p0, pstep, nl, n, wi);
+  else if (longest <= 8) sum_lines_ch<8, T>(V, R, lpad, junk, p0, pstep, nl, n, wi);
+  else if (longest <= 16) sum_lines_ch<16, T>(V, R, lpad, junk, p0, pstep, nl, n, wi);
+  else sum_lines_ch<32, T>(V, R, lpad, junk, p0, pstep, nl, n, wi);
 }
 
 // ---- the pass kernel ----------------------------------------------------------------------------------------
@@ -457,11 +469,15 @@ __global__ void __launch_bounds__(512) k_vhgw(VPass a, VPtrs<T> ptr) {
   const int W = nthr >> 6;
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  // V: P staged lines; per wave wrows rows of lpad + 1 slots (the last one: the junk slot), H then
-  // G (one row: G only, the line's H goes over its staged samples)
+  // V: P staged lines. Max / min: per wave wrows rows of lpad + 1 slots, H then G (one row: G only,
+  // the line's H goes over its staged samples); the outputs over V. Box sum: P output rows R after
+  // V and one junk slot.
+  constexpr bool kSum = std::is_same_v<S, T>;
   S* V = reinterpret_cast<S*>(smem);
   S* Hb = V + a.P * a.lpad + wave * a.wrows * (a.lpad + 1);
   S* Gb = a.wrows == 2 ? Hb + a.lpad + 1 : Hb;
+  S* R = V + a.P * a.lpad;
+  S* Y = kSum ? R : V;  // the rows the write-out reads
   const bool contiguous = a.inner == 1;
   int64_t base0;
   int32_t lines;
@@ -529,13 +545,14 @@ __global__ void __launch_bounds__(512) k_vhgw(VPass a, VPtrs<T> ptr) {
           put(p * a.lpad + pidx(i), src[base0 + static_cast<int64_t>(i) * a.step + p]);
     }
     __syncthreads();
-    for (int p = wave; p < lines; p += W) {
-      S* Vl = V + p * a.lpad;
-      if constexpr (std::is_same_v<S, T>) {
-        const WinInfo wi = win_info(n, w);
-        sum_line<T>(Vl, Hb, Gb, a.lpad, n, wi, [&](int32_t x, T y) { Vl[pidx(x)] = y; });
-        wave_sync();
-      } else {
+    if constexpr (kSum) {
+      // wave w owns lines w, w + W, ...
+      const WinInfo wi = win_info(n, w);
+      const int32_t nl = wave < lines ? (lines - wave + W - 1) / W : 0;
+      sum_lines<T>(V, R, a.lpad, a.P * a.lpad, wave, W, nl, n, wi);
+    } else {
+      for (int p = wave; p < lines; p += W) {
+        S* Vl = V + p * a.lpad;
         // the tiles read up to the span: repeat x[n-1] there
         const S tail = Vl[pidx(n - 1)];
         for (int32_t i = n + lane; i < a.span; i += 64) Vl[pidx(i)] = tail;
@@ -576,8 +593,8 @@ __global__ void __launch_bounds__(512) k_vhgw(VPass a, VPtrs<T> ptr) {
     T* dst = ptr.out[f] + rung * ptr.out_stride;
     const auto get = [&](int32_t slot) {
       T y;
-      if constexpr (std::is_same_v<S, T>) y = V[slot];
-      else y = okey_inv(V[slot]);
+      if constexpr (kSum) y = Y[slot];
+      else y = okey_inv(Y[slot]);
       return a.scale ? y * fac : y;
     };
     if (a.flat) {
@@ -619,9 +636,9 @@ __global__ void __launch_bounds__(512) k_vhgw(VPass a, VPtrs<T> ptr) {
 // ---- the streaming pass of the contiguous axis (lines of at most 64 * 32 samples) ------------------------
 // One wave per line, grid-stride over the (rung, element, line) items; each lane holds KT consecutive
 // samples of the line in registers, loaded with 16-byte loads where the line allows, and the next
-// item's samples are loaded while the current one is folded. Max / min fold the registers directly
-// (G and H through the wave's two LDS rows); the box sum stages the line in a third row for its
-// chunk walks. Outputs go straight to global memory, lanes on consecutive samples.
+// item's samples are loaded while the current one is folded. Max / min only: the registers fold
+// directly (G and H through the wave's two LDS rows), and the outputs go straight to global memory,
+// lanes on consecutive samples.
 struct RPass {
   Geom g;
   int32_t n;       // line length
@@ -630,7 +647,6 @@ struct RPass {
   int32_t nf;      // output fields: 1, or 2 (ops[0] into out[0], ops[1] into out[1])
   int32_t two_in;  // nf == 2 with two inputs (else field 1 folds the input of field 0)
   int32_t ops[2];
-  int32_t scale;   // multiply by fl_T(1 / prod of the clamped sides) (the last box pass)
   int32_t vec;     // n % 4 == 0: 16-byte loads
 };
 
@@ -642,22 +658,21 @@ struct LineRegs {
 
 template <class T, class S, int KT>
 __global__ void __launch_bounds__(256) k_vhgw_rows(RPass a, VPtrs<T> ptr) {
+  static_assert(!std::is_same_v<S, T>, "max / min only");
   extern __shared__ __align__(16) unsigned char smem[];
-  constexpr bool kSum = std::is_same_v<S, T>;
   constexpr int32_t kSpan = 64 * KT;
-  constexpr int32_t kSlots = kSpan + kSpan / 32 + 1;  // pidx(kSpan - 1) + 1, and the junk slot
+  constexpr int32_t kSlots = kSpan + kSpan / 32;  // pidx(kSpan - 1) + 1
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int W = blockDim.x >> 6;
-  S* Hb = reinterpret_cast<S*>(smem) + wave * (kSum ? 3 : 2) * kSlots;
+  S* Hb = reinterpret_cast<S*>(smem) + wave * 2 * kSlots;
   S* Gb = Hb + kSlots;
-  T* Vb = reinterpret_cast<T*>(Gb + kSlots);  // the box sum's staged line
   const int32_t n = a.n;
   const int64_t B = a.g.batch;
   const int32_t i0 = lane * KT;
   const auto window = [&](int64_t it) { return ptr.win[it / a.lines]; };
   const auto load_field = [&](const T* src, T(&v)[KT], T& tail) {
-    if constexpr (!kSum) tail = src[n - 1];
+    tail = src[n - 1];
     if constexpr (KT >= 4) {
       if (a.vec) {
         using V4 = std::conditional_t<sizeof(T) == 4, float4, double4>;
@@ -702,48 +717,29 @@ __global__ void __launch_bounds__(256) k_vhgw_rows(RPass a, VPtrs<T> ptr) {
       const int64_t line = it - rb * a.lines;
       const int64_t rung = rb / B;
       const int64_t b = rb - rung * B;
-      const int32_t w = w_cur;
-      const int32_t half = (w - 1) / 2;
+      const int32_t half = (w_cur - 1) / 2;
       const int32_t r = half < n - 1 ? half : n - 1;
-      T fac = T(1);
-      if (a.scale) {
-        double prod = 1.0;
-        for (int ax = 0; ax < a.g.dim; ++ax) prod *= static_cast<double>(2 * min(half, a.g.n[ax] - 1) + 1);
-        fac = static_cast<T>(1.0 / prod);
-      }
       for (int f = 0; f < a.nf; ++f) {
         T* dst = ptr.out[f] + rung * ptr.out_stride + b * a.g.count + line * n;
-        // the field's samples, selected without indexing the register arrays at run time
+        // the field's keys, selected without indexing the register arrays at run time; samples past
+        // the line's end repeat x[n-1]
         const bool second = a.two_in && f == 1;
-        T x[KT];
+        const T tail = second ? cur.tail[1] : cur.tail[0];
+        S v[KT];
 #pragma unroll
-        for (int k = 0; k < KT; ++k) x[k] = second ? cur.v[1][k] : cur.v[0][k];
-        if constexpr (kSum) {
-#pragma unroll
-          for (int k = 0; k < KT; ++k)
-            if (i0 + k < n) Vb[pidx(i0) + k] = x[k];
+        for (int k = 0; k < KT; ++k) v[k] = okey(i0 + k < n ? (second ? cur.v[1][k] : cur.v[0][k]) : tail);
+        const MmCtl c = mm_ctl<KT>(n, r);
+        const auto emit = [&](int32_t xo, S y) { dst[xo] = okey_inv(y); };
+        if (a.ops[f] == kVMax) {
+          mm_tile_h<kVMax, S, KT>(v, i0, mm_ident<kVMax, S>(), Hb, c);
+          mm_tile_g<kVMax, S, KT>(v, i0, mm_ident<kVMax, S>(), Gb, c);
           wave_sync();
-          const WinInfo wi = win_info(n, w);
-          sum_line<T>(Vb, Hb, Gb, kSlots - 1, n, wi, [&](int32_t xo, T y) { dst[xo] = a.scale ? y * fac : y; });
+          mm_out<kVMax, S>(Hb, Gb, c, emit);
         } else {
-          // keys; samples past the line's end repeat x[n-1]
-          const T tail = second ? cur.tail[1] : cur.tail[0];
-          S v[KT];
-#pragma unroll
-          for (int k = 0; k < KT; ++k) v[k] = okey(i0 + k < n ? x[k] : tail);
-          const MmCtl c = mm_ctl<KT>(n, r);
-          const auto emit = [&](int32_t xo, S y) { dst[xo] = okey_inv(y); };
-          if (a.ops[f] == kVMax) {
-            mm_tile_h<kVMax, S, KT>(v, i0, mm_ident<kVMax, S>(), Hb, c);
-            mm_tile_g<kVMax, S, KT>(v, i0, mm_ident<kVMax, S>(), Gb, c);
-            wave_sync();
-            mm_out<kVMax, S>(Hb, Gb, c, emit);
-          } else {
-            mm_tile_h<kVMin, S, KT>(v, i0, mm_ident<kVMin, S>(), Hb, c);
-            mm_tile_g<kVMin, S, KT>(v, i0, mm_ident<kVMin, S>(), Gb, c);
-            wave_sync();
-            mm_out<kVMin, S>(Hb, Gb, c, emit);
-          }
+          mm_tile_h<kVMin, S, KT>(v, i0, mm_ident<kVMin, S>(), Hb, c);
+          mm_tile_g<kVMin, S, KT>(v, i0, mm_ident<kVMin, S>(), Gb, c);
+          wave_sync();
+          mm_out<kVMin, S>(Hb, Gb, c, emit);
         }
         wave_sync();
       }
@@ -757,6 +753,12 @@ __global__ void __launch_bounds__(256) k_vhgw_rows(RPass a, VPtrs<T> ptr) {
 
 #ifndef EDA_VHGW_ROWS
 #define EDA_VHGW_ROWS 16  // rows per workgroup on the contiguous axis
+#endif
+#ifndef EDA_VHGW_SUMW
+#define EDA_VHGW_SUMW 4  // waves per workgroup of the box sum
+#endif
+#ifndef EDA_VHGW_SUMLDS
+#define EDA_VHGW_SUMLDS 40000  // the LDS a box-sum workgroup aims at (bytes)
 #endif
 #ifndef EDA_VHGW_LDS
 #define EDA_VHGW_LDS 65536  // the LDS a workgroup aims at (bytes)
@@ -795,12 +797,15 @@ Shape pass_shape(const VPass& a, size_t s, bool sum) {
   }
   // max / min of a line that fits one tile keep H over the staged line: one LDS row per wave, and
   // up to 8 waves; otherwise two rows and 4 waves
-  sh.wrows = !sum && sh.span == tile ? 1 : 2;
-  sh.W = std::min(sh.wrows == 1 ? 8 : 4, sh.P);
+  // (the box sum: an output row per line and one junk slot)
+  sh.wrows = sum ? 0 : sh.span == tile ? 1 : 2;
+  sh.W = std::min(sum ? EDA_VHGW_SUMW : sh.wrows == 1 ? 8 : 4, sh.P);
   const auto bytes = [&](int32_t P, int32_t W) {
+    if (sum) return (2 * static_cast<size_t>(P) * sh.lpad + 1) * s;
     return (static_cast<size_t>(P) * sh.lpad + static_cast<size_t>(sh.wrows * W) * (sh.lpad + 1)) * s;
   };
-  while (bytes(sh.P, sh.W) > EDA_VHGW_LDS && sh.P > 1) {
+  const size_t budget = sum ? EDA_VHGW_SUMLDS : EDA_VHGW_LDS;
+  while (bytes(sh.P, sh.W) > budget && sh.P > 1) {
     sh.P /= 2;
     sh.W = std::min(sh.W, sh.P);
   }
@@ -832,9 +837,8 @@ void launch_kt(int kt, dim3 grid, dim3 block, size_t lds, hipStream_t s, const V
 
 template <class T, class S, int KT>
 void go_rows(hipStream_t s, int64_t items, const RPass& r, const VPtrs<T>& p) {
-  constexpr bool sum = std::is_same_v<S, T>;
-  constexpr size_t slots = 64 * KT + 2 * KT + 1;
-  const size_t per_wave = (sum ? 3 : 2) * slots * sizeof(S);
+  constexpr size_t slots = 64 * KT + 2 * KT;
+  const size_t per_wave = 2 * slots * sizeof(S);
   int W = 4;
   while (W > 1 && W * per_wave > 64 * 1024) W /= 2;
   const size_t lds = W * per_wave;
@@ -846,14 +850,14 @@ void go_rows(hipStream_t s, int64_t items, const RPass& r, const VPtrs<T>& p) {
   hipLaunchKernelGGL(k, dim3(static_cast<unsigned>(wgs)), dim3(64 * W), lds, s, r, p);
 }
 
-// The streaming pass serves a contiguous axis with n <= 2048.
-bool rows_qualify(const Geom& g, int axis) {
+// The streaming pass serves max / min along a contiguous axis with n <= 2048.
+bool rows_qualify(const Geom& g, int axis, bool sum) {
   const bool contiguous = axis == g.dim - 1 || g.st[axis] == 1;
-  return contiguous && g.n[axis] <= 2048;
+  return !sum && contiguous && g.n[axis] <= 2048;
 }
 template <class T>
-void launch_rows(hipStream_t s, const Geom& g, int axis, int nf, bool same_input, const int* ops, bool scale,
-                 const VPtrs<T>& p, int32_t rungs) {
+void launch_rows(hipStream_t s, const Geom& g, int axis, int nf, bool same_input, const int* ops, const VPtrs<T>& p,
+                 int32_t rungs) {
   const int32_t n = g.n[axis];
   RPass r{};
   r.g = g;
@@ -864,30 +868,18 @@ void launch_rows(hipStream_t s, const Geom& g, int axis, int nf, bool same_input
   r.two_in = nf == 2 && !same_input ? 1 : 0;
   r.ops[0] = ops[0];
   r.ops[1] = nf > 1 ? ops[1] : ops[0];
-  r.scale = scale ? 1 : 0;
   const auto al16 = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; };
   r.vec = n % 4 == 0 && al16(p.in[0]) && al16(p.in[1]) && (p.in_stride * sizeof(T)) % 16 == 0 ? 1 : 0;
   const int32_t per = (n + 63) / 64;
   const int kt = per <= 1 ? 1 : per <= 2 ? 2 : per <= 4 ? 4 : per <= 8 ? 8 : per <= 16 ? 16 : 32;
-  if (ops[0] == kVSum) {
-    switch (kt) {
-      case 1: go_rows<T, T, 1>(s, r.items, r, p); break;
-      case 2: go_rows<T, T, 2>(s, r.items, r, p); break;
-      case 4: go_rows<T, T, 4>(s, r.items, r, p); break;
-      case 8: go_rows<T, T, 8>(s, r.items, r, p); break;
-      case 16: go_rows<T, T, 16>(s, r.items, r, p); break;
-      default: go_rows<T, T, 32>(s, r.items, r, p); break;
-    }
-  } else {
-    using S = okey_t<T>;
-    switch (kt) {
-      case 1: go_rows<T, S, 1>(s, r.items, r, p); break;
-      case 2: go_rows<T, S, 2>(s, r.items, r, p); break;
-      case 4: go_rows<T, S, 4>(s, r.items, r, p); break;
-      case 8: go_rows<T, S, 8>(s, r.items, r, p); break;
-      case 16: go_rows<T, S, 16>(s, r.items, r, p); break;
-      default: go_rows<T, S, 32>(s, r.items, r, p); break;
-    }
+  using S = okey_t<T>;
+  switch (kt) {
+    case 1: go_rows<T, S, 1>(s, r.items, r, p); break;
+    case 2: go_rows<T, S, 2>(s, r.items, r, p); break;
+    case 4: go_rows<T, S, 4>(s, r.items, r, p); break;
+    case 8: go_rows<T, S, 8>(s, r.items, r, p); break;
+    case 16: go_rows<T, S, 16>(s, r.items, r, p); break;
+    default: go_rows<T, S, 32>(s, r.items, r, p); break;
   }
 }
 
@@ -910,9 +902,9 @@ void enqueue_vhgw(hipStream_t s, const Geom& g, int axis, int nf, bool same_inpu
   const bool sum = ops[0] == kVSum;
   hipEvent_t t0 = nullptr;
   const bool timed = timer_on();
-  if (rows_qualify(g, axis)) {
+  if (rows_qualify(g, axis, sum)) {
     if (timed) timer_note(s, &t0);
-    launch_rows<T>(s, g, axis, nf, same_input, ops, scale, p, rungs);
+    launch_rows<T>(s, g, axis, nf, same_input, ops, p, rungs);
     EDA_LAUNCH("k_vhgw_rows");
   } else {
     VPass a{};
