@@ -144,72 +144,73 @@ __global__ void __launch_bounds__(kThreads) k_weights(Geom g, T q, T span_m1, co
   }
 }
 
-// D-linear interpolation at y of the half-sample reflected periodic extension of f (one element).
-template <class T, int D>
-__device__ __forceinline__ T interp(const Geom& g, const T* __restrict__ f, const double* y) {
-  int64_t off0[D], off1[D];
-  T w0[D], w1[D];
-#pragma unroll
-  for (int a = 0; a < D; ++a) {
-    const double fl = floor(y[a]);
-    const double tt = y[a] - fl;
-    const int64_t i0 = static_cast<int64_t>(fl);
-    const T t1 = static_cast<T>(tt);
-    w1[a] = t1;
-    w0[a] = T(1) - t1;
-    off0[a] = reflect_any(i0, g.n[a]) * g.st[a];
-    off1[a] = reflect_any(i0 + 1, g.n[a]) * g.st[a];
-  }
-  T acc = T(0);
-#pragma unroll
-  for (int corner = 0; corner < (1 << D); ++corner) {
-    int64_t off = 0;
-#pragma unroll
-    for (int a = 0; a < D; ++a) off += ((corner >> (D - 1 - a)) & 1) ? off1[a] : off0[a];
-    T term = f[off];
-#pragma unroll
-    for (int a = 0; a < D; ++a) term = term * (((corner >> (D - 1 - a)) & 1) ? w1[a] : w0[a]);
-    acc = acc + term;
-  }
-  return acc;
+// Index i of the half-sample reflected periodic extension of an axis of n samples, in 32 bits (an
+// element holds at most 2^31 samples): i itself inside the axis, else through i mod 2n.
+__device__ __forceinline__ int32_t reflect32(int32_t i, int32_t n) {
+  if (static_cast<uint32_t>(i) < static_cast<uint32_t>(n)) return i;
+  const int32_t p = 2 * n;
+  int32_t m = i % p;
+  m += m < 0 ? p : 0;
+  return m < n ? m : p - 1 - m;
 }
+
+// D-linear interpolation at y of the half-sample reflected periodic extension of a field of one
+// element: the corner offsets and weights once (setup), then any field with that layout (at).
+template <class T, int D>
+struct Interp {
+  int32_t off0[D], off1[D];
+  T w0[D], w1[D];
+  __device__ __forceinline__ void setup(const int32_t* n, const int32_t* st, const double* y) {
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      const double fl = floor(y[a]);
+      const double tt = y[a] - fl;
+      // |y| stays far below 2^30 for finite fields; the clamp keeps any other input in range
+      const int32_t i0 = static_cast<int32_t>(fmin(fmax(fl, -1.0e9), 1.0e9));
+      const T t1 = static_cast<T>(tt);
+      w1[a] = t1;
+      w0[a] = T(1) - t1;
+      off0[a] = reflect32(i0, n[a]) * st[a];
+      off1[a] = reflect32(i0 + 1, n[a]) * st[a];
+    }
+  }
+  __device__ __forceinline__ T at(const T* __restrict__ f) const {
+    T acc = T(0);
+#pragma unroll
+    for (int corner = 0; corner < (1 << D); ++corner) {
+      int32_t off = 0;
+#pragma unroll
+      for (int a = 0; a < D; ++a) off += ((corner >> (D - 1 - a)) & 1) ? off1[a] : off0[a];
+      T term = f[off];
+#pragma unroll
+      for (int a = 0; a < D; ++a) term = term * (((corner >> (D - 1 - a)) & 1) ? w1[a] : w0[a]);
+      acc = acc + term;
+    }
+    return acc;
+  }
+};
 
 // The gather of every rung in one pass (OP 0 max, 2 max + min, 3 box): each voxel reads its lower
 // rung lo, its weight t and (when one of its two rungs carries segment points) o once, and the cubes
 // of its two rungs from the rung fields r0 (r1) + j F. The arithmetic is the per-rung one in rung
 // order: the lower rung's weighted value stores (wl = 1 - t > 0), the upper rung's adds (t > 0).
+// One rung's value at voxel i: f0[i] (f1[i]) folded with the 4 segment points when pts; the points
+// in the order (1/2 o, -1/2 o, o, -o).
 template <class T, int D, int OP>
-__device__ __forceinline__ void rung_acc(const Geom& g, int64_t i, int64_t at, int64_t F, const T* __restrict__ f0,
-                                         const T* __restrict__ f1, bool pts, const T* __restrict__ o, T* a0, T* a1) {
+__device__ __forceinline__ void rung_acc(int32_t i, const T* __restrict__ f0, const T* __restrict__ f1, bool pts,
+                                         const Interp<T, D> (&ip)[4], T* a0, T* a1) {
   T acc0 = f0[i];
   T acc1 = OP == 2 ? f1[i] : T(0);
   if (pts) {
-    int32_t x[kMaxDim];
 #pragma unroll
-    for (int a = 0; a < D; ++a) x[a] = static_cast<int32_t>((i / g.st[a]) % g.n[a]);
-    double ov[D];
-#pragma unroll
-    for (int a = 0; a < D; ++a) ov[a] = static_cast<double>(o[a * F + at]);
-#pragma unroll
-    for (int k = 1; k <= 2; ++k) {
-      const double sc = static_cast<double>(k) / 2.0;
-#pragma unroll
-      for (int sg = 0; sg < 2; ++sg) {
-        double y[D];
-#pragma unroll
-        for (int a = 0; a < D; ++a) {
-          const double off = sc * ov[a];
-          const double xa = static_cast<double>(x[a]);
-          y[a] = sg == 0 ? xa + off : xa - off;
-        }
-        if constexpr (OP == 2) {
-          acc0 = max_c(acc0, interp<T, D>(g, f0, y));
-          acc1 = min_c(acc1, interp<T, D>(g, f1, y));
-        } else if constexpr (OP == 0) {
-          acc0 = max_c(acc0, interp<T, D>(g, f0, y));
-        } else {
-          acc0 = acc0 + interp<T, D>(g, f0, y);
-        }
+    for (int q = 0; q < 4; ++q) {
+      if constexpr (OP == 2) {
+        acc0 = max_c(acc0, ip[q].at(f0));
+        acc1 = min_c(acc1, ip[q].at(f1));
+      } else if constexpr (OP == 0) {
+        acc0 = max_c(acc0, ip[q].at(f0));
+      } else {
+        acc0 = acc0 + ip[q].at(f0);
       }
     }
     if constexpr (OP == 3) acc0 = acc0 / T(5);
@@ -229,27 +230,71 @@ __global__ void __launch_bounds__(kThreads) k_gather_all(Geom g, const ElemDev* 
   const uint32_t pts = D >= 2 ? el[b].pts : 0u;
   const int64_t base = b * g.count;
   const int64_t F = g.batch * g.count;
-  for (int64_t i = blockIdx.x * static_cast<int64_t>(kThreads) + threadIdx.x; i < g.count;
-       i += static_cast<int64_t>(gridDim.x) * kThreads) {
+  const int32_t count = static_cast<int32_t>(g.count);  // at most 2^31 - 1 in this kernel's indices
+  int32_t nn[D], st[D];
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    nn[a] = g.n[a];
+    st[a] = static_cast<int32_t>(g.st[a]);
+  }
+  for (int64_t i64 = blockIdx.x * static_cast<int64_t>(kThreads) + threadIdx.x; i64 < count;
+       i64 += static_cast<int64_t>(gridDim.x) * kThreads) {
+    const int32_t i = static_cast<int32_t>(i64);
     const int64_t at = base + i;
     const int l = lo[at];
     const T tt = t[at];
     const T wl = T(1) - tt;
+    const bool use_lo = wl > T(0), use_up = tt > T(0);
+    const bool pts_lo = use_lo && ((pts >> l) & 1u) != 0u;
+    const bool pts_up = use_up && ((pts >> (l + 1)) & 1u) != 0u;
+    // the 4 segment points of the voxel (shared by its two rungs), coordinates in double
+    Interp<T, D> ip[4];
+    if (pts_lo || pts_up) {
+      int32_t x[D];
+      if constexpr (D == 1) {
+        x[0] = i;
+      } else if constexpr (D == 2) {
+        x[0] = static_cast<int32_t>(static_cast<uint32_t>(i) / static_cast<uint32_t>(nn[1]));
+        x[1] = i - x[0] * nn[1];
+      } else {
+        x[0] = static_cast<int32_t>(static_cast<uint32_t>(i) / static_cast<uint32_t>(st[0]));
+        const int32_t rem = i - x[0] * st[0];
+        x[1] = static_cast<int32_t>(static_cast<uint32_t>(rem) / static_cast<uint32_t>(nn[2]));
+        x[2] = rem - x[1] * nn[2];
+      }
+      double ov[D];
+#pragma unroll
+      for (int a = 0; a < D; ++a) ov[a] = static_cast<double>(o[a * F + at]);
+#pragma unroll
+      for (int k = 1; k <= 2; ++k) {
+        const double sc = static_cast<double>(k) / 2.0;
+#pragma unroll
+        for (int sg = 0; sg < 2; ++sg) {
+          double y[D];
+#pragma unroll
+          for (int a = 0; a < D; ++a) {
+            const double off = sc * ov[a];
+            const double xa = static_cast<double>(x[a]);
+            y[a] = sg == 0 ? xa + off : xa - off;
+          }
+          ip[2 * (k - 1) + sg].setup(nn, st, y);
+        }
+      }
+    }
     T v0 = T(0), v1 = T(0);
-    if (wl > T(0)) {
+    if (use_lo) {
       T a0, a1;
-      rung_acc<T, D, OP>(g, i, at, F, r0 + l * F + base, OP == 2 ? r1 + l * F + base : nullptr,
-                         ((pts >> l) & 1u) != 0u, o, &a0, &a1);
+      rung_acc<T, D, OP>(i, r0 + l * F + base, OP == 2 ? r1 + l * F + base : nullptr, pts_lo, ip, &a0, &a1);
       v0 = wl * a0;
       v1 = wl * a1;
     }
-    if (tt > T(0)) {
+    if (use_up) {
       T a0, a1;
-      rung_acc<T, D, OP>(g, i, at, F, r0 + (l + 1) * F + base, OP == 2 ? r1 + (l + 1) * F + base : nullptr,
-                         ((pts >> (l + 1)) & 1u) != 0u, o, &a0, &a1);
+      rung_acc<T, D, OP>(i, r0 + (l + 1) * F + base, OP == 2 ? r1 + (l + 1) * F + base : nullptr, pts_up, ip, &a0,
+                         &a1);
       const T u0 = tt * a0, u1 = tt * a1;
-      v0 = wl > T(0) ? v0 + u0 : u0;
-      v1 = wl > T(0) ? v1 + u1 : u1;
+      v0 = use_lo ? v0 + u0 : u0;
+      v1 = use_lo ? v1 + u1 : u1;
     }
     out0[at] = v0;
     if constexpr (OP == 2) out1[at] = v1;
