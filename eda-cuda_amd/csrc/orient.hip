@@ -29,8 +29,8 @@ namespace {
 
 // ---- the batched GEMM -----------------------------------------------------------------------------
 // C[bt](i, j) = sum_k A[bt](i, k) B[bt](k, j) with element strides; batch bt runs when
-// on[bt / per_elem] != 0. 64 x 64 tiles, K steps of 16 through LDS, 4 x 4 outputs per thread,
-// fused multiply-adds in T.
+// on[bt / per_elem] != 0. The float64 kernel: 64 x 64 tiles, K steps of 16 through LDS, 4 x 4
+// outputs per thread, fused multiply-adds.
 struct GemmDims {
   int64_t M, N, K;
   int64_t sAb, sAi, sAk, sBb, sBk, sBj, sCb, sCi, sCj;
@@ -116,12 +116,134 @@ __global__ void __launch_bounds__(kThreads) k_gemm(GemmDims d, const T* __restri
   }
 }
 
+// The float32 GEMM on the matrix cores: v_mfma_f32_16x16x4f32 (lane l supplies A(l % 16, l / 16)
+// and B(l / 16, l % 16) and holds D(4 (l / 16) + v, l % 16), v < 4). A workgroup of 4 waves owns a
+// 64 x 64 tile of C, each wave 32 x 32 (2 x 2 MFMA tiles); K runs in steps of 32 through LDS
+// (k-major rows of 64 + 16 slots, so that the two k rows a half-wave reads fall in distinct banks).
+// Every output sums its K terms in one fixed order, whatever the batch or the grid.
+constexpr int kMK = 32, kMStride = kTM + 16;
+typedef float mf4 __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(kThreads) k_gemm_mfma(GemmDims d, const float* __restrict__ A,
+                                                        const float* __restrict__ Bm, float* __restrict__ C,
+                                                        const int32_t* __restrict__ on) {
+  __shared__ __align__(16) float As[kMK][kMStride];
+  __shared__ __align__(16) float Bs[kMK][kMStride];
+  const int64_t bt = blockIdx.z;
+  if (on != nullptr && on[bt / d.per_elem] == 0) return;
+  const int64_t i0 = static_cast<int64_t>(blockIdx.y) * kTM;
+  const int64_t j0 = static_cast<int64_t>(blockIdx.x) * kTN;
+  const float* Ab = A + bt * d.sAb;
+  const float* Bb = Bm + bt * d.sBb;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
+  mf4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) acc[a][c] = mf4{0.f, 0.f, 0.f, 0.f};
+  // 16-byte loads when every tile is full and every 4-run along the contiguous index is aligned
+  const bool vec = d.M % kTM == 0 && d.N % kTN == 0 && d.K % kMK == 0 &&
+                   (reinterpret_cast<uintptr_t>(Ab) & 15) == 0 && (reinterpret_cast<uintptr_t>(Bb) & 15) == 0 &&
+                   (d.sAk == 1 ? d.sAi % 4 == 0 : d.sAi == 1 && d.sAk % 4 == 0) &&
+                   (d.sBj == 1 ? d.sBk % 4 == 0 : d.sBk == 1 && d.sBj % 4 == 0);
+  for (int64_t k0 = 0; k0 < d.K; k0 += kMK) {
+    if (vec) {
+      // A: 4-runs along k (rows of A) or along i; B: along j or along k
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        const int e = tid + r * kThreads;  // 512 runs of 4 cover the 64 x 32 tile
+        if (d.sAk == 1) {
+          const int ii = e >> 3, kk = (e & 7) * 4;
+          const float4 q = *reinterpret_cast<const float4*>(Ab + (i0 + ii) * d.sAi + (k0 + kk));
+          As[kk][ii] = q.x;
+          As[kk + 1][ii] = q.y;
+          As[kk + 2][ii] = q.z;
+          As[kk + 3][ii] = q.w;
+        } else {
+          const int kk = e >> 4, ii = (e & 15) * 4;
+          *reinterpret_cast<float4*>(&As[kk][ii]) = *reinterpret_cast<const float4*>(Ab + (i0 + ii) + (k0 + kk) * d.sAk);
+        }
+        if (d.sBj == 1) {
+          const int kk = e >> 4, jj = (e & 15) * 4;
+          *reinterpret_cast<float4*>(&Bs[kk][jj]) = *reinterpret_cast<const float4*>(Bb + (k0 + kk) * d.sBk + (j0 + jj));
+        } else {
+          const int jj = e >> 3, kk = (e & 7) * 4;
+          const float4 q = *reinterpret_cast<const float4*>(Bb + (k0 + kk) + (j0 + jj) * d.sBj);
+          Bs[kk][jj] = q.x;
+          Bs[kk + 1][jj] = q.y;
+          Bs[kk + 2][jj] = q.z;
+          Bs[kk + 3][jj] = q.w;
+        }
+      }
+    } else {
+      // consecutive threads walk the faster-varying index of each operand's layout
+      for (int e = tid; e < kTM * kMK; e += kThreads) {
+        int ii, kk;
+        if (d.sAk == 1) {
+          kk = e % kMK;
+          ii = e / kMK;
+        } else {
+          ii = e % kTM;
+          kk = e / kTM;
+        }
+        const int64_t gi = i0 + ii, gk = k0 + kk;
+        As[kk][ii] = (gi < d.M && gk < d.K) ? Ab[gi * d.sAi + gk * d.sAk] : 0.f;
+      }
+      for (int e = tid; e < kTN * kMK; e += kThreads) {
+        int jj, kk;
+        if (d.sBj == 1) {
+          jj = e % kTN;
+          kk = e / kTN;
+        } else {
+          kk = e % kMK;
+          jj = e / kMK;
+        }
+        const int64_t gj = j0 + jj, gk = k0 + kk;
+        Bs[kk][jj] = (gj < d.N && gk < d.K) ? Bb[gk * d.sBk + gj * d.sBj] : 0.f;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < kMK; ks += 4) {
+      float av[2], bv[2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) av[a] = As[ks + lk][wm + 16 * a + lr];
+#pragma unroll
+      for (int c = 0; c < 2; ++c) bv[c] = Bs[ks + lk][wn + 16 * c + lr];
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[a], bv[c], acc[a][c], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  float* Cb = C + bt * d.sCb;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t gi = i0 + wm + 16 * a + 4 * lk + v;
+        const int64_t gj = j0 + wn + 16 * c + lr;
+        if (gi < d.M && gj < d.N) Cb[gi * d.sCi + gj * d.sCj] = acc[a][c][v];
+      }
+}
+
 template <class T>
 void gemm(hipStream_t s, const GemmDims& d, const T* A, const T* B, T* C, const int32_t* on) {
   const dim3 grid(static_cast<unsigned>((d.N + kTN - 1) / kTN), static_cast<unsigned>((d.M + kTM - 1) / kTM),
                   static_cast<unsigned>(d.batches));
-  hipLaunchKernelGGL(k_gemm<T>, grid, dim3(kThreads), 0, s, d, A, B, C, on);
-  EDA_LAUNCH("k_gemm");
+  if constexpr (std::is_same_v<T, float>) {
+    hipLaunchKernelGGL(k_gemm_mfma, grid, dim3(kThreads), 0, s, d, A, B, C, on);
+    EDA_LAUNCH("k_gemm_mfma");
+  } else {
+    hipLaunchKernelGGL(k_gemm<T>, grid, dim3(kThreads), 0, s, d, A, B, C, on);
+    EDA_LAUNCH("k_gemm");
+  }
 }
 
 // One axis of a separable transform: out = M applied along axis a of every running element
