@@ -324,14 +324,18 @@ __host__ __device__ __forceinline__ WinInfo win_info(int32_t n, int32_t w) {
 }
 
 // The box sums of the lines one wave owns, their chunk tasks packed into 64-lane rounds (a chain of
-// carries never leaves the wave). Line li < nl of the wave is staged at V + p lpad, p = p0 + li
-// pstep, its output row at R + p lpad. DIR 1 writes H (the G of the reversed line) at slot x = e + r
-// of the forward position e; DIR 0 computes G at e = x + r and writes y = H + G over it, halved
-// where e ends a segment. One chunk per lane, CH >= the longest chunk; samples past a chunk's end
-// add -0, which leaves every fold's bits unchanged; lanes with no output touch the junk slot.
+// carries never leaves the wave). Line li < nl of the wave is staged reflected at E + p epad,
+// p = p0 + li pstep (the extended position e at index e + lead, with at least 32 slots of slack at
+// both ends), its output row at R + p lpad. DIR 1 writes H (the G of the reversed line) at slot
+// x = e + r of the forward position e; DIR 0 computes G at e = x + r and writes y = H + G over it,
+// halved where e ends a segment. One chunk per lane, CH >= the longest chunk: its samples are
+// consecutive in the extended row, so each one's slot is a base, the sample's index as an
+// immediate offset, and one more pad word past the chunk's 32-sample boundary. Samples past a
+// chunk's end add -0, which leaves every fold's bits unchanged; lanes with no output touch the junk
+// slot.
 template <int DIR, int CH, class T>
-__device__ void sum_dir(const T* __restrict__ V, T* __restrict__ R, int32_t lpad, int32_t junk, int32_t p0,
-                        int32_t pstep, int32_t nl, int32_t n, const WinInfo& wi) {
+__device__ void sum_dir(const T* __restrict__ E, int32_t epad, int32_t lead, T* __restrict__ R, int32_t lpad,
+                        int32_t junk, int32_t p0, int32_t pstep, int32_t nl, int32_t n, const WinInfo& wi) {
   const int lane = threadIdx.x & 63;
   const int32_t r = wi.r, wa = wi.wa;
   const int32_t per = wi.nsegs * wi.cs;  // tasks per line
@@ -353,20 +357,19 @@ __device__ void sum_dir(const T* __restrict__ V, T* __restrict__ R, int32_t lpad
     const int32_t e_al = s + wa - 1;  // the segment's end
     const int32_t c0 = s + 32 * q;
     const int32_t len = u < total ? min(32, e_last - c0 + 1) : 0;
-    const int32_t c1 = min(c0 + (len > 0 ? len - 1 : 0), e_max);  // the last sample read
+    const int32_t ca = min(c0, e_max);  // the chunk's anchor (lanes without a chunk read in range)
     const int32_t p = p0 + li * pstep;
-    const T* Vl = V + p * lpad;
-    T* Rl = R + p * lpad;
+    // sample j: extended index ib + j (DIR 0) or ib - j (DIR 1)
+    const int32_t ib = DIR ? n - 1 - ca + lead : ca + lead;
+    const T* row0 = E + p * epad + pidx(ib);
+    const T* row1 = DIR ? row0 - 1 : row0 + 1;
+    const int32_t jb = DIR ? (ib & 31) : 32 - (ib & 31);
     T v[CH];
     T acc = nz;
 #pragma unroll
     for (int j = 0; j < CH; ++j) {
-      const int32_t e = min(c0 + j, c1);
-      // half-sample reflection of e in [-r, n + r) (r <= n - 1): ~e below 0, 2n - 1 - e above n - 1
-      const int32_t ef = e ^ (e >> 31);
-      int32_t i = min(ef, 2 * n - 1 - ef);
-      if (DIR) i = n - 1 - i;
-      const T xv = Vl[pidx(i)];
+      const T* rp = (DIR ? j > jb : j >= jb) ? row1 : row0;
+      const T xv = DIR ? rp[-j] : rp[j];
       const T add = j < len ? xv : nz;
       acc = j == 0 ? add : acc + add;
       v[j] = acc;
@@ -382,41 +385,46 @@ __device__ void sum_dir(const T* __restrict__ V, T* __restrict__ R, int32_t lpad
       }
     }
     carry = readlane(out, 63);
+    // output slot x of sample j: xb - j (DIR 1) or xb + j (DIR 0), valid when the sample is in its
+    // chunk and e = c0 + j >= r
+    const int32_t xb = DIR ? n - 1 + r - ca : ca - r;
+    T* out0 = R + p * lpad + xb + (xb >> 5);
+    T* out1 = DIR ? out0 - 1 : out0 + 1;
+    const int32_t xm = DIR ? (xb & 31) : 32 - (xb & 31);
+    const int32_t jmin = r - c0;  // the first j with an output
 #pragma unroll
     for (int j = 0; j < CH; ++j) {
-      const int32_t e = c0 + j;
-      int32_t x = e - r;
-      if (DIR) x = n - 1 - x;
-      const bool valid = j < len && e >= r;
-      T* at = valid ? Rl + pidx(x) : R + junk;
+      const bool valid = j < len && j >= jmin;
+      T* slot = (DIR ? j > xm : j >= xm) ? out1 : out0;
+      T* at = valid ? (DIR ? slot - j : slot + j) : R + junk;
       const T gv = q > 0 ? cin + v[j] : v[j];
       if constexpr (DIR == 1) {
         *at = gv;
       } else {
         const T sum = *at + gv;
-        *at = e == e_al ? sum * T(0.5) : sum;
+        *at = c0 + j == e_al ? sum * T(0.5) : sum;
       }
     }
   }
 }
 
 template <int CH, class T>
-__device__ void sum_lines_ch(const T* V, T* R, int32_t lpad, int32_t junk, int32_t p0, int32_t pstep, int32_t nl,
-                             int32_t n, const WinInfo& wi) {
-  sum_dir<1, CH, T>(V, R, lpad, junk, p0, pstep, nl, n, wi);
+__device__ void sum_lines_ch(const T* E, int32_t epad, int32_t lead, T* R, int32_t lpad, int32_t junk, int32_t p0,
+                             int32_t pstep, int32_t nl, int32_t n, const WinInfo& wi) {
+  sum_dir<1, CH, T>(E, epad, lead, R, lpad, junk, p0, pstep, nl, n, wi);
   wave_sync();
-  sum_dir<0, CH, T>(V, R, lpad, junk, p0, pstep, nl, n, wi);
+  sum_dir<0, CH, T>(E, epad, lead, R, lpad, junk, p0, pstep, nl, n, wi);
   wave_sync();
 }
 
 template <class T>
-__device__ void sum_lines(const T* V, T* R, int32_t lpad, int32_t junk, int32_t p0, int32_t pstep, int32_t nl,
-                          int32_t n, const WinInfo& wi) {
+__device__ void sum_lines(const T* E, int32_t epad, int32_t lead, T* R, int32_t lpad, int32_t junk, int32_t p0,
+                          int32_t pstep, int32_t nl, int32_t n, const WinInfo& wi) {
   const int32_t longest = wi.wa < 32 ? wi.wa : 32;
-  if (longest <= 4) sum_lines_ch<4, T>(V, R, lpad, junk, p0, pstep, nl, n, wi);
-  else if (longest <= 8) sum_lines_ch<8, T>(V, R, lpad, junk, p0, pstep, nl, n, wi);
-  else if (longest <= 16) sum_lines_ch<16, T>(V, R, lpad, junk, p0, pstep, nl, n, wi);
-  else sum_lines_ch<32, T>(V, R, lpad, junk, p0, pstep, nl, n, wi);
+  if (longest <= 4) sum_lines_ch<4, T>(E, epad, lead, R, lpad, junk, p0, pstep, nl, n, wi);
+  else if (longest <= 8) sum_lines_ch<8, T>(E, epad, lead, R, lpad, junk, p0, pstep, nl, n, wi);
+  else if (longest <= 16) sum_lines_ch<16, T>(E, epad, lead, R, lpad, junk, p0, pstep, nl, n, wi);
+  else sum_lines_ch<32, T>(E, epad, lead, R, lpad, junk, p0, pstep, nl, n, wi);
 }
 
 // ---- the pass kernel ----------------------------------------------------------------------------------------
@@ -437,6 +445,8 @@ struct VPass {
   int32_t xcd;      // remap workgroup ids: consecutive groups on one XCD
   int32_t vec;      // strided axis: 16-byte global accesses over 16 / sizeof(T) adjacent columns
   int32_t wrows;    // LDS rows per wave: 1 (max / min of one tile: H in place), 2 (H and G)
+  int32_t epad;     // box sum: LDS slots per reflected line (extended position e at e + lead)
+  int32_t lead;     // 32 + the largest radius of the launch
 };
 
 // gridDim.y passes (rungs) in one launch: rung j reads win + j B, in + j in_stride and writes
@@ -469,14 +479,14 @@ __global__ void __launch_bounds__(512) k_vhgw(VPass a, VPtrs<T> ptr) {
   const int W = nthr >> 6;
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  // V: P staged lines. Max / min: per wave wrows rows of lpad + 1 slots, H then G (one row: G only,
-  // the line's H goes over its staged samples); the outputs over V. Box sum: P output rows R after
-  // V and one junk slot.
+  // Max / min: V = P staged lines of lpad slots; per wave wrows rows of lpad + 1 slots, H then G
+  // (one row: G only, the line's H goes over its staged samples); the outputs over V. Box sum: P
+  // reflected lines of epad slots, then P output rows R of lpad slots and one junk slot.
   constexpr bool kSum = std::is_same_v<S, T>;
   S* V = reinterpret_cast<S*>(smem);
   S* Hb = V + a.P * a.lpad + wave * a.wrows * (a.lpad + 1);
   S* Gb = a.wrows == 2 ? Hb + a.lpad + 1 : Hb;
-  S* R = V + a.P * a.lpad;
+  S* R = V + a.P * a.epad;
   S* Y = kSum ? R : V;  // the rows the write-out reads
   const bool contiguous = a.inner == 1;
   int64_t base0;
@@ -506,19 +516,48 @@ __global__ void __launch_bounds__(512) k_vhgw(VPass a, VPtrs<T> ptr) {
   }
   const int32_t half = (w - 1) / 2;
   const int32_t r = half < n - 1 ? half : n - 1;
+  if (kSum && r > a.lead - 32) return;  // a window above the launch's largest (the host prevents it)
+  const int32_t rmax = a.lead - 32;
   for (int f = 0; f < a.nf; ++f) {
     const T* src = ptr.in[f] + rung * ptr.in_stride;
-    const auto put = [&](int32_t slot, T v) {
-      if constexpr (std::is_same_v<S, T>) V[slot] = v;
-      else V[slot] = okey(v);
+    // sample i of line p; the box sum also writes its reflections at e = -1 - i (i < rmax) and
+    // e = 2n - 1 - i (i >= n - rmax)
+    const auto put = [&](int32_t p, int32_t i, T v) {
+      if constexpr (kSum) {
+        S* El = V + p * a.epad;
+        El[pidx(i + a.lead)] = v;
+        if (i < rmax) El[pidx(a.lead - 1 - i)] = v;
+        if (i >= n - rmax) El[pidx(2 * n - 1 - i + a.lead)] = v;
+      } else {
+        V[p * a.lpad + pidx(i)] = okey(v);
+      }
     };
     if (a.flat) {
-      for (int32_t idx = threadIdx.x; idx < total; idx += nthr) put(pidx(idx), src[base0 + idx]);
+      // max / min rows in one run: pidx(p n + i) = p lpad + pidx(i)
+      if constexpr (!kSum)
+        for (int32_t idx = threadIdx.x; idx < total; idx += nthr) V[pidx(idx)] = okey(src[base0 + idx]);
+    } else if (contiguous && a.vec) {
+      // 16-byte loads of VW consecutive samples of one line (n is a multiple of VW)
+      constexpr int VW = 16 / static_cast<int>(sizeof(T));
+      using V16 = std::conditional_t<sizeof(T) == 4, float4, double2>;
+      for (int32_t idx = threadIdx.x * VW; idx < total; idx += nthr * VW) {
+        const int32_t p = div_small(idx, n, inv_n);
+        const int32_t i = idx - p * n;
+        const V16 q = *reinterpret_cast<const V16*>(src + base0 + idx);
+        if constexpr (VW == 4) {
+          put(p, i, q.x);
+          put(p, i + 1, q.y);
+          put(p, i + 2, q.z);
+          put(p, i + 3, q.w);
+        } else {
+          put(p, i, q.x);
+          put(p, i + 1, q.y);
+        }
+      }
     } else if (contiguous) {
       for (int32_t idx = threadIdx.x; idx < total; idx += nthr) {
         const int32_t p = div_small(idx, n, inv_n);
-        const int32_t i = idx - p * n;
-        put(p * a.lpad + pidx(i), src[base0 + idx]);
+        put(p, idx - p * n, src[base0 + idx]);
       }
     } else if (a.vec && lines == a.P) {
       // 16-byte loads: tpr threads per row, each VW adjacent columns
@@ -529,27 +568,27 @@ __global__ void __launch_bounds__(512) k_vhgw(VPass a, VPtrs<T> ptr) {
       for (int32_t i = threadIdx.x / tpr; i < n; i += nthr / tpr) {
         const V16 q = *reinterpret_cast<const V16*>(src + base0 + static_cast<int64_t>(i) * a.step + cg * VW);
         if constexpr (VW == 4) {
-          put((cg * 4 + 0) * a.lpad + pidx(i), q.x);
-          put((cg * 4 + 1) * a.lpad + pidx(i), q.y);
-          put((cg * 4 + 2) * a.lpad + pidx(i), q.z);
-          put((cg * 4 + 3) * a.lpad + pidx(i), q.w);
+          put(cg * 4 + 0, i, q.x);
+          put(cg * 4 + 1, i, q.y);
+          put(cg * 4 + 2, i, q.z);
+          put(cg * 4 + 3, i, q.w);
         } else {
-          put((cg * 2 + 0) * a.lpad + pidx(i), q.x);
-          put((cg * 2 + 1) * a.lpad + pidx(i), q.y);
+          put(cg * 2 + 0, i, q.x);
+          put(cg * 2 + 1, i, q.y);
         }
       }
     } else {
       const int32_t p = threadIdx.x & (a.P - 1);
       if (p < lines)
         for (int32_t i = threadIdx.x >> pshift; i < n; i += nthr >> pshift)
-          put(p * a.lpad + pidx(i), src[base0 + static_cast<int64_t>(i) * a.step + p]);
+          put(p, i, src[base0 + static_cast<int64_t>(i) * a.step + p]);
     }
     __syncthreads();
     if constexpr (kSum) {
       // wave w owns lines w, w + W, ...
       const WinInfo wi = win_info(n, w);
       const int32_t nl = wave < lines ? (lines - wave + W - 1) / W : 0;
-      sum_lines<T>(V, R, a.lpad, a.P * a.lpad, wave, W, nl, n, wi);
+      sum_lines<T>(V, a.epad, a.lead, R, a.lpad, a.P * a.lpad, wave, W, nl, n, wi);
     } else {
       for (int p = wave; p < lines; p += W) {
         S* Vl = V + p * a.lpad;
@@ -599,6 +638,24 @@ __global__ void __launch_bounds__(512) k_vhgw(VPass a, VPtrs<T> ptr) {
     };
     if (a.flat) {
       for (int32_t idx = threadIdx.x; idx < total; idx += nthr) dst[base0 + idx] = get(pidx(idx));
+    } else if (contiguous && a.vec) {
+      constexpr int VW = 16 / static_cast<int>(sizeof(T));
+      using V16 = std::conditional_t<sizeof(T) == 4, float4, double2>;
+      for (int32_t idx = threadIdx.x * VW; idx < total; idx += nthr * VW) {
+        const int32_t p = div_small(idx, n, inv_n);
+        const int32_t sl = p * a.lpad + pidx(idx - p * n);  // VW samples in one 32-block: consecutive slots
+        V16 q;
+        if constexpr (VW == 4) {
+          q.x = get(sl);
+          q.y = get(sl + 1);
+          q.z = get(sl + 2);
+          q.w = get(sl + 3);
+        } else {
+          q.x = get(sl);
+          q.y = get(sl + 1);
+        }
+        *reinterpret_cast<V16*>(dst + base0 + idx) = q;
+      }
     } else if (contiguous) {
       for (int32_t idx = threadIdx.x; idx < total; idx += nthr) {
         const int32_t p = div_small(idx, n, inv_n);
@@ -771,17 +828,22 @@ int kt_for(int32_t n) {
 
 // The launch shape of one pass: P lines, W waves, LDS slots per line, the tile span.
 struct Shape {
-  int32_t P, W, lpad, span, flat, wrows;
+  int32_t P, W, lpad, span, flat, wrows, epad, lead;
   size_t lds;
 };
-Shape pass_shape(const VPass& a, size_t s, bool sum) {
+Shape pass_shape(const VPass& a, size_t s, bool sum, int32_t max_w) {
   const int32_t n = a.n;
   const bool contiguous = a.inner == 1;
   Shape sh{};
   const int32_t tile = 64 * kt_for(n);
   sh.span = sum ? n : (n + tile - 1) / tile * tile;
   const int32_t need = pidx(sh.span - 1) + 1;
-  if (contiguous && (n & 31) == 0 && sh.span == n) {
+  // the box sum's reflected lines: extended positions [-rmax, n + rmax) at e + lead, and 32 slots
+  // of slack past both ends for the chunk walks
+  const int32_t rmax = std::min((max_w - 1) / 2, n - 1);
+  sh.lead = 32 + rmax;
+  sh.epad = sum ? (pidx(n + 2 * rmax + 64) + 1) | 1 : 0;
+  if (!sum && contiguous && (n & 31) == 0 && sh.span == n) {
     sh.lpad = n + n / 32;  // rows in one run: pidx(p n + i) = p lpad + pidx(i)
     sh.flat = 1;
   } else {
@@ -801,7 +863,7 @@ Shape pass_shape(const VPass& a, size_t s, bool sum) {
   sh.wrows = sum ? 0 : sh.span == tile ? 1 : 2;
   sh.W = std::min(sum ? EDA_VHGW_SUMW : sh.wrows == 1 ? 8 : 4, sh.P);
   const auto bytes = [&](int32_t P, int32_t W) {
-    if (sum) return (2 * static_cast<size_t>(P) * sh.lpad + 1) * s;
+    if (sum) return (static_cast<size_t>(P) * (sh.epad + sh.lpad) + 1) * s;
     return (static_cast<size_t>(P) * sh.lpad + static_cast<size_t>(sh.wrows * W) * (sh.lpad + 1)) * s;
   };
   const size_t budget = sum ? EDA_VHGW_SUMLDS : EDA_VHGW_LDS;
@@ -913,12 +975,14 @@ void enqueue_vhgw(hipStream_t s, const Geom& g, int axis, int nf, bool same_inpu
     a.step = g.st[axis];
     a.inner = axis == g.dim - 1 ? 1 : g.st[axis];
     a.outer = g.count / (static_cast<int64_t>(a.n) * a.inner);
-    const Shape sh = pass_shape(a, sizeof(T), sum);
+    const Shape sh = pass_shape(a, sizeof(T), sum, max_w);
     a.P = sh.P;
     a.lpad = sh.lpad;
     a.span = sh.span;
     a.flat = sh.flat;
     a.wrows = sh.wrows;
+    a.epad = sh.epad;
+    a.lead = sh.lead;
     if (a.inner == 1) {
       a.groups = (a.outer + a.P - 1) / a.P;
     } else {
@@ -935,8 +999,10 @@ void enqueue_vhgw(hipStream_t s, const Geom& g, int axis, int nf, bool same_inpu
     {
       const auto al16 = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; };
       constexpr int64_t VW = 16 / static_cast<int64_t>(sizeof(T));
-      a.vec = a.inner > 1 && a.P % VW == 0 && a.inner % VW == 0 && al16(p.in[0]) && al16(p.in[1]) &&
-                      al16(p.out[0]) && al16(p.out[1]) && (p.in_stride % VW) == 0 && (p.out_stride % VW) == 0
+      // strided axis: P and the column count multiples of VW; contiguous axis: n a multiple of VW
+      const bool shape_ok = a.inner > 1 ? a.P % VW == 0 && a.inner % VW == 0 : a.n % VW == 0;
+      a.vec = shape_ok && al16(p.in[0]) && al16(p.in[1]) && al16(p.out[0]) && al16(p.out[1]) &&
+                      (p.in_stride % VW) == 0 && (p.out_stride % VW) == 0
                   ? 1
                   : 0;
     }
