@@ -1,10 +1,15 @@
 // SPDX-License-Identifier: Apache-2.0
 // Face-connected components (scipy.ndimage.label's default structure; the domain border does not
 // connect), by lock-free union-find (Playne and Hawick's scheme; reference
-// cpp/src/method/stats_scale.cu:1-83): every voxel of the mask links with its +1 neighbour along
-// every axis, a link always hangs the larger root under the smaller one by atomicMin, so every
-// parent is smaller than its child and a component's root is its smallest in-element index whatever
-// the order of the atomics; a last pass writes every voxel's root.
+// cpp/src/method/stats_scale.cu:1-83): a link always hangs the larger root under the smaller one by
+// atomicMin, so every parent is smaller than its child and a component's root is its smallest
+// in-element index whatever the order of the atomics; a last pass writes every voxel's root.
+//
+// gfx950 mapping. Along the contiguous axis a run of mask voxels is one component piece whose
+// smallest index is its first voxel: one wave per line labels every voxel with its run's first
+// voxel by a max-scan of the run starts (so runs need no links). Across the other axes a voxel v
+// links with v + st only where the two runs begin to touch: when v - 1 and v - 1 + st are not both
+// in the mask (otherwise that earlier pair already joins the same two runs).
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -37,26 +42,93 @@ __device__ __forceinline__ void unite(int32_t* L, int32_t a, int32_t c) {
   }
 }
 
+// Inclusive max-scan over the 64 lanes of a wave.
+__device__ __forceinline__ int32_t wave_max_scan(int32_t v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int32_t u = __shfl_up(v, d, 64);
+    if (lane >= d) v = u > v ? u : v;
+  }
+  return v;
+}
+
+constexpr int kRun = 8;  // samples per lane per tile
+
+// One wave per line of the contiguous (last) axis: labels = the index of the run's first voxel for
+// mask voxels, -1 elsewhere.
+__global__ void __launch_bounds__(kThreads) k_ccl_runs(Geom g, const uint8_t* __restrict__ mask, uint8_t bits,
+                                                       int32_t* __restrict__ labels, const int32_t* __restrict__ on) {
+  const int64_t b = blockIdx.y;
+  if (on != nullptr && on[b] == 0) return;
+  const int32_t n = g.n[g.dim - 1];
+  const int64_t lines = g.count / n;
+  const int lane = threadIdx.x & 63;
+  const int64_t line = static_cast<int64_t>(blockIdx.x) * (kThreads / 64) + (threadIdx.x >> 6);
+  if (line >= lines) return;  // uniform over the wave
+  const uint8_t* m = mask + b * g.count + line * n;
+  int32_t* out = labels + b * g.count;
+  const int32_t first = static_cast<int32_t>(line * n);  // the in-element index of the line's sample 0
+  int32_t carry = -1;  // the last run start of the earlier tiles
+  for (int32_t t0 = 0; t0 < n; t0 += 64 * kRun) {
+    const int32_t i0 = t0 + lane * kRun;
+    bool prev = i0 > 0 && i0 - 1 < n && (m[i0 - 1] & bits) != 0;
+    bool in[kRun];
+    int32_t st[kRun];
+    int32_t run = -1;
+#pragma unroll
+    for (int k = 0; k < kRun; ++k) {
+      const int32_t i = i0 + k;
+      in[k] = i < n && (m[i] & bits) != 0;
+      if (in[k] && !prev) run = i;
+      st[k] = run;
+      prev = in[k];
+    }
+    const int32_t incl = wave_max_scan(run);
+    int32_t excl = __shfl_up(incl, 1, 64);
+    if (lane == 0) excl = -1;
+    excl = excl > carry ? excl : carry;
+#pragma unroll
+    for (int k = 0; k < kRun; ++k) {
+      const int32_t i = i0 + k;
+      if (i < n) out[first + i] = in[k] ? first + (st[k] >= 0 ? st[k] : excl) : -1;
+    }
+    const int32_t last = __shfl(incl, 63, 64);
+    carry = last > carry ? last : carry;
+  }
+}
+
 }  // namespace
 
 void enqueue_ccl(hipStream_t s, const Geom& g, const uint8_t* mask, uint8_t bits, int32_t* labels, const int32_t* on) {
-  each(s, g, on, [=] __device__(int64_t b, int64_t i) {
-    labels[b * g.count + i] = (mask[b * g.count + i] & bits) != 0 ? static_cast<int32_t>(i) : -1;
-  });
-  each(s, g, on, [=] __device__(int64_t b, int64_t i) {
+  const int32_t nl = g.n[g.dim - 1];
+  const int64_t lines = g.count / nl;
+  const int64_t gx = (lines + kThreads / 64 - 1) / (kThreads / 64);
+  hipLaunchKernelGGL(k_ccl_runs, dim3(static_cast<unsigned>(gx), static_cast<unsigned>(g.batch)), dim3(kThreads), 0, s,
+                     g, mask, bits, labels, on);
+  EDA_LAUNCH("k_ccl_runs");
+  if (g.dim == 1) return;  // a line's runs are its components
+  each(s, g, on, [=] __device__(int64_t b, int64_t i64) {
     int32_t* L = labels + b * g.count;
-    const int32_t v = static_cast<int32_t>(i);
+    const int32_t v = static_cast<int32_t>(i64);
     if (L[v] < 0) return;
-    for (int a = 0; a < g.dim; ++a) {
-      if ((i / g.st[a]) % g.n[a] + 1 >= g.n[a]) continue;
-      const int32_t u = v + static_cast<int32_t>(g.st[a]);
+    const uint32_t uv = static_cast<uint32_t>(v);
+    const bool left = uv % static_cast<uint32_t>(nl) != 0;  // v - 1 lies on v's line
+    const bool left_in = left && L[v - 1] >= 0;
+    for (int a = 0; a < g.dim - 1; ++a) {
+      const uint32_t st = static_cast<uint32_t>(g.st[a]);
+      const uint32_t na = static_cast<uint32_t>(g.n[a]);
+      const uint32_t coord = a == 0 ? uv / st : (uv / st) % na;
+      if (coord + 1 >= na) continue;
+      const int32_t u = v + static_cast<int32_t>(st);
       if (L[u] < 0) continue;
+      if (left_in && L[u - 1] >= 0) continue;  // the pair (v - 1, u - 1) joins the same runs
       unite(L, v, u);
     }
   });
-  each(s, g, on, [=] __device__(int64_t b, int64_t i) {
+  each(s, g, on, [=] __device__(int64_t b, int64_t i64) {
     int32_t* L = labels + b * g.count;
-    const int32_t v = static_cast<int32_t>(i);
+    const int32_t v = static_cast<int32_t>(i64);
     if (L[v] < 0) return;
     const int32_t root = find_root(L, v);
     reinterpret_cast<volatile int32_t*>(L)[v] = root;

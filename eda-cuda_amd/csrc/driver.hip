@@ -1555,7 +1555,7 @@ void run_call(const eda_array_desc& x, const eda_params& p, Run& run) {
   run.n_imfs.assign(B, 0);
   run.error.assign(B, 0);
   // the chunk: bounded by max_chunk and by the free device memory (halving)
-  int64_t Bc = B;
+  int64_t Bc = std::min<int64_t>(B, 65535);  // the kernels put the elements of a chunk on grid.y
   if (p.max_chunk > 0) Bc = std::min<int64_t>(Bc, p.max_chunk);
   size_t free_b = 0, total_b = 0;
   EDA_HIP(hipMemGetInfo(&free_b, &total_b));

@@ -32,19 +32,22 @@ struct FlagLines {
   int32_t axis;
 };
 
+// 32-bit unsigned quotient (indices inside one element stay below 2^31).
+__device__ __forceinline__ uint32_t udiv(uint32_t a, uint32_t b) { return a / b; }
+
 template <class T>
 __global__ void __launch_bounds__(kThreads) k_flag_blocks(FlagLines L, int64_t batch, const T* __restrict__ r,
                                                           T* __restrict__ sum, const int32_t* __restrict__ on) {
-  const int64_t id = blockIdx.x * static_cast<int64_t>(kThreads) + threadIdx.x;
-  if (id >= batch * L.entries) return;
-  const int64_t b = id / L.entries;
+  const int64_t b = blockIdx.y;
+  const uint32_t e = blockIdx.x * kThreads + threadIdx.x;
+  if (e >= static_cast<uint64_t>(L.entries)) return;
   if (on != nullptr && on[b] == 0) return;
-  const int64_t e = id - b * L.entries;
-  const int64_t j = e % L.inner;
-  const int64_t t = e / L.inner;
-  const int64_t o = t / L.nb;
-  const int32_t c = static_cast<int32_t>(t - o * L.nb);
-  const T* x = r + b * L.count + o * L.n * L.inner + j;
+  const uint32_t inner = static_cast<uint32_t>(L.inner);
+  const uint32_t t = udiv(e, inner);
+  const uint32_t j = e - t * inner;
+  const uint32_t o = udiv(t, static_cast<uint32_t>(L.nb));
+  const int32_t c = static_cast<int32_t>(t - o * static_cast<uint32_t>(L.nb));
+  const T* x = r + b * L.count + static_cast<int64_t>(o) * L.n * L.inner + j;
   const int32_t i0 = c * kBlk;
   const int32_t i1 = min(i0 + kBlk, L.n);
   T mn = x[i0 * L.step];
@@ -121,21 +124,23 @@ __global__ void __launch_bounds__(kThreads) k_flags(FlagLines L, int64_t batch, 
   const int64_t b = blockIdx.y;
   if (on != nullptr && on[b] == 0) return;
   const int32_t n = L.n;
+  const uint32_t inner = static_cast<uint32_t>(L.inner);
   uint64_t count = 0;
-  for (int64_t e = blockIdx.x * static_cast<int64_t>(kThreads) + threadIdx.x; e < L.count;
-       e += static_cast<int64_t>(gridDim.x) * kThreads) {
-    const int64_t j = e % L.inner;
-    const int64_t t = e / L.inner;
-    const int64_t o = t / L.n;
-    const int32_t s = static_cast<int32_t>(t - o * L.n);
+  for (int64_t e64 = blockIdx.x * static_cast<int64_t>(kThreads) + threadIdx.x; e64 < L.count;
+       e64 += static_cast<int64_t>(gridDim.x) * kThreads) {
+    const uint32_t e = static_cast<uint32_t>(e64);
+    const uint32_t t = udiv(e, inner);
+    const uint32_t j = e - t * inner;
+    const uint32_t o = udiv(t, static_cast<uint32_t>(n));
+    const int32_t s = static_cast<int32_t>(t - o * static_cast<uint32_t>(n));
     if (s < 1 || s > n - 2) continue;
-    const int64_t base = b * L.count + o * static_cast<int64_t>(n) * L.inner + j;
+    const int64_t base = b * L.count + static_cast<int64_t>(o) * n * L.inner + j;
     const T* x = r + base;
     const int64_t st = L.step;
     const T cur = x[s * st];
     const T prev = x[(s - 1) * st];
     if (prev < cur || prev > cur) {
-      const T* bmin = sum + b * L.count + o * L.nb * L.inner + j;
+      const T* bmin = sum + b * L.count + static_cast<int64_t>(o) * L.nb * L.inner + j;
       const T* bmax = bmin + L.entries;
       const int64_t sst = L.inner;
       const int32_t en = run_end(x, st, n, L.nb, s, cur, bmin, bmax, sst);
@@ -189,9 +194,9 @@ void enqueue_sig_flags(hipStream_t s, const Geom& g, const T* r, const T* tau, u
     L.entries = L.outer * L.nb * L.inner;
     L.axis = a;
     if (n > kBlk) {
-      const int64_t ids = g.batch * L.entries;
-      hipLaunchKernelGGL(k_flag_blocks<T>, dim3(static_cast<unsigned>((ids + kThreads - 1) / kThreads)), dim3(kThreads),
-                         0, s, L, g.batch, r, scratch, on);
+      hipLaunchKernelGGL(k_flag_blocks<T>,
+                         dim3(static_cast<unsigned>((L.entries + kThreads - 1) / kThreads), static_cast<unsigned>(g.batch)),
+                         dim3(kThreads), 0, s, L, g.batch, r, scratch, on);
       EDA_LAUNCH("k_flag_blocks");
     }
     // (workgroups per element, elements): at most 128 workgroups, so 128 atomics, per element

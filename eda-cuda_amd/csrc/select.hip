@@ -50,26 +50,30 @@ struct SelArgs {
   int64_t items;
 };
 
-template <class T>
-__device__ __forceinline__ bool sel_value(const SelArgs<T>& a, int64_t b, int64_t t, double* out) {
-  double val;
+// The values of sample v of element b (every one goes to visit(double)): the field value, or the
+// forward difference (in T) along each selected axis where v is not the axis's last sample;
+// optionally |value - center[b]|. Indices inside one element stay below 2^31.
+template <class T, class Visit>
+__device__ __forceinline__ void sel_values(const SelArgs<T>& a, int64_t b, uint32_t v, Visit visit) {
+  const int64_t i = b * a.g.count + v;
   if (a.sel.src == SelSrc::kField) {
-    const int64_t i = b * a.g.count + t;
-    if (a.sel.mask != nullptr && (a.sel.mask[i] & a.sel.bits) == 0) return false;
-    val = static_cast<double>(a.sel.field[i]);
-  } else {
-    const int64_t k = t / a.g.count;
-    const int64_t v = t - k * a.g.count;
-    const int axis = a.ax[k];
-    const int64_t st = a.g.st[axis];
-    if ((v / st) % a.g.n[axis] >= a.g.n[axis] - 1) return false;
-    const int64_t i = b * a.g.count + v;
-    const T d = a.sel.field[i + st] - a.sel.field[i];  // the difference in T
-    val = static_cast<double>(d);
+    if (a.sel.mask != nullptr && (a.sel.mask[i] & a.sel.bits) == 0) return;
+    double val = static_cast<double>(a.sel.field[i]);
+    if (a.sel.center != nullptr) val = fabs(val - a.sel.center[b]);
+    visit(val);
+    return;
   }
-  if (a.sel.center != nullptr) val = fabs(val - a.sel.center[b]);
-  *out = val;
-  return true;
+  for (int k = 0; k < a.naxes; ++k) {
+    const int axis = a.ax[k];
+    const uint32_t st = static_cast<uint32_t>(a.g.st[axis]);
+    const uint32_t nn = static_cast<uint32_t>(a.g.n[axis]);
+    const uint32_t coord = axis == 0 ? v / st : (st == 1 ? v % nn : (v / st) % nn);
+    if (coord >= nn - 1) continue;
+    const T d = a.sel.field[i + st] - a.sel.field[i];  // the difference in T
+    double val = static_cast<double>(d);
+    if (a.sel.center != nullptr) val = fabs(val - a.sel.center[b]);
+    visit(val);
+  }
 }
 
 __global__ void k_sel_init(int64_t B, SelState* st, uint64_t* hist, const int32_t* on) {
@@ -95,10 +99,9 @@ __global__ void __launch_bounds__(kThreads) k_sel_hist(SelArgs<T> a, const SelSt
   __syncthreads();
   const int hi = shift + width;
   const uint64_t dmask = (1ULL << width) - 1ULL;
-  const int64_t stride = static_cast<int64_t>(nblk) * kThreads;
-  for (int64_t t = chunk * kThreads + threadIdx.x; t < a.items; t += stride) {
-    double v;
-    if (!sel_value(a, b, t, &v)) continue;
+  const uint32_t stride = static_cast<uint32_t>(nblk) * kThreads;
+  const uint32_t count = static_cast<uint32_t>(a.g.count);
+  const auto tally = [&](double v) {
     const uint64_t k = order_key(v);
     const uint32_t bin = static_cast<uint32_t>((k >> shift) & dmask);
     if (first) {
@@ -107,7 +110,9 @@ __global__ void __launch_bounds__(kThreads) k_sel_hist(SelArgs<T> a, const SelSt
       if ((k >> hi) == (S.prefix[0] >> hi)) atomicAdd(&h[0][bin], 1u);
       if (nr == 2 && (k >> hi) == (S.prefix[1] >> hi)) atomicAdd(&h[1][bin], 1u);
     }
-  }
+  };
+  for (uint32_t v = static_cast<uint32_t>(chunk) * kThreads + threadIdx.x; v < count; v += stride)
+    sel_values(a, b, v, tally);
   __syncthreads();
   const int nb = 1 << width;
   for (int i = threadIdx.x; i < nr * nb; i += kThreads) {
@@ -276,7 +281,7 @@ void enqueue_select(hipStream_t s, const Geom& g, const Selection<T>& sel, int m
   hipLaunchKernelGGL(k_sel_init, dim3(static_cast<unsigned>((ni + kThreads - 1) / kThreads)), dim3(kThreads), 0, s, B,
                      st, hist, on);
   EDA_LAUNCH("k_sel_init");
-  int64_t nblk = (a.items + int64_t{kThreads} * 32 - 1) / (int64_t{kThreads} * 32);
+  int64_t nblk = (g.count + int64_t{kThreads} * 32 - 1) / (int64_t{kThreads} * 32);  // blocks per element
   nblk = nblk < 1 ? 1 : (nblk > 512 ? 512 : nblk);
   if (a.items > 0) {
     for (int p = 0; p < kPasses; ++p) {
