@@ -35,7 +35,21 @@
 #include "kernels.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <limits>
+
+#ifndef EDA_VHGW_SUMU
+#define EDA_VHGW_SUMU 4  // 16-byte reads in flight per thread while a box-sum workgroup stages its lines
+#endif
+#ifndef EDA_VHGW_MMU
+#define EDA_VHGW_MMU 2  // the same for max / min
+#endif
+#ifndef EDA_VHGW_PIPE
+#define EDA_VHGW_PIPE 1  // box sums through the pipelined kernel where its tile allows
+#endif
+#ifndef EDA_BOX_ATTR
+#define EDA_BOX_ATTR  // extra attributes of the pipelined box kernel (occupancy hints)
+#endif
 
 namespace eda {
 namespace {
@@ -64,6 +78,22 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// k = first, first + stride, ... < end: U reads issued back to back before any is used (the memory
+// parallelism a tile's staging needs: one read in flight per thread leaves HBM mostly idle).
+template <int U, class Read, class Use>
+__device__ __forceinline__ void batched(int32_t first, int32_t end, int32_t stride, Read read, Use use) {
+  using Q = decltype(read(first));
+  for (int32_t k0 = first; k0 < end; k0 += U * stride) {
+    Q q[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (k0 + u * stride < end) q[u] = read(k0 + u * stride);
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (k0 + u * stride < end) use(k0 + u * stride, q[u]);
+  }
 }
 
 // ---- cross-lane moves -----------------------------------------------------------------------------
@@ -323,18 +353,96 @@ __host__ __device__ __forceinline__ WinInfo win_info(int32_t n, int32_t w) {
   return v;
 }
 
-// The box sums of the lines one wave owns, their chunk tasks packed into 64-lane rounds (a chain of
-// carries never leaves the wave). Line li < nl of the wave is staged reflected at E + p epad,
-// p = p0 + li pstep (the extended position e at index e + lead, with at least 32 slots of slack at
-// both ends), its output row at R + p lpad. DIR 1 writes H (the G of the reversed line) at slot
-// x = e + r of the forward position e; DIR 0 computes G at e = x + r and writes y = H + G over it,
-// halved where e ends a segment. One chunk per lane, CH >= the longest chunk: its samples are
-// consecutive in the extended row, so each one's slot is a base, the sample's index as an
-// immediate offset, and one more pad word past the chunk's 32-sample boundary. Samples past a
-// chunk's end add -0, which leaves every fold's bits unchanged; lanes with no output touch the junk
-// slot.
+// The box sums of the nl lines of a tile. Line p is staged at E + p epad + elead: the extended
+// position e in [-r, n + r) at offset e, with at least 32 slots of slack past both ends; its H row
+// (H + p epad + elead) has the same shape. One lane per segment (line, segment k), the tasks of the
+// tile spread over the workgroup's waves in rounds of 64. Each lane walks its segment in the
+// canonical order: 32-sample chunks, a sequential fold inside a chunk, G = carry + fold, the carry
+// the sequential fold of the chunk totals before (kept in a register). All lanes of a round step
+// together, so the words they touch lie wa (odd) apart. A chunk that starts past the end of the
+// extended line is skipped (its lane's later chunks too); a chunk that runs past it reads and writes
+// the 32 slots of slack there. DIR 1 walks the reversed line (its position e' reads e = n - 1 - e')
+// and writes H(e') at H row offset e, so H(x - r) sits at offset x - r; DIR 0 walks the line and
+// writes y(x) = H(x - r) + G(x + r) over its input at offset e = x + r, halved where e ends its
+// segment (every input sample is read by the one lane whose segment holds it, in the step that
+// overwrites it). Stores need no mask: positions without an output are the lane's own slack or
+// samples.
+template <int DIR, class T>
+__device__ void box_dir(T* __restrict__ E, T* __restrict__ H, int32_t epad, int32_t elead, int32_t nl, int32_t n,
+                        const WinInfo& wi) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int W = blockDim.x >> 6;
+  const int32_t r = wi.r, wa = wi.wa, nsegs = wi.nsegs, cs = wi.cs;
+  const int32_t e_max = n - 1 + r;
+  const int32_t total = nl * nsegs;
+  const float inv_ns = 1.0f / static_cast<float>(nsegs);
+  const int32_t lb = wa - 32 * (cs - 1);  // the last chunk's length
+  const T nz = -T(0);
+  for (int32_t u0 = wave * 64; u0 < total; u0 += W * 64) {
+    const int32_t u = u0 + lane;
+    const int32_t p = min(div_small(u, nsegs, inv_ns), nl - 1);
+    const int32_t k = u - p * nsegs;
+    const int32_t s = wi.o + (wi.k_lo + k) * wa;  // the segment's first position
+    const bool live = u < total;
+    // chunk q, sample j: position s + 32 q + j, at ep[-(32 q + j)] (DIR 1) or ep[32 q + j] (DIR 0)
+    T* ep = E + p * epad + elead + (DIR ? n - 1 - s : s);
+    T* hp = H + p * epad + elead + (DIR ? n - 1 - s : s - 2 * r);
+    T carry = nz;
+    for (int32_t q = 0; q < cs; ++q) {
+      const int32_t len = q < cs - 1 ? 32 : lb;
+      if (live && s + 32 * q <= e_max) {
+        T acc = nz;
+        const auto step = [&](int32_t j) {
+          acc = acc + (DIR ? ep[-j] : ep[j]);
+          const T g = q > 0 ? carry + acc : acc;
+          if constexpr (DIR == 1) {
+            hp[-j] = g;
+          } else {
+            ep[j] = hp[j] + g;
+          }
+        };
+        int32_t j = 0;
+        for (; j + 8 <= len; j += 8) {
+#pragma unroll
+          for (int v = 0; v < 8; ++v) step(j + v);
+        }
+        for (; j < len; ++j) step(j);
+        carry = q > 0 ? carry + acc : acc;
+      }
+      ep = DIR ? ep - 32 : ep + 32;
+      hp = DIR ? hp - 32 : hp + 32;
+    }
+    // DIR 0: the window that is the segment (ep is now 32 cs past the segment's start)
+    if (DIR == 0 && live && s + 32 * (cs - 1) <= e_max) {
+      T* y = ep - 32 + (lb - 1);
+      *y = *y * T(0.5);
+    }
+  }
+}
+
+// The box sums of a tile's nl lines (all the workgroup's threads); y(x) of line p ends at
+// E + p epad + elead + r + x.
+template <class T>
+__device__ void box_lines(T* E, T* H, int32_t epad, int32_t elead, int32_t nl, int32_t n, const WinInfo& wi) {
+  box_dir<1, T>(E, H, epad, elead, nl, n, wi);
+  __syncthreads();
+  box_dir<0, T>(E, H, epad, elead, nl, n, wi);
+}
+
+// The box sums of windows longer than 32 samples (segments of several chunks), for the lines one
+// wave owns: one lane per chunk, the chunk tasks of the lines packed into 64-lane rounds and the
+// chunk carries passed lane to lane by DPP wave shifts (a chain never leaves the wave). The rows are
+// padded (sample e at pidx(e + lead)), so that the chunks of one segment, 32 samples apart, fall in
+// distinct banks. Line li < nl of the wave is staged reflected at E + p epad, p = p0 + li pstep,
+// its output row at R + p lpad. DIR 1 writes H (the G of the reversed line) at slot x = e + r of the
+// forward position e; DIR 0 computes G at e = x + r and writes y = H + G over it, halved where e
+// ends a segment. One chunk per lane, CH >= the longest chunk: its samples are consecutive in the
+// extended row, so each one's slot is a base, the sample's index as an immediate offset, and one
+// more pad word past the chunk's 32-sample boundary. Samples past a chunk's end add -0, which
+// leaves every fold's bits unchanged; lanes with no output touch the junk slot.
 template <int DIR, int CH, class T>
-__device__ void sum_dir(const T* __restrict__ E, int32_t epad, int32_t lead, T* __restrict__ R, int32_t lpad,
+__device__ void chunk_dir(const T* __restrict__ E, int32_t epad, int32_t lead, T* __restrict__ R, int32_t lpad,
                         int32_t junk, int32_t p0, int32_t pstep, int32_t nl, int32_t n, const WinInfo& wi) {
   const int lane = threadIdx.x & 63;
   const int32_t r = wi.r, wa = wi.wa;
@@ -364,18 +472,19 @@ __device__ void sum_dir(const T* __restrict__ E, int32_t epad, int32_t lead, T* 
     const T* row0 = E + p * epad + pidx(ib);
     const T* row1 = DIR ? row0 - 1 : row0 + 1;
     const int32_t jb = DIR ? (ib & 31) : 32 - (ib & 31);
+    // samples past the chunk's end fold in too: only samples with no output, and totals no
+    // output uses (a segment's last chunk, chunks past the line's end), see them
     T v[CH];
     T acc = nz;
 #pragma unroll
     for (int j = 0; j < CH; ++j) {
       const T* rp = (DIR ? j > jb : j >= jb) ? row1 : row0;
-      const T xv = DIR ? rp[-j] : rp[j];
-      const T add = j < len ? xv : nz;
-      acc = j == 0 ? add : acc + add;
+      acc = acc + (DIR ? rp[-j] : rp[j]);
       v[j] = acc;
     }
     // the carries: out = the fold of the segment up to this chunk's end (chunk 0: its total);
-    // cin = the previous chunk's out; lane 0 continues the previous round's lane 63
+    // cin = the previous chunk's out (-0 for chunk 0: adding it changes no bits); lane 0 continues
+    // the previous round's lane 63
     T out = acc, cin = nz;
     for (int it = 0; it < iters; ++it) {
       const T prev = dpp<kWaveShr1>(carry, out);
@@ -397,34 +506,32 @@ __device__ void sum_dir(const T* __restrict__ E, int32_t epad, int32_t lead, T* 
       const bool valid = j < len && j >= jmin;
       T* slot = (DIR ? j > xm : j >= xm) ? out1 : out0;
       T* at = valid ? (DIR ? slot - j : slot + j) : R + junk;
-      const T gv = q > 0 ? cin + v[j] : v[j];
+      const T gv = cin + v[j];
       if constexpr (DIR == 1) {
         *at = gv;
       } else {
-        const T sum = *at + gv;
-        *at = c0 + j == e_al ? sum * T(0.5) : sum;
+        *at = *at + gv;
+      }
+    }
+    if (DIR == 0) {
+      // the window that is the segment: halve its sum
+      const int32_t ja = e_al - c0;
+      if (ja < len && ja >= 0 && ja >= jmin) {
+        T* at = (ja >= xm ? out1 : out0) + ja;
+        *at = *at * T(0.5);
       }
     }
   }
 }
 
-template <int CH, class T>
-__device__ void sum_lines_ch(const T* E, int32_t epad, int32_t lead, T* R, int32_t lpad, int32_t junk, int32_t p0,
-                             int32_t pstep, int32_t nl, int32_t n, const WinInfo& wi) {
-  sum_dir<1, CH, T>(E, epad, lead, R, lpad, junk, p0, pstep, nl, n, wi);
-  wave_sync();
-  sum_dir<0, CH, T>(E, epad, lead, R, lpad, junk, p0, pstep, nl, n, wi);
-  wave_sync();
-}
 
 template <class T>
-__device__ void sum_lines(const T* E, int32_t epad, int32_t lead, T* R, int32_t lpad, int32_t junk, int32_t p0,
-                          int32_t pstep, int32_t nl, int32_t n, const WinInfo& wi) {
-  const int32_t longest = wi.wa < 32 ? wi.wa : 32;
-  if (longest <= 4) sum_lines_ch<4, T>(E, epad, lead, R, lpad, junk, p0, pstep, nl, n, wi);
-  else if (longest <= 8) sum_lines_ch<8, T>(E, epad, lead, R, lpad, junk, p0, pstep, nl, n, wi);
-  else if (longest <= 16) sum_lines_ch<16, T>(E, epad, lead, R, lpad, junk, p0, pstep, nl, n, wi);
-  else sum_lines_ch<32, T>(E, epad, lead, R, lpad, junk, p0, pstep, nl, n, wi);
+__device__ void chunk_lines(const T* E, int32_t epad, int32_t lead, T* R, int32_t lpad, int32_t junk, int32_t p0,
+                            int32_t pstep, int32_t nl, int32_t n, const WinInfo& wi) {
+  chunk_dir<1, 32, T>(E, epad, lead, R, lpad, junk, p0, pstep, nl, n, wi);
+  wave_sync();
+  chunk_dir<0, 32, T>(E, epad, lead, R, lpad, junk, p0, pstep, nl, n, wi);
+  wave_sync();
 }
 
 // ---- the pass kernel ----------------------------------------------------------------------------------------
@@ -481,13 +588,12 @@ __global__ void __launch_bounds__(512) k_vhgw(VPass a, VPtrs<T> ptr) {
   const int lane = threadIdx.x & 63;
   // Max / min: V = P staged lines of lpad slots; per wave wrows rows of lpad + 1 slots, H then G
   // (one row: G only, the line's H goes over its staged samples); the outputs over V. Box sum: P
-  // reflected lines of epad slots, then P output rows R of lpad slots and one junk slot.
+  // reflected lines of epad slots, then their P H rows R of as many; the outputs over V.
   constexpr bool kSum = std::is_same_v<S, T>;
   S* V = reinterpret_cast<S*>(smem);
   S* Hb = V + a.P * a.lpad + wave * a.wrows * (a.lpad + 1);
   S* Gb = a.wrows == 2 ? Hb + a.lpad + 1 : Hb;
   S* R = V + a.P * a.epad;
-  S* Y = kSum ? R : V;  // the rows the write-out reads
   const bool contiguous = a.inner == 1;
   int64_t base0;
   int32_t lines;
@@ -517,17 +623,20 @@ __global__ void __launch_bounds__(512) k_vhgw(VPass a, VPtrs<T> ptr) {
   const int32_t half = (w - 1) / 2;
   const int32_t r = half < n - 1 ? half : n - 1;
   if (kSum && r > a.lead - 32) return;  // a window above the launch's largest (the host prevents it)
-  const int32_t rmax = a.lead - 32;
+  // box sum: windows over 32 samples fold chunk-parallel on padded rows, others walk their segments
+  // on plain rows; the offset of extended position e in a staged row
+  const bool padded = kSum && r >= 16;
+  const auto eslot = [&](int32_t e) { return padded ? pidx(e + a.lead) : e + a.lead; };
+  constexpr int VW = 16 / static_cast<int>(sizeof(T));
+  using V16 = std::conditional_t<sizeof(T) == 4, float4, double2>;
+  constexpr int kU = kSum ? EDA_VHGW_SUMU : EDA_VHGW_MMU;  // staging reads in flight per thread
+  constexpr int kO = kSum ? EDA_VHGW_SUMU : EDA_VHGW_MMU;  // write-out rows per thread between waits
   for (int f = 0; f < a.nf; ++f) {
     const T* src = ptr.in[f] + rung * ptr.in_stride;
-    // sample i of line p; the box sum also writes its reflections at e = -1 - i (i < rmax) and
-    // e = 2n - 1 - i (i >= n - rmax)
+    // sample i of line p (the box sum: at its extended position e = i)
     const auto put = [&](int32_t p, int32_t i, T v) {
       if constexpr (kSum) {
-        S* El = V + p * a.epad;
-        El[pidx(i + a.lead)] = v;
-        if (i < rmax) El[pidx(a.lead - 1 - i)] = v;
-        if (i >= n - rmax) El[pidx(2 * n - 1 - i + a.lead)] = v;
+        V[p * a.epad + eslot(i)] = v;
       } else {
         V[p * a.lpad + pidx(i)] = okey(v);
       }
@@ -535,60 +644,85 @@ __global__ void __launch_bounds__(512) k_vhgw(VPass a, VPtrs<T> ptr) {
     if (a.flat) {
       // max / min rows in one run: pidx(p n + i) = p lpad + pidx(i)
       if constexpr (!kSum)
-        for (int32_t idx = threadIdx.x; idx < total; idx += nthr) V[pidx(idx)] = okey(src[base0 + idx]);
+        batched<kU>(
+            threadIdx.x, total, nthr, [&](int32_t idx) { return src[base0 + idx]; },
+            [&](int32_t idx, T v) { V[pidx(idx)] = okey(v); });
     } else if (contiguous && a.vec) {
       // 16-byte loads of VW consecutive samples of one line (n is a multiple of VW)
-      constexpr int VW = 16 / static_cast<int>(sizeof(T));
-      using V16 = std::conditional_t<sizeof(T) == 4, float4, double2>;
-      for (int32_t idx = threadIdx.x * VW; idx < total; idx += nthr * VW) {
-        const int32_t p = div_small(idx, n, inv_n);
-        const int32_t i = idx - p * n;
-        const V16 q = *reinterpret_cast<const V16*>(src + base0 + idx);
-        if constexpr (VW == 4) {
-          put(p, i, q.x);
-          put(p, i + 1, q.y);
-          put(p, i + 2, q.z);
-          put(p, i + 3, q.w);
-        } else {
-          put(p, i, q.x);
-          put(p, i + 1, q.y);
-        }
-      }
+      batched<kU>(
+          threadIdx.x * VW, total, nthr * VW,
+          [&](int32_t idx) { return *reinterpret_cast<const V16*>(src + base0 + idx); },
+          [&](int32_t idx, const V16& q) {
+            const int32_t p = div_small(idx, n, inv_n);
+            const int32_t i = idx - p * n;
+            if constexpr (VW == 4) {
+              put(p, i, q.x);
+              put(p, i + 1, q.y);
+              put(p, i + 2, q.z);
+              put(p, i + 3, q.w);
+            } else {
+              put(p, i, q.x);
+              put(p, i + 1, q.y);
+            }
+          });
     } else if (contiguous) {
-      for (int32_t idx = threadIdx.x; idx < total; idx += nthr) {
-        const int32_t p = div_small(idx, n, inv_n);
-        put(p, idx - p * n, src[base0 + idx]);
-      }
+      batched<kU>(
+          threadIdx.x, total, nthr, [&](int32_t idx) { return src[base0 + idx]; },
+          [&](int32_t idx, T v) {
+            const int32_t p = div_small(idx, n, inv_n);
+            put(p, idx - p * n, v);
+          });
     } else if (a.vec && lines == a.P) {
       // 16-byte loads: tpr threads per row, each VW adjacent columns
-      constexpr int VW = 16 / static_cast<int>(sizeof(T));
-      using V16 = std::conditional_t<sizeof(T) == 4, float4, double2>;
       const int tpr = a.P / VW;
       const int cg = threadIdx.x % tpr;
-      for (int32_t i = threadIdx.x / tpr; i < n; i += nthr / tpr) {
-        const V16 q = *reinterpret_cast<const V16*>(src + base0 + static_cast<int64_t>(i) * a.step + cg * VW);
-        if constexpr (VW == 4) {
-          put(cg * 4 + 0, i, q.x);
-          put(cg * 4 + 1, i, q.y);
-          put(cg * 4 + 2, i, q.z);
-          put(cg * 4 + 3, i, q.w);
-        } else {
-          put(cg * 2 + 0, i, q.x);
-          put(cg * 2 + 1, i, q.y);
-        }
-      }
+      batched<kU>(
+          static_cast<int32_t>(threadIdx.x) / tpr, n, nthr / tpr,
+          [&](int32_t i) {
+            return *reinterpret_cast<const V16*>(src + base0 + static_cast<int64_t>(i) * a.step + cg * VW);
+          },
+          [&](int32_t i, const V16& q) {
+            if constexpr (VW == 4) {
+              put(cg * 4 + 0, i, q.x);
+              put(cg * 4 + 1, i, q.y);
+              put(cg * 4 + 2, i, q.z);
+              put(cg * 4 + 3, i, q.w);
+            } else {
+              put(cg * 2 + 0, i, q.x);
+              put(cg * 2 + 1, i, q.y);
+            }
+          });
     } else {
       const int32_t p = threadIdx.x & (a.P - 1);
       if (p < lines)
-        for (int32_t i = threadIdx.x >> pshift; i < n; i += nthr >> pshift)
-          put(p, i, src[base0 + static_cast<int64_t>(i) * a.step + p]);
+        batched<kU>(
+            static_cast<int32_t>(threadIdx.x >> pshift), n, nthr >> pshift,
+            [&](int32_t i) { return src[base0 + static_cast<int64_t>(i) * a.step + p]; },
+            [&](int32_t i, T v) { put(p, i, v); });
     }
     __syncthreads();
     if constexpr (kSum) {
-      // wave w owns lines w, w + W, ...
+      // the reflections the window reads: e = -1 - i for i < r, e = 2n - 1 - i for i >= n - r
+      if (r > 0) {
+        const int32_t nref = 2 * r;
+        const float inv_ref = 1.0f / static_cast<float>(nref);
+        for (int32_t idx = threadIdx.x; idx < lines * nref; idx += nthr) {
+          const int32_t p = div_small(idx, nref, inv_ref);
+          const int32_t t = idx - p * nref;
+          S* El = V + p * a.epad;
+          const int32_t i = t < r ? t : n - 1 - (t - r);
+          const int32_t e = t < r ? -1 - i : 2 * n - 1 - i;
+          El[eslot(e)] = El[eslot(i)];
+        }
+        __syncthreads();
+      }
       const WinInfo wi = win_info(n, w);
-      const int32_t nl = wave < lines ? (lines - wave + W - 1) / W : 0;
-      sum_lines<T>(V, a.epad, a.lead, R, a.lpad, a.P * a.lpad, wave, W, nl, n, wi);
+      if (padded) {
+        const int32_t nl = wave < lines ? (lines - wave + W - 1) / W : 0;  // wave w owns lines w, w + W, ...
+        chunk_lines<T>(V, a.epad, a.lead, R, a.lpad, a.P * a.lpad, wave, W, nl, n, wi);
+      } else {
+        box_lines<T>(V, R, a.epad, a.lead, lines, n, wi);
+      }
     } else {
       for (int p = wave; p < lines; p += W) {
         S* Vl = V + p * a.lpad;
@@ -632,18 +766,257 @@ __global__ void __launch_bounds__(512) k_vhgw(VPass a, VPtrs<T> ptr) {
     T* dst = ptr.out[f] + rung * ptr.out_stride;
     const auto get = [&](int32_t slot) {
       T y;
-      if constexpr (kSum) y = Y[slot];
-      else y = okey_inv(Y[slot]);
+      if constexpr (kSum) y = padded ? R[slot] : V[slot];
+      else y = okey_inv(V[slot]);
       return a.scale ? y * fac : y;
     };
+    // the slot of output i of line p
+    const auto slot = [&](int32_t p, int32_t i) {
+      return kSum && !padded ? p * a.epad + a.lead + r + i : p * a.lpad + pidx(i);
+    };
     if (a.flat) {
-      for (int32_t idx = threadIdx.x; idx < total; idx += nthr) dst[base0 + idx] = get(pidx(idx));
+      batched<kO>(
+          threadIdx.x, total, nthr, [&](int32_t idx) { return get(pidx(idx)); },
+          [&](int32_t idx, T y) { dst[base0 + idx] = y; });
     } else if (contiguous && a.vec) {
-      constexpr int VW = 16 / static_cast<int>(sizeof(T));
-      using V16 = std::conditional_t<sizeof(T) == 4, float4, double2>;
-      for (int32_t idx = threadIdx.x * VW; idx < total; idx += nthr * VW) {
+      batched<kO>(
+          threadIdx.x * VW, total, nthr * VW,
+          [&](int32_t idx) {
+            const int32_t p = div_small(idx, n, inv_n);
+            const int32_t sl = slot(p, idx - p * n);  // VW samples in one 32-block: consecutive slots
+            V16 q;
+            if constexpr (VW == 4) {
+              q.x = get(sl);
+              q.y = get(sl + 1);
+              q.z = get(sl + 2);
+              q.w = get(sl + 3);
+            } else {
+              q.x = get(sl);
+              q.y = get(sl + 1);
+            }
+            return q;
+          },
+          [&](int32_t idx, const V16& q) { *reinterpret_cast<V16*>(dst + base0 + idx) = q; });
+    } else if (contiguous) {
+      batched<kO>(
+          threadIdx.x, total, nthr,
+          [&](int32_t idx) {
+            const int32_t p = div_small(idx, n, inv_n);
+            return get(slot(p, idx - p * n));
+          },
+          [&](int32_t idx, T y) { dst[base0 + idx] = y; });
+    } else if (a.vec && lines == a.P) {
+      const int tpr = a.P / VW;
+      const int cg = threadIdx.x % tpr;
+      batched<kO>(
+          static_cast<int32_t>(threadIdx.x) / tpr, n, nthr / tpr,
+          [&](int32_t i) {
+            V16 q;
+            if constexpr (VW == 4) {
+              q.x = get(slot(cg * 4 + 0, i));
+              q.y = get(slot(cg * 4 + 1, i));
+              q.z = get(slot(cg * 4 + 2, i));
+              q.w = get(slot(cg * 4 + 3, i));
+            } else {
+              q.x = get(slot(cg * 2 + 0, i));
+              q.y = get(slot(cg * 2 + 1, i));
+            }
+            return q;
+          },
+          [&](int32_t i, const V16& q) {
+            *reinterpret_cast<V16*>(dst + base0 + static_cast<int64_t>(i) * a.step + cg * VW) = q;
+          });
+    } else {
+      const int32_t p = threadIdx.x & (a.P - 1);
+      if (p < lines)
+        batched<kO>(
+            static_cast<int32_t>(threadIdx.x >> pshift), n, nthr >> pshift,
+            [&](int32_t i) { return get(slot(p, i)); },
+            [&](int32_t i, T y) { dst[base0 + static_cast<int64_t>(i) * a.step + p] = y; });
+    }
+    __syncthreads();
+  }
+}
+
+// ---- the box sum, persistent and pipelined ------------------------------------------------------------
+// The same tiles as k_vhgw's box sum (P lines staged reflected), for tiles that 16-byte accesses
+// cover, one launch per row layout (PAD: windows over 32 samples; each launch skips the other's
+// items): each workgroup walks its share of the (rung, element, tile) items and reads the next
+// running item's samples into registers (PF 16-byte reads per thread) before it sums the current
+// one, so that the HBM reads of one tile overlap the LDS work of the one before. Workgroup i runs
+// on XCD i mod 8; at each step of the walk an XCD's workgroups take consecutive items, so that the
+// tiles they work on at a moment are neighbours in memory and share its L2, and every XCD works on
+// every rung.
+template <class T, int PF, bool PAD>
+__global__ void __launch_bounds__(256) EDA_BOX_ATTR k_box(VPass a, VPtrs<T> ptr, int32_t rungs) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int VW = 16 / static_cast<int>(sizeof(T));
+  using V16 = std::conditional_t<sizeof(T) == 4, float4, double2>;
+  T* E = reinterpret_cast<T*>(smem);
+  T* R = E + a.P * a.epad;
+  const int nthr = blockDim.x;
+  const int W = nthr >> 6;
+  const int wave = threadIdx.x >> 6;
+  const int32_t n = a.n;
+  const bool contiguous = a.inner == 1;
+  const float inv_n = 1.0f / static_cast<float>(n);
+  const int64_t per_rung = a.g.batch * a.groups;
+  const int64_t items = per_rung * rungs;
+  // grid-stride over the items in the workgroup's place among the XCD's workgroups: each step,
+  // XCD x takes the gridDim.x / 8 consecutive items from x gridDim.x / 8 on
+  const int64_t hi = items;
+  const int64_t nloc = gridDim.x;
+  const int64_t vid = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+  // strided axis: tpr threads per row, each VW adjacent columns; rows t / tpr + u nthr / tpr
+  const int tpr = a.P / VW;
+  const int cg = threadIdx.x % tpr;
+  const int32_t row0 = static_cast<int32_t>(threadIdx.x) / tpr;
+  const int32_t rstep = nthr / tpr;
+
+  struct Item {
+    int64_t base0;  // sample 0 of line 0 in the field
+    int32_t lines, w;
+    int64_t rung;
+  };
+  const auto decode = [&](int64_t it) {
+    Item d;
+    d.rung = it / per_rung;
+    const int64_t rem = it - d.rung * per_rung;
+    const int64_t b = rem / a.groups;
+    const int64_t grp = rem - b * a.groups;
+    d.w = ptr.win[d.rung * a.g.batch + b];
+    if (contiguous) {
+      const int64_t l0 = grp * a.P;
+      d.lines = static_cast<int32_t>(min<int64_t>(a.P, a.outer - l0));
+      d.base0 = b * a.g.count + l0 * n;
+    } else {
+      const int64_t gpo = a.inner / a.P;  // the host keeps inner a multiple of P
+      const int64_t o = grp / gpo;
+      d.lines = a.P;
+      d.base0 = b * a.g.count + o * static_cast<int64_t>(n) * a.inner + (grp - o * gpo) * a.P;
+    }
+    return d;
+  };
+  // the first running item of this kernel's class at or after it (hi if none); a window above the
+  // launch's largest (the host prevents it) does not run
+  const auto next_running = [&](int64_t it) {
+    for (; it < hi; it += nloc) {
+      const int64_t rung = it / per_rung;
+      const int64_t b = (it - rung * per_rung) / a.groups;
+      const int32_t w = ptr.win[rung * a.g.batch + b];
+      const int32_t rw = min((w - 1) / 2, n - 1);
+      if (w >= 1 && rw <= a.lead - 32 && (rw >= 16) == PAD) break;
+    }
+    return it < hi ? it : hi;
+  };
+  V16 buf[PF];
+  const auto fetch = [&](const Item& d) {
+    const T* src = ptr.in[0] + d.rung * ptr.in_stride + d.base0;
+    const int32_t total = d.lines * n;
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+      if (contiguous) {
+        const int32_t idx = (static_cast<int32_t>(threadIdx.x) + u * nthr) * VW;
+        if (idx < total) buf[u] = *reinterpret_cast<const V16*>(src + idx);
+      } else {
+        const int32_t i = row0 + u * rstep;
+        if (i < n) buf[u] = *reinterpret_cast<const V16*>(src + static_cast<int64_t>(i) * a.step + cg * VW);
+      }
+    }
+  };
+
+  int64_t it = next_running(vid);
+  if (it >= hi) return;
+  Item cur = decode(it);
+  fetch(cur);
+  while (true) {
+    const int32_t half = (cur.w - 1) / 2;
+    const int32_t r = half < n - 1 ? half : n - 1;
+    // windows over 32 samples (PAD) fold chunk-parallel on padded rows, others walk their segments
+    // on plain rows; the offset of extended position e in a staged row
+    constexpr bool padded = PAD;
+    const auto eslot = [&](int32_t e) { return padded ? pidx(e + a.lead) : e + a.lead; };
+    // stage: sample i of line p at its extended position e = i
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+      if (contiguous) {
+        const int32_t idx = (static_cast<int32_t>(threadIdx.x) + u * nthr) * VW;
+        if (idx >= cur.lines * n) continue;
         const int32_t p = div_small(idx, n, inv_n);
-        const int32_t sl = p * a.lpad + pidx(idx - p * n);  // VW samples in one 32-block: consecutive slots
+        const int32_t i = idx - p * n;
+        T* El = E + p * a.epad;
+        if constexpr (VW == 4) {
+          El[eslot(i)] = buf[u].x;
+          El[eslot(i + 1)] = buf[u].y;
+          El[eslot(i + 2)] = buf[u].z;
+          El[eslot(i + 3)] = buf[u].w;
+        } else {
+          El[eslot(i)] = buf[u].x;
+          El[eslot(i + 1)] = buf[u].y;
+        }
+      } else {
+        const int32_t i = row0 + u * rstep;
+        if (i >= n) continue;
+        T* dst = E + cg * VW * a.epad + eslot(i);
+        if constexpr (VW == 4) {
+          dst[0] = buf[u].x;
+          dst[a.epad] = buf[u].y;
+          dst[2 * a.epad] = buf[u].z;
+          dst[3 * a.epad] = buf[u].w;
+        } else {
+          dst[0] = buf[u].x;
+          dst[a.epad] = buf[u].y;
+        }
+      }
+    }
+    __syncthreads();
+    // the reflections the window reads: e = -1 - i for i < r, e = 2n - 1 - i for i >= n - r
+    if (r > 0) {
+      const int32_t nref = 2 * r;
+      const float inv_ref = 1.0f / static_cast<float>(nref);
+      for (int32_t idx = threadIdx.x; idx < cur.lines * nref; idx += nthr) {
+        const int32_t p = div_small(idx, nref, inv_ref);
+        const int32_t t = idx - p * nref;
+        T* El = E + p * a.epad;
+        const int32_t i = t < r ? t : n - 1 - (t - r);
+        const int32_t e = t < r ? -1 - i : 2 * n - 1 - i;
+        El[eslot(e)] = El[eslot(i)];
+      }
+      __syncthreads();
+    }
+    // the next item's reads go out now and land while this one is summed
+    const int64_t nx = next_running(it + nloc);
+    Item nxt{};
+    if (nx < hi) {
+      nxt = decode(nx);
+      fetch(nxt);
+    }
+    {
+      const WinInfo wi = win_info(n, cur.w);
+      if constexpr (PAD) {
+        const int32_t nl = wave < cur.lines ? (cur.lines - wave + W - 1) / W : 0;  // wave w owns lines w, w + W, ...
+        chunk_lines<T>(E, a.epad, a.lead, R, a.lpad, a.P * a.lpad, wave, W, nl, n, wi);
+      } else {
+        box_lines<T>(E, R, a.epad, a.lead, cur.lines, n, wi);
+      }
+    }
+    __syncthreads();
+    T fac = T(1);
+    if (a.scale) {
+      double prod = 1.0;
+      for (int ax = 0; ax < a.g.dim; ++ax) prod *= static_cast<double>(2 * min(half, a.g.n[ax] - 1) + 1);
+      fac = static_cast<T>(1.0 / prod);
+    }
+    T* dst = ptr.out[0] + cur.rung * ptr.out_stride + cur.base0;
+    // output x of line p: padded R + p lpad + pidx(x), else E + p epad + lead + r + x
+    const T* Y = padded ? R : E;
+    const int32_t ypad = padded ? a.lpad : a.epad;
+    const auto yslot = [&](int32_t x) { return padded ? pidx(x) : a.lead + r + x; };
+    const auto get = [&](int32_t slot) { return a.scale ? Y[slot] * fac : Y[slot]; };
+    if (contiguous) {
+      for (int32_t idx = threadIdx.x * VW; idx < cur.lines * n; idx += nthr * VW) {
+        const int32_t p = div_small(idx, n, inv_n);
+        const int32_t sl = p * ypad + yslot(idx - p * n);  // VW samples in one 32-block: consecutive slots
         V16 q;
         if constexpr (VW == 4) {
           q.x = get(sl);
@@ -654,39 +1027,28 @@ __global__ void __launch_bounds__(512) k_vhgw(VPass a, VPtrs<T> ptr) {
           q.x = get(sl);
           q.y = get(sl + 1);
         }
-        *reinterpret_cast<V16*>(dst + base0 + idx) = q;
-      }
-    } else if (contiguous) {
-      for (int32_t idx = threadIdx.x; idx < total; idx += nthr) {
-        const int32_t p = div_small(idx, n, inv_n);
-        const int32_t i = idx - p * n;
-        dst[base0 + idx] = get(p * a.lpad + pidx(i));
-      }
-    } else if (a.vec && lines == a.P) {
-      constexpr int VW = 16 / static_cast<int>(sizeof(T));
-      using V16 = std::conditional_t<sizeof(T) == 4, float4, double2>;
-      const int tpr = a.P / VW;
-      const int cg = threadIdx.x % tpr;
-      for (int32_t i = threadIdx.x / tpr; i < n; i += nthr / tpr) {
-        V16 q;
-        if constexpr (VW == 4) {
-          q.x = get((cg * 4 + 0) * a.lpad + pidx(i));
-          q.y = get((cg * 4 + 1) * a.lpad + pidx(i));
-          q.z = get((cg * 4 + 2) * a.lpad + pidx(i));
-          q.w = get((cg * 4 + 3) * a.lpad + pidx(i));
-        } else {
-          q.x = get((cg * 2 + 0) * a.lpad + pidx(i));
-          q.y = get((cg * 2 + 1) * a.lpad + pidx(i));
-        }
-        *reinterpret_cast<V16*>(dst + base0 + static_cast<int64_t>(i) * a.step + cg * VW) = q;
+        *reinterpret_cast<V16*>(dst + idx) = q;
       }
     } else {
-      const int32_t p = threadIdx.x & (a.P - 1);
-      if (p < lines)
-        for (int32_t i = threadIdx.x >> pshift; i < n; i += nthr >> pshift)
-          dst[base0 + static_cast<int64_t>(i) * a.step + p] = get(p * a.lpad + pidx(i));
+      for (int32_t i = row0; i < n; i += rstep) {
+        const int32_t sl = cg * VW * ypad + yslot(i);
+        V16 q;
+        if constexpr (VW == 4) {
+          q.x = get(sl);
+          q.y = get(sl + ypad);
+          q.z = get(sl + 2 * ypad);
+          q.w = get(sl + 3 * ypad);
+        } else {
+          q.x = get(sl);
+          q.y = get(sl + ypad);
+        }
+        *reinterpret_cast<V16*>(dst + static_cast<int64_t>(i) * a.step + cg * VW) = q;
+      }
     }
-    __syncthreads();
+    if (nx >= hi) break;
+    __syncthreads();  // the outputs sit in the rows the next item is staged into
+    it = nx;
+    cur = nxt;
   }
 }
 
@@ -831,7 +1193,9 @@ struct Shape {
   int32_t P, W, lpad, span, flat, wrows, epad, lead;
   size_t lds;
 };
-Shape pass_shape(const VPass& a, size_t s, bool sum, int32_t max_w) {
+// Box sums: layout 1 = plain rows only (windows up to 32 samples), 2 = padded rows only (longer
+// windows), 0 = room for either (the kernel that serves both).
+Shape pass_shape(const VPass& a, size_t s, bool sum, int32_t max_w, int layout = 0) {
   const int32_t n = a.n;
   const bool contiguous = a.inner == 1;
   Shape sh{};
@@ -839,11 +1203,15 @@ Shape pass_shape(const VPass& a, size_t s, bool sum, int32_t max_w) {
   sh.span = sum ? n : (n + tile - 1) / tile * tile;
   const int32_t need = pidx(sh.span - 1) + 1;
   // the box sum's reflected lines: extended positions [-rmax, n + rmax) at e + lead, and 32 slots
-  // of slack past both ends for the chunk walks
+  // of slack past both ends, plain or padded (pidx); then as many rows again for H (plain), or the
+  // output rows of lpad slots and a junk slot (padded)
   const int32_t rmax = std::min((max_w - 1) / 2, n - 1);
   sh.lead = 32 + rmax;
-  sh.epad = sum ? (pidx(n + 2 * rmax + 64) + 1) | 1 : 0;
-  if (!sum && contiguous && (n & 31) == 0 && sh.span == n) {
+  sh.epad = !sum ? 0 : layout == 1 ? (n + 2 * rmax + 64) | 1 : (pidx(n + 2 * rmax + 64) + 1) | 1;
+  if (sum) {
+    sh.lpad = need | 1;
+    sh.flat = 0;
+  } else if (contiguous && (n & 31) == 0 && sh.span == n) {
     sh.lpad = n + n / 32;  // rows in one run: pidx(p n + i) = p lpad + pidx(i)
     sh.flat = 1;
   } else {
@@ -859,11 +1227,12 @@ Shape pass_shape(const VPass& a, size_t s, bool sum, int32_t max_w) {
   }
   // max / min of a line that fits one tile keep H over the staged line: one LDS row per wave, and
   // up to 8 waves; otherwise two rows and 4 waves
-  // (the box sum: an output row per line and one junk slot)
+  // (the box sum: the second rows per line)
   sh.wrows = sum ? 0 : sh.span == tile ? 1 : 2;
   sh.W = std::min(sum ? EDA_VHGW_SUMW : sh.wrows == 1 ? 8 : 4, sh.P);
   const auto bytes = [&](int32_t P, int32_t W) {
-    if (sum) return (static_cast<size_t>(P) * (sh.epad + sh.lpad) + 1) * s;
+    if (sum && layout == 2) return (static_cast<size_t>(P) * (sh.epad + sh.lpad) + 1) * s;
+    if (sum) return static_cast<size_t>(P) * 2 * sh.epad * s;
     return (static_cast<size_t>(P) * sh.lpad + static_cast<size_t>(sh.wrows * W) * (sh.lpad + 1)) * s;
   };
   const size_t budget = sum ? EDA_VHGW_SUMLDS : EDA_VHGW_LDS;
@@ -910,6 +1279,49 @@ void go_rows(hipStream_t s, int64_t items, const RPass& r, const VPtrs<T>& p) {
     EDA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
                                 static_cast<int>(lds)));
   hipLaunchKernelGGL(k, dim3(static_cast<unsigned>(wgs)), dim3(64 * W), lds, s, r, p);
+}
+
+// Compute units of the current device (cached per device).
+int device_cus() {
+  static std::atomic<int> cache[64];
+  int dev = 0;
+  EDA_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) dev = 0;
+  int v = cache[dev].load(std::memory_order_relaxed);
+  if (v <= 0) {
+    EDA_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev));
+    v = std::max(v, 1);
+    cache[dev].store(v, std::memory_order_relaxed);
+  }
+  return v;
+}
+
+// The pipelined box sum takes a tile that its 16-byte reads cover in at most 8 per thread: returns
+// the reads per thread it needs (4 or 8), or 0 for k_vhgw.
+template <class T>
+int box_reads(const VPass& a, int nthr) {
+  constexpr int64_t VW = 16 / static_cast<int64_t>(sizeof(T));
+  if (!a.vec || a.nf != 1) return 0;
+  if (a.inner > 1 && a.inner % a.P != 0) return 0;
+  const int64_t per_thread = (static_cast<int64_t>(a.P) * a.n + nthr * VW - 1) / (nthr * VW);
+  if (a.inner > 1 && (a.n + nthr / (a.P / VW) - 1) / (nthr / (a.P / VW)) > per_thread) return 0;
+  return per_thread <= 4 ? 4 : per_thread <= 8 ? 8 : 0;
+}
+
+template <class T, bool PAD>
+void launch_box(int pf, hipStream_t s, const VPass& a, const VPtrs<T>& p, int32_t rungs, int W, size_t lds) {
+  const int64_t items = a.g.batch * a.groups * rungs;
+  const int64_t per_cu = std::max<int64_t>(1, std::min<int64_t>(4, static_cast<int64_t>(160 * 1024 / lds)));
+  int64_t wgs = std::min<int64_t>((items + 7) / 8 * 8, device_cus() * per_cu);
+  wgs = std::max<int64_t>(8, wgs / 8 * 8);
+  const auto go = [&](auto k) {
+    if (lds > 65536)
+      EDA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  static_cast<int>(lds)));
+    hipLaunchKernelGGL(k, dim3(static_cast<unsigned>(wgs)), dim3(64 * W), lds, s, a, p, rungs);
+  };
+  if (pf == 4) go(k_box<T, 4, PAD>);
+  else go(k_box<T, 8, PAD>);
 }
 
 // The streaming pass serves max / min along a contiguous axis with n <= 2048.
@@ -975,43 +1387,73 @@ void enqueue_vhgw(hipStream_t s, const Geom& g, int axis, int nf, bool same_inpu
     a.step = g.st[axis];
     a.inner = axis == g.dim - 1 ? 1 : g.st[axis];
     a.outer = g.count / (static_cast<int64_t>(a.n) * a.inner);
-    const Shape sh = pass_shape(a, sizeof(T), sum, max_w);
-    a.P = sh.P;
-    a.lpad = sh.lpad;
-    a.span = sh.span;
-    a.flat = sh.flat;
-    a.wrows = sh.wrows;
-    a.epad = sh.epad;
-    a.lead = sh.lead;
-    if (a.inner == 1) {
-      a.groups = (a.outer + a.P - 1) / a.P;
-    } else {
-      const int64_t gpo = (a.inner + a.P - 1) / a.P;
-      a.groups = a.outer * gpo;
-    }
     a.nf = nf;
     a.ops[0] = ops[0];
     a.ops[1] = nf > 1 ? ops[1] : ops[0];
     a.scale = scale ? 1 : 0;
-    const int64_t wgs = g.batch * a.groups;
-    if (wgs > 0x7fffffff) throw std::invalid_argument("eda: too many workgroups for one pass");
-    a.xcd = (wgs & 7) == 0 ? 1 : 0;
-    {
+    // the tile of a shape: lines, rows, workgroups, the 16-byte access test
+    const auto shaped = [&](const Shape& sh) {
+      VPass v = a;
+      v.P = sh.P;
+      v.lpad = sh.lpad;
+      v.span = sh.span;
+      v.flat = sh.flat;
+      v.wrows = sh.wrows;
+      v.epad = sh.epad;
+      v.lead = sh.lead;
+      if (v.inner == 1) {
+        v.groups = (v.outer + v.P - 1) / v.P;
+      } else {
+        const int64_t gpo = (v.inner + v.P - 1) / v.P;
+        v.groups = v.outer * gpo;
+      }
+      const int64_t wgs = g.batch * v.groups;
+      if (wgs > 0x7fffffff) throw std::invalid_argument("eda: too many workgroups for one pass");
+      v.xcd = (wgs & 7) == 0 ? 1 : 0;
       const auto al16 = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; };
       constexpr int64_t VW = 16 / static_cast<int64_t>(sizeof(T));
       // strided axis: P and the column count multiples of VW; contiguous axis: n a multiple of VW
-      const bool shape_ok = a.inner > 1 ? a.P % VW == 0 && a.inner % VW == 0 : a.n % VW == 0;
-      a.vec = shape_ok && al16(p.in[0]) && al16(p.in[1]) && al16(p.out[0]) && al16(p.out[1]) &&
+      const bool shape_ok = v.inner > 1 ? v.P % VW == 0 && v.inner % VW == 0 : v.n % VW == 0;
+      v.vec = shape_ok && al16(p.in[0]) && al16(p.in[1]) && al16(p.out[0]) && al16(p.out[1]) &&
                       (p.in_stride % VW) == 0 && (p.out_stride % VW) == 0
                   ? 1
                   : 0;
-    }
+      return v;
+    };
     if (timed) timer_note(s, &t0);
-    const dim3 grid(static_cast<unsigned>(wgs), static_cast<unsigned>(rungs));
-    const dim3 block(static_cast<unsigned>(64 * sh.W));
-    if (sum) launch_kt<T, T>(1, grid, block, sh.lds, s, a, p);
-    else launch_kt<T, okey_t<T>>(kt_for(a.n), grid, block, sh.lds, s, a, p);
-    EDA_LAUNCH("k_vhgw");
+    bool done = false;
+    if (sum && EDA_VHGW_PIPE) {
+      // the pipelined kernel, one launch per row layout: windows up to 32 samples on plain rows
+      // (radius <= 15), longer ones on padded rows
+      const bool longer = (max_w - 1) / 2 >= 16 && a.n > 16;
+      const Shape s1 = pass_shape(a, sizeof(T), true, std::min(max_w, 31), 1);
+      const VPass a1 = shaped(s1);
+      const int pf1 = box_reads<T>(a1, 64 * s1.W);
+      Shape s2{};
+      VPass a2{};
+      int pf2 = 4;
+      if (longer) {
+        s2 = pass_shape(a, sizeof(T), true, max_w, 2);
+        a2 = shaped(s2);
+        pf2 = box_reads<T>(a2, 64 * s2.W);
+      }
+      if (pf1 && pf2) {
+        launch_box<T, false>(pf1, s, a1, p, rungs, s1.W, s1.lds);
+        if (longer) launch_box<T, true>(pf2, s, a2, p, rungs, s2.W, s2.lds);
+        EDA_LAUNCH("k_box");
+        done = true;
+      }
+    }
+    if (!done) {
+      const Shape sh = pass_shape(a, sizeof(T), sum, max_w);
+      const VPass v = shaped(sh);
+      const int64_t wgs = g.batch * v.groups;
+      const dim3 grid(static_cast<unsigned>(wgs), static_cast<unsigned>(rungs));
+      const dim3 block(static_cast<unsigned>(64 * sh.W));
+      if (sum) launch_kt<T, T>(1, grid, block, sh.lds, s, v, p);
+      else launch_kt<T, okey_t<T>>(kt_for(v.n), grid, block, sh.lds, s, v, p);
+      EDA_LAUNCH("k_vhgw");
+    }
   }
   if (timed) {
     int64_t active = mirror_active(win, g.batch * rungs);

@@ -24,7 +24,7 @@ def bits_equal(a, b):
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
-@pytest.mark.parametrize("shape", [(1000,), (37, 70), (9, 10, 11), (512, 512)])
+@pytest.mark.parametrize("shape", [(1000,), (3000,), (37, 70), (9, 10, 11), (512, 512)])
 @pytest.mark.parametrize("op", ["max", "min", "box"])
 def test_cube_is_bitwise_the_oracle(dtype, shape, op):
     rng = np.random.default_rng(hash((shape, op)) % 2**32)
