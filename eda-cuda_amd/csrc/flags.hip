@@ -175,11 +175,255 @@ __global__ void __launch_bounds__(kThreads) k_flags(FlagLines L, int64_t batch, 
   }
 }
 
+// ---- the tiled pass: P lines of one axis staged in LDS -------------------------------------------------
+// The side walk of side_deep on a line in LDS (stride 1), eight samples a loop iteration: each
+// sample is tested branch-free in walk order (state 0 open, 1 deep, 2 ended not deep), so that a
+// wave's lanes diverge once per eight samples, not per sample. Reads past the line's ends are
+// clamped into it and their verdicts ignored.
+template <bool Max, class T>
+__device__ __forceinline__ int side_look(int st, T u, T v, double vd, double td) {
+  const bool ends = Max ? !(u <= v) : !(u >= v);
+  const bool dp = (Max ? vd - static_cast<double>(u) : static_cast<double>(u) - vd) >= td;
+  return st != 0 ? st : (ends ? 2 : (dp ? 1 : 0));
+}
+
+// Samples first, first + Dir, ... (cnt of them) in walk order; returns the state.
+template <bool Max, int Dir, class T>
+__device__ __forceinline__ int walk8(const T* __restrict__ x, int32_t n, int32_t first, int32_t cnt, T v, double vd,
+                                     double td) {
+  int st = 0;
+  for (int32_t j0 = 0; j0 < cnt; j0 += 8) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int32_t k = min(max(first + Dir * (j0 + u), 0), n - 1);
+      const int nst = side_look<Max>(st, x[k], v, vd, td);
+      st = j0 + u < cnt ? nst : st;
+    }
+    if (st != 0) break;
+  }
+  return st;
+}
+
+template <bool Max, int Dir, class T>
+__device__ bool side_deep_lds(const T* __restrict__ x, int32_t n, int32_t nb, int32_t p, T v, T tau,
+                              const T* __restrict__ bmin, const T* __restrict__ bmax) {
+  const double vd = static_cast<double>(v), td = static_cast<double>(tau);
+  const int32_t cp = p / kBlk;
+  const int32_t lim = Dir < 0 ? cp * kBlk : min(cp * kBlk + kBlk - 1, n - 1);
+  const int st = walk8<Max, Dir>(x, n, p + Dir, Dir < 0 ? p - lim : lim - p, v, vd, td);
+  if (st != 0) return st == 1;
+  for (int32_t c = cp + Dir; Dir < 0 ? c >= 0 : c < nb; c += Dir) {
+    const T hi = Max ? bmax[c] : bmin[c];
+    if (Max ? !(hi <= v) : !(hi >= v)) {
+      const int32_t k0 = Dir < 0 ? c * kBlk + kBlk - 1 : c * kBlk;
+      const int32_t k1 = Dir < 0 ? c * kBlk : min(c * kBlk + kBlk - 1, n - 1);
+      return walk8<Max, Dir>(x, n, k0, Dir < 0 ? k0 - k1 + 1 : k1 - k0 + 1, v, vd, td) == 1;
+    }
+    const T lo = Max ? bmin[c] : bmax[c];
+    if (Max ? deep<T>(v, lo, tau) : deep<T>(lo, v, tau)) return true;
+  }
+  return false;
+}
+
+// A workgroup stages P lines (P consecutive rows of the contiguous axis, or P adjacent columns of a
+// strided one) and their 32-sample block minima and maxima in LDS, finds and walks the peaks there
+// with the same steps as k_flags (one thread per sample), marks them in an LDS byte tile, and writes
+// the tile back over the flags (the first axis) or ORs it in (later axes), rows at a time.
+struct FlagTile {
+  int32_t n, nb, P, lpad;  // line length, blocks per line, lines per tile, LDS slots per line
+  int64_t step, inner, outer, groups, count;
+  int32_t axis, first, wide;  // first: the first axis pass (write, not OR); wide: 4-byte flag words
+};
+
+template <class T>
+__global__ void __launch_bounds__(kThreads) k_flags_tile(FlagTile a, const T* __restrict__ r,
+                                                         const T* __restrict__ tau, uint8_t* __restrict__ flags,
+                                                         uint64_t* __restrict__ ext, const int32_t* __restrict__ on) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int64_t b = blockIdx.x / a.groups;
+  const int64_t grp = blockIdx.x - b * a.groups;
+  if (on != nullptr && on[b] == 0) return;
+  const int32_t n = a.n, P = a.P, nb = a.nb;
+  T* X = reinterpret_cast<T*>(smem);  // line p at X + p lpad
+  T* BMN = X + P * a.lpad;            // block c of line p at p nb + c
+  T* BMX = BMN + P * nb;
+  uint8_t* F = reinterpret_cast<uint8_t*>(BMX + P * nb);  // contiguous: p n + i; strided: i P + p
+  const bool contiguous = a.inner == 1;
+  int64_t base0;
+  int32_t lines;
+  if (contiguous) {
+    const int64_t l0 = grp * P;
+    lines = static_cast<int32_t>(min<int64_t>(P, a.outer - l0));
+    base0 = b * a.count + l0 * n;
+  } else {
+    const int64_t gpo = (a.inner + P - 1) / P;
+    const int64_t o = grp / gpo;
+    const int64_t j0 = (grp - o * gpo) * P;
+    lines = static_cast<int32_t>(min<int64_t>(P, a.inner - j0));
+    base0 = b * a.count + o * static_cast<int64_t>(n) * a.inner + j0;
+  }
+  // sample i of line p in the field
+  const auto gpos = [&](int32_t p, int32_t i) {
+    return contiguous ? base0 + static_cast<int64_t>(p) * n + i : base0 + static_cast<int64_t>(i) * a.inner + p;
+  };
+  const auto fidx = [&](int32_t p, int32_t i) { return contiguous ? p * n + i : i * P + p; };
+  const int32_t total = lines * n;
+  const float inv_n = 1.0f / static_cast<float>(n);
+  const int pshift = __ffs(P) - 1;  // strided: P is a power of two
+  // stage: consecutive threads on consecutive samples of a row (coalesced either way)
+  for (int32_t idx = threadIdx.x; idx < P * n; idx += kThreads) {
+    int32_t p, i;
+    if (contiguous) {
+      p = static_cast<int32_t>(static_cast<float>(idx) * inv_n);
+      i = idx - p * n;
+      if (i < 0) { --p; i += n; } else if (i >= n) { ++p; i -= n; }
+    } else {
+      p = idx & (P - 1);
+      i = idx >> pshift;
+    }
+    if (p < lines) X[p * a.lpad + i] = r[gpos(p, i)];
+    F[fidx(p, i)] = 0;
+  }
+  __syncthreads();
+  for (int32_t idx = threadIdx.x; idx < lines * nb; idx += kThreads) {
+    const int32_t p = idx / nb;
+    const int32_t c = idx - p * nb;
+    const T* x = X + p * a.lpad;
+    const int32_t i0 = c * kBlk;
+    const int32_t i1 = min(i0 + kBlk, n);
+    T mn = x[i0];
+    T mx = mn;
+    for (int32_t i = i0 + 1; i < i1; ++i) {
+      const T v = x[i];
+      mn = (v < mn || v != v) ? v : mn;
+      mx = (v > mx || v != v) ? v : mx;
+    }
+    BMN[p * nb + c] = mn;
+    BMX[p * nb + c] = mx;
+  }
+  __syncthreads();
+  uint64_t count = 0;
+  const uint8_t bit = static_cast<uint8_t>(1u << a.axis);
+  for (int32_t idx = threadIdx.x; idx < total; idx += kThreads) {
+    int32_t p = static_cast<int32_t>(static_cast<float>(idx) * inv_n);
+    int32_t s = idx - p * n;
+    if (s < 0) { --p; s += n; } else if (s >= n) { ++p; s -= n; }
+    if (s < 1 || s > n - 2) continue;
+    const T* x = X + p * a.lpad;
+    const T cur = x[s];
+    const T prev = x[s - 1];
+    if (!(prev < cur || prev > cur)) continue;
+    const T* bmin = BMN + p * nb;
+    const T* bmax = BMX + p * nb;
+    const int32_t en = run_end(x, 1, n, nb, s, cur, bmin, bmax, 1);
+    if (en > n - 2) continue;
+    const T nxt = x[en + 1];
+    const bool is_max = prev < cur && nxt < cur;
+    const bool is_min = prev > cur && nxt > cur;
+    if (!is_max && !is_min) continue;
+    const int32_t pk = (s + en) / 2;
+    const T tp = tau[gpos(p, pk)];
+    const bool sig = is_max ? side_deep_lds<true, -1>(x, n, nb, pk, cur, tp, bmin, bmax) &&
+                                  side_deep_lds<true, 1>(x, n, nb, pk, cur, tp, bmin, bmax)
+                            : side_deep_lds<false, -1>(x, n, nb, pk, cur, tp, bmin, bmax) &&
+                                  side_deep_lds<false, 1>(x, n, nb, pk, cur, tp, bmin, bmax);
+    if (sig) {
+      F[fidx(p, pk)] = bit;
+      ++count;
+    }
+  }
+  __syncthreads();
+  // write-out: rows of the tile (a row: P adjacent columns, or a line), 4 flags a word where aligned
+  if (a.wide) {
+    const int32_t words = contiguous ? lines * n / 4 : n * P / 4;
+    for (int32_t w = threadIdx.x; w < words; w += kThreads) {
+      int64_t gp;
+      if (contiguous) {
+        gp = base0 + 4 * static_cast<int64_t>(w);  // lines are consecutive rows
+      } else {
+        const int32_t i = (4 * w) >> pshift;
+        const int32_t p = 4 * w - (i << pshift);
+        if (p >= lines) continue;
+        gp = gpos(p, i);
+      }
+      const uint32_t f = *reinterpret_cast<const uint32_t*>(F + 4 * w);
+      uint32_t* dst = reinterpret_cast<uint32_t*>(flags + gp);
+      *dst = a.first ? f : (*dst | f);
+    }
+  } else {
+    for (int32_t idx = threadIdx.x; idx < P * n; idx += kThreads) {
+      int32_t p, i;
+      if (contiguous) {
+        p = idx / n;
+        i = idx - p * n;
+      } else {
+        p = idx & (P - 1);
+        i = idx >> pshift;
+      }
+      if (p >= lines) continue;
+      const uint8_t f = F[fidx(p, i)];
+      uint8_t* dst = flags + gpos(p, i);
+      *dst = a.first ? f : static_cast<uint8_t>(*dst | f);
+    }
+  }
+  __shared__ uint64_t red[kThreads / 64];
+  count = wave_sum_u64(count);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = count;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t c = 0;
+    for (int w = 0; w < kThreads / 64; ++w) c += red[w];
+    if (c) atomicAdd(reinterpret_cast<unsigned long long*>(&ext[3 * b + a.axis]), static_cast<unsigned long long>(c));
+  }
+}
+
 }  // namespace
 
 template <class T>
 void enqueue_sig_flags(hipStream_t s, const Geom& g, const T* r, const T* tau, uint8_t* flags, uint64_t* ext,
                        const int32_t* on, T* scratch) {
+  // the tiled pass for lines up to 4096 samples: P lines a tile (a power of two on a strided axis)
+  bool first = true;
+  bool all_tiled = true;
+  for (int a = 0; a < g.dim; ++a) all_tiled = all_tiled && (g.n[a] < 3 || g.n[a] <= 4096);
+  if (all_tiled) {
+    for (int a = 0; a < g.dim; ++a) {
+      const int32_t n = g.n[a];
+      if (n < 3) continue;
+      FlagTile t{};
+      t.n = n;
+      t.nb = (n + kBlk - 1) / kBlk;
+      t.step = g.st[a];
+      t.inner = a == g.dim - 1 ? 1 : g.st[a];
+      t.outer = g.count / (static_cast<int64_t>(n) * t.inner);
+      t.count = g.count;
+      t.axis = a;
+      t.first = first ? 1 : 0;
+      t.lpad = n | 1;
+      int32_t P = 8;
+      if (t.inner > 1) {
+        while (P > 1 && P / 2 >= t.inner) P /= 2;
+      } else {
+        P = static_cast<int32_t>(std::min<int64_t>(P, t.outer));
+      }
+      const auto bytes = [&](int32_t q) {
+        return (static_cast<size_t>(q) * t.lpad + 2 * static_cast<size_t>(q) * t.nb) * sizeof(T) +
+               static_cast<size_t>(q) * n;
+      };
+      while (P > 1 && bytes(P) > 40000) P /= 2;
+      t.P = P;
+      t.groups = t.inner == 1 ? (t.outer + P - 1) / P : t.outer * ((t.inner + P - 1) / P);
+      const bool al4 = (reinterpret_cast<uintptr_t>(flags) & 3) == 0 && g.count % 4 == 0;
+      t.wide = al4 && (t.inner == 1 ? n % 4 == 0 : (P % 4 == 0 && t.inner % 4 == 0)) ? 1 : 0;
+      const int64_t wgs = g.batch * t.groups;
+      if (wgs > 0x7fffffff) throw std::invalid_argument("eda: too many workgroups for the extrema pass");
+      hipLaunchKernelGGL(k_flags_tile<T>, dim3(static_cast<unsigned>(wgs)), dim3(kThreads), bytes(P), s, t, r, tau,
+                         flags, ext, on);
+      EDA_LAUNCH("k_flags_tile");
+      first = false;
+    }
+    if (!first) return;
+  }
   each(s, g, on, [=] __device__(int64_t b, int64_t i) { flags[b * g.count + i] = 0; });
   for (int a = 0; a < g.dim; ++a) {
     const int32_t n = g.n[a];

@@ -104,7 +104,7 @@ def test_threshold_field_is_bitwise(dtype, shape):
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
-@pytest.mark.parametrize("shape", [(3000,), (70, 90), (20, 21, 22), (3, 40)])
+@pytest.mark.parametrize("shape", [(3000,), (5000,), (70, 90), (64, 512), (20, 21, 22), (3, 40)])
 @pytest.mark.parametrize("coarse", [False, True])
 def test_extrema_flags_are_exact(dtype, shape, coarse):
     rng = np.random.default_rng(4)
