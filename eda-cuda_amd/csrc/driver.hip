@@ -683,17 +683,19 @@ class Chunk {
   }
   // An operator of the element over every rung (the rung scratch grows to the mode's ladder).
   void elem_op(int op, const T* in, T* out0, T* out1) {
+    // the rung fields: r0, r1 (K_max fields each), then the max and min side by side (2 K_max)
     const size_t need = static_cast<size_t>(K_max_) * static_cast<size_t>(B_ * g_.count);
     if (need > rung_cap_) {
       EDA_HIP(hipStreamSynchronize(s_));
       rung_mem_ = DevMem();
-      rung_mem_ = DevMem(2 * need * sizeof(T));
+      rung_mem_ = DevMem(4 * need * sizeof(T));
       rung_cap_ = need;
     }
     T* r0 = static_cast<T*>(rung_mem_.p);
     T* r1 = r0 + rung_cap_;
+    T* pairs = r1 + rung_cap_;
     enqueue_element_op<T>(s_, g_, op, d_el_, lo_, t_, o_, d_rung_win_, rung_max_.data(), K_max_, in, out0, out1, r0, r1,
-                          on_op_);
+                          pairs, on_op_);
   }
 
   Context& ctx_;

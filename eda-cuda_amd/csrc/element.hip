@@ -174,6 +174,33 @@ struct Interp {
       off1[a] = reflect32(i0 + 1, n[a]) * st[a];
     }
   }
+  // two fields stored side by side (field k of sample v at f + 2 v + k), each interpolated as at()
+  // does, from one aligned read per corner
+  __device__ __forceinline__ void at2(const T* __restrict__ f, T& r0, T& r1) const {
+    struct alignas(2 * sizeof(T)) Pair {
+      T a, b;
+    };
+    const Pair* pf = reinterpret_cast<const Pair*>(f);
+    T acc0 = T(0), acc1 = T(0);
+#pragma unroll
+    for (int corner = 0; corner < (1 << D); ++corner) {
+      int32_t off = 0;
+#pragma unroll
+      for (int a = 0; a < D; ++a) off += ((corner >> (D - 1 - a)) & 1) ? off1[a] : off0[a];
+      const Pair q = pf[off];
+      T term0 = q.a, term1 = q.b;
+#pragma unroll
+      for (int a = 0; a < D; ++a) {
+        const T wa = ((corner >> (D - 1 - a)) & 1) ? w1[a] : w0[a];
+        term0 = term0 * wa;
+        term1 = term1 * wa;
+      }
+      acc0 = acc0 + term0;
+      acc1 = acc1 + term1;
+    }
+    r0 = acc0;
+    r1 = acc1;
+  }
   __device__ __forceinline__ T at(const T* __restrict__ f) const {
     T acc = T(0);
 #pragma unroll
@@ -219,7 +246,27 @@ __device__ __forceinline__ void rung_acc(int32_t i, const T* __restrict__ f0, co
   *a1 = acc1;
 }
 
-template <class T, int D, int OP>
+// Max and min of one rung read side by side (pr + 2 v + k): the same arithmetic as rung_acc for op 2.
+template <class T, int D>
+__device__ __forceinline__ void rung_acc_pairs(int32_t i, const T* __restrict__ pr, bool pts,
+                                               const Interp<T, D> (&ip)[4], T* a0, T* a1) {
+  T acc0 = pr[2 * i];
+  T acc1 = pr[2 * i + 1];
+  if (pts) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      T m0, m1;
+      ip[q].at2(pr, m0, m1);
+      acc0 = max_c(acc0, m0);
+      acc1 = min_c(acc1, m1);
+    }
+  }
+  *a0 = acc0;
+  *a1 = acc1;
+}
+
+// IL (op 2): the rung fields are the max and min side by side in r0 (rung j at r0 + 2 j F).
+template <class T, int D, int OP, bool IL = false>
 __global__ void __launch_bounds__(kThreads) k_gather_all(Geom g, const ElemDev* __restrict__ el,
                                                          const uint8_t* __restrict__ lo, const T* __restrict__ t,
                                                          const T* __restrict__ o, const T* __restrict__ r0,
@@ -284,14 +331,18 @@ __global__ void __launch_bounds__(kThreads) k_gather_all(Geom g, const ElemDev* 
     T v0 = T(0), v1 = T(0);
     if (use_lo) {
       T a0, a1;
-      rung_acc<T, D, OP>(i, r0 + l * F + base, OP == 2 ? r1 + l * F + base : nullptr, pts_lo, ip, &a0, &a1);
+      if constexpr (IL) rung_acc_pairs<T, D>(i, r0 + 2 * (l * F + base), pts_lo, ip, &a0, &a1);
+      else rung_acc<T, D, OP>(i, r0 + l * F + base, OP == 2 ? r1 + l * F + base : nullptr, pts_lo, ip, &a0, &a1);
       v0 = wl * a0;
       v1 = wl * a1;
     }
     if (use_up) {
       T a0, a1;
-      rung_acc<T, D, OP>(i, r0 + (l + 1) * F + base, OP == 2 ? r1 + (l + 1) * F + base : nullptr, pts_up, ip, &a0,
-                         &a1);
+      if constexpr (IL)
+        rung_acc_pairs<T, D>(i, r0 + 2 * ((l + 1) * F + base), pts_up, ip, &a0, &a1);
+      else
+        rung_acc<T, D, OP>(i, r0 + (l + 1) * F + base, OP == 2 ? r1 + (l + 1) * F + base : nullptr, pts_up, ip, &a0,
+                           &a1);
       const T u0 = tt * a0, u1 = tt * a1;
       v0 = use_lo ? v0 + u0 : u0;
       v1 = use_lo ? v1 + u1 : u1;
@@ -308,6 +359,16 @@ void launch_gather_all(hipStream_t s, const Geom& g, const ElemDev* el, const ui
   if (g.dim == 1) hipLaunchKernelGGL((k_gather_all<T, 1, OP>), grid, dim3(kThreads), 0, s, g, el, lo, t, o, r0, r1, out0, out1, on);
   else if (g.dim == 2) hipLaunchKernelGGL((k_gather_all<T, 2, OP>), grid, dim3(kThreads), 0, s, g, el, lo, t, o, r0, r1, out0, out1, on);
   else hipLaunchKernelGGL((k_gather_all<T, 3, OP>), grid, dim3(kThreads), 0, s, g, el, lo, t, o, r0, r1, out0, out1, on);
+  EDA_LAUNCH("k_gather_all");
+}
+
+// op 2 from the max and min side by side (D >= 2)
+template <class T>
+void launch_gather_pairs(hipStream_t s, const Geom& g, const ElemDev* el, const uint8_t* lo, const T* t, const T* o,
+                         const T* pairs, T* out0, T* out1, const int32_t* on) {
+  const dim3 grid = elem_grid(g);
+  if (g.dim == 2) hipLaunchKernelGGL((k_gather_all<T, 2, 2, true>), grid, dim3(kThreads), 0, s, g, el, lo, t, o, pairs, nullptr, out0, out1, on);
+  else hipLaunchKernelGGL((k_gather_all<T, 3, 2, true>), grid, dim3(kThreads), 0, s, g, el, lo, t, o, pairs, nullptr, out0, out1, on);
   EDA_LAUNCH("k_gather_all");
 }
 
@@ -332,7 +393,7 @@ void enqueue_element_weights(hipStream_t s, const Geom& g, const MethodCtl& c, c
 template <class T>
 void enqueue_element_op(hipStream_t s, const Geom& g, int op, const ElemDev* el, const uint8_t* lo, const T* t,
                         const T* o, const int32_t* rung_win, const int32_t* rung_max, int K_max, const T* in, T* out0,
-                        T* out1, T* rungs0, T* rungs1, const int32_t* on) {
+                        T* out1, T* rungs0, T* rungs1, T* pairs, const int32_t* on) {
   if (K_max < 1) return;
   int32_t mw = 0;
   for (int j = 0; j < K_max; ++j) mw = std::max(mw, rung_max[j]);
@@ -341,8 +402,10 @@ void enqueue_element_op(hipStream_t s, const Geom& g, int op, const ElemDev* el,
     enqueue_cube_rungs<T>(s, g, CubeOp::kBox, in, rungs0, nullptr, rung_win, K_max, F, mw);
     launch_gather_all<T, 3>(s, g, el, lo, t, o, rungs0, nullptr, out0, nullptr, on);
   } else if (op == 2) {
-    enqueue_cube_rungs<T>(s, g, CubeOp::kMaxMin, in, rungs0, rungs1, rung_win, K_max, F, mw);
-    launch_gather_all<T, 2>(s, g, el, lo, t, o, rungs0, rungs1, out0, out1, on);
+    if (enqueue_cube_rungs<T>(s, g, CubeOp::kMaxMin, in, rungs0, rungs1, rung_win, K_max, F, mw, pairs))
+      launch_gather_pairs<T>(s, g, el, lo, t, o, pairs, out0, out1, on);
+    else
+      launch_gather_all<T, 2>(s, g, el, lo, t, o, rungs0, rungs1, out0, out1, on);
   } else {
     enqueue_cube_rungs<T>(s, g, CubeOp::kMax, in, rungs0, nullptr, rung_win, K_max, F, mw);
     launch_gather_all<T, 0>(s, g, el, lo, t, o, rungs0, nullptr, out0, nullptr, on);
@@ -359,9 +422,9 @@ template void enqueue_element_weights<double>(hipStream_t, const Geom&, const Me
                                               const ElemDev*, uint8_t*, double*, const int32_t*, uint32_t*);
 template void enqueue_element_op<float>(hipStream_t, const Geom&, int, const ElemDev*, const uint8_t*, const float*,
                                         const float*, const int32_t*, const int32_t*, int, const float*, float*, float*,
-                                        float*, float*, const int32_t*);
+                                        float*, float*, float*, const int32_t*);
 template void enqueue_element_op<double>(hipStream_t, const Geom&, int, const ElemDev*, const uint8_t*, const double*,
                                          const double*, const int32_t*, const int32_t*, int, const double*, double*,
-                                         double*, double*, double*, const int32_t*);
+                                         double*, double*, double*, double*, const int32_t*);
 
 }  // namespace eda

@@ -47,10 +47,12 @@ template <class T>
 void enqueue_cube(hipStream_t s, const Geom& g, CubeOp op, const T* in, T* out0, T* out1, const int32_t* win,
                   int32_t max_w);
 // The cubes of `rungs` window arrays (win + j B) of one input in one launch per pass, rung j into
-// out0 + j stride (out1 + j stride); each rung's outputs are bitwise those of enqueue_cube.
+// out0 + j stride (out1 + j stride); each rung's outputs are bitwise those of enqueue_cube. With
+// pairs (kMaxMin), the last pass may write the max and min of a voxel side by side instead:
+// pairs + j 2 stride + 2 v (+ 1); returns whether it did (out0 / out1 then hold intermediates).
 template <class T>
-void enqueue_cube_rungs(hipStream_t s, const Geom& g, CubeOp op, const T* in, T* out0, T* out1, const int32_t* win,
-                        int32_t rungs, int64_t stride, int32_t max_w);
+bool enqueue_cube_rungs(hipStream_t s, const Geom& g, CubeOp op, const T* in, T* out0, T* out1, const int32_t* win,
+                        int32_t rungs, int64_t stride, int32_t max_w, T* pairs = nullptr);
 
 // ---- reductions (reduce.hip) ---------------------------------------------------------------------
 // Per element: energy[b] = the canonical pairwise sum of squares (double; nullptr: none),
@@ -148,10 +150,11 @@ void enqueue_element_weights(hipStream_t s, const Geom& g, const MethodCtl& c, c
 // rung_win [K_max * B]: rung j of element b at rung_win[j B + b] (0: unused); rung_max [K_max]:
 // the largest side of rung j over the batch (host). The cubes of every rung run in one launch per
 // pass into rungs0 (rungs1: the min cubes of op 2), K_max fields each; then one gather over all
-// rungs. on [B]: the elements that run (the others' outputs are left untouched).
+// rungs (op 2 reads the max and min side by side from pairs, 2 K_max fields, when the cube wrote
+// them there). on [B]: the elements that run (the others' outputs are left untouched).
 template <class T>
 void enqueue_element_op(hipStream_t s, const Geom& g, int op, const ElemDev* el, const uint8_t* lo, const T* t,
                         const T* o, const int32_t* rung_win, const int32_t* rung_max, int K_max, const T* in, T* out0,
-                        T* out1, T* rungs0, T* rungs1, const int32_t* on);
+                        T* out1, T* rungs0, T* rungs1, T* pairs, const int32_t* on);
 
 }  // namespace eda

@@ -286,6 +286,39 @@ __device__ __forceinline__ void mm_out(const S* __restrict__ Hb, const S* __rest
 }
 
 // mm_out into registers: y[j] = y(64 j + lane) (junk where that is n or more).
+// The max (H0, G0) and the min (H1, G1) of each sample side by side, as mm_out computes each.
+template <class S, class Emit2>
+__device__ __forceinline__ void mm_out_pair(const S* __restrict__ H0, const S* __restrict__ G0,
+                                            const S* __restrict__ H1, const S* __restrict__ G1, const MmCtl& c,
+                                            Emit2 emit) {
+  const int lane = threadIdx.x & 63;
+  const int32_t n = c.n, r = c.r;
+  const int32_t last = div_small(n - 1, c.L, c.inv) * c.L;  // the last block's start
+  for (int32_t x0 = 0; x0 < n; x0 += 64) {
+    const int32_t x = x0 + lane;
+    if (x0 >= r && x0 + 63 + r <= n - 1) {  // every window of the 64 lies inside the line
+      emit(x, mm_op<kVMax>(H0[pidx(x - r)], G0[pidx(x + r)]), mm_op<kVMin>(H1[pidx(x - r)], G1[pidx(x + r)]));
+    } else if (x < n) {
+      const int32_t lo = x - r > 0 ? x - r : 0;
+      const int32_t hi = x + r < n - 1 ? x + r : n - 1;
+      const S h0 = H0[pidx(lo)], g0 = G0[pidx(hi)];
+      const S h1 = H1[pidx(lo)], g1 = G1[pidx(hi)];
+      S y0, y1;
+      if (x < r) {
+        y0 = g0;
+        y1 = g1;
+      } else if (x + r > n - 1 && lo >= last) {
+        y0 = h0;
+        y1 = h1;
+      } else {
+        y0 = mm_op<kVMax>(h0, g0);
+        y1 = mm_op<kVMin>(h1, g1);
+      }
+      emit(x, y0, y1);
+    }
+  }
+}
+
 template <int OP, class S, int KT>
 __device__ __forceinline__ void mm_out_regs(const S* __restrict__ Hb, const S* __restrict__ Gb, const MmCtl& c,
                                             S (&y)[KT]) {
@@ -1067,6 +1100,7 @@ struct RPass {
   int32_t two_in;  // nf == 2 with two inputs (else field 1 folds the input of field 0)
   int32_t ops[2];
   int32_t vec;     // n % 4 == 0: 16-byte loads
+  int32_t il;      // nf == 2 (max, min): both outputs of sample v side by side at out[0] + 2 v
 };
 
 template <class T, int KT>
@@ -1084,7 +1118,7 @@ __global__ void __launch_bounds__(256) k_vhgw_rows(RPass a, VPtrs<T> ptr) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int W = blockDim.x >> 6;
-  S* Hb = reinterpret_cast<S*>(smem) + wave * 2 * kSlots;
+  S* Hb = reinterpret_cast<S*>(smem) + wave * (a.il ? 4 : 2) * kSlots;
   S* Gb = Hb + kSlots;
   const int32_t n = a.n;
   const int64_t B = a.g.batch;
@@ -1138,7 +1172,35 @@ __global__ void __launch_bounds__(256) k_vhgw_rows(RPass a, VPtrs<T> ptr) {
       const int64_t b = rb - rung * B;
       const int32_t half = (w_cur - 1) / 2;
       const int32_t r = half < n - 1 ? half : n - 1;
-      for (int f = 0; f < a.nf; ++f) {
+      if (a.il) {
+        // the max of field 0 and the min of field 1, written side by side
+        S* H1 = Gb + kSlots;
+        S* G1 = H1 + kSlots;
+        const MmCtl c = mm_ctl<KT>(n, r);
+        S v0[KT], v1[KT];
+#pragma unroll
+        for (int k = 0; k < KT; ++k) {
+          v0[k] = okey(i0 + k < n ? cur.v[0][k] : cur.tail[0]);
+          v1[k] = okey(i0 + k < n ? (a.two_in ? cur.v[1][k] : cur.v[0][k]) : (a.two_in ? cur.tail[1] : cur.tail[0]));
+        }
+        mm_tile_h<kVMax, S, KT>(v0, i0, mm_ident<kVMax, S>(), Hb, c);
+        mm_tile_g<kVMax, S, KT>(v0, i0, mm_ident<kVMax, S>(), Gb, c);
+        mm_tile_h<kVMin, S, KT>(v1, i0, mm_ident<kVMin, S>(), H1, c);
+        mm_tile_g<kVMin, S, KT>(v1, i0, mm_ident<kVMin, S>(), G1, c);
+        wave_sync();
+        struct alignas(2 * sizeof(T)) Pair {
+          T a, b;
+        };
+        Pair* dst = reinterpret_cast<Pair*>(ptr.out[0] + rung * ptr.out_stride) + b * a.g.count + line * n;
+        mm_out_pair<S>(Hb, Gb, H1, G1, c, [&](int32_t xo, S y0, S y1) {
+          Pair q;
+          q.a = okey_inv(y0);
+          q.b = okey_inv(y1);
+          dst[xo] = q;
+        });
+        wave_sync();
+      }
+      for (int f = 0; f < (a.il ? 0 : a.nf); ++f) {
         T* dst = ptr.out[f] + rung * ptr.out_stride + b * a.g.count + line * n;
         // the field's keys, selected without indexing the register arrays at run time; samples past
         // the line's end repeat x[n-1]
@@ -1269,7 +1331,7 @@ void launch_kt(int kt, dim3 grid, dim3 block, size_t lds, hipStream_t s, const V
 template <class T, class S, int KT>
 void go_rows(hipStream_t s, int64_t items, const RPass& r, const VPtrs<T>& p) {
   constexpr size_t slots = 64 * KT + 2 * KT;
-  const size_t per_wave = 2 * slots * sizeof(S);
+  const size_t per_wave = (r.il ? 4 : 2) * slots * sizeof(S);
   int W = 4;
   while (W > 1 && W * per_wave > 64 * 1024) W /= 2;
   const size_t lds = W * per_wave;
@@ -1331,7 +1393,7 @@ bool rows_qualify(const Geom& g, int axis, bool sum) {
 }
 template <class T>
 void launch_rows(hipStream_t s, const Geom& g, int axis, int nf, bool same_input, const int* ops, const VPtrs<T>& p,
-                 int32_t rungs) {
+                 int32_t rungs, bool il) {
   const int32_t n = g.n[axis];
   RPass r{};
   r.g = g;
@@ -1342,6 +1404,7 @@ void launch_rows(hipStream_t s, const Geom& g, int axis, int nf, bool same_input
   r.two_in = nf == 2 && !same_input ? 1 : 0;
   r.ops[0] = ops[0];
   r.ops[1] = nf > 1 ? ops[1] : ops[0];
+  r.il = il && nf == 2 && r.ops[0] == kVMax && r.ops[1] == kVMin ? 1 : 0;
   const auto al16 = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; };
   r.vec = n % 4 == 0 && al16(p.in[0]) && al16(p.in[1]) && (p.in_stride * sizeof(T)) % 16 == 0 ? 1 : 0;
   const int32_t per = (n + 63) / 64;
@@ -1362,7 +1425,7 @@ void launch_rows(hipStream_t s, const Geom& g, int axis, int nf, bool same_input
 template <class T>
 void enqueue_vhgw(hipStream_t s, const Geom& g, int axis, int nf, bool same_input, const int* ops, bool scale,
                   const T* const* in, T* const* out, const int32_t* win, int32_t max_w, int32_t rungs,
-                  int64_t in_stride, int64_t out_stride) {
+                  int64_t in_stride, int64_t out_stride, T* pairs = nullptr) {
   if (max_w < 1 || rungs < 1) return;
   if (rungs > 65535) throw std::invalid_argument("eda: too many rungs for one launch");
   VPtrs<T> p{};
@@ -1374,11 +1437,17 @@ void enqueue_vhgw(hipStream_t s, const Geom& g, int axis, int nf, bool same_inpu
   p.in_stride = in_stride;
   p.out_stride = out_stride;
   const bool sum = ops[0] == kVSum;
+  const bool il = pairs != nullptr && nf == 2 && !sum && rows_qualify(g, axis, sum);  // the row pass writes pairs
+  if (il) {
+    p.out[0] = pairs;
+    p.out[1] = pairs;
+    p.out_stride = 2 * out_stride;
+  }
   hipEvent_t t0 = nullptr;
   const bool timed = timer_on();
   if (rows_qualify(g, axis, sum)) {
     if (timed) timer_note(s, &t0);
-    launch_rows<T>(s, g, axis, nf, same_input, ops, p, rungs);
+    launch_rows<T>(s, g, axis, nf, same_input, ops, p, rungs, il);
     EDA_LAUNCH("k_vhgw_rows");
   } else {
     VPass a{};
@@ -1468,9 +1537,9 @@ void enqueue_vhgw(hipStream_t s, const Geom& g, int axis, int nf, bool same_inpu
 // rungs > 1 every pass runs all rungs in one launch: rung j filters in with the windows win + j B
 // into out0 + j stride (out1 + j stride).
 template <class T>
-void enqueue_cube_rungs(hipStream_t s, const Geom& g, CubeOp op, const T* in, T* out0, T* out1, const int32_t* win,
-                        int32_t rungs, int64_t stride, int32_t max_w) {
-  if (max_w < 1) return;
+bool enqueue_cube_rungs(hipStream_t s, const Geom& g, CubeOp op, const T* in, T* out0, T* out1, const int32_t* win,
+                        int32_t rungs, int64_t stride, int32_t max_w, T* pairs) {
+  if (max_w < 1) return false;
   const int D = g.dim;
   if (op == CubeOp::kBox) {
     for (int a = 0; a < D; ++a) {
@@ -1479,16 +1548,21 @@ void enqueue_cube_rungs(hipStream_t s, const Geom& g, CubeOp op, const T* in, T*
       T* dst[1] = {out0};
       enqueue_vhgw<T>(s, g, a, 1, true, ops, a == D - 1, src, dst, win, max_w, rungs, a == 0 ? 0 : stride, stride);
     }
-    return;
+    return false;
   }
-  for (int a = D - 1; a >= 0; --a) {
-    const bool first = a == D - 1;
+  // The max and min side by side: the streaming row pass writes them, so it runs last (axes 0 ..
+  // D - 1; max and min give the same bits in any order of the axes).
+  const bool il = pairs != nullptr && op == CubeOp::kMaxMin && D >= 2 && rows_qualify(g, D - 1, false);
+  for (int k = 0; k < D; ++k) {
+    const int a = il ? k : D - 1 - k;
+    const bool first = k == 0;
     const int64_t ins = first ? 0 : stride;
     if (op == CubeOp::kMaxMin) {
       const int ops[2] = {kVMax, kVMin};
       const T* src[2] = {first ? in : out0, first ? in : out1};
       T* dst[2] = {out0, out1};
-      enqueue_vhgw<T>(s, g, a, 2, first, ops, false, src, dst, win, max_w, rungs, ins, stride);
+      enqueue_vhgw<T>(s, g, a, 2, first, ops, false, src, dst, win, max_w, rungs, ins, stride,
+                      il && k == D - 1 ? pairs : nullptr);
     } else {
       const int ops[1] = {op == CubeOp::kMax ? kVMax : kVMin};
       const T* src[1] = {first ? in : out0};
@@ -1496,6 +1570,7 @@ void enqueue_cube_rungs(hipStream_t s, const Geom& g, CubeOp op, const T* in, T*
       enqueue_vhgw<T>(s, g, a, 1, true, ops, false, src, dst, win, max_w, rungs, ins, stride);
     }
   }
+  return il;
 }
 
 template <class T>
@@ -1507,9 +1582,9 @@ void enqueue_cube(hipStream_t s, const Geom& g, CubeOp op, const T* in, T* out0,
 template void enqueue_cube<float>(hipStream_t, const Geom&, CubeOp, const float*, float*, float*, const int32_t*, int32_t);
 template void enqueue_cube<double>(hipStream_t, const Geom&, CubeOp, const double*, double*, double*, const int32_t*,
                                    int32_t);
-template void enqueue_cube_rungs<float>(hipStream_t, const Geom&, CubeOp, const float*, float*, float*, const int32_t*,
-                                        int32_t, int64_t, int32_t);
-template void enqueue_cube_rungs<double>(hipStream_t, const Geom&, CubeOp, const double*, double*, double*,
-                                         const int32_t*, int32_t, int64_t, int32_t);
+template bool enqueue_cube_rungs<float>(hipStream_t, const Geom&, CubeOp, const float*, float*, float*, const int32_t*,
+                                        int32_t, int64_t, int32_t, float*);
+template bool enqueue_cube_rungs<double>(hipStream_t, const Geom&, CubeOp, const double*, double*, double*,
+                                         const int32_t*, int32_t, int64_t, int32_t, double*);
 
 }  // namespace eda
