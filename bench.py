@@ -8,11 +8,12 @@ into a device tensor). W untimed warm-up steps, then K timed steps bracketed by 
 synchronization. Prints one JSON line on stdout (progress goes to stderr):
 
   value      Msamples/s = 64 * 512 * 512 / (seconds per step)
-  roofline   the dominant kernel, the cube filter's vHGW pass (k_vhgw): algorithmic bytes per
-             launch (its input field(s) read once, output field(s) written once, over the running
-             elements) / its average launch duration, both measured live with HIP events on the
-             library's stream over the timed steps, against 8 TB/s; `traffic` the HBM bytes per
-             launch from rocprofv3 FETCH_SIZE + WRITE_SIZE (profiles/), null when not measured
+  roofline   the dominant kernel, the cube filter's vHGW pass (one pass = one launch of k_vhgw_rows
+             or k_vhgw, or the one or two launches of k_box of a box-sum pass): algorithmic bytes
+             per pass (its input field(s) read once, output field(s) written once, over the running
+             elements) / its average duration, both measured live with HIP events on the library's
+             stream over the timed steps, against 8 TB/s; `traffic` the HBM bytes per pass from
+             rocprofv3 FETCH_SIZE + WRITE_SIZE (profiles/), null when not measured
   cpu_baseline  the oracle (a C++ restatement of the method, FP64, one image per host thread)
              on a bounded sample of the same workload, on this host's cores
 
@@ -45,13 +46,13 @@ def log(*a):
 
 
 def traffic_from_profiles():
-    """HBM bytes per k_vhgw launch measured by rocprofv3 (profiles/vhgw_pass_64x512x512_f32.json),
-    or None."""
+    """HBM bytes per vHGW pass measured by rocprofv3 (profiles/vhgw_pass_64x512x512_f32.json), or
+    None."""
     p = ROOT / "profiles" / "vhgw_pass_64x512x512_f32.json"
     if not p.exists():
         return None
     try:
-        return json.loads(p.read_text()).get("traffic_bytes_per_launch")
+        return json.loads(p.read_text()).get("traffic_bytes_per_pass")
     except Exception:
         return None
 
@@ -70,22 +71,26 @@ def _counter_rows(directory):
 
 
 def pmc_summary(fetch_dir, write_dir, out_json):
-    """HBM traffic per k_vhgw launch from two rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE,
-    KiB per dispatch) of `bench.py --steps 1 --warmup 0 --no-cpu --copy-probe`, each scaled by
-    what the counter reads for the probe's copy kernel (torch.neg over 1 GiB: 1 GiB read, 1 GiB
-    written)."""
+    """HBM traffic per vHGW pass from two rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE, KiB
+    per dispatch) of `bench.py --steps 1 --warmup 0 --no-cpu --copy-probe`, each scaled by what the
+    counter reads for the probe's copy kernel (torch.neg over 1 GiB: 1 GiB read, 1 GiB written).
+    The pass kernels' KiB are summed and divided by the number of passes: every k_vhgw_rows or
+    k_vhgw dispatch is a pass, and a box-sum pass is one k_box<..., false> dispatch and at most one
+    k_box<..., true>."""
     def averages(directory, counter):
-        vh, cp = [], []
+        total, passes, cp = 0.0, 0, []
         for name, cname, value in _counter_rows(directory):
             if cname != counter:
                 continue
-            if "k_vhgw" in name:
-                vh.append(value)
+            if "k_vhgw" in name or "k_box" in name:
+                total += value
+                if "k_vhgw" in name or "false>" in name:
+                    passes += 1
             elif "neg" in name:
                 cp.append(value)
-        if not vh or not cp:
-            raise SystemExit(f"bench: no {counter} rows for k_vhgw and the copy probe under {directory}")
-        return sum(vh) / len(vh), max(cp), len(vh)
+        if not passes or not cp:
+            raise SystemExit(f"bench: no {counter} rows for the vHGW pass and the copy probe under {directory}")
+        return total / passes, max(cp), passes
 
     f_vh, f_cp, n_f = averages(fetch_dir, "FETCH_SIZE")
     w_vh, w_cp, n_w = averages(write_dir, "WRITE_SIZE")
@@ -93,16 +98,16 @@ def pmc_summary(fetch_dir, write_dir, out_json):
     w_scale = COPY_PROBE_BYTES / (w_cp * 1024.0)
     traffic = f_vh * 1024.0 * f_scale + w_vh * 1024.0 * w_scale
     res = {
-        "kernel": "k_vhgw",
+        "kernel": "vHGW pass (k_vhgw_rows, k_vhgw, k_box)",
         "workload": "eda.decompose fwd, 64 x 512 x 512 float32 (one call)",
-        "launches": n_f,
-        "fetch_kib_per_launch": round(f_vh, 1),
-        "write_kib_per_launch": round(w_vh, 1),
+        "passes": n_f,
+        "fetch_kib_per_pass": round(f_vh, 1),
+        "write_kib_per_pass": round(w_vh, 1),
         "copy_probe_fetch_kib": round(f_cp, 1),
         "copy_probe_write_kib": round(w_cp, 1),
         "fetch_scale": round(f_scale, 4),
         "write_scale": round(w_scale, 4),
-        "traffic_bytes_per_launch": round(traffic),
+        "traffic_bytes_per_pass": round(traffic),
     }
     Path(out_json).write_text(json.dumps(res, indent=1) + "\n")
     print(json.dumps(res))
@@ -209,7 +214,7 @@ def main() -> int:
     _native.check(lib.eda_timer_read(ctypes.byref(ms), ctypes.byref(launches), ctypes.byref(nbytes), err, len(err)),
                   err)
     lib.eda_timer_enable(0)
-    log(f"bench: {args.steps} steps, {dt * 1e3:.1f} ms per step; k_vhgw: {launches.value} launches, "
+    log(f"bench: {args.steps} steps, {dt * 1e3:.1f} ms per step; vHGW pass: {launches.value} passes, "
         f"{ms.value:.1f} ms in total")
 
     if args.copy_probe:
@@ -230,15 +235,15 @@ def main() -> int:
     achieved = per_launch_bytes / avg_s / 1e9 if avg_s > 0 else 0.0
     roofline = {
         "bound": "hbm",
-        "kernel": "k_vhgw",
+        "kernel": "vHGW pass (k_vhgw_rows, k_vhgw, k_box)",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": traffic_from_profiles(),
-        "algorithmic_bytes_per_launch": round(per_launch_bytes),
-        "avg_launch_us": round(avg_s * 1e6, 2),
-        "launches_per_step": launches.value // max(args.steps, 1),
+        "algorithmic_bytes_per_pass": round(per_launch_bytes),
+        "avg_pass_us": round(avg_s * 1e6, 2),
+        "passes_per_step": launches.value // max(args.steps, 1),
         "share_of_step": round(ms.value * 1e-3 / max(args.steps, 1) / dt, 4),
     }
     out = {
