@@ -386,7 +386,9 @@ struct Run {
   int64_t batch = 0, count = 0;
   int32_t cap = 0;
   int64_t chunk = 0;
-  DevMem rows;  // (batch, cap, count) in the compute dtype
+  DevMem rows;    // (batch, cap, count) in the compute dtype
+  DevMem orient;  // with the orientation kept: (batch, cap - 1, dim + 1, count), n then c of each mode
+  bool kept_orient = false;
   std::vector<int32_t> n_imfs, error, windows, sifts;
   std::vector<double> spacing, energy;
   int32_t P = 1, K = 0;
@@ -446,9 +448,9 @@ template <class T>
 class Chunk {
  public:
   Chunk(Context& ctx, const Geom& g, const MethodCtl& c, double tau, bool check_finite, int32_t max_imf, T* rows,
-        int32_t cap)
+        int32_t cap, T* orient)
       : ctx_(ctx), g_(g), c_(c), s_(ctx.stream), tau_(tau), check_finite_(check_finite), max_imf_(max_imf),
-        rows_(rows), cap_(cap), B_(g.batch) {
+        rows_(rows), cap_(cap), orient_(orient), B_(g.batch) {
     const int64_t F = g.batch * g.count;
     const int D = g.dim;
     const int K = D * (D + 1) / 2;
@@ -627,6 +629,36 @@ class Chunk {
     }
   }
 
+  // return_orientation: row n_imfs (the row of the mode this element commits) of orient_ (B, cap - 1,
+  // D + 1, count) <- n with the canonical sign (components 0 .. D - 1) and c (component D) of the
+  // active elements (reference driver.cpp:553-559, orientation.cu:495-522). The canonical sign
+  // negates n when its first component, in axis order, of magnitude above the dtype's default tie
+  // tolerance (2^-14 float32, 2^-30 float64) is negative (orientation.cu:405-417).
+  void store_orientation(const int32_t* on) {
+    std::vector<int32_t> kv(B_, 0);
+    for (int64_t b = 0; b < B_; ++b)
+      if (rec_[b].status == 0) kv[b] = rec_[b].n_imfs;
+    const int32_t* drow = ctx_.up.put(kv);
+    const Geom g = g_;
+    const int D = g.dim;
+    const int64_t B = B_, rows = cap_ - 1;
+    const T zero = std::is_same_v<T, float> ? T(0x1p-14) : T(0x1p-30);
+    const T* nv = nvec_;
+    const T* cv = coh_;
+    T* dst = orient_;
+    each(s_, g, on, [=] __device__(int64_t b, int64_t i) {
+      T v[3];
+      int sgn = 0;
+      for (int a = 0; a < D; ++a) {
+        v[a] = nv[(a * B + b) * g.count + i];
+        if (sgn == 0 && fabs(v[a]) > zero) sgn = v[a] > T(0) ? 1 : -1;
+      }
+      T* o = dst + (b * rows + drow[b]) * (D + 1) * g.count + i;
+      for (int a = 0; a < D; ++a) o[a * g.count] = sgn < 0 ? -v[a] : v[a];
+      o[D * g.count] = cv[b * g.count + i];
+    });
+  }
+
   void alloc_history() {
     hist_w_.assign(B_, std::vector<int32_t>(cap_ - 1, 0));
     hist_s_.assign(B_, std::vector<int32_t>(cap_ - 1, 0));
@@ -707,6 +739,7 @@ class Chunk {
   int32_t max_imf_;
   T* rows_;
   int32_t cap_;
+  T* orient_;  // nullptr: the orientation is not kept
   int64_t B_;
   int nfields_ = 0;
   DevMem mem_;
@@ -1224,6 +1257,7 @@ bool Chunk<T>::mode() {
     const int32_t* dw = ctx_.up.put(wv);
     EDA_HIP(hipMemsetAsync(d_nonfinite_, 0, sizeof(int32_t) * B, s_));
     enqueue_orientation<T>(s_, ctx_.device, g_, r_, dw, on, nvec_, coh_, owork_, d_nonfinite_, d_oscr_);
+    if (orient_ != nullptr) store_orientation(on);
   }
   {
     std::vector<uint64_t> init(2 * B);
@@ -1556,6 +1590,10 @@ void run_call(const eda_array_desc& x, const eda_params& p, Run& run) {
   run.energy.assign(he, 0.0);
   run.n_imfs.assign(B, 0);
   run.error.assign(B, 0);
+  if (p.orientation && x.dim >= 2) {
+    run.orient = DevMem(static_cast<size_t>(B) * (run.cap - 1) * (x.dim + 1) * gall.count * sizeof(T));
+    run.kept_orient = true;
+  }
   // the chunk: bounded by max_chunk and by the free device memory (halving)
   int64_t Bc = std::min<int64_t>(B, 65535);  // the kernels put the elements of a chunk on grid.y
   if (p.max_chunk > 0) Bc = std::min<int64_t>(Bc, p.max_chunk);
@@ -1568,7 +1606,10 @@ void run_call(const eda_array_desc& x, const eda_params& p, Run& run) {
     const int64_t nb = std::min(Bc, B - b0);
     const Geom g = make_geom(x.dim, nb, x.n);
     T* rows = static_cast<T*>(run.rows.p) + static_cast<size_t>(b0) * run.cap * g.count;
-    Chunk<T> ch(ctx, g, c, tau, p.check_finite != 0, p.max_imf, rows, run.cap);
+    T* orient = run.kept_orient ? static_cast<T*>(run.orient.p) +
+                                      static_cast<size_t>(b0) * (run.cap - 1) * (x.dim + 1) * g.count
+                                : nullptr;
+    Chunk<T> ch(ctx, g, c, tau, p.check_finite != 0, p.max_imf, rows, run.cap, orient);
     ch.alloc_history();
     if (x.device < 0) {
       ch.start(static_cast<const T*>(x.data) + b0 * g.count, false, nullptr);
@@ -1652,7 +1693,6 @@ void validate(const eda_array_desc& x, const eda_params& p) {
   if (x.batch < 0) throw std::invalid_argument("eda: the batch must be >= 0");
   if (p.sifts < 1) throw std::invalid_argument("eda: sifts must be >= 1");
   if (p.max_chunk < 0) throw std::invalid_argument("eda: max_chunk must be >= 0");
-  if (p.orientation) throw std::invalid_argument("eda: return_orientation is not supported by this build");
   if (p.tie_tolerance >= 0 && !std::isfinite(p.tie_tolerance))
     throw std::invalid_argument("eda: tie_tolerance must be a finite value >= 0");
   if (x.batch > 0 && x.data == nullptr) throw std::invalid_argument("eda: the input has no data pointer");
@@ -1690,6 +1730,47 @@ void collect_rows(const Run& run, void* dst, int32_t dst_device) {
     out[idx] = v;
   });
   if (dst_device < 0) EDA_HIP(hipMemcpyAsync(dst, out, elems * sizeof(T), hipMemcpyDeviceToHost, ctx.stream));
+  EDA_HIP(hipStreamSynchronize(ctx.stream));
+  ctx.up.reset();
+}
+
+// dir (B, K, D, count) and coh (B, K, count) <- rows 0 .. n_imfs[b] - 1 of the kept orientation; 0
+// above them and for failed elements (reference orientation.cu:524-549, driver.cpp:177-196).
+template <class T>
+void collect_orientation(const Run& run, void* dir, void* coh, int32_t dst_device, int32_t K) {
+  Context& ctx = context(run.device);
+  const int64_t B = run.batch, count = run.count;
+  const int D = run.dim;
+  const int64_t rows = run.cap - 1;
+  const size_t nd = static_cast<size_t>(B) * K * D * count, nc = static_cast<size_t>(B) * K * count;
+  DevMem tmp;
+  T *od, *oc;
+  if (dst_device >= 0) {
+    od = static_cast<T*>(dir);
+    oc = static_cast<T*>(coh);
+  } else {
+    tmp = DevMem((nd + nc) * sizeof(T));
+    od = static_cast<T*>(tmp.p);
+    oc = od + nd;
+  }
+  const int32_t* dn = ctx.up.put(run.n_imfs);
+  const int32_t* de = ctx.up.put(run.error);
+  const T* src = static_cast<const T*>(run.orient.p);
+  flat(ctx.stream, static_cast<int64_t>(nd + nc), [=] __device__(int64_t idx) {
+    const int64_t i = idx % count;
+    const int64_t q = idx / count;
+    const int64_t a = q % (D + 1);
+    const int64_t bk = q / (D + 1);
+    const int64_t k = bk % K;
+    const int64_t b = bk / K;
+    const T v = de[b] == 0 && k < dn[b] ? src[((b * rows + k) * (D + 1) + a) * count + i] : T(0);
+    if (a < D) od[(bk * D + a) * count + i] = v;
+    else oc[bk * count + i] = v;
+  });
+  if (dst_device < 0) {
+    EDA_HIP(hipMemcpyAsync(dir, od, nd * sizeof(T), hipMemcpyDeviceToHost, ctx.stream));
+    EDA_HIP(hipMemcpyAsync(coh, oc, nc * sizeof(T), hipMemcpyDeviceToHost, ctx.stream));
+  }
   EDA_HIP(hipStreamSynchronize(ctx.stream));
   ctx.up.reset();
 }
@@ -1780,6 +1861,19 @@ int eda_collect(const eda_result* r, void* dst, int32_t dst_device, char* err, s
     if (dst == nullptr) throw std::invalid_argument("eda: collect: null destination");
     if (run.dtype == EDA_F32) collect_rows<float>(run, dst, dst_device);
     else collect_rows<double>(run, dst, dst_device);
+  });
+}
+
+int eda_collect_orientation(const eda_result* r, void* dir, void* coh, int32_t dst_device, char* err,
+                            size_t errlen) {
+  return guarded(err, errlen, [&] {
+    if (r == nullptr || r->handle == nullptr) throw std::invalid_argument("eda: collect: no result");
+    const Run& run = *static_cast<const Run*>(r->handle);
+    if (run.batch > 0 && !run.kept_orient) throw std::logic_error("eda: the call kept no orientation");
+    if (run.batch == 0 || run.K == 0) return;
+    if (dir == nullptr || coh == nullptr) throw std::invalid_argument("eda: collect: null destination");
+    if (run.dtype == EDA_F32) collect_orientation<float>(run, dir, coh, dst_device, run.K);
+    else collect_orientation<double>(run, dir, coh, dst_device, run.K);
   });
 }
 

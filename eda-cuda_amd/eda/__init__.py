@@ -14,7 +14,7 @@ import operator
 from typing import Any
 
 from . import _native
-from ._api import DecompositionInfo, decompose
+from ._api import DecompositionInfo, Orientation, decompose
 from ._errors import CudaError, DecompositionError
 
 __version__ = "0.1.0"
@@ -23,6 +23,7 @@ __all__ = [
     "CudaError",
     "DecompositionError",
     "DecompositionInfo",
+    "Orientation",
     "__version__",
     "build_info",
     "decompose",

@@ -55,6 +55,26 @@ class DecompositionInfo:
     axes: tuple[int, ...]
 
 
+@dataclass(frozen=True, eq=False)
+class Orientation:
+    """The orientation of every mode, from ``eda.decompose(..., return_orientation=True)`` (the
+    reference's src/eda/_api.py:64-83). With ``m = min(axes)``, ``K`` the largest IMF count of the
+    call and ``D`` the number of spatial axes (2 or 3):
+
+    - ``direction``: shape ``x.shape[:m] + (K, D) + x.shape[m:]``, the unit direction ``n`` across
+      the stripes that the method computes at the start of each mode; component ``a`` is along the
+      ``a``-th axis of ``axes``; its first component of magnitude above the dtype's default tie
+      tolerance is positive.
+    - ``coherence``: shape ``x.shape[:m] + (K,) + x.shape[m:]``, from 0 (no direction) to 1
+      (perfect stripes).
+
+    Rows beyond an element's ``n_imfs``, and all rows of a failed element, are 0. NumPy arrays for
+    host input, PyTorch tensors on the input's device for ROCm input, in the compute dtype."""
+
+    direction: Any
+    coherence: Any
+
+
 # ----------------------------------------------------------------------------------------------
 # Arguments (reference src/eda/_api.py:433-526)
 
@@ -204,13 +224,13 @@ def _device_tensor(x: Any):
 
 
 def _call_native(desc: _native.ArrayDesc, max_imf: int, tie: float | None, sifts: int, check_finite: bool,
-                 stream: int) -> _native.Result:
+                 stream: int, orientation: bool = False) -> _native.Result:
     lib = _native.core()
     p = _native.Params()
     p.max_imf = max_imf
     p.sifts = sifts
     p.check_finite = 1 if check_finite else 0
-    p.orientation = 0
+    p.orientation = 1 if orientation else 0
     p.tie_tolerance = -1.0 if tie is None else tie
     p.max_chunk = 0
     p.stream = stream
@@ -265,8 +285,10 @@ def decompose(
     element is decomposed on its own; along the mode axis each element holds its IMFs from fine to
     coarse, then its residue, then zero rows; ``P`` is one more than the largest IMF count over the
     elements without an error. Every spatial extent must be at least 2 and at most 8192 (float32)
-    or 4096 (float64). ``return_orientation=True`` is not supported by this build and raises
-    ``ValueError``.
+    or 4096 (float64). ``return_orientation=True`` (2 or 3 spatial axes) also returns an
+    ``Orientation``: the direction and the coherence that the method computes at the start of each
+    mode. The result is ``y``, then ``info`` if ``return_info``, then the ``Orientation`` if
+    ``return_orientation``; asking for the orientation does not change ``y`` or ``info``.
 
     Host input returns NumPy arrays; a ROCm device array (a PyTorch-ROCm tensor, or a DLPack
     producer on a ROCm device) returns a PyTorch tensor on the same device. The call waits for the
@@ -304,13 +326,11 @@ def decompose(
         raise TypeError(f"eda: dtype= must be float32 or float64, got {target}")
     ndim = len(shape)
     axes = _normalize_axes(axes, ndim)
-    if return_orientation:
-        if len(axes) < 2:
-            raise ValueError(
-                "eda: return_orientation=True needs 2 or 3 spatial axes, got 1 (the method has no "
-                "orientation in 1-D)"
-            )
-        raise ValueError("eda: return_orientation=True is not supported by this build")
+    if return_orientation and len(axes) < 2:
+        raise ValueError(
+            "eda: return_orientation=True needs 2 or 3 spatial axes, got 1 (the method has no "
+            "orientation in 1-D)"
+        )
     batch_axes = tuple(i for i in range(ndim) if i not in axes)
     spatial = tuple(shape[a] for a in axes)
     batch_shape = tuple(shape[i] for i in batch_axes)
@@ -359,7 +379,7 @@ def decompose(
         desc.data = xb.ctypes.data if xb.size else None
         desc.device = -1
 
-    res = _call_native(desc, max_imf, tie, sifts, check_finite, stream_handle)
+    res = _call_native(desc, max_imf, tie, sifts, check_finite, stream_handle, return_orientation)
     try:
         info = _info_of(res, batch_shape, target, axes)
         codes = info.error.reshape(-1)
@@ -392,9 +412,49 @@ def decompose(
                                               len(err)), err)
             computed = out.reshape(batch_shape + (P,) + spatial)
             y = computed.transpose(out_final) if out_final != list(range(ndim + 1)) else computed
+        orient = None
+        if return_orientation:
+            orient = _orientation(res, batch_shape, spatial, ndim, batch_axes, axes, target,
+                                  keep_alive.device if on_device else None)
     finally:
         _native.core().eda_release(ctypes.byref(res))
     del keep_alive
+    out_t: tuple = (y,)
     if return_info:
-        return y, info
-    return y
+        out_t += (info,)
+    if return_orientation:
+        out_t += (orient,)
+    return out_t if len(out_t) > 1 else y
+
+
+def _orientation(res: _native.Result, batch_shape: tuple[int, ...], spatial: tuple[int, ...], ndim: int,
+                 batch_axes: tuple[int, ...], axes: tuple[int, ...], target: str, device: Any) -> Orientation:
+    """The ``Orientation`` of a call from the kept direction (B, K, D, *spatial) and coherence (B,
+    K, *spatial), in the public layout (reference src/eda/_api.py:558-577)."""
+    D, K = len(axes), int(res.K)
+    B = math.prod(batch_shape)
+    shapes = (batch_shape + (K, D) + spatial, batch_shape + (K,) + spatial)
+    perms = [_restore_order(ndim, batch_axes, axes, e) for e in (2, 1)]
+    lib = _native.core()
+    err = _native.errbuf()
+    if device is not None:
+        torch = _torch()
+        tdt = torch.float32 if target == "float32" else torch.float64
+        dirs = torch.empty((B, K, D) + spatial, dtype=tdt, device=device)
+        coh = torch.empty((B, K) + spatial, dtype=tdt, device=device)
+        if coh.numel():
+            _native.check(lib.eda_collect_orientation(ctypes.byref(res), ctypes.c_void_p(dirs.data_ptr()),
+                                                      ctypes.c_void_p(coh.data_ptr()), int(device.index), err,
+                                                      len(err)), err)
+        arrays = [a.reshape(sh).permute(pm) for a, sh, pm in zip((dirs, coh), shapes, perms)]
+        return Orientation(*arrays)
+    dirs = np.empty((B, K, D) + spatial, dtype=np.dtype(target))
+    coh = np.empty((B, K) + spatial, dtype=np.dtype(target))
+    if coh.size:
+        _native.check(lib.eda_collect_orientation(ctypes.byref(res), dirs.ctypes.data_as(ctypes.c_void_p),
+                                                  coh.ctypes.data_as(ctypes.c_void_p), -1, err, len(err)), err)
+    arrays = []
+    for a, sh, pm in zip((dirs, coh), shapes, perms):
+        a = a.reshape(sh)
+        arrays.append(a.transpose(pm) if pm != list(range(len(pm))) else a)
+    return Orientation(*arrays)

@@ -15,10 +15,12 @@ the reference's ``_native.core()`` can return it in place of ``eda._core`` on a 
   build_info(), device_count(), device_info(device=0), pointer_device(ptr)
 
 info holds ``n_imfs``, ``error`` [B] int32; ``windows``, ``sifts`` [B, K] int32; ``spacing``,
-``energy`` [B, K] float64; ``chunk`` int. Errors raise the reference's classes: ValueError,
-MemoryError, ``eda.CudaError``, RuntimeError. Not served: ``orientation=True`` (ValueError),
-device input whose dtype differs from ``target`` or whose batch axes do not merge into one stride
-(ValueError), and the monogenic and DLPack helpers. The device path allocates its output through
+``energy`` [B, K] float64; ``chunk`` int; with ``orientation``, for host input ``direction`` (B, K,
+D, *spatial) and ``coherence`` (B, K, *spatial) arrays, for device input ``orientation`` = (owner,
+ptr) of one buffer holding the direction followed by the coherence ((None, 0) when B or K is 0).
+Errors raise the reference's classes: ValueError, MemoryError, ``eda.CudaError``, RuntimeError.
+Not served: device input whose dtype differs from ``target`` or whose batch axes do not merge into
+one stride (ValueError), and the monogenic and DLPack helpers. The device path allocates its output through
 PyTorch-ROCm, whose tensor is the ``owner``."""
 
 from __future__ import annotations
@@ -64,8 +66,6 @@ def pointer_device(ptr: int) -> int:
 
 
 def _params(max_imf, tie_tolerance, sifts, check_finite, orientation, max_chunk, stream=0) -> _native.Params:
-    if orientation:
-        raise ValueError("eda: orientation output is not supported by this build")
     if int(sifts) < 1:
         raise ValueError(f"eda: sifts must be at least 1, got {sifts}")
     if int(max_chunk) < 0:
@@ -74,7 +74,7 @@ def _params(max_imf, tie_tolerance, sifts, check_finite, orientation, max_chunk,
     p.max_imf = int(max_imf)
     p.sifts = int(sifts)
     p.check_finite = 1 if check_finite else 0
-    p.orientation = 0
+    p.orientation = 1 if orientation else 0
     p.tie_tolerance = -1.0 if tie_tolerance is None else float(tie_tolerance)
     p.max_chunk = int(max_chunk)
     p.stream = int(stream)
@@ -136,6 +136,17 @@ def decompose_host(x, max_imf: int, tie_tolerance: float | None, sifts: int, che
             err = _native.errbuf()
             _native.check(_native.core().eda_collect(ctypes.byref(res), rows.ctypes.data_as(ctypes.c_void_p), -1,
                                                      err, len(err)), err)
+        if orientation:
+            K = int(res.K)
+            dirs = np.zeros((x.shape[0], K, desc.dim) + x.shape[1:], dtype=x.dtype)
+            coh = np.zeros((x.shape[0], K) + x.shape[1:], dtype=x.dtype)
+            if coh.size:
+                err = _native.errbuf()
+                _native.check(_native.core().eda_collect_orientation(
+                    ctypes.byref(res), dirs.ctypes.data_as(ctypes.c_void_p), coh.ctypes.data_as(ctypes.c_void_p), -1,
+                    err, len(err)), err)
+            info["direction"] = dirs
+            info["coherence"] = coh
     finally:
         _native.core().eda_release(ctypes.byref(res))
     return rows, info
@@ -200,6 +211,18 @@ def decompose_device(ptr: int, device: int, dtype: str, target: str, batch_shape
             err = _native.errbuf()
             _native.check(_native.core().eda_collect(ctypes.byref(res), ctypes.c_void_p(owner.data_ptr()),
                                                      int(device), err, len(err)), err)
+        if orientation:
+            info["orientation"] = (None, 0)
+            K, count = int(res.K), math.prod(int(n) for n in shape)
+            nc = B * K * count
+            if nc:
+                buf = torch.empty(nc * (len(shape) + 1), dtype=tdt, device=torch.device("cuda", int(device)))
+                err = _native.errbuf()
+                item = buf.element_size()
+                _native.check(_native.core().eda_collect_orientation(
+                    ctypes.byref(res), ctypes.c_void_p(buf.data_ptr()),
+                    ctypes.c_void_p(buf.data_ptr() + nc * len(shape) * item), int(device), err, len(err)), err)
+                info["orientation"] = (buf, buf.data_ptr())
     finally:
         _native.core().eda_release(ctypes.byref(res))
     return owner, owner.data_ptr(), info

@@ -52,7 +52,7 @@ typedef struct eda_params {
   int32_t max_imf;      /* <= 0: no limit */
   int32_t sifts;        /* >= 1 */
   int32_t check_finite; /* 1: an element with inf / NaN input fails with error 1 */
-  int32_t orientation;  /* must be 0 (return_orientation is not built) */
+  int32_t orientation;  /* 1: keep the orientation (n, c) of every mode (dim >= 2) for eda_collect_orientation */
   double tie_tolerance; /* < 0: the dtype's default (2^-14 float32, 2^-30 float64) */
   int64_t max_chunk;    /* > 0: at most this many elements per chunk; 0: by free memory */
   uint64_t stream;      /* a hipStream_t the call waits for before it reads device input (0: none) */
@@ -90,6 +90,16 @@ int eda_decompose(const eda_array_desc* x, const eda_params* p, eda_result* out,
  * host memory (synchronous); else device memory of that device, complete on return. Replaces
  * eda::Run::collect (cpp/src/decompose.hpp:100-101). */
 int eda_collect(const eda_result* r, void* dst, int32_t dst_device, char* err, size_t errlen);
+
+/* With eda_params.orientation: copies the direction n (batch, K, dim, *spatial) into dir and the
+ * coherence c (batch, K, *spatial) into coh, C-contiguous in the compute dtype: row k of element b
+ * is n and c as the method computed them at the start of its mode k, n with the canonical sign
+ * (its first component of magnitude above the dtype's default tie tolerance is positive); rows
+ * beyond an element's n_imfs and every row of a failed element are 0. dst_device as in
+ * eda_collect. Nothing is copied when batch or K is 0; a call that kept no orientation (orientation
+ * 0, or dim 1) fails with EDA_EINTERNAL. Replaces eda::Run::collect_orientation
+ * (cpp/src/decompose.hpp:100-103, cpp/src/method/driver.cpp:177-196). */
+int eda_collect_orientation(const eda_result* r, void* dir, void* coh, int32_t dst_device, char* err, size_t errlen);
 
 /* Frees the rows and the info arrays of a result (idempotent). */
 void eda_release(eda_result* r);

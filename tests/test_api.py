@@ -69,11 +69,20 @@ def test_axes_rules():
         _normalize_axes((True,), 2)
 
 
-def test_return_orientation_needs_2_axes_and_is_not_built():
-    with pytest.raises(ValueError, match="2 or 3 spatial axes"):
+def test_return_orientation_needs_2_axes():
+    with pytest.raises(ValueError, match="return_orientation=True needs 2 or 3 spatial axes"):
         eda.decompose(X, return_orientation=True)
-    with pytest.raises(ValueError, match="not supported"):
-        eda.decompose(np.zeros((8, 8)), return_orientation=True)
+    with pytest.raises(ValueError, match="needs 2 or 3 spatial axes, got 1"):
+        eda.decompose(np.zeros((4, 64)), axes=1, return_orientation=True)
+    with pytest.raises(TypeError, match="return_orientation must be True or False"):
+        eda.decompose(np.zeros((16, 16)), return_orientation=1)
+    assert eda.Orientation.__dataclass_fields__.keys() == {"direction", "coherence"}
+
+
+def test_restore_order_of_the_orientation():
+    # direction (B, K, D, *spatial) of x (b0, s0, b1, s1), axes (1, 3) -> (b0, K, D, s0, b1, s1)
+    perm = _restore_order(4, (0, 2), (1, 3), 2)
+    assert np.zeros((2, 3, 7, 2, 5, 6)).transpose(perm).shape == (2, 7, 2, 5, 3, 6)
 
 
 def test_restore_order_inserts_the_mode_axis_before_the_first_spatial_axis():
@@ -111,8 +120,6 @@ def test_core_hip_argument_rules_before_any_gpu_work():
     from eda import _core_hip
 
     x = np.zeros((2, 16), np.float32)
-    with pytest.raises(ValueError):
-        _core_hip.decompose_host(x, -1, None, 3, True, True, 0)  # orientation
     with pytest.raises(ValueError):
         _core_hip.decompose_host(x, -1, None, 0, True, False, 0)  # sifts
     with pytest.raises(TypeError):
