@@ -17,7 +17,10 @@ synchronization. Prints one JSON line on stdout (progress goes to stderr):
   cpu_baseline  the oracle (a C++ restatement of the method, FP64, one image per host thread)
              on a bounded sample of the same workload, on this host's cores
 
-Usage: python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--no-cpu]
+With --monogenic the same batch goes through eda.monogenic instead, and the roofline is the DCT
+contraction's MFMA rate (float32 matrix peak 157.3 TFLOP/s).
+
+Usage: python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--no-cpu] [--monogenic]
 """
 
 from __future__ import annotations
@@ -155,6 +158,100 @@ def cpu_baseline():
     }
 
 
+MONO_METRIC = "Msamples/s monogenic signal (64×512² FP32)"
+MFMA_F32_PEAK_TFS = 157.3
+
+
+def bench_monogenic(args, x, lib, torch) -> dict:
+    """eda.monogenic of the same 64 x 512 x 512 float32 batch (input resident in HBM): throughput,
+    the DCT contraction's live MFMA rate (the transform GEMMs, timed with HIP events on the
+    library's stream; flops 2 M N K per launch entry) against the float32 matrix peak, and the FP64
+    oracle's rate on a bounded sample."""
+    import numpy as np
+
+    import eda
+    from eda import _native
+
+    for _ in range(args.warmup):
+        r = eda.monogenic(x, axes=(1, 2))
+        torch.cuda.synchronize()
+        del r
+    err = _native.errbuf()
+    lib.eda_timer_enable(2)
+    _native.check(lib.eda_timer_read(ctypes.byref(ctypes.c_double()), ctypes.byref(ctypes.c_int64()),
+                                     ctypes.byref(ctypes.c_double()), err, len(err)), err)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = eda.monogenic(x, axes=(1, 2))
+        del r
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / max(args.steps, 1)
+    ms, launches, flops = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double()
+    _native.check(lib.eda_timer_read(ctypes.byref(ms), ctypes.byref(launches), ctypes.byref(flops), err, len(err)),
+                  err)
+    lib.eda_timer_enable(0)
+    avg_s = ms.value * 1e-3 / max(launches.value, 1)
+    per_launch = flops.value / max(launches.value, 1)
+    achieved = per_launch / avg_s / 1e12 if avg_s > 0 else 0.0
+    log(f"bench: monogenic: {args.steps} steps, {dt * 1e3:.1f} ms per step; DCT GEMM: {launches.value} launches, "
+        f"{ms.value:.1f} ms in total")
+    out = {
+        "metric": MONO_METRIC,
+        "value": round(B * N * N / dt / 1e6, 3),
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "eda.monogenic, 64 x 512 x 512 float32 images (the decomposition benchmark's "
+                               "inputs), input resident in HBM", "batch": B, "shape": [N, N]},
+        "roofline": {
+            "bound": "mfma",
+            "kernel": "DCT contraction (k_gemm_mfma, one axis of one transform per launch)",
+            "achieved": round(achieved, 2),
+            "peak": MFMA_F32_PEAK_TFS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / MFMA_F32_PEAK_TFS, 4),
+            "traffic": None,
+            "flops_per_launch": round(per_launch),
+            "avg_launch_us": round(avg_s * 1e6, 2),
+            "launches_per_step": launches.value // max(args.steps, 1),
+            "share_of_step": round(ms.value * 1e-3 / max(args.steps, 1) / dt, 4),
+        },
+    }
+    if not args.no_cpu:
+        import oracle
+        import workloads
+        from concurrent.futures import ThreadPoolExecutor
+
+        try:
+            cores = len(os.sched_getaffinity(0))
+        except AttributeError:
+            cores = os.cpu_count() or 1
+        threads = max(1, min(16, cores))
+        images = threads
+        xs = workloads.rings_stripes_blobs(images, N).astype(np.float64)
+        log(f"bench: monogenic cpu baseline: the FP64 oracle on {images} image(s), {threads} thread(s)")
+        t = time.perf_counter()
+        with ThreadPoolExecutor(threads) as pool:
+            list(pool.map(oracle.monogenic, list(xs)))
+        cdt = time.perf_counter() - t
+        out["cpu_baseline"] = {
+            "value": round(images * N * N / cdt / 1e6, 6),
+            "unit": "Msamples/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"images 0..{images - 1} of the 64 ({N}x{N}, float64 oracle, one image per thread), {cdt:.1f} s",
+        }
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -163,6 +260,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one copy kernel of known size (for scaling rocprofv3 byte counters)")
+    ap.add_argument("--monogenic", action="store_true",
+                    help="measure eda.monogenic on the same batch instead of eda.decompose")
     ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "OUT_JSON"),
                     help="summarize two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) and exit")
     args = ap.parse_args()
@@ -188,6 +287,9 @@ def main() -> int:
     log("bench: generating 64 x 512 x 512 float32 inputs")
     x = torch.from_numpy(workloads.rings_stripes_blobs(B, N, np.float32)).to("cuda:0")
     torch.cuda.synchronize()
+    if args.monogenic:
+        print(json.dumps(bench_monogenic(args, x, lib, torch)), flush=True)
+        return 0
 
     info = None
     for i in range(args.warmup):

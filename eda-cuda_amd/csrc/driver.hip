@@ -44,7 +44,7 @@ void throw_hip(hipError_t e, const char* expr, const char* file, int line) {
 namespace {
 struct TimerState {
   std::mutex mu;
-  std::atomic<bool> on{false};
+  std::atomic<int> on{0};  // 0 off, 1 the vHGW pass, 2 the DCT contraction
   std::vector<hipEvent_t> pool;
   struct Span {
     hipEvent_t a, b;
@@ -74,8 +74,9 @@ struct Mirror {
 thread_local std::vector<Mirror> g_mirrors;
 }  // namespace
 
-void timer_set(bool on) { timer_state().on = on; }
-bool timer_on() { return timer_state().on.load(std::memory_order_relaxed); }
+void timer_set(int kind) { timer_state().on = kind; }
+bool timer_on() { return timer_state().on.load(std::memory_order_relaxed) == 1; }
+bool timer_gemm_on() { return timer_state().on.load(std::memory_order_relaxed) == 2; }
 void timer_note(hipStream_t s, hipEvent_t* start) {
   TimerState& t = timer_state();
   std::lock_guard<std::mutex> lock(t.mu);
@@ -1775,6 +1776,102 @@ void collect_orientation(const Run& run, void* dir, void* coh, int32_t dst_devic
   ctx.up.reset();
 }
 
+int current_device_of(const eda_array_desc& x);
+
+// eda.monogenic (reference cpp/src/method/monogenic.cpp:61-158): per chunk of elements sized by
+// the free device memory, the input copy (host staged or device gathered through its strides), the
+// non-finite counts and max |x|, the exact prescale f 2^-k with max |f| in [1, 2), and the monogenic
+// signal into the call's planes. Returns the chunk.
+template <class T>
+int64_t run_monogenic(const eda_array_desc& x, bool check_finite, int64_t max_chunk, uint64_t stream, void* out,
+                      int32_t out_device, int32_t* nonfinite) {
+  const int device = current_device_of(x);
+  Context& ctx = context(device);
+  hipStream_t s = ctx.stream;
+  if (stream) {
+    hipEvent_t ev;
+    EDA_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    EDA_HIP(hipEventRecord(ev, reinterpret_cast<hipStream_t>(stream)));
+    EDA_HIP(hipStreamWaitEvent(s, ev, 0));
+    EDA_HIP(hipEventDestroy(ev));
+  }
+  const Geom gall = make_geom(x.dim, x.batch, x.n);
+  const int D = x.dim;
+  const int64_t B = x.batch, count = gall.count;
+  const size_t out_elems = static_cast<size_t>(B) * monogenic_planes(D) * count;
+  DevMem dout;
+  T* o = static_cast<T*>(out);
+  if (out_device < 0) {
+    dout = DevMem(out_elems * sizeof(T));
+    o = static_cast<T*>(dout.p);
+  }
+  const double tiny = method_ctl(gall, std::is_same_v<T, float>).tiny;
+  int64_t Bc = std::min<int64_t>(B, 65535);
+  if (max_chunk > 0) Bc = std::min(Bc, max_chunk);
+  const auto ws_bytes = [&](int64_t nb) {
+    const Geom g = make_geom(x.dim, nb, x.n);
+    const size_t F = static_cast<size_t>(nb) * count;
+    return (monogenic_fields(D) + 1) * (F * sizeof(T) + 256) + elem_stats_ws_bytes(g) + 256 +
+           static_cast<size_t>(orient_scratch_doubles(nb)) * sizeof(double) + static_cast<size_t>(nb) * 24 + 4096;
+  };
+  size_t free_b = 0, total_b = 0;
+  EDA_HIP(hipMemGetInfo(&free_b, &total_b));
+  while (Bc > 1 && ws_bytes(Bc) + (size_t{512} << 20) > free_b) Bc = (Bc + 1) / 2;
+  DevMem mem(ws_bytes(Bc));
+  std::vector<uint64_t> amax(Bc), nf(Bc);
+  std::vector<int32_t> kexp(Bc);
+  for (int64_t b0 = 0; b0 < B; b0 += Bc) {
+    const int64_t nb = std::min(Bc, B - b0);
+    const Geom g = make_geom(x.dim, nb, x.n);
+    const int64_t F = nb * count;
+    Carve cv{static_cast<char*>(mem.p)};
+    T* f = cv.take<T>(F);
+    T* work = cv.take<T>(static_cast<size_t>(F) * monogenic_fields(D));
+    char* sws = cv.take<char>(elem_stats_ws_bytes(g));
+    double* dscr = cv.take<double>(static_cast<size_t>(orient_scratch_doubles(nb)));
+    uint64_t* d_amax = cv.take<uint64_t>(nb);
+    uint64_t* d_nf = cv.take<uint64_t>(nb);
+    ctx.up.reset();
+    if (x.device < 0) {
+      EDA_HIP(hipMemcpyAsync(f, static_cast<const T*>(x.data) + b0 * count, static_cast<size_t>(F) * sizeof(T),
+                             hipMemcpyHostToDevice, s));
+    } else {
+      const T* xp = static_cast<const T*>(x.data) + b0 * x.batch_stride;
+      const int64_t s0 = x.batch_stride, s1 = x.strides[0], s2 = x.strides[1], s3 = x.strides[2];
+      each(s, g, nullptr, [=] __device__(int64_t b, int64_t i) {
+        int64_t off = b * s0 + ((i / g.st[0]) % g.n[0]) * s1;
+        if (g.dim >= 2) off += ((i / g.st[1]) % g.n[1]) * s2;
+        if (g.dim >= 3) off += ((i / g.st[2]) % g.n[2]) * s3;
+        f[b * g.count + i] = xp[off];
+      });
+    }
+    enqueue_elem_stats<T>(s, g, f, nullptr, nullptr, d_amax, d_nf, sws);
+    EDA_HIP(hipMemcpyAsync(amax.data(), d_amax, sizeof(uint64_t) * nb, hipMemcpyDeviceToHost, s));
+    EDA_HIP(hipMemcpyAsync(nf.data(), d_nf, sizeof(uint64_t) * nb, hipMemcpyDeviceToHost, s));
+    EDA_HIP(hipStreamSynchronize(s));
+    for (int64_t b = 0; b < nb; ++b) {
+      const double a = bits_d(amax[b]);
+      kexp[b] = (a > 0.0 && std::isfinite(a)) ? std::ilogb(a) : 0;
+      if (nonfinite != nullptr) nonfinite[b0 + b] = check_finite && nf[b] != 0 ? 1 : 0;
+    }
+    const int32_t* dk = ctx.up.put(std::vector<int32_t>(kexp.begin(), kexp.begin() + nb));
+    each(s, g, nullptr, [=] __device__(int64_t b, int64_t i) {
+      const int32_t k = dk[b];
+      if (k == 0) return;
+      const int64_t at = b * g.count + i;
+      if constexpr (std::is_same_v<T, float>) f[at] = static_cast<float>(ldexp(static_cast<double>(f[at]), -k));
+      else f[at] = ldexp(f[at], -k);
+    });
+    enqueue_monogenic<T>(s, device, g, f, dk, tiny, work, dscr, o, B, b0);
+    EDA_HIP(hipStreamSynchronize(s));
+  }
+  if (out_device < 0 && out_elems)
+    EDA_HIP(hipMemcpyAsync(out, o, out_elems * sizeof(T), hipMemcpyDeviceToHost, s));
+  EDA_HIP(hipStreamSynchronize(s));
+  ctx.up.reset();
+  return Bc;
+}
+
 int current_device_of(const eda_array_desc& x) {
   if (x.device >= 0) return x.device;
   int d = 0;
@@ -1877,6 +1974,26 @@ int eda_collect_orientation(const eda_result* r, void* dir, void* coh, int32_t d
   });
 }
 
+int eda_monogenic(const eda_array_desc* x, int32_t check_finite, int64_t max_chunk, uint64_t stream, void* out,
+                  int32_t out_device, int32_t* nonfinite, int64_t* chunk, char* err, size_t errlen) {
+  return guarded(err, errlen, [&] {
+    if (x == nullptr) throw std::invalid_argument("eda: null argument");
+    eda_params p{};
+    p.sifts = 3;
+    p.tie_tolerance = -1.0;
+    validate(*x, p);
+    if (max_chunk < 0) throw std::invalid_argument("eda: max_chunk must be >= 0");
+    if (chunk != nullptr) *chunk = 0;
+    if (x->batch == 0) return;
+    if (out == nullptr) throw std::invalid_argument("eda: monogenic: null output");
+    EDA_HIP(hipSetDevice(current_device_of(*x)));
+    const int64_t c = x->dtype == EDA_F32
+                          ? run_monogenic<float>(*x, check_finite != 0, max_chunk, stream, out, out_device, nonfinite)
+                          : run_monogenic<double>(*x, check_finite != 0, max_chunk, stream, out, out_device, nonfinite);
+    if (chunk != nullptr) *chunk = c;
+  });
+}
+
 void eda_release(eda_result* r) {
   if (r == nullptr) return;
   delete static_cast<Run*>(r->handle);
@@ -1931,7 +2048,8 @@ int eda_pointer_device(const void* ptr, int32_t* device) {
 }
 
 int eda_timer_enable(int32_t on) {
-  timer_set(on != 0);
+  if (on < 0 || on > 2) return EDA_EINVAL;
+  timer_set(on);
   return EDA_OK;
 }
 

@@ -29,8 +29,11 @@ MethodCtl method_ctl(const Geom& g, bool f32);
 // events on its stream, and its algorithmic bytes are counted: the pass's input field(s) read once
 // and output field(s) written once, over the elements whose window is non-zero. The count of those
 // elements comes from the host mirror of the window array (no device read-back).
-void timer_set(bool on);
+// kind 2 times the DCT contraction's GEMM launches instead, counting their flops (2 M N K per
+// batch entry, every entry) in place of bytes.
+void timer_set(int kind);
 bool timer_on();
+bool timer_gemm_on();
 void timer_note(hipStream_t s, hipEvent_t* start);                      // before a launch
 void timer_done(hipStream_t s, hipEvent_t start, double bytes);         // after it
 void timer_read(double* ms, int64_t* launches, double* bytes);          // synchronizes, sums, clears
@@ -122,6 +125,16 @@ template <class T>
 void enqueue_orientation(hipStream_t s, int device, const Geom& g, const T* r, const int32_t* w, const int32_t* on,
                          T* nvec, T* coh, T* work, int32_t* nonfinite, double* dscratch);
 inline int64_t orient_scratch_doubles(int64_t batch) { return batch * 65; }
+// The monogenic signal of every element of f (the prescaled fields: the input times 2^-kexp[b])
+// into the planes of out, the elements at b0 .. b0 + B - 1 of out_batch: amplitude (times
+// 2^kexp[b]), phase, frequency [out_batch][count] each, then for D >= 2 the direction
+// [out_batch][D][count] and the coherence [out_batch][count]. work: monogenic_fields(D) fields;
+// dscratch: orient_scratch_doubles(B) doubles.
+template <class T>
+void enqueue_monogenic(hipStream_t s, int device, const Geom& g, const T* f, const int32_t* kexp, double tiny,
+                       T* work, double* dscratch, T* out, int64_t out_batch, int64_t b0);
+inline int monogenic_fields(int D) { return D >= 2 ? 3 + 3 * D + D * (D + 1) / 2 + 1 : 6; }
+inline int monogenic_planes(int D) { return D == 1 ? 3 : 4 + D; }
 // The orthonormal DCT-II of every element (inverse = 0) or its inverse with the sin basis on the
 // axes of the bit mask odd (inverse = 1), through the same GEMMs; tmp: one field.
 template <class T>

@@ -237,6 +237,9 @@ template <class T>
 void gemm(hipStream_t s, const GemmDims& d, const T* A, const T* B, T* C, const int32_t* on) {
   const dim3 grid(static_cast<unsigned>((d.N + kTN - 1) / kTN), static_cast<unsigned>((d.M + kTM - 1) / kTM),
                   static_cast<unsigned>(d.batches));
+  const bool timed = timer_gemm_on();
+  hipEvent_t t0 = nullptr;
+  if (timed) timer_note(s, &t0);
   if constexpr (std::is_same_v<T, float>) {
     hipLaunchKernelGGL(k_gemm_mfma, grid, dim3(kThreads), 0, s, d, A, B, C, on);
     EDA_LAUNCH("k_gemm_mfma");
@@ -244,6 +247,7 @@ void gemm(hipStream_t s, const GemmDims& d, const T* A, const T* B, T* C, const 
     hipLaunchKernelGGL(k_gemm<T>, grid, dim3(kThreads), 0, s, d, A, B, C, on);
     EDA_LAUNCH("k_gemm");
   }
+  if (timed) timer_done(s, t0, 2.0 * static_cast<double>(d.M) * d.N * d.K * d.batches);
 }
 
 // One axis of a separable transform: out = M applied along axis a of every running element
@@ -489,6 +493,32 @@ double band_two_sigma2() {
   return 2.0 * (sg * sg);
 }
 
+// The component kinds of the Riesz and derivative transforms (reference orientation.cu:86-128):
+// the multiplier at DCT index k (0 at the DC) and the axes whose inverse uses the sin basis:
+//   0 R1_a    u_a / rad                  sin on a
+//   1 R2_aa   -(u_a / rad)^2             -
+//   2 R2_ab   (u_a / rad) (u_b / rad)    sin on a, b   (a < b)
+//   3 D_a     -u_a                       sin on a      (d/dx_a / (2 pi))
+//   4 D_b R_a u_a (u_a / rad)            -             (a == b)
+//             -u_a (u_b / rad)           sin on a, b   (a < b)
+// The reference inverts the odd components through a spectral reversal and the sign (-1)^(x_a);
+// the sin basis is the same sum without either.
+__device__ __forceinline__ double riesz_mult(int kind, int a, int b, const double* u, double rad) {
+  if (!(rad > 0.0)) return 0.0;
+  const double inv = 1.0 / rad;
+  const double ua = u[a] * inv;
+  if (kind == 0) return ua;
+  if (kind == 1) return (-ua) * ua;
+  if (kind == 2) return ua * (u[b] * inv);
+  if (kind == 3) return -u[a];
+  return a == b ? u[a] * ua : -(u[a] * (u[b] * inv));
+}
+int riesz_odd(int kind, int a, int b) {
+  if (kind == 0 || kind == 3) return 1 << a;
+  if (kind == 1 || (kind == 4 && a == b)) return 0;
+  return (1 << a) | (1 << b);
+}
+
 }  // namespace
 
 template <class T>
@@ -521,6 +551,45 @@ template <class T>
 void enqueue_eigen(hipStream_t s, const Geom& g, const T* t, const int32_t* on, T* nvec, T* coh, int32_t* nonfinite) {
   if (g.dim == 2) eigen_pass<T, 2>(s, g, t, on, nvec, coh, nonfinite);
   else if (g.dim == 3) eigen_pass<T, 3>(s, g, t, on, nvec, coh, nonfinite);
+}
+
+// T = R1 R1^T + R2 R2^T per voxel, written over R2 (pair order), then the eigen pass into n and c.
+template <class T>
+void tensor_eigen(hipStream_t s, const Geom& g, const T* R1, T* R2, const int32_t* on, T* nvec, T* coh,
+                  int32_t* nonfinite) {
+  const int D = g.dim;
+  const int K = D * (D + 1) / 2;
+  const int64_t F = g.batch * g.count;
+  each(s, g, on, [=] __device__(int64_t b, int64_t i) {
+    const int64_t at = b * g.count + i;
+    T r1[kMaxDim], r2[6], out[6];
+    for (int a = 0; a < D; ++a) r1[a] = R1[a * F + at];
+    for (int p = 0; p < K; ++p) r2[p] = R2[p * F + at];
+    for (int a = 0; a < D; ++a)
+      for (int c = a; c < D; ++c) {
+        T v = r1[a] * r1[c];
+        for (int q = 0; q < D; ++q) v = v + r2[tensor_pair(D, a, q)] * r2[tensor_pair(D, q, c)];
+        out[tensor_pair(D, a, c)] = v;
+      }
+    for (int p = 0; p < K; ++p) R2[p * F + at] = out[p];
+  });
+  enqueue_eigen<T>(s, g, R2, on, nvec, coh, nonfinite);
+}
+
+// One component of X (a kind of riesz_mult) into dst: Y = T(m X), then the inverse transform with
+// the sin basis on the component's odd axes (T1: the transform's scratch field).
+template <class T>
+void riesz_component(hipStream_t s, int device, const Geom& g, const T* X, int kind, int a, int bb, T* Y, T* T1,
+                     T* dst, const int32_t* on) {
+  each(s, g, on, [=] __device__(int64_t b, int64_t i) {
+    int32_t k[kMaxDim];
+    double u[kMaxDim];
+    const double rad = dct_freq(g, i, k, u);
+    const double m = riesz_mult(kind, a, bb, u, rad);
+    const int64_t at = b * g.count + i;
+    Y[at] = static_cast<T>(m * static_cast<double>(X[at]));
+  });
+  enqueue_dct<T>(s, device, g, Y, dst, T1, 1, riesz_odd(kind, a, bb));
 }
 
 template <class T>
@@ -569,43 +638,99 @@ void enqueue_orientation(hipStream_t s, int device, const Geom& g, const T* r, c
       }
       bb = a + p;
     }
-    {
-      const int kind = ci < D ? 0 : (a == bb ? 1 : 2);
-      const int odd = kind == 0 ? (1 << a) : (kind == 1 ? 0 : ((1 << a) | (1 << bb)));
-      T* dst = kind == 0 ? R1 + a * F : R2 + tensor_pair(D, a, bb) * F;
-      each(s, g, on, [=] __device__(int64_t b, int64_t i) {
-        int32_t k[kMaxDim];
-        double u[kMaxDim];
-        const double rad = dct_freq(g, i, k, u);
-        double m = 0.0;
-        if (rad > 0.0) {
-          const double inv = 1.0 / rad;
-          const double ua = u[a] * inv;
-          if (kind == 0) m = ua;
-          else if (kind == 1) m = (-ua) * ua;
-          else m = ua * (u[bb] * inv);
-        }
-        const int64_t at = b * g.count + i;
-        Y[at] = static_cast<T>(m * static_cast<double>(X[at]));
-      });
-      enqueue_dct<T>(s, device, g, Y, dst, T1, 1, odd);
-    }
+    const int kind = ci < D ? 0 : (a == bb ? 1 : 2);
+    riesz_component<T>(s, device, g, X, kind, a, bb, Y, T1, kind == 0 ? R1 + a * F : R2 + tensor_pair(D, a, bb) * F,
+                       on);
   }
-  // T = R1 R1^T + R2 R2^T over R2, then the eigen pass into n (over R1) and c
-  each(s, g, on, [=] __device__(int64_t b, int64_t i) {
+  tensor_eigen<T>(s, g, R1, R2, on, nvec, coh, nonfinite);
+}
+
+// The monogenic signal of every element of f (the prescaled fields f 2^-kexp[b]) into the planes
+// of out (reference orientation.cu:420-493,789-831, monogenic.cpp:61-158).
+template <class T>
+void enqueue_monogenic(hipStream_t s, int device, const Geom& g, const T* f, const int32_t* kexp, double tiny,
+                       T* work, double* dscratch, T* out, int64_t out_batch, int64_t b0) {
+  const int D = g.dim;
+  const int64_t F = g.batch * g.count;
+  const int K = D >= 2 ? D * (D + 1) / 2 : 1;
+  T* X = work;
+  T* Y = work + F;
+  T* T1 = work + 2 * F;
+  T* R1 = work + 3 * F;
+  T* DF = R1 + D * F;
+  T* R2 = DF + D * F;  // R2, then T, then D_b R_a
+  T* NV = R2 + K * F;
+  T* C = NV + D * F;
+  double* mean = dscratch;
+  enqueue_mean<T>(s, g, f, nullptr, mean, mean + g.batch);
+  each(s, g, nullptr, [=] __device__(int64_t b, int64_t i) {
     const int64_t at = b * g.count + i;
-    T r1[kMaxDim], r2[6], out[6];
-    for (int a = 0; a < D; ++a) r1[a] = R1[a * F + at];
-    for (int p = 0; p < K; ++p) r2[p] = R2[p * F + at];
-    for (int a = 0; a < D; ++a)
-      for (int c = a; c < D; ++c) {
-        T v = r1[a] * r1[c];
-        for (int q = 0; q < D; ++q) v = v + r2[tensor_pair(D, a, q)] * r2[tensor_pair(D, q, c)];
-        out[tensor_pair(D, a, c)] = v;
-      }
-    for (int p = 0; p < K; ++p) R2[p * F + at] = out[p];
+    Y[at] = static_cast<T>(static_cast<double>(f[at]) - mean[b]);
   });
-  enqueue_eigen<T>(s, g, R2, on, nvec, coh, nonfinite);
+  enqueue_dct<T>(s, device, g, Y, X, T1, 0, 0);
+  for (int a = 0; a < D; ++a) riesz_component<T>(s, device, g, X, 0, a, a, Y, T1, R1 + a * F, nullptr);
+  for (int a = 0; a < D; ++a) riesz_component<T>(s, device, g, X, 3, a, a, Y, T1, DF + a * F, nullptr);
+  if (D >= 2) {
+    for (int a = 0; a < D; ++a)
+      for (int bb = a; bb < D; ++bb)
+        riesz_component<T>(s, device, g, X, a == bb ? 1 : 2, a, bb, Y, T1, R2 + tensor_pair(D, a, bb) * F, nullptr);
+    tensor_eigen<T>(s, g, R1, R2, nullptr, NV, C, nullptr);
+    for (int a = 0; a < D; ++a)
+      for (int bb = a; bb < D; ++bb)
+        riesz_component<T>(s, device, g, X, 4, a, bb, Y, T1, R2 + tensor_pair(D, a, bb) * F, nullptr);
+  } else {
+    riesz_component<T>(s, device, g, X, 4, 0, 0, Y, T1, R2, nullptr);
+  }
+  // per voxel, in double: q = n . R1, A^2 = f^2 + |R1|^2, D_n f, D_n q; the phase and the frequency
+  // where A^2 > tiny (else 0); the amplitude sqrt(A^2) 2^kexp rounded once; a phase that rounds to
+  // -pi in T is pi
+  const T zero = std::is_same_v<T, float> ? T(0x1p-14) : T(0x1p-30);
+  const int64_t plane = out_batch * g.count;
+  const T pi_t = static_cast<T>(3.141592653589793);
+  each(s, g, nullptr, [=] __device__(int64_t b, int64_t i) {
+    const int64_t at = b * g.count + i;
+    T nt[kMaxDim];
+    if (D == 1) {
+      nt[0] = T(1);
+    } else {
+      int sgn = 0;
+      for (int a = 0; a < D; ++a) {
+        nt[a] = NV[a * F + at];
+        if (sgn == 0 && fabs(nt[a]) > zero) sgn = nt[a] > T(0) ? 1 : -1;
+      }
+      for (int a = 0; a < D; ++a) nt[a] = sgn < 0 ? -nt[a] : nt[a];
+    }
+    double n[kMaxDim];
+    for (int a = 0; a < D; ++a) n[a] = static_cast<double>(nt[a]);
+    const double fv = static_cast<double>(f[at]);
+    double q = 0.0, a2 = fv * fv, dnf = 0.0, dnq = 0.0;
+    for (int a = 0; a < D; ++a) {
+      const double r = static_cast<double>(R1[a * F + at]);
+      q = q + n[a] * r;
+      a2 = a2 + r * r;
+      dnf = dnf + n[a] * static_cast<double>(DF[a * F + at]);
+    }
+    int p = 0;
+    for (int a = 0; a < D; ++a)
+      for (int c = a; c < D; ++c, ++p) {
+        const double m = a == c ? n[a] * n[a] : 2.0 * (n[a] * n[c]);
+        dnq = dnq + m * static_cast<double>(R2[(D == 1 ? 0 : tensor_pair(D, a, c)) * F + at]);
+      }
+    double phase = 0.0, freq = 0.0;
+    if (a2 > tiny) {
+      phase = atan2(q, fv);
+      freq = (fv * dnq - q * dnf) / a2;
+    }
+    const int64_t ot = (b0 + b) * g.count + i;
+    out[ot] = static_cast<T>(ldexp(sqrt(a2), kexp[b]));
+    const T pt = static_cast<T>(phase);
+    out[plane + ot] = pt == -pi_t ? pi_t : pt;
+    out[2 * plane + ot] = static_cast<T>(freq);
+    if (D >= 2) {
+      for (int a = 0; a < D; ++a) out[3 * plane + ((b0 + b) * D + a) * g.count + i] = nt[a];
+      out[(3 + D) * plane + ot] = C[at];
+    }
+  });
 }
 
 template const AxisMats<float>& axis_mats<float>(int, int32_t);
@@ -618,5 +743,9 @@ template void enqueue_orientation<float>(hipStream_t, int, const Geom&, const fl
                                          float*, float*, float*, int32_t*, double*);
 template void enqueue_orientation<double>(hipStream_t, int, const Geom&, const double*, const int32_t*, const int32_t*,
                                           double*, double*, double*, int32_t*, double*);
+template void enqueue_monogenic<float>(hipStream_t, int, const Geom&, const float*, const int32_t*, double, float*,
+                                       double*, float*, int64_t, int64_t);
+template void enqueue_monogenic<double>(hipStream_t, int, const Geom&, const double*, const int32_t*, double, double*,
+                                        double*, double*, int64_t, int64_t);
 
 }  // namespace eda

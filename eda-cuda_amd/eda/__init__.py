@@ -14,7 +14,7 @@ import operator
 from typing import Any
 
 from . import _native
-from ._api import DecompositionInfo, Orientation, decompose
+from ._api import DecompositionInfo, Monogenic, Orientation, decompose, monogenic
 from ._errors import CudaError, DecompositionError
 
 __version__ = "0.1.0"
@@ -23,12 +23,14 @@ __all__ = [
     "CudaError",
     "DecompositionError",
     "DecompositionInfo",
+    "Monogenic",
     "Orientation",
     "__version__",
     "build_info",
     "decompose",
     "device_count",
     "device_info",
+    "monogenic",
 ]
 
 

@@ -75,6 +75,29 @@ class Orientation:
     coherence: Any
 
 
+@dataclass(frozen=True, eq=False)
+class Monogenic:
+    """The monogenic signal of every field, from ``eda.monogenic`` (the reference's
+    src/eda/_api.py:86-108).
+
+    - ``amplitude``, ``phase``, ``frequency``: the shape of ``x``. The phase lies in (-pi, pi]; the
+      frequency is in cycles per sample along ``direction``.
+    - ``direction``: shape ``x.shape[:m] + (D,) + x.shape[m:]`` with ``m = min(axes)``, the unit
+      direction across the local stripes, component ``a`` along the ``a``-th axis of ``axes``, its
+      first component of magnitude above the dtype's default tie tolerance positive; ``None`` in
+      1-D.
+    - ``coherence``: the shape of ``x``, from 0 (no direction) to 1; ``None`` in 1-D.
+
+    NumPy arrays for host input, PyTorch tensors on the input's device for ROCm input, in the
+    compute dtype."""
+
+    amplitude: Any
+    phase: Any
+    frequency: Any
+    direction: Any
+    coherence: Any
+
+
 # ----------------------------------------------------------------------------------------------
 # Arguments (reference src/eda/_api.py:433-526)
 
@@ -263,46 +286,28 @@ def _info_of(res: _native.Result, batch_shape: tuple[int, ...], dtype: str, axes
     )
 
 
-def decompose(
-    x: Any,
-    axes: int | Sequence[int] | None = None,
-    max_imf: int = -1,
-    *,
-    dtype: Any = None,
-    tie_tolerance: float | None = None,
-    sifts: int = 3,
-    check_finite: bool = True,
-    errors: str = "raise",
-    stream: Any = None,
-    return_info: bool = False,
-    return_orientation: bool = False,
-) -> Any:
-    """Decompose every 1-D, 2-D or 3-D field of ``x`` into intrinsic mode functions (IMFs) and a
-    residue, on an AMD Instinct GPU (gfx950).
+@dataclass
+class _Call:
+    """One call's input as the C ABI takes it: the descriptor, the array it points into (kept alive
+    until the call returns), the layout and the stream to wait for."""
 
-    The arguments, the result layout ``x.shape[:m] + (P,) + x.shape[m:]`` with ``m = min(axes)``,
-    the information and the error policy are those of the reference's ``eda.decompose``: each batch
-    element is decomposed on its own; along the mode axis each element holds its IMFs from fine to
-    coarse, then its residue, then zero rows; ``P`` is one more than the largest IMF count over the
-    elements without an error. Every spatial extent must be at least 2 and at most 8192 (float32)
-    or 4096 (float64). ``return_orientation=True`` (2 or 3 spatial axes) also returns an
-    ``Orientation``: the direction and the coherence that the method computes at the start of each
-    mode. The result is ``y``, then ``info`` if ``return_info``, then the ``Orientation`` if
-    ``return_orientation``; asking for the orientation does not change ``y`` or ``info``.
+    desc: _native.ArrayDesc
+    keep_alive: Any
+    on_device: bool
+    target: str
+    code: int
+    ndim: int
+    axes: tuple[int, ...]
+    batch_axes: tuple[int, ...]
+    spatial: tuple[int, ...]
+    batch_shape: tuple[int, ...]
+    B: int
+    stream_handle: int
 
-    Host input returns NumPy arrays; a ROCm device array (a PyTorch-ROCm tensor, or a DLPack
-    producer on a ROCm device) returns a PyTorch tensor on the same device. The call waits for the
-    current PyTorch stream of the device (or ``stream``, a ``torch.cuda.Stream`` or a raw HIP
-    stream handle) before it reads device input, and its result is complete on return."""
-    max_imf = _max_imf(max_imf)
-    tie = _tie_tolerance(tie_tolerance)
-    sifts = _sifts(sifts)
-    check_finite = _flag(check_finite, "check_finite")
-    return_info = _flag(return_info, "return_info")
-    return_orientation = _flag(return_orientation, "return_orientation")
-    if not isinstance(errors, str) or errors not in _ERRORS:
-        raise ValueError(f"eda: errors={errors!r} is not one of 'raise', 'status'")
 
+def _prepare(x: Any, axes: Any, dtype: Any, stream: Any) -> _Call:
+    """The intake of ``decompose`` and ``monogenic`` (reference src/eda/_api.py:143-181): the dtype
+    rules, the axes, the batch layout, host staging or the device tensor read in place."""
     on_device = _is_device_input(x)
     if on_device:
         data = _device_tensor(x)
@@ -326,16 +331,10 @@ def decompose(
         raise TypeError(f"eda: dtype= must be float32 or float64, got {target}")
     ndim = len(shape)
     axes = _normalize_axes(axes, ndim)
-    if return_orientation and len(axes) < 2:
-        raise ValueError(
-            "eda: return_orientation=True needs 2 or 3 spatial axes, got 1 (the method has no "
-            "orientation in 1-D)"
-        )
     batch_axes = tuple(i for i in range(ndim) if i not in axes)
     spatial = tuple(shape[a] for a in axes)
     batch_shape = tuple(shape[i] for i in batch_axes)
     B = math.prod(batch_shape)
-    out_final = _restore_order(ndim, batch_axes, axes)
     order = list(batch_axes) + list(axes)
     code = 0 if target == "float32" else 1
 
@@ -379,6 +378,62 @@ def decompose(
         desc.data = xb.ctypes.data if xb.size else None
         desc.device = -1
 
+    return _Call(desc, keep_alive, on_device, target, code, ndim, axes, batch_axes, spatial, batch_shape, B,
+                 stream_handle)
+
+
+def decompose(
+    x: Any,
+    axes: int | Sequence[int] | None = None,
+    max_imf: int = -1,
+    *,
+    dtype: Any = None,
+    tie_tolerance: float | None = None,
+    sifts: int = 3,
+    check_finite: bool = True,
+    errors: str = "raise",
+    stream: Any = None,
+    return_info: bool = False,
+    return_orientation: bool = False,
+) -> Any:
+    """Decompose every 1-D, 2-D or 3-D field of ``x`` into intrinsic mode functions (IMFs) and a
+    residue, on an AMD Instinct GPU (gfx950).
+
+    The arguments, the result layout ``x.shape[:m] + (P,) + x.shape[m:]`` with ``m = min(axes)``,
+    the information and the error policy are those of the reference's ``eda.decompose``: each batch
+    element is decomposed on its own; along the mode axis each element holds its IMFs from fine to
+    coarse, then its residue, then zero rows; ``P`` is one more than the largest IMF count over the
+    elements without an error. Every spatial extent must be at least 2 and at most 8192 (float32)
+    or 4096 (float64). ``return_orientation=True`` (2 or 3 spatial axes) also returns an
+    ``Orientation``: the direction and the coherence that the method computes at the start of each
+    mode. The result is ``y``, then ``info`` if ``return_info``, then the ``Orientation`` if
+    ``return_orientation``; asking for the orientation does not change ``y`` or ``info``.
+
+    Host input returns NumPy arrays; a ROCm device array (a PyTorch-ROCm tensor, or a DLPack
+    producer on a ROCm device) returns a PyTorch tensor on the same device. The call waits for the
+    current PyTorch stream of the device (or ``stream``, a ``torch.cuda.Stream`` or a raw HIP
+    stream handle) before it reads device input, and its result is complete on return."""
+    max_imf = _max_imf(max_imf)
+    tie = _tie_tolerance(tie_tolerance)
+    sifts = _sifts(sifts)
+    check_finite = _flag(check_finite, "check_finite")
+    return_info = _flag(return_info, "return_info")
+    return_orientation = _flag(return_orientation, "return_orientation")
+    if not isinstance(errors, str) or errors not in _ERRORS:
+        raise ValueError(f"eda: errors={errors!r} is not one of 'raise', 'status'")
+
+    call = _prepare(x, axes, dtype, stream)
+    if return_orientation and len(call.axes) < 2:
+        raise ValueError(
+            "eda: return_orientation=True needs 2 or 3 spatial axes, got 1 (the method has no "
+            "orientation in 1-D)"
+        )
+    desc, keep_alive, on_device, target, code = call.desc, call.keep_alive, call.on_device, call.target, call.code
+    ndim, axes, batch_axes, spatial, batch_shape, B = (call.ndim, call.axes, call.batch_axes, call.spatial,
+                                                       call.batch_shape, call.B)
+    stream_handle = call.stream_handle
+    out_final = _restore_order(ndim, batch_axes, axes)
+    del call
     res = _call_native(desc, max_imf, tie, sifts, check_finite, stream_handle, return_orientation)
     try:
         info = _info_of(res, batch_shape, target, axes)
@@ -458,3 +513,72 @@ def _orientation(res: _native.Result, batch_shape: tuple[int, ...], spatial: tup
         a = a.reshape(sh)
         arrays.append(a.transpose(pm) if pm != list(range(len(pm))) else a)
     return Orientation(*arrays)
+
+
+def monogenic(
+    x: Any,
+    axes: int | Sequence[int] | None = None,
+    *,
+    dtype: Any = None,
+    check_finite: bool = True,
+    stream: Any = None,
+) -> Monogenic:
+    """The monogenic signal of every 1-D, 2-D or 3-D field of ``x`` on an AMD Instinct GPU (gfx950):
+    its local amplitude, phase, frequency and, in 2-D and 3-D, the direction and the coherence of
+    the local stripes (reference src/eda/_api.py:356-426).
+
+    ``x``, ``axes``, ``dtype``, ``check_finite`` and ``stream`` follow the rules of
+    ``eda.decompose``. Each field is processed on its own, and its result is the same, bit for bit,
+    alone or at any position of any batch. For a field f with the half-sample symmetric extension
+    at the borders, the first-order Riesz transform R1 = (R_a f), the second-order one R2 = (R_a R_b
+    f) and the direction n (the top eigenvector of T = R1 R1^T + R2 R2^T in 2-D and 3-D, 1 in 1-D),
+    with q = n . R1: amplitude = sqrt(f^2 + |R1|^2); phase = atan2(q, f) in (-pi, pi]; frequency =
+    (f dq/dn - q df/dn) / (2 pi amplitude^2) in cycles per sample along n; coherence = (mu1 - mu2)
+    / (mu1 + mu2) from the two largest eigenvalues of T. The phase and the frequency are 0 where the
+    amplitude vanishes. With ``check_finite=True`` a field that holds inf or NaN raises
+    ``ValueError``."""
+    check_finite = _flag(check_finite, "check_finite")
+    call = _prepare(x, axes, dtype, stream)
+    D, B = len(call.axes), call.B
+    count = math.prod(call.spatial)
+    planes = 3 if D == 1 else 4 + D
+    n = B * count
+    lib = _native.core()
+    err = _native.errbuf()
+    nonfinite = np.zeros(max(B, 1), np.int32)
+    chunk = ctypes.c_int64(0)
+    if call.on_device:
+        torch = _torch()
+        tdt = torch.float32 if call.code == 0 else torch.float64
+        out = torch.empty(planes * n, dtype=tdt, device=call.keep_alive.device)
+        ptr, out_device = (out.data_ptr() if n else None), int(call.desc.device)
+    else:
+        out = np.empty(planes * n, dtype=np.dtype(call.target))
+        ptr, out_device = (out.ctypes.data if n else None), -1
+    _native.check(lib.eda_monogenic(ctypes.byref(call.desc), 1 if check_finite else 0, 0, call.stream_handle,
+                                    ctypes.c_void_p(ptr), out_device,
+                                    nonfinite.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), ctypes.byref(chunk),
+                                    err, len(err)), err)
+    bad = np.flatnonzero(nonfinite[:B])
+    if check_finite and bad.size:
+        shown = ", ".join(str(int(i)) for i in bad[:8]) + (", ..." if bad.size > 8 else "")
+        raise ValueError(
+            f"eda: the input has inf or NaN at batch index {shown} ({bad.size} of {B} field(s)); check_finite=True"
+        )
+    # the planes: amplitude, phase, frequency, then in 2-D and 3-D the direction (D components per
+    # sample) and the coherence
+    parts, offset = [], 0
+    for comps in ((0, 0, 0, D, 0) if D >= 2 else (0, 0, 0)):
+        extra = (comps,) if comps else ()
+        computed = call.batch_shape + extra + call.spatial
+        perm = _restore_order(call.ndim, call.batch_axes, call.axes, len(extra))
+        size = n * max(comps, 1)
+        part = out[offset:offset + size]
+        if call.on_device:
+            parts.append(part.view(computed).permute(perm))
+        else:
+            part = part.reshape(computed)
+            parts.append(part.transpose(perm) if perm != list(range(len(perm))) else part)
+        offset += size
+    del call
+    return Monogenic(*parts) if D >= 2 else Monogenic(*parts, None, None)

@@ -12,6 +12,9 @@ the reference's ``_native.core()`` can return it in place of ``eda._core`` on a 
   decompose_device(ptr, device, dtype, target, batch_shape, batch_strides, shape, strides, fences,
                    max_imf, tie_tolerance, sifts, check_finite, orientation, max_chunk)
       -> (owner, out_ptr, info dict); rows C-contiguous (B, P, *spatial) on ``device``
+  monogenic_host(x, check_finite, max_chunk) -> (out flat planes NumPy array, info dict)
+  monogenic_device(ptr, device, dtype, target, batch_shape, batch_strides, shape, strides, fences,
+                   check_finite, max_chunk) -> (owner, out_ptr, info dict)
   build_info(), device_count(), device_info(device=0), pointer_device(ptr)
 
 info holds ``n_imfs``, ``error`` [B] int32; ``windows``, ``sifts`` [B, K] int32; ``spacing``,
@@ -19,9 +22,11 @@ info holds ``n_imfs``, ``error`` [B] int32; ``windows``, ``sifts`` [B, K] int32;
 D, *spatial) and ``coherence`` (B, K, *spatial) arrays, for device input ``orientation`` = (owner,
 ptr) of one buffer holding the direction followed by the coherence ((None, 0) when B or K is 0).
 Errors raise the reference's classes: ValueError, MemoryError, ``eda.CudaError``, RuntimeError.
-Not served: device input whose dtype differs from ``target`` or whose batch axes do not merge into
-one stride (ValueError), and the monogenic and DLPack helpers. The device path allocates its output through
-PyTorch-ROCm, whose tensor is the ``owner``."""
+The monogenic calls' info holds ``nonfinite`` [B] int32 and ``chunk``; their output holds the planes
+amplitude, phase, frequency (B, *spatial) each, then for 2 or 3 spatial axes the direction (B, D,
+*spatial) and the coherence (B, *spatial). Not served: device input whose dtype differs from
+``target`` or whose batch axes do not merge into one stride (ValueError), and the DLPack helpers.
+The device path allocates its output through PyTorch-ROCm, whose tensor is the ``owner``."""
 
 from __future__ import annotations
 
@@ -226,3 +231,80 @@ def decompose_device(ptr: int, device: int, dtype: str, target: str, batch_shape
     finally:
         _native.core().eda_release(ctypes.byref(res))
     return owner, owner.data_ptr(), info
+
+
+def _monogenic_planes(dim: int) -> int:
+    return 3 if dim == 1 else 4 + dim
+
+
+def monogenic_host(x, check_finite: bool, max_chunk: int) -> tuple:
+    """The monogenic signal of the C-contiguous host array x (B, *spatial), float32 or float64: (out,
+    info), out the flat planes (reference cpp/python/monogenic.cpp, src/eda/_core.pyi:25-28)."""
+    x = np.asarray(x)
+    if x.dtype.name not in _CODES:
+        raise TypeError(f"eda: monogenic_host needs float32 or float64, got {x.dtype}")
+    if not x.flags.c_contiguous:
+        raise ValueError("eda: monogenic_host needs a C-contiguous array")
+    if not 2 <= x.ndim <= 4:
+        raise ValueError(f"eda: expected (B, *spatial) with 1 to 3 spatial axes, got {x.ndim} axes")
+    desc = _native.ArrayDesc()
+    desc.dtype = _CODES[x.dtype.name]
+    desc.dim = x.ndim - 1
+    desc.batch = x.shape[0]
+    for a, n in enumerate(x.shape[1:]):
+        desc.n[a] = n
+    desc.device = -1
+    desc.data = x.ctypes.data if x.size else None
+    out = np.empty(_monogenic_planes(desc.dim) * x.size, dtype=x.dtype)
+    nonfinite = np.zeros(max(x.shape[0], 1), np.int32)
+    chunk = ctypes.c_int64(0)
+    err = _native.errbuf()
+    _native.check(_native.core().eda_monogenic(
+        ctypes.byref(desc), 1 if check_finite else 0, int(max_chunk), 0,
+        ctypes.c_void_p(out.ctypes.data if out.size else None), -1,
+        nonfinite.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), ctypes.byref(chunk), err, len(err)), err)
+    return out, {"nonfinite": nonfinite[:x.shape[0]].copy(), "chunk": int(chunk.value)}
+
+
+def monogenic_device(ptr: int, device: int, dtype: str, target: str, batch_shape: Sequence[int],
+                     batch_strides: Sequence[int], shape: Sequence[int], strides: Sequence[int],
+                     fences: Sequence[int], check_finite: bool, max_chunk: int) -> tuple:
+    """The monogenic signal of a ROCm device array after waiting for the fence streams: (owner,
+    out_ptr, info), the planes of monogenic_host in one device buffer, complete on return."""
+    import torch  # noqa: PLC0415 - the device path allocates its output through PyTorch-ROCm
+
+    if target not in _CODES:
+        raise ValueError(f"eda: target must be float32 or float64, got {target}")
+    if dtype != target:
+        raise ValueError(f"eda: device input of dtype {dtype} converted to {target} is not supported by this build")
+    if not 1 <= len(shape) <= 3 or len(strides) != len(shape):
+        raise ValueError("eda: expected 1 to 3 spatial axes with one stride each")
+    B = math.prod(int(n) for n in batch_shape)
+    count = math.prod(int(n) for n in shape)
+    fences = [int(f) for f in fences]
+    stream = 0
+    if fences:
+        stream = fences[0]
+        if len(fences) > 1 or stream in (1, 2):
+            torch.cuda.synchronize(int(device))
+            stream = 0
+    desc = _native.ArrayDesc()
+    desc.dtype = _CODES[target]
+    desc.dim = len(shape)
+    desc.batch = B
+    for a, (n, st) in enumerate(zip(shape, strides)):
+        desc.n[a] = int(n)
+        desc.strides[a] = int(st)
+    desc.batch_stride = _merged_batch_stride(batch_shape, batch_strides)
+    desc.device = int(device)
+    desc.data = int(ptr) if B and count else None
+    tdt = torch.float32 if target == "float32" else torch.float64
+    owner = torch.empty(_monogenic_planes(len(shape)) * B * count, dtype=tdt, device=torch.device("cuda", int(device)))
+    nonfinite = np.zeros(max(B, 1), np.int32)
+    chunk = ctypes.c_int64(0)
+    err = _native.errbuf()
+    _native.check(_native.core().eda_monogenic(
+        ctypes.byref(desc), 1 if check_finite else 0, int(max_chunk), stream,
+        ctypes.c_void_p(owner.data_ptr() if owner.numel() else None), int(device),
+        nonfinite.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), ctypes.byref(chunk), err, len(err)), err)
+    return owner, owner.data_ptr(), {"nonfinite": nonfinite[:B].copy(), "chunk": int(chunk.value)}

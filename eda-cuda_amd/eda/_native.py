@@ -71,6 +71,8 @@ SIGNATURES = {
     "eda_decompose": ([_c.POINTER(ArrayDesc), _c.POINTER(Params), _c.POINTER(Result), *_err], _c.c_int),
     "eda_collect": ([_c.POINTER(Result), _vp, _i32, *_err], _c.c_int),
     "eda_collect_orientation": ([_c.POINTER(Result), _vp, _vp, _i32, *_err], _c.c_int),
+    "eda_monogenic": ([_c.POINTER(ArrayDesc), _i32, _i64, _c.c_uint64, _vp, _i32, _i32p, _c.POINTER(_i64), *_err],
+                      _c.c_int),
     "eda_release": ([_c.POINTER(Result)], None),
     "eda_build_info": ([_c.c_char_p, _c.c_size_t], _c.c_int),
     "eda_device_count": ([], _c.c_int),

@@ -101,6 +101,18 @@ int eda_collect(const eda_result* r, void* dst, int32_t dst_device, char* err, s
  * (cpp/src/decompose.hpp:100-103, cpp/src/method/driver.cpp:177-196). */
 int eda_collect_orientation(const eda_result* r, void* dir, void* coh, int32_t dst_device, char* err, size_t errlen);
 
+/* The monogenic signal of every element of x (reference `eda._core.monogenic_host` /
+ * `monogenic_device`, cpp/python/monogenic.cpp; cpp/src/method/monogenic.cpp:61-158,
+ * orientation.cu:420-493,789-831). out: the planes, C-contiguous in the compute dtype, one after the
+ * other: amplitude, phase, frequency (batch, *spatial) each, then for dim >= 2 the direction
+ * (batch, dim, *spatial) and the coherence (batch, *spatial): 3 planes for dim 1, 4 + dim
+ * otherwise. out_device -1: host memory; else device memory of that device. nonfinite (host
+ * [batch], may be NULL): 1 where check_finite and the element holds inf or NaN (its outputs are
+ * still computed). chunk (may be NULL): the elements per chunk. max_chunk > 0 bounds it; stream: a
+ * hipStream_t the call waits for before it reads device input (0: none). Complete on return. */
+int eda_monogenic(const eda_array_desc* x, int32_t check_finite, int64_t max_chunk, uint64_t stream, void* out,
+                  int32_t out_device, int32_t* nonfinite, int64_t* chunk, char* err, size_t errlen);
+
 /* Frees the rows and the info arrays of a result (idempotent). */
 void eda_release(eda_result* r);
 
@@ -111,12 +123,13 @@ int eda_device_count(void);                                  /* 0 without a driv
 int eda_device_info(int32_t device, char* buf, size_t len);  /* a JSON object */
 int eda_pointer_device(const void* ptr, int32_t* device);    /* -1 for host memory */
 
-/* Kernel timing, for the live roofline figure of a benchmark: while enabled, every launch of the
+/* Kernel timing, for the live roofline figure of a benchmark: with on = 1, every launch of the
  * cube filter's pass kernel is bracketed by HIP events on the stream it runs on, and its algorithmic
  * bytes (input fields read once, output fields written once, over the running elements) are
- * counted. eda_timer_read waits for the recorded launches, returns the total kernel time in ms, the
- * launch count and the bytes, and clears them. No counterpart in the reference (it has no timing
- * in code). */
+ * counted; with on = 2, every launch of the DCT contraction (the transform GEMMs) instead, with its
+ * flops (2 M N K per batch entry) in place of the bytes; 0 turns it off. eda_timer_read waits for
+ * the recorded launches, returns the total kernel time in ms, the launch count and the bytes (or
+ * flops), and clears them. No counterpart in the reference (it has no timing in code). */
 int eda_timer_enable(int32_t on);
 int eda_timer_read(double* ms, int64_t* launches, double* bytes, char* err, size_t errlen);
 

@@ -28,6 +28,8 @@
 //   ccl                       scipy.ndimage.label after canonical relabelling.
 //   dct / riesz               scipy.fft.dctn / idctn(norm='ortho') and DCT-basis plane waves.
 //   eigen                     numpy.linalg.eigh.
+//   monogenic                 DCT-basis fields (exact amplitude 1, phase, frequency, direction,
+//                              coherence 1) and a NumPy FFT computation of the same definitions.
 //   rung_ladder               the KAT cpp/tests/rungs_table.inc (66 rows).
 //   interp (gather)           scipy.ndimage.map_coordinates(order=1) on the reflected extension.
 //   decompose (T = double)    the golden corpus tests/data/golden (9 FP64 cases: decisions exact,
@@ -935,6 +937,139 @@ bool orientation(const OrientPlan& P, const T* r, int32_t w, std::vector<std::ve
 }
 
 // ------------------------------------------------------------------------------------------------
+// The monogenic signal (cpp/src/method/orientation.cu:1-26,86-128,420-493,789-831;
+// monogenic.cpp:61-158): f = x 2^-k with k = ilogb(max |x|) (0 for a zero or non-finite max),
+// X = the orthonormal DCT-II of f - mean(f) (rounded to T), no band, and the components of the whole
+// spectrum by the inverse transform with the sin basis on their odd axes, each multiplier 0 at the DC:
+//   R1_a u_a / rad (sin a); R2_aa -(u_a / rad)^2; R2_ab (u_a / rad)(u_b / rad) (sin a, b);
+//   D_a -u_a (sin a); D_a R_a u_a (u_a / rad); D_b R_a -u_a (u_b / rad) (sin a, b), a < b.
+// D >= 2: n, c from T = R1 R1^T + R2 R2^T (n with the canonical sign: its first component of
+// magnitude above the dtype's default tie tolerance positive); n = 1 in 1-D. Per voxel in double:
+// q = n . R1, A^2 = f^2 + |R1|^2, D_n f = n . D f, D_n q = sum_{a <= b} m_ab D_b R_a (m_aa = n_a^2,
+// m_ab = 2 n_a n_b); where A^2 > tiny: phase = atan2(q, f) (a phase that rounds to -pi in T is pi),
+// frequency = (f D_n q - q D_n f) / A^2, else both 0; amplitude = sqrt(A^2) 2^k rounded once.
+// out: amplitude, phase, frequency, then (D >= 2) the D direction planes and the coherence, count
+// samples each.
+template <class T>
+void monogenic(const OrientPlan& P, const T* x, T* out) {
+  const Shape& s = P.s;
+  const int D = s.D;
+  const int64_t N = s.count;
+  double amax = 0.0;
+  for (int64_t i = 0; i < N; ++i) amax = std::fmax(amax, std::fabs(static_cast<double>(x[i])));
+  bool finite = true;
+  for (int64_t i = 0; i < N; ++i) finite = finite && std::isfinite(static_cast<double>(x[i]));
+  const int k = (amax > 0.0 && finite) ? std::ilogb(amax) : 0;
+  std::vector<T> f(static_cast<size_t>(N));
+  for (int64_t i = 0; i < N; ++i) f[i] = static_cast<T>(std::ldexp(static_cast<double>(x[i]), -k));
+  double mean = 0.0;
+  for (int64_t i = 0; i < N; ++i) mean += static_cast<double>(f[i]);
+  mean = mean / static_cast<double>(N);
+  std::vector<double> X(static_cast<size_t>(N));
+  for (int64_t i = 0; i < N; ++i) X[i] = static_cast<double>(static_cast<T>(static_cast<double>(f[i]) - mean));
+  for (int a = 0; a < D; ++a) axis_apply(s, a, P.ax[a].fwd, X);
+  std::vector<double> rad(static_cast<size_t>(N)), uk(static_cast<size_t>(D * N));
+  for (int64_t i = 0; i < N; ++i) {
+    int32_t kk[3];
+    coords_of(s, i, kk);
+    double u[3];
+    rad[i] = dct_freq(s, kk, u);
+    for (int a = 0; a < D; ++a) uk[a * N + i] = u[a];
+    X[i] = static_cast<double>(static_cast<T>(X[i]));
+  }
+  auto component = [&](int kind, int a, int b) {
+    std::vector<double> y(static_cast<size_t>(N));
+    for (int64_t i = 0; i < N; ++i) {
+      double m = 0.0;
+      if (rad[i] > 0.0) {
+        const double inv = 1.0 / rad[i];
+        const double ua = uk[a * N + i] * inv;
+        if (kind == 0) m = ua;
+        else if (kind == 1) m = (-ua) * ua;
+        else if (kind == 2) m = ua * (uk[b * N + i] * inv);
+        else if (kind == 3) m = -uk[a * N + i];
+        else m = a == b ? uk[a * N + i] * ua : -(uk[a * N + i] * (uk[b * N + i] * inv));
+      }
+      y[i] = static_cast<double>(static_cast<T>(m * X[i]));
+    }
+    int odd = 0;
+    if (kind == 0 || kind == 3) odd = 1 << a;
+    else if (kind == 2 || (kind == 4 && a != b)) odd = (1 << a) | (1 << b);
+    for (int ax = 0; ax < D; ++ax) axis_apply(s, ax, ((odd >> ax) & 1) ? P.ax[ax].isin : P.ax[ax].icos, y);
+    std::vector<T> o(static_cast<size_t>(N));
+    for (int64_t i = 0; i < N; ++i) o[i] = static_cast<T>(y[i]);
+    return o;
+  };
+  const int K = D * (D + 1) / 2;
+  std::vector<std::vector<T>> r1(D), df(D), dr(K);
+  for (int a = 0; a < D; ++a) r1[a] = component(0, a, a);
+  for (int a = 0; a < D; ++a) df[a] = component(3, a, a);
+  for (int a = 0; a < D; ++a)
+    for (int b = a; b < D; ++b) dr[tensor_pair(D == 1 ? 2 : D, a, b)] = component(4, a, b);
+  std::vector<std::vector<T>> nv(D, std::vector<T>(static_cast<size_t>(N), T(1)));
+  std::vector<T> coh(static_cast<size_t>(N), T(0));
+  if (D >= 2) {
+    std::vector<std::vector<T>> r2(K);
+    for (int a = 0; a < D; ++a)
+      for (int b = a; b < D; ++b) r2[tensor_pair(D, a, b)] = component(a == b ? 1 : 2, a, b);
+    for (int64_t i = 0; i < N; ++i) {
+      T out6[6];
+      for (int a = 0; a < D; ++a)
+        for (int c = a; c < D; ++c) {
+          T v = r1[a][i] * r1[c][i];
+          for (int q = 0; q < D; ++q) v = v + r2[tensor_pair(D, a, q)][i] * r2[tensor_pair(D, q, c)][i];
+          out6[tensor_pair(D, a, c)] = v;
+        }
+      double t[6], n3[3], cv;
+      for (int p = 0; p < K; ++p) t[p] = static_cast<double>(out6[p]);
+      eigen(D, t, n3, &cv);
+      coh[i] = static_cast<T>(cv);
+      for (int a = 0; a < D; ++a) nv[a][i] = static_cast<T>(n3[a]);
+    }
+  }
+  const T zero = static_cast<T>(std::is_same_v<T, float> ? std::ldexp(1.0, -14) : std::ldexp(1.0, -30));
+  const T pi_t = static_cast<T>(3.141592653589793);
+  for (int64_t i = 0; i < N; ++i) {
+    T nt[3];
+    int sgn = 0;
+    for (int a = 0; a < D; ++a) {
+      nt[a] = nv[a][i];
+      if (D >= 2 && sgn == 0 && std::fabs(nt[a]) > zero) sgn = nt[a] > T(0) ? 1 : -1;
+    }
+    for (int a = 0; a < D; ++a) nt[a] = sgn < 0 ? -nt[a] : nt[a];
+    double n[3];
+    for (int a = 0; a < D; ++a) n[a] = static_cast<double>(nt[a]);
+    const double fv = static_cast<double>(f[i]);
+    double q = 0.0, a2 = fv * fv, dnf = 0.0, dnq = 0.0;
+    for (int a = 0; a < D; ++a) {
+      const double r = static_cast<double>(r1[a][i]);
+      q = q + n[a] * r;
+      a2 = a2 + r * r;
+      dnf = dnf + n[a] * static_cast<double>(df[a][i]);
+    }
+    for (int a = 0; a < D; ++a)
+      for (int c = a; c < D; ++c) {
+        const double m = a == c ? n[a] * n[a] : 2.0 * (n[a] * n[c]);
+        dnq = dnq + m * static_cast<double>(dr[tensor_pair(D == 1 ? 2 : D, a, c)][i]);
+      }
+    double phase = 0.0, freq = 0.0;
+    const double tiny = std::is_same_v<T, float> ? static_cast<double>(FLT_MIN) : 1e-300;
+    if (a2 > tiny) {
+      phase = std::atan2(q, fv);
+      freq = (fv * dnq - q * dnf) / a2;
+    }
+    out[i] = static_cast<T>(std::ldexp(std::sqrt(a2), k));
+    const T pt = static_cast<T>(phase);
+    out[N + i] = pt == -pi_t ? pi_t : pt;
+    out[2 * N + i] = static_cast<T>(freq);
+    if (D >= 2) {
+      for (int a = 0; a < D; ++a) out[(3 + a) * N + i] = nt[a];
+      out[(3 + D) * N + i] = coh[i];
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
 // The element (cpp/src/method/element.hpp:1-29, element.cuh:53-69, element.cu:166-303).
 template <class T>
 struct DE {
@@ -1650,6 +1785,15 @@ int oracle_orientation(int dtype, const void* r, int dim, const int64_t* n, int3
   for (int a = 0; a < dim; ++a) std::copy(nv[a].begin(), nv[a].end(), static_cast<double*>(nvec) + a * s.count);
   std::copy(c.begin(), c.end(), static_cast<double*>(coh));
   return ok;
+}
+
+// The monogenic signal of one element into out (3 planes for dim 1, 4 + dim otherwise, count samples
+// each: amplitude, phase, frequency, the direction components, the coherence).
+void oracle_monogenic(int dtype, const void* x, int dim, const int64_t* n, void* out) {
+  const orc::Shape s = orc::make_shape(dim, n);
+  const orc::OrientPlan P = orc::orient_plan(s);
+  if (dtype == 0) orc::monogenic<float>(P, static_cast<const float*>(x), static_cast<float*>(out));
+  else orc::monogenic<double>(P, static_cast<const double*>(x), static_cast<double*>(out));
 }
 
 // The Riesz components (R1: dim fields, R2: dim (dim + 1) / 2 fields) in double.

@@ -59,6 +59,8 @@ def lib() -> ctypes.CDLL:
         L.oracle_orientation.restype = ctypes.c_int
         L.oracle_riesz.argtypes = [_f64p, ctypes.c_int, _i64p, ctypes.c_int32, _f64p, _f64p]
         L.oracle_eigen.argtypes = [ctypes.c_int, _f64p, ctypes.c_int64, _f64p, _f64p]
+        L.oracle_monogenic.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, _i64p, ctypes.c_void_p]
+        L.oracle_monogenic.restype = None
         L.oracle_rung_ladder.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int32, ctypes.c_int32, _i32p]
         L.oracle_rung_ladder.restype = ctypes.c_int
         L.oracle_interp.argtypes = [_f64p, ctypes.c_int, _i64p, _f64p, ctypes.c_int64, _f64p]
@@ -214,6 +216,21 @@ def orientation(r: np.ndarray, w: int):
     c = np.empty(r.shape, r.dtype)
     ok = lib().oracle_orientation(_code(r.dtype), _p(r), r.ndim, _shape(r.shape), w, _p(n), _p(c))
     return n, c, bool(ok)
+
+
+def monogenic(x: np.ndarray) -> dict:
+    """The monogenic signal of one field x (*spatial) in x's dtype: amplitude, phase, frequency (the
+    shape of x) and, in 2-D and 3-D, direction (D, *spatial) and coherence (None in 1-D)."""
+    x = np.ascontiguousarray(x)
+    D = x.ndim
+    planes = 3 if D == 1 else 4 + D
+    out = np.empty((planes,) + x.shape, x.dtype)
+    lib().oracle_monogenic(_code(x.dtype), _p(x), D, _shape(x.shape), _p(out))
+    res = {"amplitude": out[0], "phase": out[1], "frequency": out[2], "direction": None, "coherence": None}
+    if D >= 2:
+        res["direction"] = out[3:3 + D]
+        res["coherence"] = out[3 + D]
+    return res
 
 
 def riesz(r: np.ndarray, w: int):

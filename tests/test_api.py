@@ -101,6 +101,9 @@ _CORE_SIGNATURES = {
     "device_info": ["device"],
     "pointer_device": ["ptr"],
     "decompose_host": ["x", "max_imf", "tie_tolerance", "sifts", "check_finite", "orientation", "max_chunk"],
+    "monogenic_host": ["x", "check_finite", "max_chunk"],
+    "monogenic_device": ["ptr", "device", "dtype", "target", "batch_shape", "batch_strides", "shape", "strides",
+                         "fences", "check_finite", "max_chunk"],
     "decompose_device": ["ptr", "device", "dtype", "target", "batch_shape", "batch_strides", "shape", "strides",
                          "fences", "max_imf", "tie_tolerance", "sifts", "check_finite", "orientation", "max_chunk"],
 }
@@ -146,3 +149,22 @@ def test_core_hip_batch_axes_that_do_not_merge():
 
     with pytest.raises(ValueError):
         _merged_batch_stride((2, 3), (100, 300))
+
+
+@pytest.mark.parametrize(("kwargs", "error", "fragment"), [
+    (dict(check_finite=1), TypeError, "eda: check_finite must be True or False, got int"),
+    (dict(dtype="float16"), TypeError, "eda: dtype= must be float32 or float64, got float16"),
+    (dict(axes=(0, 0)), ValueError, "eda: axes must be unique"),
+    (dict(axes=()), ValueError, "eda: 1 to 3 spatial axes, got 0"),
+])
+def test_monogenic_argument_errors(kwargs, error, fragment):
+    import re
+
+    with pytest.raises(error, match=re.escape(fragment)):
+        eda.monogenic(np.zeros((4, 16), np.float32), **kwargs)
+
+
+def test_monogenic_result_type():
+    assert set(eda.Monogenic.__dataclass_fields__) == {"amplitude", "phase", "frequency", "direction", "coherence"}
+    with pytest.raises(TypeError, match="expected a float32 or float64 array"):
+        eda.monogenic(np.zeros((4, 16), np.int32))

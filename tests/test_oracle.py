@@ -295,3 +295,165 @@ def test_interp_is_map_coordinates_on_the_reflected_extension():
     # the periodic half-sample reflection of period 2n per axis = map_coordinates(mode='grid-mirror')
     ref = scipy.ndimage.map_coordinates(f, pts.T, order=1, mode="grid-mirror")
     assert np.allclose(got, ref, atol=1e-12)
+
+
+# ----------------------------------------------------------------------------------------------
+# The monogenic signal (reference src/eda/_api.py:356-426, cpp/src/method/orientation.cu:420-493)
+
+MONO_EPS = {np.float32: float(np.finfo(np.float32).eps), np.float64: float(np.finfo(np.float64).eps)}
+MONO_ZERO = {np.float32: 2.0**-14, np.float64: 2.0**-30}
+
+
+def _wrap(d):
+    return np.abs(np.angle(np.exp(1j * np.asarray(d, dtype=np.float64))))
+
+
+def _dct_basis(n, m):
+    """cos(pi m (x + 1/2) / n) and its angle reduced exactly to [0, 2 pi)."""
+    theta = np.pi * ((m * (2 * np.arange(n) + 1)) % (4 * n)) / (2 * n)
+    return np.cos(theta), theta
+
+
+def _waves(shape, seed, offset=0.3):
+    rng = np.random.default_rng(seed)
+    g = np.meshgrid(*[np.arange(n, dtype=np.float64) for n in shape], indexing="ij")
+    out = np.full(shape, float(offset))
+    for period in (5.0, 9.0, 23.0):
+        k = rng.normal(size=len(shape))
+        k = k / np.linalg.norm(k) * 2 * np.pi / period
+        out += rng.uniform(0.5, 1.0) * np.sin(sum(ki * gi for ki, gi in zip(k, g)) + rng.uniform(0, 6))
+    return out
+
+
+def _fft_monogenic(f, zero):
+    """The same definitions through the FFT of the half-sample symmetric extension (2 n per axis),
+    float64, an algorithm independent of the oracle's DCT sums: R_a has the multiplier -i u_a / |u|,
+    d/dx_a / (2 pi) i u_a, D_b R_a u_a u_b / |u|, R_a R_b -u_a u_b / |u|^2 (u in cycles per
+    sample, every multiplier 0 at the DC). Returns the outputs and (mu1, mu2) of T."""
+    f = np.asarray(f, np.float64)
+    D = f.ndim
+    ext = f
+    for a in range(D):
+        ext = np.concatenate([ext, np.flip(ext, axis=a)], axis=a)
+    F = np.fft.fftn(ext)
+    u = np.meshgrid(*[np.fft.fftfreq(2 * n) for n in f.shape], indexing="ij")
+    rad = np.sqrt(sum(v * v for v in u))
+    inv = np.divide(1.0, rad, out=np.zeros_like(rad), where=rad > 0)
+    crop = tuple(slice(0, n) for n in f.shape)
+
+    def back(mult):
+        return np.real(np.fft.ifftn(F * mult))[crop]
+
+    r1 = np.stack([back(-1j * u[a] * inv) for a in range(D)])
+    df = np.stack([back(1j * u[a]) for a in range(D)])
+    pairs = [(a, b) for a in range(D) for b in range(D)]
+    dr = {(a, b): back(u[a] * u[b] * inv) for a, b in pairs}
+    out = {"mu": None}
+    if D == 1:
+        n = np.ones((1,) + f.shape)
+    else:
+        h = np.zeros(f.shape + (D, D))
+        for a, b in pairs:
+            h[..., a, b] = back(-u[a] * u[b] * inv**2)
+        T = np.einsum("a...,b...->...ab", r1, r1) + h @ h
+        w, V = np.linalg.eigh(T)
+        n = np.moveaxis(V[..., :, -1], -1, 0)
+        sign = np.zeros(f.shape)
+        for a in range(D):
+            sign = np.where((sign == 0) & (np.abs(n[a]) > zero), np.sign(n[a]), sign)
+        n = n * np.where(sign < 0, -1.0, 1.0)
+        s = w[..., -1] + w[..., -2]
+        c = np.divide(w[..., -1] - w[..., -2], s, out=np.zeros_like(s), where=s > 0)
+        out.update(direction=n, coherence=np.clip(c, 0, 1), mu=(w[..., -1], w[..., -2]))
+    q = (n * r1).sum(axis=0)
+    a2 = f * f + (r1 * r1).sum(axis=0)
+    dnq = sum(n[a] * n[b] * dr[(a, b)] for a, b in pairs)
+    dnf = (n * df).sum(axis=0)
+    out.update(amplitude=np.sqrt(a2), phase=np.arctan2(q, f), frequency=(f * dnq - q * dnf) / a2)
+    return out
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize(("n", "m"), [(16, 1), (37, 5), (64, 21), (255, 254), (1000, 3)])
+def test_monogenic_of_a_dct_basis_field_1d(n, m, dtype):
+    """cos(pi m (x + 1/2) / n) is a plane wave of frequency m / (2 n) on the extension: amplitude 1,
+    phase pi m (x + 1/2) / n and frequency m / (2 n) at every sample."""
+    f, theta = _dct_basis(n, m)
+    r = oracle.monogenic(f.astype(dtype))
+    eps = MONO_EPS[dtype]
+    assert r["direction"] is None and r["coherence"] is None
+    assert np.abs(r["amplitude"] - 1).max() <= 64 * eps
+    assert np.abs(r["frequency"] - m / (2 * n)).max() <= 24 * eps
+    assert _wrap(r["phase"] - theta).max() <= 48 * eps
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("shape", [(12, 20), (6, 9, 10)])
+def test_monogenic_of_a_dct_basis_field_along_one_axis(shape, dtype):
+    """The basis field along axis a of a 2-D or 3-D field: the same values, direction the unit vector
+    of axis a, coherence 1."""
+    eps = MONO_EPS[dtype]
+    for axis, n in enumerate(shape):
+        m = n // 3 + 1
+        f, theta = _dct_basis(n, m)
+        view = [1] * len(shape)
+        view[axis] = n
+        r = oracle.monogenic(np.broadcast_to(f.reshape(view), shape).astype(dtype))
+        assert np.abs(r["amplitude"] - 1).max() <= 64 * eps
+        assert np.abs(r["frequency"] - m / (2 * n)).max() <= 24 * eps
+        assert _wrap(r["phase"] - np.broadcast_to(theta.reshape(view), shape)).max() <= 48 * eps
+        unit = np.zeros((len(shape),) + (1,) * len(shape))
+        unit[axis] = 1
+        assert np.abs(r["direction"] - unit).max() <= 8 * eps
+        assert np.abs(r["coherence"] - 1).max() <= 4 * eps
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("shape", [(300,), (37, 45), (17, 20, 23)], ids=["1d", "2d", "3d"])
+def test_monogenic_matches_the_fft_computation(shape, dtype, seed):
+    """Against the FFT of the extension in float64, in eps of the field's scale: the amplitude, the
+    phase weighted by A, the frequency by A^2, n by the eigenvalue gap and c by mu1 + mu2 (each
+    output is only as well defined as its weight)."""
+    x = _waves(shape, seed).astype(dtype)
+    r = oracle.monogenic(x)
+    ref = _fft_monogenic(x, MONO_ZERO[dtype])
+    eps, scale = MONO_EPS[dtype], float(np.abs(x).max())
+    A = ref["amplitude"]
+    errors = {
+        "amplitude": np.abs(r["amplitude"] - A).max() / scale,
+        "phase": (_wrap(r["phase"] - ref["phase"]) * A).max() / scale,
+        "frequency": (np.abs(r["frequency"] - ref["frequency"]) * A**2).max() / scale**2,
+    }
+    if ref["mu"] is not None:
+        mu1, mu2 = ref["mu"]
+        dn = np.abs(r["direction"] - ref["direction"]).max(axis=0)
+        errors["direction"] = (dn * (mu1 - mu2)).max() / mu1.max()
+        errors["coherence"] = (np.abs(r["coherence"] - ref["coherence"]) * (mu1 + mu2)).max() / mu1.max()
+    tol = {"amplitude": 32, "phase": 512, "frequency": 16, "direction": 32, "coherence": 64}
+    for name, value in errors.items():
+        assert value / eps <= tol[name], (name, value / eps)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("shape", [(50,), (9, 13), (5, 6, 7)])
+def test_monogenic_of_constant_and_zero_fields(shape, dtype):
+    for value in (0.0, 0.75, -32.0):
+        r = oracle.monogenic(np.full(shape, value, dtype=dtype))
+        assert (r["amplitude"] == abs(value)).all() and (r["frequency"] == 0).all()
+        assert (r["phase"] == (np.asarray(np.pi, dtype) if value < 0 else 0)).all()
+        if len(shape) > 1:
+            assert (r["direction"][0] == 1).all() and (r["direction"][1:] == 0).all()
+            assert (r["coherence"] == 0).all()
+
+
+@pytest.mark.parametrize("j", [-8, 3, 20])
+def test_monogenic_gain_by_a_power_of_two(j):
+    """The field is prescaled to max |f| in [1, 2): a gain 2^j scales the amplitude and leaves every
+    other output's bits."""
+    x = _waves((30, 28), 8)
+    r = oracle.monogenic(x)
+    g = oracle.monogenic(x * 2.0**j)
+    assert np.array_equal(g["amplitude"], r["amplitude"] * 2.0**j)
+    for name in ("phase", "frequency", "direction", "coherence"):
+        assert np.array_equal(g[name], r[name])
