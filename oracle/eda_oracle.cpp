@@ -33,7 +33,10 @@
 //   rung_ladder               the KAT cpp/tests/rungs_table.inc (66 rows).
 //   interp (gather)           scipy.ndimage.map_coordinates(order=1) on the reflected extension.
 //   decompose (T = double)    the golden corpus tests/data/golden (9 FP64 cases: decisions exact,
-//                              rows within the corpus's own bound).
+//                              rows within the corpus's own bound), and the decisions of the
+//                              parity manifest tests/data/parity_v2.json (143 float64 cases).
+//   decompose (T = float)     the decisions of the parity manifest (142 of its 144 float32 cases
+//                              on the CPU; the other two run on the GPU).
 // The G2/G3 stages, the scale field, the floor and the sift kernels are pinned only end to end
 // (golden corpus: g3blobs_2d_64 exercises G2/G3) and by invariants (reconstruction, gain 2^j).
 #include <algorithm>

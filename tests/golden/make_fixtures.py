@@ -7,7 +7,11 @@ text and data only; nothing of the reference is imported or run):
 - quantile_table.json: the NumPy quantile table of cpp/tests/test_stats.cu:282-300
   (numpy.quantile(v, q) for v = [i * i + 0.5 for i in range(n)]);
 - corpus/: a copy of tests/data/golden (9 FP64 cases: inputs, modes, windows, sifts, sensitivity,
-  and its index.json).
+  and its index.json);
+- parity_decisions.json: the decision part of tests/data/parity_v2.json (287 cases): per case its
+  recipe, dtype, call arguments, input_sha256, y_shape, n_imfs, error, windows and sifts (values,
+  or null where the manifest stores only a hash), chunks, conditioning class and quick flag. The
+  manifest's row hashes are CUDA bits, which no other build reproduces, and are left out.
 
 Run in a container that mounts the reference at /root/reference:
     python tests/golden/make_fixtures.py
@@ -48,6 +52,24 @@ def quantile_table() -> dict:
     return {"q": qs, "rows": rows}
 
 
+PARITY_KEYS = ("id", "recipe", "dtype", "quick", "shape", "axes", "max_imf", "tie_tolerance", "check_finite",
+               "errors", "max_chunk", "input_sha256", "y_shape", "n_imfs", "error", "info_shape", "windows",
+               "sifts", "chunks", "conditioning_class")
+
+
+def parity_decisions() -> dict:
+    src = json.loads((REF / "tests/data/parity_v2.json").read_text())
+    cases = []
+    for c in src["cases"]:
+        d = {k: c.get(k) for k in PARITY_KEYS}
+        for k in ("windows", "sifts"):
+            if not isinstance(d[k], list):
+                d[k] = None
+        cases.append(d)
+    return {"format": "parity_v2 decisions", "source": "tests/data/parity_v2.json of the reference",
+            "count": len(cases), "cases": cases}
+
+
 def main() -> None:
     rt = rungs_table()
     assert len(rt) == 66, len(rt)
@@ -59,7 +81,11 @@ def main() -> None:
     dst.mkdir(exist_ok=True)
     for f in sorted((REF / "tests/data/golden").iterdir()):
         shutil.copy(f, dst / f.name)
-    print("wrote", len(rt), "ladder rows,", len(qt["rows"]), "quantile rows and the golden corpus")
+    pd = parity_decisions()
+    assert pd["count"] == 287
+    (HERE / "parity_decisions.json").write_text(json.dumps(pd) + "\n")
+    print("wrote", len(rt), "ladder rows,", len(qt["rows"]), "quantile rows, the golden corpus and",
+          pd["count"], "parity cases")
 
 
 if __name__ == "__main__":
