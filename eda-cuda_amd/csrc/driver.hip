@@ -445,6 +445,94 @@ __global__ void __launch_bounds__(kThreads) k_sift_step(int64_t count, const int
   }
 }
 
+// ---- device input: the strided gather and the cast to the compute dtype (reference
+// cpp/src/field/convert.hpp:33-49, convert.cu; describe_device_input decompose.cpp:25-82) ----------
+size_t source_size(int32_t source, int32_t dtype) {
+  switch (source) {
+    case EDA_SRC_SAME: return dtype == EDA_F32 ? 4 : 8;
+    case EDA_SRC_BOOL: case EDA_SRC_I8: case EDA_SRC_U8: return 1;
+    case EDA_SRC_I16: case EDA_SRC_U16: case EDA_SRC_F16: case EDA_SRC_BF16: return 2;
+    case EDA_SRC_I32: case EDA_SRC_U32: case EDA_SRC_F32: return 4;
+    case EDA_SRC_I64: case EDA_SRC_U64: case EDA_SRC_F64: return 8;
+    default: throw std::invalid_argument("eda: unknown source element type " + std::to_string(source));
+  }
+}
+
+// The batch offset of every element b0 .. b0 + nb - 1 of device input, in source elements.
+std::vector<int64_t> batch_offsets(const eda_array_desc& x, int64_t b0, int64_t nb) {
+  std::vector<int64_t> off(static_cast<size_t>(nb));
+  if (x.batch_axes <= 1) {
+    for (int64_t b = 0; b < nb; ++b) off[b] = (b0 + b) * x.batch_stride;
+    return off;
+  }
+  for (int64_t b = 0; b < nb; ++b) {
+    int64_t rest = b0 + b, o = 0;
+    for (int k = x.batch_axes - 1; k >= 0; --k) {
+      const int64_t e = x.batch_shape[k];
+      o += (rest % e) * x.batch_strides[k];
+      rest /= e;
+    }
+    off[b] = o;
+  }
+  return off;
+}
+
+struct Bf16 {
+  uint16_t bits;
+};
+struct Bool8 {
+  uint8_t v;
+};
+template <class T, class S>
+__device__ __forceinline__ T cast_in(S v) {
+  if constexpr (std::is_same_v<S, Bool8>) return v.v != 0 ? T(1) : T(0);
+  else if constexpr (std::is_same_v<S, Bf16>) return static_cast<T>(__uint_as_float(static_cast<uint32_t>(v.bits) << 16));
+  else return static_cast<T>(v);
+}
+
+template <class T, class S>
+void gather_cast(hipStream_t s, const Geom& g, const S* xp, const int64_t* boff, const int64_t* st, T* dst) {
+  const int64_t s1 = st[0], s2 = st[1], s3 = st[2];
+  each(s, g, nullptr, [=] __device__(int64_t b, int64_t i) {
+    int64_t off = boff[b] + ((i / g.st[0]) % g.n[0]) * s1;
+    if (g.dim >= 2) off += ((i / g.st[1]) % g.n[1]) * s2;
+    if (g.dim >= 3) off += ((i / g.st[2]) % g.n[2]) * s3;
+    dst[b * g.count + i] = cast_in<T, S>(xp[off]);
+  });
+}
+
+// dst (C-contiguous (g.batch, count)) <- elements b0 .. b0 + g.batch - 1 of x: host input copied,
+// device input gathered through its strides and cast.
+template <class T>
+void fill_input(Context& ctx, const Geom& g, const eda_array_desc& x, int64_t b0, T* dst) {
+  hipStream_t s = ctx.stream;
+  if (x.device < 0) {
+    EDA_HIP(hipMemcpyAsync(dst, static_cast<const T*>(x.data) + b0 * g.count,
+                           static_cast<size_t>(g.batch * g.count) * sizeof(T), hipMemcpyHostToDevice, s));
+    return;
+  }
+  const int64_t* boff = ctx.up.put(batch_offsets(x, b0, g.batch));
+  const int64_t* st = x.strides;
+  const void* p = x.data;
+  const int32_t src = x.source == EDA_SRC_SAME ? (x.dtype == EDA_F32 ? EDA_SRC_F32 : EDA_SRC_F64) : x.source;
+  switch (src) {
+    case EDA_SRC_BOOL: gather_cast<T>(s, g, static_cast<const Bool8*>(p), boff, st, dst); break;
+    case EDA_SRC_I8: gather_cast<T>(s, g, static_cast<const int8_t*>(p), boff, st, dst); break;
+    case EDA_SRC_I16: gather_cast<T>(s, g, static_cast<const int16_t*>(p), boff, st, dst); break;
+    case EDA_SRC_I32: gather_cast<T>(s, g, static_cast<const int32_t*>(p), boff, st, dst); break;
+    case EDA_SRC_I64: gather_cast<T>(s, g, static_cast<const int64_t*>(p), boff, st, dst); break;
+    case EDA_SRC_U8: gather_cast<T>(s, g, static_cast<const uint8_t*>(p), boff, st, dst); break;
+    case EDA_SRC_U16: gather_cast<T>(s, g, static_cast<const uint16_t*>(p), boff, st, dst); break;
+    case EDA_SRC_U32: gather_cast<T>(s, g, static_cast<const uint32_t*>(p), boff, st, dst); break;
+    case EDA_SRC_U64: gather_cast<T>(s, g, static_cast<const uint64_t*>(p), boff, st, dst); break;
+    case EDA_SRC_F16: gather_cast<T>(s, g, static_cast<const _Float16*>(p), boff, st, dst); break;
+    case EDA_SRC_BF16: gather_cast<T>(s, g, static_cast<const Bf16*>(p), boff, st, dst); break;
+    case EDA_SRC_F32: gather_cast<T>(s, g, static_cast<const float*>(p), boff, st, dst); break;
+    case EDA_SRC_F64: gather_cast<T>(s, g, static_cast<const double*>(p), boff, st, dst); break;
+    default: throw std::invalid_argument("eda: unknown source element type");
+  }
+}
+
 template <class T>
 class Chunk {
  public:
@@ -509,25 +597,13 @@ class Chunk {
   }
 
   // Copies the input (device pointer with element strides, or contiguous host memory) into r.
-  void start(const void* x, bool on_device, const int64_t* xs) {
+  // Copies elements b0 .. b0 + B - 1 of the call's input into r (host input copied, device input
+  // gathered through its strides and cast).
+  void start(const eda_array_desc& x, int64_t b0) {
     ctx_.up.reset();
-    const int64_t F = B_ * g_.count;
     const Geom g = g_;
     T* r = r_;
-    if (!on_device) {
-      EDA_HIP(hipMemcpyAsync(r, x, static_cast<size_t>(F) * sizeof(T), hipMemcpyHostToDevice, s_));
-    } else {
-      const T* xp = static_cast<const T*>(x);
-      const int64_t s0 = xs[0], s1 = xs[1], s2 = xs[2], s3 = xs[3];
-      each(s_, g, nullptr, [=] __device__(int64_t b, int64_t i) {
-        int64_t off = b * s0;
-        const int64_t c0 = (i / g.st[0]) % g.n[0];
-        off += c0 * s1;
-        if (g.dim >= 2) off += ((i / g.st[1]) % g.n[1]) * s2;
-        if (g.dim >= 3) off += ((i / g.st[2]) % g.n[2]) * s3;
-        r[b * g.count + i] = xp[off];
-      });
-    }
+    fill_input<T>(ctx_, g, x, b0, r);
     // non-finite samples and max |x|; the exact power-of-two prescale (reference mode.cu:43-58,271-282)
     enqueue_elem_stats<T>(s_, g_, r_, nullptr, nullptr, d_amax_, d_nonfin_cnt_, stats_ws_);
     std::vector<uint64_t> amax(B_), nf(B_);
@@ -1612,12 +1688,7 @@ void run_call(const eda_array_desc& x, const eda_params& p, Run& run) {
                                 : nullptr;
     Chunk<T> ch(ctx, g, c, tau, p.check_finite != 0, p.max_imf, rows, run.cap, orient);
     ch.alloc_history();
-    if (x.device < 0) {
-      ch.start(static_cast<const T*>(x.data) + b0 * g.count, false, nullptr);
-    } else {
-      const int64_t xs[4] = {x.batch_stride, x.strides[0], x.strides[1], x.strides[2]};
-      ch.start(static_cast<const T*>(x.data) + b0 * x.batch_stride, true, xs);
-    }
+    ch.start(x, b0);
     while (ch.mode()) {
     }
     ch.finish(run, b0);
@@ -1697,8 +1768,24 @@ void validate(const eda_array_desc& x, const eda_params& p) {
   if (p.tie_tolerance >= 0 && !std::isfinite(p.tie_tolerance))
     throw std::invalid_argument("eda: tie_tolerance must be a finite value >= 0");
   if (x.batch > 0 && x.data == nullptr) throw std::invalid_argument("eda: the input has no data pointer");
-  if (x.device >= 0 && reinterpret_cast<uintptr_t>(x.data) % (x.dtype == EDA_F32 ? 4 : 8) != 0)
-    throw std::invalid_argument("eda: the device pointer is not aligned to its item size");
+  if (x.device < 0 && x.source != EDA_SRC_SAME)
+    throw std::invalid_argument("eda: host input must already be in the compute dtype");
+  const size_t item = source_size(x.source, x.dtype);
+  if (x.device >= 0 && x.batch > 0 && reinterpret_cast<uintptr_t>(x.data) % item != 0) {
+    throw std::invalid_argument("eda: the device array's data pointer is not aligned to its item size (" +
+                                std::to_string(item) + " bytes)");
+  }
+  if (x.device >= 0 && x.batch_axes > 1) {
+    if (x.batch_axes > 64) throw std::invalid_argument("eda: a device array may have at most 64 batch axes");
+    if (x.batch_shape == nullptr || x.batch_strides == nullptr)
+      throw std::invalid_argument("eda: batch_axes > 1 needs batch_shape and batch_strides");
+    int64_t prod = 1;
+    for (int k = 0; k < x.batch_axes; ++k) {
+      if (x.batch_shape[k] < 0) throw std::invalid_argument("eda: negative extent");
+      prod *= x.batch_shape[k];
+    }
+    if (prod != x.batch) throw std::invalid_argument("eda: the batch shape does not hold `batch` elements");
+  }
 }
 
 template <class T>
@@ -1832,19 +1919,7 @@ int64_t run_monogenic(const eda_array_desc& x, bool check_finite, int64_t max_ch
     uint64_t* d_amax = cv.take<uint64_t>(nb);
     uint64_t* d_nf = cv.take<uint64_t>(nb);
     ctx.up.reset();
-    if (x.device < 0) {
-      EDA_HIP(hipMemcpyAsync(f, static_cast<const T*>(x.data) + b0 * count, static_cast<size_t>(F) * sizeof(T),
-                             hipMemcpyHostToDevice, s));
-    } else {
-      const T* xp = static_cast<const T*>(x.data) + b0 * x.batch_stride;
-      const int64_t s0 = x.batch_stride, s1 = x.strides[0], s2 = x.strides[1], s3 = x.strides[2];
-      each(s, g, nullptr, [=] __device__(int64_t b, int64_t i) {
-        int64_t off = b * s0 + ((i / g.st[0]) % g.n[0]) * s1;
-        if (g.dim >= 2) off += ((i / g.st[1]) % g.n[1]) * s2;
-        if (g.dim >= 3) off += ((i / g.st[2]) % g.n[2]) * s3;
-        f[b * g.count + i] = xp[off];
-      });
-    }
+    fill_input<T>(ctx, g, x, b0, f);
     enqueue_elem_stats<T>(s, g, f, nullptr, nullptr, d_amax, d_nf, sws);
     EDA_HIP(hipMemcpyAsync(amax.data(), d_amax, sizeof(uint64_t) * nb, hipMemcpyDeviceToHost, s));
     EDA_HIP(hipMemcpyAsync(nf.data(), d_nf, sizeof(uint64_t) * nb, hipMemcpyDeviceToHost, s));

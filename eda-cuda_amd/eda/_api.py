@@ -303,6 +303,7 @@ class _Call:
     batch_shape: tuple[int, ...]
     B: int
     stream_handle: int
+    device: Any = None  # the torch.device of device input
 
 
 def _prepare(x: Any, axes: Any, dtype: Any, stream: Any) -> _Call:
@@ -348,22 +349,28 @@ def _prepare(x: Any, axes: Any, dtype: Any, stream: Any) -> _Call:
     stream_handle = 0
     if on_device:
         torch = _torch()
-        tdt = torch.float32 if code == 0 else torch.float64
-        t = data if data.dtype == tdt else data.to(tdt)
-        tv = t.permute(order) if order != list(range(ndim)) else t
-        # read in place when the batch axes merge into one stride (reference
-        # cpp/src/decompose.cpp:44-63); else a contiguous copy on the device
-        try:
-            flat = tv.view((B,) + spatial) if B > 0 else tv.reshape((0,) + spatial)
-        except RuntimeError:
-            flat = tv.contiguous().view((B,) + spatial)
-        keep_alive = flat
-        device = flat.device.index if flat.device.index is not None else torch.cuda.current_device()
-        desc.data = flat.data_ptr() if flat.numel() else None
+        if src not in _native.SOURCE_CODES:
+            raise TypeError(f"eda: arrays of dtype {src} are not supported (real bool, integer and float arrays only)")
+        # read in place through the strides of every axis, cast to the compute dtype on the fly
+        # (reference cpp/src/decompose.cpp:25-82, cpp/src/field/convert.cu)
+        tv = data.permute(order) if order != list(range(ndim)) else data
+        nb = len(batch_axes)
+        bshape, bstrides = tuple(tv.shape[:nb]), tuple(tv.stride()[:nb])
+        device = tv.device.index if tv.device.index is not None else torch.cuda.current_device()
+        desc.data = tv.data_ptr() if tv.numel() else None
         desc.device = device
-        desc.batch_stride = flat.stride(0) if B > 0 else 0
+        desc.source = 0 if src == target else _native.SOURCE_CODES[src]
         for a in range(len(axes)):
-            desc.strides[a] = flat.stride(1 + a)
+            desc.strides[a] = tv.stride(nb + a)
+        if nb == 1:
+            desc.batch_stride = bstrides[0]
+        elif nb > 1:
+            arrays = ((ctypes.c_int64 * nb)(*bshape), (ctypes.c_int64 * nb)(*bstrides))
+            desc.batch_axes = nb
+            desc.batch_shape = ctypes.cast(arrays[0], ctypes.POINTER(ctypes.c_int64))
+            desc.batch_strides = ctypes.cast(arrays[1], ctypes.POINTER(ctypes.c_int64))
+            tv = (tv, arrays)
+        keep_alive = tv
         if stream is None:
             stream_handle = int(torch.cuda.current_stream(device).cuda_stream)
         elif isinstance(stream, int):
@@ -379,7 +386,7 @@ def _prepare(x: Any, axes: Any, dtype: Any, stream: Any) -> _Call:
         desc.device = -1
 
     return _Call(desc, keep_alive, on_device, target, code, ndim, axes, batch_axes, spatial, batch_shape, B,
-                 stream_handle)
+                 stream_handle, data.device if on_device else None)
 
 
 def decompose(
@@ -432,6 +439,7 @@ def decompose(
     ndim, axes, batch_axes, spatial, batch_shape, B = (call.ndim, call.axes, call.batch_axes, call.spatial,
                                                        call.batch_shape, call.B)
     stream_handle = call.stream_handle
+    tdevice = call.device
     out_final = _restore_order(ndim, batch_axes, axes)
     del call
     res = _call_native(desc, max_imf, tie, sifts, check_finite, stream_handle, return_orientation)
@@ -455,7 +463,7 @@ def decompose(
         if on_device:
             torch = _torch()
             out = torch.empty((B, P) + spatial, dtype=torch.float32 if code == 0 else torch.float64,
-                              device=keep_alive.device)
+                              device=tdevice)
             if out.numel():
                 _native.check(lib.eda_collect(ctypes.byref(res), ctypes.c_void_p(out.data_ptr()), int(desc.device),
                                               err, len(err)), err)
@@ -470,7 +478,7 @@ def decompose(
         orient = None
         if return_orientation:
             orient = _orientation(res, batch_shape, spatial, ndim, batch_axes, axes, target,
-                                  keep_alive.device if on_device else None)
+                                  tdevice if on_device else None)
     finally:
         _native.core().eda_release(ctypes.byref(res))
     del keep_alive
@@ -550,7 +558,7 @@ def monogenic(
     if call.on_device:
         torch = _torch()
         tdt = torch.float32 if call.code == 0 else torch.float64
-        out = torch.empty(planes * n, dtype=tdt, device=call.keep_alive.device)
+        out = torch.empty(planes * n, dtype=tdt, device=call.device)
         ptr, out_device = (out.data_ptr() if n else None), int(call.desc.device)
     else:
         out = np.empty(planes * n, dtype=np.dtype(call.target))

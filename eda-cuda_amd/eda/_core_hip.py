@@ -24,8 +24,9 @@ ptr) of one buffer holding the direction followed by the coherence ((None, 0) wh
 Errors raise the reference's classes: ValueError, MemoryError, ``eda.CudaError``, RuntimeError.
 The monogenic calls' info holds ``nonfinite`` [B] int32 and ``chunk``; their output holds the planes
 amplitude, phase, frequency (B, *spatial) each, then for 2 or 3 spatial axes the direction (B, D,
-*spatial) and the coherence (B, *spatial). Not served: device input whose dtype differs from
-``target`` or whose batch axes do not merge into one stride (ValueError), and the DLPack helpers.
+*spatial) and the coherence (B, *spatial). Device input of any of the 13 element types
+(bool, int8..int64, uint8..uint64, float16, bfloat16, float32, float64) and any batch strides is
+read in place and cast to ``target`` on the device. Not served: the DLPack helpers.
 The device path allocates its output through PyTorch-ROCm, whose tensor is the ``owner``."""
 
 from __future__ import annotations
@@ -157,18 +158,72 @@ def decompose_host(x, max_imf: int, tie_tolerance: float | None, sifts: int, che
     return rows, info
 
 
-def _merged_batch_stride(batch_shape: Sequence[int], batch_strides: Sequence[int]) -> int:
-    """The one stride of the batch axes, when they merge (reference cpp/src/decompose.cpp:44-63)."""
-    axes = [(int(n), int(s)) for n, s in zip(batch_shape, batch_strides) if int(n) != 1]
-    if not axes:
+def _merged_batch_axes(batch_shape: Sequence[int], batch_strides: Sequence[int]) -> tuple[list[int], list[int]]:
+    """The batch axes without the extent-1 ones, neighbours whose strides chain merged into one
+    (reference cpp/src/decompose.cpp:48-59)."""
+    shape: list[int] = []
+    stride: list[int] = []
+    for n, st in zip(batch_shape, batch_strides):
+        n, st = int(n), int(st)
+        if n == 1:
+            continue
+        if shape and stride[-1] == st * n:
+            shape[-1] *= n
+            stride[-1] = st
+        else:
+            shape.append(n)
+            stride.append(st)
+    return shape, stride
+
+
+def _device_desc(ptr: int, device: int, dtype: str, target: str, batch_shape: Sequence[int],
+                 batch_strides: Sequence[int], shape: Sequence[int], strides: Sequence[int]):
+    """The descriptor of a device array read in place (element strides of its own dtype, any
+    sign), cast to ``target`` on the fly; returns it and the ctypes arrays it points to."""
+    if target not in _CODES:
+        raise ValueError(f"eda: target must be float32 or float64, got {target}")
+    if dtype not in _native.SOURCE_CODES:
+        raise TypeError(f"eda: arrays of dtype {dtype} are not supported (real bool, integer and float arrays only)")
+    if not 1 <= len(shape) <= 3 or len(strides) != len(shape):
+        raise ValueError("eda: expected 1 to 3 spatial axes with one stride each")
+    if len(batch_shape) != len(batch_strides):
+        raise ValueError("eda: batch shape and strides differ in length")
+    if len(batch_shape) > 64:
+        raise ValueError("eda: a device array may have at most 64 batch axes")
+    B = math.prod(int(n) for n in batch_shape)
+    count = math.prod(int(n) for n in shape)
+    desc = _native.ArrayDesc()
+    desc.dtype = _CODES[target]
+    desc.source = 0 if dtype == target else _native.SOURCE_CODES[dtype]
+    desc.dim = len(shape)
+    desc.batch = B
+    for a, (n, st) in enumerate(zip(shape, strides)):
+        desc.n[a] = int(n)
+        desc.strides[a] = int(st)
+    mshape, mstride = _merged_batch_axes(batch_shape, batch_strides)
+    keep = None
+    if len(mshape) == 1:
+        desc.batch_stride = mstride[0]
+    elif len(mshape) > 1:
+        keep = ((ctypes.c_int64 * len(mshape))(*mshape), (ctypes.c_int64 * len(mshape))(*mstride))
+        desc.batch_axes = len(mshape)
+        desc.batch_shape = ctypes.cast(keep[0], ctypes.POINTER(ctypes.c_int64))
+        desc.batch_strides = ctypes.cast(keep[1], ctypes.POINTER(ctypes.c_int64))
+    desc.device = int(device)
+    desc.data = int(ptr) if B and count else None
+    return desc, keep
+
+
+def _fence_stream(fences: Sequence[int], device: int, torch) -> int:
+    """The one stream handle the call waits for by event (0: none); the legacy and per-thread
+    default streams (1, 2) and several fences make it wait for the whole device instead."""
+    fences = [int(f) for f in fences]
+    if not fences:
         return 0
-    stride = axes[-1][1]
-    span = axes[-1][0]
-    for n, s in reversed(axes[:-1]):
-        if s != stride * span:
-            raise ValueError("eda: the batch axes of device input must merge into one stride in this build")
-        span *= n
-    return stride
+    if len(fences) > 1 or fences[0] in (1, 2):
+        torch.cuda.synchronize(int(device))
+        return 0
+    return fences[0]
 
 
 def decompose_device(ptr: int, device: int, dtype: str, target: str, batch_shape: Sequence[int],
@@ -180,33 +235,12 @@ def decompose_device(ptr: int, device: int, dtype: str, target: str, batch_shape
     complete on return (reference cpp/python/decompose.cpp:452-509)."""
     import torch  # noqa: PLC0415 - the device path allocates its output through PyTorch-ROCm
 
-    if target not in _CODES:
-        raise ValueError(f"eda: target must be float32 or float64, got {target}")
-    if dtype != target:
-        raise ValueError(f"eda: device input of dtype {dtype} converted to {target} is not supported by this build")
-    if not 1 <= len(shape) <= 3 or len(strides) != len(shape):
-        raise ValueError("eda: expected 1 to 3 spatial axes with one stride each")
-    B = math.prod(int(n) for n in batch_shape)
-    fences = [int(f) for f in fences]
-    stream = 0
-    if fences:
-        stream = fences[0]
-        if len(fences) > 1 or stream in (1, 2):
-            # legacy / per-thread default streams and extra fences: wait for the whole device
-            torch.cuda.synchronize(int(device))
-            stream = 0
+    desc, keep = _device_desc(ptr, device, dtype, target, batch_shape, batch_strides, shape, strides)
+    B = int(desc.batch)
+    stream = _fence_stream(fences, device, torch)
     p = _params(max_imf, tie_tolerance, sifts, check_finite, orientation, max_chunk, stream)
-    desc = _native.ArrayDesc()
-    desc.dtype = _CODES[target]
-    desc.dim = len(shape)
-    desc.batch = B
-    for a, (n, s) in enumerate(zip(shape, strides)):
-        desc.n[a] = int(n)
-        desc.strides[a] = int(s)
-    desc.batch_stride = _merged_batch_stride(batch_shape, batch_strides)
-    desc.device = int(device)
-    desc.data = int(ptr) if B and math.prod(int(n) for n in shape) else None
     res = _run(desc, p)
+    del keep
     try:
         info = _info(res)
         tdt = torch.float32 if target == "float32" else torch.float64
@@ -273,31 +307,10 @@ def monogenic_device(ptr: int, device: int, dtype: str, target: str, batch_shape
     out_ptr, info), the planes of monogenic_host in one device buffer, complete on return."""
     import torch  # noqa: PLC0415 - the device path allocates its output through PyTorch-ROCm
 
-    if target not in _CODES:
-        raise ValueError(f"eda: target must be float32 or float64, got {target}")
-    if dtype != target:
-        raise ValueError(f"eda: device input of dtype {dtype} converted to {target} is not supported by this build")
-    if not 1 <= len(shape) <= 3 or len(strides) != len(shape):
-        raise ValueError("eda: expected 1 to 3 spatial axes with one stride each")
-    B = math.prod(int(n) for n in batch_shape)
+    desc, keep = _device_desc(ptr, device, dtype, target, batch_shape, batch_strides, shape, strides)
+    B = int(desc.batch)
     count = math.prod(int(n) for n in shape)
-    fences = [int(f) for f in fences]
-    stream = 0
-    if fences:
-        stream = fences[0]
-        if len(fences) > 1 or stream in (1, 2):
-            torch.cuda.synchronize(int(device))
-            stream = 0
-    desc = _native.ArrayDesc()
-    desc.dtype = _CODES[target]
-    desc.dim = len(shape)
-    desc.batch = B
-    for a, (n, st) in enumerate(zip(shape, strides)):
-        desc.n[a] = int(n)
-        desc.strides[a] = int(st)
-    desc.batch_stride = _merged_batch_stride(batch_shape, batch_strides)
-    desc.device = int(device)
-    desc.data = int(ptr) if B and count else None
+    stream = _fence_stream(fences, device, torch)
     tdt = torch.float32 if target == "float32" else torch.float64
     owner = torch.empty(_monogenic_planes(len(shape)) * B * count, dtype=tdt, device=torch.device("cuda", int(device)))
     nonfinite = np.zeros(max(B, 1), np.int32)
@@ -307,4 +320,5 @@ def monogenic_device(ptr: int, device: int, dtype: str, target: str, batch_shape
         ctypes.byref(desc), 1 if check_finite else 0, int(max_chunk), stream,
         ctypes.c_void_p(owner.data_ptr() if owner.numel() else None), int(device),
         nonfinite.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), ctypes.byref(chunk), err, len(err)), err)
+    del keep
     return owner, owner.data_ptr(), {"nonfinite": nonfinite[:B].copy(), "chunk": int(chunk.value)}

@@ -22,12 +22,27 @@ class ArrayDesc(ctypes.Structure):
         ("device", ctypes.c_int32),
         ("dtype", ctypes.c_int32),
         ("dim", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("source", ctypes.c_int32),
         ("batch", ctypes.c_int64),
         ("n", ctypes.c_int64 * 3),
         ("batch_stride", ctypes.c_int64),
         ("strides", ctypes.c_int64 * 3),
+        ("batch_axes", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("batch_shape", ctypes.POINTER(ctypes.c_int64)),
+        ("batch_strides", ctypes.POINTER(ctypes.c_int64)),
     ]
+
+
+# Element types of device input (EDA_SRC_* of include/eda_hip.h), by NumPy / PyTorch dtype name.
+SOURCE_CODES = {
+    "bool": 1, "int8": 2, "int16": 3, "int32": 4, "int64": 5, "uint8": 6, "uint16": 7, "uint32": 8,
+    "uint64": 9, "float16": 10, "bfloat16": 11, "float32": 12, "float64": 13,
+}
+SOURCE_SIZES = {
+    "bool": 1, "int8": 1, "int16": 2, "int32": 4, "int64": 8, "uint8": 1, "uint16": 2, "uint32": 4,
+    "uint64": 8, "float16": 2, "bfloat16": 2, "float32": 4, "float64": 8,
+}
 
 
 class Params(ctypes.Structure):

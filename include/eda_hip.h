@@ -32,19 +32,44 @@ extern "C" {
 #define EDA_F32 0
 #define EDA_F64 1
 
+/* Element types of device input (eda_array_desc.source; reference cpp/src/field/convert.hpp:17-22).
+ * 0: the compute dtype itself. Each element is cast to the compute dtype with round to nearest even
+ * (bool to 0 or 1; float16, bfloat16 and float32 to float64 exactly). */
+#define EDA_SRC_SAME 0
+#define EDA_SRC_BOOL 1
+#define EDA_SRC_I8 2
+#define EDA_SRC_I16 3
+#define EDA_SRC_I32 4
+#define EDA_SRC_I64 5
+#define EDA_SRC_U8 6
+#define EDA_SRC_U16 7
+#define EDA_SRC_U32 8
+#define EDA_SRC_U64 9
+#define EDA_SRC_F16 10
+#define EDA_SRC_BF16 11
+#define EDA_SRC_F32 12
+#define EDA_SRC_F64 13
+
 /* The input of a call: `batch` elements of `dim` (1..3) spatial axes. Host input (device = -1) is
- * C-contiguous (batch, n[0], .., n[dim-1]); device input is read in place with the given element
- * strides (any sign), the batch axes merged into one stride. */
+ * C-contiguous (batch, n[0], .., n[dim-1]) in the compute dtype. Device input is read in place,
+ * element (b, i) at data[offset(b) + sum_a i_a strides[a]] (element strides of the source type, any
+ * sign), with offset(b) = b batch_stride, or, when batch_axes > 1, sum_k j_k batch_strides[k] for
+ * j the C-order multi-index of b over batch_shape[0 .. batch_axes) (reference
+ * cpp/src/decompose.cpp:25-82, the strided gather of cpp/src/field/convert.hpp:33-49). */
 typedef struct eda_array_desc {
   const void* data;
   int32_t device; /* -1: host memory; else the HIP device that holds data */
   int32_t dtype;  /* EDA_F32 or EDA_F64: the compute dtype */
   int32_t dim;
-  int32_t reserved;
+  int32_t source; /* device input: its element type, EDA_SRC_* (EDA_SRC_SAME: the compute dtype) */
   int64_t batch;
   int64_t n[3];
   int64_t batch_stride; /* in elements (device input) */
   int64_t strides[3];   /* in elements (device input) */
+  int32_t batch_axes;   /* device input: > 1 to use batch_shape / batch_strides (at most 64 axes) */
+  int32_t reserved;
+  const int64_t* batch_shape;   /* host arrays of batch_axes entries (batch = their product) */
+  const int64_t* batch_strides; /* in elements of the source type */
 } eda_array_desc;
 
 /* The parameters of a call (reference cpp/src/decompose.hpp:69-76, src/eda/_api.py:184-353). */

@@ -132,23 +132,35 @@ def test_core_hip_argument_rules_before_any_gpu_work():
 
 
 @pytest.mark.parametrize("shape,strides,want", [
-    ((4,), (100,), 100),
-    ((2, 3), (300, 100), 100),
-    ((2, 1, 3), (300, 7, 100), 100),
-    ((), (), 0),
-    ((3, 2), (-200, -100), -100),
+    ((4,), (100,), ([4], [100])),
+    ((2, 3), (300, 100), ([6], [100])),
+    ((2, 1, 3), (300, 7, 100), ([6], [100])),
+    ((), (), ([], [])),
+    ((3, 2), (-200, -100), ([6], [-100])),
+    ((2, 3), (100, 300), ([2, 3], [100, 300])),  # does not merge: two axes
+    ((1, 1), (5, 9), ([], [])),
 ])
-def test_core_hip_merged_batch_stride(shape, strides, want):
-    from eda._core_hip import _merged_batch_stride
+def test_core_hip_merged_batch_axes(shape, strides, want):
+    """The reference's merge of the batch axes (cpp/src/decompose.cpp:48-59): extent-1 axes dropped,
+    neighbours whose strides chain merged."""
+    from eda._core_hip import _merged_batch_axes
 
-    assert _merged_batch_stride(shape, strides) == want
+    assert _merged_batch_axes(shape, strides) == want
 
 
-def test_core_hip_batch_axes_that_do_not_merge():
-    from eda._core_hip import _merged_batch_stride
+def test_core_hip_device_descriptor_rules():
+    from eda import _native
+    from eda._core_hip import _device_desc
 
+    desc, keep = _device_desc(4096, 0, "int16", "float32", (2, 3), (100, 300), (5, 7), (7, 1))
+    assert desc.source == _native.SOURCE_CODES["int16"] and desc.batch_axes == 2 and desc.batch == 6
+    assert [desc.batch_shape[k] for k in range(2)] == [2, 3] and [desc.batch_strides[k] for k in range(2)] == [100, 300]
+    desc, keep = _device_desc(4096, 0, "float32", "float32", (4,), (35,), (5, 7), (7, 1))
+    assert desc.source == 0 and desc.batch_axes == 0 and desc.batch_stride == 35
+    with pytest.raises(TypeError):
+        _device_desc(4096, 0, "complex64", "float32", (4,), (35,), (5, 7), (7, 1))
     with pytest.raises(ValueError):
-        _merged_batch_stride((2, 3), (100, 300))
+        _device_desc(4096, 0, "float32", "float16", (4,), (35,), (5, 7), (7, 1))
 
 
 @pytest.mark.parametrize(("kwargs", "error", "fragment"), [
