@@ -29,8 +29,8 @@ MethodCtl method_ctl(const Geom& g, bool f32);
 // events on its stream, and its algorithmic bytes are counted: the pass's input field(s) read once
 // and output field(s) written once, over the elements whose window is non-zero. The count of those
 // elements comes from the host mirror of the window array (no device read-back).
-// kind 2 times the DCT contraction's GEMM launches instead, counting their flops (2 M N K per
-// batch entry, every entry) in place of bytes.
+// kind 2 times the DCT's axis passes instead (the fast transform of power-of-two axes and the dense
+// contraction of the others), each counting one read and one write of the field.
 void timer_set(int kind);
 bool timer_on();
 bool timer_gemm_on();

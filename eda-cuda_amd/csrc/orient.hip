@@ -237,9 +237,6 @@ template <class T>
 void gemm(hipStream_t s, const GemmDims& d, const T* A, const T* B, T* C, const int32_t* on) {
   const dim3 grid(static_cast<unsigned>((d.N + kTN - 1) / kTN), static_cast<unsigned>((d.M + kTM - 1) / kTM),
                   static_cast<unsigned>(d.batches));
-  const bool timed = timer_gemm_on();
-  hipEvent_t t0 = nullptr;
-  if (timed) timer_note(s, &t0);
   if constexpr (std::is_same_v<T, float>) {
     hipLaunchKernelGGL(k_gemm_mfma, grid, dim3(kThreads), 0, s, d, A, B, C, on);
     EDA_LAUNCH("k_gemm_mfma");
@@ -247,13 +244,12 @@ void gemm(hipStream_t s, const GemmDims& d, const T* A, const T* B, T* C, const 
     hipLaunchKernelGGL(k_gemm<T>, grid, dim3(kThreads), 0, s, d, A, B, C, on);
     EDA_LAUNCH("k_gemm");
   }
-  if (timed) timer_done(s, t0, 2.0 * static_cast<double>(d.M) * d.N * d.K * d.batches);
 }
 
 // One axis of a separable transform: out = M applied along axis a of every running element
 // (M row-major (out index, in index), n x n).
 template <class T>
-void axis_transform(hipStream_t s, const Geom& g, int a, const T* M, const T* in, T* out, const int32_t* on) {
+void axis_gemm(hipStream_t s, const Geom& g, int a, const T* M, const T* in, T* out, const int32_t* on) {
   const int64_t n = g.n[a];
   const int64_t inner = g.st[a];
   const int64_t outer = g.count / (n * inner);
@@ -281,6 +277,609 @@ void axis_transform(hipStream_t s, const Geom& g, int a, const T* M, const T* in
     d.per_elem = outer;
     gemm<T>(s, d, M, in, out, on);
   }
+}
+// The pass timer's DCT passes (kind 2) count one read and one write of the field per axis.
+template <class T>
+void axis_transform(hipStream_t s, const Geom& g, int a, const T* M, const T* in, T* out, const int32_t* on) {
+  const bool timed = timer_gemm_on();
+  hipEvent_t t0 = nullptr;
+  if (timed) timer_note(s, &t0);
+  axis_gemm<T>(s, g, a, M, in, out, on);
+  if (timed) timer_done(s, t0, 2.0 * static_cast<double>(g.batch) * g.count * sizeof(T));
+}
+
+// ---- the fast transform of power-of-two axes --------------------------------------------------------
+// An axis of n = 2^m samples (16 <= n <= kFftTile) is transformed in LDS through Makhoul's
+// reordering and an n-point complex FFT, two lines at a time (one in the real part, one in the
+// imaginary part): memory-bound, one read and one write of the field per axis.
+//   forward:  v(m) = f(2m), v(n-1-m) = f(2m+1); Z = FFT(v1 + i v2); V1 = (Z_k + conj Z_{n-k}) / 2,
+//             V2 = (Z_k - conj Z_{n-k}) / 2i; X_k = c_k Re(e^{-i pi k / 2n} V_k)
+//   inverse:  X'_k = s_k X_k (s_0 = 1 / sqrt n, else 1 / sqrt 2n); W_k = e^{i pi k / 2n}
+//             (X'_k - i X'_{n-k}) (X'_n = 0); z = sum_k (W1 + i W2)_k e^{2 pi i m k / n};
+//             f(2m) = v(m), f(2m+1) = v(n-1-m) with v1 = Re z, v2 = Im z
+//   sin basis on the axis: the inverse of the reversed spectrum (X_{n-k}, 0 at k = 0), times (-1)^x
+// The FFT is Stockham's self-sorting form, radix 8 and one last radix 2 or 4 stage, each butterfly
+// reading its R inputs n / R apart and writing its outputs Ns apart (Ns the product of the earlier
+// radices), in place in LDS with a barrier between the reads and the writes. The two lines of a pair
+// are two lines of one element (lines 2q and 2q + 1, a zero line past the last), so an element's
+// transform depends on its own samples only.
+constexpr int kFftTile = 4096;  // complex points per workgroup (float32; half for float64)
+// LDS slot of point m of line pair p: one pad word per 8 points inside a line, and an odd stride
+// sp = fpad(n) | 1 between pairs, so that the points one wave touches fall in distinct banks whether
+// its lanes walk one line or the same point of adjacent pairs; the imaginary parts start 16 banks
+// after the real parts.
+__host__ __device__ __forceinline__ int32_t fpad(int32_t i) { return i + (i >> 3); }
+__host__ __device__ __forceinline__ int32_t fft_sp(int32_t n) { return fpad(n) | 1; }
+__host__ __device__ __forceinline__ int32_t fft_im_off(int32_t L, int32_t sp) {
+  const int32_t base = L * sp;
+  return base + ((16 - base % 32) + 32) % 32;
+}
+
+template <class T>
+struct FftTabs {
+  int32_t n = 0;
+  const T* tc = nullptr;  // cos(2 pi t / n), t < n
+  const T* ts = nullptr;  // sin(2 pi t / n)
+  const T* pc = nullptr;  // c_k cos(pi k / 2n)
+  const T* ps = nullptr;  // c_k sin(pi k / 2n)
+  const T* qc = nullptr;  // cos(pi k / 2n)
+  const T* qs = nullptr;  // sin(pi k / 2n)
+};
+
+// One radix-R Stockham stage over the tile (L pairs of n points), sign -1 (forward) or +1.
+__device__ __forceinline__ void wave_sync_o() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// WAVE: each wave owns the pairs p = wave + 4 i of the tile (L a multiple of 4) and syncs alone;
+// else the workgroup's threads share every pair.
+template <class T, int R, int SIGN, bool WAVE>
+__device__ __forceinline__ void fft_stage(T* re, T* im, int32_t ln, int32_t lL, int32_t lNs, int32_t sp,
+                                          const FftTabs<T>& tb) {
+  constexpr int IPT = (sizeof(T) == 4 ? kFftTile : kFftTile / 2) / (R * 256);  // butterflies per thread
+  constexpr int LR = R == 8 ? 3 : (R == 4 ? 2 : 1);
+  const int32_t n = 1 << ln;
+  const int32_t lper = ln - LR;
+  const int32_t per = 1 << lper;
+  const int32_t total = WAVE ? per << (lL - 2) : per << lL;  // butterflies of the wave / workgroup
+  const int32_t me = WAVE ? static_cast<int32_t>(threadIdx.x & 63) : static_cast<int32_t>(threadIdx.x);
+  constexpr int32_t kStep = WAVE ? 64 : 256;
+  const int32_t wave = static_cast<int32_t>(threadIdx.x >> 6);
+  const int32_t Ns = 1 << lNs;
+  const int32_t ltstep = ln - lNs - LR;  // the twiddle index step: n / (Ns R)
+  const auto pair_of = [&](int32_t u) { return WAVE ? wave + 4 * (u >> lper) : u >> lper; };
+  T ar[IPT][R], ai[IPT][R];
+#pragma unroll
+  for (int it = 0; it < IPT; ++it) {
+    const int32_t u = me + it * kStep;
+    if (u >= total) continue;  // (continue, not break: the loop stays unrolled, the arrays in registers)
+    const int32_t p = pair_of(u);
+    const int32_t j = u & (per - 1);
+    const int32_t k = j & (Ns - 1);
+    const int32_t base = p * sp;
+    // the twiddles w^r, w = e^{SIGN 2 pi i k / (Ns R)}: w, w^2 (and w^4) from the table, the other
+    // powers as their products
+    T wr[R], wi[R];
+    wr[0] = T(1);
+    wi[0] = T(0);
+    if (lNs > 0) {
+      const auto tab = [&](int q, T& c, T& sn) {
+        const int32_t t = ((k * q) << ltstep) & (n - 1);
+        c = tb.tc[t];
+        sn = SIGN < 0 ? -tb.ts[t] : tb.ts[t];
+      };
+      tab(1, wr[1], wi[1]);
+      if constexpr (R >= 4) {
+        tab(2, wr[2], wi[2]);
+        wr[3] = wr[1] * wr[2] - wi[1] * wi[2];
+        wi[3] = wr[1] * wi[2] + wi[1] * wr[2];
+      }
+      if constexpr (R == 8) {
+        tab(4, wr[4], wi[4]);
+#pragma unroll
+        for (int q = 5; q < 8; ++q) {
+          wr[q] = wr[q - 4] * wr[4] - wi[q - 4] * wi[4];
+          wi[q] = wr[q - 4] * wi[4] + wi[q - 4] * wr[4];
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int32_t at = base + fpad(j + r * per);
+      T xr = re[at], xi = im[at];
+      if (r > 0 && lNs > 0) {
+        const T yr = xr * wr[r] - xi * wi[r];
+        xi = xr * wi[r] + xi * wr[r];
+        xr = yr;
+      }
+      ar[it][r] = xr;
+      ai[it][r] = xi;
+    }
+  }
+  if (WAVE) wave_sync_o();
+  else __syncthreads();
+#pragma unroll
+  for (int it = 0; it < IPT; ++it) {
+    const int32_t u = me + it * kStep;
+    if (u >= total) continue;
+    const int32_t p = pair_of(u);
+    const int32_t j = u & (per - 1);
+    const int32_t k = j & (Ns - 1);
+    T* xr = ar[it];
+    T* xi = ai[it];
+    // the R-point DFT with the factor e^{SIGN 2 pi i / R}; (a, b) * (SIGN i) = (-SIGN b, SIGN a)
+    if constexpr (R == 2) {
+      const T r0 = xr[0] + xr[1], i0 = xi[0] + xi[1];
+      xr[1] = xr[0] - xr[1];
+      xi[1] = xi[0] - xi[1];
+      xr[0] = r0;
+      xi[0] = i0;
+    } else {
+      // radix 4 on (0, 2, 4, 6) / 2 ... written for the general even R = 4 or 8
+      constexpr int H = R / 2;
+      T er[H], ei[H], orr[H], oi[H];
+      if constexpr (R == 4) {
+        // evens (0, 2), odds (1, 3): 2-point DFTs, then the combine with w4 = SIGN i
+        er[0] = xr[0] + xr[2]; ei[0] = xi[0] + xi[2];
+        er[1] = xr[0] - xr[2]; ei[1] = xi[0] - xi[2];
+        orr[0] = xr[1] + xr[3]; oi[0] = xi[1] + xi[3];
+        const T dr = xr[1] - xr[3], di = xi[1] - xi[3];
+        orr[1] = -SIGN * di; oi[1] = SIGN * dr;
+      } else {
+        // evens (0, 2, 4, 6) and odds (1, 3, 5, 7) by 4-point DFTs
+        const T b0r = xr[0] + xr[4], b0i = xi[0] + xi[4];
+        const T b1r = xr[0] - xr[4], b1i = xi[0] - xi[4];
+        const T b2r = xr[2] + xr[6], b2i = xi[2] + xi[6];
+        const T d2r = xr[2] - xr[6], d2i = xi[2] - xi[6];
+        const T b3r = -SIGN * d2i, b3i = SIGN * d2r;
+        er[0] = b0r + b2r; ei[0] = b0i + b2i;
+        er[2] = b0r - b2r; ei[2] = b0i - b2i;
+        er[1] = b1r + b3r; ei[1] = b1i + b3i;
+        er[3] = b1r - b3r; ei[3] = b1i - b3i;
+        const T b4r = xr[1] + xr[5], b4i = xi[1] + xi[5];
+        const T b5r = xr[1] - xr[5], b5i = xi[1] - xi[5];
+        const T b6r = xr[3] + xr[7], b6i = xi[3] + xi[7];
+        const T d6r = xr[3] - xr[7], d6i = xi[3] - xi[7];
+        const T b7r = -SIGN * d6i, b7i = SIGN * d6r;
+        orr[0] = b4r + b6r; oi[0] = b4i + b6i;
+        orr[2] = b4r - b6r; oi[2] = b4i - b6i;
+        orr[1] = b5r + b7r; oi[1] = b5i + b7i;
+        orr[3] = b5r - b7r; oi[3] = b5i - b7i;
+        // odds times w8^q = e^{SIGN 2 pi i q / 8}, q = 1, 2, 3
+        const T h = static_cast<T>(0.70710678118654752440);
+        {
+          const T a = orr[1], b = oi[1];  // (a + ib)(1 + SIGN i) h
+          orr[1] = (a - SIGN * b) * h;
+          oi[1] = (b + SIGN * a) * h;
+        }
+        {
+          const T a = orr[2], b = oi[2];  // (a + ib)(SIGN i)
+          orr[2] = -SIGN * b;
+          oi[2] = SIGN * a;
+        }
+        {
+          const T a = orr[3], b = oi[3];  // (a + ib)(-1 + SIGN i) h
+          orr[3] = (-a - SIGN * b) * h;
+          oi[3] = (-b + SIGN * a) * h;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < H; ++q) {
+        xr[q] = er[q] + orr[q];
+        xi[q] = ei[q] + oi[q];
+        xr[q + H] = er[q] - orr[q];
+        xi[q + H] = ei[q] - oi[q];
+      }
+    }
+    const int32_t dst = ((j >> lNs) << (lNs + LR)) + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int32_t at = p * sp + fpad(dst + r * Ns);
+      re[at] = xr[r];
+      im[at] = xi[r];
+    }
+  }
+  if (WAVE) wave_sync_o();
+  else __syncthreads();
+}
+
+// The stages from Ns = 2^LNS on, for a length 2^LN known at compile time (its index arithmetic folds).
+template <class T, int SIGN, bool WAVE, int LN, int LNS>
+__device__ __forceinline__ void fft_stages(T* re, T* im, int32_t lL, int32_t sp, const FftTabs<T>& tb) {
+  if constexpr (LN - LNS >= 3) {
+    fft_stage<T, 8, SIGN, WAVE>(re, im, LN, lL, LNS, sp, tb);
+    fft_stages<T, SIGN, WAVE, LN, LNS + 3>(re, im, lL, sp, tb);
+  } else if constexpr (LN - LNS == 2) {
+    fft_stage<T, 4, SIGN, WAVE>(re, im, LN, lL, LNS, sp, tb);
+  } else if constexpr (LN - LNS == 1) {
+    fft_stage<T, 2, SIGN, WAVE>(re, im, LN, lL, LNS, sp, tb);
+  }
+}
+
+// LN > 0: the length 2^LN at compile time; LN = 0: ln at run time.
+template <class T, int SIGN, bool WAVE, int LN>
+__device__ void fft_tile(T* re, T* im, int32_t ln, int32_t lL, int32_t sp, const FftTabs<T>& tb) {
+  if constexpr (LN > 0) {
+    fft_stages<T, SIGN, WAVE, LN, 0>(re, im, lL, sp, tb);
+  } else {
+    int32_t lNs = 0;
+    while (ln - lNs >= 3) {
+      fft_stage<T, 8, SIGN, WAVE>(re, im, ln, lL, lNs, sp, tb);
+      lNs += 3;
+    }
+    if (ln - lNs == 2) fft_stage<T, 4, SIGN, WAVE>(re, im, ln, lL, lNs, sp, tb);
+    else if (ln - lNs == 1) fft_stage<T, 2, SIGN, WAVE>(re, im, ln, lL, lNs, sp, tb);
+  }
+}
+
+struct FftDims {
+  int32_t n;       // axis length
+  int32_t ln, lL;  // log2 n, log2 L
+  int64_t inner;   // sample stride along the axis
+  int64_t count;   // samples per element
+  int64_t lines;   // lines per element (count / n)
+  int64_t pairs;   // (lines + 1) / 2
+  int32_t L;       // pairs per workgroup
+  int64_t tiles;   // workgroups per element
+  int64_t batch;
+};
+
+// The line l of element b: sample x at b count + (l / inner) n inner + x inner + l mod inner.
+__device__ __forceinline__ int64_t fft_at(const FftDims& d, int64_t b, int64_t l, int32_t x) {
+  const int64_t o = l / d.inner;
+  return b * d.count + o * d.n * d.inner + static_cast<int64_t>(x) * d.inner + (l - o * d.inner);
+}
+
+// INV 0: the orthonormal DCT-II along the axis; INV 1: its inverse with the cos basis (odd 0) or
+// the sin basis (odd 1).
+template <class T, int INV, int LN>
+__global__ void __launch_bounds__(256) k_dct_fft(FftDims d, FftTabs<T> tb, const T* __restrict__ in,
+                                                 T* __restrict__ out, int32_t odd, const int32_t* __restrict__ on) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int32_t n = LN > 0 ? (1 << LN) : d.n;
+  const int32_t L = d.L;
+  const int32_t sp = fft_sp(n);
+  T* re = reinterpret_cast<T*>(smem);
+  const int32_t im_off = fft_im_off(L, sp);
+  T* im = re + im_off;
+  // the tables in LDS: the FFT twiddles, then this direction's pre / post factors
+  T* tab = im + L * sp;
+  FftTabs<T> tl;
+  tl.n = n;
+  tl.tc = tab;
+  tl.ts = tab + n;
+  tl.pc = tl.qc = tab + 2 * n;
+  tl.ps = tl.qs = tab + 3 * n;
+  {
+    const T* xc = INV ? tb.qc : tb.pc;
+    const T* xs = INV ? tb.qs : tb.ps;
+    for (int32_t t = threadIdx.x; t < n; t += 256) {
+      const T a = tb.tc[t], bsn = tb.ts[t], c = xc[t], sn = xs[t];
+      tab[t] = a;
+      tab[n + t] = bsn;
+      tab[2 * n + t] = c;
+      tab[3 * n + t] = sn;
+    }
+  }
+  const int32_t ln = LN > 0 ? LN : d.ln, l2L = d.lL + 1;  // log2 n, log2 (2 L)
+  const int64_t b = blockIdx.x / d.tiles;
+  if (b >= d.batch || (on != nullptr && on[b] == 0)) return;  // uniform per workgroup
+  const int64_t q0 = (blockIdx.x - b * d.tiles) * L;          // the tile's first pair
+  const int32_t pts = L * n;
+  // loads: 2 L lines; along the contiguous axis sample-fastest, else line-fastest (adjacent columns)
+  const bool contiguous = d.inner == 1;
+  // a strided axis with 2 L <= 256: each thread keeps one line of the tile (its offset computed once)
+  const bool fixed_line = !contiguous && 2 * L <= 256;
+  // offsets inside the element (< 2^31 samples); in and out advance to the element
+  in += b * d.count;
+  out += b * d.count;
+  const int32_t inner = static_cast<int32_t>(d.inner);
+  int32_t my_off = 0;
+  if (fixed_line) {
+    const int32_t l = static_cast<int32_t>(2 * q0) + static_cast<int32_t>(threadIdx.x & (2 * L - 1));
+    const int32_t o = l / inner;
+    my_off = o * n * inner + (l - o * inner);
+  }
+  const auto addr = [=](int64_t l, int32_t x) -> int32_t {
+    if (contiguous) return static_cast<int32_t>(l) * n + x;
+    if (fixed_line) return my_off + x * inner;
+    return static_cast<int32_t>(fft_at(d, 0, l, x));
+  };
+  // WAVE (L a multiple of 4): wave w owns the pairs w, w + 4, ..: on the contiguous axis it loads,
+  // transforms and stores them alone; on a strided axis the workgroup loads and stores the tile
+  // together (adjacent columns) and the waves transform their pairs alone
+  const bool wave_mode = (L & 3) == 0;
+  const int32_t wave = static_cast<int32_t>(threadIdx.x >> 6);
+  const int32_t lane = static_cast<int32_t>(threadIdx.x & 63);
+  const bool wave_io = wave_mode && contiguous;
+  // the i-th load / store item of this thread: (tile line li, sample x)
+  const int32_t n_io = wave_io ? pts / 2 : 2 * pts;  // items per wave (its L / 4 pairs) or per workgroup
+  const int32_t io_first = wave_io ? lane : static_cast<int32_t>(threadIdx.x);
+  const int32_t io_step = wave_io ? 64 : 256;
+  struct Io {
+    int32_t li, x;
+  };
+  const auto io_item = [=](int32_t u) -> Io {
+    if (wave_io) {
+      const int32_t lw = u >> ln;  // the wave's line: pair wave + 4 (lw / 2), line lw mod 2
+      return Io{2 * (wave + 4 * (lw >> 1)) + (lw & 1), u & (n - 1)};
+    }
+    if (contiguous) return Io{u >> ln, u & (n - 1)};
+    return Io{u & (2 * L - 1), u >> l2L};
+  };
+  // the LDS slot of sample x of tile line li on the way in: Makhoul's order (forward), the spectrum
+  // or the reversed spectrum (inverse; slot 0 of the reversed one is zero, written apart)
+  const auto slot_in = [=](int32_t li, int32_t x) -> int32_t {
+    int32_t m;
+    if (INV == 0) m = (x & 1) ? n - 1 - (x >> 1) : (x >> 1);
+    else m = odd ? (x == 0 ? -1 : n - x) : x;
+    return m < 0 ? -1 : (li & 1 ? im_off : 0) + (li >> 1) * sp + fpad(m);
+  };
+  // 16-byte global accesses where the layout allows: VW consecutive samples of a line (contiguous
+  // axis) or the same sample of VW adjacent lines (strided axis with inner a multiple of VW); each
+  // thread's vector item u: (first line li, first sample x), the lines or samples consecutive
+  constexpr int VW = 16 / static_cast<int>(sizeof(T));
+  constexpr int LVW = VW == 4 ? 2 : 1;
+  using V16 = std::conditional_t<sizeof(T) == 4, float4, double2>;
+  const bool vec = contiguous ? n >= VW : (fixed_line && inner % VW == 0 && (2 * L) % VW == 0);
+  const int32_t nv_io = n_io >> LVW;
+  int32_t my_off_v = 0;
+  if (vec && !contiguous) {
+    const int32_t l = static_cast<int32_t>(2 * q0) + VW * static_cast<int32_t>(threadIdx.x & ((2 * L >> LVW) - 1));
+    const int32_t o = l / inner;
+    my_off_v = o * n * inner + (l - o * inner);
+  }
+  const int32_t l2Lv = l2L - LVW;
+  const auto vio = [=](int32_t u) -> Io {
+    if (wave_io) {
+      const int32_t lw = u >> (ln - LVW);
+      return Io{2 * (wave + 4 * (lw >> 1)) + (lw & 1), (u & ((n >> LVW) - 1)) << LVW};
+    }
+    if (contiguous) return Io{u >> (ln - LVW), (u & ((n >> LVW) - 1)) << LVW};
+    return Io{(u & ((2 * L >> LVW) - 1)) << LVW, u >> l2Lv};
+  };
+  const auto voff = [=](const Io& io) -> int32_t {
+    return contiguous ? (static_cast<int32_t>(2 * q0) + io.li) * n + io.x : my_off_v + io.x * inner;
+  };
+  T* lds = re;
+  if (vec) {
+    constexpr int FV = 4;  // 16-byte reads in flight per thread
+    for (int32_t u0 = io_first; u0 < nv_io; u0 += FV * io_step) {
+      V16 v[FV];
+#pragma unroll
+      for (int f = 0; f < FV; ++f) {
+        const int32_t u = u0 + f * io_step;
+        const Io io = vio(u);
+        const bool ok = u < nv_io && 2 * q0 + io.li < d.lines;
+        v[f] = ok ? *reinterpret_cast<const V16*>(in + voff(io)) : V16{};
+      }
+#pragma unroll
+      for (int f = 0; f < FV; ++f) {
+        const int32_t u = u0 + f * io_step;
+        if (u >= nv_io) continue;
+        const Io io = vio(u);
+        const T e[4] = {v[f].x, v[f].y, VW == 4 ? reinterpret_cast<const T*>(&v[f])[2] : T(0),
+                        VW == 4 ? reinterpret_cast<const T*>(&v[f])[3] : T(0)};
+#pragma unroll
+        for (int j = 0; j < VW; ++j) {
+          const int32_t li = contiguous ? io.li : io.li + j;
+          const int32_t x = contiguous ? io.x + j : io.x;
+          const int32_t at = slot_in(li, x);
+          if (at >= 0) lds[at] = e[j];
+          if (INV == 1 && odd && x == 0) lds[(li & 1 ? im_off : 0) + (li >> 1) * sp] = T(0);
+        }
+      }
+    }
+  } else {
+    constexpr int FU = 8;  // reads in flight per thread
+    for (int32_t u0 = io_first; u0 < n_io; u0 += FU * io_step) {
+      T v[FU];
+#pragma unroll
+      for (int f = 0; f < FU; ++f) {
+        const int32_t u = u0 + f * io_step;
+        const Io io = io_item(u);
+        const int64_t l = 2 * q0 + io.li;
+        v[f] = u < n_io && l < d.lines ? in[addr(l, io.x)] : T(0);
+      }
+#pragma unroll
+      for (int f = 0; f < FU; ++f) {
+        const int32_t u = u0 + f * io_step;
+        if (u >= n_io) continue;
+        const Io io = io_item(u);
+        const int32_t at = slot_in(io.li, io.x);
+        if (at >= 0) lds[at] = v[f];
+        if (INV == 1 && odd && io.x == 0) lds[(io.li & 1 ? im_off : 0) + (io.li >> 1) * sp] = T(0);
+      }
+    }
+  }
+  __syncthreads();  // the tables, and the tile when the workgroup loaded it
+  if (INV == 1) {
+    // W1 + i W2 for the pair (k, n - k) of every line pair, in place; k < half, then k = half
+    const int32_t half = n / 2;
+    const int32_t Lo = wave_mode ? L / 4 : L;  // pairs of the wave / workgroup
+    const int32_t pfirst = wave_mode ? lane : static_cast<int32_t>(threadIdx.x);
+    const int32_t pstep = wave_mode ? 64 : 256;
+    for (int32_t u = pfirst; u < Lo * (half + 1); u += pstep) {
+      const bool top = u >= Lo * half;
+      const int32_t po = top ? u - Lo * half : u >> (ln - 1);
+      const int32_t p = wave_mode ? wave + 4 * po : po;
+      const int32_t k = top ? half : u & (half - 1);
+      const int32_t kk = k == 0 ? 0 : n - k;
+      const T sk = k == 0 ? static_cast<T>(1.0 / sqrt(static_cast<double>(n))) : static_cast<T>(1.0 / sqrt(2.0 * n));
+      const T sr = static_cast<T>(1.0 / sqrt(2.0 * n));
+      const int32_t a0 = p * sp + fpad(k), a1 = p * sp + fpad(kk);
+      const T x1 = re[a0] * sk, x2 = im[a0] * sk;                            // X'_k of the two lines
+      const T y1 = k == 0 ? T(0) : re[a1] * sr, y2 = k == 0 ? T(0) : im[a1] * sr;  // X'_{n-k}
+      // W_k = e^{i phi_k} (X'_k - i X'_{n-k}); W_{n-k} = e^{i phi_{n-k}} (X'_{n-k} - i X'_k)
+      const T c0 = tl.qc[k], s0 = tl.qs[k];
+      const T w1r = c0 * x1 + s0 * y1, w1i = s0 * x1 - c0 * y1;
+      const T w2r = c0 * x2 + s0 * y2, w2i = s0 * x2 - c0 * y2;
+      re[a0] = w1r - w2i;
+      im[a0] = w1i + w2r;
+      if (k != 0 && k != half) {
+        const T c1 = tl.qc[kk], s1 = tl.qs[kk];
+        const T v1r = c1 * y1 + s1 * x1, v1i = s1 * y1 - c1 * x1;
+        const T v2r = c1 * y2 + s1 * x2, v2i = s1 * y2 - c1 * x2;
+        re[a1] = v1r - v2i;
+        im[a1] = v1i + v2r;
+      }
+    }
+  }
+  if (wave_mode) {
+    wave_sync_o();
+    fft_tile<T, INV ? +1 : -1, true, LN>(re, im, ln, d.lL, sp, tl);
+    if (!wave_io) __syncthreads();  // the workgroup stores the tile
+  } else {
+    __syncthreads();
+    fft_tile<T, INV ? +1 : -1, false, LN>(re, im, ln, d.lL, sp, tl);
+  }
+  // the output value of tile line li at sample (or frequency) x
+  const auto value_out = [&](int32_t li, int32_t x) -> T {
+    const int32_t p = li >> 1;
+    if (INV == 1) {
+      // f(2m) = v(m), f(2m+1) = v(n-1-m); the sin basis: times (-1)^x
+      const int32_t m = (x & 1) ? n - 1 - (x >> 1) : (x >> 1);
+      const T v = ((li & 1) ? im : re)[p * sp + fpad(m)];
+      return odd && (x & 1) ? -v : v;
+    }
+    // X_k of the pair's first line from V1 = (Z_k + conj Z_{n-k}) / 2, of its second from
+    // V2 = (Z_k - conj Z_{n-k}) / 2i
+    const int32_t k = x;
+    const int32_t kk = k == 0 ? 0 : n - k;
+    const T zr = re[p * sp + fpad(k)], zi = im[p * sp + fpad(k)];
+    const T yr = re[p * sp + fpad(kk)], yi = im[p * sp + fpad(kk)];
+    T vr, vi;
+    if ((li & 1) == 0) {
+      vr = (zr + yr) * T(0.5);
+      vi = (zi - yi) * T(0.5);
+    } else {
+      vr = (zi + yi) * T(0.5);
+      vi = (yr - zr) * T(0.5);
+    }
+    return tl.pc[k] * vr + tl.ps[k] * vi;
+  };
+  if (vec) {
+    for (int32_t u = io_first; u < nv_io; u += io_step) {
+      const Io io = vio(u);
+      if (2 * q0 + io.li >= d.lines) continue;
+      T e[4];
+#pragma unroll
+      for (int j = 0; j < VW; ++j) e[j] = contiguous ? value_out(io.li, io.x + j) : value_out(io.li + j, io.x);
+      V16 q;
+      if constexpr (VW == 4) q = V16{e[0], e[1], e[2], e[3]};
+      else q = V16{e[0], e[1]};
+      *reinterpret_cast<V16*>(out + voff(io)) = q;
+    }
+  } else {
+    for (int32_t u = io_first; u < n_io; u += io_step) {
+      const Io io = io_item(u);
+      const int64_t l = 2 * q0 + io.li;
+      if (l >= d.lines) continue;
+      out[addr(l, io.x)] = value_out(io.li, io.x);
+    }
+  }
+}
+
+template <class T>
+FftTabs<T> build_fft_tabs(int32_t n) {
+  const long double pi = 3.141592653589793238462643383279502884L;
+  std::vector<T> h(6 * static_cast<size_t>(n));
+  for (int64_t t = 0; t < n; ++t) {
+    const long double a = 2 * pi * static_cast<long double>(t) / static_cast<long double>(n);
+    const long double ph = pi * static_cast<long double>(t) / static_cast<long double>(2 * n);
+    const long double ck = std::sqrt((t ? 2.0L : 1.0L) / static_cast<long double>(n));
+    h[t] = static_cast<T>(std::cos(a));
+    h[n + t] = static_cast<T>(std::sin(a));
+    h[2 * n + t] = static_cast<T>(ck * std::cos(ph));
+    h[3 * n + t] = static_cast<T>(ck * std::sin(ph));
+    h[4 * n + t] = static_cast<T>(std::cos(ph));
+    h[5 * n + t] = static_cast<T>(std::sin(ph));
+  }
+  T* dv;
+  EDA_HIP(hipMalloc(&dv, h.size() * sizeof(T)));
+  EDA_HIP(hipMemcpy(dv, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  FftTabs<T> tb;
+  tb.n = n;
+  tb.tc = dv;
+  tb.ts = dv + n;
+  tb.pc = dv + 2 * n;
+  tb.ps = dv + 3 * n;
+  tb.qc = dv + 4 * n;
+  tb.qs = dv + 5 * n;
+  return tb;
+}
+
+template <class T>
+const FftTabs<T>& fft_tabs(int device, int32_t n) {
+  static std::mutex mu;
+  static std::map<std::pair<int, int32_t>, FftTabs<T>> cache;
+  std::lock_guard<std::mutex> lock(mu);
+  const auto key = std::make_pair(device, n);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  return cache.emplace(key, build_fft_tabs<T>(n)).first->second;
+}
+
+// Axes that take the fast transform: powers of two from 16 to the tile.
+template <class T>
+bool fft_axis(int32_t n) {
+  const int32_t tile = sizeof(T) == 4 ? kFftTile : kFftTile / 2;
+  return n >= 16 && n <= tile && (n & (n - 1)) == 0;
+}
+
+template <class T>
+void fft_transform(hipStream_t s, int device, const Geom& g, int a, const T* in, T* out, int inverse, int odd,
+                   const int32_t* on) {
+  const int32_t n = static_cast<int32_t>(g.n[a]);
+  const int32_t tile = sizeof(T) == 4 ? kFftTile : kFftTile / 2;
+  FftDims d{};
+  d.n = n;
+  d.inner = g.st[a];
+  d.count = g.count;
+  d.lines = g.count / n;
+  d.pairs = (d.lines + 1) / 2;
+  // a power of two; pairs past the element's last are zero lines. Rows of the contiguous axis take
+  // half the tile (more workgroups per CU); columns of a strided axis the whole (16 adjacent columns
+  // of 512 samples: 64-byte runs)
+  d.L = g.st[a] == 1 ? std::max(1, tile / n / 2) : tile / n;
+  while (d.L > 1 && d.L / 2 >= d.pairs) d.L /= 2;
+  d.tiles = (d.pairs + d.L - 1) / d.L;
+  d.ln = 0;
+  while ((1 << d.ln) < n) ++d.ln;
+  d.lL = 0;
+  while ((1 << d.lL) < d.L) ++d.lL;
+  d.batch = g.batch;
+  const FftTabs<T>& tb = fft_tabs<T>(device, n);
+  const size_t lds = (static_cast<size_t>(fft_im_off(d.L, fft_sp(n))) + static_cast<size_t>(d.L) * fft_sp(n) +
+                      4 * static_cast<size_t>(n)) * sizeof(T);
+  const int64_t grid = d.tiles * g.batch;
+  const bool timed = timer_gemm_on();
+  hipEvent_t t0 = nullptr;
+  if (timed) timer_note(s, &t0);
+  const dim3 gr(static_cast<unsigned>(grid));
+  const auto go = [&](auto kern) { hipLaunchKernelGGL(kern, gr, dim3(256), lds, s, d, tb, in, out, odd, on); };
+  // lengths 2^5 .. 2^10 with their index arithmetic folded at compile time
+  switch (inverse ? 100 + d.ln : d.ln) {
+    case 5: go(k_dct_fft<T, 0, 5>); break;
+    case 6: go(k_dct_fft<T, 0, 6>); break;
+    case 7: go(k_dct_fft<T, 0, 7>); break;
+    case 8: go(k_dct_fft<T, 0, 8>); break;
+    case 9: go(k_dct_fft<T, 0, 9>); break;
+    case 10: go(k_dct_fft<T, 0, 10>); break;
+    case 105: go(k_dct_fft<T, 1, 5>); break;
+    case 106: go(k_dct_fft<T, 1, 6>); break;
+    case 107: go(k_dct_fft<T, 1, 7>); break;
+    case 108: go(k_dct_fft<T, 1, 8>); break;
+    case 109: go(k_dct_fft<T, 1, 9>); break;
+    case 110: go(k_dct_fft<T, 1, 10>); break;
+    default:
+      if (inverse) go(k_dct_fft<T, 1, 0>);
+      else go(k_dct_fft<T, 0, 0>);
+  }
+  EDA_LAUNCH("k_dct_fft");
+  if (timed) timer_done(s, t0, 2.0 * static_cast<double>(g.batch) * g.count * sizeof(T));
 }
 
 // ---- matrices -----------------------------------------------------------------------------------
@@ -534,15 +1133,20 @@ const AxisMats<T>& axis_mats(int device, int32_t n) {
 
 template <class T>
 void enqueue_dct(hipStream_t s, int device, const Geom& g, const T* in, T* out, T* tmp, int inverse, int odd) {
-  // the axes from last to first, ping-ponging between tmp and out so that the last pass lands in out
+  // the axes from last to first, ping-ponging between tmp and out so that the last pass lands in out;
+  // power-of-two axes through the fast transform, the others through their dense matrix
   const int D = g.dim;
   const T* src = in;
   for (int k = 0; k < D; ++k) {
     const int a = D - 1 - k;
-    const AxisMats<T>& m = axis_mats<T>(device, g.n[a]);
-    const T* M = !inverse ? m.fwd : (((odd >> a) & 1) ? m.isin : m.icos);
     T* dst = ((D - 1 - k) % 2 == 0) ? out : tmp;
-    axis_transform<T>(s, g, a, M, src, dst, nullptr);
+    if (fft_axis<T>(static_cast<int32_t>(g.n[a]))) {
+      fft_transform<T>(s, device, g, a, src, dst, inverse, (odd >> a) & 1, nullptr);
+    } else {
+      const AxisMats<T>& m = axis_mats<T>(device, g.n[a]);
+      const T* M = !inverse ? m.fwd : (((odd >> a) & 1) ? m.isin : m.icos);
+      axis_transform<T>(s, g, a, M, src, dst, nullptr);
+    }
     src = dst;
   }
 }

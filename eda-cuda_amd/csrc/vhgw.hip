@@ -388,13 +388,14 @@ __host__ __device__ __forceinline__ WinInfo win_info(int32_t n, int32_t w) {
 
 // The box sums of the nl lines of a tile. Line p is staged at E + p epad + elead: the extended
 // position e in [-r, n + r) at offset e, with at least 32 slots of slack past both ends; its H row
-// (H + p epad + elead) has the same shape. One lane per segment (line, segment k), the tasks of the
-// tile spread over the workgroup's waves in rounds of 64. Each lane walks its segment in the
-// canonical order: 32-sample chunks, a sequential fold inside a chunk, G = carry + fold, the carry
-// the sequential fold of the chunk totals before (kept in a register). All lanes of a round step
-// together, so the words they touch lie wa (odd) apart. A chunk that starts past the end of the
-// extended line is skipped (its lane's later chunks too); a chunk that runs past it reads and writes
-// the 32 slots of slack there. DIR 1 walks the reversed line (its position e' reads e = n - 1 - e')
+// (H + p epad + elead) has the same shape. Windows of at most 31 samples (r < 16): a segment is one
+// 32-sample chunk of the canonical order, so G is the sequential fold from the segment's start. One
+// lane per segment (line, segment k), the tasks of the tile spread over the workgroup's waves in
+// rounds of 64; each lane walks its wa samples in order, the first wa mod 8 through a fallthrough
+// chain and the rest in unrolled groups of eight (no per-sample loop control). All lanes of a round
+// step together, so the words they touch lie wa (odd) apart. A segment that starts past the end of
+// the extended line is skipped; one that runs past it reads and writes the slack there. DIR 1 walks
+// the reversed line (its position e' reads e = n - 1 - e')
 // and writes H(e') at H row offset e, so H(x - r) sits at offset x - r; DIR 0 walks the line and
 // writes y(x) = H(x - r) + G(x + r) over its input at offset e = x + r, halved where e ends its
 // segment (every input sample is read by the one lane whose segment holds it, in the step that
@@ -406,51 +407,59 @@ __device__ void box_dir(T* __restrict__ E, T* __restrict__ H, int32_t epad, int3
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int W = blockDim.x >> 6;
-  const int32_t r = wi.r, wa = wi.wa, nsegs = wi.nsegs, cs = wi.cs;
+  // a segment here is at most 31 samples long (r < 16): one chunk, so G is the plain running fold
+  const int32_t r = wi.r, wa = wi.wa, nsegs = wi.nsegs;
   const int32_t e_max = n - 1 + r;
   const int32_t total = nl * nsegs;
   const float inv_ns = 1.0f / static_cast<float>(nsegs);
-  const int32_t lb = wa - 32 * (cs - 1);  // the last chunk's length
-  const T nz = -T(0);
+  const int32_t rem = wa & 7;  // the steps before the groups of eight
   for (int32_t u0 = wave * 64; u0 < total; u0 += W * 64) {
     const int32_t u = u0 + lane;
     const int32_t p = min(div_small(u, nsegs, inv_ns), nl - 1);
     const int32_t k = u - p * nsegs;
     const int32_t s = wi.o + (wi.k_lo + k) * wa;  // the segment's first position
-    const bool live = u < total;
-    // chunk q, sample j: position s + 32 q + j, at ep[-(32 q + j)] (DIR 1) or ep[32 q + j] (DIR 0)
+    if (u >= total || s > e_max) continue;
+    // sample j of the walk: position s + j, at ep[-j] (DIR 1) or ep[j] (DIR 0)
     T* ep = E + p * epad + elead + (DIR ? n - 1 - s : s);
     T* hp = H + p * epad + elead + (DIR ? n - 1 - s : s - 2 * r);
-    T carry = nz;
-    for (int32_t q = 0; q < cs; ++q) {
-      const int32_t len = q < cs - 1 ? 32 : lb;
-      if (live && s + 32 * q <= e_max) {
-        T acc = nz;
-        const auto step = [&](int32_t j) {
-          acc = acc + (DIR ? ep[-j] : ep[j]);
-          const T g = q > 0 ? carry + acc : acc;
-          if constexpr (DIR == 1) {
-            hp[-j] = g;
-          } else {
-            ep[j] = hp[j] + g;
-          }
-        };
-        int32_t j = 0;
-        for (; j + 8 <= len; j += 8) {
-#pragma unroll
-          for (int v = 0; v < 8; ++v) step(j + v);
+    T acc = -T(0);
+    {
+      // steps j = 0 .. rem - 1 as j = rem - i from bases rem samples along: entering the chain at
+      // case rem runs them in order
+      T* e2 = DIR ? ep - rem : ep + rem;
+      T* h2 = DIR ? hp - rem : hp + rem;
+      const auto first = [&](int i) {
+        acc = acc + (DIR ? e2[i] : e2[-i]);
+        if constexpr (DIR == 1) {
+          h2[i] = acc;
+        } else {
+          e2[-i] = h2[-i] + acc;
         }
-        for (; j < len; ++j) step(j);
-        carry = q > 0 ? carry + acc : acc;
+      };
+      switch (rem) {
+        case 7: first(7); [[fallthrough]];
+        case 6: first(6); [[fallthrough]];
+        case 5: first(5); [[fallthrough]];
+        case 4: first(4); [[fallthrough]];
+        case 3: first(3); [[fallthrough]];
+        case 2: first(2); [[fallthrough]];
+        case 1: first(1); [[fallthrough]];
+        default: break;
       }
-      ep = DIR ? ep - 32 : ep + 32;
-      hp = DIR ? hp - 32 : hp + 32;
     }
-    // DIR 0: the window that is the segment (ep is now 32 cs past the segment's start)
-    if (DIR == 0 && live && s + 32 * (cs - 1) <= e_max) {
-      T* y = ep - 32 + (lb - 1);
-      *y = *y * T(0.5);
+    for (int32_t j = rem; j < wa; j += 8) {
+#pragma unroll
+      for (int v = 0; v < 8; ++v) {
+        acc = acc + (DIR ? ep[-(j + v)] : ep[j + v]);
+        if constexpr (DIR == 1) {
+          hp[-(j + v)] = acc;
+        } else {
+          ep[j + v] = hp[j + v] + acc;
+        }
+      }
     }
+    // DIR 0: the window that is the segment
+    if (DIR == 0) ep[wa - 1] = ep[wa - 1] * T(0.5);
   }
 }
 

@@ -151,10 +151,10 @@ int eda_pointer_device(const void* ptr, int32_t* device);    /* -1 for host memo
 /* Kernel timing, for the live roofline figure of a benchmark: with on = 1, every launch of the
  * cube filter's pass kernel is bracketed by HIP events on the stream it runs on, and its algorithmic
  * bytes (input fields read once, output fields written once, over the running elements) are
- * counted; with on = 2, every launch of the DCT contraction (the transform GEMMs) instead, with its
- * flops (2 M N K per batch entry) in place of the bytes; 0 turns it off. eda_timer_read waits for
- * the recorded launches, returns the total kernel time in ms, the launch count and the bytes (or
- * flops), and clears them. No counterpart in the reference (it has no timing in code). */
+ * counted; with on = 2, every axis pass of the DCT instead (one read and one write of the field
+ * each); 0 turns it off. eda_timer_read waits for the recorded launches, returns the total kernel
+ * time in ms, the launch count and the bytes, and clears them. No counterpart in the reference (it
+ * has no timing in code). */
 int eda_timer_enable(int32_t on);
 int eda_timer_read(double* ms, int64_t* launches, double* bytes, char* err, size_t errlen);
 

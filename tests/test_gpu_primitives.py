@@ -142,8 +142,10 @@ def test_components_are_exact(shape):
 
 
 @pytest.mark.parametrize("dtype,tol", [(np.float32, 3e-6), (np.float64, 1e-13)])
-@pytest.mark.parametrize("shape", [(64, 48), (16, 20, 24), (512, 512)])
+@pytest.mark.parametrize("shape", [(64, 48), (16, 20, 24), (512, 512), (128, 33), (32, 64, 16), (2048,), (33, 4096)])
 def test_dct_matches_the_oracle(dtype, tol, shape):
+    """Both transforms: the dense matrices (axes of other lengths) and the fast transform of
+    power-of-two axes from 16 samples (an odd number of lines pairs the last with a zero line)."""
     x = np.random.default_rng(7).standard_normal((2,) + shape).astype(dtype)
     for inv, odd in ((0, 0), (1, 0), (1, 1), (1, 2), (1, 3)):
         got = G.dct(x, inv, odd)
