@@ -104,7 +104,7 @@ void enqueue_ccl(hipStream_t s, const Geom& g, const uint8_t* mask, uint8_t bits
   const int32_t nl = g.n[g.dim - 1];
   const int64_t lines = g.count / nl;
   const int64_t gx = (lines + kThreads / 64 - 1) / (kThreads / 64);
-  hipLaunchKernelGGL(k_ccl_runs, dim3(static_cast<unsigned>(gx), static_cast<unsigned>(g.batch)), dim3(kThreads), 0, s,
+  EDA_GO(k_ccl_runs, dim3(static_cast<unsigned>(gx), static_cast<unsigned>(g.batch)), dim3(kThreads), 0, s,
                      g, mask, bits, labels, on);
   EDA_LAUNCH("k_ccl_runs");
   if (g.dim == 1) return;  // a line's runs are its components

@@ -45,6 +45,16 @@ struct OutOfDeviceMemory : std::runtime_error {
     const hipError_t eda_e_ = (expr);                                 \
     if (eda_e_ != hipSuccess) ::eda::throw_hip(eda_e_, #expr, __FILE__, __LINE__); \
   } while (0)
+// Issues the parameter uploads the calling thread has staged since the last call (as one copy, on
+// the stream they were staged for). Every kernel launch and every device-side copy that may read
+// an upload runs it first (EDA_GO wraps the launches).
+void flush_uploads();
+#define EDA_GO(...)                 \
+  do {                              \
+    ::eda::flush_uploads();         \
+    hipLaunchKernelGGL(__VA_ARGS__); \
+  } while (0)
+
 #define EDA_LAUNCH(name)                                              \
   do {                                                                \
     const hipError_t eda_e_ = hipGetLastError();                      \
@@ -182,7 +192,7 @@ void each(hipStream_t s, const Geom& g, const int32_t* on, F f) {
   int64_t gx = (g.count + kThreads - 1) / kThreads;
   gx = gx < 1 ? 1 : (gx > 8192 ? 8192 : gx);
   const int64_t gy = g.batch < 65535 ? g.batch : 65535;
-  hipLaunchKernelGGL(k_each<F>, dim3(static_cast<unsigned>(gx), static_cast<unsigned>(gy)), dim3(kThreads), 0, s,
+  EDA_GO(k_each<F>, dim3(static_cast<unsigned>(gx), static_cast<unsigned>(gy)), dim3(kThreads), 0, s,
                      g.count, g.batch, on, f);
   EDA_LAUNCH("k_each");
 }
@@ -198,7 +208,7 @@ void flat(hipStream_t s, int64_t n, F f) {
   if (n <= 0) return;
   int64_t gx = (n + kThreads - 1) / kThreads;
   gx = gx > 65536 ? 65536 : gx;
-  hipLaunchKernelGGL(k_flat<F>, dim3(static_cast<unsigned>(gx)), dim3(kThreads), 0, s, n, f);
+  EDA_GO(k_flat<F>, dim3(static_cast<unsigned>(gx)), dim3(kThreads), 0, s, n, f);
   EDA_LAUNCH("k_flat");
 }
 

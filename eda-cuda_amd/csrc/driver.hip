@@ -78,6 +78,7 @@ void timer_set(int kind) { timer_state().on = kind; }
 bool timer_on() { return timer_state().on.load(std::memory_order_relaxed) == 1; }
 bool timer_gemm_on() { return timer_state().on.load(std::memory_order_relaxed) == 2; }
 void timer_note(hipStream_t s, hipEvent_t* start) {
+  flush_uploads();  // a staged upload is not part of the timed launch
   TimerState& t = timer_state();
   std::lock_guard<std::mutex> lock(t.mu);
   *start = t.take();
@@ -215,11 +216,24 @@ struct Carve {
 };
 
 // The parameter arena: host values copied through pinned memory into device memory.
+struct Uploader;
+thread_local Uploader* t_staged = nullptr;  // the uploader with staged, not yet copied, bytes
+
+// Per-step parameter arrays: put() stages them in pinned host memory and returns their device
+// address; the staged bytes go to the device as one copy when the next kernel or device copy is
+// issued (flush_uploads), so that a burst of uploads costs one copy, not one each.
 struct Uploader {
   char* host = nullptr;
   DevMem dev;
   size_t cap = 0, off = 0;
+  size_t pbeg = 0, pend = 0;  // the staged range [pbeg, pend)
   hipStream_t s = nullptr;
+  void flush() {
+    if (pend > pbeg) EDA_HIP(hipMemcpyAsync(static_cast<char*>(dev.p) + pbeg, host + pbeg, pend - pbeg,
+                                           hipMemcpyHostToDevice, s));
+    pbeg = pend = 0;
+    if (t_staged == this) t_staged = nullptr;
+  }
   void init(size_t bytes, hipStream_t stream) {
     cap = bytes;
     s = stream;
@@ -227,6 +241,7 @@ struct Uploader {
     dev = DevMem(cap);
   }
   ~Uploader() {
+    if (t_staged == this) t_staged = nullptr;
     if (host) (void)hipHostFree(host);
   }
   template <class V>
@@ -234,9 +249,12 @@ struct Uploader {
     const size_t bytes = n * sizeof(V);
     const size_t a = (off + 255) / 256 * 256;
     if (a + bytes > cap) throw std::logic_error("eda: the parameter arena is full");
+    if (t_staged != nullptr && t_staged != this) t_staged->flush();
     std::memcpy(host + a, data, bytes);
     char* d = static_cast<char*>(dev.p) + a;
-    EDA_HIP(hipMemcpyAsync(d, host + a, bytes, hipMemcpyHostToDevice, s));
+    if (pend == pbeg) pbeg = a;
+    pend = a + bytes;
+    t_staged = this;
     off = a + bytes;
     if (timer_on()) mirror_note(d, host + a, bytes);
     return reinterpret_cast<V*>(d);
@@ -246,16 +264,27 @@ struct Uploader {
     return put(v.data(), v.size());
   }
   void reset() {  // only after the stream is synchronized and no upload is read again
+    flush();
     off = 0;
     mirror_clear();
   }
   void reserve(size_t bytes) {
     if (bytes <= cap) return;
+    flush();
+    EDA_HIP(hipStreamSynchronize(s));
     if (host) EDA_HIP(hipHostFree(host));
     host = nullptr;
     init(bytes, s);
   }
 };
+
+}  // namespace
+
+void flush_uploads() {
+  if (t_staged != nullptr) t_staged->flush();
+}
+
+namespace {
 
 // The per-(thread, device) context: the stream and the parameter arena.
 struct Context {
@@ -1343,6 +1372,7 @@ bool Chunk<T>::mode() {
       init[2 * b + 1] = 0;
     }
     const uint64_t* dinit = ctx_.up.put(init);
+    flush_uploads();
     EDA_HIP(hipMemcpyAsync(d_dminmax_, dinit, sizeof(uint64_t) * 2 * B, hipMemcpyDeviceToDevice, s_));
   }
   enqueue_element_build<T>(s_, g_, c_, L_, nvec_, coh_, o_, on, d_dminmax_);
@@ -1372,6 +1402,7 @@ bool Chunk<T>::mode() {
   on = upload_on(active);
   {
     const ElemDev* del = ctx_.up.put(el);
+    flush_uploads();
     EDA_HIP(hipMemcpyAsync(d_el_, del, sizeof(ElemDev) * B, hipMemcpyDeviceToDevice, s_));
   }
   EDA_HIP(hipMemsetAsync(d_bits_, 0, sizeof(uint32_t) * 2 * B, s_));
@@ -1398,9 +1429,11 @@ bool Chunk<T>::mode() {
     }
   {
     const ElemDev* del = ctx_.up.put(el);
+    flush_uploads();
     EDA_HIP(hipMemcpyAsync(d_el_, del, sizeof(ElemDev) * B, hipMemcpyDeviceToDevice, s_));
     if (K_max_ > 0) {
       const int32_t* drw = ctx_.up.put(rw);
+      flush_uploads();
       EDA_HIP(hipMemcpyAsync(d_rung_win_, drw, sizeof(int32_t) * rw.size(), hipMemcpyDeviceToDevice, s_));
     }
   }
@@ -1482,6 +1515,7 @@ bool Chunk<T>::mode() {
         for (int64_t b = 0; b < B; ++b)
           if (!sifting(rec_[b])) rw[jj * B + b] = 0;
       const int32_t* drw = ctx_.up.put(rw);
+      flush_uploads();
       EDA_HIP(hipMemcpyAsync(d_rung_win_, drw, sizeof(int32_t) * rw.size(), hipMemcpyDeviceToDevice, s_));
       rw_cur_ = rw;
       if (timer_on() && !rw_cur_.empty()) mirror_note(d_rung_win_, rw_cur_.data(), rw_cur_.size() * sizeof(int32_t));
@@ -1568,7 +1602,7 @@ bool Chunk<T>::mode() {
     {
       int64_t gx = (g.count + kThreads - 1) / kThreads;
       gx = gx > 64 ? 64 : gx;  // 64 workgroups per element: 64 atomics per element and maximum
-      hipLaunchKernelGGL(k_sift_step<T>, dim3(static_cast<unsigned>(gx), static_cast<unsigned>(B)), dim3(kThreads), 0,
+      EDA_GO(k_sift_step<T>, dim3(static_cast<unsigned>(gx), static_cast<unsigned>(B)), dim3(kThreads), 0,
                          s_, g.count, son, a, Mn, h, r, hn, gmax);
       EDA_LAUNCH("k_sift_step");
     }

@@ -356,9 +356,9 @@ template <class T, int OP>
 void launch_gather_all(hipStream_t s, const Geom& g, const ElemDev* el, const uint8_t* lo, const T* t, const T* o,
                        const T* r0, const T* r1, T* out0, T* out1, const int32_t* on) {
   const dim3 grid = elem_grid(g);
-  if (g.dim == 1) hipLaunchKernelGGL((k_gather_all<T, 1, OP>), grid, dim3(kThreads), 0, s, g, el, lo, t, o, r0, r1, out0, out1, on);
-  else if (g.dim == 2) hipLaunchKernelGGL((k_gather_all<T, 2, OP>), grid, dim3(kThreads), 0, s, g, el, lo, t, o, r0, r1, out0, out1, on);
-  else hipLaunchKernelGGL((k_gather_all<T, 3, OP>), grid, dim3(kThreads), 0, s, g, el, lo, t, o, r0, r1, out0, out1, on);
+  if (g.dim == 1) EDA_GO((k_gather_all<T, 1, OP>), grid, dim3(kThreads), 0, s, g, el, lo, t, o, r0, r1, out0, out1, on);
+  else if (g.dim == 2) EDA_GO((k_gather_all<T, 2, OP>), grid, dim3(kThreads), 0, s, g, el, lo, t, o, r0, r1, out0, out1, on);
+  else EDA_GO((k_gather_all<T, 3, OP>), grid, dim3(kThreads), 0, s, g, el, lo, t, o, r0, r1, out0, out1, on);
   EDA_LAUNCH("k_gather_all");
 }
 
@@ -367,8 +367,8 @@ template <class T>
 void launch_gather_pairs(hipStream_t s, const Geom& g, const ElemDev* el, const uint8_t* lo, const T* t, const T* o,
                          const T* pairs, T* out0, T* out1, const int32_t* on) {
   const dim3 grid = elem_grid(g);
-  if (g.dim == 2) hipLaunchKernelGGL((k_gather_all<T, 2, 2, true>), grid, dim3(kThreads), 0, s, g, el, lo, t, o, pairs, nullptr, out0, out1, on);
-  else hipLaunchKernelGGL((k_gather_all<T, 3, 2, true>), grid, dim3(kThreads), 0, s, g, el, lo, t, o, pairs, nullptr, out0, out1, on);
+  if (g.dim == 2) EDA_GO((k_gather_all<T, 2, 2, true>), grid, dim3(kThreads), 0, s, g, el, lo, t, o, pairs, nullptr, out0, out1, on);
+  else EDA_GO((k_gather_all<T, 3, 2, true>), grid, dim3(kThreads), 0, s, g, el, lo, t, o, pairs, nullptr, out0, out1, on);
   EDA_LAUNCH("k_gather_all");
 }
 
@@ -377,7 +377,7 @@ void launch_gather_pairs(hipStream_t s, const Geom& g, const ElemDev* el, const 
 template <class T>
 void enqueue_element_build(hipStream_t s, const Geom& g, const MethodCtl& c, const T* L, const T* nvec, const T* coh,
                            T* o, const int32_t* on, uint64_t* dminmax) {
-  hipLaunchKernelGGL(k_build<T>, elem_grid(g, 64), dim3(kThreads), 0, s, g, static_cast<T>(c.q),
+  EDA_GO(k_build<T>, elem_grid(g, 64), dim3(kThreads), 0, s, g, static_cast<T>(c.q),
                      static_cast<T>(c.span - 1.0), L, nvec, g.dim >= 2 ? coh : nullptr, o, on, dminmax);
   EDA_LAUNCH("k_build");
 }
@@ -385,7 +385,7 @@ void enqueue_element_build(hipStream_t s, const Geom& g, const MethodCtl& c, con
 template <class T>
 void enqueue_element_weights(hipStream_t s, const Geom& g, const MethodCtl& c, const T* L, const T* coh,
                              const ElemDev* el, uint8_t* lo, T* t, const int32_t* on, uint32_t* bits) {
-  hipLaunchKernelGGL(k_weights<T>, elem_grid(g, 64), dim3(kThreads), 0, s, g, static_cast<T>(c.q),
+  EDA_GO(k_weights<T>, elem_grid(g, 64), dim3(kThreads), 0, s, g, static_cast<T>(c.q),
                      static_cast<T>(c.span - 1.0), L, g.dim >= 2 ? coh : nullptr, el, lo, t, on, bits);
   EDA_LAUNCH("k_weights");
 }

@@ -417,7 +417,7 @@ void enqueue_sig_flags(hipStream_t s, const Geom& g, const T* r, const T* tau, u
       t.wide = al4 && (t.inner == 1 ? n % 4 == 0 : (P % 4 == 0 && t.inner % 4 == 0)) ? 1 : 0;
       const int64_t wgs = g.batch * t.groups;
       if (wgs > 0x7fffffff) throw std::invalid_argument("eda: too many workgroups for the extrema pass");
-      hipLaunchKernelGGL(k_flags_tile<T>, dim3(static_cast<unsigned>(wgs)), dim3(kThreads), bytes(P), s, t, r, tau,
+      EDA_GO(k_flags_tile<T>, dim3(static_cast<unsigned>(wgs)), dim3(kThreads), bytes(P), s, t, r, tau,
                          flags, ext, on);
       EDA_LAUNCH("k_flags_tile");
       first = false;
@@ -438,7 +438,7 @@ void enqueue_sig_flags(hipStream_t s, const Geom& g, const T* r, const T* tau, u
     L.entries = L.outer * L.nb * L.inner;
     L.axis = a;
     if (n > kBlk) {
-      hipLaunchKernelGGL(k_flag_blocks<T>,
+      EDA_GO(k_flag_blocks<T>,
                          dim3(static_cast<unsigned>((L.entries + kThreads - 1) / kThreads), static_cast<unsigned>(g.batch)),
                          dim3(kThreads), 0, s, L, g.batch, r, scratch, on);
       EDA_LAUNCH("k_flag_blocks");
@@ -446,7 +446,7 @@ void enqueue_sig_flags(hipStream_t s, const Geom& g, const T* r, const T* tau, u
     // (workgroups per element, elements): at most 128 workgroups, so 128 atomics, per element
     int64_t gx = (g.count + kThreads - 1) / kThreads;
     gx = std::min<int64_t>(gx, std::max<int64_t>(16, std::min<int64_t>(128, 16384 / std::max<int64_t>(g.batch, 1))));
-    hipLaunchKernelGGL(k_flags<T>, dim3(static_cast<unsigned>(gx), static_cast<unsigned>(g.batch)), dim3(kThreads), 0,
+    EDA_GO(k_flags<T>, dim3(static_cast<unsigned>(gx), static_cast<unsigned>(g.batch)), dim3(kThreads), 0,
                        s, L, g.batch, r, tau, flags, ext, on, scratch);
     EDA_LAUNCH("k_flags");
   }

@@ -238,10 +238,10 @@ void gemm(hipStream_t s, const GemmDims& d, const T* A, const T* B, T* C, const 
   const dim3 grid(static_cast<unsigned>((d.N + kTN - 1) / kTN), static_cast<unsigned>((d.M + kTM - 1) / kTM),
                   static_cast<unsigned>(d.batches));
   if constexpr (std::is_same_v<T, float>) {
-    hipLaunchKernelGGL(k_gemm_mfma, grid, dim3(kThreads), 0, s, d, A, B, C, on);
+    EDA_GO(k_gemm_mfma, grid, dim3(kThreads), 0, s, d, A, B, C, on);
     EDA_LAUNCH("k_gemm_mfma");
   } else {
-    hipLaunchKernelGGL(k_gemm<T>, grid, dim3(kThreads), 0, s, d, A, B, C, on);
+    EDA_GO(k_gemm<T>, grid, dim3(kThreads), 0, s, d, A, B, C, on);
     EDA_LAUNCH("k_gemm");
   }
 }
@@ -859,7 +859,7 @@ void fft_transform(hipStream_t s, int device, const Geom& g, int a, const T* in,
   hipEvent_t t0 = nullptr;
   if (timed) timer_note(s, &t0);
   const dim3 gr(static_cast<unsigned>(grid));
-  const auto go = [&](auto kern) { hipLaunchKernelGGL(kern, gr, dim3(256), lds, s, d, tb, in, out, odd, on); };
+  const auto go = [&](auto kern) { EDA_GO(kern, gr, dim3(256), lds, s, d, tb, in, out, odd, on); };
   // lengths 2^5 .. 2^10 with their index arithmetic folded at compile time
   switch (inverse ? 100 + d.ln : d.ln) {
     case 5: go(k_dct_fft<T, 0, 5>); break;
@@ -1060,7 +1060,7 @@ constexpr int kMeanParts = 64;
 // with a fixed tree each, then their sum in order (the value depends on the element's samples only).
 template <class T>
 void enqueue_mean(hipStream_t s, const Geom& g, const T* r, const int32_t* on, double* mean, double* part) {
-  hipLaunchKernelGGL(k_mean_part<T>, dim3(kMeanParts, static_cast<unsigned>(g.batch)), dim3(kThreads), 0, s, r, g.count,
+  EDA_GO(k_mean_part<T>, dim3(kMeanParts, static_cast<unsigned>(g.batch)), dim3(kThreads), 0, s, r, g.count,
                      on, part);
   EDA_LAUNCH("k_mean_part");
   const int64_t count = g.count;

@@ -154,10 +154,10 @@ void enqueue_elem_stats(hipStream_t s, const Geom& g, const T* f, const int32_t*
   double* row_e = static_cast<double*>(rows_ws);
   uint64_t* row_a = reinterpret_cast<uint64_t*>(row_e + rows);
   const int64_t blocks = (rows + (kThreads / 64) - 1) / (kThreads / 64);
-  hipLaunchKernelGGL(k_row_stats<T>, dim3(static_cast<unsigned>(blocks)), dim3(kThreads), 0, s, f, rows, nl, rpe, on,
+  EDA_GO(k_row_stats<T>, dim3(static_cast<unsigned>(blocks)), dim3(kThreads), 0, s, f, rows, nl, rpe, on,
                      row_e, row_a);
   EDA_LAUNCH("k_row_stats");
-  hipLaunchKernelGGL(k_elem_stats, dim3(static_cast<unsigned>(g.batch)), dim3(kThreads), 0, s, rpe, on, row_e, row_a,
+  EDA_GO(k_elem_stats, dim3(static_cast<unsigned>(g.batch)), dim3(kThreads), 0, s, rpe, on, row_e, row_a,
                      energy, amax_bits, nonfinite);
   EDA_LAUNCH("k_elem_stats");
 }

@@ -278,7 +278,7 @@ void enqueue_select(hipStream_t s, const Geom& g, const Selection<T>& sel, int m
   SelState* st = static_cast<SelState*>(ws);
   uint64_t* hist = reinterpret_cast<uint64_t*>(static_cast<char*>(ws) + align_up(static_cast<size_t>(B) * sizeof(SelState)));
   const int64_t ni = B * 2 * kBins;
-  hipLaunchKernelGGL(k_sel_init, dim3(static_cast<unsigned>((ni + kThreads - 1) / kThreads)), dim3(kThreads), 0, s, B,
+  EDA_GO(k_sel_init, dim3(static_cast<unsigned>((ni + kThreads - 1) / kThreads)), dim3(kThreads), 0, s, B,
                      st, hist, on);
   EDA_LAUNCH("k_sel_init");
   int64_t nblk = (g.count + int64_t{kThreads} * 32 - 1) / (int64_t{kThreads} * 32);  // blocks per element
@@ -288,15 +288,15 @@ void enqueue_select(hipStream_t s, const Geom& g, const Selection<T>& sel, int m
       const int top = 64 - kBits * p;
       const int shift = top - kBits > 0 ? top - kBits : 0;
       const int width = top - shift;
-      hipLaunchKernelGGL(k_sel_hist<T>, dim3(static_cast<unsigned>(nblk * B)), dim3(kThreads), 0, s, a, st, hist,
+      EDA_GO(k_sel_hist<T>, dim3(static_cast<unsigned>(nblk * B)), dim3(kThreads), 0, s, a, st, hist,
                          static_cast<int32_t>(nblk), shift, width, p == 0 ? 1 : 0, on);
       EDA_LAUNCH("k_sel_hist");
-      hipLaunchKernelGGL(k_sel_pick, dim3(static_cast<unsigned>(B)), dim3(kThreads), 0, s, st, hist, shift, width,
+      EDA_GO(k_sel_pick, dim3(static_cast<unsigned>(B)), dim3(kThreads), 0, s, st, hist, shift, width,
                          p == 0 ? 1 : 0, mode, q, on);
       EDA_LAUNCH("k_sel_pick");
     }
   }
-  hipLaunchKernelGGL(k_sel_final, dim3(static_cast<unsigned>((B + kThreads - 1) / kThreads)), dim3(kThreads), 0, s, B,
+  EDA_GO(k_sel_final, dim3(static_cast<unsigned>((B + kThreads - 1) / kThreads)), dim3(kThreads), 0, s, B,
                      st, mode, out, count, on);
   EDA_LAUNCH("k_sel_final");
 }

@@ -1322,7 +1322,7 @@ void launch_kt(int kt, dim3 grid, dim3 block, size_t lds, hipStream_t s, const V
     if (lds > 65536)
       EDA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
                                   static_cast<int>(lds)));
-    hipLaunchKernelGGL(k, grid, block, lds, s, a, p);
+    EDA_GO(k, grid, block, lds, s, a, p);
   };
   if constexpr (std::is_same_v<S, T>) {
     go(k_vhgw<T, S, 1>);
@@ -1349,7 +1349,7 @@ void go_rows(hipStream_t s, int64_t items, const RPass& r, const VPtrs<T>& p) {
   if (lds > 65536)
     EDA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
                                 static_cast<int>(lds)));
-  hipLaunchKernelGGL(k, dim3(static_cast<unsigned>(wgs)), dim3(64 * W), lds, s, r, p);
+  EDA_GO(k, dim3(static_cast<unsigned>(wgs)), dim3(64 * W), lds, s, r, p);
 }
 
 // Compute units of the current device (cached per device).
@@ -1389,7 +1389,7 @@ void launch_box(int pf, hipStream_t s, const VPass& a, const VPtrs<T>& p, int32_
     if (lds > 65536)
       EDA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
                                   static_cast<int>(lds)));
-    hipLaunchKernelGGL(k, dim3(static_cast<unsigned>(wgs)), dim3(64 * W), lds, s, a, p, rungs);
+    EDA_GO(k, dim3(static_cast<unsigned>(wgs)), dim3(64 * W), lds, s, a, p, rungs);
   };
   if (pf == 4) go(k_box<T, 4, PAD>);
   else go(k_box<T, 8, PAD>);
