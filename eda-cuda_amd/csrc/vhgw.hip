@@ -577,6 +577,21 @@ __device__ void chunk_lines(const T* E, int32_t epad, int32_t lead, T* R, int32_
 }
 
 // ---- the pass kernel ----------------------------------------------------------------------------------------
+// Workgroup i runs on XCD i mod 8. The item of workgroup i when chunks of c consecutive items
+// rotate over the XCDs (the grid a multiple of 8 c): the tiles of one chunk are neighbours in
+// memory and share an XCD's L2, and the items of any few elements spread over all eight XCDs (a
+// launch in which only some elements run keeps the whole chip busy).
+__device__ __forceinline__ int64_t xcd_item(int64_t i, int32_t c) {
+  const int64_t j = i >> 3;
+  return ((j / c) * 8 + (i & 7)) * c + j % c;
+}
+// The largest chunk of 8, 4, 2 items whose rotation tiles a grid of wgs workgroups (0: none).
+int32_t xcd_chunk(int64_t wgs) {
+  for (int32_t c = 8; c >= 2; c >>= 1)
+    if (wgs % (8 * c) == 0) return c;
+  return 0;
+}
+
 struct VPass {
   Geom g;
   int32_t n;        // line length
@@ -591,7 +606,7 @@ struct VPass {
   int32_t nf;       // output fields: 1, or 2 (ops[0] into out[0], ops[1] into out[1])
   int32_t ops[2];
   int32_t scale;    // multiply by fl_T(1 / prod of the clamped sides) (the last box pass)
-  int32_t xcd;      // remap workgroup ids: consecutive groups on one XCD
+  int32_t xcd;      // remap workgroup ids: chunks of xcd consecutive groups rotate over the 8 XCDs (0: none)
   int32_t vec;      // strided axis: 16-byte global accesses over 16 / sizeof(T) adjacent columns
   int32_t wrows;    // LDS rows per wave: 1 (max / min of one tile: H in place), 2 (H and G)
   int32_t epad;     // box sum: LDS slots per reflected line (extended position e at e + lead)
@@ -613,11 +628,7 @@ struct VPtrs {
 template <class T, class S, int KT>
 __global__ void __launch_bounds__(512) k_vhgw(VPass a, VPtrs<T> ptr) {
   extern __shared__ __align__(16) unsigned char smem[];
-  int64_t wg = blockIdx.x;
-  if (a.xcd) {
-    const int64_t per = gridDim.x >> 3;
-    wg = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-  }
+  const int64_t wg = a.xcd ? xcd_item(blockIdx.x, a.xcd) : static_cast<int64_t>(blockIdx.x);
   const int64_t b = wg / a.groups;
   const int64_t grp = wg - b * a.groups;
   const int64_t rung = blockIdx.y;
@@ -885,10 +896,9 @@ __global__ void __launch_bounds__(512) k_vhgw(VPass a, VPtrs<T> ptr) {
 // cover, one launch per row layout (PAD: windows over 32 samples; each launch skips the other's
 // items): each workgroup walks its share of the (rung, element, tile) items and reads the next
 // running item's samples into registers (PF 16-byte reads per thread) before it sums the current
-// one, so that the HBM reads of one tile overlap the LDS work of the one before. Workgroup i runs
-// on XCD i mod 8; at each step of the walk an XCD's workgroups take consecutive items, so that the
-// tiles they work on at a moment are neighbours in memory and share its L2, and every XCD works on
-// every rung.
+// one, so that the HBM reads of one tile overlap the LDS work of the one before. At each step of
+// the walk the workgroups take the next gridDim.x items in the XCD rotation (xcd_item): chunks of
+// neighbouring tiles share an XCD's L2, and every XCD works on every rung and element.
 template <class T, int PF, bool PAD>
 __global__ void __launch_bounds__(256) EDA_BOX_ATTR k_box(VPass a, VPtrs<T> ptr, int32_t rungs) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -904,11 +914,10 @@ __global__ void __launch_bounds__(256) EDA_BOX_ATTR k_box(VPass a, VPtrs<T> ptr,
   const float inv_n = 1.0f / static_cast<float>(n);
   const int64_t per_rung = a.g.batch * a.groups;
   const int64_t items = per_rung * rungs;
-  // grid-stride over the items in the workgroup's place among the XCD's workgroups: each step,
-  // XCD x takes the gridDim.x / 8 consecutive items from x gridDim.x / 8 on
+  // grid-stride over the items from the workgroup's place in the XCD rotation
   const int64_t hi = items;
   const int64_t nloc = gridDim.x;
-  const int64_t vid = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+  const int64_t vid = a.xcd ? xcd_item(blockIdx.x, a.xcd) : static_cast<int64_t>(blockIdx.x);
   // strided axis: tpr threads per row, each VW adjacent columns; rows t / tpr + u nthr / tpr
   const int tpr = a.P / VW;
   const int cg = threadIdx.x % tpr;
@@ -939,17 +948,25 @@ __global__ void __launch_bounds__(256) EDA_BOX_ATTR k_box(VPass a, VPtrs<T> ptr,
     }
     return d;
   };
-  // the first running item of this kernel's class at or after it (hi if none); a window above the
-  // launch's largest (the host prevents it) does not run
+  // the first running item of this kernel's class among it, it + nloc, ... (hi if none); a window
+  // above the launch's largest (the host prevents it) does not run. The lanes of a wave test 64
+  // items at once, so that a launch in which few elements run skips the others in few reads.
+  const int lane = threadIdx.x & 63;
   const auto next_running = [&](int64_t it) {
-    for (; it < hi; it += nloc) {
-      const int64_t rung = it / per_rung;
-      const int64_t b = (it - rung * per_rung) / a.groups;
-      const int32_t w = ptr.win[rung * a.g.batch + b];
-      const int32_t rw = min((w - 1) / 2, n - 1);
-      if (w >= 1 && rw <= a.lead - 32 && (rw >= 16) == PAD) break;
+    for (; it < hi; it += 64 * nloc) {
+      const int64_t c = it + lane * nloc;
+      bool run = false;
+      if (c < hi) {
+        const int64_t rung = c / per_rung;
+        const int64_t b = (c - rung * per_rung) / a.groups;
+        const int32_t w = ptr.win[rung * a.g.batch + b];
+        const int32_t rw = min((w - 1) / 2, n - 1);
+        run = w >= 1 && rw <= a.lead - 32 && (rw >= 16) == PAD;
+      }
+      const uint64_t m = __ballot(run);
+      if (m != 0) return it + static_cast<int64_t>(__ffsll(static_cast<unsigned long long>(m)) - 1) * nloc;
     }
-    return it < hi ? it : hi;
+    return hi;
   };
   V16 buf[PF];
   const auto fetch = [&](const Item& d) {
@@ -1389,7 +1406,9 @@ void launch_box(int pf, hipStream_t s, const VPass& a, const VPtrs<T>& p, int32_
     if (lds > 65536)
       EDA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
                                   static_cast<int>(lds)));
-    EDA_GO(k, dim3(static_cast<unsigned>(wgs)), dim3(64 * W), lds, s, a, p, rungs);
+    VPass v = a;
+    v.xcd = xcd_chunk(wgs);
+    EDA_GO(k, dim3(static_cast<unsigned>(wgs)), dim3(64 * W), lds, s, v, p, rungs);
   };
   if (pf == 4) go(k_box<T, 4, PAD>);
   else go(k_box<T, 8, PAD>);
@@ -1487,7 +1506,7 @@ void enqueue_vhgw(hipStream_t s, const Geom& g, int axis, int nf, bool same_inpu
       }
       const int64_t wgs = g.batch * v.groups;
       if (wgs > 0x7fffffff) throw std::invalid_argument("eda: too many workgroups for one pass");
-      v.xcd = (wgs & 7) == 0 ? 1 : 0;
+      v.xcd = xcd_chunk(wgs);
       const auto al16 = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; };
       constexpr int64_t VW = 16 / static_cast<int64_t>(sizeof(T));
       // strided axis: P and the column count multiples of VW; contiguous axis: n a multiple of VW
