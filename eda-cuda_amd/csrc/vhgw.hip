@@ -1180,18 +1180,41 @@ __global__ void __launch_bounds__(256) k_vhgw_rows(RPass a, VPtrs<T> ptr) {
     if (a.two_in) load_field(ptr.in[1] + off, R.v[1], R.tail[1]);
   };
   const int64_t nw = static_cast<int64_t>(gridDim.x) * W;
-  // the windows run two items ahead of the samples, so that a load never waits for its window
-  int64_t it = static_cast<int64_t>(blockIdx.x) * W + wave;
-  int32_t w_cur = it < a.items ? window(it) : 0;
-  int32_t w_nxt = it + nw < a.items ? window(it + nw) : 0;
+  // The wave's items are sb0, sb0 + nw, ...; the lanes read the windows of 64 of them at once
+  // (lane l: item sb + l nw), and the running ones are taken in order from the ballot's mask, so
+  // that the items of elements that do not run cost no reads of their own.
+  int64_t sb = static_cast<int64_t>(blockIdx.x) * W + wave - 64 * nw;
+  int32_t sw = 0;
+  uint64_t mask = 0;
+  const auto next = [&](int64_t& item, int32_t& w) {
+    while (mask == 0) {
+      sb += 64 * nw;
+      if (sb >= a.items) {
+        item = a.items;
+        w = 0;
+        return;
+      }
+      const int64_t c = sb + lane * nw;
+      sw = c < a.items ? window(c) : 0;
+      mask = __ballot(sw >= 1);
+    }
+    const int l = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+    mask &= mask - 1;
+    item = sb + l * nw;
+    w = __builtin_amdgcn_readlane(sw, l);
+  };
+  int64_t it = 0;
+  int32_t w_cur = 0;
+  next(it, w_cur);
   LineRegs<T, KT> cur{};
-  if (w_cur >= 1) load(it, cur);
+  if (it < a.items) load(it, cur);
   while (it < a.items) {
-    const int64_t it2 = it + nw;
-    const int32_t w_far = it2 + nw < a.items ? window(it2 + nw) : 0;
+    int64_t it2 = 0;
+    int32_t w_nxt = 0;
+    next(it2, w_nxt);
     LineRegs<T, KT> nxt{};
-    if (w_nxt >= 1) load(it2, nxt);
-    if (w_cur >= 1) {
+    if (it2 < a.items) load(it2, nxt);
+    {
       const int64_t rb = it / a.lines;
       const int64_t line = it - rb * a.lines;
       const int64_t rung = rb / B;
@@ -1253,7 +1276,6 @@ __global__ void __launch_bounds__(256) k_vhgw_rows(RPass a, VPtrs<T> ptr) {
     }
     cur = nxt;
     w_cur = w_nxt;
-    w_nxt = w_far;
     it = it2;
   }
 }
