@@ -302,31 +302,62 @@ __global__ void __launch_bounds__(kThreads) k_flags_tile(FlagTile a, const T* __
     BMX[p * nb + c] = mx;
   }
   __syncthreads();
+  // the peaks of the tile into a list (one 16-bit entry each: pk << 4 | p << 1 | is_max), packed
+  // per wave by a ballot, so that the side walks run on consecutive lanes
+  uint16_t* PK = reinterpret_cast<uint16_t*>(F + ((P * n + 3) & ~3));
+  __shared__ uint32_t npk;
+  if (threadIdx.x == 0) npk = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  for (int32_t i0 = static_cast<int32_t>(threadIdx.x) - lane; i0 < total; i0 += kThreads) {
+    const int32_t idx = i0 + lane;
+    uint32_t entry = 0;
+    bool peak = false;
+    if (idx < total) {
+      int32_t p = static_cast<int32_t>(static_cast<float>(idx) * inv_n);
+      int32_t s = idx - p * n;
+      if (s < 0) { --p; s += n; } else if (s >= n) { ++p; s -= n; }
+      if (s >= 1 && s <= n - 2) {
+        const T* x = X + p * a.lpad;
+        const T cur = x[s];
+        const T prev = x[s - 1];
+        if (prev < cur || prev > cur) {
+          const int32_t en = run_end(x, 1, n, nb, s, cur, BMN + p * nb, BMX + p * nb, 1);
+          if (en <= n - 2) {
+            const T nxt = x[en + 1];
+            const bool is_max = prev < cur && nxt < cur;
+            const bool is_min = prev > cur && nxt > cur;
+            peak = is_max || is_min;
+            entry = (static_cast<uint32_t>((s + en) / 2) << 4) | (static_cast<uint32_t>(p) << 1) | (is_max ? 1u : 0u);
+          }
+        }
+      }
+    }
+    const uint64_t m = __ballot(peak);
+    if (m != 0) {
+      uint32_t at = 0;
+      if (lane == 0) at = atomicAdd(&npk, static_cast<uint32_t>(__popcll(m)));
+      at = static_cast<uint32_t>(__shfl(static_cast<int>(at), 0));
+      if (peak) PK[at + static_cast<uint32_t>(__popcll(m & ((1ULL << lane) - 1ULL)))] = static_cast<uint16_t>(entry);
+    }
+  }
+  __syncthreads();
   uint64_t count = 0;
   const uint8_t bit = static_cast<uint8_t>(1u << a.axis);
-  for (int32_t idx = threadIdx.x; idx < total; idx += kThreads) {
-    int32_t p = static_cast<int32_t>(static_cast<float>(idx) * inv_n);
-    int32_t s = idx - p * n;
-    if (s < 0) { --p; s += n; } else if (s >= n) { ++p; s -= n; }
-    if (s < 1 || s > n - 2) continue;
+  const uint32_t np = npk;
+  for (uint32_t q = threadIdx.x; q < np; q += kThreads) {
+    const uint32_t e = PK[q];
+    const int32_t pk = static_cast<int32_t>(e >> 4);
+    const int32_t p = static_cast<int32_t>((e >> 1) & 7u);
     const T* x = X + p * a.lpad;
-    const T cur = x[s];
-    const T prev = x[s - 1];
-    if (!(prev < cur || prev > cur)) continue;
     const T* bmin = BMN + p * nb;
     const T* bmax = BMX + p * nb;
-    const int32_t en = run_end(x, 1, n, nb, s, cur, bmin, bmax, 1);
-    if (en > n - 2) continue;
-    const T nxt = x[en + 1];
-    const bool is_max = prev < cur && nxt < cur;
-    const bool is_min = prev > cur && nxt > cur;
-    if (!is_max && !is_min) continue;
-    const int32_t pk = (s + en) / 2;
+    const T cur = x[pk];  // a plateau's samples are equal
     const T tp = tau[gpos(p, pk)];
-    const bool sig = is_max ? side_deep_lds<true, -1>(x, n, nb, pk, cur, tp, bmin, bmax) &&
-                                  side_deep_lds<true, 1>(x, n, nb, pk, cur, tp, bmin, bmax)
-                            : side_deep_lds<false, -1>(x, n, nb, pk, cur, tp, bmin, bmax) &&
-                                  side_deep_lds<false, 1>(x, n, nb, pk, cur, tp, bmin, bmax);
+    const bool sig = (e & 1u) ? side_deep_lds<true, -1>(x, n, nb, pk, cur, tp, bmin, bmax) &&
+                                    side_deep_lds<true, 1>(x, n, nb, pk, cur, tp, bmin, bmax)
+                              : side_deep_lds<false, -1>(x, n, nb, pk, cur, tp, bmin, bmax) &&
+                                    side_deep_lds<false, 1>(x, n, nb, pk, cur, tp, bmin, bmax);
     if (sig) {
       F[fidx(p, pk)] = bit;
       ++count;
@@ -406,9 +437,10 @@ void enqueue_sig_flags(hipStream_t s, const Geom& g, const T* r, const T* tau, u
       } else {
         P = static_cast<int32_t>(std::min<int64_t>(P, t.outer));
       }
+      // lines, block minima and maxima, the flag tile (rounded to 4 bytes), the peak list
       const auto bytes = [&](int32_t q) {
         return (static_cast<size_t>(q) * t.lpad + 2 * static_cast<size_t>(q) * t.nb) * sizeof(T) +
-               static_cast<size_t>(q) * n;
+               (static_cast<size_t>(q) * n + 3) / 4 * 4 + 2 * static_cast<size_t>(q) * n;
       };
       while (P > 1 && bytes(P) > 40000) P /= 2;
       t.P = P;
