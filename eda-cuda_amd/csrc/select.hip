@@ -10,9 +10,13 @@
 // pick kernel (one workgroup per element) finds the bin holding the rank. The median is the mean
 // of the two middle values (numpy.median); the quantile follows numpy.quantile(method='linear'):
 // virtual index (n - 1) q, the lerp that is exact at its end. An empty set gives NaN, count 0.
+// The third pass also keeps the keys that match a rank's 22-bit prefix (the only values the later
+// passes count) in a per-element candidate list; the last three passes count the list instead of
+// reading every value again, unless it overflowed its slots.
 #include "common.hpp"
 #include "kernels.hpp"
 
+#include <algorithm>
 #include <cmath>
 
 namespace eda {
@@ -21,6 +25,17 @@ namespace {
 constexpr int kBits = 11;
 constexpr int kBins = 1 << kBits;
 constexpr int kPasses = 6;
+constexpr int64_t kCandMax = int64_t{1} << 16;       // candidate slots per element at most
+constexpr int64_t kCandTotal = int64_t{1} << 25;     // candidate slots of all elements at most (256 MiB)
+constexpr uint32_t kBlockCand = 1024;                // candidates one workgroup gathers in LDS
+
+// The candidate lists: element b's keys at keys + b slots, their number at n[b] (it counts past
+// slots when the list overflows).
+struct SelCand {
+  uint64_t* keys;
+  uint32_t* n;
+  int64_t slots;
+};
 
 struct alignas(16) SelState {
   uint64_t prefix[2];
@@ -76,18 +91,24 @@ __device__ __forceinline__ void sel_values(const SelArgs<T>& a, int64_t b, uint3
   }
 }
 
-__global__ void k_sel_init(int64_t B, SelState* st, uint64_t* hist, const int32_t* on) {
+__global__ void k_sel_init(int64_t B, SelState* st, uint64_t* hist, uint32_t* ncand, const int32_t* on) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= B * 2 * kBins) return;
   const int64_t b = i / (2 * kBins);
   if (on != nullptr && on[b] == 0) return;
   hist[i] = 0;
-  if (i % (2 * kBins) == 0) st[b] = SelState{};
+  if (i % (2 * kBins) == 0) {
+    st[b] = SelState{};
+    ncand[b] = 0;
+  }
 }
 
-template <class T>
+// phase 0 counts every value; 1 also keeps the values that match a prefix as candidates; 2 counts
+// the candidates when the list holds them all (else every value).
+template <class T, int PHASE>
 __global__ void __launch_bounds__(kThreads) k_sel_hist(SelArgs<T> a, const SelState* st, uint64_t* hist, int32_t nblk,
-                                                       int32_t shift, int32_t width, int32_t first, const int32_t* on) {
+                                                       int32_t shift, int32_t width, int32_t first, const int32_t* on,
+                                                       SelCand cand) {
   __shared__ uint32_t h[2][kBins];
   const int64_t b = blockIdx.x / nblk;
   const int64_t chunk = blockIdx.x % nblk;
@@ -101,18 +122,51 @@ __global__ void __launch_bounds__(kThreads) k_sel_hist(SelArgs<T> a, const SelSt
   const uint64_t dmask = (1ULL << width) - 1ULL;
   const uint32_t stride = static_cast<uint32_t>(nblk) * kThreads;
   const uint32_t count = static_cast<uint32_t>(a.g.count);
-  const auto tally = [&](double v) {
-    const uint64_t k = order_key(v);
+  // counts key k where its higher digits match a rank's prefix; returns whether it matched one
+  const auto tally = [&](uint64_t k) {
     const uint32_t bin = static_cast<uint32_t>((k >> shift) & dmask);
     if (first) {
       atomicAdd(&h[0][bin], 1u);
-    } else {
-      if ((k >> hi) == (S.prefix[0] >> hi)) atomicAdd(&h[0][bin], 1u);
-      if (nr == 2 && (k >> hi) == (S.prefix[1] >> hi)) atomicAdd(&h[1][bin], 1u);
+      return true;
     }
+    const bool m0 = (k >> hi) == (S.prefix[0] >> hi);
+    const bool m1 = nr == 2 && (k >> hi) == (S.prefix[1] >> hi);
+    if (m0) atomicAdd(&h[0][bin], 1u);
+    if (m1) atomicAdd(&h[1][bin], 1u);
+    return m0 || m1;
   };
-  for (uint32_t v = static_cast<uint32_t>(chunk) * kThreads + threadIdx.x; v < count; v += stride)
-    sel_values(a, b, v, tally);
+  const uint32_t nc = PHASE == 2 ? cand.n[b] : 0u;
+  if (PHASE == 2 && nc <= static_cast<uint64_t>(cand.slots)) {
+    const uint64_t* ck = cand.keys + b * cand.slots;
+    for (uint32_t v = static_cast<uint32_t>(chunk) * kThreads + threadIdx.x; v < nc; v += stride) tally(ck[v]);
+  } else if constexpr (PHASE == 1) {
+    // the block's candidates gather in LDS (kBlockCand at most: more mark the list overflowed),
+    // then one reservation in the element's list
+    __shared__ uint64_t lk[kBlockCand];
+    __shared__ uint32_t ln, lat;
+    if (threadIdx.x == 0) ln = 0;
+    __syncthreads();
+    for (uint32_t v = static_cast<uint32_t>(chunk) * kThreads + threadIdx.x; v < count; v += stride)
+      sel_values(a, b, v, [&](double val) {
+        const uint64_t k = order_key(val);
+        if (tally(k)) {
+          const uint32_t at = atomicAdd(&ln, 1u);
+          if (at < kBlockCand) lk[at] = k;
+        }
+      });
+    __syncthreads();
+    const uint32_t nl = ln;
+    if (threadIdx.x == 0 && nl > 0)
+      lat = atomicAdd(&cand.n[b], nl <= kBlockCand ? nl : static_cast<uint32_t>(cand.slots) + 1u);
+    __syncthreads();
+    if (nl <= kBlockCand && lat + nl <= static_cast<uint64_t>(cand.slots)) {
+      uint64_t* ck = cand.keys + b * cand.slots + lat;
+      for (uint32_t i = threadIdx.x; i < nl; i += kThreads) ck[i] = lk[i];
+    }
+  } else {
+    for (uint32_t v = static_cast<uint32_t>(chunk) * kThreads + threadIdx.x; v < count; v += stride)
+      sel_values(a, b, v, [&](double val) { tally(order_key(val)); });
+  }
   __syncthreads();
   const int nb = 1 << width;
   for (int i = threadIdx.x; i < nr * nb; i += kThreads) {
@@ -252,11 +306,25 @@ __global__ void k_sel_final(int64_t B, const SelState* st, int32_t mode, double*
 
 size_t align_up(size_t v) { return (v + 255) / 256 * 256; }
 
+
+// Candidate slots per element (0: the passes read every value): the values of the largest
+// selection (every axis's differences) up to kCandMax, within kCandTotal over the batch.
+int64_t cand_slots(const Geom& g) {
+  const int64_t items = static_cast<int64_t>(g.dim) * g.count;
+  const int64_t c = std::min({items, kCandMax, kCandTotal / std::max<int64_t>(1, g.batch)});
+  return c < 64 ? 0 : c;
+}
+
+// the selection's state, histograms, candidate counts and lists
+size_t sel_state_bytes(const Geom& g) { return align_up(static_cast<size_t>(g.batch) * sizeof(SelState)); }
+size_t sel_hist_bytes(const Geom& g) { return align_up(static_cast<size_t>(g.batch) * 2 * kBins * sizeof(uint64_t)); }
+size_t sel_count_bytes(const Geom& g) { return align_up(static_cast<size_t>(g.batch) * sizeof(uint32_t)); }
+
 }  // namespace
 
 size_t select_ws_bytes(const Geom& g) {
-  return align_up(static_cast<size_t>(g.batch) * sizeof(SelState)) +
-         align_up(static_cast<size_t>(g.batch) * 2 * kBins * sizeof(uint64_t));
+  return sel_state_bytes(g) + sel_hist_bytes(g) + sel_count_bytes(g) +
+         align_up(static_cast<size_t>(g.batch) * static_cast<size_t>(cand_slots(g)) * sizeof(uint64_t));
 }
 size_t noise_ws_bytes(const Geom& g) { return select_ws_bytes(g) + 3 * align_up(static_cast<size_t>(g.batch) * 8); }
 
@@ -275,11 +343,16 @@ void enqueue_select(hipStream_t s, const Geom& g, const Selection<T>& sel, int m
     a.items = static_cast<int64_t>(a.naxes) * g.count;
   }
   const int64_t B = g.batch;
-  SelState* st = static_cast<SelState*>(ws);
-  uint64_t* hist = reinterpret_cast<uint64_t*>(static_cast<char*>(ws) + align_up(static_cast<size_t>(B) * sizeof(SelState)));
+  char* w = static_cast<char*>(ws);
+  SelState* st = reinterpret_cast<SelState*>(w);
+  uint64_t* hist = reinterpret_cast<uint64_t*>(w + sel_state_bytes(g));
+  SelCand cand{};
+  cand.n = reinterpret_cast<uint32_t*>(w + sel_state_bytes(g) + sel_hist_bytes(g));
+  cand.keys = reinterpret_cast<uint64_t*>(w + sel_state_bytes(g) + sel_hist_bytes(g) + sel_count_bytes(g));
+  cand.slots = cand_slots(g);
   const int64_t ni = B * 2 * kBins;
   EDA_GO(k_sel_init, dim3(static_cast<unsigned>((ni + kThreads - 1) / kThreads)), dim3(kThreads), 0, s, B,
-                     st, hist, on);
+                     st, hist, cand.n, on);
   EDA_LAUNCH("k_sel_init");
   int64_t nblk = (g.count + int64_t{kThreads} * 32 - 1) / (int64_t{kThreads} * 32);  // blocks per element
   nblk = nblk < 1 ? 1 : (nblk > 512 ? 512 : nblk);
@@ -288,8 +361,18 @@ void enqueue_select(hipStream_t s, const Geom& g, const Selection<T>& sel, int m
       const int top = 64 - kBits * p;
       const int shift = top - kBits > 0 ? top - kBits : 0;
       const int width = top - shift;
-      EDA_GO(k_sel_hist<T>, dim3(static_cast<unsigned>(nblk * B)), dim3(kThreads), 0, s, a, st, hist,
-                         static_cast<int32_t>(nblk), shift, width, p == 0 ? 1 : 0, on);
+      const int phase = cand.slots == 0 ? 0 : p == 2 ? 1 : p > 2 ? 2 : 0;
+      const dim3 grid(static_cast<unsigned>(nblk * B));
+      const int32_t fst = p == 0 ? 1 : 0;
+      if (phase == 1)
+        EDA_GO((k_sel_hist<T, 1>), grid, dim3(kThreads), 0, s, a, st, hist, static_cast<int32_t>(nblk), shift,
+               width, fst, on, cand);
+      else if (phase == 2)
+        EDA_GO((k_sel_hist<T, 2>), grid, dim3(kThreads), 0, s, a, st, hist, static_cast<int32_t>(nblk), shift,
+               width, fst, on, cand);
+      else
+        EDA_GO((k_sel_hist<T, 0>), grid, dim3(kThreads), 0, s, a, st, hist, static_cast<int32_t>(nblk), shift,
+               width, fst, on, cand);
       EDA_LAUNCH("k_sel_hist");
       EDA_GO(k_sel_pick, dim3(static_cast<unsigned>(B)), dim3(kThreads), 0, s, st, hist, shift, width,
                          p == 0 ? 1 : 0, mode, q, on);
