@@ -1148,16 +1148,26 @@ bool Chunk<T>::mode() {
       }
     }
     bool any_acc = false;
+    std::vector<int32_t> acc_src(B, 0);  // 1 + the mask buffer of an accepted population, 0: none
     for (int64_t b = 0; b < B; ++b) {
       if (rec_[b].status != 0 || !mach[b].accepted) continue;
       Rec& e = rec_[b];
       e.fine = true;
       e.p_odd = odd_window(mach[b].cand.p);
-      EDA_HIP(hipMemcpyAsync(Pacc_ + b * g.count, P_[mach[b].acc_index & 1] + b * g.count, g.count,
-                             hipMemcpyDeviceToDevice, s_));
+      acc_src[b] = 1 + (mach[b].acc_index & 1);
       any_acc = true;
     }
     if (any_acc) {
+      // the accepted populations into Pacc in one launch
+      const int32_t* dsrc = ctx_.up.put(acc_src);
+      uint8_t* const pacc = Pacc_;
+      const uint8_t* const p0 = P_[0];
+      const uint8_t* const p1 = P_[1];
+      const int64_t cnt = g.count;
+      each(s_, g, dsrc, [=] __device__(int64_t b, int64_t i) {
+        const int64_t at = b * cnt + i;
+        pacc[at] = dsrc[b] == 2 ? p1[at] : p0[at];
+      });
       // G3's coarse scale (reference stats_rungs.cu:90-121,161-187)
       std::vector<int32_t> wc(B, 0), wrr(B, 0), fon(B, 0);
       int32_t mwc = 0, mwr = 0;

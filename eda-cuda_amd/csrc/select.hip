@@ -195,10 +195,33 @@ __device__ __forceinline__ void quantile_index(int64_t n, double q, int64_t* i, 
   }
 }
 
+// The exclusive prefix sum of v over the workgroup's threads in thread order, and the total
+// (integers: exact in any order). wsum: kThreads / 64 LDS words.
+__device__ __forceinline__ uint64_t block_exclusive(uint64_t v, uint64_t* total, uint64_t* wsum) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  unsigned long x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const unsigned long y = __shfl_up(x, d);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63) wsum[wave] = x;
+  __syncthreads();
+  uint64_t off = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kThreads / 64; ++w) {
+    const uint64_t t = wsum[w];
+    off += w < wave ? t : 0;
+    tot += t;
+  }
+  __syncthreads();
+  *total = tot;
+  return off + x - v;
+}
+
 __global__ void __launch_bounds__(kThreads) k_sel_pick(SelState* st, uint64_t* hist, int32_t shift, int32_t width,
                                                        int32_t first, int32_t mode, double q, const int32_t* on) {
-  __shared__ uint64_t part[kThreads];
-  __shared__ uint64_t base[kThreads];
+  __shared__ uint64_t wsum[kThreads / 64];
   __shared__ SelState S;
   __shared__ int32_t found_bin;
   __shared__ uint64_t found_rank;
@@ -214,11 +237,9 @@ __global__ void __launch_bounds__(kThreads) k_sel_pick(SelState* st, uint64_t* h
   if (first) {
     uint64_t sum = 0;
     for (int j = 0; j < per; ++j) sum += h0[tid * per + j];
-    part[tid] = sum;
-    __syncthreads();
+    uint64_t n = 0;
+    block_exclusive(sum, &n, wsum);
     if (tid == 0) {
-      uint64_t n = 0;
-      for (int t = 0; t < kThreads; ++t) n += part[t];
       S.count = static_cast<int64_t>(n);
       S.valid = n > 0;
       S.prefix[0] = S.prefix[1] = 0;
@@ -247,19 +268,11 @@ __global__ void __launch_bounds__(kThreads) k_sel_pick(SelState* st, uint64_t* h
     const uint64_t* hr = hist + (b * 2 + (S.same ? 0 : r)) * kBins;
     uint64_t sum = 0;
     for (int j = 0; j < per; ++j) sum += hr[tid * per + j];
-    part[tid] = sum;
-    __syncthreads();
-    if (tid == 0) {
-      uint64_t acc = 0;
-      for (int t = 0; t < kThreads; ++t) {
-        base[t] = acc;
-        acc += part[t];
-      }
-    }
-    __syncthreads();
+    uint64_t tot = 0;
+    const uint64_t base = block_exclusive(sum, &tot, wsum);
     const uint64_t rank = S.rank[r];
-    if (rank >= base[tid] && rank < base[tid] + part[tid]) {
-      uint64_t acc = base[tid];
+    if (rank >= base && rank < base + sum) {
+      uint64_t acc = base;
       for (int j = 0; j < per; ++j) {
         const uint64_t c = hr[tid * per + j];
         if (rank < acc + c) {
