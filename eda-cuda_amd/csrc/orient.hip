@@ -18,6 +18,7 @@
 #include "common.hpp"
 #include "kernels.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <map>
 #include <mutex>
@@ -1180,19 +1181,43 @@ void tensor_eigen(hipStream_t s, const Geom& g, const T* R1, T* R2, const int32_
   enqueue_eigen<T>(s, g, R2, on, nvec, coh, nonfinite);
 }
 
+// Y = T(m X) for a kind of riesz_mult: the multiplier depends on the sample's position only, so
+// each thread computes it once for a position and applies it to that position of every running
+// element (kU elements' reads in flight at a time).
+template <class T>
+__global__ void __launch_bounds__(kThreads) k_riesz_mult(Geom g, const T* __restrict__ X, T* __restrict__ Y, int kind,
+                                                         int a, int bb, const int32_t* __restrict__ on) {
+  constexpr int kU = 8;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(kThreads) + threadIdx.x; i < g.count;
+       i += static_cast<int64_t>(gridDim.x) * kThreads) {
+    int32_t k[kMaxDim];
+    double u[kMaxDim];
+    const double rad = dct_freq(g, i, k, u);
+    const double m = riesz_mult(kind, a, bb, u, rad);
+    for (int64_t b0 = 0; b0 < g.batch; b0 += kU) {
+      T x[kU];
+#pragma unroll
+      for (int j = 0; j < kU; ++j) {
+        const int64_t b = b0 + j;
+        x[j] = b < g.batch && (on == nullptr || on[b] != 0) ? X[b * g.count + i] : T(0);
+      }
+#pragma unroll
+      for (int j = 0; j < kU; ++j) {
+        const int64_t b = b0 + j;
+        if (b < g.batch && (on == nullptr || on[b] != 0)) Y[b * g.count + i] = static_cast<T>(m * static_cast<double>(x[j]));
+      }
+    }
+  }
+}
+
 // One component of X (a kind of riesz_mult) into dst: Y = T(m X), then the inverse transform with
 // the sin basis on the component's odd axes (T1: the transform's scratch field).
 template <class T>
 void riesz_component(hipStream_t s, int device, const Geom& g, const T* X, int kind, int a, int bb, T* Y, T* T1,
                      T* dst, const int32_t* on) {
-  each(s, g, on, [=] __device__(int64_t b, int64_t i) {
-    int32_t k[kMaxDim];
-    double u[kMaxDim];
-    const double rad = dct_freq(g, i, k, u);
-    const double m = riesz_mult(kind, a, bb, u, rad);
-    const int64_t at = b * g.count + i;
-    Y[at] = static_cast<T>(m * static_cast<double>(X[at]));
-  });
+  const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((g.count + kThreads - 1) / kThreads, 65535));
+  EDA_GO(k_riesz_mult<T>, dim3(static_cast<unsigned>(blocks)), dim3(kThreads), 0, s, g, X, Y, kind, a, bb, on);
+  EDA_LAUNCH("k_riesz_mult");
   enqueue_dct<T>(s, device, g, Y, dst, T1, 1, riesz_odd(kind, a, bb));
 }
 
