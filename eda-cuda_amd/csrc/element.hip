@@ -154,6 +154,21 @@ __device__ __forceinline__ int32_t reflect32(int32_t i, int32_t n) {
   return m < n ? m : p - 1 - m;
 }
 
+// Two and four consecutive values read by one load at the alignment of one value (two): the
+// hardware serves 8- and 16-byte global loads at 4-byte alignment.
+template <class T>
+struct Wide;
+template <>
+struct Wide<float> {
+  typedef float two __attribute__((ext_vector_type(2), aligned(4)));
+  typedef float four __attribute__((ext_vector_type(4), aligned(8)));
+};
+template <>
+struct Wide<double> {
+  typedef double two __attribute__((ext_vector_type(2), aligned(8)));
+  typedef double four __attribute__((ext_vector_type(4), aligned(16)));
+};
+
 // D-linear interpolation at y of the half-sample reflected periodic extension of a field of one
 // element: the corner offsets and weights once (setup), then any field with that layout (at).
 template <class T, int D>
@@ -174,44 +189,69 @@ struct Interp {
       off1[a] = reflect32(i0 + 1, n[a]) * st[a];
     }
   }
+  // Corners 2k and 2k + 1 differ in the last axis only: where its two samples are adjacent (inside
+  // the axis; the last axis is contiguous) one read covers both, else two. Every corner's term and
+  // the sums keep the corner order.
+  __device__ __forceinline__ bool adjacent() const { return off1[D - 1] == off0[D - 1] + 1; }
+  // the offset of corner 2k's sample (the last axis at its lower sample)
+  __device__ __forceinline__ int32_t pair_off(int c2) const {
+    int32_t off = off0[D - 1];
+#pragma unroll
+    for (int a = 0; a < D - 1; ++a) off += ((c2 >> (D - 1 - a)) & 1) ? off1[a] : off0[a];
+    return off;
+  }
+  __device__ __forceinline__ T weigh(int corner, T v) const {
+#pragma unroll
+    for (int a = 0; a < D; ++a) v = v * (((corner >> (D - 1 - a)) & 1) ? w1[a] : w0[a]);
+    return v;
+  }
   // two fields stored side by side (field k of sample v at f + 2 v + k), each interpolated as at()
-  // does, from one aligned read per corner
+  // does
   __device__ __forceinline__ void at2(const T* __restrict__ f, T& r0, T& r1) const {
-    struct alignas(2 * sizeof(T)) Pair {
-      T a, b;
-    };
-    const Pair* pf = reinterpret_cast<const Pair*>(f);
+    using Four = typename Wide<T>::four;
+    using Two = typename Wide<T>::two;
+    const bool adj = adjacent();
+    const int32_t step = off1[D - 1] - off0[D - 1];
     T acc0 = T(0), acc1 = T(0);
 #pragma unroll
-    for (int corner = 0; corner < (1 << D); ++corner) {
-      int32_t off = 0;
-#pragma unroll
-      for (int a = 0; a < D; ++a) off += ((corner >> (D - 1 - a)) & 1) ? off1[a] : off0[a];
-      const Pair q = pf[off];
-      T term0 = q.a, term1 = q.b;
-#pragma unroll
-      for (int a = 0; a < D; ++a) {
-        const T wa = ((corner >> (D - 1 - a)) & 1) ? w1[a] : w0[a];
-        term0 = term0 * wa;
-        term1 = term1 * wa;
+    for (int c2 = 0; c2 < (1 << D); c2 += 2) {
+      const int32_t off = pair_off(c2);
+      Four q;
+      if (adj) {
+        q = *reinterpret_cast<const Four*>(f + 2 * static_cast<int64_t>(off));
+      } else {
+        const Two lo = *reinterpret_cast<const Two*>(f + 2 * static_cast<int64_t>(off));
+        const Two hi = *reinterpret_cast<const Two*>(f + 2 * static_cast<int64_t>(off + step));
+        q.x = lo.x;
+        q.y = lo.y;
+        q.z = hi.x;
+        q.w = hi.y;
       }
-      acc0 = acc0 + term0;
-      acc1 = acc1 + term1;
+      acc0 = acc0 + weigh(c2, q.x);
+      acc1 = acc1 + weigh(c2, q.y);
+      acc0 = acc0 + weigh(c2 + 1, q.z);
+      acc1 = acc1 + weigh(c2 + 1, q.w);
     }
     r0 = acc0;
     r1 = acc1;
   }
   __device__ __forceinline__ T at(const T* __restrict__ f) const {
+    using Two = typename Wide<T>::two;
+    const bool adj = adjacent();
+    const int32_t step = off1[D - 1] - off0[D - 1];
     T acc = T(0);
 #pragma unroll
-    for (int corner = 0; corner < (1 << D); ++corner) {
-      int32_t off = 0;
-#pragma unroll
-      for (int a = 0; a < D; ++a) off += ((corner >> (D - 1 - a)) & 1) ? off1[a] : off0[a];
-      T term = f[off];
-#pragma unroll
-      for (int a = 0; a < D; ++a) term = term * (((corner >> (D - 1 - a)) & 1) ? w1[a] : w0[a]);
-      acc = acc + term;
+    for (int c2 = 0; c2 < (1 << D); c2 += 2) {
+      const int32_t off = pair_off(c2);
+      Two q;
+      if (adj) {
+        q = *reinterpret_cast<const Two*>(f + off);
+      } else {
+        q.x = f[off];
+        q.y = f[off + step];
+      }
+      acc = acc + weigh(c2, q.x);
+      acc = acc + weigh(c2 + 1, q.y);
     }
     return acc;
   }
