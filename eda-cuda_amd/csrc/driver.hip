@@ -809,18 +809,14 @@ class Chunk {
                  const double* tol, const double* floor2, T* tau, T* tmp);
   void raw_window(const uint8_t* flags, const int32_t* win, int32_t maxw, const int32_t* on, T* om, T* tmp);
   void envelope_mean(const T* h, T* m, T* u, T* l) {
-    const Geom g = g_;
-    elem_op(2, h, u, l);
+    // the envelopes' mean 0.5 (u + l) comes out of the max / min gather
     T* mid = X_[3];
-    const int32_t* on = on_op_;
-    each(s_, g, on, [=] __device__(int64_t b, int64_t i) {
-      const int64_t at = b * g.count + i;
-      mid[at] = T(0.5) * (u[at] + l[at]);
-    });
+    elem_op(2, h, u, l, mid);
     elem_op(3, mid, m, nullptr);
   }
-  // An operator of the element over every rung (the rung scratch grows to the mode's ladder).
-  void elem_op(int op, const T* in, T* out0, T* out1) {
+  // An operator of the element over every rung (the rung scratch grows to the mode's ladder); mid
+  // (op 2): the mean of the two outputs as well.
+  void elem_op(int op, const T* in, T* out0, T* out1, T* mid = nullptr) {
     // the rung fields: r0, r1 (K_max fields each), then the max and min side by side (2 K_max)
     const size_t need = static_cast<size_t>(K_max_) * static_cast<size_t>(B_ * g_.count);
     if (need > rung_cap_) {
@@ -833,7 +829,7 @@ class Chunk {
     T* r1 = r0 + rung_cap_;
     T* pairs = r1 + rung_cap_;
     enqueue_element_op<T>(s_, g_, op, d_el_, lo_, t_, o_, d_rung_win_, rung_max_.data(), K_max_, in, out0, out1, r0, r1,
-                          pairs, on_op_);
+                          pairs, on_op_, mid);
   }
 
   Context& ctx_;

@@ -290,8 +290,9 @@ __device__ __forceinline__ void rung_acc(int32_t i, const T* __restrict__ f0, co
 template <class T, int D>
 __device__ __forceinline__ void rung_acc_pairs(int32_t i, const T* __restrict__ pr, bool pts,
                                                const Interp<T, D> (&ip)[4], T* a0, T* a1) {
-  T acc0 = pr[2 * i];
-  T acc1 = pr[2 * i + 1];
+  const typename Wide<T>::two q0 = *reinterpret_cast<const typename Wide<T>::two*>(pr + 2 * static_cast<int64_t>(i));
+  T acc0 = q0.x;
+  T acc1 = q0.y;
   if (pts) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -311,7 +312,8 @@ __global__ void __launch_bounds__(kThreads) k_gather_all(Geom g, const ElemDev* 
                                                          const uint8_t* __restrict__ lo, const T* __restrict__ t,
                                                          const T* __restrict__ o, const T* __restrict__ r0,
                                                          const T* __restrict__ r1, T* __restrict__ out0,
-                                                         T* __restrict__ out1, const int32_t* __restrict__ on) {
+                                                         T* __restrict__ out1, const int32_t* __restrict__ on,
+                                                         T* __restrict__ mid) {
   const int64_t b = blockIdx.y;
   if (on != nullptr && on[b] == 0) return;
   const uint32_t pts = D >= 2 ? el[b].pts : 0u;
@@ -388,27 +390,38 @@ __global__ void __launch_bounds__(kThreads) k_gather_all(Geom g, const ElemDev* 
       v1 = use_lo ? v1 + u1 : u1;
     }
     out0[at] = v0;
-    if constexpr (OP == 2) out1[at] = v1;
+    if constexpr (OP == 2) {
+      out1[at] = v1;
+      if (mid != nullptr) mid[at] = T(0.5) * (v0 + v1);
+    }
   }
 }
 
+// mid (op 2, optional): (out0 + out1) / 2 as well
 template <class T, int OP>
 void launch_gather_all(hipStream_t s, const Geom& g, const ElemDev* el, const uint8_t* lo, const T* t, const T* o,
-                       const T* r0, const T* r1, T* out0, T* out1, const int32_t* on) {
+                       const T* r0, const T* r1, T* out0, T* out1, const int32_t* on, T* mid = nullptr) {
   const dim3 grid = elem_grid(g);
-  if (g.dim == 1) EDA_GO((k_gather_all<T, 1, OP>), grid, dim3(kThreads), 0, s, g, el, lo, t, o, r0, r1, out0, out1, on);
-  else if (g.dim == 2) EDA_GO((k_gather_all<T, 2, OP>), grid, dim3(kThreads), 0, s, g, el, lo, t, o, r0, r1, out0, out1, on);
-  else EDA_GO((k_gather_all<T, 3, OP>), grid, dim3(kThreads), 0, s, g, el, lo, t, o, r0, r1, out0, out1, on);
+  if (g.dim == 1)
+    EDA_GO((k_gather_all<T, 1, OP>), grid, dim3(kThreads), 0, s, g, el, lo, t, o, r0, r1, out0, out1, on, mid);
+  else if (g.dim == 2)
+    EDA_GO((k_gather_all<T, 2, OP>), grid, dim3(kThreads), 0, s, g, el, lo, t, o, r0, r1, out0, out1, on, mid);
+  else
+    EDA_GO((k_gather_all<T, 3, OP>), grid, dim3(kThreads), 0, s, g, el, lo, t, o, r0, r1, out0, out1, on, mid);
   EDA_LAUNCH("k_gather_all");
 }
 
 // op 2 from the max and min side by side (D >= 2)
 template <class T>
 void launch_gather_pairs(hipStream_t s, const Geom& g, const ElemDev* el, const uint8_t* lo, const T* t, const T* o,
-                         const T* pairs, T* out0, T* out1, const int32_t* on) {
+                         const T* pairs, T* out0, T* out1, const int32_t* on, T* mid) {
   const dim3 grid = elem_grid(g);
-  if (g.dim == 2) EDA_GO((k_gather_all<T, 2, 2, true>), grid, dim3(kThreads), 0, s, g, el, lo, t, o, pairs, nullptr, out0, out1, on);
-  else EDA_GO((k_gather_all<T, 3, 2, true>), grid, dim3(kThreads), 0, s, g, el, lo, t, o, pairs, nullptr, out0, out1, on);
+  if (g.dim == 2)
+    EDA_GO((k_gather_all<T, 2, 2, true>), grid, dim3(kThreads), 0, s, g, el, lo, t, o, pairs, nullptr, out0, out1, on,
+           mid);
+  else
+    EDA_GO((k_gather_all<T, 3, 2, true>), grid, dim3(kThreads), 0, s, g, el, lo, t, o, pairs, nullptr, out0, out1, on,
+           mid);
   EDA_LAUNCH("k_gather_all");
 }
 
@@ -433,7 +446,7 @@ void enqueue_element_weights(hipStream_t s, const Geom& g, const MethodCtl& c, c
 template <class T>
 void enqueue_element_op(hipStream_t s, const Geom& g, int op, const ElemDev* el, const uint8_t* lo, const T* t,
                         const T* o, const int32_t* rung_win, const int32_t* rung_max, int K_max, const T* in, T* out0,
-                        T* out1, T* rungs0, T* rungs1, T* pairs, const int32_t* on) {
+                        T* out1, T* rungs0, T* rungs1, T* pairs, const int32_t* on, T* mid) {
   if (K_max < 1) return;
   int32_t mw = 0;
   for (int j = 0; j < K_max; ++j) mw = std::max(mw, rung_max[j]);
@@ -443,9 +456,9 @@ void enqueue_element_op(hipStream_t s, const Geom& g, int op, const ElemDev* el,
     launch_gather_all<T, 3>(s, g, el, lo, t, o, rungs0, nullptr, out0, nullptr, on);
   } else if (op == 2) {
     if (enqueue_cube_rungs<T>(s, g, CubeOp::kMaxMin, in, rungs0, rungs1, rung_win, K_max, F, mw, pairs))
-      launch_gather_pairs<T>(s, g, el, lo, t, o, pairs, out0, out1, on);
+      launch_gather_pairs<T>(s, g, el, lo, t, o, pairs, out0, out1, on, mid);
     else
-      launch_gather_all<T, 2>(s, g, el, lo, t, o, rungs0, rungs1, out0, out1, on);
+      launch_gather_all<T, 2>(s, g, el, lo, t, o, rungs0, rungs1, out0, out1, on, mid);
   } else {
     enqueue_cube_rungs<T>(s, g, CubeOp::kMax, in, rungs0, nullptr, rung_win, K_max, F, mw);
     launch_gather_all<T, 0>(s, g, el, lo, t, o, rungs0, nullptr, out0, nullptr, on);
@@ -462,9 +475,9 @@ template void enqueue_element_weights<double>(hipStream_t, const Geom&, const Me
                                               const ElemDev*, uint8_t*, double*, const int32_t*, uint32_t*);
 template void enqueue_element_op<float>(hipStream_t, const Geom&, int, const ElemDev*, const uint8_t*, const float*,
                                         const float*, const int32_t*, const int32_t*, int, const float*, float*, float*,
-                                        float*, float*, float*, const int32_t*);
+                                        float*, float*, float*, const int32_t*, float*);
 template void enqueue_element_op<double>(hipStream_t, const Geom&, int, const ElemDev*, const uint8_t*, const double*,
                                          const double*, const int32_t*, const int32_t*, int, const double*, double*,
-                                         double*, double*, double*, double*, const int32_t*);
+                                         double*, double*, double*, double*, const int32_t*, double*);
 
 }  // namespace eda

@@ -168,6 +168,6 @@ void enqueue_element_weights(hipStream_t s, const Geom& g, const MethodCtl& c, c
 template <class T>
 void enqueue_element_op(hipStream_t s, const Geom& g, int op, const ElemDev* el, const uint8_t* lo, const T* t,
                         const T* o, const int32_t* rung_win, const int32_t* rung_max, int K_max, const T* in, T* out0,
-                        T* out1, T* rungs0, T* rungs1, T* pairs, const int32_t* on);
+                        T* out1, T* rungs0, T* rungs1, T* pairs, const int32_t* on, T* mid = nullptr);
 
 }  // namespace eda
