@@ -808,15 +808,22 @@ class Chunk {
   void tau_field(const int32_t* on, const int32_t* dWd, int32_t mWd, const int32_t* dWl, int32_t mWl,
                  const double* tol, const double* floor2, T* tau, T* tmp);
   void raw_window(const uint8_t* flags, const int32_t* win, int32_t maxw, const int32_t* on, T* om, T* tmp);
-  void envelope_mean(const T* h, T* m, T* u, T* l) {
-    // the envelopes' mean 0.5 (u + l) comes out of the max / min gather
-    T* mid = X_[3];
-    elem_op(2, h, u, l, mid);
-    elem_op(3, mid, m, nullptr);
+  // The envelopes u, l of h and the box of their mean 0.5 (u + l) into m (the mean comes out of
+  // the max / min gather); optionally also hd = 0.5 (u - l), R = max(u - l, 0) and d = h - m.
+  void envelope_mean(const T* h, T* m, T* u, T* l, T* hd = nullptr, T* R = nullptr, T* d = nullptr) {
+    GatherExtra<T> mm;
+    mm.mid = X_[3];
+    mm.hdiff = hd;
+    mm.pdiff = R;
+    elem_op(2, h, u, l, mm);
+    GatherExtra<T> bx;
+    bx.minuend = h;
+    bx.diff = d;
+    elem_op(3, mm.mid, m, nullptr, bx);
   }
-  // An operator of the element over every rung (the rung scratch grows to the mode's ladder); mid
-  // (op 2): the mean of the two outputs as well.
-  void elem_op(int op, const T* in, T* out0, T* out1, T* mid = nullptr) {
+  // An operator of the element over every rung (the rung scratch grows to the mode's ladder), with
+  // the gather's extra fields.
+  void elem_op(int op, const T* in, T* out0, T* out1, const GatherExtra<T>& extra = {}) {
     // the rung fields: r0, r1 (K_max fields each), then the max and min side by side (2 K_max)
     const size_t need = static_cast<size_t>(K_max_) * static_cast<size_t>(B_ * g_.count);
     if (need > rung_cap_) {
@@ -829,7 +836,7 @@ class Chunk {
     T* r1 = r0 + rung_cap_;
     T* pairs = r1 + rung_cap_;
     enqueue_element_op<T>(s_, g_, op, d_el_, lo_, t_, o_, d_rung_win_, rung_max_.data(), K_max_, in, out0, out1, r0, r1,
-                          pairs, on_op_, mid);
+                          pairs, on_op_, extra);
   }
 
   Context& ctx_;
@@ -1453,27 +1460,14 @@ bool Chunk<T>::mode() {
   T* const* X = X_;
   T* r = r_;
   T* h = h_;
-  envelope_mean(r_, X[0], X[1], X[2]);
-  {
-    T* m = X[0];
-    each(s_, g, on, [=] __device__(int64_t b, int64_t i) {
-      const int64_t at = b * g.count + i;
-      h[at] = r[at] - m[at];
-    });
-  }
+  // h = r - m, and R = max(u - l, 0) for the continuation weight, out of the gathers
+  envelope_mean(r_, X[0], X[1], X[2], nullptr, c.sifts > 1 ? X[5] : nullptr, h);
   for (Rec& e : rec_)
     if (e.status == 0) e.n_sifts = 1;
   T* a = X[8];
   const double* dtol = upload_f64([](const Rec& e) { return e.tol; }, active);
   if (c.sifts > 1) {
     T* R = X[5];
-    T* u = X[1];
-    T* l = X[2];
-    each(s_, g, on, [=] __device__(int64_t b, int64_t i) {
-      const int64_t at = b * g.count + i;
-      const T dv = u[at] - l[at];
-      R[at] = dv >= T(0) ? dv : T(0);
-    });
     T* U = R;
     T* pp[2] = {X[6], X[7]};
     for (int32_t it = 0; it < c.u; ++it) {
@@ -1529,12 +1523,8 @@ bool Chunk<T>::mode() {
     T* plain = X[0];
     T* up = X[1];
     T* lo = X[2];
-    envelope_mean(h, plain, up, lo);
     T* hd = X[5];
-    each(s_, g, son, [=] __device__(int64_t b, int64_t i) {
-      const int64_t at = b * g.count + i;
-      hd[at] = T(0.5) * (up[at] - lo[at]);
-    });
+    envelope_mean(h, plain, up, lo, hd);
     T* e0 = X[6];
     elem_op(3, hd, e0, nullptr);
     int32_t mWs;
@@ -1570,6 +1560,7 @@ bool Chunk<T>::mode() {
       EDA_HIP(hipMemcpyAsync(e, e0, sizeof(T) * B * g.count, hipMemcpyDeviceToDevice, s_));
     }
     for (int32_t it = 0; it < c.anorm_smooth; ++it) enqueue_cube<T>(s_, g_, CubeOp::kBox, it ? e : e0, e, nullptr, dWs, mWs);
+    T* q = X[6];
     {
       Selection<T> sel{};
       sel.src = SelSrc::kField;
@@ -1584,14 +1575,11 @@ bool Chunk<T>::mode() {
         thr = thr >= tl ? thr : tl;
         thr = thr >= tiny ? thr : tiny;
         const T ev = e[at];
-        e[at] = static_cast<double>(ev) >= thr ? ev : static_cast<T>(thr);
+        const T ef = static_cast<double>(ev) >= thr ? ev : static_cast<T>(thr);
+        e[at] = ef;
+        q[at] = h[at] / ef;
       });
     }
-    T* q = X[6];
-    each(s_, g, son, [=] __device__(int64_t b, int64_t i) {
-      const int64_t at = b * g.count + i;
-      q[at] = h[at] / e[at];
-    });
     T* Mn = X[5];
     envelope_mean(q, Mn, X[10], X[11]);
     each(s_, g, son, [=] __device__(int64_t b, int64_t i) {

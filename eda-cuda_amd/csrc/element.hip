@@ -313,7 +313,7 @@ __global__ void __launch_bounds__(kThreads) k_gather_all(Geom g, const ElemDev* 
                                                          const T* __restrict__ o, const T* __restrict__ r0,
                                                          const T* __restrict__ r1, T* __restrict__ out0,
                                                          T* __restrict__ out1, const int32_t* __restrict__ on,
-                                                         T* __restrict__ mid) {
+                                                         GatherExtra<T> ex) {
   const int64_t b = blockIdx.y;
   if (on != nullptr && on[b] == 0) return;
   const uint32_t pts = D >= 2 ? el[b].pts : 0u;
@@ -392,36 +392,43 @@ __global__ void __launch_bounds__(kThreads) k_gather_all(Geom g, const ElemDev* 
     out0[at] = v0;
     if constexpr (OP == 2) {
       out1[at] = v1;
-      if (mid != nullptr) mid[at] = T(0.5) * (v0 + v1);
+      if (ex.mid != nullptr) ex.mid[at] = T(0.5) * (v0 + v1);
+      if (ex.hdiff != nullptr) ex.hdiff[at] = T(0.5) * (v0 - v1);
+      if (ex.pdiff != nullptr) {
+        const T dv = v0 - v1;
+        ex.pdiff[at] = dv >= T(0) ? dv : T(0);
+      }
+    }
+    if constexpr (OP == 3) {
+      if (ex.diff != nullptr) ex.diff[at] = ex.minuend[at] - v0;
     }
   }
 }
 
-// mid (op 2, optional): (out0 + out1) / 2 as well
 template <class T, int OP>
 void launch_gather_all(hipStream_t s, const Geom& g, const ElemDev* el, const uint8_t* lo, const T* t, const T* o,
-                       const T* r0, const T* r1, T* out0, T* out1, const int32_t* on, T* mid = nullptr) {
+                       const T* r0, const T* r1, T* out0, T* out1, const int32_t* on, const GatherExtra<T>& ex) {
   const dim3 grid = elem_grid(g);
   if (g.dim == 1)
-    EDA_GO((k_gather_all<T, 1, OP>), grid, dim3(kThreads), 0, s, g, el, lo, t, o, r0, r1, out0, out1, on, mid);
+    EDA_GO((k_gather_all<T, 1, OP>), grid, dim3(kThreads), 0, s, g, el, lo, t, o, r0, r1, out0, out1, on, ex);
   else if (g.dim == 2)
-    EDA_GO((k_gather_all<T, 2, OP>), grid, dim3(kThreads), 0, s, g, el, lo, t, o, r0, r1, out0, out1, on, mid);
+    EDA_GO((k_gather_all<T, 2, OP>), grid, dim3(kThreads), 0, s, g, el, lo, t, o, r0, r1, out0, out1, on, ex);
   else
-    EDA_GO((k_gather_all<T, 3, OP>), grid, dim3(kThreads), 0, s, g, el, lo, t, o, r0, r1, out0, out1, on, mid);
+    EDA_GO((k_gather_all<T, 3, OP>), grid, dim3(kThreads), 0, s, g, el, lo, t, o, r0, r1, out0, out1, on, ex);
   EDA_LAUNCH("k_gather_all");
 }
 
 // op 2 from the max and min side by side (D >= 2)
 template <class T>
 void launch_gather_pairs(hipStream_t s, const Geom& g, const ElemDev* el, const uint8_t* lo, const T* t, const T* o,
-                         const T* pairs, T* out0, T* out1, const int32_t* on, T* mid) {
+                         const T* pairs, T* out0, T* out1, const int32_t* on, const GatherExtra<T>& ex) {
   const dim3 grid = elem_grid(g);
   if (g.dim == 2)
     EDA_GO((k_gather_all<T, 2, 2, true>), grid, dim3(kThreads), 0, s, g, el, lo, t, o, pairs, nullptr, out0, out1, on,
-           mid);
+           ex);
   else
     EDA_GO((k_gather_all<T, 3, 2, true>), grid, dim3(kThreads), 0, s, g, el, lo, t, o, pairs, nullptr, out0, out1, on,
-           mid);
+           ex);
   EDA_LAUNCH("k_gather_all");
 }
 
@@ -446,22 +453,22 @@ void enqueue_element_weights(hipStream_t s, const Geom& g, const MethodCtl& c, c
 template <class T>
 void enqueue_element_op(hipStream_t s, const Geom& g, int op, const ElemDev* el, const uint8_t* lo, const T* t,
                         const T* o, const int32_t* rung_win, const int32_t* rung_max, int K_max, const T* in, T* out0,
-                        T* out1, T* rungs0, T* rungs1, T* pairs, const int32_t* on, T* mid) {
+                        T* out1, T* rungs0, T* rungs1, T* pairs, const int32_t* on, GatherExtra<T> extra) {
   if (K_max < 1) return;
   int32_t mw = 0;
   for (int j = 0; j < K_max; ++j) mw = std::max(mw, rung_max[j]);
   const int64_t F = g.batch * g.count;
   if (op == 3) {
     enqueue_cube_rungs<T>(s, g, CubeOp::kBox, in, rungs0, nullptr, rung_win, K_max, F, mw);
-    launch_gather_all<T, 3>(s, g, el, lo, t, o, rungs0, nullptr, out0, nullptr, on);
+    launch_gather_all<T, 3>(s, g, el, lo, t, o, rungs0, nullptr, out0, nullptr, on, extra);
   } else if (op == 2) {
     if (enqueue_cube_rungs<T>(s, g, CubeOp::kMaxMin, in, rungs0, rungs1, rung_win, K_max, F, mw, pairs))
-      launch_gather_pairs<T>(s, g, el, lo, t, o, pairs, out0, out1, on, mid);
+      launch_gather_pairs<T>(s, g, el, lo, t, o, pairs, out0, out1, on, extra);
     else
-      launch_gather_all<T, 2>(s, g, el, lo, t, o, rungs0, rungs1, out0, out1, on, mid);
+      launch_gather_all<T, 2>(s, g, el, lo, t, o, rungs0, rungs1, out0, out1, on, extra);
   } else {
     enqueue_cube_rungs<T>(s, g, CubeOp::kMax, in, rungs0, nullptr, rung_win, K_max, F, mw);
-    launch_gather_all<T, 0>(s, g, el, lo, t, o, rungs0, nullptr, out0, nullptr, on);
+    launch_gather_all<T, 0>(s, g, el, lo, t, o, rungs0, nullptr, out0, nullptr, on, extra);
   }
 }
 
@@ -475,9 +482,9 @@ template void enqueue_element_weights<double>(hipStream_t, const Geom&, const Me
                                               const ElemDev*, uint8_t*, double*, const int32_t*, uint32_t*);
 template void enqueue_element_op<float>(hipStream_t, const Geom&, int, const ElemDev*, const uint8_t*, const float*,
                                         const float*, const int32_t*, const int32_t*, int, const float*, float*, float*,
-                                        float*, float*, float*, const int32_t*, float*);
+                                        float*, float*, float*, const int32_t*, GatherExtra<float>);
 template void enqueue_element_op<double>(hipStream_t, const Geom&, int, const ElemDev*, const uint8_t*, const double*,
                                          const double*, const int32_t*, const int32_t*, int, const double*, double*,
-                                         double*, double*, double*, double*, const int32_t*, double*);
+                                         double*, double*, double*, double*, const int32_t*, GatherExtra<double>);
 
 }  // namespace eda

@@ -165,9 +165,21 @@ void enqueue_element_weights(hipStream_t s, const Geom& g, const MethodCtl& c, c
 // pass into rungs0 (rungs1: the min cubes of op 2), K_max fields each; then one gather over all
 // rungs (op 2 reads the max and min side by side from pairs, 2 K_max fields, when the cube wrote
 // them there). on [B]: the elements that run (the others' outputs are left untouched).
+// Fields the gather of an element operator writes beside its outputs (each optional): of the max
+// u and min l (op 2) mid = 0.5 (u + l), hdiff = 0.5 (u - l), pdiff = max(u - l, 0); of the box m
+// (op 3) diff = minuend - m.
+template <class T>
+struct GatherExtra {
+  T* mid = nullptr;
+  T* hdiff = nullptr;
+  T* pdiff = nullptr;
+  const T* minuend = nullptr;
+  T* diff = nullptr;
+};
+
 template <class T>
 void enqueue_element_op(hipStream_t s, const Geom& g, int op, const ElemDev* el, const uint8_t* lo, const T* t,
                         const T* o, const int32_t* rung_win, const int32_t* rung_max, int K_max, const T* in, T* out0,
-                        T* out1, T* rungs0, T* rungs1, T* pairs, const int32_t* on, T* mid = nullptr);
+                        T* out1, T* rungs0, T* rungs1, T* pairs, const int32_t* on, GatherExtra<T> extra = {});
 
 }  // namespace eda
