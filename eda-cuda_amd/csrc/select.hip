@@ -122,17 +122,29 @@ __global__ void __launch_bounds__(kThreads) k_sel_hist(SelArgs<T> a, const SelSt
   const uint64_t dmask = (1ULL << width) - 1ULL;
   const uint32_t stride = static_cast<uint32_t>(nblk) * kThreads;
   const uint32_t count = static_cast<uint32_t>(a.g.count);
+  // one count into bin of row r: where every active lane of the wave counts into the same bin (the
+  // common case of the leading digits), the lowest one adds them all
+  const int lane = threadIdx.x & 63;
+  const auto count_in = [&](int r, uint32_t bin) {
+    const uint64_t act = __ballot(1);
+    const uint32_t b0 = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(bin)));
+    if (__ballot(bin == b0) == act) {
+      if (lane == __ffsll(static_cast<unsigned long long>(act)) - 1) atomicAdd(&h[r][b0], static_cast<uint32_t>(__popcll(act)));
+    } else {
+      atomicAdd(&h[r][bin], 1u);
+    }
+  };
   // counts key k where its higher digits match a rank's prefix; returns whether it matched one
   const auto tally = [&](uint64_t k) {
     const uint32_t bin = static_cast<uint32_t>((k >> shift) & dmask);
     if (first) {
-      atomicAdd(&h[0][bin], 1u);
+      count_in(0, bin);
       return true;
     }
     const bool m0 = (k >> hi) == (S.prefix[0] >> hi);
     const bool m1 = nr == 2 && (k >> hi) == (S.prefix[1] >> hi);
-    if (m0) atomicAdd(&h[0][bin], 1u);
-    if (m1) atomicAdd(&h[1][bin], 1u);
+    if (m0) count_in(0, bin);
+    if (m1) count_in(1, bin);
     return m0 || m1;
   };
   const uint32_t nc = PHASE == 2 ? cand.n[b] : 0u;
