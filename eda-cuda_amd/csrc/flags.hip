@@ -233,6 +233,7 @@ struct FlagTile {
   int32_t n, nb, P, lpad;  // line length, blocks per line, lines per tile, LDS slots per line
   int64_t step, inner, outer, groups, count;
   int32_t axis, first, wide;  // first: the first axis pass (write, not OR); wide: 4-byte flag words
+  int32_t xcd;                 // the XCD rotation's chunk (0: none)
 };
 
 template <class T>
@@ -240,8 +241,10 @@ __global__ void __launch_bounds__(kThreads) k_flags_tile(FlagTile a, const T* __
                                                          const T* __restrict__ tau, uint8_t* __restrict__ flags,
                                                          uint64_t* __restrict__ ext, const int32_t* __restrict__ on) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int64_t b = blockIdx.x / a.groups;
-  const int64_t grp = blockIdx.x - b * a.groups;
+  // the XCD rotation: neighbouring column groups (which share cache lines) on one XCD
+  const int64_t wg = a.xcd ? xcd_item(blockIdx.x, a.xcd) : static_cast<int64_t>(blockIdx.x);
+  const int64_t b = wg / a.groups;
+  const int64_t grp = wg - b * a.groups;
   if (on != nullptr && on[b] == 0) return;
   const int32_t n = a.n, P = a.P, nb = a.nb;
   T* X = reinterpret_cast<T*>(smem);  // line p at X + p lpad
@@ -449,6 +452,7 @@ void enqueue_sig_flags(hipStream_t s, const Geom& g, const T* r, const T* tau, u
       t.wide = al4 && (t.inner == 1 ? n % 4 == 0 : (P % 4 == 0 && t.inner % 4 == 0)) ? 1 : 0;
       const int64_t wgs = g.batch * t.groups;
       if (wgs > 0x7fffffff) throw std::invalid_argument("eda: too many workgroups for the extrema pass");
+      t.xcd = t.inner > 1 ? xcd_chunk(wgs) : 0;
       EDA_GO(k_flags_tile<T>, dim3(static_cast<unsigned>(wgs)), dim3(kThreads), bytes(P), s, t, r, tau,
                          flags, ext, on);
       EDA_LAUNCH("k_flags_tile");

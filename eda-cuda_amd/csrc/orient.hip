@@ -525,6 +525,7 @@ struct FftDims {
   int32_t L;       // pairs per workgroup
   int64_t tiles;   // workgroups per element
   int64_t batch;
+  int32_t xcd;     // the XCD rotation's chunk (0: none): neighbouring column tiles share cache lines
 };
 
 // The line l of element b: sample x at b count + (l / inner) n inner + x inner + l mod inner.
@@ -565,9 +566,10 @@ __global__ void __launch_bounds__(256) k_dct_fft(FftDims d, FftTabs<T> tb, const
     }
   }
   const int32_t ln = LN > 0 ? LN : d.ln, l2L = d.lL + 1;  // log2 n, log2 (2 L)
-  const int64_t b = blockIdx.x / d.tiles;
+  const int64_t wg = d.xcd ? xcd_item(blockIdx.x, d.xcd) : static_cast<int64_t>(blockIdx.x);
+  const int64_t b = wg / d.tiles;
   if (b >= d.batch || (on != nullptr && on[b] == 0)) return;  // uniform per workgroup
-  const int64_t q0 = (blockIdx.x - b * d.tiles) * L;          // the tile's first pair
+  const int64_t q0 = (wg - b * d.tiles) * L;                  // the tile's first pair
   const int32_t pts = L * n;
   // loads: 2 L lines; along the contiguous axis sample-fastest, else line-fastest (adjacent columns)
   const bool contiguous = d.inner == 1;
@@ -856,6 +858,7 @@ void fft_transform(hipStream_t s, int device, const Geom& g, int a, const T* in,
   const size_t lds = (static_cast<size_t>(fft_im_off(d.L, fft_sp(n))) + static_cast<size_t>(d.L) * fft_sp(n) +
                       4 * static_cast<size_t>(n)) * sizeof(T);
   const int64_t grid = d.tiles * g.batch;
+  d.xcd = g.st[a] != 1 ? xcd_chunk(grid) : 0;
   const bool timed = timer_gemm_on();
   hipEvent_t t0 = nullptr;
   if (timed) timer_note(s, &t0);

@@ -577,21 +577,6 @@ __device__ void chunk_lines(const T* E, int32_t epad, int32_t lead, T* R, int32_
 }
 
 // ---- the pass kernel ----------------------------------------------------------------------------------------
-// Workgroup i runs on XCD i mod 8. The item of workgroup i when chunks of c consecutive items
-// rotate over the XCDs (the grid a multiple of 8 c): the tiles of one chunk are neighbours in
-// memory and share an XCD's L2, and the items of any few elements spread over all eight XCDs (a
-// launch in which only some elements run keeps the whole chip busy).
-__device__ __forceinline__ int64_t xcd_item(int64_t i, int32_t c) {
-  const int64_t j = i >> 3;
-  return ((j / c) * 8 + (i & 7)) * c + j % c;
-}
-// The largest chunk of 8, 4, 2 items whose rotation tiles a grid of wgs workgroups (0: none).
-int32_t xcd_chunk(int64_t wgs) {
-  for (int32_t c = 8; c >= 2; c >>= 1)
-    if (wgs % (8 * c) == 0) return c;
-  return 0;
-}
-
 struct VPass {
   Geom g;
   int32_t n;        // line length
