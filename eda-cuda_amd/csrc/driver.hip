@@ -916,11 +916,12 @@ void Chunk<T>::raw_window(const uint8_t* flags, const int32_t* win, int32_t maxw
   const Geom g = g_;
   const T two_kappa = static_cast<T>(2.0 * c_.kappa);
   const T tiny = static_cast<T>(c_.tiny);
+  // axis a's flags as 0 / 1 into tmp (for a > 0 in the pass that accumulates axis a - 1's square)
+  each(s_, g, on, [=] __device__(int64_t b, int64_t i) {
+    const int64_t at = b * g.count + i;
+    tmp[at] = (flags[at] & 1u) ? T(1) : T(0);
+  });
   for (int a = 0; a < g.dim; ++a) {
-    each(s_, g, on, [=] __device__(int64_t b, int64_t i) {
-      const int64_t at = b * g.count + i;
-      tmp[at] = ((flags[at] >> a) & 1u) ? T(1) : T(0);
-    });
     enqueue_cube<T>(s_, g_, CubeOp::kBox, tmp, tmp, nullptr, win, maxw);
     const bool last = a == g.dim - 1;
     each(s_, g, on, [=] __device__(int64_t b, int64_t i) {
@@ -930,6 +931,8 @@ void Chunk<T>::raw_window(const uint8_t* flags, const int32_t* win, int32_t maxw
       if (last) {
         const T dens = sqrt(acc);
         acc = dens > T(0) ? two_kappa / (dens > tiny ? dens : tiny) : inf_t<T>();
+      } else {
+        tmp[at] = ((flags[at] >> (a + 1)) & 1u) ? T(1) : T(0);
       }
       om[at] = acc;
     });
