@@ -697,14 +697,15 @@ class Chunk {
       if (nr < 0) return;
       rows[(b * cap + nr) * g.count + i] = r[b * g.count + i];
     });
-    const int64_t per = cap * g.count;
-    const int64_t Bc = B_;
-    flat(s_, Bc * per, [=] __device__(int64_t idx) {
-      const int64_t b = idx / per;
+    // rows 0 .. nr of element b: samples i < (nr + 1) count of its cap rows (no index division)
+    Geom gr = g;
+    gr.count = cap * g.count;
+    each(s_, gr, nullptr, [=] __device__(int64_t b, int64_t i) {
       const int32_t k = dk[b];
       const int32_t nr = drow[b];
       if (k == 0 || nr < 0) return;
-      if ((idx - b * per) / g.count > nr) return;
+      if (i >= (static_cast<int64_t>(nr) + 1) * g.count) return;
+      const int64_t idx = b * gr.count + i;
       T v;
       if constexpr (std::is_same_v<T, float>) v = static_cast<float>(ldexp(static_cast<double>(rows[idx]), k));
       else v = ldexp(rows[idx], k);
@@ -1837,16 +1838,18 @@ void collect_rows(const Run& run, void* dst, int32_t dst_device) {
   const int32_t* dn = ctx.up.put(run.n_imfs);
   const int32_t* de = ctx.up.put(run.error);
   const T* src = static_cast<const T*>(run.rows.p);
-  flat(ctx.stream, static_cast<int64_t>(elems), [=] __device__(int64_t idx) {
-    const int64_t i = idx % count;
-    const int64_t bk = idx / count;
-    const int64_t k = bk % P;
+  // row k of element b (bk = b P + k) over a (B P, count) grid
+  Geom gr{};
+  gr.count = count;
+  gr.batch = B * P;
+  each(ctx.stream, gr, nullptr, [=] __device__(int64_t bk, int64_t i) {
     const int64_t b = bk / P;
+    const int64_t k = bk - b * P;
     T v;
     if (de[b] != 0) v = static_cast<T>(__longlong_as_double(0x7ff8000000000000LL));
     else if (k <= dn[b]) v = src[(b * cap + k) * count + i];
     else v = T(0);
-    out[idx] = v;
+    out[bk * count + i] = v;
   });
   if (dst_device < 0) EDA_HIP(hipMemcpyAsync(dst, out, elems * sizeof(T), hipMemcpyDeviceToHost, ctx.stream));
   EDA_HIP(hipStreamSynchronize(ctx.stream));
