@@ -697,15 +697,18 @@ class Chunk {
       if (nr < 0) return;
       rows[(b * cap + nr) * g.count + i] = r[b * g.count + i];
     });
-    // rows 0 .. nr of element b: samples i < (nr + 1) count of its cap rows (no index division)
+    // rows 0 .. nr of element b: samples i < (nr + 1) count of its rows (no index division), over
+    // the rows up to the largest nr
+    int32_t top = 0;
+    for (int64_t b = 0; b < B_; ++b) top = std::max(top, row_of[b]);
     Geom gr = g;
-    gr.count = cap * g.count;
+    gr.count = (static_cast<int64_t>(top) + 1) * g.count;
     each(s_, gr, nullptr, [=] __device__(int64_t b, int64_t i) {
       const int32_t k = dk[b];
       const int32_t nr = drow[b];
       if (k == 0 || nr < 0) return;
       if (i >= (static_cast<int64_t>(nr) + 1) * g.count) return;
-      const int64_t idx = b * gr.count + i;
+      const int64_t idx = b * cap * g.count + i;
       T v;
       if constexpr (std::is_same_v<T, float>) v = static_cast<float>(ldexp(static_cast<double>(rows[idx]), k));
       else v = ldexp(rows[idx], k);
