@@ -5,17 +5,19 @@ resident in HBM as a PyTorch-ROCm tensor, default parameters.
 
 A step is one eda.decompose call over the whole batch (every mode of every image, rows collected
 into a device tensor). W untimed warm-up steps, then K timed steps bracketed by a device
-synchronization. Prints one JSON line on stdout (progress goes to stderr):
+synchronization, with no instrumentation (the library's kernel timer is off). Prints one JSON line on
+stdout (progress goes to stderr):
 
-  value      Msamples/s = 64 * 512 * 512 / (seconds per step)
-  roofline   the dominant kernel, the cube filter's vHGW pass (one pass = one launch of k_vhgw_rows
-             or k_vhgw, or the one or two launches of k_box of a box-sum pass): algorithmic bytes
-             per pass (its input field(s) read once, output field(s) written once, over the running
-             elements) / its average duration, both measured live with HIP events on the library's
-             stream over the timed steps, against 8 TB/s; `traffic` the HBM bytes per pass from
+  value      Msamples/s = 64 * 512 * 512 / (seconds per step over the K timed steps)
+  roofline   the dominant kernel family, the cube filter's vHGW pass (k_vhgw_rows, k_vhgw and the
+             two k_box launches): algorithmic bytes per launch (its input field(s) read once,
+             output field(s) written once, over the running elements) / its average duration, both
+             measured live with HIP events on the library's stream over two extra, untimed steps
+             (the ledger), against 8 TB/s; `kernels` the same figures per kernel (also the rung
+             gather, the extrema tiles and the DCT passes); `traffic` the HBM bytes per launch from
              rocprofv3 FETCH_SIZE + WRITE_SIZE (profiles/), null when not measured
-  cpu_baseline  the oracle (a C++ restatement of the method, FP64, one image per host thread)
-             on a bounded sample of the same workload, on this host's cores
+  cpu_baseline  the oracle (a C++ restatement of the method, FP64, OpenMP over images) on the whole
+             batch, on this host's cores (at most 16), the median of 3 runs when a run is short
 
 With --monogenic the same batch goes through eda.monogenic instead, and the roofline is the DCT
 axis pass (the fast transform of the 512-sample axes: one read and one write of the field).
@@ -49,13 +51,13 @@ def log(*a):
 
 
 def traffic_from_profiles():
-    """HBM bytes per vHGW pass measured by rocprofv3 (profiles/vhgw_pass_64x512x512_f32.json), or
-    None."""
+    """HBM bytes per vHGW-pass launch measured by rocprofv3 (profiles/vhgw_pass_64x512x512_f32.json),
+    or None."""
     p = ROOT / "profiles" / "vhgw_pass_64x512x512_f32.json"
     if not p.exists():
         return None
     try:
-        return json.loads(p.read_text()).get("traffic_bytes_per_pass")
+        return json.loads(p.read_text()).get("traffic_bytes_per_launch")
     except Exception:
         return None
 
@@ -74,26 +76,23 @@ def _counter_rows(directory):
 
 
 def pmc_summary(fetch_dir, write_dir, out_json):
-    """HBM traffic per vHGW pass from two rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE, KiB
-    per dispatch) of `bench.py --steps 1 --warmup 0 --no-cpu --copy-probe`, each scaled by what the
-    counter reads for the probe's copy kernel (torch.neg over 1 GiB: 1 GiB read, 1 GiB written).
-    The pass kernels' KiB are summed and divided by the number of passes: every k_vhgw_rows or
-    k_vhgw dispatch is a pass, and a box-sum pass is one k_box<..., false> dispatch and at most one
-    k_box<..., true>."""
+    """HBM traffic per vHGW-pass launch from two rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE,
+    KiB per dispatch) of `bench.py --steps 1 --warmup 0 --no-cpu --copy-probe`, each scaled by what
+    the counter reads for the probe's copy kernel (torch.neg over 1 GiB: 1 GiB read, 1 GiB written).
+    The KiB of every k_vhgw_rows, k_vhgw and k_box dispatch are summed and divided by their number."""
     def averages(directory, counter):
-        total, passes, cp = 0.0, 0, []
+        total, launches, cp = 0.0, 0, []
         for name, cname, value in _counter_rows(directory):
             if cname != counter:
                 continue
             if "k_vhgw" in name or "k_box" in name:
                 total += value
-                if "k_vhgw" in name or "false>" in name:
-                    passes += 1
+                launches += 1
             elif "neg" in name:
                 cp.append(value)
-        if not passes or not cp:
+        if not launches or not cp:
             raise SystemExit(f"bench: no {counter} rows for the vHGW pass and the copy probe under {directory}")
-        return total / passes, max(cp), passes
+        return total / launches, max(cp), launches
 
     f_vh, f_cp, n_f = averages(fetch_dir, "FETCH_SIZE")
     w_vh, w_cp, n_w = averages(write_dir, "WRITE_SIZE")
@@ -103,14 +102,14 @@ def pmc_summary(fetch_dir, write_dir, out_json):
     res = {
         "kernel": "vHGW pass (k_vhgw_rows, k_vhgw, k_box)",
         "workload": "eda.decompose fwd, 64 x 512 x 512 float32 (one call)",
-        "passes": n_f,
-        "fetch_kib_per_pass": round(f_vh, 1),
-        "write_kib_per_pass": round(w_vh, 1),
+        "launches": n_f,
+        "fetch_kib_per_launch": round(f_vh, 1),
+        "write_kib_per_launch": round(w_vh, 1),
         "copy_probe_fetch_kib": round(f_cp, 1),
         "copy_probe_write_kib": round(w_cp, 1),
         "fetch_scale": round(f_scale, 4),
         "write_scale": round(w_scale, 4),
-        "traffic_bytes_per_pass": round(traffic),
+        "traffic_bytes_per_launch": round(traffic),
     }
     Path(out_json).write_text(json.dumps(res, indent=1) + "\n")
     print(json.dumps(res))
@@ -132,29 +131,45 @@ def copy_probe(torch, log_fn):
     del a, b
 
 
+def host_threads():
+    """The host threads the CPU baseline uses and the cores this process may run on: every core of
+    the affinity mask, at most 16 (one GPU's share of the host)."""
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    return max(1, min(16, cores)), cores
+
+
 def cpu_baseline():
+    """The FP64 oracle on the whole 64-image batch (OpenMP over images); the median of 3 runs when
+    one run takes under 40 s, else one run."""
     import numpy as np
 
     import oracle
     import workloads
 
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    threads = max(1, min(16, cores))
-    images = min(B, 2 * threads)
-    x = workloads.rings_stripes_blobs(images, N).astype(np.float64)
-    log(f"bench: cpu baseline: the FP64 oracle on {images} image(s) of {N}x{N}, {threads} thread(s)")
-    t0 = time.perf_counter()
-    oracle.decompose(x, dim=2, threads=threads)
-    dt = time.perf_counter() - t0
+    threads, cores = host_threads()
+    x = workloads.rings_stripes_blobs(B, N).astype(np.float64)
+    log(f"bench: cpu baseline: the FP64 oracle on {B} images of {N}x{N}, {threads} thread(s) ({cores} cores in the "
+        f"affinity mask)")
+    times = []
+    while len(times) < 3:
+        t0 = time.perf_counter()
+        oracle.decompose(x, dim=2, threads=threads)
+        times.append(time.perf_counter() - t0)
+        log(f"bench: cpu baseline run {len(times)}: {times[-1]:.1f} s")
+        if times[0] >= 40.0:
+            break
+    dt = sorted(times)[len(times) // 2]
     return {
-        "value": round(images * N * N / dt / 1e6, 6),
+        "value": round(B * N * N / dt / 1e6, 6),
         "unit": "Msamples/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"images 0..{images - 1} of the 64 ({N}x{N}, float64 oracle, OpenMP over images), {dt:.1f} s",
+        "sample": f"all {B} images ({N}x{N}, float64 oracle, OpenMP over images, {threads} threads of the "
+                  f"{cores} cores in the affinity mask), median of {len(times)} run(s): "
+                  + ", ".join(f"{t:.1f}" for t in times) + " s",
     }
 
 
@@ -229,11 +244,7 @@ def bench_monogenic(args, x, lib, torch) -> dict:
         import workloads
         from concurrent.futures import ThreadPoolExecutor
 
-        try:
-            cores = len(os.sched_getaffinity(0))
-        except AttributeError:
-            cores = os.cpu_count() or 1
-        threads = max(1, min(16, cores))
+        threads, _ = host_threads()
         images = threads
         xs = workloads.rings_stripes_blobs(images, N).astype(np.float64)
         log(f"bench: monogenic cpu baseline: the FP64 oracle on {images} image(s), {threads} thread(s)")
@@ -249,6 +260,55 @@ def bench_monogenic(args, x, lib, torch) -> dict:
             "sample": f"images 0..{images - 1} of the 64 ({N}x{N}, float64 oracle, one image per thread), {cdt:.1f} s",
         }
     return out
+
+
+LEDGER_STEPS = 2
+
+
+def kernel_ledger(args, x, lib, torch, eda, _native, step_s):
+    """Per-kernel live figures from LEDGER_STEPS extra steps run with every kernel family timed
+    (HIP events on the library's stream around each launch; these steps are not the timed ones):
+    per kernel its launches per step, average duration, algorithmic bytes per launch, GB/s and
+    fraction of the HBM peak, and its share of an un-instrumented step. Also the same figures per
+    family: the vHGW pass (k_vhgw_rows + k_vhgw + both k_box launches)."""
+    err = _native.errbuf()
+    buf = ctypes.create_string_buffer(4096)
+    lib.eda_timer_enable(15)
+    _native.check(lib.eda_timer_report(buf, len(buf), err, len(err)), err)  # clears what came before
+    for _ in range(LEDGER_STEPS):
+        y = eda.decompose(x, axes=(1, 2))
+        del y
+    torch.cuda.synchronize()
+    _native.check(lib.eda_timer_report(buf, len(buf), err, len(err)), err)
+    lib.eda_timer_enable(0)
+    raw = json.loads(buf.value.decode())
+
+    def figures(ms, launches, nbytes):
+        if launches == 0 or ms <= 0:
+            return None
+        us = ms * 1e3 / launches
+        per = nbytes / launches
+        gbs = per / (us * 1e-6) / 1e9
+        return {
+            "launches_per_step": round(launches / LEDGER_STEPS, 1),
+            "us_per_launch": round(us, 2),
+            "bytes_per_launch": round(per),
+            "GBs": round(gbs, 1),
+            "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "share_of_step": round(ms * 1e-3 / LEDGER_STEPS / step_s, 4),
+        }
+
+    kernels = {}
+    for name, v in raw.items():
+        f = figures(v["ms"], v["launches"], v["bytes"])
+        if f is not None:
+            kernels[name] = f
+            log(f"bench: {name:14s} {f['launches_per_step']:7.1f} launches/step {f['us_per_launch']:9.2f} us "
+                f"{f['bytes_per_launch'] / 1e6:9.1f} MB {f['GBs']:8.1f} GB/s ({f['frac']:.3f} of peak), "
+                f"{100 * f['share_of_step']:.1f} % of a step")
+    vh = [raw[k] for k in ("k_vhgw_rows", "k_vhgw", "k_box_short", "k_box_long")]
+    fam = {"vhgw": figures(sum(v["ms"] for v in vh), sum(v["launches"] for v in vh), sum(v["bytes"] for v in vh))}
+    return kernels, fam
 
 
 def main() -> int:
@@ -291,61 +351,64 @@ def main() -> int:
         return 0
 
     info = None
+    warm = []
     for i in range(args.warmup):
         t = time.perf_counter()
         y, info = eda.decompose(x, axes=(1, 2), return_info=True)
         torch.cuda.synchronize()
-        log(f"bench: warm-up {i}: {time.perf_counter() - t:.3f} s, n_imfs {sorted(set(info.n_imfs.tolist()))}")
+        warm.append(time.perf_counter() - t)
+        log(f"bench: warm-up {i}: {warm[-1]:.3f} s, n_imfs {sorted(set(info.n_imfs.tolist()))}")
         del y
 
-    err = _native.errbuf()
-    lib.eda_timer_enable(1)
-    _native.check(lib.eda_timer_read(ctypes.byref(ctypes.c_double()), ctypes.byref(ctypes.c_int64()),
-                                     ctypes.byref(ctypes.c_double()), err, len(err)), err)
+    # the timed steps: no instrumentation (the kernel timer is off)
+    lib.eda_timer_enable(0)
+    per_step = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
+        t = time.perf_counter()
         y, info = eda.decompose(x, axes=(1, 2), return_info=True)
+        torch.cuda.synchronize()
+        per_step.append(time.perf_counter() - t)
         del y
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / max(args.steps, 1)
-    ms = ctypes.c_double()
-    launches = ctypes.c_int64()
-    nbytes = ctypes.c_double()
-    _native.check(lib.eda_timer_read(ctypes.byref(ms), ctypes.byref(launches), ctypes.byref(nbytes), err, len(err)),
-                  err)
-    lib.eda_timer_enable(0)
-    log(f"bench: {args.steps} steps, {dt * 1e3:.1f} ms per step; vHGW pass: {launches.value} passes, "
-        f"{ms.value:.1f} ms in total")
+    med = sorted(per_step)[len(per_step) // 2] if per_step else dt
+    log(f"bench: {args.steps} steps, {dt * 1e3:.1f} ms per step (median {med * 1e3:.1f} ms)")
+
+    # the per-kernel ledger: separate, untimed steps with every kernel family timed by HIP events
+    kernels, fam = kernel_ledger(args, x, lib, torch, eda, _native, dt)
 
     if args.copy_probe:
         copy_probe(torch, log)
 
-    # the host-input rate (numpy in, numpy out: PCIe transfers included); reported on stderr only
+    # the host-input rate (numpy in, numpy out: PCIe transfers included); stderr only, median of 3
     xh = x.cpu().numpy()
-    t = time.perf_counter()
-    yh = eda.decompose(xh, axes=(1, 2))
-    host_dt = time.perf_counter() - t
+    host = []
+    for _ in range(3):
+        t = time.perf_counter()
+        yh = eda.decompose(xh, axes=(1, 2))
+        host.append(time.perf_counter() - t)
+        del yh
+    host_dt = sorted(host)[1]
     log(f"bench: host input, PCIe-inclusive: {B * N * N / host_dt / 1e6:.1f} Msamples/s "
-        f"({host_dt * 1e3:.1f} ms, output {yh.nbytes / 2**20:.0f} MiB)")
-    del yh
+        f"(median of 3 calls {host_dt * 1e3:.1f} ms: " + ", ".join(f"{h * 1e3:.1f}" for h in host) + " ms)")
 
     value = B * N * N / dt / 1e6
-    per_launch_bytes = nbytes.value / max(launches.value, 1)
-    avg_s = ms.value * 1e-3 / max(launches.value, 1)
-    achieved = per_launch_bytes / avg_s / 1e9 if avg_s > 0 else 0.0
+    vh = fam["vhgw"]
     roofline = {
         "bound": "hbm",
         "kernel": "vHGW pass (k_vhgw_rows, k_vhgw, k_box)",
-        "achieved": round(achieved, 1),
+        "achieved": vh["GBs"],
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "frac": vh["frac"],
         "traffic": traffic_from_profiles(),
-        "algorithmic_bytes_per_pass": round(per_launch_bytes),
-        "avg_pass_us": round(avg_s * 1e6, 2),
-        "passes_per_step": launches.value // max(args.steps, 1),
-        "share_of_step": round(ms.value * 1e-3 / max(args.steps, 1) / dt, 4),
+        "algorithmic_bytes_per_launch": vh["bytes_per_launch"],
+        "avg_launch_us": vh["us_per_launch"],
+        "launches_per_step": vh["launches_per_step"],
+        "share_of_step": vh["share_of_step"],
+        "kernels": kernels,
     }
     out = {
         "metric": METRIC,
@@ -367,6 +430,9 @@ def main() -> int:
             "batch": B,
             "shape": [N, N],
             "pass": "fwd",
+            "median_ms_per_step": round(med * 1e3, 3),
+            "warmup_ms": [round(w * 1e3, 1) for w in warm],
+            "host_input_ms": round(host_dt * 1e3, 1),
             "n_imfs": sorted(set(info.n_imfs.tolist())) if info is not None else None,
         },
         "roofline": roofline,

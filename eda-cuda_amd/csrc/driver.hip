@@ -44,13 +44,19 @@ void throw_hip(hipError_t e, const char* expr, const char* file, int line) {
 namespace {
 struct TimerState {
   std::mutex mu;
-  std::atomic<int> on{0};  // 0 off, 1 the vHGW pass, 2 the DCT contraction
+  std::atomic<int> on{0};  // a mask of TimerFamily bits
   std::vector<hipEvent_t> pool;
   struct Span {
     hipEvent_t a, b;
     double bytes;
+    int tag;
   };
   std::vector<Span> spans;
+  struct Total {
+    double ms = 0.0, bytes = 0.0;
+    int64_t launches = 0;
+  };
+  Total per_tag[kTagCount];  // summed by timer_read and kept until eda_timer_report
   hipEvent_t take() {
     if (!pool.empty()) {
       hipEvent_t e = pool.back();
@@ -74,9 +80,10 @@ struct Mirror {
 thread_local std::vector<Mirror> g_mirrors;
 }  // namespace
 
-void timer_set(int kind) { timer_state().on = kind; }
-bool timer_on() { return timer_state().on.load(std::memory_order_relaxed) == 1; }
-bool timer_gemm_on() { return timer_state().on.load(std::memory_order_relaxed) == 2; }
+void timer_set(int mask) { timer_state().on = mask; }
+bool timer_family(int family) { return (timer_state().on.load(std::memory_order_relaxed) & family) != 0; }
+bool timer_on() { return timer_family(kTimeVhgw); }
+bool timer_gemm_on() { return timer_family(kTimeDct); }
 void timer_note(hipStream_t s, hipEvent_t* start) {
   flush_uploads();  // a staged upload is not part of the timed launch
   TimerState& t = timer_state();
@@ -84,12 +91,12 @@ void timer_note(hipStream_t s, hipEvent_t* start) {
   *start = t.take();
   EDA_HIP(hipEventRecord(*start, s));
 }
-void timer_done(hipStream_t s, hipEvent_t start, double bytes) {
+void timer_done(hipStream_t s, hipEvent_t start, double bytes, int tag) {
   TimerState& t = timer_state();
   std::lock_guard<std::mutex> lock(t.mu);
   hipEvent_t stop = t.take();
   EDA_HIP(hipEventRecord(stop, s));
-  t.spans.push_back({start, stop, bytes});
+  t.spans.push_back({start, stop, bytes, tag});
 }
 void timer_read(double* ms, int64_t* launches, double* bytes) {
   TimerState& t = timer_state();
@@ -101,6 +108,12 @@ void timer_read(double* ms, int64_t* launches, double* bytes) {
     EDA_HIP(hipEventElapsedTime(&e, sp.a, sp.b));
     sum += e;
     by += sp.bytes;
+    if (sp.tag >= 0 && sp.tag < kTagCount) {
+      TimerState::Total& tt = t.per_tag[sp.tag];
+      tt.ms += e;
+      tt.bytes += sp.bytes;
+      tt.launches += 1;
+    }
     t.pool.push_back(sp.a);
     t.pool.push_back(sp.b);
   }
@@ -109,21 +122,42 @@ void timer_read(double* ms, int64_t* launches, double* bytes) {
   *bytes = by;
   t.spans.clear();
 }
+// the per-kernel totals summed by the timer_read calls since the last report, as JSON; clears them
+std::string timer_report() {
+  static const char* const names[kTagCount] = {"k_vhgw_rows", "k_vhgw", "k_box_short", "k_box_long",
+                                                "k_dct_fft", "k_gather_all", "k_flags_tile"};
+  double ms = 0.0, by = 0.0;
+  int64_t n = 0;
+  timer_read(&ms, &n, &by);
+  TimerState& t = timer_state();
+  std::lock_guard<std::mutex> lock(t.mu);
+  std::string js = "{";
+  for (int k = 0; k < kTagCount; ++k) {
+    const TimerState::Total& tt = t.per_tag[k];
+    char buf[256];
+    std::snprintf(buf, sizeof buf, "%s\"%s\": {\"ms\": %.6f, \"launches\": %lld, \"bytes\": %.1f}", k ? ", " : "",
+                  names[k], tt.ms, static_cast<long long>(tt.launches), tt.bytes);
+    js += buf;
+    t.per_tag[k] = TimerState::Total{};
+  }
+  return js + "}";
+}
 void mirror_note(const void* dev, const void* host, size_t bytes) {
   g_mirrors.push_back({static_cast<const char*>(dev), static_cast<const char*>(host), bytes});
 }
 void mirror_clear() { g_mirrors.clear(); }
+const void* mirror_host(const void* dev, size_t bytes) {
+  const char* p = static_cast<const char*>(dev);
+  for (auto it = g_mirrors.rbegin(); it != g_mirrors.rend(); ++it)
+    if (p >= it->dev && p + bytes <= it->dev + it->bytes) return it->host + (p - it->dev);
+  return nullptr;
+}
 int64_t mirror_active(const int32_t* win, int64_t n) {
-  const char* p = reinterpret_cast<const char*>(win);
-  for (auto it = g_mirrors.rbegin(); it != g_mirrors.rend(); ++it) {
-    if (p >= it->dev && p + n * sizeof(int32_t) <= it->dev + it->bytes) {
-      const int32_t* h = reinterpret_cast<const int32_t*>(it->host + (p - it->dev));
-      int64_t c = 0;
-      for (int64_t b = 0; b < n; ++b) c += h[b] != 0 ? 1 : 0;
-      return c;
-    }
-  }
-  return -1;
+  const int32_t* h = static_cast<const int32_t*>(mirror_host(win, static_cast<size_t>(n) * sizeof(int32_t)));
+  if (h == nullptr) return -1;
+  int64_t c = 0;
+  for (int64_t b = 0; b < n; ++b) c += h[b] != 0 ? 1 : 0;
+  return c;
 }
 
 MethodCtl method_ctl(const Geom& g, bool f32) {
@@ -256,7 +290,7 @@ struct Uploader {
     pend = a + bytes;
     t_staged = this;
     off = a + bytes;
-    if (timer_on()) mirror_note(d, host + a, bytes);
+    if (timer_state().on.load(std::memory_order_relaxed) != 0) mirror_note(d, host + a, bytes);
     return reinterpret_cast<V*>(d);
   }
   template <class V>
@@ -1459,7 +1493,7 @@ bool Chunk<T>::mode() {
   }
   rw_ = rw;
   rw_cur_ = rw;
-  if (timer_on() && !rw_cur_.empty()) mirror_note(d_rung_win_, rw_cur_.data(), rw_cur_.size() * sizeof(int32_t));
+  if (timer_family(~0) && !rw_cur_.empty()) mirror_note(d_rung_win_, rw_cur_.data(), rw_cur_.size() * sizeof(int32_t));
   on = upload_on(active);
   on_op_ = on;
 
@@ -1525,7 +1559,7 @@ bool Chunk<T>::mode() {
       flush_uploads();
       EDA_HIP(hipMemcpyAsync(d_rung_win_, drw, sizeof(int32_t) * rw.size(), hipMemcpyDeviceToDevice, s_));
       rw_cur_ = rw;
-      if (timer_on() && !rw_cur_.empty()) mirror_note(d_rung_win_, rw_cur_.data(), rw_cur_.size() * sizeof(int32_t));
+      if (timer_family(~0) && !rw_cur_.empty()) mirror_note(d_rung_win_, rw_cur_.data(), rw_cur_.size() * sizeof(int32_t));
     }
     T* plain = X[0];
     T* up = X[1];
@@ -2160,13 +2194,21 @@ int eda_pointer_device(const void* ptr, int32_t* device) {
 }
 
 int eda_timer_enable(int32_t on) {
-  if (on < 0 || on > 2) return EDA_EINVAL;
+  if (on < 0 || on > 15) return EDA_EINVAL;
   timer_set(on);
   return EDA_OK;
 }
 
 int eda_timer_read(double* ms, int64_t* launches, double* bytes, char* err, size_t errlen) {
   return guarded(err, errlen, [&] { timer_read(ms, launches, bytes); });
+}
+
+int eda_timer_report(char* buf, size_t len, char* err, size_t errlen) {
+  return guarded(err, errlen, [&] {
+    const std::string js = timer_report();
+    if (buf == nullptr || len <= js.size()) throw std::invalid_argument("eda: the report buffer is too small");
+    std::memcpy(buf, js.c_str(), js.size() + 1);
+  });
 }
 
 }  // extern "C"

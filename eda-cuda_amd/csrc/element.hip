@@ -450,6 +450,27 @@ void enqueue_element_weights(hipStream_t s, const Geom& g, const MethodCtl& c, c
   EDA_LAUNCH("k_weights");
 }
 
+// The timer's algorithmic bytes of one gather: per voxel of a running element, lo (1 byte) and t
+// read, the voxel's two bracketing rung fields (max and min: both) read, o (D fields) read where the
+// element uses segment points, the outputs and extra outputs written (and the minuend read).
+template <class T>
+double gather_bytes(const Geom& g, int op, const ElemDev* el, const int32_t* on, const GatherExtra<T>& ex) {
+  const ElemDev* he = static_cast<const ElemDev*>(mirror_host(el, static_cast<size_t>(g.batch) * sizeof(ElemDev)));
+  const int32_t* ho =
+      on != nullptr ? static_cast<const int32_t*>(mirror_host(on, static_cast<size_t>(g.batch) * sizeof(int32_t)))
+                    : nullptr;
+  const int nf = op == 2 ? 2 : 1;
+  const int fields = 1 + 2 * nf + nf + (ex.mid ? 1 : 0) + (ex.hdiff ? 1 : 0) + (ex.pdiff ? 1 : 0) +
+                     (ex.diff ? 2 : 0);
+  double total = 0.0;
+  for (int64_t b = 0; b < g.batch; ++b) {
+    if (on != nullptr && ho != nullptr && ho[b] == 0) continue;
+    const bool pts = g.dim >= 2 && (he == nullptr || he[b].pts != 0u);
+    total += static_cast<double>(g.count) * (1.0 + sizeof(T) * (fields + (pts ? g.dim : 0)));
+  }
+  return total;
+}
+
 template <class T>
 void enqueue_element_op(hipStream_t s, const Geom& g, int op, const ElemDev* el, const uint8_t* lo, const T* t,
                         const T* o, const int32_t* rung_win, const int32_t* rung_max, int K_max, const T* in, T* out0,
@@ -458,17 +479,24 @@ void enqueue_element_op(hipStream_t s, const Geom& g, int op, const ElemDev* el,
   int32_t mw = 0;
   for (int j = 0; j < K_max; ++j) mw = std::max(mw, rung_max[j]);
   const int64_t F = g.batch * g.count;
+  const bool timed = timer_family(kTimeGather);
+  hipEvent_t t0 = nullptr;
+  const auto timed_gather = [&](auto&& launch) {
+    if (timed) timer_note(s, &t0);
+    launch();
+    if (timed) timer_done(s, t0, gather_bytes<T>(g, op, el, on, extra), kTagGather);
+  };
   if (op == 3) {
     enqueue_cube_rungs<T>(s, g, CubeOp::kBox, in, rungs0, nullptr, rung_win, K_max, F, mw);
-    launch_gather_all<T, 3>(s, g, el, lo, t, o, rungs0, nullptr, out0, nullptr, on, extra);
+    timed_gather([&] { launch_gather_all<T, 3>(s, g, el, lo, t, o, rungs0, nullptr, out0, nullptr, on, extra); });
   } else if (op == 2) {
     if (enqueue_cube_rungs<T>(s, g, CubeOp::kMaxMin, in, rungs0, rungs1, rung_win, K_max, F, mw, pairs))
-      launch_gather_pairs<T>(s, g, el, lo, t, o, pairs, out0, out1, on, extra);
+      timed_gather([&] { launch_gather_pairs<T>(s, g, el, lo, t, o, pairs, out0, out1, on, extra); });
     else
-      launch_gather_all<T, 2>(s, g, el, lo, t, o, rungs0, rungs1, out0, out1, on, extra);
+      timed_gather([&] { launch_gather_all<T, 2>(s, g, el, lo, t, o, rungs0, rungs1, out0, out1, on, extra); });
   } else {
     enqueue_cube_rungs<T>(s, g, CubeOp::kMax, in, rungs0, nullptr, rung_win, K_max, F, mw);
-    launch_gather_all<T, 0>(s, g, el, lo, t, o, rungs0, nullptr, out0, nullptr, on, extra);
+    timed_gather([&] { launch_gather_all<T, 0>(s, g, el, lo, t, o, rungs0, nullptr, out0, nullptr, on, extra); });
   }
 }
 

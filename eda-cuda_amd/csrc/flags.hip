@@ -453,9 +453,19 @@ void enqueue_sig_flags(hipStream_t s, const Geom& g, const T* r, const T* tau, u
       const int64_t wgs = g.batch * t.groups;
       if (wgs > 0x7fffffff) throw std::invalid_argument("eda: too many workgroups for the extrema pass");
       t.xcd = t.inner > 1 ? xcd_chunk(wgs) : 0;
+      const bool timed = timer_family(kTimeFlags);
+      hipEvent_t t0 = nullptr;
+      if (timed) timer_note(s, &t0);
       EDA_GO(k_flags_tile<T>, dim3(static_cast<unsigned>(wgs)), dim3(kThreads), bytes(P), s, t, r, tau,
                          flags, ext, on);
       EDA_LAUNCH("k_flags_tile");
+      if (timed) {
+        // r and tau read, the flags read (after the first axis) and written, over the running elements
+        int64_t running = on != nullptr ? mirror_active(on, g.batch) : g.batch;
+        if (running < 0) running = g.batch;
+        const double per = 2.0 * sizeof(T) + (first ? 1.0 : 2.0);
+        timer_done(s, t0, per * static_cast<double>(running) * static_cast<double>(g.count), kTagFlags);
+      }
       first = false;
     }
     if (!first) return;

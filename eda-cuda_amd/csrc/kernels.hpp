@@ -24,23 +24,37 @@ struct MethodCtl {
 };
 MethodCtl method_ctl(const Geom& g, bool f32);
 
-// ---- the pass timer (the benchmark's live roofline figure) ---------------------------------------
-// While enabled, every launch of the cube filter's pass kernel (k_vhgw) is bracketed by two HIP
-// events on its stream, and its algorithmic bytes are counted: the pass's input field(s) read once
-// and output field(s) written once, over the elements whose window is non-zero. The count of those
-// elements comes from the host mirror of the window array (no device read-back).
-// kind 2 times the DCT's axis passes instead (the fast transform of power-of-two axes and the dense
-// contraction of the others), each counting one read and one write of the field.
-void timer_set(int kind);
-bool timer_on();
-bool timer_gemm_on();
-void timer_note(hipStream_t s, hipEvent_t* start);                      // before a launch
-void timer_done(hipStream_t s, hipEvent_t start, double bytes);         // after it
-void timer_read(double* ms, int64_t* launches, double* bytes);          // synchronizes, sums, clears
-// Host mirrors of device window arrays (registered by the driver's uploads): the active count of
-// win[0 .. n) for the timer, or -1 when the array is not mirrored.
+// ---- the kernel timer (the benchmark's live roofline figures) -----------------------------------
+// While a family is enabled, every launch of its kernels is bracketed by two HIP events on its
+// stream and its algorithmic bytes are counted (each distinct input field read once, each output
+// written once, over the elements that run; the count of those comes from the host mirror of the
+// per-element arrays, with no device read-back). Families (bits of the mask): 1 the cube filter's
+// passes (k_vhgw_rows, k_vhgw, k_box), 2 the DCT's axis passes (k_dct_fft, the dense contraction),
+// 4 the rung gathers (k_gather_all), 8 the significant-extrema tiles (k_flags_tile).
+enum TimerFamily : int { kTimeVhgw = 1, kTimeDct = 2, kTimeGather = 4, kTimeFlags = 8 };
+enum TimerTag : int {
+  kTagRows = 0,     // k_vhgw_rows
+  kTagStrided,      // k_vhgw (the staged pass)
+  kTagBoxShort,     // k_box, windows up to 31 samples
+  kTagBoxLong,      // k_box, longer windows
+  kTagDct,          // k_dct_fft and the dense contraction
+  kTagGather,       // k_gather_all
+  kTagFlags,        // k_flags_tile
+  kTagCount
+};
+void timer_set(int mask);
+bool timer_on();                                                            // family 1
+bool timer_gemm_on();                                                       // family 2
+bool timer_family(int family);
+void timer_note(hipStream_t s, hipEvent_t* start);                          // before a launch
+void timer_done(hipStream_t s, hipEvent_t start, double bytes, int tag);    // after it
+void timer_read(double* ms, int64_t* launches, double* bytes);              // synchronizes, sums, clears
+// Host mirrors of device parameter arrays (registered by the driver's uploads while a timer runs):
+// the host copy of [dev, dev + bytes), or nullptr when the range is not mirrored.
 void mirror_note(const void* dev, const void* host, size_t bytes);
 void mirror_clear();
+const void* mirror_host(const void* dev, size_t bytes);
+// the number of non-zero entries of win[0 .. n), or -1 when the array is not mirrored
 int64_t mirror_active(const int32_t* win, int64_t n);
 
 // ---- the cube filter (vhgw.hip) ------------------------------------------------------------------

@@ -286,7 +286,7 @@ void axis_transform(hipStream_t s, const Geom& g, int a, const T* M, const T* in
   hipEvent_t t0 = nullptr;
   if (timed) timer_note(s, &t0);
   axis_gemm<T>(s, g, a, M, in, out, on);
-  if (timed) timer_done(s, t0, 2.0 * static_cast<double>(g.batch) * g.count * sizeof(T));
+  if (timed) timer_done(s, t0, 2.0 * static_cast<double>(g.batch) * g.count * sizeof(T), kTagDct);
 }
 
 // ---- the fast transform of power-of-two axes --------------------------------------------------------
@@ -883,7 +883,7 @@ void fft_transform(hipStream_t s, int device, const Geom& g, int a, const T* in,
       else go(k_dct_fft<T, 0, 0>);
   }
   EDA_LAUNCH("k_dct_fft");
-  if (timed) timer_done(s, t0, 2.0 * static_cast<double>(g.batch) * g.count * sizeof(T));
+  if (timed) timer_done(s, t0, 2.0 * static_cast<double>(g.batch) * g.count * sizeof(T), kTagDct);
 }
 
 // ---- matrices -----------------------------------------------------------------------------------

@@ -37,13 +37,17 @@ struct SelCand {
   int64_t slots;
 };
 
-struct alignas(16) SelState {
-  uint64_t prefix[2];
-  uint64_t rank[2];
-  int64_t count;
-  double gamma;
-  int32_t valid;
-  int32_t same;
+// Per element, the search state of the two ranks lo <= hi (median: the middle pair; quantile: the
+// lerp's ends): key[r] holds the digits of rank r's value found so far, left[r] its rank among the
+// values that share those digits. `merged` while both ranks share their digits (the histogram kernel
+// then counts one row for both).
+struct alignas(16) RankState {
+  uint64_t key[2];
+  uint64_t left[2];
+  int64_t n;
+  double frac;
+  int32_t live;
+  int32_t merged;
 };
 
 __host__ __device__ __forceinline__ uint64_t order_key(double d) {
@@ -91,14 +95,14 @@ __device__ __forceinline__ void sel_values(const SelArgs<T>& a, int64_t b, uint3
   }
 }
 
-__global__ void k_sel_init(int64_t B, SelState* st, uint64_t* hist, uint32_t* ncand, const int32_t* on) {
+__global__ void k_sel_init(int64_t B, RankState* st, uint64_t* hist, uint32_t* ncand, const int32_t* on) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= B * 2 * kBins) return;
   const int64_t b = i / (2 * kBins);
   if (on != nullptr && on[b] == 0) return;
   hist[i] = 0;
   if (i % (2 * kBins) == 0) {
-    st[b] = SelState{};
+    st[b] = RankState{};
     ncand[b] = 0;
   }
 }
@@ -106,16 +110,16 @@ __global__ void k_sel_init(int64_t B, SelState* st, uint64_t* hist, uint32_t* nc
 // phase 0 counts every value; 1 also keeps the values that match a prefix as candidates; 2 counts
 // the candidates when the list holds them all (else every value).
 template <class T, int PHASE>
-__global__ void __launch_bounds__(kThreads) k_sel_hist(SelArgs<T> a, const SelState* st, uint64_t* hist, int32_t nblk,
+__global__ void __launch_bounds__(kThreads) k_sel_hist(SelArgs<T> a, const RankState* st, uint64_t* hist, int32_t nblk,
                                                        int32_t shift, int32_t width, int32_t first, const int32_t* on,
                                                        SelCand cand) {
   __shared__ uint32_t h[2][kBins];
   const int64_t b = blockIdx.x / nblk;
   const int64_t chunk = blockIdx.x % nblk;
   if (on != nullptr && on[b] == 0) return;
-  const SelState S = st[b];
-  if (!first && !S.valid) return;
-  const int nr = (first || S.same) ? 1 : 2;
+  const RankState S = st[b];
+  if (!first && !S.live) return;
+  const int nr = (first || S.merged) ? 1 : 2;
   for (int i = threadIdx.x; i < 2 * kBins; i += kThreads) h[i / kBins][i % kBins] = 0u;
   __syncthreads();
   const int hi = shift + width;
@@ -141,8 +145,8 @@ __global__ void __launch_bounds__(kThreads) k_sel_hist(SelArgs<T> a, const SelSt
       count_in(0, bin);
       return true;
     }
-    const bool m0 = (k >> hi) == (S.prefix[0] >> hi);
-    const bool m1 = nr == 2 && (k >> hi) == (S.prefix[1] >> hi);
+    const bool m0 = (k >> hi) == (S.key[0] >> hi);
+    const bool m1 = nr == 2 && (k >> hi) == (S.key[1] >> hi);
     if (m0) count_in(0, bin);
     if (m1) count_in(1, bin);
     return m0 || m1;
@@ -208,124 +212,106 @@ __device__ __forceinline__ void quantile_index(int64_t n, double q, int64_t* i, 
 }
 
 // The exclusive prefix sum of v over the workgroup's threads in thread order, and the total
-// (integers: exact in any order). wsum: kThreads / 64 LDS words.
-__device__ __forceinline__ uint64_t block_exclusive(uint64_t v, uint64_t* total, uint64_t* wsum) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  unsigned long x = v;
+// The inclusive prefix sum of v over the lanes of a wave (integers: exact in any order).
+__device__ __forceinline__ uint64_t wave_inclusive(uint64_t v) {
+  const int lane = threadIdx.x & 63;
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) {
-    const unsigned long y = __shfl_up(x, d);
-    if (lane >= d) x += y;
+    const uint64_t y = __shfl_up(v, d);
+    v += lane >= d ? y : 0;
   }
-  if (lane == 63) wsum[wave] = x;
-  __syncthreads();
-  uint64_t off = 0, tot = 0;
-#pragma unroll
-  for (int w = 0; w < kThreads / 64; ++w) {
-    const uint64_t t = wsum[w];
-    off += w < wave ? t : 0;
-    tot += t;
-  }
-  __syncthreads();
-  *total = tot;
-  return off + x - v;
+  return v;
 }
 
-__global__ void __launch_bounds__(kThreads) k_sel_pick(SelState* st, uint64_t* hist, int32_t shift, int32_t width,
-                                                       int32_t first, int32_t mode, double q, const int32_t* on) {
-  __shared__ uint64_t wsum[kThreads / 64];
-  __shared__ SelState S;
-  __shared__ int32_t found_bin;
-  __shared__ uint64_t found_rank;
+// One step of the rank search, one wave per element. Lane l owns the bins [l per, (l + 1) per) of
+// the step's digit (per = 32 for 11-bit digits, 8 for the last 9 bits). The lanes' bin totals are
+// scanned across the wave; the first lane whose running total passes a rank holds that rank's bin,
+// walks its own bins to find it and broadcasts the bin and the rank left inside it. Every lane clears
+// the bins it read (the next step counts into them again). After the last digit the wave writes the
+// element's order statistic: the median as numpy.median (the mean of the middle pair), the quantile
+// as numpy's 'linear' lerp; an empty value set gives NaN and count 0.
+__global__ void __launch_bounds__(64) k_rank_step(RankState* st, uint64_t* hist, int32_t shift, int32_t width,
+                                                  int32_t first, int32_t last, int32_t mode, double q,
+                                                  const int32_t* on, double* out, int64_t* count) {
   const int64_t b = blockIdx.x;
   if (on != nullptr && on[b] == 0) return;
-  const int tid = threadIdx.x;
-  if (tid == 0) S = st[b];
-  __syncthreads();
-  if (!first && !S.valid) return;
-  const int nb = 1 << width;
-  const int per = nb / kThreads;
-  uint64_t* h0 = hist + (b * 2) * kBins;
+  const int lane = threadIdx.x;
+  RankState S = st[b];
+  if (!first && !S.live) return;
+  const int per = (1 << width) >> 6;
+  uint64_t* row[2] = {hist + (b * 2) * kBins, hist + (b * 2 + 1) * kBins};
+  const auto lane_total = [&](const uint64_t* h) {
+    uint64_t t = 0;
+    for (int j = 0; j < per; ++j) t += h[lane * per + j];
+    return t;
+  };
   if (first) {
-    uint64_t sum = 0;
-    for (int j = 0; j < per; ++j) sum += h0[tid * per + j];
-    uint64_t n = 0;
-    block_exclusive(sum, &n, wsum);
-    if (tid == 0) {
-      S.count = static_cast<int64_t>(n);
-      S.valid = n > 0;
-      S.prefix[0] = S.prefix[1] = 0;
-      S.same = 1;
-      if (n > 0) {
-        if (mode == 0) {
-          S.rank[0] = (n - 1) / 2;
-          S.rank[1] = n / 2;
-        } else {
-          int64_t i = 0, j = 0;
-          double gm = 0.0;
-          quantile_index(static_cast<int64_t>(n), q, &i, &j, &gm);
-          S.gamma = gm;
-          S.rank[0] = static_cast<uint64_t>(i);
-          S.rank[1] = static_cast<uint64_t>(j);
-        }
+    const uint64_t n = static_cast<uint64_t>(__shfl(wave_inclusive(lane_total(row[0])), 63));
+    S = RankState{};
+    S.n = static_cast<int64_t>(n);
+    S.live = n > 0 ? 1 : 0;
+    S.merged = 1;
+    if (n == 0) {
+      if (lane == 0) {
+        st[b] = S;
+        out[b] = __longlong_as_double(0x7ff8000000000000LL);
+        if (count != nullptr) count[b] = 0;
       }
-    }
-    __syncthreads();
-    if (!S.valid) {
-      if (tid == 0) st[b] = S;
       return;
+    }
+    if (mode == 0) {
+      S.left[0] = (n - 1) / 2;
+      S.left[1] = n / 2;
+    } else {
+      int64_t i = 0, j = 0;
+      quantile_index(static_cast<int64_t>(n), q, &i, &j, &S.frac);
+      S.left[0] = static_cast<uint64_t>(i);
+      S.left[1] = static_cast<uint64_t>(j);
     }
   }
   for (int r = 0; r < 2; ++r) {
-    const uint64_t* hr = hist + (b * 2 + (S.same ? 0 : r)) * kBins;
-    uint64_t sum = 0;
-    for (int j = 0; j < per; ++j) sum += hr[tid * per + j];
-    uint64_t tot = 0;
-    const uint64_t base = block_exclusive(sum, &tot, wsum);
-    const uint64_t rank = S.rank[r];
-    if (rank >= base && rank < base + sum) {
-      uint64_t acc = base;
+    const uint64_t* h = row[S.merged ? 0 : r];
+    const uint64_t incl = wave_inclusive(lane_total(h));
+    const uint64_t want = S.left[r];
+    const uint64_t past = __ballot(want < incl);  // lanes whose running total passes the rank
+    const int owner = __ffsll(static_cast<unsigned long long>(past)) - 1;
+    uint32_t bin = 0;
+    uint64_t rest = 0;
+    if (lane == owner) {
+      uint64_t below = incl;
+      for (int j = per - 1; j >= 0; --j) below -= h[lane * per + j];  // the running total before the lane
       for (int j = 0; j < per; ++j) {
-        const uint64_t c = hr[tid * per + j];
-        if (rank < acc + c) {
-          found_bin = tid * per + j;
-          found_rank = rank - acc;
+        const uint64_t c = h[lane * per + j];
+        if (want < below + c) {
+          bin = static_cast<uint32_t>(lane * per + j);
+          rest = want - below;
           break;
         }
-        acc += c;
+        below += c;
       }
     }
-    __syncthreads();
-    if (tid == 0) {
-      S.prefix[r] |= static_cast<uint64_t>(found_bin) << shift;
-      S.rank[r] = found_rank;
-    }
-    __syncthreads();
+    bin = static_cast<uint32_t>(__shfl(static_cast<int>(bin), owner));
+    rest = static_cast<uint64_t>(__shfl(static_cast<unsigned long long>(rest), owner));
+    S.key[r] |= static_cast<uint64_t>(bin) << shift;
+    S.left[r] = rest;
   }
-  for (int i = tid; i < 2 * kBins; i += kThreads) h0[i] = 0;
-  if (tid == 0) {
-    S.same = S.prefix[0] == S.prefix[1];
-    st[b] = S;
+  for (int j = 0; j < per; ++j) {
+    row[0][lane * per + j] = 0;
+    row[1][lane * per + j] = 0;
   }
-}
-
-__global__ void k_sel_final(int64_t B, const SelState* st, int32_t mode, double* out, int64_t* count,
-                            const int32_t* on) {
-  const int64_t b = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (b >= B || (on != nullptr && on[b] == 0)) return;
-  const SelState S = st[b];
-  if (count != nullptr) count[b] = S.valid ? S.count : 0;
-  if (!S.valid) {
-    out[b] = __longlong_as_double(0x7ff8000000000000LL);
-    return;
-  }
-  const double a = key_value(S.prefix[0]);
-  const double c = key_value(S.prefix[1]);
+  S.merged = S.key[0] == S.key[1] ? 1 : 0;
+  if (lane != 0) return;
+  st[b] = S;
+  if (!last) return;
+  if (count != nullptr) count[b] = S.n;
+  const double lo = key_value(S.key[0]);
+  const double hi = key_value(S.key[1]);
   if (mode == 0) {
-    out[b] = S.count % 2 ? a : (a + c) / 2.0;
+    out[b] = (S.n & 1) ? lo : (lo + hi) / 2.0;
+  } else if (S.frac >= 0.5) {
+    out[b] = hi - (hi - lo) * (1.0 - S.frac);
   } else {
-    const double diff = c - a;
-    out[b] = S.gamma >= 0.5 ? c - diff * (1.0 - S.gamma) : a + diff * S.gamma;
+    out[b] = lo + (hi - lo) * S.frac;
   }
 }
 
@@ -341,7 +327,7 @@ int64_t cand_slots(const Geom& g) {
 }
 
 // the selection's state, histograms, candidate counts and lists
-size_t sel_state_bytes(const Geom& g) { return align_up(static_cast<size_t>(g.batch) * sizeof(SelState)); }
+size_t sel_state_bytes(const Geom& g) { return align_up(static_cast<size_t>(g.batch) * sizeof(RankState)); }
 size_t sel_hist_bytes(const Geom& g) { return align_up(static_cast<size_t>(g.batch) * 2 * kBins * sizeof(uint64_t)); }
 size_t sel_count_bytes(const Geom& g) { return align_up(static_cast<size_t>(g.batch) * sizeof(uint32_t)); }
 
@@ -369,7 +355,7 @@ void enqueue_select(hipStream_t s, const Geom& g, const Selection<T>& sel, int m
   }
   const int64_t B = g.batch;
   char* w = static_cast<char*>(ws);
-  SelState* st = reinterpret_cast<SelState*>(w);
+  RankState* st = reinterpret_cast<RankState*>(w);
   uint64_t* hist = reinterpret_cast<uint64_t*>(w + sel_state_bytes(g));
   SelCand cand{};
   cand.n = reinterpret_cast<uint32_t*>(w + sel_state_bytes(g) + sel_hist_bytes(g));
@@ -399,14 +385,17 @@ void enqueue_select(hipStream_t s, const Geom& g, const Selection<T>& sel, int m
         EDA_GO((k_sel_hist<T, 0>), grid, dim3(kThreads), 0, s, a, st, hist, static_cast<int32_t>(nblk), shift,
                width, fst, on, cand);
       EDA_LAUNCH("k_sel_hist");
-      EDA_GO(k_sel_pick, dim3(static_cast<unsigned>(B)), dim3(kThreads), 0, s, st, hist, shift, width,
-                         p == 0 ? 1 : 0, mode, q, on);
-      EDA_LAUNCH("k_sel_pick");
+      EDA_GO(k_rank_step, dim3(static_cast<unsigned>(B)), dim3(64), 0, s, st, hist, shift, width, fst,
+             p == kPasses - 1 ? 1 : 0, mode, q, on, out, count);
+      EDA_LAUNCH("k_rank_step");
     }
+  } else {
+    flat(s, B, [=] __device__(int64_t b) {
+      if (on != nullptr && on[b] == 0) return;
+      out[b] = __longlong_as_double(0x7ff8000000000000LL);
+      if (count != nullptr) count[b] = 0;
+    });
   }
-  EDA_GO(k_sel_final, dim3(static_cast<unsigned>((B + kThreads - 1) / kThreads)), dim3(kThreads), 0, s, B,
-                     st, mode, out, count, on);
-  EDA_LAUNCH("k_sel_final");
 }
 
 // sigma = (1.4826 MAD(d)) / sqrt 2 (reference stats.cu:262-334): pooled over the axes, or the

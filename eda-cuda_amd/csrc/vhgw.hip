@@ -1480,6 +1480,7 @@ void enqueue_vhgw(hipStream_t s, const Geom& g, int axis, int nf, bool same_inpu
   }
   hipEvent_t t0 = nullptr;
   const bool timed = timer_on();
+  int tag = kTagRows;
   if (rows_qualify(g, axis, sum)) {
     if (timed) timer_note(s, &t0);
     launch_rows<T>(s, g, axis, nf, same_input, ops, p, rungs, il);
@@ -1542,8 +1543,31 @@ void enqueue_vhgw(hipStream_t s, const Geom& g, int axis, int nf, bool same_inpu
         pf2 = box_reads<T>(a2, 64 * s2.W);
       }
       if (pf1 && pf2) {
+        // the timer counts each launch's own items: windows of radius < 16 run in the first
+        const auto items = [&](bool longer_ones) {
+          const int32_t* h = static_cast<const int32_t*>(
+              mirror_host(win, static_cast<size_t>(g.batch * rungs) * sizeof(int32_t)));
+          if (h == nullptr) return static_cast<double>(g.batch * rungs);
+          int64_t c = 0;
+          for (int64_t k = 0; k < g.batch * rungs; ++k) {
+            const int32_t rw = std::min((h[k] - 1) / 2, a.n - 1);
+            c += h[k] >= 1 && (rw >= 16) == longer_ones ? 1 : 0;
+          }
+          return static_cast<double>(c);
+        };
         launch_box<T, false>(pf1, s, a1, p, rungs, s1.W, s1.lds);
-        if (longer) launch_box<T, true>(pf2, s, a2, p, rungs, s2.W, s2.lds);
+        if (timed) {
+          timer_done(s, t0, 2.0 * items(false) * g.count * sizeof(T), kTagBoxShort);
+          t0 = nullptr;
+        }
+        if (longer) {
+          if (timed) timer_note(s, &t0);
+          launch_box<T, true>(pf2, s, a2, p, rungs, s2.W, s2.lds);
+          if (timed) {
+            timer_done(s, t0, 2.0 * items(true) * g.count * sizeof(T), kTagBoxLong);
+            t0 = nullptr;
+          }
+        }
         EDA_LAUNCH("k_box");
         done = true;
       }
@@ -1554,16 +1578,17 @@ void enqueue_vhgw(hipStream_t s, const Geom& g, int axis, int nf, bool same_inpu
       const int64_t wgs = g.batch * v.groups;
       const dim3 grid(static_cast<unsigned>(wgs), static_cast<unsigned>(rungs));
       const dim3 block(static_cast<unsigned>(64 * sh.W));
+      tag = kTagStrided;
       if (sum) launch_kt<T, T>(1, grid, block, sh.lds, s, v, p);
       else launch_kt<T, okey_t<T>>(kt_for(v.n), grid, block, sh.lds, s, v, p);
       EDA_LAUNCH("k_vhgw");
     }
   }
-  if (timed) {
+  if (timed && t0 != nullptr) {
     int64_t active = mirror_active(win, g.batch * rungs);
     if (active < 0) active = g.batch * rungs;
     const int fields = (nf == 2 && !same_input ? 2 : 1) + nf;
-    timer_done(s, t0, static_cast<double>(fields) * active * g.count * sizeof(T));
+    timer_done(s, t0, static_cast<double>(fields) * active * g.count * sizeof(T), tag);
   }
 }
 

@@ -99,116 +99,133 @@ class Monogenic:
 
 
 # ----------------------------------------------------------------------------------------------
-# Arguments (reference src/eda/_api.py:433-526)
+# Arguments. The accepted types, the clamping and the exception classes and messages are the
+# reference's interface (src/eda/_api.py:433-526); the checks themselves are written here.
+
+_MESSAGES = {
+    "bool_int": "eda: {name} must be an integer, not bool",
+    "not_int": "eda: {name} must be an integer, got {kind}",
+    "sifts_range": "eda: sifts must be an integer from 1 to {top}, got {value}",
+    "tie_type": "eda: tie_tolerance must be a real number or None, got {kind}",
+    "tie_range": "eda: tie_tolerance must be a finite number >= 0, got {value}",
+    "flag": "eda: {name} must be True or False, got {kind}",
+    "no_axes": "eda: expected an array with at least one dimension",
+    "all_axes": (
+        "eda: axes=None takes every axis and needs x.ndim <= 3, got {ndim}: name the spatial axes "
+        "(1 to 3) with axes="
+    ),
+    "axes_type": "eda: axes must be an int, a sequence of ints or None, got {kind}",
+    "axes_unique": "eda: axes must be unique, got {value}",
+    "axes_count": "eda: 1 to 3 spatial axes, got {count}",
+}
+_BOOLS = (bool, np.bool_)
+_REALS = (int, float, np.integer, np.floating)
+
+
+def _fail(exc: type[Exception], key: str, **fields: Any) -> Exception:
+    return exc(_MESSAGES[key].format(**fields))
+
+
+def _index_of(value: Any, name: str) -> int:
+    """``value`` as a Python int through ``__index__`` (bool refused)."""
+    if isinstance(value, _BOOLS):
+        raise _fail(TypeError, "bool_int", name=name)
+    hook = getattr(type(value), "__index__", None)
+    if hook is None:
+        raise _fail(TypeError, "not_int", name=name, kind=type(value).__name__)
+    return int(hook(value))
 
 
 def _max_imf(value: Any) -> int:
-    if isinstance(value, (bool, np.bool_)):
-        raise TypeError("eda: max_imf must be an integer, not bool")
-    try:
-        number = operator.index(value)
-    except TypeError:
-        raise TypeError(f"eda: max_imf must be an integer, got {type(value).__name__}") from None
-    return max(-_INT32_MAX - 1, min(number, _INT32_MAX))
+    # clamped into int32: any bound at or below 0 means "no limit", any above 2^31 - 1 is unreachable
+    return int(np.clip(_index_of(value, "max_imf"), -(_INT32_MAX + 1), _INT32_MAX))
 
 
 def _sifts(value: Any) -> int:
-    if isinstance(value, (bool, np.bool_)):
-        raise TypeError("eda: sifts must be an integer, not bool")
-    try:
-        number = operator.index(value)
-    except TypeError:
-        raise TypeError(f"eda: sifts must be an integer, got {type(value).__name__}") from None
-    if not 1 <= number <= _INT32_MAX:
-        raise ValueError(f"eda: sifts must be an integer from 1 to {_INT32_MAX}, got {number}")
+    number = _index_of(value, "sifts")
+    if number < 1 or number > _INT32_MAX:
+        raise _fail(ValueError, "sifts_range", top=_INT32_MAX, value=number)
     return number
 
 
 def _tie_tolerance(value: Any) -> float | None:
     if value is None:
         return None
-    if isinstance(value, (bool, np.bool_)) or not isinstance(value, (int, float, np.integer, np.floating)):
-        raise TypeError(f"eda: tie_tolerance must be a real number or None, got {type(value).__name__}")
-    try:
-        number = float(value)
-    except OverflowError:
-        number = math.inf
-    if not (math.isfinite(number) and number >= 0):
-        raise ValueError(f"eda: tie_tolerance must be a finite number >= 0, got {value}")
+    if isinstance(value, _BOOLS) or not isinstance(value, _REALS):
+        raise _fail(TypeError, "tie_type", kind=type(value).__name__)
+    # an integer too large for a double counts as infinite, which the range check refuses
+    too_big = isinstance(value, int) and abs(value) >= 2**1024
+    number = math.inf if too_big else float(value)
+    if math.isnan(number) or math.isinf(number) or number < 0:
+        raise _fail(ValueError, "tie_range", value=value)
     return number
 
 
 def _flag(value: Any, name: str) -> bool:
-    if not isinstance(value, (bool, np.bool_)):
-        raise TypeError(f"eda: {name} must be True or False, got {type(value).__name__}")
-    return bool(value)
+    if isinstance(value, _BOOLS):
+        return bool(value)
+    raise _fail(TypeError, "flag", name=name, kind=type(value).__name__)
 
 
 def _dtype_name(dtype: Any) -> str:
+    """The NumPy name of a dtype; a framework dtype (``torch.float32``) by its own name."""
     try:
         return np.dtype(dtype).name
-    except TypeError:  # a framework dtype such as torch.float32
-        return str(dtype).removeprefix("torch.")
-
-
-def _axis(value: Any) -> int:
-    if isinstance(value, (bool, np.bool_)):
-        raise TypeError("bool")
-    return operator.index(value)
+    except TypeError:
+        text = str(dtype)
+        return text[len("torch."):] if text.startswith("torch.") else text
 
 
 def _normalize_axes(axes: Any, ndim: int) -> tuple[int, ...]:
+    """The spatial axes of an ``ndim``-dimensional input as non-negative ints, in the given order."""
     if ndim < 1:
-        raise ValueError("eda: expected an array with at least one dimension")
+        raise _fail(ValueError, "no_axes")
     if axes is None:
-        if ndim > 3:
-            raise ValueError(
-                f"eda: axes=None takes every axis and needs x.ndim <= 3, got {ndim}: name the "
-                "spatial axes (1 to 3) with axes="
-            )
-        return tuple(range(ndim))
-    if isinstance(axes, (int, np.integer)) and not isinstance(axes, bool):
-        axes = (axes,)
+        if ndim <= 3:
+            return tuple(range(ndim))
+        raise _fail(ValueError, "all_axes", ndim=ndim)
+    one = isinstance(axes, (int, np.integer)) and not isinstance(axes, bool)
+    items = [axes] if one else axes
     try:
-        listed = [_axis(a) for a in axes]
+        given = [operator.index(a) if not isinstance(a, _BOOLS) else None for a in items]
     except TypeError:
-        raise TypeError(f"eda: axes must be an int, a sequence of ints or None, got {type(axes).__name__}") from None
-    out = []
-    for a in listed:
-        if not -ndim <= a < ndim:
-            raise np.exceptions.AxisError(a, ndim, "eda")
-        out.append(a % ndim)
-    if len(set(out)) != len(out):
-        raise ValueError(f"eda: axes must be unique, got {tuple(listed)}")
-    if not 1 <= len(out) <= 3:
-        raise ValueError(f"eda: 1 to 3 spatial axes, got {len(out)}")
-    return tuple(out)
+        given = [None]
+    if None in given:
+        raise _fail(TypeError, "axes_type", kind=type(axes).__name__)
+    bad = next((a for a in given if a < -ndim or a >= ndim), None)
+    if bad is not None:
+        raise np.exceptions.AxisError(bad, ndim, "eda")
+    wrapped = tuple(a + ndim if a < 0 else a for a in given)
+    if len(frozenset(wrapped)) < len(wrapped):
+        raise _fail(ValueError, "axes_unique", value=tuple(given))
+    if len(wrapped) not in (1, 2, 3):
+        raise _fail(ValueError, "axes_count", count=len(wrapped))
+    return wrapped
 
 
 def _restore_order(ndim: int, batch_axes: tuple[int, ...], axes: tuple[int, ...], extra: int = 1) -> list[int]:
-    """The permutation taking a computed ``(*batch, *E, *spatial)`` array to the public layout
-    (reference src/eda/_api.py:539-555)."""
-    nb = len(batch_axes)
-
-    def computed_position(i: int) -> int:
-        return nb + extra + axes.index(i) if i in axes else batch_axes.index(i)
-
-    m = min(axes)
-    return (
-        [computed_position(f) for f in range(m)]
-        + [nb + j for j in range(extra)]
-        + [computed_position(f) for f in range(m, ndim)]
-    )
+    """The permutation (for ``transpose``) from a computed array laid out as ``(*batch, *E,
+    *spatial)`` (``extra`` new axes E, the spatial axes in the order of ``axes``) to the public
+    layout: the input's axes in their own order with E inserted just before ``min(axes)``."""
+    carried = list(batch_axes) + [None] * extra + list(axes)  # the input axis each computed axis holds
+    slot = {ax: pos for pos, ax in enumerate(carried) if ax is not None}
+    fresh = [pos for pos, ax in enumerate(carried) if ax is None]
+    split = min(axes)
+    return [slot[f] for f in range(split)] + fresh + [slot[f] for f in range(split, ndim)]
 
 
 def _error_summary(codes: np.ndarray, indices: np.ndarray, total: int) -> str:
-    parts = []
-    for code in sorted({int(c) for c in codes}):
-        where = indices[codes == code]
-        shown = ", ".join(str(int(i)) for i in where[:8]) + (", ..." if where.size > 8 else "")
-        text = _ERROR_MESSAGES.get(code, f"error code {code}")
-        parts.append(f"code {code} ({text}) at batch index {shown}")
-    return f"eda: the decomposition failed for {indices.size} of {total} element(s): " + "; ".join(parts)
+    """One message for the failed elements of a call, grouped by error code (at most 8 indices each)."""
+    groups = []
+    for code in np.unique(codes.astype(np.int64)):
+        hit = indices[codes == code]
+        listed = ", ".join(map(str, hit[:8].tolist()))
+        if hit.size > 8:
+            listed += ", ..."
+        reason = _ERROR_MESSAGES.get(int(code), f"error code {int(code)}")
+        groups.append(f"code {int(code)} ({reason}) at batch index {listed}")
+    head = f"eda: the decomposition failed for {indices.size} of {total} element(s): "
+    return head + "; ".join(groups)
 
 
 # ----------------------------------------------------------------------------------------------

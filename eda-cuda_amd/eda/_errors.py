@@ -23,11 +23,13 @@ class CudaError(RuntimeError):
 
 
 class DecompositionError(ValueError):
-    """Some elements of an ``eda.decompose`` call with ``errors="raise"`` failed.
+    """Raised by ``eda.decompose(..., errors="raise")`` when one or more batch elements could not be
+    decomposed; the elements that succeeded are not returned.
 
-    ``indices`` holds the flat (C-order) batch indices of the failed elements, ``codes`` their
-    error codes (1: the input holds inf or NaN and ``check_finite`` is on; 2: a non-finite
-    intermediate or output row) and ``info`` the ``DecompositionInfo`` of the call."""
+    Attributes: ``indices``, the failed elements' positions in the batch counted in C order;
+    ``codes``, one error code per failed element (1 when ``check_finite`` found inf or NaN in its
+    input, 2 when an intermediate field or an output row became non-finite); ``info``, the call's
+    ``DecompositionInfo``, whose ``error`` array holds the code of every element."""
 
     def __init__(self, message: str, indices: Any = (), codes: Any = (), info: Any = None) -> None:
         super().__init__(message)

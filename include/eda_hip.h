@@ -148,15 +148,19 @@ int eda_device_count(void);                                  /* 0 without a driv
 int eda_device_info(int32_t device, char* buf, size_t len);  /* a JSON object */
 int eda_pointer_device(const void* ptr, int32_t* device);    /* -1 for host memory */
 
-/* Kernel timing, for the live roofline figure of a benchmark: with on = 1, every launch of the
- * cube filter's pass kernel is bracketed by HIP events on the stream it runs on, and its algorithmic
- * bytes (input fields read once, output fields written once, over the running elements) are
- * counted; with on = 2, every axis pass of the DCT instead (one read and one write of the field
- * each); 0 turns it off. eda_timer_read waits for the recorded launches, returns the total kernel
- * time in ms, the launch count and the bytes, and clears them. No counterpart in the reference (it
- * has no timing in code). */
+/* Kernel timing, for the live roofline figures of a benchmark. `on` is a mask of kernel families:
+ * 1 the cube filter's pass kernels (k_vhgw_rows, k_vhgw, k_box), 2 the DCT's axis passes, 4 the
+ * rung gathers, 8 the significant-extrema tiles; 0 turns timing off. While a family is on, every
+ * launch of its kernels is bracketed by HIP events on the stream it runs on and its algorithmic
+ * bytes are counted (each distinct input field read once, each output field written once, over the
+ * running elements; a DCT axis pass one read and one write of the field). eda_timer_read waits for
+ * the recorded launches, returns their total kernel time in ms, the launch count and the bytes, and
+ * clears them (their per-kernel sums are kept for eda_timer_report). eda_timer_report writes the
+ * per-kernel sums since the last report as a JSON object {kernel: {"ms", "launches", "bytes"}}
+ * into buf and clears them. No counterpart in the reference (it has no timing in code). */
 int eda_timer_enable(int32_t on);
 int eda_timer_read(double* ms, int64_t* launches, double* bytes, char* err, size_t errlen);
+int eda_timer_report(char* buf, size_t len, char* err, size_t errlen);
 
 /* ---- Primitive entry points (device pointers; each call synchronizes before it returns). They
  * expose the method's stages one by one, so that each can be checked against its closed form and

@@ -7,6 +7,8 @@ tolerance."""
 
 from __future__ import annotations
 
+import zlib
+
 import numpy as np
 import pytest
 
@@ -27,7 +29,7 @@ def bits_equal(a, b):
 @pytest.mark.parametrize("shape", [(1000,), (3000,), (37, 70), (9, 10, 11), (512, 512)])
 @pytest.mark.parametrize("op", ["max", "min", "box"])
 def test_cube_is_bitwise_the_oracle(dtype, shape, op):
-    rng = np.random.default_rng(hash((shape, op)) % 2**32)
+    rng = np.random.default_rng(zlib.crc32(repr((shape, op)).encode()))
     B = 3
     x = rng.standard_normal((B,) + shape).astype(dtype)
     for w in (1, 3, 33, 65, 257, 2 * max(shape) + 5):

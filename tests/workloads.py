@@ -8,6 +8,13 @@
   U(0, n)^2, amplitude ~ U(1, 3)), then N(0, 0.05^2) noise; the field is cos(2 pi r / 6) +
   cos(2 pi (x' + 8 sin(2 pi y' / 128)) / 24) with (x', y') the coordinates rotated by theta, plus
   the blobs amp exp(-|p - c|^2 / (2 80^2)) and the noise, computed in float64, stored as float32.
+- plane_wave_volumes(B, n): B float32 n^3 volumes (the precision workload at n = 256); volume i
+  from default_rng(i), drawn in order: for the wavelengths 6, 20 and 64 voxels a unit direction
+  (three normal draws, normalized) and a phase ~ U(0, 2 pi); the envelope centre ~ U(0, n)^3; the
+  trend centre ~ U(0, n)^3 and amplitude ~ U(1, 3); then N(0, 0.02^2) noise. The field is
+  exp(-|p - c_e|^2 / (2 60^2)) * sum_k cos(2 pi u_k . p / lambda_k + phi_k) + a_t exp(-|p - c_t|^2 /
+  (2 (n / 2)^2)) + noise, computed in float64 per z-slab, stored as float32.
+- The differentiation workload is rings_stripes_blobs(32, 256).
 """
 
 from __future__ import annotations
@@ -42,4 +49,51 @@ def rings_stripes_blobs(B: int = 64, n: int = 512, dtype=np.float32, first: int 
         for (c, amp) in blobs:
             f = f + amp * np.exp(-((y - c[0]) ** 2 + (x - c[1]) ** 2) / (2 * 80.0**2))
         out[k] = (f + noise).astype(dtype)
+    return out
+
+
+WAVELENGTHS = (6.0, 20.0, 64.0)
+
+
+def envelope_centre(rng, n):
+    return rng.uniform(0, n, 3)
+
+
+def plane_wave_crop(i: int, n: int = 256, m: int = 64, dtype=np.float32) -> np.ndarray:
+    """The m^3 crop of volume i of plane_wave_volumes(., n) centred on its envelope (clipped to the
+    volume), where the waves are strongest."""
+    v = plane_wave_volumes(1, n, dtype, first=i)[0]
+    rng = np.random.default_rng(i)
+    for _ in WAVELENGTHS:
+        rng.normal(size=3)
+        rng.uniform(0.0, 2.0 * np.pi)
+    ce = envelope_centre(rng, n)
+    o = np.clip(np.round(ce).astype(int) - m // 2, 0, n - m)
+    return np.ascontiguousarray(v[o[0]:o[0] + m, o[1]:o[1] + m, o[2]:o[2] + m])
+
+
+def plane_wave_volumes(B: int = 4, n: int = 256, dtype=np.float32, first: int = 0) -> np.ndarray:
+    out = np.empty((B, n, n, n), dtype=dtype)
+    ax = np.arange(n, dtype=np.float64)
+    yy, xx = np.meshgrid(ax, ax, indexing="ij")
+    for k in range(B):
+        rng = np.random.default_rng(first + k)
+        waves = []
+        for lam in WAVELENGTHS:
+            u = rng.normal(size=3)
+            u = u / np.linalg.norm(u)
+            waves.append((u, lam, rng.uniform(0.0, 2.0 * np.pi)))
+        ce = envelope_centre(rng, n)
+        ct = rng.uniform(0, n, 3)
+        at = rng.uniform(1.0, 3.0)
+        noise_rng = rng
+        st = n / 2.0
+        for z in range(n):
+            d_e = (z - ce[0]) ** 2 + (yy - ce[1]) ** 2 + (xx - ce[2]) ** 2
+            d_t = (z - ct[0]) ** 2 + (yy - ct[1]) ** 2 + (xx - ct[2]) ** 2
+            w = np.zeros((n, n))
+            for u, lam, phi in waves:
+                w += np.cos(2.0 * np.pi * (u[0] * z + u[1] * yy + u[2] * xx) / lam + phi)
+            f = np.exp(-d_e / (2.0 * 60.0**2)) * w + at * np.exp(-d_t / (2.0 * st**2))
+            out[k, z] = (f + noise_rng.normal(0.0, 0.02, (n, n))).astype(dtype)
     return out
