@@ -1804,6 +1804,12 @@ int guarded(char* err, size_t len, F&& f) {
   }
 }
 
+}  // namespace
+
+int abi_guard(char* err, size_t errlen, const std::function<void()>& f) { return guarded(err, errlen, f); }
+
+namespace {
+
 void validate(const eda_array_desc& x, const eda_params& p) {
   if (x.dim < 1 || x.dim > 3) throw std::invalid_argument("eda: D must be 1, 2 or 3");
   if (x.dtype != EDA_F32 && x.dtype != EDA_F64) throw std::invalid_argument("eda: the input must be float32 or float64");

@@ -6,6 +6,8 @@
 
 #include "common.hpp"
 
+#include <functional>
+
 #include <cstdint>
 
 namespace eda {
@@ -23,6 +25,9 @@ struct MethodCtl {
   double tiny;
 };
 MethodCtl method_ctl(const Geom& g, bool f32);
+
+// Runs f, mapping its exceptions onto the C ABI's return codes and message (include/eda_hip.h).
+int abi_guard(char* err, size_t errlen, const std::function<void()>& f);
 
 // ---- the kernel timer (the benchmark's live roofline figures) -----------------------------------
 // While a family is enabled, every launch of its kernels is bracketed by two HIP events on its
@@ -96,10 +101,13 @@ struct Selection {
 };
 // mode 0: median (numpy.median); 1: quantile q (numpy 'linear'). out[b]: NaN for an empty set;
 // count[b] (nullptr: none): the size of the set.
+// slots: the candidate slots per element (-1: select_cand_slots(g); 0: every pass reads every
+// value); ws: select_ws_bytes(g, slots) bytes.
 template <class T>
 void enqueue_select(hipStream_t s, const Geom& g, const Selection<T>& sel, int mode, double q, double* out,
-                    int64_t* count, const int32_t* on, void* ws);
-size_t select_ws_bytes(const Geom& g);
+                    int64_t* count, const int32_t* on, void* ws, int64_t slots = -1);
+int64_t select_cand_slots(const Geom& g);
+size_t select_ws_bytes(const Geom& g, int64_t slots = -1);
 // Noise levels: (1.4826 MAD) / sqrt 2 of the forward differences, pooled over the axes or the
 // largest per axis. ws: select_ws_bytes + 3 [B] doubles.
 template <class T>

@@ -333,15 +333,17 @@ size_t sel_count_bytes(const Geom& g) { return align_up(static_cast<size_t>(g.ba
 
 }  // namespace
 
-size_t select_ws_bytes(const Geom& g) {
+int64_t select_cand_slots(const Geom& g) { return cand_slots(g); }
+size_t select_ws_bytes(const Geom& g, int64_t slots) {
+  if (slots < 0) slots = cand_slots(g);
   return sel_state_bytes(g) + sel_hist_bytes(g) + sel_count_bytes(g) +
-         align_up(static_cast<size_t>(g.batch) * static_cast<size_t>(cand_slots(g)) * sizeof(uint64_t));
+         align_up(static_cast<size_t>(g.batch) * static_cast<size_t>(slots) * sizeof(uint64_t));
 }
 size_t noise_ws_bytes(const Geom& g) { return select_ws_bytes(g) + 3 * align_up(static_cast<size_t>(g.batch) * 8); }
 
 template <class T>
 void enqueue_select(hipStream_t s, const Geom& g, const Selection<T>& sel, int mode, double q, double* out,
-                    int64_t* count, const int32_t* on, void* ws) {
+                    int64_t* count, const int32_t* on, void* ws, int64_t slots) {
   SelArgs<T> a{};
   a.sel = sel;
   a.g = g;
@@ -360,7 +362,7 @@ void enqueue_select(hipStream_t s, const Geom& g, const Selection<T>& sel, int m
   SelCand cand{};
   cand.n = reinterpret_cast<uint32_t*>(w + sel_state_bytes(g) + sel_hist_bytes(g));
   cand.keys = reinterpret_cast<uint64_t*>(w + sel_state_bytes(g) + sel_hist_bytes(g) + sel_count_bytes(g));
-  cand.slots = cand_slots(g);
+  cand.slots = slots < 0 ? cand_slots(g) : slots;
   const int64_t ni = B * 2 * kBins;
   EDA_GO(k_sel_init, dim3(static_cast<unsigned>((ni + kThreads - 1) / kThreads)), dim3(kThreads), 0, s, B,
                      st, hist, cand.n, on);
@@ -440,9 +442,9 @@ void enqueue_noise_sigma(hipStream_t s, const Geom& g, const T* r, bool per_axis
 }
 
 template void enqueue_select<float>(hipStream_t, const Geom&, const Selection<float>&, int, double, double*, int64_t*,
-                                    const int32_t*, void*);
+                                    const int32_t*, void*, int64_t);
 template void enqueue_select<double>(hipStream_t, const Geom&, const Selection<double>&, int, double, double*, int64_t*,
-                                     const int32_t*, void*);
+                                     const int32_t*, void*, int64_t);
 template void enqueue_noise_sigma<float>(hipStream_t, const Geom&, const float*, bool, double*, const int32_t*, void*);
 template void enqueue_noise_sigma<double>(hipStream_t, const Geom&, const double*, bool, double*, const int32_t*,
                                           void*);

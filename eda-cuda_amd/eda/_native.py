@@ -99,6 +99,8 @@ SIGNATURES = {
     "eda_prim_cube": ([_i32, _i32, _i64, _i64p, _i32, _i32p, _vp, _vp, _vp, *_err], _c.c_int),
     "eda_prim_stats": ([_i32, _i32, _i64, _i64p, _vp, _f64p, _f64p, _u64p, *_err], _c.c_int),
     "eda_prim_select": ([_i32, _i32, _i64, _i64p, _vp, _vp, _i32, _i32, _f64, _f64p, _i64p, *_err], _c.c_int),
+    "eda_prim_select_noalloc": ([_i32, _i32, _i64, _i64p, _vp, _vp, _i32, _i32, _f64, _vp, _vp, _u64, *_err],
+                                _c.c_int),
     "eda_prim_noise": ([_i32, _i32, _i64, _i64p, _vp, _i32, _f64p, *_err], _c.c_int),
     "eda_prim_tau": ([_i32, _i32, _i64, _i64p, _vp, _i32p, _i32p, _f64p, _f64p, _vp, _i32p, *_err], _c.c_int),
     "eda_prim_flags": ([_i32, _i32, _i64, _i64p, _vp, _vp, _vp, _u64p, *_err], _c.c_int),

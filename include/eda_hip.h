@@ -182,6 +182,17 @@ int eda_prim_stats(int32_t dtype, int32_t dim, int64_t batch, const int64_t* n, 
 int eda_prim_select(int32_t dtype, int32_t dim, int64_t batch, const int64_t* n, const void* field,
                     const uint8_t* mask, int32_t bits, int32_t mode, double q, double* out, int64_t* count, char* err,
                     size_t errlen);
+/* The same order statistics as eda_prim_select without allocating anything: no counterpart in
+ * the reference; for callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops that forbid hipMalloc). field, mask (may be NULL), out [batch] and count [batch]
+ * (may be NULL) are device pointers of the current device. The workspace is a static 32 MiB device
+ * array per GPU (held until the process ends); a call whose batch needs more for its fixed part
+ * (about 33 KiB per element: at most about 990 elements) fails with EDA_EINVAL. The kernels run on
+ * `stream` (a hipStream_t; 0: the null stream) and the call returns without waiting for them.
+ * Calls that overlap on one GPU share the array and race: the caller orders them. */
+int eda_prim_select_noalloc(int32_t dtype, int32_t dim, int64_t batch, const int64_t* n, const void* field,
+                            const uint8_t* mask, int32_t bits, int32_t mode, double q, double* out, int64_t* count,
+                            uint64_t stream, char* err, size_t errlen);
 /* The noise level (1.4826 MAD / sqrt 2 of the forward differences), pooled or the largest per axis
  * (reference cpp/src/method/stats.cu:299-334); out: host [batch]. */
 int eda_prim_noise(int32_t dtype, int32_t dim, int64_t batch, const int64_t* n, const void* r, int32_t per_axis,
