@@ -14,6 +14,7 @@
 // pinned staging area into a device parameter arena, reset after every synchronization.
 #include "common.hpp"
 #include "kernels.hpp"
+#include "tape.hpp"
 #include "../../include/eda_hip.h"
 
 #include <algorithm>
@@ -236,6 +237,24 @@ struct DevMem {
   }
 };
 
+}  // namespace
+
+TapeBuf::TapeBuf(size_t bytes) {
+  if (bytes == 0) return;
+  const hipError_t e = hipMalloc(&p, bytes);
+  if (e == hipErrorOutOfMemory) {
+    (void)hipGetLastError();
+    throw OutOfDeviceMemory("eda: the device memory does not hold the tape (" + std::to_string(bytes >> 20) +
+                            " MiB more)");
+  }
+  EDA_HIP(e);
+}
+TapeBuf::~TapeBuf() {
+  if (p) (void)hipFree(p);
+}
+
+namespace {
+
 // A bump allocator over one device allocation.
 struct Carve {
   char* base;
@@ -456,6 +475,7 @@ struct Run {
   std::vector<int32_t> n_imfs, error, windows, sifts;
   std::vector<double> spacing, energy;
   int32_t P = 1, K = 0;
+  std::unique_ptr<Tape> tape;  // with eda_params.keep_tape
 };
 
 // Per-element host state of a chunk.
@@ -803,6 +823,20 @@ class Chunk {
     });
   }
 
+  // Keep the decisions of every mode in t (the adjoint's tape).
+  void keep_tape(ChunkTape* t) { tape_ = t; }
+  // The per-element results the backward reads: the prescale exponents, the IMF counts, the errors.
+  void tape_results(ChunkTape& t, const Run& run, int64_t b0) const {
+    t.kexp.assign(B_, 0);
+    t.n_imfs.assign(B_, 0);
+    t.error.assign(B_, 0);
+    for (int64_t b = 0; b < B_; ++b) {
+      t.kexp[b] = rec_[b].kexp;
+      t.n_imfs[b] = run.n_imfs[b0 + b];
+      t.error[b] = run.error[b0 + b];
+    }
+  }
+
   void alloc_history() {
     hist_w_.assign(B_, std::vector<int32_t>(cap_ - 1, 0));
     hist_s_.assign(B_, std::vector<int32_t>(cap_ - 1, 0));
@@ -907,6 +941,7 @@ class Chunk {
   int32_t* d_rung_win_;
   double* d_oscr_;
   const int32_t* on_op_ = nullptr;
+  ChunkTape* tape_ = nullptr;
   std::vector<int32_t> rw_, rw_cur_;
   std::vector<int32_t> rung_max_;
   int K_max_ = 0;
@@ -1496,6 +1531,34 @@ bool Chunk<T>::mode() {
   if (timer_family(~0) && !rw_cur_.empty()) mirror_note(d_rung_win_, rw_cur_.data(), rw_cur_.size() * sizeof(int32_t));
   on = upload_on(active);
   on_op_ = on;
+  TapeMode* tm = nullptr;
+  if (tape_ != nullptr) {
+    tape_->modes.emplace_back();
+    tm = &tape_->modes.back();
+    tm->active.assign(B, 0);
+    tm->w.assign(B, 0);
+    tm->W.assign(B, 0);
+    tm->tol.assign(B, 0.0);
+    for (int64_t b = 0; b < B; ++b) {
+      const Rec& e = rec_[b];
+      if (e.status != 0) continue;
+      tm->active[b] = 1;
+      tm->w[b] = e.w;
+      tm->W[b] = e.W;
+      tm->tol[b] = e.tol;
+    }
+    tm->el = el;
+    tm->K_max = K_max_;
+    tm->rung_win = rw_;
+    tm->rung_max = rung_max_;
+    const size_t fb = static_cast<size_t>(B) * g.count;
+    tm->r = TapeBuf(fb * sizeof(T));
+    tm->L = TapeBuf(fb * sizeof(T));
+    tm->lo = TapeBuf(fb);
+    EDA_HIP(hipMemcpyAsync(tm->r.p, r_, fb * sizeof(T), hipMemcpyDeviceToDevice, s_));
+    EDA_HIP(hipMemcpyAsync(tm->L.p, L_, fb * sizeof(T), hipMemcpyDeviceToDevice, s_));
+    EDA_HIP(hipMemcpyAsync(tm->lo.p, lo_, fb, hipMemcpyDeviceToDevice, s_));
+  }
 
   // step 8: sift 1 and the continuation weight (reference driver.cpp:606-638; sift.cu:55-85)
   T* const* X = X_;
@@ -1651,11 +1714,30 @@ bool Chunk<T>::mode() {
     });
     std::vector<uint64_t> gm(2 * B);
     d2h(gm.data(), gmax, 2 * B);
+    std::vector<double> emed_h;
+    if (tm != nullptr) {
+      emed_h.resize(B);
+      d2h(emed_h.data(), d_e_med_, B);
+    }
     sync();
+    if (tm != nullptr) {
+      tm->ran.emplace_back(B, 0);
+      tm->acc.emplace_back(B, 0);
+      tm->thr.emplace_back(B, 0.0);
+    }
     for (int64_t b = 0; b < B; ++b) {
       Rec& er = rec_[b];
       if (!sifting(er)) continue;
       const bool ok = !(bits_d(gm[2 * b]) > er.r_inf) && !(bits_d(gm[2 * b + 1]) > er.r_inf);
+      if (tm != nullptr) {
+        // the floor as the device computed it: max(anorm_floor median(e), tol, tiny) in double
+        double thr = c.anorm_floor * emed_h[b];
+        thr = thr >= er.tol ? thr : er.tol;
+        thr = thr >= c.tiny ? thr : c.tiny;
+        tm->ran.back()[b] = 1;
+        tm->acc.back()[b] = ok ? 1 : 0;
+        tm->thr.back()[b] = thr;
+      }
       if (ok) ++er.n_sifts;
       else er.sifting = false;
     }
@@ -1740,6 +1822,11 @@ void run_call(const eda_array_desc& x, const eda_params& p, Run& run) {
     run.orient = DevMem(static_cast<size_t>(B) * (run.cap - 1) * (x.dim + 1) * gall.count * sizeof(T));
     run.kept_orient = true;
   }
+  if (p.keep_tape) {
+    run.tape = std::make_unique<Tape>();
+    run.tape->dtype = std::is_same_v<T, float> ? EDA_F32 : EDA_F64;
+    run.tape->c = c;
+  }
   // the chunk: bounded by max_chunk and by the free device memory (halving)
   int64_t Bc = std::min<int64_t>(B, 65535);  // the kernels put the elements of a chunk on grid.y
   if (p.max_chunk > 0) Bc = std::min<int64_t>(Bc, p.max_chunk);
@@ -1756,11 +1843,20 @@ void run_call(const eda_array_desc& x, const eda_params& p, Run& run) {
                                       static_cast<size_t>(b0) * (run.cap - 1) * (x.dim + 1) * g.count
                                 : nullptr;
     Chunk<T> ch(ctx, g, c, tau, p.check_finite != 0, p.max_imf, rows, run.cap, orient);
+    ChunkTape* ct = nullptr;
+    if (run.tape) {
+      run.tape->chunks.emplace_back();
+      ct = &run.tape->chunks.back();
+      ct->b0 = b0;
+      ct->g = g;
+      ch.keep_tape(ct);
+    }
     ch.alloc_history();
     ch.start(x, b0);
     while (ch.mode()) {
     }
     ch.finish(run, b0);
+    if (ct != nullptr) ch.tape_results(*ct, run, b0);
   }
   EDA_HIP(hipStreamSynchronize(ctx.stream));
   int32_t P = 0, K = 0;
@@ -2207,6 +2303,51 @@ int eda_timer_enable(int32_t on) {
 
 int eda_timer_read(double* ms, int64_t* launches, double* bytes, char* err, size_t errlen) {
   return guarded(err, errlen, [&] { timer_read(ms, launches, bytes); });
+}
+
+int eda_backward(const eda_result* r, const void* grad_rows, void* grad_x, int32_t flags, char* err, size_t errlen) {
+  return guarded(err, errlen, [&] {
+    if (r == nullptr || r->handle == nullptr) throw std::invalid_argument("eda: backward: no result");
+    Run& run = *static_cast<Run*>(r->handle);
+    if (!run.tape) throw std::invalid_argument("eda: backward: the call kept no tape (eda_params.keep_tape)");
+    if (run.batch == 0) return;
+    if (grad_rows == nullptr || grad_x == nullptr) throw std::invalid_argument("eda: backward: null pointer");
+    Context& ctx = context(run.device);
+    const bool orient = (flags & 1) == 0;
+    for (const ChunkTape& ct : run.tape->chunks) {
+      const size_t item = run.dtype == EDA_F32 ? 4 : 8;
+      const char* gr = static_cast<const char*>(grad_rows) + static_cast<size_t>(ct.b0) * run.P * run.count * item;
+      char* gx = static_cast<char*>(grad_x) + static_cast<size_t>(ct.b0) * run.count * item;
+      if (run.dtype == EDA_F32)
+        adjoint_chunk<float>(ctx.stream, run.device, *run.tape, ct, reinterpret_cast<const float*>(gr), run.P,
+                             reinterpret_cast<float*>(gx), orient);
+      else
+        adjoint_chunk<double>(ctx.stream, run.device, *run.tape, ct, reinterpret_cast<const double*>(gr), run.P,
+                              reinterpret_cast<double*>(gx), orient);
+    }
+  });
+}
+
+int eda_replay(const eda_result* r, const void* x, void* rows, char* err, size_t errlen) {
+  return guarded(err, errlen, [&] {
+    if (r == nullptr || r->handle == nullptr) throw std::invalid_argument("eda: replay: no result");
+    Run& run = *static_cast<Run*>(r->handle);
+    if (!run.tape) throw std::invalid_argument("eda: replay: the call kept no tape (eda_params.keep_tape)");
+    if (run.batch == 0) return;
+    if (x == nullptr || rows == nullptr) throw std::invalid_argument("eda: replay: null pointer");
+    Context& ctx = context(run.device);
+    for (const ChunkTape& ct : run.tape->chunks) {
+      const size_t item = run.dtype == EDA_F32 ? 4 : 8;
+      const char* xs = static_cast<const char*>(x) + static_cast<size_t>(ct.b0) * run.count * item;
+      char* rs = static_cast<char*>(rows) + static_cast<size_t>(ct.b0) * run.P * run.count * item;
+      if (run.dtype == EDA_F32)
+        replay_chunk<float>(ctx.stream, run.device, *run.tape, ct, reinterpret_cast<const float*>(xs),
+                            reinterpret_cast<float*>(rs), run.P);
+      else
+        replay_chunk<double>(ctx.stream, run.device, *run.tape, ct, reinterpret_cast<const double*>(xs),
+                             reinterpret_cast<double*>(rs), run.P);
+    }
+  });
 }
 
 int eda_timer_report(char* buf, size_t len, char* err, size_t errlen) {

@@ -68,6 +68,11 @@ enum class CubeOp : int8_t { kMax = 0, kMin = 1, kMaxMin = 2, kBox = 3 };
 template <class T>
 void enqueue_cube(hipStream_t s, const Geom& g, CubeOp op, const T* in, T* out0, T* out1, const int32_t* win,
                   int32_t max_w);
+// One max (min) pass along `axis` for `rungs` window arrays: rung j reads in + j in_stride and writes
+// out + j out_stride (the pass the cube runs per axis).
+template <class T>
+void enqueue_axis_pass(hipStream_t s, const Geom& g, int axis, bool min, const T* in, T* out, const int32_t* win,
+                       int32_t max_w, int32_t rungs, int64_t in_stride, int64_t out_stride);
 // The cubes of `rungs` window arrays (win + j B) of one input in one launch per pass, rung j into
 // out0 + j stride (out1 + j stride); each rung's outputs are bitwise those of enqueue_cube. With
 // pairs (kMaxMin), the last pass may write the max and min of a voxel side by side instead:
@@ -147,6 +152,12 @@ template <class T>
 void enqueue_orientation(hipStream_t s, int device, const Geom& g, const T* r, const int32_t* w, const int32_t* on,
                          T* nvec, T* coh, T* work, int32_t* nonfinite, double* dscratch);
 inline int64_t orient_scratch_doubles(int64_t batch) { return batch * 65; }
+// The adjoint of the orientation: gr += (dn / dr)^T gn + (dc / dr)^T gc for the n and c that
+// enqueue_orientation computes from r (D >= 2). work: D + D (D + 1) / 2 + 4 fields of T;
+// dscratch: orient_scratch_doubles(B) doubles.
+template <class T>
+void enqueue_orientation_vjp(hipStream_t s, int device, const Geom& g, const T* r, const int32_t* w, const int32_t* on,
+                             const T* gn, const T* gc, T* gr, T* work, double* dscratch);
 // The monogenic signal of every element of f (the prescaled fields: the input times 2^-kexp[b])
 // into the planes of out, the elements at b0 .. b0 + B - 1 of out_batch: amplitude (times
 // 2^kexp[b]), phase, frequency [out_batch][count] each, then for D >= 2 the direction

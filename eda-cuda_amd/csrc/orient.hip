@@ -1213,6 +1213,23 @@ __global__ void __launch_bounds__(kThreads) k_riesz_mult(Geom g, const T* __rest
   }
 }
 
+// Component ci of the orientation's D + D (D + 1) / 2: ci < D is R1_ci (kind 0); then the pairs
+// (a, bb), a <= bb, in pair order: R2_aa (kind 1), R2_ab (kind 2).
+inline void riesz_component_of(int D, int ci, int* a, int* bb, int* kind) {
+  int aa = ci, b2 = ci;
+  if (ci >= D) {
+    int p = ci - D;
+    for (aa = 0; aa < D; ++aa) {
+      if (p < D - aa) break;
+      p -= D - aa;
+    }
+    b2 = aa + p;
+  }
+  *a = aa;
+  *bb = b2;
+  *kind = ci < D ? 0 : (aa == b2 ? 1 : 2);
+}
+
 // One component of X (a kind of riesz_mult) into dst: Y = T(m X), then the inverse transform with
 // the sin basis on the component's odd axes (T1: the transform's scratch field).
 template <class T>
@@ -1224,18 +1241,18 @@ void riesz_component(hipStream_t s, int device, const Geom& g, const T* X, int k
   enqueue_dct<T>(s, device, g, Y, dst, T1, 1, riesz_odd(kind, a, bb));
 }
 
+// The Riesz components of the band of window w[b] around 1 / w[b] (reference
+// orientation.cu:86-360): R1 (D fields) and R2 (D (D + 1) / 2 fields, pair order) of r - mean(r).
+// work: 3 fields (X, Y, the transform's scratch).
 template <class T>
-void enqueue_orientation(hipStream_t s, int device, const Geom& g, const T* r, const int32_t* w, const int32_t* on,
-                         T* nvec, T* coh, T* work, int32_t* nonfinite, double* dscratch) {
+void riesz_fields(hipStream_t s, int device, const Geom& g, const T* r, const int32_t* w, const int32_t* on, T* R1,
+                  T* R2, T* work, double* dscratch) {
   const int D = g.dim;
-  if (D < 2) return;
   const int64_t F = g.batch * g.count;
   const int K = D * (D + 1) / 2;
   T* X = work;
   T* Y = work + F;
   T* T1 = work + 2 * F;
-  T* R2 = work + 3 * F;
-  T* R1 = nvec;  // n is written last, over R1
   double* mean = dscratch;
   enqueue_mean<T>(s, g, r, on, mean, mean + g.batch);
   each(s, g, on, [=] __device__(int64_t b, int64_t i) {
@@ -1260,21 +1277,265 @@ void enqueue_orientation(hipStream_t s, int device, const Geom& g, const T* r, c
   });
   // the components: kind 0 R1_a (sin on a), 1 R2_aa, 2 R2_ab (sin on a and b)
   for (int ci = 0; ci < D + K; ++ci) {
-    // ci < D: R1_ci; then the pairs (a, bb), a <= bb, in the pair order
-    int a = ci, bb = ci;
-    if (ci >= D) {
-      int p = ci - D;
-      for (a = 0; a < D; ++a) {
-        if (p < D - a) break;
-        p -= D - a;
-      }
-      bb = a + p;
-    }
-    const int kind = ci < D ? 0 : (a == bb ? 1 : 2);
+    int a, bb, kind;
+    riesz_component_of(D, ci, &a, &bb, &kind);
     riesz_component<T>(s, device, g, X, kind, a, bb, Y, T1, kind == 0 ? R1 + a * F : R2 + tensor_pair(D, a, bb) * F,
                        on);
   }
+}
+
+template <class T>
+void enqueue_orientation(hipStream_t s, int device, const Geom& g, const T* r, const int32_t* w, const int32_t* on,
+                         T* nvec, T* coh, T* work, int32_t* nonfinite, double* dscratch) {
+  const int D = g.dim;
+  if (D < 2) return;
+  const int64_t F = g.batch * g.count;
+  T* R2 = work + 3 * F;
+  T* R1 = nvec;  // n is written last, over R1
+  riesz_fields<T>(s, device, g, r, w, on, R1, R2, work, dscratch);
   tensor_eigen<T>(s, g, R1, R2, on, nvec, coh, nonfinite);
+}
+
+// ---- the orientation, transposed (the adjoint's path from n and c back to r) ----------------------
+// Per voxel: the structure tensor T = R1 R1^T + R2 R2^T as the forward forms it, its eigen pairs
+// (the forward's Jacobi rotations in double; the 3-D scaling undone), then, for the gradients gn of
+// n (the top eigenvector, with the forward's sign) and gc of c = clip((mu1 - mu2) / (mu1 + mu2), 0,
+// 1): dL/dT = gmu1 v1 v1^T + gmu2 v2 v2^T + sum_{j != 1} (v_j . gn) / (mu1 - mu_j) sym(v_j v1^T) (the
+// first-order perturbation of a symmetric eigen problem; a term whose eigenvalue gap is 0 is
+// dropped), and through T's products the gradients of R1 and R2.
+template <int D>
+__device__ __forceinline__ bool eigen_pairs(const double* t, double (&mu)[3], double (&V)[3][3]) {
+  double A[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) V[a][c] = a == c ? 1.0 : 0.0;
+  bool finite = true;
+  int k = 0;
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+#pragma unroll
+    for (int c = a; c < D; ++c, ++k) {
+      finite = finite && isfinite(t[k]);
+      A[a][c] = A[c][a] = t[k];
+    }
+  double sc = 1.0;
+  if constexpr (D == 2) {
+    jacobi_rotate(A, V, 0, 1, -1, 2);
+  } else {
+    double amax = 0.0;
+    for (int a = 0; a < 3; ++a)
+      for (int c = a; c < 3; ++c) amax = fmax(amax, fabs(A[a][c]));
+    if (amax > 0.0 && amax < __builtin_huge_val()) {
+      sc = scalbn(1.0, -ilogb(amax));
+      for (int a = 0; a < 3; ++a)
+        for (int c = 0; c < 3; ++c) A[a][c] = A[a][c] * sc;
+    }
+#pragma unroll 1
+    for (int sweep = 0; sweep < 6; ++sweep) {
+      jacobi_rotate(A, V, 0, 1, 2, 3);
+      jacobi_rotate(A, V, 0, 2, 1, 3);
+      jacobi_rotate(A, V, 1, 2, 0, 3);
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < D; ++a) mu[a] = A[a][a] / sc;
+  return finite;
+}
+
+template <class T, int D>
+__global__ void __launch_bounds__(kThreads) k_eigen_vjp(Geom g, const T* R1, const T* R2, const T* __restrict__ gn,
+                                                        const T* __restrict__ gc, T* gR1, T* gR2,
+                                                        const int32_t* __restrict__ on) {
+  constexpr int K = D * (D + 1) / 2;
+  const int64_t b = blockIdx.y;
+  if (on != nullptr && on[b] == 0) return;
+  const int64_t F = g.batch * g.count;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(kThreads) + threadIdx.x; i < g.count;
+       i += static_cast<int64_t>(gridDim.x) * kThreads) {
+    const int64_t at = b * g.count + i;
+    T r1[D], r2[K], tt[K];
+#pragma unroll
+    for (int a = 0; a < D; ++a) r1[a] = R1[a * F + at];
+#pragma unroll
+    for (int p = 0; p < K; ++p) r2[p] = R2[p * F + at];
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+      for (int c = a; c < D; ++c) {
+        T v = r1[a] * r1[c];
+#pragma unroll
+        for (int q = 0; q < D; ++q) v = v + r2[tensor_pair(D, a, q)] * r2[tensor_pair(D, q, c)];
+        tt[tensor_pair(D, a, c)] = v;
+      }
+    double td[K], mu[3], V[3][3];
+#pragma unroll
+    for (int p = 0; p < K; ++p) td[p] = static_cast<double>(tt[p]);
+    double G[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    if (eigen_pairs<D>(td, mu, V)) {
+      int top = 0;
+#pragma unroll
+      for (int a = 1; a < D; ++a) top = mu[a] > mu[top] ? a : top;
+      int sec = top == 0 ? 1 : 0;
+#pragma unroll
+      for (int a = 0; a < D; ++a) sec = (a != top && mu[a] > mu[sec]) ? a : sec;
+      const double mu1 = mu[top], mu2 = mu[sec];
+      double gmu1 = 0.0, gmu2 = 0.0;
+      const double ssum = mu1 + mu2;
+      const double gcv = static_cast<double>(gc[at]);
+      if (ssum > 0.0) {
+        const double craw = (mu1 - mu2) / ssum;
+        if (craw >= 0.0 && craw <= 1.0) {
+          gmu1 = gcv * (2.0 * mu2) / (ssum * ssum);
+          gmu2 = -gcv * (2.0 * mu1) / (ssum * ssum);
+        }
+      }
+      double gnv[3] = {0, 0, 0};
+#pragma unroll
+      for (int a = 0; a < D; ++a) gnv[a] = static_cast<double>(gn[a * F + at]);
+#pragma unroll
+      for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int c = 0; c < D; ++c) G[a][c] = gmu1 * V[a][top] * V[c][top] + gmu2 * V[a][sec] * V[c][sec];
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        if (j == top) continue;
+        const double gap = mu1 - mu[j];
+        if (!(gap != 0.0)) continue;
+        double dot = 0.0;
+#pragma unroll
+        for (int a = 0; a < D; ++a) dot += V[a][j] * gnv[a];
+        const double coef = dot / gap;
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+          for (int c = 0; c < D; ++c) G[a][c] += 0.5 * coef * (V[a][j] * V[c][top] + V[a][top] * V[c][j]);
+      }
+    }
+    // the pair gradients (an off-diagonal pair stands for two entries), then T's products
+    double gp[K];
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+      for (int c = a; c < D; ++c) gp[tensor_pair(D, a, c)] = a == c ? G[a][a] : G[a][c] + G[c][a];
+    double g1[D], g2[K];
+#pragma unroll
+    for (int a = 0; a < D; ++a) g1[a] = 0.0;
+#pragma unroll
+    for (int p = 0; p < K; ++p) g2[p] = 0.0;
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+      for (int c = a; c < D; ++c) {
+        const double gv = gp[tensor_pair(D, a, c)];
+        g1[a] += gv * static_cast<double>(r1[c]);
+        g1[c] += gv * static_cast<double>(r1[a]);
+#pragma unroll
+        for (int q = 0; q < D; ++q) {
+          g2[tensor_pair(D, a, q)] += gv * static_cast<double>(r2[tensor_pair(D, q, c)]);
+          g2[tensor_pair(D, q, c)] += gv * static_cast<double>(r2[tensor_pair(D, a, q)]);
+        }
+      }
+#pragma unroll
+    for (int a = 0; a < D; ++a) gR1[a * F + at] = static_cast<T>(g1[a]);
+#pragma unroll
+    for (int p = 0; p < K; ++p) gR2[p * F + at] = static_cast<T>(g2[p]);
+  }
+}
+
+// One component transposed: the forward took Y = T(m Xb) and R = Inv_odd(Y) (the inverse DCT with
+// the sin basis on the odd axes). Inv_odd^T on an odd axis is the sum with sin(pi k (2x + 1) / 2n),
+// which equals (-1)^x cos(pi (n - k) (2x + 1) / 2n): flip the signs of gR along the odd axes, take
+// the forward DCT, read it at n_a - k_a on the odd axes (0 at k_a = 0), and gXb += m gY.
+template <class T>
+__global__ void __launch_bounds__(kThreads) k_riesz_acc(Geom g, const T* __restrict__ Z, T* __restrict__ gXb,
+                                                        int kind, int a, int bb, int odd,
+                                                        const int32_t* __restrict__ on) {
+  const int64_t b = blockIdx.y;
+  if (on != nullptr && on[b] == 0) return;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(kThreads) + threadIdx.x; i < g.count;
+       i += static_cast<int64_t>(gridDim.x) * kThreads) {
+    int32_t k[kMaxDim];
+    double u[kMaxDim];
+    const double rad = dct_freq(g, i, k, u);
+    const double m = riesz_mult(kind, a, bb, u, rad);
+    int64_t src = 0;
+    bool zero = false;
+    for (int ax = 0; ax < g.dim; ++ax) {
+      int32_t kk = k[ax];
+      if ((odd >> ax) & 1) {
+        if (kk == 0) zero = true;
+        kk = g.n[ax] - kk;
+      }
+      src += static_cast<int64_t>(kk) * g.st[ax];
+    }
+    const int64_t at = b * g.count + i;
+    const double z = zero ? 0.0 : static_cast<double>(Z[b * g.count + src]);
+    gXb[at] = static_cast<T>(static_cast<double>(gXb[at]) + m * z);
+  }
+}
+
+// gr += the gradient of r through the orientation, from gn (D fields) and gc; work: D + K + 4 fields.
+template <class T>
+void enqueue_orientation_vjp(hipStream_t s, int device, const Geom& g, const T* r, const int32_t* w,
+                             const int32_t* on, const T* gn, const T* gc, T* gr, T* work, double* dscratch) {
+  const int D = g.dim;
+  if (D < 2) return;
+  const int64_t F = g.batch * g.count;
+  const int K = D * (D + 1) / 2;
+  T* R1 = work;
+  T* R2 = work + D * F;
+  T* W3 = work + (D + K) * F;  // 3 fields for the forward's transforms, then gXb, Y, Z, scratch
+  riesz_fields<T>(s, device, g, r, w, on, R1, R2, W3, dscratch);
+  // the gradients of R1, R2 over R1, R2 themselves (each voxel reads its own first)
+  const dim3 grid = dim3(static_cast<unsigned>(std::min<int64_t>((g.count + kThreads - 1) / kThreads, 1024)),
+                         static_cast<unsigned>(g.batch));
+  if (D == 2) EDA_GO((k_eigen_vjp<T, 2>), grid, dim3(kThreads), 0, s, g, R1, R2, gn, gc, R1, R2, on);
+  else EDA_GO((k_eigen_vjp<T, 3>), grid, dim3(kThreads), 0, s, g, R1, R2, gn, gc, R1, R2, on);
+  EDA_LAUNCH("k_eigen_vjp");
+  T* gXb = W3;
+  T* Y = W3 + F;
+  T* Z = W3 + 2 * F;
+  T* T1 = W3 + 3 * F;
+  each(s, g, on, [=] __device__(int64_t b, int64_t i) { gXb[b * g.count + i] = T(0); });
+  for (int ci = 0; ci < D + K; ++ci) {
+    int a, bb, kind;
+    riesz_component_of(D, ci, &a, &bb, &kind);
+    const int odd = riesz_odd(kind, a, bb);
+    const T* src = kind == 0 ? R1 + a * F : R2 + tensor_pair(D, a, bb) * F;
+    each(s, g, on, [=] __device__(int64_t b, int64_t i) {
+      int par = 0;
+      for (int ax = 0; ax < g.dim; ++ax)
+        if ((odd >> ax) & 1) par += static_cast<int>((i / g.st[ax]) % g.n[ax]);
+      const int64_t at = b * g.count + i;
+      Y[at] = (par & 1) ? -src[at] : src[at];
+    });
+    enqueue_dct<T>(s, device, g, Y, Z, T1, 0, 0);
+    EDA_GO(k_riesz_acc<T>, grid, dim3(kThreads), 0, s, g, Z, gXb, kind, a, bb, odd, on);
+    EDA_LAUNCH("k_riesz_acc");
+  }
+  // the band, then the transposed forward DCT (its inverse) and the mean's projection
+  const double two_sigma2 = band_two_sigma2();
+  each(s, g, on, [=] __device__(int64_t b, int64_t i) {
+    int32_t k[kMaxDim];
+    double u[kMaxDim];
+    const double rad = dct_freq(g, i, k, u);
+    double G = 0.0;
+    if (rad > 0.0) {
+      const double f0 = 1.0 / (1.0 * static_cast<double>(w[b]));
+      const double l = log(rad / f0);
+      G = exp(-(l * l) / two_sigma2);
+    }
+    const int64_t at = b * g.count + i;
+    gXb[at] = static_cast<T>(static_cast<double>(gXb[at]) * G);
+  });
+  enqueue_dct<T>(s, device, g, gXb, Y, T1, 1, 0);
+  double* mean = dscratch;
+  enqueue_mean<T>(s, g, Y, on, mean, mean + g.batch);
+  each(s, g, on, [=] __device__(int64_t b, int64_t i) {
+    const int64_t at = b * g.count + i;
+    gr[at] = static_cast<T>(static_cast<double>(gr[at]) + (static_cast<double>(Y[at]) - mean[b]));
+  });
 }
 
 // The monogenic signal of every element of f (the prescaled fields f 2^-kexp[b]) into the planes
@@ -1371,6 +1632,10 @@ template void enqueue_dct<float>(hipStream_t, int, const Geom&, const float*, fl
 template void enqueue_dct<double>(hipStream_t, int, const Geom&, const double*, double*, double*, int, int);
 template void enqueue_eigen<float>(hipStream_t, const Geom&, const float*, const int32_t*, float*, float*, int32_t*);
 template void enqueue_eigen<double>(hipStream_t, const Geom&, const double*, const int32_t*, double*, double*, int32_t*);
+template void enqueue_orientation_vjp<float>(hipStream_t, int, const Geom&, const float*, const int32_t*,
+                                             const int32_t*, const float*, const float*, float*, float*, double*);
+template void enqueue_orientation_vjp<double>(hipStream_t, int, const Geom&, const double*, const int32_t*,
+                                              const int32_t*, const double*, const double*, double*, double*, double*);
 template void enqueue_orientation<float>(hipStream_t, int, const Geom&, const float*, const int32_t*, const int32_t*,
                                          float*, float*, float*, int32_t*, double*);
 template void enqueue_orientation<double>(hipStream_t, int, const Geom&, const double*, const int32_t*, const int32_t*,

@@ -1639,6 +1639,19 @@ void enqueue_cube(hipStream_t s, const Geom& g, CubeOp op, const T* in, T* out0,
   enqueue_cube_rungs<T>(s, g, op, in, out0, out1, win, 1, 0, max_w);
 }
 
+template <class T>
+void enqueue_axis_pass(hipStream_t s, const Geom& g, int axis, bool min, const T* in, T* out, const int32_t* win,
+                       int32_t max_w, int32_t rungs, int64_t in_stride, int64_t out_stride) {
+  const int ops[1] = {min ? kVMin : kVMax};
+  const T* src[1] = {in};
+  T* dst[1] = {out};
+  enqueue_vhgw<T>(s, g, axis, 1, true, ops, false, src, dst, win, max_w, rungs, in_stride, out_stride);
+}
+template void enqueue_axis_pass<float>(hipStream_t, const Geom&, int, bool, const float*, float*, const int32_t*,
+                                       int32_t, int32_t, int64_t, int64_t);
+template void enqueue_axis_pass<double>(hipStream_t, const Geom&, int, bool, const double*, double*, const int32_t*,
+                                        int32_t, int32_t, int64_t, int64_t);
+
 template void enqueue_cube<float>(hipStream_t, const Geom&, CubeOp, const float*, float*, float*, const int32_t*, int32_t);
 template void enqueue_cube<double>(hipStream_t, const Geom&, CubeOp, const double*, double*, double*, const int32_t*,
                                    int32_t);

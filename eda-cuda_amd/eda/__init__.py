@@ -15,6 +15,7 @@ from typing import Any
 
 from . import _native
 from ._api import DecompositionInfo, Monogenic, Orientation, decompose, monogenic
+from ._autograd import TapedDecomposition, decompose_function, differentiable_decompose
 from ._errors import CudaError, DecompositionError
 
 __version__ = "0.1.0"
@@ -25,10 +26,13 @@ __all__ = [
     "DecompositionInfo",
     "Monogenic",
     "Orientation",
+    "TapedDecomposition",
     "__version__",
     "build_info",
     "decompose",
+    "decompose_function",
     "device_count",
+    "differentiable_decompose",
     "device_info",
     "monogenic",
 ]

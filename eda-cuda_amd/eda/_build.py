@@ -19,7 +19,7 @@ INCLUDE = ROOT.parent / "include"
 BUILD = ROOT / "_build"
 LIB = PKG / "libeda_hip.so"
 
-SOURCES = ["vhgw.hip", "reduce.hip", "select.hip", "flags.hip", "ccl.hip", "orient.hip", "element.hip", "driver.hip", "noalloc.hip"]
+SOURCES = ["vhgw.hip", "reduce.hip", "select.hip", "flags.hip", "ccl.hip", "orient.hip", "element.hip", "driver.hip", "noalloc.hip", "adjoint.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -54,6 +54,8 @@ class Params(ctypes.Structure):
         ("tie_tolerance", ctypes.c_double),
         ("max_chunk", ctypes.c_int64),
         ("stream", ctypes.c_uint64),
+        ("keep_tape", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -89,6 +91,8 @@ SIGNATURES = {
     "eda_monogenic": ([_c.POINTER(ArrayDesc), _i32, _i64, _c.c_uint64, _vp, _i32, _i32p, _c.POINTER(_i64), *_err],
                       _c.c_int),
     "eda_release": ([_c.POINTER(Result)], None),
+    "eda_backward": ([_c.POINTER(Result), _vp, _vp, _i32, *_err], _c.c_int),
+    "eda_replay": ([_c.POINTER(Result), _vp, _vp, *_err], _c.c_int),
     "eda_build_info": ([_c.c_char_p, _c.c_size_t], _c.c_int),
     "eda_device_count": ([], _c.c_int),
     "eda_device_info": ([_i32, _c.c_char_p, _c.c_size_t], _c.c_int),

@@ -81,6 +81,8 @@ typedef struct eda_params {
   double tie_tolerance; /* < 0: the dtype's default (2^-14 float32, 2^-30 float64) */
   int64_t max_chunk;    /* > 0: at most this many elements per chunk; 0: by free memory */
   uint64_t stream;      /* a hipStream_t the call waits for before it reads device input (0: none) */
+  int32_t keep_tape;    /* 1: keep every mode's frozen decisions in the result, for eda_backward / eda_replay */
+  int32_t reserved;
 } eda_params;
 
 /* The result of eda_decompose. The rows stay in a library-owned device buffer until eda_collect
@@ -137,6 +139,21 @@ int eda_collect_orientation(const eda_result* r, void* dir, void* coh, int32_t d
  * hipStream_t the call waits for before it reads device input (0: none). Complete on return. */
 int eda_monogenic(const eda_array_desc* x, int32_t check_finite, int64_t max_chunk, uint64_t stream, void* out,
                   int32_t out_device, int32_t* nonfinite, int64_t* chunk, char* err, size_t errlen);
+
+/* The adjoint of a decomposition whose call kept its tape (eda_params.keep_tape = 1): with the
+ * decisions of every mode frozen (windows, rung ladders and the rungs used, the scale field, the
+ * lower rung of every voxel, the sifts run and accepted, the amplitude floors), grad_x (batch,
+ * *spatial) = J^T grad_rows, J the Jacobian of the rows (batch, P, *spatial) in the input, both
+ * C-contiguous device arrays in the compute dtype on the result's device; complete on return. flags
+ * bit 0: hold the orientation (direction and coherence) fixed instead of differentiating it. Failed
+ * elements get 0. Axes longer than 2048 samples are refused (EDA_EINVAL). No counterpart in the
+ * reference (it has no adjoint); the autograd form of eda.decompose sits on it. */
+int eda_backward(const eda_result* r, const void* grad_rows, void* grad_x, int32_t flags, char* err, size_t errlen);
+
+/* The frozen forward of a result with a tape at a new input x (batch, *spatial): rows (batch, P,
+ * *spatial), device arrays as in eda_backward; at the call's own input it gives the call's rows.
+ * For checking the adjoint against finite differences. No counterpart in the reference. */
+int eda_replay(const eda_result* r, const void* x, void* rows, char* err, size_t errlen);
 
 /* Frees the rows and the info arrays of a result (idempotent). */
 void eda_release(eda_result* r);

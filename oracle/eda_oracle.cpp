@@ -1249,9 +1249,25 @@ struct ElemOut {
   std::vector<double> spacing, energy;
 };
 
+// The frozen decisions of one element's decomposition, one record per mode, for the reference of
+// the adjoint (the sift map with windows, rungs, selections and labels frozen): the residual r at
+// the mode's start, the scale field L, the lower rung lo of every voxel, and per mode the integers
+// [w, W, K, used, pts, rung_w[0..31], sifts run after the first, accepted bits (bit j: sift j),
+// kexp] and the doubles [tol, r_inf, thr_j (j = 1 .. sifts - 1, at index 1 + j)].
+constexpr int kTapeInts = 40;
+constexpr int kTapeDbls = 34;
+struct TapeSink {
+  int32_t modes = 0;  // records available
+  void* r = nullptr;
+  void* L = nullptr;
+  uint8_t* lo = nullptr;
+  int32_t* ints = nullptr;
+  double* dbls = nullptr;
+};
+
 template <class T>
 ElemOut decompose_one(const Shape& s, const Ctl& c, const T* x, int32_t max_imf, double tau_rel, bool check_finite,
-                      int32_t cap, T* rows /* cap * count */, const OrientPlan* plan) {
+                      int32_t cap, T* rows /* cap * count */, const OrientPlan* plan, const TapeSink* tape = nullptr) {
   ElemOut out;
   const int64_t N = s.count;
   const int D = s.D;
@@ -1461,6 +1477,27 @@ ElemOut decompose_one(const Shape& s, const Ctl& c, const T* x, int32_t max_imf,
       el.used |= bits;
       if (de.e > T(0.25)) el.pts |= bits;
     }
+    int32_t* tape_i = nullptr;
+    double* tape_d = nullptr;
+    if (tape != nullptr && n_imfs < tape->modes) {
+      const size_t off = static_cast<size_t>(n_imfs) * N;
+      std::copy(r.begin(), r.end(), static_cast<T*>(tape->r) + off);
+      std::copy(L.begin(), L.end(), static_cast<T*>(tape->L) + off);
+      std::copy(el.lo.begin(), el.lo.end(), tape->lo + off);
+      tape_i = tape->ints + static_cast<size_t>(n_imfs) * kTapeInts;
+      tape_d = tape->dbls + static_cast<size_t>(n_imfs) * kTapeDbls;
+      std::fill(tape_i, tape_i + kTapeInts, 0);
+      std::fill(tape_d, tape_d + kTapeDbls, 0.0);
+      tape_i[0] = w;
+      tape_i[1] = W;
+      tape_i[2] = el.K;
+      tape_i[3] = static_cast<int32_t>(el.used);
+      tape_i[4] = static_cast<int32_t>(el.pts);
+      for (int j = 0; j < el.K; ++j) tape_i[5 + j] = el.rung_w[j];
+      tape_i[39] = kexp;
+      tape_d[0] = tol;
+      tape_d[1] = r_inf;
+    }
     // step 8: sift 1 and the continuation weight (driver.cpp:606-638; sift.cu:55-85)
     envelope_mean<T>(s, el, r.data(), m.data(), u.data(), l.data());
     for (int64_t i = 0; i < N; ++i) h[i] = r[i] - m[i];
@@ -1531,6 +1568,7 @@ ElemOut decompose_one(const Shape& s, const Ctl& c, const T* x, int32_t max_imf,
         thr = thr >= tol ? thr : tol;
         thr = thr >= c.tiny ? thr : c.tiny;
         for (int64_t i = 0; i < N; ++i) e[i] = static_cast<double>(e[i]) >= thr ? e[i] : static_cast<T>(thr);
+        if (tape_d != nullptr && j < kTapeDbls - 2) tape_d[2 + j - 1] = thr;
       }
       for (int64_t i = 0; i < N; ++i) q[i] = h[i] / e[i];
       envelope_mean<T>(s, el, q.data(), Mn.data(), u.data(), l.data());
@@ -1548,6 +1586,10 @@ ElemOut decompose_one(const Shape& s, const Ctl& c, const T* x, int32_t max_imf,
         m_h = std::max(m_h, abs_bits<T>(xv));
       }
       const bool accepted = !(bits_d(m_res) > r_inf) && !(bits_d(m_h) > r_inf);
+      if (tape_i != nullptr) {
+        tape_i[37] += 1;
+        if (accepted && j < 31) tape_i[38] |= 1 << j;
+      }
       if (accepted) {
         h.swap(hn);
         ++n_sifts;
@@ -1668,6 +1710,41 @@ int oracle_decompose(int dtype, const void* x, int dim, int64_t batch, const int
       }
     }
     return failed ? -1 : 0;
+  } catch (...) {
+    return -1;
+  }
+}
+
+// One element's decomposition with its tape (see TapeSink): rows cap * count; r, L: modes * count
+// in dtype; lo: modes * count; ints: modes * 40; dbls: modes * 34. Returns 0, or -1 on an exception.
+int oracle_decompose_tape(int dtype, const void* x, int dim, const int64_t* n, int32_t sifts, void* rows,
+                          int32_t* n_imfs, int32_t* error, int32_t modes, void* r, void* L, uint8_t* lo,
+                          int32_t* ints, double* dbls) {
+  try {
+    const orc::Shape s = orc::make_shape(dim, n);
+    orc::Ctl c = orc::make_ctl(dim, s.n, s.count, dtype == 0);
+    c.sifts = sifts;
+    const int32_t cap = oracle_rows_cap(dim, n, -1);
+    const double tau = dtype == 0 ? std::ldexp(1.0, -14) : std::ldexp(1.0, -30);
+    orc::OrientPlan plan;
+    if (dim >= 2) plan = orc::orient_plan(s);
+    orc::TapeSink sink;
+    sink.modes = modes;
+    sink.r = r;
+    sink.L = L;
+    sink.lo = lo;
+    sink.ints = ints;
+    sink.dbls = dbls;
+    orc::ElemOut o;
+    if (dtype == 0)
+      o = orc::decompose_one<float>(s, c, static_cast<const float*>(x), -1, tau, true, cap, static_cast<float*>(rows),
+                                    &plan, &sink);
+    else
+      o = orc::decompose_one<double>(s, c, static_cast<const double*>(x), -1, tau, true, cap,
+                                     static_cast<double*>(rows), &plan, &sink);
+    *n_imfs = o.n_imfs;
+    *error = o.error;
+    return 0;
   } catch (...) {
     return -1;
   }

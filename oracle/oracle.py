@@ -64,6 +64,9 @@ def lib() -> ctypes.CDLL:
         L.oracle_rung_ladder.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int32, ctypes.c_int32, _i32p]
         L.oracle_rung_ladder.restype = ctypes.c_int
         L.oracle_interp.argtypes = [_f64p, ctypes.c_int, _i64p, _f64p, ctypes.c_int64, _f64p]
+        L.oracle_decompose_tape.argtypes = [ctypes.c_int, _vp, ctypes.c_int, _i64p, ctypes.c_int32, _vp, _i32p, _i32p,
+                                            ctypes.c_int32, _vp, _vp, _u8p, _i32p, _f64p]
+        L.oracle_decompose_tape.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -272,3 +275,48 @@ def cpu_threads() -> int:
         return len(os.sched_getaffinity(0))
     except AttributeError:  # pragma: no cover
         return os.cpu_count() or 1
+
+
+TAPE_INTS, TAPE_DBLS = 40, 34
+
+
+def decompose_tape(x: np.ndarray, *, sifts: int = 3) -> dict:
+    """One element's decomposition (x: the spatial array, float32 or float64) with its tape: the
+    frozen decisions of every mode (the residual r at the mode's start, the scale field L, the lower
+    rung lo per voxel, the windows w and W, the rung ladder with its used / segment-point bits, the
+    tie tolerance, the sifts run and accepted, the amplitude floors thr_j) and the rows (P,
+    *spatial), n_imfs, error, kexp."""
+    x = np.ascontiguousarray(x)
+    spatial = tuple(x.shape)
+    cap = rows_cap(spatial)
+    modes = cap - 1
+    rows = np.zeros((cap,) + spatial, dtype=x.dtype)
+    r = np.zeros((modes,) + spatial, dtype=x.dtype)
+    L = np.zeros((modes,) + spatial, dtype=x.dtype)
+    lo = np.zeros((modes,) + spatial, dtype=np.uint8)
+    ints = np.zeros((modes, TAPE_INTS), np.int32)
+    dbls = np.zeros((modes, TAPE_DBLS), np.float64)
+    n_imfs = np.zeros(1, np.int32)
+    error = np.zeros(1, np.int32)
+    rc = lib().oracle_decompose_tape(_code(x.dtype), _p(x), len(spatial), _shape(spatial), sifts, _p(rows),
+                                     _p(n_imfs, _i32p), _p(error, _i32p), modes, _p(r), _p(L), _p(lo, _u8p),
+                                     _p(ints, _i32p), _p(dbls, _f64p))
+    if rc != 0:
+        raise RuntimeError("oracle: the decomposition raised (an internal invariant failed)")
+    k = int(n_imfs[0])
+    mode_list = []
+    for m in range(k):
+        iv = ints[m]
+        K = int(iv[2])
+        ran = int(iv[37])
+        mode_list.append({
+            "r": r[m], "L": L[m], "lo": lo[m], "w": int(iv[0]), "W": int(iv[1]),
+            "rung_w": [int(v) for v in iv[5:5 + K]], "used": int(iv[3]) & 0xffffffff,
+            "pts": int(iv[4]) & 0xffffffff, "ran": ran,
+            "accepted": [bool((int(iv[38]) >> j) & 1) for j in range(1, ran + 1)],
+            "tol": float(dbls[m, 0]), "r_inf": float(dbls[m, 1]),
+            "thr": [float(dbls[m, 1 + j]) for j in range(1, ran + 1)],
+        })
+    kexp = int(ints[0, 39]) if k > 0 else 0
+    return {"rows": rows[: k + 1].copy(), "n_imfs": k, "error": int(error[0]), "kexp": kexp, "modes": mode_list,
+            "sifts": sifts}
