@@ -263,6 +263,56 @@ def bench_monogenic(args, x, lib, torch) -> dict:
 
 
 LEDGER_STEPS = 2
+BWD_STEPS = 3
+
+
+def bench_fwd_bwd(args, x, torch, eda):
+    """fwd+bwd: eda.decompose with its tape kept, the differentiation workload's loss gradient
+    (sum_k w_k ||row_k - target_k||^2, w_k ~ U(0.5, 2), targets ~ N(0, 1) from a seed) and the
+    library's adjoint back to the input; BWD_STEPS timed steps after one warm-up, each bracketed by
+    a device synchronization. Returns the rate and the split between the forward and the backward."""
+    import numpy as np
+
+    rng = np.random.default_rng(5)
+
+    def step():
+        t0 = time.perf_counter()
+        td = eda.TapedDecomposition(x)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        P = td.rows.shape[1]
+        w = torch.from_numpy(rng.uniform(0.5, 2.0, P)).to(x.device, x.dtype).reshape(1, P, 1, 1)
+        tgt = torch.randn(td.rows.shape[1:], generator=gen, device=x.device, dtype=x.dtype)[None]
+        g = 2.0 * w * (td.rows - tgt)
+        gx = td.backward(g)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        del td, g, gx
+        return t1 - t0, t2 - t1
+
+    gen = torch.Generator(device=x.device)
+    gen.manual_seed(5)
+    step()
+    fwd, bwd = [], []
+    for _ in range(BWD_STEPS):
+        a, b = step()
+        fwd.append(a)
+        bwd.append(b)
+    tot = [a + b for a, b in zip(fwd, bwd)]
+    dt = sum(tot) / len(tot)
+    log(f"bench: fwd+bwd: {BWD_STEPS} steps, {dt * 1e3:.1f} ms per step (forward with tape "
+        f"{sum(fwd) / len(fwd) * 1e3:.1f} ms, loss gradient and backward {sum(bwd) / len(bwd) * 1e3:.1f} ms)")
+    return {
+        "value": round(B * N * N / dt / 1e6, 3),
+        "unit": "Msamples/s",
+        "ms_per_step": round(dt * 1e3, 3),
+        "fwd_ms": round(sum(fwd) / len(fwd) * 1e3, 3),
+        "bwd_ms": round(sum(bwd) / len(bwd) * 1e3, 3),
+        "steps": BWD_STEPS,
+        "workload": "eda.differentiable forward (tape kept) + loss sum_k w_k ||row_k - target_k||^2 + the "
+                    "hand-written adjoint back to the 64 x 512 x 512 float32 input (orientation differentiated)",
+    }
+
 
 
 def kernel_ledger(args, x, lib, torch, eda, _native, step_s):
@@ -317,6 +367,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-bwd", action="store_true", help="skip the fwd+bwd measurement")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one copy kernel of known size (for scaling rocprofv3 byte counters)")
     ap.add_argument("--monogenic", action="store_true",
@@ -437,6 +488,8 @@ def main() -> int:
         },
         "roofline": roofline,
     }
+    if not args.no_bwd:
+        out["fwd_bwd"] = bench_fwd_bwd(args, x, torch, eda)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

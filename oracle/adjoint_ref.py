@@ -290,4 +290,4 @@ def reference_gradient(x: np.ndarray, tape: dict, seed: int, *, orientation_grad
     w, tgt = loss_weights(rows.shape[0], seed, x.shape)
     loss = (w.reshape((-1,) + (1,) * x.ndim) * (rows - tgt) ** 2).sum()
     loss.backward()
-    return float(loss), xt.grad.numpy().copy(), rows.detach().numpy()
+    return float(loss.detach()), xt.grad.numpy().copy(), rows.detach().numpy()
