@@ -173,6 +173,82 @@ def cpu_baseline():
     }
 
 
+PREC_METRIC = "Msamples/s decomposed (4×256³ FP32), exact vs split-BF16 DCT, error vs FP64 per mode"
+
+
+def bench_precision(args, lib, torch) -> dict:
+    """The precision workload: 4 x 256^3 float32 plane-wave volumes decomposed with the exact DCT /
+    Riesz transforms and with split-BF16 (3-term) MFMA contractions; per path the call time (median
+    of the timed steps), the DCT axis passes' live rate (HIP events; algorithmic bytes: one read and
+    one write of the field; flops 2 n per sample and axis, three products each for split-BF16) and
+    the per-mode relative L2 error against the float64 decomposition of the same volumes."""
+    import numpy as np
+
+    import eda
+    import workloads
+    from eda import _native
+
+    x = torch.from_numpy(workloads.plane_wave_volumes(4, 256)).to("cuda:0")
+    y64, i64 = eda.decompose(x.double(), axes=(1, 2, 3), return_info=True)
+    n_samples = x.numel()
+    err = _native.errbuf()
+    buf = ctypes.create_string_buffer(4096)
+    out = {}
+    for dct in ("exact", "bf16x3"):
+        for _ in range(args.warmup):
+            eda.decompose(x, axes=(1, 2, 3), dct=dct)
+        times = []
+        for _ in range(max(args.steps, 1)):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            y, info = eda.decompose(x, axes=(1, 2, 3), return_info=True, dct=dct)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t)
+        lib.eda_timer_enable(2)
+        _native.check(lib.eda_timer_report(buf, len(buf), err, len(err)), err)
+        eda.decompose(x, axes=(1, 2, 3), dct=dct)
+        torch.cuda.synchronize()
+        _native.check(lib.eda_timer_report(buf, len(buf), err, len(err)), err)
+        lib.eda_timer_enable(0)
+        d = json.loads(buf.value.decode())["k_dct_fft"]
+        us = d["ms"] * 1e3 / max(d["launches"], 1)
+        per = d["bytes"] / max(d["launches"], 1)
+        flop = (2 * 256 * (3 if dct == "bf16x3" else 1)) * per / 8.0  # 2 n per sample and axis (x3)
+        errs = []
+        for k in range(y.shape[1]):
+            es = [float(torch.linalg.vector_norm((y[b, k] - y64[b, k]).double()) /
+                        torch.linalg.vector_norm(y64[b, k].double()).clamp_min(1e-300))
+                  for b in range(y.shape[0]) if k <= int(info.n_imfs[b])]
+            errs.append(round(max(es), 9) if es else 0.0)
+        dt = sorted(times)[len(times) // 2]
+        out[dct] = {
+            "ms_per_call": round(dt * 1e3, 2),
+            "Msamples_per_s": round(n_samples / dt / 1e6, 2),
+            "decisions_equal_f64": bool(np.array_equal(info.n_imfs, i64.n_imfs) and
+                                        np.array_equal(info.windows, i64.windows)),
+            "per_mode_rel_l2_vs_f64": errs,
+            "dct_axis_pass": {"launches": d["launches"], "us_per_launch": round(us, 2),
+                              "GBs": round(per / (us * 1e-6) / 1e9, 1) if us > 0 else None,
+                              "TFLOPs": round(flop / (us * 1e-6) / 1e12, 2) if us > 0 else None},
+        }
+        log(f"bench: precision {dct}: {dt * 1e3:.1f} ms per call, DCT axis pass {us:.1f} us "
+            f"({out[dct]['dct_axis_pass']['GBs']} GB/s), per-mode error vs f64 {errs}")
+    return {
+        "metric": PREC_METRIC,
+        "value": out["bf16x3"]["Msamples_per_s"],
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "higher_is_better": True,
+        "dtype": "f32 (bf16x3 DCT contractions)",
+        "data": "synthetic",
+        "config": {"workload": "eda.decompose, 4 x 256 x 256 x 256 float32 plane-wave volumes (3 waves of "
+                               "wavelength 6, 20, 64, Gaussian envelope sigma 60, Gaussian trend, N(0, 0.02^2))"},
+        "paths": out,
+    }
+
+
 MONO_METRIC = "Msamples/s monogenic signal (64×512² FP32)"
 
 
@@ -372,6 +448,8 @@ def main() -> int:
                     help="also run one copy kernel of known size (for scaling rocprofv3 byte counters)")
     ap.add_argument("--monogenic", action="store_true",
                     help="measure eda.monogenic on the same batch instead of eda.decompose")
+    ap.add_argument("--precision", action="store_true",
+                    help="measure the 4 x 256^3 precision workload: exact vs split-BF16 DCT contractions")
     ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "OUT_JSON"),
                     help="summarize two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) and exit")
     args = ap.parse_args()
@@ -394,6 +472,9 @@ def main() -> int:
 
     assert torch.cuda.is_available(), "bench: no GPU"
     lib = _native.core()
+    if args.precision:
+        print(json.dumps(bench_precision(args, lib, torch)), flush=True)
+        return 0
     log("bench: generating 64 x 512 x 512 float32 inputs")
     x = torch.from_numpy(workloads.rings_stripes_blobs(B, N, np.float32)).to("cuda:0")
     torch.cuda.synchronize()

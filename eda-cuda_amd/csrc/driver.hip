@@ -1792,9 +1792,18 @@ bool Chunk<T>::mode() {
 }
 
 // ------------------------------------------------------------------------------------------------
+// Sets the calling thread's DCT contraction mode for a scope.
+struct DctModeScope {
+  int prev;
+  explicit DctModeScope(int mode) : prev(set_dct_mode(mode)) {}
+  ~DctModeScope() { set_dct_mode(prev); }
+};
+
 template <class T>
 void run_call(const eda_array_desc& x, const eda_params& p, Run& run) {
   Context& ctx = context(run.device);
+  if (p.dct_mode != 0 && p.dct_mode != 1) throw std::invalid_argument("eda: dct_mode must be 0 or 1");
+  const DctModeScope dct_scope(std::is_same_v<T, float> ? p.dct_mode : 0);
   if (p.stream) {
     hipEvent_t ev;
     EDA_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
@@ -1825,6 +1834,7 @@ void run_call(const eda_array_desc& x, const eda_params& p, Run& run) {
   if (p.keep_tape) {
     run.tape = std::make_unique<Tape>();
     run.tape->dtype = std::is_same_v<T, float> ? EDA_F32 : EDA_F64;
+    run.tape->dct_mode = std::is_same_v<T, float> ? p.dct_mode : 0;
     run.tape->c = c;
   }
   // the chunk: bounded by max_chunk and by the free device memory (halving)
@@ -2313,6 +2323,7 @@ int eda_backward(const eda_result* r, const void* grad_rows, void* grad_x, int32
     if (run.batch == 0) return;
     if (grad_rows == nullptr || grad_x == nullptr) throw std::invalid_argument("eda: backward: null pointer");
     Context& ctx = context(run.device);
+    const DctModeScope dct_scope(run.tape->dct_mode);
     const bool orient = (flags & 1) == 0;
     for (const ChunkTape& ct : run.tape->chunks) {
       const size_t item = run.dtype == EDA_F32 ? 4 : 8;
@@ -2336,6 +2347,7 @@ int eda_replay(const eda_result* r, const void* x, void* rows, char* err, size_t
     if (run.batch == 0) return;
     if (x == nullptr || rows == nullptr) throw std::invalid_argument("eda: replay: null pointer");
     Context& ctx = context(run.device);
+    const DctModeScope dct_scope(run.tape->dct_mode);
     for (const ChunkTape& ct : run.tape->chunks) {
       const size_t item = run.dtype == EDA_F32 ? 4 : 8;
       const char* xs = static_cast<const char*>(x) + static_cast<size_t>(ct.b0) * run.count * item;

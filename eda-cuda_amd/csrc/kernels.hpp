@@ -135,6 +135,9 @@ void enqueue_ccl(hipStream_t s, const Geom& g, const uint8_t* mask, uint8_t bits
                  const int32_t* on);
 
 // ---- orientation (orient.hip) --------------------------------------------------------------------
+// The DCT contraction mode of the calling thread (0 exact float32; 1 split-BF16, three terms on
+// the matrix cores, float32 calls only); returns the previous mode.
+int set_dct_mode(int mode);
 // The dense DCT matrices of one axis length, on the device (built once per (n, dtype)).
 template <class T>
 struct AxisMats {

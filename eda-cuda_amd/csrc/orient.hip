@@ -234,11 +234,166 @@ __global__ void __launch_bounds__(kThreads) k_gemm_mfma(GemmDims d, const float*
       }
 }
 
+// The float32 GEMM as split-BF16 products on the matrix cores: every operand value v is held as
+// hi = bf16(v) and lo = bf16(v - hi) (16 significant bits together), and each product as the three
+// terms lo_a hi_b + hi_a lo_b + hi_a hi_b (the lo lo term, 2^-16 relative, dropped) accumulated in
+// float32 by v_mfma_f32_16x16x32_bf16 (lane l supplies A(l % 16, 8 (l / 16) + e) and
+// B(8 (l / 16) + e, l % 16), e < 8, and holds D(4 (l / 16) + v, l % 16), v < 4). The tiles are those
+// of k_gemm_mfma (64 x 64 per workgroup, 32 x 32 per wave, K in steps of 32), staged in LDS as bf16
+// rows along k (16-byte operand reads).
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+constexpr int kBK = 32, kBRow = kBK + 8;
+
+__device__ __forceinline__ void split_bf16(float v, __bf16* hi, __bf16* lo) {
+  const __bf16 h = static_cast<__bf16>(v);
+  *hi = h;
+  *lo = static_cast<__bf16>(v - static_cast<float>(h));
+}
+__device__ __forceinline__ void split4(const float4& q, bf16x4& h, bf16x4& l) {
+  __bf16 hh[4], ll[4];
+  split_bf16(q.x, &hh[0], &ll[0]);
+  split_bf16(q.y, &hh[1], &ll[1]);
+  split_bf16(q.z, &hh[2], &ll[2]);
+  split_bf16(q.w, &hh[3], &ll[3]);
+  h = bf16x4{hh[0], hh[1], hh[2], hh[3]};
+  l = bf16x4{ll[0], ll[1], ll[2], ll[3]};
+}
+
+__global__ void __launch_bounds__(kThreads) k_gemm_bf16x3(GemmDims d, const float* __restrict__ A,
+                                                          const float* __restrict__ Bm, float* __restrict__ C,
+                                                          const int32_t* __restrict__ on) {
+  __shared__ __align__(16) __bf16 Ah[kTM][kBRow];
+  __shared__ __align__(16) __bf16 Al[kTM][kBRow];
+  __shared__ __align__(16) __bf16 Bh[kTN][kBRow];
+  __shared__ __align__(16) __bf16 Bl[kTN][kBRow];
+  const int64_t bt = blockIdx.z;
+  if (on != nullptr && on[bt / d.per_elem] == 0) return;
+  const int64_t i0 = static_cast<int64_t>(blockIdx.y) * kTM;
+  const int64_t j0 = static_cast<int64_t>(blockIdx.x) * kTN;
+  const float* Ab = A + bt * d.sAb;
+  const float* Bb = Bm + bt * d.sBb;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int lr = lane & 15, lk = (lane >> 4) * 8;
+  mf4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) acc[a][c] = mf4{0.f, 0.f, 0.f, 0.f};
+  const bool vec = d.M % kTM == 0 && d.N % kTN == 0 && d.K % kBK == 0 &&
+                   (reinterpret_cast<uintptr_t>(Ab) & 15) == 0 && (reinterpret_cast<uintptr_t>(Bb) & 15) == 0 &&
+                   (d.sAk == 1 ? d.sAi % 4 == 0 : d.sAi == 1 && d.sAk % 4 == 0) &&
+                   (d.sBk == 1 ? d.sBj % 4 == 0 : d.sBj == 1 && d.sBk % 4 == 0);
+  for (int64_t k0 = 0; k0 < d.K; k0 += kBK) {
+    if (vec) {
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        const int e = tid + r * kThreads;  // 512 runs of 4 cover a 64 x 32 tile
+        if (d.sAk == 1) {
+          const int ii = e >> 3, kk = (e & 7) * 4;
+          const float4 q = *reinterpret_cast<const float4*>(Ab + (i0 + ii) * d.sAi + (k0 + kk));
+          bf16x4 h, l;
+          split4(q, h, l);
+          *reinterpret_cast<bf16x4*>(&Ah[ii][kk]) = h;
+          *reinterpret_cast<bf16x4*>(&Al[ii][kk]) = l;
+        } else {
+          const int kk = e >> 4, ii = (e & 15) * 4;
+          const float4 q = *reinterpret_cast<const float4*>(Ab + (i0 + ii) + (k0 + kk) * d.sAk);
+          split_bf16(q.x, &Ah[ii][kk], &Al[ii][kk]);
+          split_bf16(q.y, &Ah[ii + 1][kk], &Al[ii + 1][kk]);
+          split_bf16(q.z, &Ah[ii + 2][kk], &Al[ii + 2][kk]);
+          split_bf16(q.w, &Ah[ii + 3][kk], &Al[ii + 3][kk]);
+        }
+        if (d.sBk == 1) {
+          const int jj = e >> 3, kk = (e & 7) * 4;
+          const float4 q = *reinterpret_cast<const float4*>(Bb + (k0 + kk) + (j0 + jj) * d.sBj);
+          bf16x4 h, l;
+          split4(q, h, l);
+          *reinterpret_cast<bf16x4*>(&Bh[jj][kk]) = h;
+          *reinterpret_cast<bf16x4*>(&Bl[jj][kk]) = l;
+        } else {
+          const int kk = e >> 4, jj = (e & 15) * 4;
+          const float4 q = *reinterpret_cast<const float4*>(Bb + (k0 + kk) * d.sBk + (j0 + jj));
+          split_bf16(q.x, &Bh[jj][kk], &Bl[jj][kk]);
+          split_bf16(q.y, &Bh[jj + 1][kk], &Bl[jj + 1][kk]);
+          split_bf16(q.z, &Bh[jj + 2][kk], &Bl[jj + 2][kk]);
+          split_bf16(q.w, &Bh[jj + 3][kk], &Bl[jj + 3][kk]);
+        }
+      }
+    } else {
+      for (int e = tid; e < kTM * kBK; e += kThreads) {
+        int ii, kk;
+        if (d.sAk == 1) {
+          kk = e % kBK;
+          ii = e / kBK;
+        } else {
+          ii = e % kTM;
+          kk = e / kTM;
+        }
+        const int64_t gi = i0 + ii, gk = k0 + kk;
+        split_bf16((gi < d.M && gk < d.K) ? Ab[gi * d.sAi + gk * d.sAk] : 0.f, &Ah[ii][kk], &Al[ii][kk]);
+      }
+      for (int e = tid; e < kTN * kBK; e += kThreads) {
+        int jj, kk;
+        if (d.sBj == 1) {
+          jj = e % kTN;
+          kk = e / kTN;
+        } else {
+          kk = e % kBK;
+          jj = e / kBK;
+        }
+        const int64_t gj = j0 + jj, gk = k0 + kk;
+        split_bf16((gj < d.N && gk < d.K) ? Bb[gk * d.sBk + gj * d.sBj] : 0.f, &Bh[jj][kk], &Bl[jj][kk]);
+      }
+    }
+    __syncthreads();
+    bf16x8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      ah[a] = *reinterpret_cast<const bf16x8*>(&Ah[wm + 16 * a + lr][lk]);
+      al[a] = *reinterpret_cast<const bf16x8*>(&Al[wm + 16 * a + lr][lk]);
+      bh[a] = *reinterpret_cast<const bf16x8*>(&Bh[wn + 16 * a + lr][lk]);
+      bl[a] = *reinterpret_cast<const bf16x8*>(&Bl[wn + 16 * a + lr][lk]);
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[c], acc[a][c], 0, 0, 0);
+        acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[c], acc[a][c], 0, 0, 0);
+        acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[c], acc[a][c], 0, 0, 0);
+      }
+    __syncthreads();
+  }
+  float* Cb = C + bt * d.sCb;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t gi = i0 + wm + 16 * a + (lane >> 4) * 4 + v;
+        const int64_t gj = j0 + wn + 16 * c + lr;
+        if (gi < d.M && gj < d.N) Cb[gi * d.sCi + gj * d.sCj] = acc[a][c][v];
+      }
+}
+
+// The DCT contraction mode of the calling thread: 0 the exact float32 forms (the LDS FFT for
+// power-of-two axes, the float32 MFMA GEMM for the others), 1 every axis as a split-BF16 GEMM.
+thread_local int t_dct_mode = 0;
+
 template <class T>
 void gemm(hipStream_t s, const GemmDims& d, const T* A, const T* B, T* C, const int32_t* on) {
   const dim3 grid(static_cast<unsigned>((d.N + kTN - 1) / kTN), static_cast<unsigned>((d.M + kTM - 1) / kTM),
                   static_cast<unsigned>(d.batches));
   if constexpr (std::is_same_v<T, float>) {
+    if (t_dct_mode == 1) {
+      EDA_GO(k_gemm_bf16x3, grid, dim3(kThreads), 0, s, d, A, B, C, on);
+      EDA_LAUNCH("k_gemm_bf16x3");
+      return;
+    }
     EDA_GO(k_gemm_mfma, grid, dim3(kThreads), 0, s, d, A, B, C, on);
     EDA_LAUNCH("k_gemm_mfma");
   } else {
@@ -1124,6 +1279,12 @@ int riesz_odd(int kind, int a, int b) {
 
 }  // namespace
 
+int set_dct_mode(int mode) {
+  const int prev = t_dct_mode;
+  t_dct_mode = mode;
+  return prev;
+}
+
 template <class T>
 const AxisMats<T>& axis_mats(int device, int32_t n) {
   static std::mutex mu;
@@ -1144,7 +1305,7 @@ void enqueue_dct(hipStream_t s, int device, const Geom& g, const T* in, T* out, 
   for (int k = 0; k < D; ++k) {
     const int a = D - 1 - k;
     T* dst = ((D - 1 - k) % 2 == 0) ? out : tmp;
-    if (fft_axis<T>(static_cast<int32_t>(g.n[a]))) {
+    if (fft_axis<T>(static_cast<int32_t>(g.n[a])) && !(std::is_same_v<T, float> && t_dct_mode == 1)) {
       fft_transform<T>(s, device, g, a, src, dst, inverse, (odd >> a) & 1, nullptr);
     } else {
       const AxisMats<T>& m = axis_mats<T>(device, g.n[a]);

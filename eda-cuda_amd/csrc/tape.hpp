@@ -58,6 +58,7 @@ struct ChunkTape {
 
 struct Tape {
   int dtype = 0;
+  int dct_mode = 0;  // the call's DCT contraction mode (set_dct_mode), reused by the backward
   MethodCtl c{};
   std::vector<ChunkTape> chunks;
 };

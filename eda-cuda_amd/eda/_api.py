@@ -161,6 +161,16 @@ def _tie_tolerance(value: Any) -> float | None:
     return number
 
 
+_DCT_MODES = {"exact": 0, "bf16x3": 1}
+
+
+def _dct_mode(value: Any) -> int:
+    """The DCT contraction mode: 'exact' (float32 / float64 transforms) or 'bf16x3' (split-BF16)."""
+    if not isinstance(value, str) or value not in _DCT_MODES:
+        raise ValueError(f"eda: dct={value!r} is not one of 'exact', 'bf16x3'")
+    return _DCT_MODES[value]
+
+
 def _flag(value: Any, name: str) -> bool:
     if isinstance(value, _BOOLS):
         return bool(value)
@@ -264,7 +274,7 @@ def _device_tensor(x: Any):
 
 
 def _call_native(desc: _native.ArrayDesc, max_imf: int, tie: float | None, sifts: int, check_finite: bool,
-                 stream: int, orientation: bool = False) -> _native.Result:
+                 stream: int, orientation: bool = False, dct_mode: int = 0) -> _native.Result:
     lib = _native.core()
     p = _native.Params()
     p.max_imf = max_imf
@@ -274,6 +284,7 @@ def _call_native(desc: _native.ArrayDesc, max_imf: int, tie: float | None, sifts
     p.tie_tolerance = -1.0 if tie is None else tie
     p.max_chunk = 0
     p.stream = stream
+    p.dct_mode = dct_mode
     res = _native.Result()
     err = _native.errbuf()
     _native.check(lib.eda_decompose(ctypes.byref(desc), ctypes.byref(p), ctypes.byref(res), err, len(err)), err)
@@ -419,6 +430,7 @@ def decompose(
     stream: Any = None,
     return_info: bool = False,
     return_orientation: bool = False,
+    dct: str = "exact",
 ) -> Any:
     """Decompose every 1-D, 2-D or 3-D field of ``x`` into intrinsic mode functions (IMFs) and a
     residue, on an AMD Instinct GPU (gfx950).
@@ -432,6 +444,9 @@ def decompose(
     ``Orientation``: the direction and the coherence that the method computes at the start of each
     mode. The result is ``y``, then ``info`` if ``return_info``, then the ``Orientation`` if
     ``return_orientation``; asking for the orientation does not change ``y`` or ``info``.
+    ``dct="bf16x3"`` (float32 only; an addition to the reference's arguments) runs every DCT and
+    Riesz contraction of the orientation as split-BF16 products on the matrix cores (three bf16
+    products per term, float32 accumulation) instead of the exact float32 transforms.
 
     Host input returns NumPy arrays; a ROCm device array (a PyTorch-ROCm tensor, or a DLPack
     producer on a ROCm device) returns a PyTorch tensor on the same device. The call waits for the
@@ -445,6 +460,7 @@ def decompose(
     return_orientation = _flag(return_orientation, "return_orientation")
     if not isinstance(errors, str) or errors not in _ERRORS:
         raise ValueError(f"eda: errors={errors!r} is not one of 'raise', 'status'")
+    dct_mode = _dct_mode(dct)
 
     call = _prepare(x, axes, dtype, stream)
     if return_orientation and len(call.axes) < 2:
@@ -459,7 +475,7 @@ def decompose(
     tdevice = call.device
     out_final = _restore_order(ndim, batch_axes, axes)
     del call
-    res = _call_native(desc, max_imf, tie, sifts, check_finite, stream_handle, return_orientation)
+    res = _call_native(desc, max_imf, tie, sifts, check_finite, stream_handle, return_orientation, dct_mode)
     try:
         info = _info_of(res, batch_shape, target, axes)
         codes = info.error.reshape(-1)

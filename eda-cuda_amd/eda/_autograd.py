@@ -26,7 +26,7 @@ class TapedDecomposition:
     *spatial), ``info`` as ``eda._core_hip`` returns it, ``backward(grad_rows)`` and
     ``replay(x)``. Holds the library's result (and its device tape) until deleted."""
 
-    def __init__(self, x, *, sifts: int = 3, max_imf: int = -1):
+    def __init__(self, x, *, sifts: int = 3, max_imf: int = -1, dct: str = "exact"):
         import torch  # noqa: PLC0415
 
         if not isinstance(x, torch.Tensor) or x.device.type != "cuda":
@@ -48,6 +48,7 @@ class TapedDecomposition:
         p = _core_hip._params(max_imf, None, sifts, True, False, 0,
                               int(torch.cuda.current_stream(x.device).cuda_stream))
         p.keep_tape = 1
+        p.dct_mode = {"exact": 0, "bf16x3": 1}[dct]
         self._res = _core_hip._run(desc, p)
         del keep
         self.info = _core_hip._info(self._res)

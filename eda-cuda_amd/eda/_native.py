@@ -55,7 +55,7 @@ class Params(ctypes.Structure):
         ("max_chunk", ctypes.c_int64),
         ("stream", ctypes.c_uint64),
         ("keep_tape", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("dct_mode", ctypes.c_int32),
     ]
 
 

@@ -82,7 +82,9 @@ typedef struct eda_params {
   int64_t max_chunk;    /* > 0: at most this many elements per chunk; 0: by free memory */
   uint64_t stream;      /* a hipStream_t the call waits for before it reads device input (0: none) */
   int32_t keep_tape;    /* 1: keep every mode's frozen decisions in the result, for eda_backward / eda_replay */
-  int32_t reserved;
+  int32_t dct_mode;     /* float32 only: 0 the exact DCT / Riesz contractions (an LDS FFT for power-of-two
+                           axes, float32 MFMA GEMMs otherwise); 1 every axis as a split-BF16 GEMM on the
+                           matrix cores (3 products per term, float32 accumulation) */
 } eda_params;
 
 /* The result of eda_decompose. The rows stay in a library-owned device buffer until eda_collect

@@ -180,3 +180,12 @@ def test_monogenic_result_type():
     assert set(eda.Monogenic.__dataclass_fields__) == {"amplitude", "phase", "frequency", "direction", "coherence"}
     with pytest.raises(TypeError, match="expected a float32 or float64 array"):
         eda.monogenic(np.zeros((4, 16), np.int32))
+
+
+def test_dct_mode_rules():
+    from eda._api import _dct_mode
+
+    assert _dct_mode("exact") == 0 and _dct_mode("bf16x3") == 1
+    for bad in ("fp32", None, 1):
+        with pytest.raises(ValueError):
+            _dct_mode(bad)
