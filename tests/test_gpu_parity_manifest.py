@@ -1,6 +1,7 @@
 # SPDX-License-Identifier: Apache-2.0
 """The reference's parity manifest on the GPU (tests/golden/parity_decisions.json, from the
-reference's tests/data/parity_v2.json): every one of the 287 cases through ``eda.decompose`` (and,
+reference's tests/data/parity_v2.json; inputs tests/golden/parity_inputs.npz): every one of the 287
+cases through ``eda.decompose`` (and,
 where the case sets max_chunk, through the reference's native interface with that chunk), in its
 dtype, layout and arguments. The decisions (n_imfs, error, windows, sifts), the output shape, the
 chunk count and the error policy equal the manifest's exactly. The manifest's row hashes are the
@@ -17,8 +18,7 @@ import numpy as np
 import pytest
 
 import eda
-import parity_inputs
-from test_parity_manifest import CASES, batch_layout, decisions_equal
+from parity_cases import CASES, batch_layout, decisions_equal, load_input
 
 pytestmark = pytest.mark.gpu
 
@@ -26,7 +26,7 @@ pytestmark = pytest.mark.gpu
 @pytest.mark.parametrize("case", CASES, ids=lambda c: c["id"])
 def test_gpu_decisions_match_the_manifest(case):
     print("parity case", case["id"], flush=True)
-    x = parity_inputs.build(case["recipe"], case["dtype"])
+    x = load_input(case["id"])
     axes = tuple(case["axes"]) if case["axes"] is not None else None
     y, info = eda.decompose(x, axes=axes, max_imf=case["max_imf"], dtype=case["dtype"],
                             tie_tolerance=case["tie_tolerance"], check_finite=case["check_finite"],
