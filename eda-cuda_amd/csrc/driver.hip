@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -1919,7 +1920,9 @@ namespace {
 void validate(const eda_array_desc& x, const eda_params& p) {
   if (x.dim < 1 || x.dim > 3) throw std::invalid_argument("eda: D must be 1, 2 or 3");
   if (x.dtype != EDA_F32 && x.dtype != EDA_F64) throw std::invalid_argument("eda: the input must be float32 or float64");
-  const int64_t lim = x.dtype == EDA_F32 ? 8192 : 4096;
+  // 1-D lines take the long-line paths of the cube filter, the extrema and the reductions (rows of
+  // up to 2^18 samples); fields of 2 or 3 axes keep the limits of their staged kernels and transforms
+  const int64_t lim = x.dim == 1 ? (int64_t{1} << 18) : (x.dtype == EDA_F32 ? 8192 : 4096);
   for (int a = 0; a < x.dim; ++a) {
     if (x.n[a] < 1) throw std::invalid_argument("eda: every spatial extent must be >= 1");
     if (x.n[a] > lim) {
@@ -2359,6 +2362,55 @@ int eda_replay(const eda_result* r, const void* x, void* rows, char* err, size_t
         replay_chunk<double>(ctx.stream, run.device, *run.tape, ct, reinterpret_cast<const double*>(xs),
                              reinterpret_cast<double*>(rs), run.P);
     }
+  });
+}
+
+int eda_tape_mode(const eda_result* r, int64_t element, int32_t mode, int32_t* ints, double* dbls, void* residual,
+                  void* L, uint8_t* lo, char* err, size_t errlen) {
+  return guarded(err, errlen, [&] {
+    if (r == nullptr || r->handle == nullptr) throw std::invalid_argument("eda: tape: no result");
+    Run& run = *static_cast<Run*>(r->handle);
+    if (!run.tape) throw std::invalid_argument("eda: tape: the call kept no tape (eda_params.keep_tape)");
+    if (element < 0 || element >= run.batch) throw std::invalid_argument("eda: tape: element out of range");
+    if (ints == nullptr || dbls == nullptr) throw std::invalid_argument("eda: tape: null pointer");
+    for (const ChunkTape& ct : run.tape->chunks) {
+      const int64_t B = static_cast<int64_t>(ct.n_imfs.size());
+      if (element < ct.b0 || element >= ct.b0 + B) continue;
+      const int64_t b = element - ct.b0;
+      if (mode < 0 || mode >= static_cast<int32_t>(ct.modes.size()) || !ct.modes[mode].active[b])
+        throw std::invalid_argument("eda: tape: the element ran no such mode");
+      const TapeMode& tm = ct.modes[mode];
+      std::fill(ints, ints + 40, 0);
+      std::fill(dbls, dbls + 34, 0.0);
+      const ElemDev& el = tm.el[b];
+      ints[0] = tm.w[b];
+      ints[1] = tm.W[b];
+      ints[2] = el.n_rungs;
+      ints[3] = static_cast<int32_t>(el.used);
+      ints[4] = static_cast<int32_t>(el.pts);
+      for (int j = 0; j < el.n_rungs && j < 32; ++j) ints[5 + j] = el.rung_w[j];
+      for (size_t j = 0; j < tm.ran.size(); ++j) {
+        if (!tm.ran[j][b]) continue;
+        ints[37] += 1;
+        if (tm.acc[j][b] && j + 1 < 31) ints[38] |= 1 << (j + 1);
+        if (j + 2 < 34) dbls[j + 2] = tm.thr[j][b];
+      }
+      ints[39] = ct.kexp[b];
+      dbls[0] = tm.tol[b];
+      dbls[1] = std::numeric_limits<double>::quiet_NaN();  // the amplitude bound is not kept
+      const size_t item = run.dtype == EDA_F32 ? 4 : 8;
+      const size_t n = static_cast<size_t>(ct.g.count);
+      Context& ctx = context(run.device);
+      EDA_HIP(hipStreamSynchronize(ctx.stream));
+      if (residual != nullptr)
+        EDA_HIP(hipMemcpy(residual, static_cast<const char*>(tm.r.p) + b * n * item, n * item, hipMemcpyDeviceToHost));
+      if (L != nullptr)
+        EDA_HIP(hipMemcpy(L, static_cast<const char*>(tm.L.p) + b * n * item, n * item, hipMemcpyDeviceToHost));
+      if (lo != nullptr)
+        EDA_HIP(hipMemcpy(lo, static_cast<const uint8_t*>(tm.lo.p) + b * n, n, hipMemcpyDeviceToHost));
+      return;
+    }
+    throw std::invalid_argument("eda: tape: element out of range");
   });
 }
 

@@ -67,6 +67,61 @@ __global__ void __launch_bounds__(kThreads) k_row_stats(const T* __restrict__ f,
   }
 }
 
+// Rows longer than 8192 samples (up to kLongChunks chunks of 32): one workgroup per row, the chunk
+// sums in LDS, then the pairwise tree padded to a power of two in place (the canonical order).
+constexpr int kLongChunks = 8192;
+
+template <class T>
+__global__ void __launch_bounds__(kThreads) k_row_stats_long(const T* __restrict__ f, int32_t nl, int64_t rows_per_elem,
+                                                             const int32_t* __restrict__ on, double* __restrict__ row_e,
+                                                             uint64_t* __restrict__ row_a) {
+  __shared__ double cs[kLongChunks];
+  __shared__ uint64_t red[2][kThreads / 64];
+  const int64_t q = blockIdx.x;
+  if (on != nullptr && on[q / rows_per_elem] == 0) return;
+  const T* row = f + q * nl;
+  const int32_t nchunks = (nl + 31) >> 5;
+  int32_t m = 1;
+  while (m < nchunks) m <<= 1;
+  uint64_t amax = 0, bad = 0;
+  for (int32_t c = threadIdx.x; c < m; c += kThreads) {
+    double acc = 0.0;
+    if (c < nchunks) {
+      const int32_t e0 = c * 32, e1 = min(e0 + 32, nl);
+      for (int32_t e = e0; e < e1; ++e) {
+        const double d = static_cast<double>(row[e]);
+        acc = acc + d * d;
+        const uint64_t ab = abs_bits(row[e]);
+        amax = ab > amax ? ab : amax;
+        bad += isfinite(d) ? 0 : 1;
+      }
+    }
+    cs[c] = acc;
+  }
+  __syncthreads();
+  for (int32_t d = 1; d < m; d <<= 1) {
+    for (int32_t i = threadIdx.x * 2 * d; i + d < m; i += kThreads * 2 * d) cs[i] = cs[i] + cs[i + d];
+    __syncthreads();
+  }
+  amax = wave_max_u64(amax);
+  bad = wave_sum_u64(bad);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = amax;
+    red[1][threadIdx.x >> 6] = bad;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t mx = 0, c = 0;
+    for (int w = 0; w < kThreads / 64; ++w) {
+      mx = red[0][w] > mx ? red[0][w] : mx;
+      c += red[1][w];
+    }
+    row_e[q] = cs[0];
+    row_a[2 * q] = mx;
+    row_a[2 * q + 1] = c;
+  }
+}
+
 constexpr int kLeaf = 1024;
 
 __global__ void __launch_bounds__(kThreads) k_elem_stats(int64_t rows_per_elem, const int32_t* __restrict__ on,
@@ -153,10 +208,17 @@ void enqueue_elem_stats(hipStream_t s, const Geom& g, const T* f, const int32_t*
   const int64_t rows = g.batch * rpe;
   double* row_e = static_cast<double*>(rows_ws);
   uint64_t* row_a = reinterpret_cast<uint64_t*>(row_e + rows);
-  const int64_t blocks = (rows + (kThreads / 64) - 1) / (kThreads / 64);
-  EDA_GO(k_row_stats<T>, dim3(static_cast<unsigned>(blocks)), dim3(kThreads), 0, s, f, rows, nl, rpe, on,
-                     row_e, row_a);
-  EDA_LAUNCH("k_row_stats");
+  if (nl > kRowSlots * 64 * 32) {
+    if ((nl + 31) / 32 > kLongChunks) throw std::invalid_argument("eda: a row is too long for the reductions");
+    EDA_GO(k_row_stats_long<T>, dim3(static_cast<unsigned>(rows)), dim3(kThreads), 0, s, f, nl, rpe, on, row_e,
+           row_a);
+    EDA_LAUNCH("k_row_stats_long");
+  } else {
+    const int64_t blocks = (rows + (kThreads / 64) - 1) / (kThreads / 64);
+    EDA_GO(k_row_stats<T>, dim3(static_cast<unsigned>(blocks)), dim3(kThreads), 0, s, f, rows, nl, rpe, on,
+                       row_e, row_a);
+    EDA_LAUNCH("k_row_stats");
+  }
   EDA_GO(k_elem_stats, dim3(static_cast<unsigned>(g.batch)), dim3(kThreads), 0, s, rpe, on, row_e, row_a,
                      energy, amax_bits, nonfinite);
   EDA_LAUNCH("k_elem_stats");

@@ -35,6 +35,8 @@
 #include "kernels.hpp"
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <atomic>
 #include <limits>
 
@@ -1457,6 +1459,261 @@ void launch_rows(hipStream_t s, const Geom& g, int axis, int nf, bool same_input
 
 }  // namespace
 
+// ---- lines too long for LDS (the whole pass in global memory) ---------------------------------------
+// The canonical vHGW order of a line of n samples and radius r = min((w - 1) / 2, n - 1): segments
+// of wa = 2r + 1 positions of the reflected extension starting at o + k wa, o = (n (wa + 1) / 2)
+// mod wa; inside a segment the prefix fold G runs in 32-sample chunks (a sequential fold in the
+// chunk, the chunk totals folded into a carry in order), the suffix fold H is G of the reversed
+// line, y(i) = op(H(i - r), G(i + r)) (the sum of an aligned window 0.5 (H + G)). Here each of its
+// three steps is a kernel over global memory: the chunk totals, the carries (one thread per segment,
+// in order), and every output from its chunk's carry and the fold from the chunk start (at most 32
+// reads). Used for passes whose line does not fit the LDS of the staged kernels.
+struct LongPass {
+  Geom g;
+  int32_t n, nf, ops[2], scale, rungs;
+  int64_t step, inner, lines, cap;  // cap: chunk slots per line and direction (n + 2 rmax)
+};
+
+template <class T>
+__device__ __forceinline__ T long_apply(int op, T a, T b) {
+  return op == kVMax ? max_c(a, b) : (op == kVMin ? min_c(a, b) : a + b);
+}
+
+struct LongGeo {
+  int32_t r, wa, o, k_first, nseg, cps;
+};
+__device__ __forceinline__ LongGeo long_geo(int32_t n, int32_t w) {
+  LongGeo q;
+  q.r = min((w - 1) / 2, n - 1);
+  q.wa = 2 * q.r + 1;
+  q.o = static_cast<int32_t>((static_cast<int64_t>(n) * ((q.wa + 1) / 2)) % q.wa);
+  q.k_first = static_cast<int32_t>(floor_div(static_cast<int64_t>(q.r) - q.o, q.wa));
+  const int32_t k_last = static_cast<int32_t>(floor_div(static_cast<int64_t>(n) - 1 + q.r - q.o, q.wa));
+  q.nseg = k_last - q.k_first + 1;
+  q.cps = (q.wa + 31) / 32;
+  return q;
+}
+// the value at extended position e of the line (rev: of the reversed line)
+template <class T>
+__device__ __forceinline__ T long_at(const T* line, int64_t step, int32_t n, int32_t e, bool rev) {
+  int32_t i = reflect_index(e, n);
+  if (rev) i = n - 1 - i;
+  return line[static_cast<int64_t>(i) * step];
+}
+// the base of line `li` of element b along the pass's axis
+__device__ __forceinline__ int64_t long_line_base(const LongPass& a, int64_t b, int64_t li) {
+  if (a.inner == 1) return b * a.g.count + li * a.n;
+  const int64_t o = li / a.inner, c = li - o * a.inner;
+  return b * a.g.count + o * static_cast<int64_t>(a.n) * a.inner + c;
+}
+
+template <class T>
+void launch_long(hipStream_t s, const Geom& g, int axis, int nf, const int* ops, bool scale, const VPtrs<T>& p,
+                 int32_t rungs, int32_t max_w) {
+  LongPass a{};
+  a.g = g;
+  a.n = g.n[axis];
+  a.nf = nf;
+  a.ops[0] = ops[0];
+  a.ops[1] = nf > 1 ? ops[1] : ops[0];
+  a.scale = scale ? 1 : 0;
+  a.rungs = rungs;
+  a.step = g.st[axis];
+  a.inner = axis == g.dim - 1 ? 1 : g.st[axis];
+  a.lines = g.count / a.n;
+  // chunks of a line and direction: the segments meeting [-r, n - 1 + r] (at most L / wa + 2 of
+  // them, L = n + 2r) times ceil(wa / 32) chunks each, at most 1.1 L + 2
+  const int32_t rmax = std::min((max_w - 1) / 2, a.n - 1);
+  const int64_t L = static_cast<int64_t>(a.n) + 2 * rmax;
+  a.cap = L + L / 8 + 64;
+  // chunk totals and carries of both directions, per field, line, element and rung
+  const int64_t per_dir = static_cast<int64_t>(rungs) * g.batch * a.lines * a.cap;
+  const size_t fold_bytes = static_cast<size_t>(per_dir) * 2 * 2 * static_cast<size_t>(nf) * sizeof(T);
+  // The outputs fold the input from its chunk starts while other threads write outputs: a pass
+  // whose input and output share memory (every pass after a cube's first runs in place) reads a
+  // copy of its input.
+  const int64_t in_rungs = p.in_stride == 0 ? 1 : rungs;
+  const int64_t elems = g.batch * g.count;
+  const auto span = [&](const T* q, int64_t rs, int64_t stride) {
+    const uintptr_t lo = reinterpret_cast<uintptr_t>(q);
+    return std::make_pair(lo, lo + static_cast<uintptr_t>(((rs - 1) * stride + elems) * sizeof(T)));
+  };
+  bool alias = false;
+  for (int f = 0; f < nf; ++f)
+    for (int h = 0; h < nf; ++h) {
+      const auto i = span(p.in[f], in_rungs, p.in_stride);
+      const auto o = span(p.out[h], rungs, p.out_stride);
+      alias = alias || (i.first < o.second && o.first < i.second);
+    }
+  const int64_t stage_elems = alias ? static_cast<int64_t>(nf) * in_rungs * elems : 0;
+  const size_t need = fold_bytes + static_cast<size_t>(stage_elems) * sizeof(T);
+  // one workspace per (device, stream) that only grows: launches on one stream are ordered
+  static std::mutex mu;
+  static std::map<std::pair<int, hipStream_t>, std::pair<void*, size_t>> cache;
+  int dev = 0;
+  EDA_HIP(hipGetDevice(&dev));
+  T* ws = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    auto& slot = cache[std::make_pair(dev, s)];
+    if (slot.second < need) {
+      EDA_HIP(hipStreamSynchronize(s));
+      if (slot.first) EDA_HIP(hipFree(slot.first));
+      slot.first = nullptr;
+      const hipError_t e = hipMalloc(&slot.first, need);
+      if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        slot.second = 0;
+        throw OutOfDeviceMemory("eda: the device memory does not hold the long-line workspace");
+      }
+      EDA_HIP(e);
+      slot.second = need;
+    }
+    ws = static_cast<T*>(slot.first);
+  }
+  const T* in0 = p.in[0];
+  const T* in1 = p.in[1];
+  int64_t in_stride = p.in_stride;
+  if (alias) {
+    T* stage = ws + fold_bytes / sizeof(T);
+    const T* src0 = in0;
+    const T* src1 = in1;
+    const int64_t src_stride = p.in_stride;
+    flat(s, stage_elems, [=] __device__(int64_t it) {
+      const int64_t k = it % elems;
+      const int64_t q = it / elems;
+      const int64_t j = q % in_rungs;
+      const T* src = q / in_rungs == 0 ? src0 : src1;
+      stage[it] = src[j * src_stride + k];
+    });
+    in0 = stage;
+    in1 = nf > 1 ? stage + in_rungs * elems : stage;
+    in_stride = p.in_stride == 0 ? 0 : elems;
+  }
+  const int32_t* win = p.win;
+  const int64_t B = g.batch;
+  // slot of (field f, direction dir (0 G, 1 reversed), total / carry, rung j, element b, line li, chunk c)
+  const auto slot_of = [=] __device__(int f, int dir, int which, int64_t j, int64_t b, int64_t li) -> int64_t {
+    return ((((static_cast<int64_t>(f) * 2 + dir) * 2 + which) * a.rungs + j) * B + b) * a.lines * a.cap + li * a.cap;
+  };
+  T* out0 = p.out[0];
+  T* out1 = p.out[1];
+  const int64_t out_stride = p.out_stride;
+  // 1: chunk totals
+  const int64_t items1 = static_cast<int64_t>(rungs) * B * a.lines * a.cap;
+  flat(s, items1 * 2, [=] __device__(int64_t it) {
+    const int dir = static_cast<int>(it & 1);
+    int64_t q = it >> 1;
+    const int64_t c = q % a.cap;
+    q /= a.cap;
+    const int64_t li = q % a.lines;
+    q /= a.lines;
+    const int64_t b = q % B;
+    const int64_t j = q / B;
+    const int32_t w = win[j * B + b];
+    if (w < 1) return;
+    const LongGeo G = long_geo(a.n, w);
+    const int64_t seg = c / G.cps, qc = c - seg * G.cps;
+    if (seg >= G.nseg) return;
+    const int64_t s0 = G.o + (G.k_first + seg) * static_cast<int64_t>(G.wa);
+    const int64_t e0 = s0 + 32 * qc;
+    const int64_t last = static_cast<int64_t>(a.n) - 1 + G.r;  // the folds stop at n - 1 + r
+    if (e0 > last) return;
+    const int64_t e1 = min(min(e0 + 31, s0 + G.wa - 1), last);
+    const int64_t base = long_line_base(a, b, li);
+    for (int f = 0; f < a.nf; ++f) {
+      const T* line = (f == 0 ? in0 : in1) + j * in_stride + base;
+      T acc = long_at(line, a.step, a.n, static_cast<int32_t>(e0), dir == 1);
+      for (int64_t e = e0 + 1; e <= e1; ++e) acc = long_apply(a.ops[f], acc, long_at(line, a.step, a.n, static_cast<int32_t>(e), dir == 1));
+      ws[slot_of(f, dir, 0, j, b, li) + c] = acc;
+    }
+  });
+  // 2: the carry before each chunk of a segment (chunk 0 has none)
+  flat(s, items1 * 2, [=] __device__(int64_t it) {
+    const int dir = static_cast<int>(it & 1);
+    int64_t q = it >> 1;
+    const int64_t seg = q % a.cap;
+    q /= a.cap;
+    const int64_t li = q % a.lines;
+    q /= a.lines;
+    const int64_t b = q % B;
+    const int64_t j = q / B;
+    const int32_t w = win[j * B + b];
+    if (w < 1) return;
+    const LongGeo G = long_geo(a.n, w);
+    if (seg >= G.nseg) return;
+    const int64_t s0 = G.o + (G.k_first + seg) * static_cast<int64_t>(G.wa);
+    const int64_t last = static_cast<int64_t>(a.n) - 1 + G.r;
+    const int64_t chunks = min(static_cast<int64_t>(G.cps), (last - s0) / 32 + 1);  // those the folds reach
+    for (int f = 0; f < a.nf; ++f) {
+      const T* tot = ws + slot_of(f, dir, 0, j, b, li) + seg * G.cps;
+      T* car = ws + slot_of(f, dir, 1, j, b, li) + seg * G.cps;
+      T carry = tot[0];
+      for (int32_t qc = 1; qc < chunks; ++qc) {
+        car[qc] = carry;
+        carry = long_apply(a.ops[f], carry, tot[qc]);
+      }
+    }
+  });
+  // 3: the outputs
+  const int64_t items3 = static_cast<int64_t>(rungs) * B * g.count;
+  flat(s, items3, [=] __device__(int64_t it) {
+    const int64_t idx = it % g.count;
+    int64_t q = it / g.count;
+    const int64_t b = q % B;
+    const int64_t j = q / B;
+    const int32_t w = win[j * B + b];
+    if (w < 1) return;
+    const LongGeo G = long_geo(a.n, w);
+    // the line and the position of this output
+    int64_t li, i;
+    if (a.inner == 1) {
+      li = idx / a.n;
+      i = idx - li * a.n;
+    } else {
+      const int64_t o = idx / (static_cast<int64_t>(a.n) * a.inner);
+      const int64_t rem = idx - o * a.n * a.inner;
+      i = rem / a.inner;
+      li = o * a.inner + (rem - i * a.inner);
+    }
+    const int64_t base = long_line_base(a, b, li);
+    // G at e = i + r of the line, H(i - r) = G at n - 1 - i + r of the reversed line
+    const auto fold = [&](int f, int dir, int64_t e) -> T {
+      const T* line = (f == 0 ? in0 : in1) + j * in_stride + base;
+      const int64_t ks = floor_div(e - G.o, G.wa);
+      const int64_t s0 = G.o + ks * static_cast<int64_t>(G.wa);
+      const int64_t d = e - s0;
+      const int64_t qc = d / 32;
+      const int64_t c = (ks - G.k_first) * G.cps + qc;
+      const int64_t e0 = s0 + 32 * qc;
+      T local = long_at(line, a.step, a.n, static_cast<int32_t>(e0), dir == 1);
+      for (int64_t x = e0 + 1; x <= e; ++x) local = long_apply(a.ops[f], local, long_at(line, a.step, a.n, static_cast<int32_t>(x), dir == 1));
+      if (qc == 0) return local;
+      return long_apply(a.ops[f], ws[slot_of(f, dir, 1, j, b, li) + c], local);
+    };
+    T fac = T(1);
+    if (a.scale) {
+      double prod = 1.0;
+      for (int ax = 0; ax < a.g.dim; ++ax) prod *= static_cast<double>(2 * min((w - 1) / 2, a.g.n[ax] - 1) + 1);
+      fac = static_cast<T>(1.0 / prod);
+    }
+    for (int f = 0; f < a.nf; ++f) {
+      const T gv = fold(f, 0, i + G.r);
+      const T hv = fold(f, 1, static_cast<int64_t>(a.n) - 1 - i + G.r);
+      T v;
+      if (a.ops[f] == kVSum) {
+        const T sum = hv + gv;
+        v = floor_mod(i - G.r - G.o, G.wa) == 0 ? sum * T(0.5) : sum;
+        if (a.scale) v = v * fac;
+      } else {
+        v = long_apply(a.ops[f], hv, gv);
+      }
+      T* out = (f == 0 ? out0 : out1) + j * out_stride;
+      out[base + i * a.step] = v;
+    }
+  });
+}
+
 template <class T>
 void enqueue_vhgw(hipStream_t s, const Geom& g, int axis, int nf, bool same_input, const int* ops, bool scale,
                   const T* const* in, T* const* out, const int32_t* win, int32_t max_w, int32_t rungs,
@@ -1527,17 +1784,41 @@ void enqueue_vhgw(hipStream_t s, const Geom& g, int axis, int nf, bool same_inpu
     };
     if (timed) timer_note(s, &t0);
     bool done = false;
-    if (sum && EDA_VHGW_PIPE) {
+    bool fits = true;
+    try {
+      (void)pass_shape(a, sizeof(T), sum, max_w);
+    } catch (const std::invalid_argument&) {
+      fits = false;
+    }
+    if (!fits) {
+      tag = kTagStrided;
+      launch_long<T>(s, g, axis, nf, ops, scale, p, rungs, max_w);
+      EDA_LAUNCH("k_long");
+      done = true;
+    }
+    if (!done && sum && EDA_VHGW_PIPE) {
       // the pipelined kernel, one launch per row layout: windows up to 32 samples on plain rows
       // (radius <= 15), longer ones on padded rows
       const bool longer = (max_w - 1) / 2 >= 16 && a.n > 16;
-      const Shape s1 = pass_shape(a, sizeof(T), true, std::min(max_w, 31), 1);
-      const VPass a1 = shaped(s1);
-      const int pf1 = box_reads<T>(a1, 64 * s1.W);
+      bool pipe_fits = true;
+      try {
+        (void)pass_shape(a, sizeof(T), true, std::min(max_w, 31), 1);
+        if (longer) (void)pass_shape(a, sizeof(T), true, max_w, 2);
+      } catch (const std::invalid_argument&) {
+        pipe_fits = false;
+      }
+      Shape s1{};
+      VPass a1{};
+      int pf1 = 0;
+      if (pipe_fits) {
+        s1 = pass_shape(a, sizeof(T), true, std::min(max_w, 31), 1);
+        a1 = shaped(s1);
+        pf1 = box_reads<T>(a1, 64 * s1.W);
+      }
       Shape s2{};
       VPass a2{};
       int pf2 = 4;
-      if (longer) {
+      if (pipe_fits && longer) {
         s2 = pass_shape(a, sizeof(T), true, max_w, 2);
         a2 = shaped(s2);
         pf2 = box_reads<T>(a2, 64 * s2.W);

@@ -439,8 +439,9 @@ def decompose(
     the information and the error policy are those of the reference's ``eda.decompose``: each batch
     element is decomposed on its own; along the mode axis each element holds its IMFs from fine to
     coarse, then its residue, then zero rows; ``P`` is one more than the largest IMF count over the
-    elements without an error. Every spatial extent must be at least 2 and at most 8192 (float32)
-    or 4096 (float64). ``return_orientation=True`` (2 or 3 spatial axes) also returns an
+    elements without an error. Every spatial extent must be at least 2; 1-D signals may hold up to
+    2^18 samples, fields of 2 or 3 axes up to 8192 (float32) or 4096 (float64) per axis (the
+    reference stops at 8192 / 4096 for every field). ``return_orientation=True`` (2 or 3 spatial axes) also returns an
     ``Orientation``: the direction and the coherence that the method computes at the start of each
     mode. The result is ``y``, then ``info`` if ``return_info``, then the ``Orientation`` if
     ``return_orientation``; asking for the orientation does not change ``y`` or ``info``.

@@ -88,6 +88,32 @@ class TapedDecomposition:
                                                     ctypes.c_void_p(rows.data_ptr()), err, len(err)), err)
         return rows
 
+    def mode_tape(self, element: int, mode: int) -> dict:
+        """The frozen decisions of one mode of one element, in the oracle's tape layout: w, W,
+        rung_w, used, pts, ran, accepted, tol, thr, and the residual r at the mode's start, the
+        scale field L and the lower rung lo of every voxel as host arrays."""
+        import numpy as np  # noqa: PLC0415
+
+        spatial = self.shape[1:]
+        ints = np.zeros(40, np.int32)
+        dbls = np.zeros(34, np.float64)
+        dt = np.float32 if self.dtype == self._torch.float32 else np.float64
+        r = np.zeros(spatial, dt)
+        L = np.zeros(spatial, dt)
+        lo = np.zeros(spatial, np.uint8)
+        err = _native.errbuf()
+        _native.check(_native.core().eda_tape_mode(
+            ctypes.byref(self._res), int(element), int(mode), ints.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+            dbls.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.c_void_p(r.ctypes.data),
+            ctypes.c_void_p(L.ctypes.data), ctypes.c_void_p(lo.ctypes.data), err, len(err)), err)
+        K = int(ints[2])
+        ran = int(ints[37])
+        return {"r": r, "L": L, "lo": lo, "w": int(ints[0]), "W": int(ints[1]),
+                "rung_w": [int(v) for v in ints[5:5 + K]], "used": int(ints[3]) & 0xffffffff,
+                "pts": int(ints[4]) & 0xffffffff, "ran": ran,
+                "accepted": [bool((int(ints[38]) >> j) & 1) for j in range(1, ran + 1)],
+                "tol": float(dbls[0]), "thr": [float(dbls[1 + j]) for j in range(1, ran + 1)]}
+
     def __del__(self):
         res = getattr(self, "_res", None)
         if res is not None:

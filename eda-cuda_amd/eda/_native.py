@@ -93,6 +93,8 @@ SIGNATURES = {
     "eda_release": ([_c.POINTER(Result)], None),
     "eda_backward": ([_c.POINTER(Result), _vp, _vp, _i32, *_err], _c.c_int),
     "eda_replay": ([_c.POINTER(Result), _vp, _vp, *_err], _c.c_int),
+    "eda_tape_mode": ([_c.POINTER(Result), _c.c_int64, _c.c_int32, _c.POINTER(_c.c_int32), _c.POINTER(_c.c_double),
+                       _vp, _vp, _vp, *_err], _c.c_int),
     "eda_build_info": ([_c.c_char_p, _c.c_size_t], _c.c_int),
     "eda_device_count": ([], _c.c_int),
     "eda_device_info": ([_i32, _c.c_char_p, _c.c_size_t], _c.c_int),

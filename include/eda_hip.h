@@ -157,6 +157,15 @@ int eda_backward(const eda_result* r, const void* grad_rows, void* grad_x, int32
  * For checking the adjoint against finite differences. No counterpart in the reference. */
 int eda_replay(const eda_result* r, const void* x, void* rows, char* err, size_t errlen);
 
+/* The frozen decisions of one mode of one element of a result with a tape: ints[40] = [w, W, K,
+ * used bits, segment-point bits, rung_w[0..31], sifts run after the first, accepted bits (bit j:
+ * sift j), the prescale exponent], dbls[34] = [tol, NaN, thr_j at index 1 + j]; and, into host
+ * buffers of count samples where not null, the residual at the mode's start, the scale field L
+ * (the result's dtype) and the lower rung of every voxel (uint8). For checking the decisions
+ * against the oracle's; no counterpart in the reference. */
+int eda_tape_mode(const eda_result* r, int64_t element, int32_t mode, int32_t* ints, double* dbls, void* residual,
+                  void* L, uint8_t* lo, char* err, size_t errlen);
+
 /* Frees the rows and the info arrays of a result (idempotent). */
 void eda_release(eda_result* r);
 

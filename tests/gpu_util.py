@@ -40,13 +40,14 @@ def dev(a: np.ndarray):
     return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
 
 
-def cube(x: np.ndarray, op: str, wins) -> np.ndarray | tuple[np.ndarray, np.ndarray]:
-    """x: (B, *spatial) host array; op max / min / maxmin / box; one window per element."""
+def cube(x: np.ndarray, op: str, wins, inplace: bool = False) -> np.ndarray | tuple[np.ndarray, np.ndarray]:
+    """x: (B, *spatial) host array; op max / min / maxmin / box; one window per element; inplace:
+    the first output is the input's own memory."""
     torch = torch_cuda()
     B = x.shape[0]
     sh = x.shape[1:]
     xt = dev(x)
-    o0 = torch.empty_like(xt)
+    o0 = xt if inplace else torch.empty_like(xt)
     o1 = torch.empty_like(xt)
     w = (ctypes.c_int32 * B)(*wins)
     opc = {"max": 0, "min": 1, "maxmin": 2, "box": 3}[op]
