@@ -249,6 +249,65 @@ def bench_precision(args, lib, torch) -> dict:
     }
 
 
+BATCH_METRIC = "Msamples/s decomposed, ragged batch (256 uneven 1-D signals + 32 images, FP32)"
+
+
+def bench_batching(args, torch) -> dict:
+    """The batching workload (256 chirp signals of lengths log-uniform in [1024, 262144] and 32
+    images of sides in {128 .. 1024}, float32, device-resident): the per-field form (one
+    eda.decompose call per field, in order) against eda.decompose_many (fields grouped by shape,
+    the groups on 8 host threads with a stream each). The timed region of each is one pass over
+    the workload (median of the timed steps); the rows of both are compared bitwise."""
+    import eda
+    import workloads
+
+    fs = [torch.from_numpy(f).to("cuda:0") for f in workloads.ragged_fields()]
+    n_samples = sum(f.numel() for f in fs)
+    torch.cuda.synchronize()
+    log(f"bench: batching workload, {len(fs)} fields, {n_samples / 1e6:.1f} Msamples")
+
+    def per_field():
+        return [eda.decompose(f) for f in fs]
+
+    def many():
+        return eda.decompose_many(fs, workers=8)
+
+    out = {}
+    rows = {}
+    for name, fn in (("per_field", per_field), ("decompose_many", many)):
+        for _ in range(args.warmup):
+            fn()
+        times = []
+        for _ in range(max(args.steps, 1)):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            rows[name] = fn()
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t)
+        dt = sorted(times)[len(times) // 2]
+        out[name] = {"s_per_pass": round(dt, 3), "Msamples_per_s": round(n_samples / dt / 1e6, 2)}
+        log(f"bench: batching {name}: {dt:.2f} s per pass, {n_samples / dt / 1e6:.2f} Msamples/s")
+    same = all(torch.equal(a, b) for a, b in zip(rows["per_field"], rows["decompose_many"]))
+    speedup = out["per_field"]["s_per_pass"] / out["decompose_many"]["s_per_pass"]
+    return {
+        "metric": BATCH_METRIC,
+        "value": out["decompose_many"]["Msamples_per_s"],
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "higher_is_better": True,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "eda.decompose_many over 256 float32 chirp + FM + quadratic-trend signals (lengths "
+                               "log-uniform in [1024, 262144]) and 32 float32 ring / stripe / blob images (sides "
+                               "in {128, 256, 384, 512, 768, 1024}), device-resident"},
+        "paths": out,
+        "speedup_vs_per_field": round(speedup, 2),
+        "rows_equal_per_field": bool(same),
+    }
+
+
 MONO_METRIC = "Msamples/s monogenic signal (64×512² FP32)"
 
 
@@ -450,6 +509,8 @@ def main() -> int:
                     help="measure eda.monogenic on the same batch instead of eda.decompose")
     ap.add_argument("--precision", action="store_true",
                     help="measure the 4 x 256^3 precision workload: exact vs split-BF16 DCT contractions")
+    ap.add_argument("--batching", action="store_true",
+                    help="measure the ragged batching workload: per-field calls vs eda.decompose_many")
     ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "OUT_JSON"),
                     help="summarize two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) and exit")
     args = ap.parse_args()
@@ -474,6 +535,9 @@ def main() -> int:
     lib = _native.core()
     if args.precision:
         print(json.dumps(bench_precision(args, lib, torch)), flush=True)
+        return 0
+    if args.batching:
+        print(json.dumps(bench_batching(args, torch)), flush=True)
         return 0
     log("bench: generating 64 x 512 x 512 float32 inputs")
     x = torch.from_numpy(workloads.rings_stripes_blobs(B, N, np.float32)).to("cuda:0")

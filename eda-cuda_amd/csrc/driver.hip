@@ -1805,7 +1805,9 @@ void run_call(const eda_array_desc& x, const eda_params& p, Run& run) {
   Context& ctx = context(run.device);
   if (p.dct_mode != 0 && p.dct_mode != 1) throw std::invalid_argument("eda: dct_mode must be 0 or 1");
   const DctModeScope dct_scope(std::is_same_v<T, float> ? p.dct_mode : 0);
-  if (p.stream) {
+  // device input is read on the library's stream (non-blocking) after the caller's stream: handle 0
+  // is the null stream, which a non-blocking stream does not wait for by itself
+  if (p.stream || x.device >= 0) {
     hipEvent_t ev;
     EDA_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     EDA_HIP(hipEventRecord(ev, reinterpret_cast<hipStream_t>(p.stream)));
@@ -2061,7 +2063,7 @@ int64_t run_monogenic(const eda_array_desc& x, bool check_finite, int64_t max_ch
   const int device = current_device_of(x);
   Context& ctx = context(device);
   hipStream_t s = ctx.stream;
-  if (stream) {
+  if (stream || x.device >= 0) {  // as in run_call: 0 is the null stream
     hipEvent_t ev;
     EDA_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     EDA_HIP(hipEventRecord(ev, reinterpret_cast<hipStream_t>(stream)));

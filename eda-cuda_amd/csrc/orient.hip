@@ -1068,14 +1068,19 @@ AxisMats<T> build_mats(int32_t n) {
   T* fwd = m.fwd;
   T* icos = m.icos;
   T* isin = m.isin;
-  flat(nullptr, static_cast<int64_t>(nn), [=] __device__(int64_t e) {
+  // built on a stream of its own and waited for there: the callers' streams keep running
+  hipStream_t bs = nullptr;
+  EDA_HIP(hipStreamCreateWithFlags(&bs, hipStreamNonBlocking));
+  flat(bs, static_cast<int64_t>(nn), [=] __device__(int64_t e) {
     const int64_t k = e / n, x = e - k * n;
     const int64_t q = (k * (2 * x + 1)) % N4;
     fwd[k * n + x] = static_cast<T>(dck[k] * dcs[q]);
     icos[x * n + k] = static_cast<T>(dck[k] * dcs[q]);
     isin[x * n + k] = static_cast<T>(dck[k] * dsn[q]);
   });
-  EDA_HIP(hipDeviceSynchronize());
+  const hipError_t built = hipStreamSynchronize(bs);
+  (void)hipStreamDestroy(bs);
+  EDA_HIP(built);
   EDA_HIP(hipFree(dcs));
   EDA_HIP(hipFree(dsn));
   EDA_HIP(hipFree(dck));

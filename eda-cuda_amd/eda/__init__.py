@@ -17,6 +17,7 @@ from . import _native
 from ._api import DecompositionInfo, Monogenic, Orientation, decompose, monogenic
 from ._autograd import TapedDecomposition, decompose_function, differentiable_decompose
 from ._errors import CudaError, DecompositionError
+from ._many import decompose_many
 
 __version__ = "0.1.0"
 
@@ -31,6 +32,7 @@ __all__ = [
     "build_info",
     "decompose",
     "decompose_function",
+    "decompose_many",
     "device_count",
     "differentiable_decompose",
     "device_info",

@@ -441,9 +441,9 @@ def decompose(
     coarse, then its residue, then zero rows; ``P`` is one more than the largest IMF count over the
     elements without an error. Every spatial extent must be at least 2; 1-D signals may hold up to
     2^18 samples, fields of 2 or 3 axes up to 8192 (float32) or 4096 (float64) per axis (the
-    reference stops at 8192 / 4096 for every field). ``return_orientation=True`` (2 or 3 spatial axes) also returns an
-    ``Orientation``: the direction and the coherence that the method computes at the start of each
-    mode. The result is ``y``, then ``info`` if ``return_info``, then the ``Orientation`` if
+    reference stops at 8192 / 4096 for every field). ``return_orientation=True`` (2 or 3 spatial
+    axes) also returns an ``Orientation``: the direction and the coherence that the method
+    computes at the start of each mode. The result is ``y``, then ``info`` if ``return_info``, then the ``Orientation`` if
     ``return_orientation``; asking for the orientation does not change ``y`` or ``info``.
     ``dct="bf16x3"`` (float32 only; an addition to the reference's arguments) runs every DCT and
     Riesz contraction of the orientation as split-BF16 products on the matrix cores (three bf16

@@ -80,7 +80,7 @@ typedef struct eda_params {
   int32_t orientation;  /* 1: keep the orientation (n, c) of every mode (dim >= 2) for eda_collect_orientation */
   double tie_tolerance; /* < 0: the dtype's default (2^-14 float32, 2^-30 float64) */
   int64_t max_chunk;    /* > 0: at most this many elements per chunk; 0: by free memory */
-  uint64_t stream;      /* a hipStream_t the call waits for before it reads device input (0: none) */
+  uint64_t stream;      /* a hipStream_t (0: the null stream) that device input is read after */
   int32_t keep_tape;    /* 1: keep every mode's frozen decisions in the result, for eda_backward / eda_replay */
   int32_t dct_mode;     /* float32 only: 0 the exact DCT / Riesz contractions (an LDS FFT for power-of-two
                            axes, float32 MFMA GEMMs otherwise); 1 every axis as a split-BF16 GEMM on the
@@ -138,7 +138,8 @@ int eda_collect_orientation(const eda_result* r, void* dir, void* coh, int32_t d
  * otherwise. out_device -1: host memory; else device memory of that device. nonfinite (host
  * [batch], may be NULL): 1 where check_finite and the element holds inf or NaN (its outputs are
  * still computed). chunk (may be NULL): the elements per chunk. max_chunk > 0 bounds it; stream: a
- * hipStream_t the call waits for before it reads device input (0: none). Complete on return. */
+ * hipStream_t the call waits for before it reads device input (0: the null stream). Complete on
+ * return. */
 int eda_monogenic(const eda_array_desc* x, int32_t check_finite, int64_t max_chunk, uint64_t stream, void* out,
                   int32_t out_device, int32_t* nonfinite, int64_t* chunk, char* err, size_t errlen);
 

@@ -156,3 +156,27 @@ def test_the_native_interface_casts_device_input():
     ref, _ = _core_hip.decompose_host(np.ascontiguousarray(host.reshape(2, 2, 20, 18).transpose(1, 0, 2, 3))
                                       .reshape(4, 20, 18).astype(np.float64), -1, None, 3, True, False, 0)
     assert bits(owner) == bits(ref)
+
+
+def test_input_still_being_written_on_the_null_stream():
+    # the input's last writes are queued on the default (null) stream when the call starts: the
+    # call must read it after them, as eda.decompose / eda.monogenic promise for the current stream
+    import torch
+
+    rng = np.random.default_rng(11)
+    host = rng.standard_normal((16, 256, 256)).astype(np.float32)
+    x = torch.from_numpy(host).to("cuda")
+    torch.cuda.synchronize()
+    want = eda.decompose(x, axes=(1, 2))
+    want_m = eda.monogenic(x, axes=(1, 2))
+    for _ in range(2):
+        y = x.clone()
+        for _ in range(40):
+            y = y * 1.0 + 0.0  # the same values, a fresh buffer written by queued kernels
+        assert torch.equal(eda.decompose(y, axes=(1, 2)), want)
+        z = x.clone()
+        for _ in range(40):
+            z = z * 1.0 + 0.0
+        got_m = eda.monogenic(z, axes=(1, 2))
+        for key in ("amplitude", "phase", "frequency", "direction", "coherence"):
+            assert torch.equal(getattr(got_m, key), getattr(want_m, key)), key

@@ -15,6 +15,13 @@
   exp(-|p - c_e|^2 / (2 60^2)) * sum_k cos(2 pi u_k . p / lambda_k + phi_k) + a_t exp(-|p - c_t|^2 /
   (2 (n / 2)^2)) + noise, computed in float64 per z-slab, stored as float32.
 - The differentiation workload is rings_stripes_blobs(32, 256).
+- ragged_fields(S, I): the batching workload (S = 256 signals, I = 32 images): S float32 1-D
+  signals, signal i from default_rng(i) drawn in order: the length n = round(exp(U(ln 1024,
+  ln 262144))), f0 ~ U(0.001, 0.01), f1 ~ U(0.05, 0.2), omega ~ U(0.005, 0.05); the field is the
+  linear chirp cos(2 pi (f0 t + (f1 - f0) t^2 / (2 n))) + cos(omega t + 0.8 sin(omega t)) + the
+  quadratic trend (t / n)^2; then I float32 images, image j with the side drawn from
+  default_rng(10000 + j) uniform in {128, 256, 384, 512, 768, 1024} and built as
+  rings_stripes_blobs(1, side, first=10000 + j).
 """
 
 from __future__ import annotations
@@ -97,3 +104,26 @@ def plane_wave_volumes(B: int = 4, n: int = 256, dtype=np.float32, first: int = 
             f = np.exp(-d_e / (2.0 * 60.0**2)) * w + at * np.exp(-d_t / (2.0 * st**2))
             out[k, z] = (f + noise_rng.normal(0.0, 0.02, (n, n))).astype(dtype)
     return out
+
+
+def chirp_signal(i: int, dtype=np.float32, n: int | None = None) -> np.ndarray:
+    rng = np.random.default_rng(i)
+    drawn = int(round(float(np.exp(rng.uniform(np.log(1024.0), np.log(262144.0))))))
+    n = drawn if n is None else n
+    f0, f1 = rng.uniform(0.001, 0.01), rng.uniform(0.05, 0.2)
+    om = rng.uniform(0.005, 0.05)
+    t = np.arange(n, dtype=np.float64)
+    x = np.cos(2 * np.pi * (f0 * t + (f1 - f0) * t * t / (2 * n))) + np.cos(om * t + 0.8 * np.sin(om * t))
+    return (x + (t / n) ** 2).astype(dtype)
+
+
+IMAGE_SIDES = (128, 256, 384, 512, 768, 1024)
+
+
+def ragged_image(j: int, dtype=np.float32) -> np.ndarray:
+    side = int(np.random.default_rng(10000 + j).choice(IMAGE_SIDES))
+    return rings_stripes_blobs(1, side, dtype, first=10000 + j)[0]
+
+
+def ragged_fields(S: int = 256, I: int = 32, dtype=np.float32) -> list:
+    return [chirp_signal(i, dtype) for i in range(S)] + [ragged_image(j, dtype) for j in range(I)]
