@@ -211,30 +211,122 @@ constexpr double kRungRatio = 1.4142135623730951;
 constexpr int kMaxG2Rungs = 32;
 
 // ------------------------------------------------------------------------------------------------
-// Device memory.
+// Device memory. Blocks come from a per-device cache: a released block first waits for the stream
+// it was used on, then goes back to the cache for the next call of any thread. (hipFree waits for
+// the whole device, which would serialize the calls that other threads run on their own streams.)
+// Free blocks are kept up to a quarter of the device's memory; an allocation that fails empties the
+// cache and tries once more.
+class BlockCache {
+ public:
+  static BlockCache& get() {
+    static BlockCache* c = new BlockCache();  // never destroyed: blocks may be released at exit
+    return *c;
+  }
+  // a block of at least `bytes` (its size in *size)
+  void* take(int dev, size_t bytes, size_t* size) {
+    const size_t want = round(bytes);
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      auto& fl = free_[dev];
+      auto it = fl.lower_bound(want);
+      if (it != fl.end() && it->first <= want + want / 4) {
+        void* p = it->second;
+        *size = it->first;
+        held_[dev] -= it->first;
+        fl.erase(it);
+        return p;
+      }
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, want);
+    if (e == hipErrorOutOfMemory) {
+      (void)hipGetLastError();
+      trim(dev);
+      e = hipMalloc(&p, want);
+    }
+    if (e == hipErrorOutOfMemory) {
+      (void)hipGetLastError();
+      throw OutOfDeviceMemory("eda: the device memory does not hold the call's workspace (" +
+                              std::to_string(want >> 20) + " MiB more)");
+    }
+    EDA_HIP(e);
+    *size = want;
+    return p;
+  }
+  void put(int dev, void* p, size_t size, hipStream_t s) {
+    if (s == nullptr || hipStreamSynchronize(s) != hipSuccess) {
+      (void)hipGetLastError();
+      (void)hipFree(p);
+      return;
+    }
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      if (limit_.count(dev) == 0) {
+        size_t fr = 0, total = 0;
+        limit_[dev] = hipMemGetInfo(&fr, &total) == hipSuccess ? total / 4 : 0;
+      }
+      if (held_[dev] + size <= limit_[dev]) {
+        free_[dev].emplace(size, p);
+        held_[dev] += size;
+        return;
+      }
+    }
+    (void)hipFree(p);
+  }
+  // free bytes the cache holds on a device (they count as available to a call)
+  size_t held(int dev) {
+    std::lock_guard<std::mutex> lock(mu_);
+    return held_[dev];
+  }
+  void trim(int dev) {
+    std::multimap<size_t, void*> fl;
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      fl.swap(free_[dev]);
+      held_[dev] = 0;
+    }
+    for (auto& kv : fl) (void)hipFree(kv.second);
+  }
+
+ private:
+  // 64 KiB steps below 16 MiB, 2 MiB steps above
+  static size_t round(size_t b) {
+    const size_t step = b < (size_t{16} << 20) ? (size_t{64} << 10) : (size_t{2} << 20);
+    return (b + step - 1) / step * step;
+  }
+  std::mutex mu_;
+  std::map<int, std::multimap<size_t, void*>> free_;
+  std::map<int, size_t> held_, limit_;
+};
+
+// A block of device memory used on one stream (the stream is waited for before the block is reused).
 struct DevMem {
   void* p = nullptr;
+  size_t size = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
   DevMem() = default;
-  explicit DevMem(size_t bytes) {
+  DevMem(size_t bytes, hipStream_t s) : stream(s) {
     if (bytes) {
-      const hipError_t e = hipMalloc(&p, bytes);
-      if (e == hipErrorOutOfMemory) {
-        (void)hipGetLastError();
-        throw OutOfDeviceMemory("eda: the device memory does not hold the call's workspace (" +
-                                std::to_string(bytes >> 20) + " MiB more)");
-      }
-      EDA_HIP(e);
+      EDA_HIP(hipGetDevice(&device));
+      p = BlockCache::get().take(device, bytes, &size);
     }
   }
   DevMem(const DevMem&) = delete;
   DevMem& operator=(const DevMem&) = delete;
-  DevMem(DevMem&& o) noexcept : p(o.p) { o.p = nullptr; }
+  DevMem(DevMem&& o) noexcept { swap(o); }
   DevMem& operator=(DevMem&& o) noexcept {
-    std::swap(p, o.p);
+    swap(o);
     return *this;
   }
+  void swap(DevMem& o) noexcept {
+    std::swap(p, o.p);
+    std::swap(size, o.size);
+    std::swap(device, o.device);
+    std::swap(stream, o.stream);
+  }
   ~DevMem() {
-    if (p) (void)hipFree(p);
+    if (p) BlockCache::get().put(device, p, size, stream);
   }
 };
 
@@ -292,7 +384,7 @@ struct Uploader {
     cap = bytes;
     s = stream;
     EDA_HIP(hipHostMalloc(reinterpret_cast<void**>(&host), cap, hipHostMallocDefault));
-    dev = DevMem(cap);
+    dev = DevMem(cap, stream);
   }
   ~Uploader() {
     if (t_staged == this) t_staged = nullptr;
@@ -350,14 +442,44 @@ struct Context {
     EDA_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     up.init(size_t{4} << 20, stream);
   }
-  ~Context() {
-    if (stream) (void)hipStreamDestroy(stream);
+};
+// Contexts are pooled per device: a thread takes one at its first call on a device and gives it
+// back when it ends, so that a pool of worker threads reuses the streams and every stream that a
+// released block names stays valid for the life of the process.
+struct ContextPool {
+  std::mutex mu;
+  std::map<int, std::vector<Context*>> idle;
+  static ContextPool& get() {
+    static ContextPool* p = new ContextPool();  // never destroyed: threads give contexts back at exit
+    return *p;
+  }
+  Context* take(int device) {
+    {
+      std::lock_guard<std::mutex> lock(mu);
+      auto& v = idle[device];
+      if (!v.empty()) {
+        Context* c = v.back();
+        v.pop_back();
+        return c;
+      }
+    }
+    return new Context(device);
+  }
+  void give(Context* c) {
+    std::lock_guard<std::mutex> lock(mu);
+    idle[c->device].push_back(c);
   }
 };
 Context& context(int device) {
-  thread_local std::map<int, std::unique_ptr<Context>> ctxs;
-  auto it = ctxs.find(device);
-  if (it == ctxs.end()) it = ctxs.emplace(device, std::make_unique<Context>(device)).first;
+  struct Held {
+    std::map<int, Context*> m;
+    ~Held() {
+      for (auto& kv : m) ContextPool::get().give(kv.second);
+    }
+  };
+  thread_local Held held;
+  auto it = held.m.find(device);
+  if (it == held.m.end()) it = held.m.emplace(device, ContextPool::get().take(device)).first;
   EDA_HIP(hipSetDevice(device));
   return *it->second;
 }
@@ -634,7 +756,7 @@ class Chunk {
     const size_t sel = std::max(noise_ws_bytes(g), select_ws_bytes(g));
     const size_t bytes = (nfields_ + 1) * (fbytes + 256) + 8 * (static_cast<size_t>(F) + 256) +
                          static_cast<size_t>(F) * 4 + 256 + small + sel + elem_stats_ws_bytes(g) + 4096;
-    mem_ = DevMem(bytes);
+    mem_ = DevMem(bytes, s_);
     Carve cv{static_cast<char*>(mem_.p)};
     r_ = cv.take<T>(F);
     L_ = cv.take<T>(F);
@@ -902,7 +1024,7 @@ class Chunk {
     if (need > rung_cap_) {
       EDA_HIP(hipStreamSynchronize(s_));
       rung_mem_ = DevMem();
-      rung_mem_ = DevMem(4 * need * sizeof(T));
+      rung_mem_ = DevMem(4 * need * sizeof(T), s_);
       rung_cap_ = need;
     }
     T* r0 = static_cast<T*>(rung_mem_.p);
@@ -1822,7 +1944,7 @@ void run_call(const eda_array_desc& x, const eda_params& p, Run& run) {
   run.cap = modes_max + 2;
   const double tau = p.tie_tolerance >= 0 ? p.tie_tolerance : (std::is_same_v<T, float> ? std::ldexp(1.0, -14) : std::ldexp(1.0, -30));
   const int64_t B = x.batch;
-  run.rows = DevMem(static_cast<size_t>(B) * run.cap * gall.count * sizeof(T));
+  run.rows = DevMem(static_cast<size_t>(B) * run.cap * gall.count * sizeof(T), ctx.stream);
   const size_t he = static_cast<size_t>(B) * (run.cap - 1);
   run.windows.assign(he, 0);
   run.sifts.assign(he, 0);
@@ -1831,7 +1953,7 @@ void run_call(const eda_array_desc& x, const eda_params& p, Run& run) {
   run.n_imfs.assign(B, 0);
   run.error.assign(B, 0);
   if (p.orientation && x.dim >= 2) {
-    run.orient = DevMem(static_cast<size_t>(B) * (run.cap - 1) * (x.dim + 1) * gall.count * sizeof(T));
+    run.orient = DevMem(static_cast<size_t>(B) * (run.cap - 1) * (x.dim + 1) * gall.count * sizeof(T), ctx.stream);
     run.kept_orient = true;
   }
   if (p.keep_tape) {
@@ -1845,6 +1967,7 @@ void run_call(const eda_array_desc& x, const eda_params& p, Run& run) {
   if (p.max_chunk > 0) Bc = std::min<int64_t>(Bc, p.max_chunk);
   size_t free_b = 0, total_b = 0;
   EDA_HIP(hipMemGetInfo(&free_b, &total_b));
+  free_b += BlockCache::get().held(run.device);  // cached free blocks serve the call too
   const size_t per_elem = static_cast<size_t>(gall.count) * (sizeof(T) * 40 + 16) + 4096;
   while (Bc > 1 && static_cast<size_t>(Bc) * per_elem + (size_t{512} << 20) > free_b) Bc = (Bc + 1) / 2;
   run.chunk = Bc;
@@ -1986,7 +2109,7 @@ void collect_rows(const Run& run, void* dst, int32_t dst_device) {
   if (dst_device >= 0) {
     out = static_cast<T*>(dst);
   } else {
-    tmp = DevMem(elems * sizeof(T));
+    tmp = DevMem(elems * sizeof(T), ctx.stream);
     out = static_cast<T*>(tmp.p);
   }
   const int32_t* dn = ctx.up.put(run.n_imfs);
@@ -2025,7 +2148,7 @@ void collect_orientation(const Run& run, void* dir, void* coh, int32_t dst_devic
     od = static_cast<T*>(dir);
     oc = static_cast<T*>(coh);
   } else {
-    tmp = DevMem((nd + nc) * sizeof(T));
+    tmp = DevMem((nd + nc) * sizeof(T), ctx.stream);
     od = static_cast<T*>(tmp.p);
     oc = od + nd;
   }
@@ -2077,7 +2200,7 @@ int64_t run_monogenic(const eda_array_desc& x, bool check_finite, int64_t max_ch
   DevMem dout;
   T* o = static_cast<T*>(out);
   if (out_device < 0) {
-    dout = DevMem(out_elems * sizeof(T));
+    dout = DevMem(out_elems * sizeof(T), s);
     o = static_cast<T*>(dout.p);
   }
   const double tiny = method_ctl(gall, std::is_same_v<T, float>).tiny;
@@ -2091,8 +2214,9 @@ int64_t run_monogenic(const eda_array_desc& x, bool check_finite, int64_t max_ch
   };
   size_t free_b = 0, total_b = 0;
   EDA_HIP(hipMemGetInfo(&free_b, &total_b));
+  free_b += BlockCache::get().held(device);
   while (Bc > 1 && ws_bytes(Bc) + (size_t{512} << 20) > free_b) Bc = (Bc + 1) / 2;
-  DevMem mem(ws_bytes(Bc));
+  DevMem mem(ws_bytes(Bc), s);
   std::vector<uint64_t> amax(Bc), nf(Bc);
   std::vector<int32_t> kexp(Bc);
   for (int64_t b0 = 0; b0 < B; b0 += Bc) {
@@ -2468,7 +2592,7 @@ int eda_prim_stats(int32_t dtype, int32_t dim, int64_t batch, const int64_t* n, 
   return prim(dtype, err, errlen, [&](auto tv, Context& ctx) {
     using T = decltype(tv);
     const Geom g = PrimGeom<T>(dim, batch, n).g;
-    DevMem ws(elem_stats_ws_bytes(g) + 3 * 8 * batch + 1024);
+    DevMem ws(elem_stats_ws_bytes(g) + 3 * 8 * batch + 1024, ctx.stream);
     double* de = static_cast<double*>(ws.p);
     uint64_t* da = reinterpret_cast<uint64_t*>(de + batch);
     uint64_t* dn = da + batch;
@@ -2490,7 +2614,7 @@ int eda_prim_select(int32_t dtype, int32_t dim, int64_t batch, const int64_t* n,
     using T = decltype(tv);
     const Geom g = PrimGeom<T>(dim, batch, n).g;
     if (mode != 0 && !(q >= 0.0 && q <= 1.0)) throw std::invalid_argument("eda: quantile: q must lie in [0, 1]");
-    DevMem ws(select_ws_bytes(g) + 16 * batch + 1024);
+    DevMem ws(select_ws_bytes(g) + 16 * batch + 1024, ctx.stream);
     double* dout = reinterpret_cast<double*>(static_cast<char*>(ws.p) + select_ws_bytes(g));
     int64_t* dcnt = reinterpret_cast<int64_t*>(dout + batch);
     Selection<T> sel{};
@@ -2510,7 +2634,7 @@ int eda_prim_noise(int32_t dtype, int32_t dim, int64_t batch, const int64_t* n, 
   return prim(dtype, err, errlen, [&](auto tv, Context& ctx) {
     using T = decltype(tv);
     const Geom g = PrimGeom<T>(dim, batch, n).g;
-    DevMem ws(noise_ws_bytes(g) + 8 * batch + 1024);
+    DevMem ws(noise_ws_bytes(g) + 8 * batch + 1024, ctx.stream);
     double* dout = reinterpret_cast<double*>(static_cast<char*>(ws.p) + noise_ws_bytes(g));
     enqueue_noise_sigma<T>(ctx.stream, g, static_cast<const T*>(r), per_axis != 0, dout, nullptr, ws.p);
     EDA_HIP(hipMemcpyAsync(out, dout, 8 * batch, hipMemcpyDeviceToHost, ctx.stream));
@@ -2525,7 +2649,7 @@ int eda_prim_tau(int32_t dtype, int32_t dim, int64_t batch, const int64_t* n, co
     using T = decltype(tv);
     const Geom g = PrimGeom<T>(dim, batch, n).g;
     const MethodCtl c = method_ctl(g, std::is_same_v<T, float>);
-    DevMem tmp(static_cast<size_t>(g.batch * g.count) * sizeof(T) + 4 * batch + 1024);
+    DevMem tmp(static_cast<size_t>(g.batch * g.count) * sizeof(T) + 4 * batch + 1024, ctx.stream);
     T* dtmp = static_cast<T*>(tmp.p);
     int32_t* dnf = reinterpret_cast<int32_t*>(dtmp + g.batch * g.count);
     std::vector<int32_t> wd(Wd, Wd + batch), wl(Wl, Wl + batch);
@@ -2567,7 +2691,7 @@ int eda_prim_flags(int32_t dtype, int32_t dim, int64_t batch, const int64_t* n, 
   return prim(dtype, err, errlen, [&](auto tv, Context& ctx) {
     using T = decltype(tv);
     const Geom g = PrimGeom<T>(dim, batch, n).g;
-    DevMem tmp(static_cast<size_t>(g.batch * g.count) * sizeof(T) + 24 * batch + 1024);
+    DevMem tmp(static_cast<size_t>(g.batch * g.count) * sizeof(T) + 24 * batch + 1024, ctx.stream);
     T* scratch = static_cast<T*>(tmp.p);
     uint64_t* dext = reinterpret_cast<uint64_t*>(scratch + g.batch * g.count);
     EDA_HIP(hipMemsetAsync(dext, 0, 24 * batch, ctx.stream));
@@ -2584,7 +2708,7 @@ int eda_prim_raw_window(int32_t dtype, int32_t dim, int64_t batch, const int64_t
     using T = decltype(tv);
     const Geom g = PrimGeom<T>(dim, batch, n).g;
     const MethodCtl c = method_ctl(g, std::is_same_v<T, float>);
-    DevMem tmp(static_cast<size_t>(g.batch * g.count) * sizeof(T));
+    DevMem tmp(static_cast<size_t>(g.batch * g.count) * sizeof(T), ctx.stream);
     T* dtmp = static_cast<T*>(tmp.p);
     std::vector<int32_t> w(win, win + batch);
     const int32_t mw = *std::max_element(w.begin(), w.end());
@@ -2626,7 +2750,7 @@ int eda_prim_dct(int32_t dtype, int32_t dim, int64_t batch, const int64_t* n, co
   return prim(dtype, err, errlen, [&](auto tv, Context& ctx) {
     using T = decltype(tv);
     const Geom g = PrimGeom<T>(dim, batch, n).g;
-    DevMem tmp(static_cast<size_t>(g.batch * g.count) * sizeof(T));
+    DevMem tmp(static_cast<size_t>(g.batch * g.count) * sizeof(T), ctx.stream);
     if (in == out) throw std::invalid_argument("eda: dct: in and out must differ");
     enqueue_dct<T>(ctx.stream, ctx.device, g, static_cast<const T*>(in), static_cast<T*>(out), static_cast<T*>(tmp.p),
                    inverse, odd);
@@ -2642,7 +2766,7 @@ int eda_prim_orientation(int32_t dtype, int32_t dim, int64_t batch, const int64_
     const int K = dim * (dim + 1) / 2;
     const size_t fbytes = static_cast<size_t>(g.batch * g.count) * sizeof(T) * (3 + K);
     const size_t dbytes = static_cast<size_t>(orient_scratch_doubles(batch)) * sizeof(double);
-    DevMem work(fbytes + dbytes + 4 * batch + 1024);
+    DevMem work(fbytes + dbytes + 4 * batch + 1024, ctx.stream);
     double* dscr = reinterpret_cast<double*>(static_cast<char*>(work.p) + (fbytes + 255) / 256 * 256);
     int32_t* dnf = reinterpret_cast<int32_t*>(dscr + orient_scratch_doubles(batch));
     std::vector<int32_t> wv(w, w + batch);

@@ -1465,9 +1465,10 @@ void launch_rows(hipStream_t s, const Geom& g, int axis, int nf, bool same_input
 // mod wa; inside a segment the prefix fold G runs in 32-sample chunks (a sequential fold in the
 // chunk, the chunk totals folded into a carry in order), the suffix fold H is G of the reversed
 // line, y(i) = op(H(i - r), G(i + r)) (the sum of an aligned window 0.5 (H + G)). Here each of its
-// three steps is a kernel over global memory: the chunk totals, the carries (one thread per segment,
-// in order), and every output from its chunk's carry and the fold from the chunk start (at most 32
-// reads). Used for passes whose line does not fit the LDS of the staged kernels.
+// three steps is a kernel over global memory: the chunk folds (one thread per chunk: the running
+// fold from the chunk start, kept at the positions the outputs read, and the chunk total), the
+// carries (one thread per segment, in order), and every output from two carries and two running
+// folds. Used for passes whose line does not fit the LDS of the staged kernels.
 struct LongPass {
   Geom g;
   int32_t n, nf, ops[2], scale, rungs;
@@ -1546,7 +1547,11 @@ void launch_long(hipStream_t s, const Geom& g, int axis, int nf, const int* ops,
       alias = alias || (i.first < o.second && o.first < i.second);
     }
   const int64_t stage_elems = alias ? static_cast<int64_t>(nf) * in_rungs * elems : 0;
-  const size_t need = fold_bytes + static_cast<size_t>(stage_elems) * sizeof(T);
+  // the running folds from each chunk start, at the positions the outputs read ([r, n - 1 + r] of
+  // both directions), per field, direction, rung and element
+  const int64_t elems_all = g.batch * g.count;
+  const size_t pre_elems = static_cast<size_t>(nf) * 2 * static_cast<size_t>(rungs) * elems_all;
+  const size_t need = fold_bytes + (pre_elems + static_cast<size_t>(stage_elems)) * sizeof(T);
   // one workspace per (device, stream) that only grows: launches on one stream are ordered
   static std::mutex mu;
   static std::map<std::pair<int, hipStream_t>, std::pair<void*, size_t>> cache;
@@ -1575,7 +1580,7 @@ void launch_long(hipStream_t s, const Geom& g, int axis, int nf, const int* ops,
   const T* in1 = p.in[1];
   int64_t in_stride = p.in_stride;
   if (alias) {
-    T* stage = ws + fold_bytes / sizeof(T);
+    T* stage = ws + fold_bytes / sizeof(T) + pre_elems;
     const T* src0 = in0;
     const T* src1 = in1;
     const int64_t src_stride = p.in_stride;
@@ -1593,6 +1598,10 @@ void launch_long(hipStream_t s, const Geom& g, int axis, int nf, const int* ops,
   const int32_t* win = p.win;
   const int64_t B = g.batch;
   // slot of (field f, direction dir (0 G, 1 reversed), total / carry, rung j, element b, line li, chunk c)
+  T* const pre = ws + fold_bytes / sizeof(T);
+  const auto pre_of = [=] __device__(int f, int dir, int64_t j, int64_t b, int64_t li) -> int64_t {
+    return (((static_cast<int64_t>(f) * 2 + dir) * a.rungs + j) * B + b) * a.g.count + li * a.n;
+  };
   const auto slot_of = [=] __device__(int f, int dir, int which, int64_t j, int64_t b, int64_t li) -> int64_t {
     return ((((static_cast<int64_t>(f) * 2 + dir) * 2 + which) * a.rungs + j) * B + b) * a.lines * a.cap + li * a.cap;
   };
@@ -1623,8 +1632,13 @@ void launch_long(hipStream_t s, const Geom& g, int axis, int nf, const int* ops,
     const int64_t base = long_line_base(a, b, li);
     for (int f = 0; f < a.nf; ++f) {
       const T* line = (f == 0 ? in0 : in1) + j * in_stride + base;
+      T* P = pre + pre_of(f, dir, j, b, li) - G.r;  // P[e] for e in [r, n - 1 + r]
       T acc = long_at(line, a.step, a.n, static_cast<int32_t>(e0), dir == 1);
-      for (int64_t e = e0 + 1; e <= e1; ++e) acc = long_apply(a.ops[f], acc, long_at(line, a.step, a.n, static_cast<int32_t>(e), dir == 1));
+      if (e0 >= G.r) P[e0] = acc;
+      for (int64_t e = e0 + 1; e <= e1; ++e) {
+        acc = long_apply(a.ops[f], acc, long_at(line, a.step, a.n, static_cast<int32_t>(e), dir == 1));
+        if (e >= G.r) P[e] = acc;
+      }
       ws[slot_of(f, dir, 0, j, b, li) + c] = acc;
     }
   });
@@ -1648,8 +1662,20 @@ void launch_long(hipStream_t s, const Geom& g, int axis, int nf, const int* ops,
     for (int f = 0; f < a.nf; ++f) {
       const T* tot = ws + slot_of(f, dir, 0, j, b, li) + seg * G.cps;
       T* car = ws + slot_of(f, dir, 1, j, b, li) + seg * G.cps;
+      // in order; the totals load 16 at a time ahead of the fold
       T carry = tot[0];
-      for (int32_t qc = 1; qc < chunks; ++qc) {
+      int64_t qc = 1;
+      for (; qc + 16 <= chunks; qc += 16) {
+        T v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = tot[qc + u];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          car[qc + u] = carry;
+          carry = long_apply(a.ops[f], carry, v[u]);
+        }
+      }
+      for (; qc < chunks; ++qc) {
         car[qc] = carry;
         carry = long_apply(a.ops[f], carry, tot[qc]);
       }
@@ -1679,15 +1705,11 @@ void launch_long(hipStream_t s, const Geom& g, int axis, int nf, const int* ops,
     const int64_t base = long_line_base(a, b, li);
     // G at e = i + r of the line, H(i - r) = G at n - 1 - i + r of the reversed line
     const auto fold = [&](int f, int dir, int64_t e) -> T {
-      const T* line = (f == 0 ? in0 : in1) + j * in_stride + base;
       const int64_t ks = floor_div(e - G.o, G.wa);
       const int64_t s0 = G.o + ks * static_cast<int64_t>(G.wa);
-      const int64_t d = e - s0;
-      const int64_t qc = d / 32;
+      const int64_t qc = (e - s0) / 32;
       const int64_t c = (ks - G.k_first) * G.cps + qc;
-      const int64_t e0 = s0 + 32 * qc;
-      T local = long_at(line, a.step, a.n, static_cast<int32_t>(e0), dir == 1);
-      for (int64_t x = e0 + 1; x <= e; ++x) local = long_apply(a.ops[f], local, long_at(line, a.step, a.n, static_cast<int32_t>(x), dir == 1));
+      const T local = pre[pre_of(f, dir, j, b, li) + (e - G.r)];  // the fold from the chunk start to e
       if (qc == 0) return local;
       return long_apply(a.ops[f], ws[slot_of(f, dir, 1, j, b, li) + c], local);
     };
