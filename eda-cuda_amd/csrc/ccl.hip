@@ -53,24 +53,32 @@ __device__ __forceinline__ int32_t wave_max_scan(int32_t v) {
   return v;
 }
 
-constexpr int kRun = 8;  // samples per lane per tile
+constexpr int kRun = 8;                     // samples per lane per tile
+constexpr int32_t kPiece = 64 * kRun * 8;   // samples per wave: lines longer than this split
+constexpr int32_t kPending = INT32_MIN;     // the label of a run that began in an earlier piece
 
-// One wave per line of the contiguous (last) axis: labels = the index of the run's first voxel for
-// mask voxels, -1 elsewhere.
+// One wave per piece of kPiece samples of a line of the contiguous (last) axis: labels = the index
+// of the run's first voxel for mask voxels, -1 elsewhere; kPending for the leading voxels of a piece
+// whose run began in an earlier piece (k_ccl_pending resolves them).
 __global__ void __launch_bounds__(kThreads) k_ccl_runs(Geom g, const uint8_t* __restrict__ mask, uint8_t bits,
-                                                       int32_t* __restrict__ labels, const int32_t* __restrict__ on) {
+                                                       int32_t* __restrict__ labels, const int32_t* __restrict__ on,
+                                                       int32_t pieces) {
   const int64_t b = blockIdx.y;
   if (on != nullptr && on[b] == 0) return;
   const int32_t n = g.n[g.dim - 1];
   const int64_t lines = g.count / n;
   const int lane = threadIdx.x & 63;
-  const int64_t line = static_cast<int64_t>(blockIdx.x) * (kThreads / 64) + (threadIdx.x >> 6);
-  if (line >= lines) return;  // uniform over the wave
+  const int64_t wv = static_cast<int64_t>(blockIdx.x) * (kThreads / 64) + (threadIdx.x >> 6);
+  if (wv >= lines * pieces) return;  // uniform over the wave
+  const int64_t line = wv / pieces;
+  const int32_t p0 = static_cast<int32_t>(wv - line * pieces) * kPiece;
+  const int32_t p1 = min(n, p0 + kPiece);
   const uint8_t* m = mask + b * g.count + line * n;
   int32_t* out = labels + b * g.count;
   const int32_t first = static_cast<int32_t>(line * n);  // the in-element index of the line's sample 0
-  int32_t carry = -1;  // the last run start of the earlier tiles
-  for (int32_t t0 = 0; t0 < n; t0 += 64 * kRun) {
+  // the last run start of the earlier tiles (-2: the piece's leading run began before it)
+  int32_t carry = p0 > 0 && (m[p0 - 1] & bits) != 0 ? -2 : -1;
+  for (int32_t t0 = p0; t0 < p1; t0 += 64 * kRun) {
     const int32_t i0 = t0 + lane * kRun;
     bool prev = i0 > 0 && i0 - 1 < n && (m[i0 - 1] & bits) != 0;
     bool in[kRun];
@@ -79,7 +87,7 @@ __global__ void __launch_bounds__(kThreads) k_ccl_runs(Geom g, const uint8_t* __
 #pragma unroll
     for (int k = 0; k < kRun; ++k) {
       const int32_t i = i0 + k;
-      in[k] = i < n && (m[i] & bits) != 0;
+      in[k] = i < p1 && (m[i] & bits) != 0;
       if (in[k] && !prev) run = i;
       st[k] = run;
       prev = in[k];
@@ -91,10 +99,43 @@ __global__ void __launch_bounds__(kThreads) k_ccl_runs(Geom g, const uint8_t* __
 #pragma unroll
     for (int k = 0; k < kRun; ++k) {
       const int32_t i = i0 + k;
-      if (i < n) out[first + i] = in[k] ? first + (st[k] >= 0 ? st[k] : excl) : -1;
+      if (i < p1) {
+        const int32_t s0 = st[k] >= 0 ? st[k] : excl;
+        out[first + i] = !in[k] ? -1 : (s0 >= 0 ? first + s0 : kPending);
+      }
     }
     const int32_t last = __shfl(incl, 63, 64);
     carry = last > carry ? last : carry;
+  }
+}
+
+// One wave per piece after the first: its leading run, when it began in an earlier piece, takes the
+// label of that run's start, found at the last voxel of the nearest earlier piece whose label is
+// final (a piece whose last voxel is pending lies inside the run). Another wave may finalize such a
+// voxel meanwhile: it then holds the same label.
+__global__ void __launch_bounds__(kThreads) k_ccl_pending(Geom g, int32_t* __restrict__ labels,
+                                                          const int32_t* __restrict__ on, int32_t pieces) {
+  const int64_t b = blockIdx.y;
+  if (on != nullptr && on[b] == 0) return;
+  const int32_t n = g.n[g.dim - 1];
+  const int64_t lines = g.count / n;
+  const int lane = threadIdx.x & 63;
+  const int64_t wv = static_cast<int64_t>(blockIdx.x) * (kThreads / 64) + (threadIdx.x >> 6);
+  if (wv >= lines * pieces) return;
+  const int64_t line = wv / pieces;
+  const int32_t pc = static_cast<int32_t>(wv - line * pieces);
+  if (pc == 0) return;
+  const int32_t p0 = pc * kPiece;
+  const int32_t p1 = min(n, p0 + kPiece);
+  volatile int32_t* L = labels + b * g.count + line * n;
+  if (L[p0] != kPending) return;
+  int32_t lab = kPending;
+  for (int32_t q = p0 - 1; q >= 0 && lab == kPending; q -= kPiece) lab = L[q];
+  for (int32_t t0 = p0; t0 < p1; t0 += 64) {
+    const int32_t i = t0 + lane;
+    const bool pend = i < p1 && L[i] == kPending;
+    if (pend) L[i] = lab;
+    if (__ballot(pend) != ~0ULL) break;  // the leading run ended in this block
   }
 }
 
@@ -103,10 +144,16 @@ __global__ void __launch_bounds__(kThreads) k_ccl_runs(Geom g, const uint8_t* __
 void enqueue_ccl(hipStream_t s, const Geom& g, const uint8_t* mask, uint8_t bits, int32_t* labels, const int32_t* on) {
   const int32_t nl = g.n[g.dim - 1];
   const int64_t lines = g.count / nl;
-  const int64_t gx = (lines + kThreads / 64 - 1) / (kThreads / 64);
+  const int32_t pieces = (nl + kPiece - 1) / kPiece;
+  const int64_t gx = (lines * pieces + kThreads / 64 - 1) / (kThreads / 64);
   EDA_GO(k_ccl_runs, dim3(static_cast<unsigned>(gx), static_cast<unsigned>(g.batch)), dim3(kThreads), 0, s,
-                     g, mask, bits, labels, on);
+                     g, mask, bits, labels, on, pieces);
   EDA_LAUNCH("k_ccl_runs");
+  if (pieces > 1) {
+    EDA_GO(k_ccl_pending, dim3(static_cast<unsigned>(gx), static_cast<unsigned>(g.batch)), dim3(kThreads), 0, s,
+                       g, labels, on, pieces);
+    EDA_LAUNCH("k_ccl_pending");
+  }
   if (g.dim == 1) return;  // a line's runs are its components
   each(s, g, on, [=] __device__(int64_t b, int64_t i64) {
     int32_t* L = labels + b * g.count;

@@ -1508,6 +1508,314 @@ __device__ __forceinline__ int64_t long_line_base(const LongPass& a, int64_t b, 
   return b * a.g.count + o * static_cast<int64_t>(a.n) * a.inner + c;
 }
 
+// The long pass's buffers: the inputs, the chunk totals and carries (slot), the running folds (pre).
+template <class T>
+struct LongArgs {
+  LongPass a;
+  const int32_t* win;
+  const T* in[2];
+  T* out[2];
+  int64_t in_stride, out_stride, B;
+  T* ws;
+  T* pre;
+  // field f, direction dir (0 G, 1 the reversed line), total (0) / carry (1), rung j, element b, line li
+  __device__ __forceinline__ int64_t slot_of(int f, int dir, int which, int64_t j, int64_t b, int64_t li) const {
+    return ((((static_cast<int64_t>(f) * 2 + dir) * 2 + which) * a.rungs + j) * B + b) * a.lines * a.cap + li * a.cap;
+  }
+  __device__ __forceinline__ int64_t pre_of(int f, int dir, int64_t j, int64_t b, int64_t li) const {
+    return (((static_cast<int64_t>(f) * 2 + dir) * a.rungs + j) * B + b) * a.g.count + li * a.n;
+  }
+};
+
+// A grid row's task, decoded once per workgroup into LDS: rung j, element b, line li (and the
+// direction for the fold and carry steps), the window's geometry and the line's base offset.
+struct LongTask {
+  int64_t j, b, li, base;
+  int32_t dir, w;
+  LongGeo G;
+};
+template <class T>
+__device__ __forceinline__ void long_task(const LongArgs<T>& A, int64_t t, bool with_dir, LongTask* out) {
+  LongTask k;
+  int64_t q = t;
+  k.dir = 0;
+  if (with_dir) {
+    k.dir = static_cast<int32_t>(q & 1);
+    q >>= 1;
+  }
+  k.li = q % A.a.lines;
+  q /= A.a.lines;
+  k.b = q % A.B;
+  k.j = q / A.B;
+  k.w = A.win[k.j * A.B + k.b];
+  if (k.w >= 1) k.G = long_geo(A.a.n, k.w);
+  k.base = long_line_base(A.a, k.b, k.li);
+  *out = k;
+}
+
+// Step 1: one wave per 64 consecutive chunks of a task (the waves of a grid row stride over them):
+// the chunks tile the extended line, so their samples are one range of at most 2048 positions, which
+// the wave loads coalesced into LDS (a pad word every 32 keeps the lanes' chunk walks on distinct
+// banks); each lane then folds its chunk in order there, keeping the running fold in place, and the
+// wave stores the running folds at [r, n - 1 + r] coalesced, and the chunk totals.
+constexpr int kFoldRange = 64 * 32;
+__device__ __forceinline__ int fold_pad(int p) { return p + (p >> 5); }
+
+template <class T>
+constexpr int fold_waves() {
+  return sizeof(T) == 8 ? 2 : 4;  // 34 KiB of LDS per workgroup either way
+}
+// The pass's operators as one compile-time mode: max, min, box sum (one field each) or max and min
+// (two fields); field f's operator.
+constexpr int kLongMax = 0, kLongMin = 1, kLongSum = 2, kLongMaxMin = 3;
+template <int MODE>
+__host__ __device__ constexpr int long_nf() {
+  return MODE == kLongMaxMin ? 2 : 1;
+}
+template <int MODE>
+__host__ __device__ constexpr int long_op(int f) {
+  return MODE == kLongMaxMin ? (f == 0 ? kVMax : kVMin) : (MODE == kLongMax ? kVMax : (MODE == kLongMin ? kVMin : kVSum));
+}
+// the value at extended position e of a line with unit stride (the contiguous axis) or any stride
+template <class T>
+__device__ __forceinline__ T long_at_s(const T* line, int64_t step, int32_t n, int32_t e, bool rev) {
+  int32_t i = reflect_index(e, n);
+  if (rev) i = n - 1 - i;
+  return step == 1 ? line[i] : line[static_cast<int64_t>(i) * step];
+}
+
+template <class T, int MODE>
+__global__ void __launch_bounds__(256) k_long_fold(LongArgs<T> A) {
+  constexpr int kW = fold_waves<T>();
+  __shared__ LongTask st;
+  __shared__ T buf[kW][kFoldRange + kFoldRange / 32];
+  const LongPass& a = A.a;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  T* lds = buf[wave];
+  const int64_t tasks = static_cast<int64_t>(a.rungs) * A.B * a.lines * 2;
+  for (int64_t t = blockIdx.y; t < tasks; t += gridDim.y) {
+    if (threadIdx.x == 0) long_task(A, t, true, &st);
+    __syncthreads();
+    const LongTask k = st;
+    __syncthreads();
+    if (k.w < 1) continue;
+    const LongGeo& G = k.G;
+    const int32_t nchunks = G.nseg * G.cps;
+    const int32_t last = a.n - 1 + G.r;  // the folds stop at n - 1 + r
+    const bool rev = k.dir == 1;
+    // chunk c's first and last position (e0 > last: a chunk past the folds' end)
+    const auto span = [&](int32_t c, int32_t* e0, int32_t* e1) {
+      const int32_t seg = c / G.cps, qc = c - seg * G.cps;
+      const int32_t s0 = G.o + (G.k_first + seg) * G.wa;
+      *e0 = s0 + 32 * qc;
+      *e1 = min(min(*e0 + 31, s0 + G.wa - 1), last);
+    };
+    for (int32_t c0 = (blockIdx.x * kW + wave) * 64; c0 < nchunks; c0 += gridDim.x * kW * 64) {
+      const int32_t c = c0 + lane;
+      int32_t e0 = 0, e1 = -1;
+      if (c < nchunks) span(c, &e0, &e1);
+      const bool live = c < nchunks && e0 <= last;
+      int32_t lo, hi;  // the wave's range: its first chunk's start to its last live chunk's end
+      span(c0, &lo, &hi);
+      hi = live ? e1 : lo - 1;
+#pragma unroll
+      for (int d = 32; d >= 1; d >>= 1) hi = max(hi, __shfl_xor(hi, d, 64));
+      const int cnt = hi - lo + 1;  // <= 2048
+      if (cnt <= 0) continue;       // uniform over the wave
+      const int rounds = (cnt + 63) / 64;
+      const int b0 = e0 - lo, len = e1 - e0 + 1;
+#pragma unroll
+      for (int f = 0; f < long_nf<MODE>(); ++f) {
+        const int op = long_op<MODE>(f);
+        const T* line = A.in[f] + k.j * A.in_stride + k.base;
+        T* P = A.pre + A.pre_of(f, k.dir, k.j, k.b, k.li) - G.r;  // P[e] for e in [r, n - 1 + r]
+        // coalesced loads, 8 in flight at a time (clamped reads: always valid)
+        for (int u0 = 0; u0 < rounds; u0 += 8) {
+          T v[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) v[u] = long_at_s(line, a.step, a.n, lo + min(lane + 64 * (u0 + u), cnt - 1), rev);
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+            if (lane + 64 * (u0 + u) < cnt) lds[fold_pad(lane + 64 * (u0 + u))] = v[u];
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
+        if (live) {
+          T acc = lds[fold_pad(b0)];
+          for (int u0 = 1; u0 < len; u0 += 8) {
+            T v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = lds[fold_pad(b0 + min(u0 + u, len - 1))];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+              if (u0 + u < len) {
+                acc = long_apply(op, acc, v[u]);
+                lds[fold_pad(b0 + u0 + u)] = acc;
+              }
+            }
+          }
+          A.ws[A.slot_of(f, k.dir, 0, k.j, k.b, k.li) + c] = acc;
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
+        for (int u = 0; u < rounds; ++u) {
+          const int q = lane + 64 * u;
+          if (q < cnt && lo + q >= G.r) P[lo + q] = lds[fold_pad(q)];
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
+      }
+    }
+  }
+}
+
+// The fold through the chunk totals tot[1 .. chunks - 1] of one segment in order, from carry =
+// tot[0]: car[q] = the carry before chunk q. One thread runs the chain (it is sequential in the
+// canonical order): 64 totals at a time into registers (the loads in flight together), the carries
+// over them in place, 64 stores.
+template <class T, int OP>
+__device__ __forceinline__ void carry_run(const T* tot, T* car, int32_t chunks) {
+  T carry = tot[0];
+  int32_t q0 = 1;
+  for (; q0 + 64 <= chunks; q0 += 64) {
+    T t[64];
+#pragma unroll
+    for (int u = 0; u < 64; ++u) t[u] = tot[q0 + u];
+#pragma unroll
+    for (int u = 0; u < 64; ++u) {
+      const T x = t[u];
+      t[u] = carry;
+      carry = long_apply(OP, carry, x);
+    }
+#pragma unroll
+    for (int u = 0; u < 64; ++u) car[q0 + u] = t[u];
+  }
+  for (; q0 < chunks; ++q0) {
+    car[q0] = carry;
+    carry = long_apply(OP, carry, tot[q0]);
+  }
+}
+
+// The same carries for max / min, which do not depend on the order of the fold: one wave per
+// segment, 64 totals at a time through a wave prefix scan.
+template <class T, int OP>
+__device__ __forceinline__ void carry_scan(const T* tot, T* car, int32_t chunks, int lane) {
+  T carry = tot[0];
+  for (int32_t q0 = 1; q0 < chunks; q0 += 64) {
+    const int32_t qi = q0 + lane;
+    T x = tot[qi < chunks ? qi : q0];  // lanes past the end repeat a total: only the ones before read it
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const T u = __shfl_up(x, d, 64);
+      if (lane >= d) x = long_apply(OP, u, x);
+    }
+    const T before = __shfl_up(x, 1, 64);
+    if (qi < chunks) car[qi] = lane == 0 ? carry : long_apply(OP, carry, before);
+    const int m = min(64, chunks - q0);
+    carry = long_apply(OP, carry, __shfl(x, m - 1, 64));
+  }
+}
+
+// Step 2: the carry before each chunk of a segment (chunk 0 has none), for the segments of two
+// chunks or more (wa > 32): one thread per segment through carry_run, or for max / min over
+// segments of more than 64 chunks one wave per segment through carry_scan (the workgroups of a grid
+// row stride over the segments of one task).
+template <class T, int MODE>
+__global__ void __launch_bounds__(256) k_long_carry(LongArgs<T> A) {
+  __shared__ LongTask st;
+  const LongPass& a = A.a;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int64_t tasks = static_cast<int64_t>(a.rungs) * A.B * a.lines * 2;
+  for (int64_t t = blockIdx.y; t < tasks; t += gridDim.y) {
+    if (threadIdx.x == 0) long_task(A, t, true, &st);
+    __syncthreads();
+    const LongTask k = st;
+    __syncthreads();
+    if (k.w < 1 || k.G.cps < 2) continue;
+    const LongGeo& G = k.G;
+    const int32_t last = a.n - 1 + G.r;
+    if (MODE != kLongSum && G.cps > 64) {
+      for (int32_t seg = blockIdx.x * 4 + wave; seg < G.nseg; seg += gridDim.x * 4) {
+        const int32_t s0 = G.o + (G.k_first + seg) * G.wa;
+        const int32_t chunks = min(G.cps, (last - s0) / 32 + 1);
+#pragma unroll
+        for (int f = 0; f < long_nf<MODE>(); ++f) {
+          const T* tot = A.ws + A.slot_of(f, k.dir, 0, k.j, k.b, k.li) + static_cast<int64_t>(seg) * G.cps;
+          T* car = A.ws + A.slot_of(f, k.dir, 1, k.j, k.b, k.li) + static_cast<int64_t>(seg) * G.cps;
+          if (long_op<MODE>(f) == kVMax) carry_scan<T, kVMax>(tot, car, chunks, lane);
+          else carry_scan<T, kVMin>(tot, car, chunks, lane);
+        }
+      }
+      continue;
+    }
+    for (int32_t seg = blockIdx.x * 256 + threadIdx.x; seg < G.nseg; seg += gridDim.x * 256) {
+      const int32_t s0 = G.o + (G.k_first + seg) * G.wa;
+      const int32_t chunks = min(G.cps, (last - s0) / 32 + 1);  // those the folds reach
+#pragma unroll
+      for (int f = 0; f < long_nf<MODE>(); ++f) {
+        const T* tot = A.ws + A.slot_of(f, k.dir, 0, k.j, k.b, k.li) + static_cast<int64_t>(seg) * G.cps;
+        T* car = A.ws + A.slot_of(f, k.dir, 1, k.j, k.b, k.li) + static_cast<int64_t>(seg) * G.cps;
+        if (long_op<MODE>(f) == kVMax) carry_run<T, kVMax>(tot, car, chunks);
+        else if (long_op<MODE>(f) == kVMin) carry_run<T, kVMin>(tot, car, chunks);
+        else carry_run<T, kVSum>(tot, car, chunks);
+      }
+    }
+  }
+}
+
+// Step 3: every output from the two running folds and two carries: G at e = i + r of the line,
+// H(i - r) = G at n - 1 - i + r of the reversed line (the workgroups of a grid row stride over the
+// positions of one line).
+template <class T, int MODE>
+__global__ void __launch_bounds__(256) k_long_out(LongArgs<T> A) {
+  __shared__ LongTask st;
+  const LongPass& a = A.a;
+  const int64_t tasks = static_cast<int64_t>(a.rungs) * A.B * a.lines;
+  for (int64_t t = blockIdx.y; t < tasks; t += gridDim.y) {
+    if (threadIdx.x == 0) long_task(A, t, false, &st);
+    __syncthreads();
+    const LongTask k = st;
+    __syncthreads();
+    if (k.w < 1) continue;
+    const LongGeo& G = k.G;
+    T fac = T(1);
+    if (a.scale) {
+      double prod = 1.0;
+      for (int ax = 0; ax < a.g.dim; ++ax) prod *= static_cast<double>(2 * min((k.w - 1) / 2, a.g.n[ax] - 1) + 1);
+      fac = static_cast<T>(1.0 / prod);
+    }
+    for (int32_t i = blockIdx.x * 256 + threadIdx.x; i < a.n; i += gridDim.x * 256) {
+      const auto fold = [&](int f, int op, int dir, int32_t e, int32_t at) -> T {
+        const int32_t d = e - G.o;
+        const int32_t ks = d >= 0 ? d / G.wa : -((-d + G.wa - 1) / G.wa);
+        const int32_t qc = (e - (G.o + ks * G.wa)) >> 5;
+        const T local = A.pre[A.pre_of(f, dir, k.j, k.b, k.li) + at];  // the fold from the chunk start
+        if (qc == 0) return local;
+        const int64_t c = static_cast<int64_t>(ks - G.k_first) * G.cps + qc;
+        return long_apply(op, A.ws[A.slot_of(f, dir, 1, k.j, k.b, k.li) + c], local);
+      };
+#pragma unroll
+      for (int f = 0; f < long_nf<MODE>(); ++f) {
+        const int op = long_op<MODE>(f);
+        const T gv = fold(f, op, 0, i + G.r, i);
+        const T hv = fold(f, op, 1, a.n - 1 - i + G.r, a.n - 1 - i);
+        T v;
+        if (op == kVSum) {
+          const T sum = hv + gv;
+          const int32_t m = (i - G.r - G.o) % G.wa;
+          v = m == 0 ? sum * T(0.5) : sum;  // (i - r - o) mod wa == 0: the window is one segment
+          if (a.scale) v = v * fac;
+        } else {
+          v = long_apply(op, hv, gv);
+        }
+        A.out[f][k.j * A.out_stride + k.base + static_cast<int64_t>(i) * a.step] = v;
+      }
+    }
+  }
+}
+
 template <class T>
 void launch_long(hipStream_t s, const Geom& g, int axis, int nf, const int* ops, bool scale, const VPtrs<T>& p,
                  int32_t rungs, int32_t max_w) {
@@ -1597,143 +1905,42 @@ void launch_long(hipStream_t s, const Geom& g, int axis, int nf, const int* ops,
   }
   const int32_t* win = p.win;
   const int64_t B = g.batch;
-  // slot of (field f, direction dir (0 G, 1 reversed), total / carry, rung j, element b, line li, chunk c)
   T* const pre = ws + fold_bytes / sizeof(T);
-  const auto pre_of = [=] __device__(int f, int dir, int64_t j, int64_t b, int64_t li) -> int64_t {
-    return (((static_cast<int64_t>(f) * 2 + dir) * a.rungs + j) * B + b) * a.g.count + li * a.n;
+  LongArgs<T> A{};
+  A.a = a;
+  A.win = win;
+  A.in[0] = in0;
+  A.in[1] = in1;
+  A.out[0] = p.out[0];
+  A.out[1] = p.out[1];
+  A.in_stride = in_stride;
+  A.out_stride = p.out_stride;
+  A.B = B;
+  A.ws = ws;
+  A.pre = pre;
+  const int64_t tasks = static_cast<int64_t>(rungs) * B * a.lines * 2;
+  const unsigned gy = static_cast<unsigned>(std::min<int64_t>(tasks, 65535));
+  // 1: the chunk folds and totals (about 4096 workgroups in all)
+  constexpr int kW = fold_waves<T>();
+  const int64_t gx1 = std::max<int64_t>(1, std::min<int64_t>((a.cap + 64 * kW - 1) / (64 * kW), (4096 + tasks - 1) / tasks));
+  // 2: the carries (about 1024 workgroups)
+  const int64_t segcap = L / 33 + 3;
+  const int64_t gx2 = std::max<int64_t>(1, std::min<int64_t>((segcap + 255) / 256, (1024 + tasks - 1) / tasks));
+  // 3: the outputs (about 4096 workgroups)
+  const int64_t tasks3 = tasks / 2;
+  const unsigned gy3 = static_cast<unsigned>(std::min<int64_t>(tasks3, 65535));
+  const int64_t gx3 = std::max<int64_t>(1, std::min<int64_t>((a.n + 255) / 256, (4096 + tasks3 - 1) / tasks3));
+  const int mode = nf == 2 ? kLongMaxMin : (ops[0] == kVMax ? kLongMax : (ops[0] == kVMin ? kLongMin : kLongSum));
+  const auto go = [&](auto mode_c) {
+    constexpr int M = decltype(mode_c)::value;
+    EDA_GO((k_long_fold<T, M>), dim3(static_cast<unsigned>(gx1), gy), dim3(64 * kW), 0, s, A);
+    EDA_GO((k_long_carry<T, M>), dim3(static_cast<unsigned>(gx2), gy), dim3(256), 0, s, A);
+    EDA_GO((k_long_out<T, M>), dim3(static_cast<unsigned>(gx3), gy3), dim3(256), 0, s, A);
   };
-  const auto slot_of = [=] __device__(int f, int dir, int which, int64_t j, int64_t b, int64_t li) -> int64_t {
-    return ((((static_cast<int64_t>(f) * 2 + dir) * 2 + which) * a.rungs + j) * B + b) * a.lines * a.cap + li * a.cap;
-  };
-  T* out0 = p.out[0];
-  T* out1 = p.out[1];
-  const int64_t out_stride = p.out_stride;
-  // 1: chunk totals
-  const int64_t items1 = static_cast<int64_t>(rungs) * B * a.lines * a.cap;
-  flat(s, items1 * 2, [=] __device__(int64_t it) {
-    const int dir = static_cast<int>(it & 1);
-    int64_t q = it >> 1;
-    const int64_t c = q % a.cap;
-    q /= a.cap;
-    const int64_t li = q % a.lines;
-    q /= a.lines;
-    const int64_t b = q % B;
-    const int64_t j = q / B;
-    const int32_t w = win[j * B + b];
-    if (w < 1) return;
-    const LongGeo G = long_geo(a.n, w);
-    const int64_t seg = c / G.cps, qc = c - seg * G.cps;
-    if (seg >= G.nseg) return;
-    const int64_t s0 = G.o + (G.k_first + seg) * static_cast<int64_t>(G.wa);
-    const int64_t e0 = s0 + 32 * qc;
-    const int64_t last = static_cast<int64_t>(a.n) - 1 + G.r;  // the folds stop at n - 1 + r
-    if (e0 > last) return;
-    const int64_t e1 = min(min(e0 + 31, s0 + G.wa - 1), last);
-    const int64_t base = long_line_base(a, b, li);
-    for (int f = 0; f < a.nf; ++f) {
-      const T* line = (f == 0 ? in0 : in1) + j * in_stride + base;
-      T* P = pre + pre_of(f, dir, j, b, li) - G.r;  // P[e] for e in [r, n - 1 + r]
-      T acc = long_at(line, a.step, a.n, static_cast<int32_t>(e0), dir == 1);
-      if (e0 >= G.r) P[e0] = acc;
-      for (int64_t e = e0 + 1; e <= e1; ++e) {
-        acc = long_apply(a.ops[f], acc, long_at(line, a.step, a.n, static_cast<int32_t>(e), dir == 1));
-        if (e >= G.r) P[e] = acc;
-      }
-      ws[slot_of(f, dir, 0, j, b, li) + c] = acc;
-    }
-  });
-  // 2: the carry before each chunk of a segment (chunk 0 has none)
-  flat(s, items1 * 2, [=] __device__(int64_t it) {
-    const int dir = static_cast<int>(it & 1);
-    int64_t q = it >> 1;
-    const int64_t seg = q % a.cap;
-    q /= a.cap;
-    const int64_t li = q % a.lines;
-    q /= a.lines;
-    const int64_t b = q % B;
-    const int64_t j = q / B;
-    const int32_t w = win[j * B + b];
-    if (w < 1) return;
-    const LongGeo G = long_geo(a.n, w);
-    if (seg >= G.nseg) return;
-    const int64_t s0 = G.o + (G.k_first + seg) * static_cast<int64_t>(G.wa);
-    const int64_t last = static_cast<int64_t>(a.n) - 1 + G.r;
-    const int64_t chunks = min(static_cast<int64_t>(G.cps), (last - s0) / 32 + 1);  // those the folds reach
-    for (int f = 0; f < a.nf; ++f) {
-      const T* tot = ws + slot_of(f, dir, 0, j, b, li) + seg * G.cps;
-      T* car = ws + slot_of(f, dir, 1, j, b, li) + seg * G.cps;
-      // in order; the totals load 16 at a time ahead of the fold
-      T carry = tot[0];
-      int64_t qc = 1;
-      for (; qc + 16 <= chunks; qc += 16) {
-        T v[16];
-#pragma unroll
-        for (int u = 0; u < 16; ++u) v[u] = tot[qc + u];
-#pragma unroll
-        for (int u = 0; u < 16; ++u) {
-          car[qc + u] = carry;
-          carry = long_apply(a.ops[f], carry, v[u]);
-        }
-      }
-      for (; qc < chunks; ++qc) {
-        car[qc] = carry;
-        carry = long_apply(a.ops[f], carry, tot[qc]);
-      }
-    }
-  });
-  // 3: the outputs
-  const int64_t items3 = static_cast<int64_t>(rungs) * B * g.count;
-  flat(s, items3, [=] __device__(int64_t it) {
-    const int64_t idx = it % g.count;
-    int64_t q = it / g.count;
-    const int64_t b = q % B;
-    const int64_t j = q / B;
-    const int32_t w = win[j * B + b];
-    if (w < 1) return;
-    const LongGeo G = long_geo(a.n, w);
-    // the line and the position of this output
-    int64_t li, i;
-    if (a.inner == 1) {
-      li = idx / a.n;
-      i = idx - li * a.n;
-    } else {
-      const int64_t o = idx / (static_cast<int64_t>(a.n) * a.inner);
-      const int64_t rem = idx - o * a.n * a.inner;
-      i = rem / a.inner;
-      li = o * a.inner + (rem - i * a.inner);
-    }
-    const int64_t base = long_line_base(a, b, li);
-    // G at e = i + r of the line, H(i - r) = G at n - 1 - i + r of the reversed line
-    const auto fold = [&](int f, int dir, int64_t e) -> T {
-      const int64_t ks = floor_div(e - G.o, G.wa);
-      const int64_t s0 = G.o + ks * static_cast<int64_t>(G.wa);
-      const int64_t qc = (e - s0) / 32;
-      const int64_t c = (ks - G.k_first) * G.cps + qc;
-      const T local = pre[pre_of(f, dir, j, b, li) + (e - G.r)];  // the fold from the chunk start to e
-      if (qc == 0) return local;
-      return long_apply(a.ops[f], ws[slot_of(f, dir, 1, j, b, li) + c], local);
-    };
-    T fac = T(1);
-    if (a.scale) {
-      double prod = 1.0;
-      for (int ax = 0; ax < a.g.dim; ++ax) prod *= static_cast<double>(2 * min((w - 1) / 2, a.g.n[ax] - 1) + 1);
-      fac = static_cast<T>(1.0 / prod);
-    }
-    for (int f = 0; f < a.nf; ++f) {
-      const T gv = fold(f, 0, i + G.r);
-      const T hv = fold(f, 1, static_cast<int64_t>(a.n) - 1 - i + G.r);
-      T v;
-      if (a.ops[f] == kVSum) {
-        const T sum = hv + gv;
-        v = floor_mod(i - G.r - G.o, G.wa) == 0 ? sum * T(0.5) : sum;
-        if (a.scale) v = v * fac;
-      } else {
-        v = long_apply(a.ops[f], hv, gv);
-      }
-      T* out = (f == 0 ? out0 : out1) + j * out_stride;
-      out[base + i * a.step] = v;
-    }
-  });
+  if (mode == kLongMax) go(std::integral_constant<int, kLongMax>{});
+  else if (mode == kLongMin) go(std::integral_constant<int, kLongMin>{});
+  else if (mode == kLongSum) go(std::integral_constant<int, kLongSum>{});
+  else go(std::integral_constant<int, kLongMaxMin>{});
 }
 
 template <class T>

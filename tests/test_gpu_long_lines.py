@@ -157,3 +157,17 @@ def test_long_line_primitives_are_the_oracle(dtype):
     lab = G.ccl(mask)
     for b in range(2):
         assert np.array_equal(lab[b], oracle.ccl(mask[b]))
+
+
+def test_components_across_line_pieces():
+    # 1-D lines are labelled in pieces of 4096 voxels: runs that cross one or several pieces (and a
+    # whole line in one run) take the label of their first voxel
+    n = 70000
+    m = np.ones((4, n), bool)
+    m[0, [5000, 20000, 20001, 61439]] = False  # runs over 1, 4 and 10 pieces
+    m[1] = np.random.default_rng(1).random(n) < 0.7
+    m[2, ::4096] = False  # a gap at every piece start
+    m[2, 4095::4096] = False
+    got = G.ccl(m)
+    for b in range(4):
+        assert np.array_equal(got[b], oracle.ccl(m[b])), b
