@@ -233,6 +233,7 @@ __device__ __forceinline__ uint64_t wave_inclusive(uint64_t v) {
 __global__ void __launch_bounds__(64) k_rank_step(RankState* st, uint64_t* hist, int32_t shift, int32_t width,
                                                   int32_t first, int32_t last, int32_t mode, double q,
                                                   const int32_t* on, double* out, int64_t* count) {
+  constexpr int kPer = kBins / 64;  // bins per lane at most (32)
   const int64_t b = blockIdx.x;
   if (on != nullptr && on[b] == 0) return;
   const int lane = threadIdx.x;
@@ -240,18 +241,30 @@ __global__ void __launch_bounds__(64) k_rank_step(RankState* st, uint64_t* hist,
   if (!first && !S.live) return;
   const int per = (1 << width) >> 6;
   uint64_t* row[2] = {hist + (b * 2) * kBins, hist + (b * 2 + 1) * kBins};
-  const auto lane_total = [&](const uint64_t* h) {
+  // the lane's bins of both rows, loaded once (all loads in flight together)
+  const bool two = !first && !S.merged;
+  uint64_t h[2][kPer];
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    h[0][j] = j < per ? row[0][lane * per + j] : 0;
+    h[1][j] = two && j < per ? row[1][lane * per + j] : 0;
+  }
+  const auto lane_total = [&](int r) {
     uint64_t t = 0;
-    for (int j = 0; j < per; ++j) t += h[lane * per + j];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) t += h[r][j];
     return t;
   };
   if (first) {
-    const uint64_t n = static_cast<uint64_t>(__shfl(wave_inclusive(lane_total(row[0])), 63));
+    const uint64_t n = static_cast<uint64_t>(__shfl(wave_inclusive(lane_total(0)), 63));
     S = RankState{};
     S.n = static_cast<int64_t>(n);
     S.live = n > 0 ? 1 : 0;
     S.merged = 1;
     if (n == 0) {
+#pragma unroll
+      for (int j = 0; j < kPer; ++j)
+        if (j < per) row[0][lane * per + j] = 0;
       if (lane == 0) {
         st[b] = S;
         out[b] = __longlong_as_double(0x7ff8000000000000LL);
@@ -270,22 +283,24 @@ __global__ void __launch_bounds__(64) k_rank_step(RankState* st, uint64_t* hist,
     }
   }
   for (int r = 0; r < 2; ++r) {
-    const uint64_t* h = row[S.merged ? 0 : r];
-    const uint64_t incl = wave_inclusive(lane_total(h));
+    const int hr = S.merged ? 0 : r;
+    const uint64_t tot = lane_total(hr);
+    const uint64_t incl = wave_inclusive(tot);
     const uint64_t want = S.left[r];
     const uint64_t past = __ballot(want < incl);  // lanes whose running total passes the rank
     const int owner = __ffsll(static_cast<unsigned long long>(past)) - 1;
     uint32_t bin = 0;
     uint64_t rest = 0;
     if (lane == owner) {
-      uint64_t below = incl;
-      for (int j = per - 1; j >= 0; --j) below -= h[lane * per + j];  // the running total before the lane
-      for (int j = 0; j < per; ++j) {
-        const uint64_t c = h[lane * per + j];
-        if (want < below + c) {
+      uint64_t below = incl - tot;  // the running total before the lane
+      bool found = false;
+#pragma unroll
+      for (int j = 0; j < kPer; ++j) {
+        const uint64_t c = h[hr][j];
+        if (!found && j < per && want < below + c) {
           bin = static_cast<uint32_t>(lane * per + j);
           rest = want - below;
-          break;
+          found = true;
         }
         below += c;
       }
@@ -295,9 +310,13 @@ __global__ void __launch_bounds__(64) k_rank_step(RankState* st, uint64_t* hist,
     S.key[r] |= static_cast<uint64_t>(bin) << shift;
     S.left[r] = rest;
   }
-  for (int j = 0; j < per; ++j) {
-    row[0][lane * per + j] = 0;
-    row[1][lane * per + j] = 0;
+  // clear the bins this step read (the next step counts into them again)
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    if (j < per) {
+      row[0][lane * per + j] = 0;
+      row[1][lane * per + j] = 0;
+    }
   }
   S.merged = S.key[0] == S.key[1] ? 1 : 0;
   if (lane != 0) return;
