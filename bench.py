@@ -16,13 +16,21 @@ stdout (progress goes to stderr):
              (the ledger), against 8 TB/s; `kernels` the same figures per kernel (also the rung
              gather, the extrema tiles and the DCT passes); `traffic` the HBM bytes per launch from
              rocprofv3 FETCH_SIZE + WRITE_SIZE (profiles/), null when not measured
+  per_item   the plain per-item form: the same 64 images one call per image, and the batched
+             call's speed-up over it
+  fwd_bwd    one step of the differentiation form: the forward with its tape kept, the loss
+             gradient and the hand-written adjoint back to the input
   cpu_baseline  the oracle (a C++ restatement of the method, FP64, OpenMP over images) on the whole
              batch, on this host's cores (at most 16), the median of 3 runs when a run is short
 
 With --monogenic the same batch goes through eda.monogenic instead, and the roofline is the DCT
 axis pass (the fast transform of the 512-sample axes: one read and one write of the field).
+--precision and --batching measure the precision workload (4 x 256^3 volumes, exact against
+split-BF16 contractions) and the ragged batching workload (256 uneven 1-D signals and 32 images,
+per field against eda.decompose_many) instead.
 
-Usage: python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--no-cpu] [--monogenic]
+Usage: python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--no-cpu] [--no-bwd] [--monogenic]
+       [--precision] [--batching]
 """
 
 from __future__ import annotations
@@ -450,6 +458,24 @@ def bench_fwd_bwd(args, x, torch, eda):
 
 
 
+def bench_per_item(x, torch, eda, batched_s):
+    """The plain per-item form beside the batched one: the same 64 images decomposed one call per
+    image (each image's rows bitwise its batched rows), timed over one pass after one untimed pass,
+    and the batched call's speed-up over it."""
+    for i in range(x.shape[0]):
+        eda.decompose(x[i])
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for i in range(x.shape[0]):
+        eda.decompose(x[i])
+    torch.cuda.synchronize()
+    per = time.perf_counter() - t
+    log(f"bench: per-item form: {per * 1e3:.1f} ms for {x.shape[0]} single-image calls; batched "
+        f"{batched_s * 1e3:.1f} ms ({per / batched_s:.2f}x)")
+    return {"ms_per_pass": round(per * 1e3, 2), "Msamples_per_s": round(x.numel() / per / 1e6, 2),
+            "batched_speedup": round(per / batched_s, 2)}
+
+
 def kernel_ledger(args, x, lib, torch, eda, _native, step_s):
     """Per-kernel live figures from LEDGER_STEPS extra steps run with every kernel family timed
     (HIP events on the library's stream around each launch; these steps are not the timed ones):
@@ -633,6 +659,7 @@ def main() -> int:
         },
         "roofline": roofline,
     }
+    out["per_item"] = bench_per_item(x, torch, eda, dt)
     if not args.no_bwd:
         out["fwd_bwd"] = bench_fwd_bwd(args, x, torch, eda)
     if not args.no_cpu:

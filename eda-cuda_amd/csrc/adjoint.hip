@@ -58,30 +58,60 @@ __device__ __forceinline__ int64_t line_base(const LineT& a, int64_t b, int64_t 
   return b * a.g.count + o * static_cast<int64_t>(a.n) * a.inner + c;
 }
 
-// loads rows of a rung field into s[p][i] (zeros for lines outside the tile)
-template <class T>
-__device__ __forceinline__ void stage(const LineT& a, int64_t b, int64_t tile, const T* __restrict__ src, T* s,
-                                      T mul) {
-  const int32_t total = a.P * a.n;
-  for (int32_t k = threadIdx.x; k < total; k += blockDim.x) {
-    int32_t p, i;
-    if (a.inner == 1) {
-      p = k / a.n;
-      i = k - p * a.n;
-    } else {
-      i = k / a.P;
-      p = k - i * a.P;
-    }
+// The bases of the tile's P lines (<= 16), computed once per workgroup (-1: a line outside the
+// tile); the staging loops then use 32-bit index math only.
+constexpr int kLineTileMax = 16;
+__device__ __forceinline__ void tile_bases(const LineT& a, int64_t b, int64_t tile, int64_t* sbase) {
+  if (threadIdx.x < a.P) {
     bool ok;
-    const int64_t base = line_base(a, b, tile, p, &ok);
-    s[p * a.lds_stride + i] = ok ? src[base + i * (a.inner == 1 ? 1 : a.inner)] * mul : T(0);
+    const int64_t base = line_base(a, b, tile, static_cast<int32_t>(threadIdx.x), &ok);
+    sbase[threadIdx.x] = ok ? base : -1;
+  }
+  __syncthreads();
+}
+
+// loads rows of a rung field into s[p][i] (zeros for lines outside the tile); 8 reads per thread in
+// flight at a time (clamped indices: every read is of a valid address, its value then selected)
+template <class T>
+__device__ __forceinline__ void stage(const LineT& a, const int64_t* sbase, const T* __restrict__ src, T* s, T mul) {
+  const int32_t total = a.P * a.n;
+  const int64_t step = a.inner == 1 ? 1 : a.inner;
+  const int32_t nthr = blockDim.x;
+  const auto where = [&](int32_t k, int32_t* p, int32_t* i) {
+    if (a.inner == 1) {
+      *p = k / a.n;
+      *i = k - *p * a.n;
+    } else {
+      *i = k / a.P;
+      *p = k - *i * a.P;
+    }
+  };
+  for (int32_t k0 = threadIdx.x; k0 < total; k0 += 8 * nthr) {
+    T v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      int32_t p, i;
+      where(min(k0 + u * nthr, total - 1), &p, &i);
+      const int64_t base = sbase[p];
+      v[u] = src[(base >= 0 ? base : 0) + i * (base >= 0 ? step : 0)];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int32_t k = k0 + u * nthr;
+      if (k < total) {
+        int32_t p, i;
+        where(k, &p, &i);
+        s[p * a.lds_stride + i] = sbase[p] >= 0 ? v[u] * mul : T(0);
+      }
+    }
   }
 }
 
 template <class T>
-__device__ __forceinline__ void unstage(const LineT& a, int64_t b, int64_t tile, T* __restrict__ dst, const T* s,
+__device__ __forceinline__ void unstage(const LineT& a, const int64_t* sbase, T* __restrict__ dst, const T* s,
                                         bool add) {
   const int32_t total = a.P * a.n;
+  const int64_t step = a.inner == 1 ? 1 : a.inner;
   for (int32_t k = threadIdx.x; k < total; k += blockDim.x) {
     int32_t p, i;
     if (a.inner == 1) {
@@ -91,105 +121,140 @@ __device__ __forceinline__ void unstage(const LineT& a, int64_t b, int64_t tile,
       i = k / a.P;
       p = k - i * a.P;
     }
-    bool ok;
-    const int64_t base = line_base(a, b, tile, p, &ok);
-    if (!ok) continue;
-    const int64_t at = base + i * (a.inner == 1 ? 1 : a.inner);
+    const int64_t base = sbase[p];
+    if (base < 0) continue;
+    const int64_t at = base + i * step;
     const T v = s[p * a.lds_stride + i];
     dst[at] = add ? dst[at] + v : v;
   }
 }
 
-// The transposed max (MIN: min) pass of rungs j0 .. j1 - 1. For every output x the position of the
-// window's extreme (the leftmost where values tie; the window clipped to the line, which holds the
-// same values as the half-sample reflected one) comes from van Herk / Gil-Werman blocks of 2r + 1
-// on the line extended by r on each side: the suffix extreme H from x - r to its block's end and the
-// prefix extreme G from x + r's block start. Those positions do not decrease with x, so each
-// position's gradient is the sum of a run of consecutive outputs, added by the run's first thread.
+// The transposed max (MIN: min) pass of rungs j0 .. j1 - 1. Each output x sends its gradient to the
+// position of its window's extreme, the leftmost where values tie (the window clipped to the line,
+// which holds the same values as the half-sample reflected one). Those positions come from a sparse
+// table of the line's leftmost extremes over runs of 2^k samples (levels k = 1 .. K, all lanes
+// busy): the clipped window [lo, hi] of length m, 2^k <= m < 2^(k+1), is the union of the runs of
+// 2^k from lo and to hi, and the left run's extreme wins ties. The positions do not decrease with x,
+// so the outputs that pick one position are one run of consecutive x: its gradient is the difference
+// of two prefix sums of the output gradient (in double), added by the run's first output.
 template <class T, bool MIN>
 __global__ void __launch_bounds__(kThreads) k_line_extT(LineT a, const T* __restrict__ vals, const T* __restrict__ gy,
                                                         T* __restrict__ gx, const int32_t* __restrict__ win,
-                                                        const int32_t* __restrict__ on) {
+                                                        const int32_t* __restrict__ on, int32_t levels) {
   extern __shared__ __align__(16) unsigned char lds[];
+  __shared__ int64_t sbase[kLineTileMax];
   const int32_t S = a.P * a.lds_stride;
-  T* sv = reinterpret_cast<T*>(lds);
+  double* pre = reinterpret_cast<double*>(lds);  // P (n + 1) prefix sums, stride lds_stride + 1
+  T* sv = reinterpret_cast<T*>(pre + a.P * (a.lds_stride + 1));
   T* sg = sv + S;
   T* sacc = sg + S;
-  int32_t* sh = reinterpret_cast<int32_t*>(sacc + S);
-  int32_t* sgi = sh + S;
+  uint16_t* lev = reinterpret_cast<uint16_t*>(sacc + S);  // levels 1 .. K, the picks, the run ends
   const int64_t b = blockIdx.x / a.tiles;
   const int64_t tile = blockIdx.x - b * a.tiles;
   if (on != nullptr && on[b] == 0) return;
+  tile_bases(a, b, tile, sbase);
   const int32_t j0 = a.sum_rungs ? 0 : static_cast<int32_t>(blockIdx.y);
   const int32_t j1 = a.sum_rungs ? a.rungs : j0 + 1;
   const int32_t n = a.n;
+  const int32_t st = a.lds_stride;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const auto better = [&](const T* v, int32_t i, int32_t k2) {  // i left of k2
+    return (MIN ? v[i] <= v[k2] : v[i] >= v[k2]) ? i : k2;
+  };
   for (int32_t k = threadIdx.x; k < S; k += blockDim.x) sacc[k] = T(0);
   for (int32_t j = j0; j < j1; ++j) {
     const int32_t w = win[static_cast<int64_t>(j) * a.g.batch + b];
     if (w < 1) continue;
     const int32_t r = min((w - 1) / 2, n - 1);
     const int32_t wa = 2 * r + 1;
+    int32_t K = 0;  // the levels this window needs: 2^K <= min(wa, n)
+    while (K + 1 < levels && (2 << K) <= min(wa, n)) ++K;
     __syncthreads();
-    stage<T>(a, b, tile, vals + j * a.vstride, sv, T(1));
-    stage<T>(a, b, tile, gy + j * a.F, sg, T(1));
+    stage<T>(a, sbase, vals + j * a.vstride, sv, T(1));
+    stage<T>(a, sbase, gy + j * a.F, sg, T(1));
     __syncthreads();
-    const int32_t nb = (n + 2 * r + wa - 1) / wa;
-    for (int32_t it = threadIdx.x; it < a.P * nb; it += blockDim.x) {
-      const int32_t p = it / nb, kb = it - p * nb;
-      const T* v = sv + p * a.lds_stride;
-      const int32_t e0 = -r + kb * wa, e1 = e0 + wa - 1;
-      int32_t best = -1;
-      T bv = T(0);
-      for (int32_t e = e1; e >= e0; --e) {
-        if (e >= 0 && e < n) {
-          const T x = v[e];
-          if (best < 0 || (MIN ? x <= bv : x >= bv)) {
-            best = e;
-            bv = x;
-          }
-        }
-        const int32_t xo = e + r;
-        if (xo >= 0 && xo < n) sh[p * a.lds_stride + xo] = best;
+    // levels: lev[(k - 1) S + p st + x] = the leftmost extreme of [x, x + 2^k)
+    for (int32_t k = 1; k <= K; ++k) {
+      const int32_t half = 1 << (k - 1);
+      uint16_t* cur = lev + (k - 1) * S;
+      const uint16_t* prev = lev + (k >= 2 ? k - 2 : 0) * S;
+      for (int32_t it = threadIdx.x; it < a.P * n; it += blockDim.x) {
+        const int32_t p = it / n, x = it - p * n;
+        if (x + 2 * half > n) continue;
+        const T* v = sv + p * st;
+        const int32_t l = k == 1 ? x : prev[p * st + x];
+        const int32_t rr = k == 1 ? x + 1 : prev[p * st + x + half];
+        cur[p * st + x] = static_cast<uint16_t>(better(v, l, rr));
       }
-      best = -1;
-      for (int32_t e = e0; e <= e1; ++e) {
-        if (e >= 0 && e < n) {
-          const T x = v[e];
-          if (best < 0 || (MIN ? x < bv : x > bv)) {
-            best = e;
-            bv = x;
-          }
-        }
-        const int32_t xo = e - r;
-        if (xo >= 0 && xo < n) sgi[p * a.lds_stride + xo] = best;
+      __syncthreads();
+    }
+    // the prefix sums of the output gradient, one wave per line (lane chunks, a wave scan)
+    const int32_t chunk = (n + 63) / 64;
+    for (int32_t p = wave; p < a.P; p += blockDim.x / 64) {
+      const T* gl = sg + p * st;
+      double* pl = pre + p * (st + 1);
+      const int32_t i0 = lane * chunk;
+      double tot = 0.0;
+      for (int32_t i = i0; i < min(i0 + chunk, n); ++i) tot += static_cast<double>(gl[i]);
+      double incl = tot;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const double y = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += y;
+      }
+      double run = incl - tot;
+      if (lane == 0) pl[0] = 0.0;
+      for (int32_t i = i0; i < min(i0 + chunk, n); ++i) {
+        run += static_cast<double>(gl[i]);
+        pl[i + 1] = run;
       }
     }
-    __syncthreads();
-    for (int32_t k = threadIdx.x; k < a.P * n; k += blockDim.x) {
-      const int32_t p = k / n, x = k - p * n;
-      const T* v = sv + p * a.lds_stride;
-      const int32_t h = sh[p * a.lds_stride + x], g = sgi[p * a.lds_stride + x];
-      int32_t pick;
-      if (h < 0) pick = g;
-      else if (g < 0) pick = h;
-      else pick = (MIN ? v[h] <= v[g] : v[h] >= v[g]) ? h : g;
-      sh[p * a.lds_stride + x] = pick;
+    // each output's pick
+    uint16_t* pick = lev + K * S;
+    for (int32_t it = threadIdx.x; it < a.P * n; it += blockDim.x) {
+      const int32_t p = it / n, x = it - p * n;
+      const T* v = sv + p * st;
+      const int32_t lo = max(0, x - r), hi = min(n - 1, x + r);
+      const int32_t m = hi - lo + 1;
+      const int32_t k = min(31 - __clz(m), K);  // 2^k <= m (m < 2^(K+1) unless the window is the line)
+      int32_t best = lo;
+      if (k > 0) {
+        const uint16_t* lk = lev + (k - 1) * S + p * st;
+        const int32_t len = 1 << k;
+        if (m < 2 * len) {
+          best = better(v, lk[lo], lk[hi - len + 1]);
+        } else {  // more levels than the launch holds: the runs in order, then the last one
+          best = lk[lo];
+          int32_t e = lo + len;
+          for (; e + len - 1 <= hi; e += len) best = better(v, best, lk[e]);
+          if (e <= hi) best = better(v, best, lk[hi - len + 1]);
+        }
+      } else {
+        for (int32_t e = lo + 1; e <= hi; ++e) best = better(v, best, e);  // K = 0: m = 1
+      }
+      pick[p * st + x] = static_cast<uint16_t>(best);
     }
     __syncthreads();
-    for (int32_t k = threadIdx.x; k < a.P * n; k += blockDim.x) {
-      const int32_t p = k / n, x = k - p * n;
-      const int32_t* arg = sh + p * a.lds_stride;
-      const int32_t tgt = arg[x];
-      if (x > 0 && arg[x - 1] == tgt) continue;
-      const T* gl = sg + p * a.lds_stride;
-      T s = T(0);
-      for (int32_t y = x; y < n && arg[y] == tgt; ++y) s = s + gl[y];
-      if (tgt >= 0) sacc[p * a.lds_stride + tgt] += s;
+    // the run of outputs that picks q: its first output adds the run's gradient sum
+    uint16_t* endx = pick + S;  // endx[q] = the last output that picks q
+    for (int32_t it = threadIdx.x; it < a.P * n; it += blockDim.x) {
+      const int32_t p = it / n, x = it - p * n;
+      const uint16_t* pk = pick + p * st;
+      if (x == n - 1 || pk[x + 1] != pk[x]) endx[p * st + pk[x]] = static_cast<uint16_t>(x);
+    }
+    __syncthreads();
+    for (int32_t it = threadIdx.x; it < a.P * n; it += blockDim.x) {
+      const int32_t p = it / n, x = it - p * n;
+      const uint16_t* pk = pick + p * st;
+      if (x > 0 && pk[x - 1] == pk[x]) continue;
+      const int32_t q = pk[x];
+      const double* pl = pre + p * (st + 1);
+      sacc[p * st + q] += static_cast<T>(pl[endx[p * st + q] + 1] - pl[x]);
     }
   }
   __syncthreads();
-  if (a.sum_rungs) unstage<T>(a, b, tile, gx, sacc, a.add != 0);
-  else unstage<T>(a, b, tile, gx + j0 * a.F, sacc, false);
+  if (a.sum_rungs) unstage<T>(a, sbase, gx, sacc, a.add != 0);
+  else unstage<T>(a, sbase, gx + j0 * a.F, sacc, false);
 }
 
 // The transposed box pass: y(x) = sum over p in [x - r, x + r] of v(reflect(p)) gives
@@ -204,9 +269,11 @@ __global__ void __launch_bounds__(kThreads) k_line_boxT(LineT a, const T* __rest
   double* pre = reinterpret_cast<double*>(lds);  // P (n + 1) prefix sums, stride lds_stride + 1
   T* sg = reinterpret_cast<T*>(pre + a.P * (a.lds_stride + 1));
   T* sacc = sg + S;
+  __shared__ int64_t sbase[kLineTileMax];
   const int64_t b = blockIdx.x / a.tiles;
   const int64_t tile = blockIdx.x - b * a.tiles;
   if (on != nullptr && on[b] == 0) return;
+  tile_bases(a, b, tile, sbase);
   const int32_t j0 = a.sum_rungs ? 0 : static_cast<int32_t>(blockIdx.y);
   const int32_t j1 = a.sum_rungs ? a.rungs : j0 + 1;
   const int32_t n = a.n;
@@ -223,7 +290,7 @@ __global__ void __launch_bounds__(kThreads) k_line_boxT(LineT a, const T* __rest
       fac = static_cast<T>(1.0 / prod);
     }
     __syncthreads();
-    stage<T>(a, b, tile, gy + j * a.F, sg, fac);
+    stage<T>(a, sbase, gy + j * a.F, sg, fac);
     __syncthreads();
     // prefix sums, one wave per line: lane chunks, then a wave scan of the chunk totals
     const int32_t chunk = (n + 63) / 64;
@@ -261,8 +328,8 @@ __global__ void __launch_bounds__(kThreads) k_line_boxT(LineT a, const T* __rest
     }
   }
   __syncthreads();
-  if (a.sum_rungs) unstage<T>(a, b, tile, gx, sacc, a.add != 0);
-  else unstage<T>(a, b, tile, gx + j0 * a.F, sacc, false);
+  if (a.sum_rungs) unstage<T>(a, sbase, gx, sacc, a.add != 0);
+  else unstage<T>(a, sbase, gx + j0 * a.F, sacc, false);
 }
 
 // The longest axis the transposed passes take (P n samples staged per workgroup).
@@ -292,9 +359,12 @@ LineT line_shape(const Geom& g, int axis, int32_t rungs, int64_t F, bool box) {
   return a;
 }
 
+// LDS of the transposed max / min pass: prefix sums in double, the values, the gradient and the
+// sums, and `levels` + 1 index arrays (levels 1 .. K, the picks, the run ends; K = levels - 1).
 template <class T>
-size_t ext_lds(const LineT& a) {
-  return static_cast<size_t>(a.P) * a.lds_stride * (3 * sizeof(T) + 2 * sizeof(int32_t));
+size_t ext_lds(const LineT& a, int32_t levels) {
+  return static_cast<size_t>(a.P) * (a.lds_stride + 1) * sizeof(double) +
+         static_cast<size_t>(a.P) * a.lds_stride * (3 * sizeof(T) + (levels + 1) * sizeof(uint16_t));
 }
 template <class T>
 size_t box_lds(const LineT& a) {
@@ -310,6 +380,23 @@ void line_transpose(hipStream_t s, const Geom& g, int axis, int kind /* 0 max, 1
                     bool scale, const int32_t* on) {
   const int64_t F = g.batch * g.count;
   LineT a = line_shape<T>(g, axis, rungs, F, kind == 3);
+  // max / min: the sparse table's levels for the widest window of the launch (host mirror of the
+  // windows when there is one, else the line), and tiles small enough for 3 workgroups per CU
+  int32_t levels = 1;
+  if (kind != 3) {
+    int32_t wmax = 2 * a.n - 1;
+    const int32_t* hw = static_cast<const int32_t*>(mirror_host(win, static_cast<size_t>(g.batch * rungs) * sizeof(int32_t)));
+    if (hw != nullptr) {
+      wmax = 1;
+      for (int64_t k = 0; k < g.batch * rungs; ++k) wmax = std::max(wmax, hw[k]);
+    }
+    const int32_t span = std::min(2 * std::min((wmax - 1) / 2, a.n - 1) + 1, a.n);
+    while ((2 << (levels - 1)) <= span) ++levels;  // 2^(levels - 1) <= span
+    while (a.P > 1 && ext_lds<T>(a, levels) > 48 * 1024) {
+      a.P /= 2;
+      a.tiles = a.inner == 1 ? (a.outer + a.P - 1) / a.P : a.outer * ((a.inner + a.P - 1) / a.P);
+    }
+  }
   a.vstride = vstride;
   a.sum_rungs = sum_rungs ? 1 : 0;
   a.add = add ? 1 : 0;
@@ -325,8 +412,8 @@ void line_transpose(hipStream_t s, const Geom& g, int axis, int kind /* 0 max, 1
     EDA_GO(k, grid, dim3(kThreads), lds, s, args...);
   };
   if (kind == 3) go(k_line_boxT<T>, box_lds<T>(a), a, gy, gx, win, on);
-  else if (kind == 0) go(k_line_extT<T, false>, ext_lds<T>(a), a, vals, gy, gx, win, on);
-  else go(k_line_extT<T, true>, ext_lds<T>(a), a, vals, gy, gx, win, on);
+  else if (kind == 0) go(k_line_extT<T, false>, ext_lds<T>(a, levels), a, vals, gy, gx, win, on, levels);
+  else go(k_line_extT<T, true>, ext_lds<T>(a, levels), a, vals, gy, gx, win, on, levels);
   EDA_LAUNCH("k_line_T");
 }
 

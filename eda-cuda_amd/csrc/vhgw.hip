@@ -1279,6 +1279,9 @@ __global__ void __launch_bounds__(256) k_vhgw_rows(RPass a, VPtrs<T> ptr) {
 #ifndef EDA_VHGW_LDS
 #define EDA_VHGW_LDS 65536  // the LDS a workgroup aims at (bytes)
 #endif
+#ifndef EDA_BOX_PERCU
+#define EDA_BOX_PERCU 4  // resident workgroups per CU that the persistent box sum launches
+#endif
 
 int kt_for(int32_t n) {
   const int32_t per = (n + 63) / 64;
@@ -1408,7 +1411,7 @@ int box_reads(const VPass& a, int nthr) {
 template <class T, bool PAD>
 void launch_box(int pf, hipStream_t s, const VPass& a, const VPtrs<T>& p, int32_t rungs, int W, size_t lds) {
   const int64_t items = a.g.batch * a.groups * rungs;
-  const int64_t per_cu = std::max<int64_t>(1, std::min<int64_t>(4, static_cast<int64_t>(160 * 1024 / lds)));
+  const int64_t per_cu = std::max<int64_t>(1, std::min<int64_t>(EDA_BOX_PERCU, static_cast<int64_t>(160 * 1024 / lds)));
   int64_t wgs = std::min<int64_t>((items + 7) / 8 * 8, device_cus() * per_cu);
   wgs = std::max<int64_t>(8, wgs / 8 * 8);
   const auto go = [&](auto k) {
