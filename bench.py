@@ -29,7 +29,7 @@ axis pass (the fast transform of the 512-sample axes: one read and one write of 
 split-BF16 contractions) and the ragged batching workload (256 uneven 1-D signals and 32 images,
 per field against eda.decompose_many) instead.
 
-Usage: python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--no-cpu] [--no-bwd] [--monogenic]
+Usage: python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--no-cpu] [--no-bwd] [--no-per-item] [--fwd-bwd-only] [--monogenic]
        [--precision] [--batching]
 """
 
@@ -529,6 +529,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-bwd", action="store_true", help="skip the fwd+bwd measurement")
+    ap.add_argument("--no-per-item", action="store_true", help="skip the per-item (one call per image) leg")
+    ap.add_argument("--fwd-bwd-only", action="store_true",
+                    help="run only the fwd+bwd leg (one warm-up step and the timed steps) and print its figures")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one copy kernel of known size (for scaling rocprofv3 byte counters)")
     ap.add_argument("--monogenic", action="store_true",
@@ -570,6 +573,9 @@ def main() -> int:
     torch.cuda.synchronize()
     if args.monogenic:
         print(json.dumps(bench_monogenic(args, x, lib, torch)), flush=True)
+        return 0
+    if args.fwd_bwd_only:
+        print(json.dumps(bench_fwd_bwd(args, x, torch, eda)), flush=True)
         return 0
 
     info = None
@@ -659,7 +665,8 @@ def main() -> int:
         },
         "roofline": roofline,
     }
-    out["per_item"] = bench_per_item(x, torch, eda, dt)
+    if not args.no_per_item:
+        out["per_item"] = bench_per_item(x, torch, eda, dt)
     if not args.no_bwd:
         out["fwd_bwd"] = bench_fwd_bwd(args, x, torch, eda)
     if not args.no_cpu:
