@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <limits>
 #include <vector>
 
 namespace eda {
@@ -42,7 +43,10 @@ struct LineT {
   int32_t sum_rungs;   // 1: the rungs' gradients summed into gx (+= when add); 0: rung j into gx + j F
   int32_t add;
   int32_t scale;       // box: times 1 / prod over the axes of the clamped sides
-  int32_t lds_stride;  // n rounded up to a multiple of 2, plus 1
+  int32_t lds_stride;  // LDS words per staged line (odd)
+  int32_t scr_stride;  // transposed max / min: uint16 slots of a wave's H and G positions
+  int32_t pk_stride;   // transposed max / min: uint16 slots of a wave's picks
+  int32_t xcd;         // workgroups in XCD-rotated chunks of this many tiles (0: in order)
 };
 
 __device__ __forceinline__ int64_t line_base(const LineT& a, int64_t b, int64_t tile, int32_t p, bool* ok) {
@@ -72,7 +76,9 @@ __device__ __forceinline__ void tile_bases(const LineT& a, int64_t b, int64_t ti
 
 // loads rows of a rung field into s[p][i] (zeros for lines outside the tile); 8 reads per thread in
 // flight at a time (clamped indices: every read is of a valid address, its value then selected)
-template <class T>
+__device__ __forceinline__ int32_t slot32(int32_t i) { return i + (i >> 5); }
+
+template <class T, bool SLOTS = false>
 __device__ __forceinline__ void stage(const LineT& a, const int64_t* sbase, const T* __restrict__ src, T* s, T mul) {
   const int32_t total = a.P * a.n;
   const int64_t step = a.inner == 1 ? 1 : a.inner;
@@ -101,13 +107,13 @@ __device__ __forceinline__ void stage(const LineT& a, const int64_t* sbase, cons
       if (k < total) {
         int32_t p, i;
         where(k, &p, &i);
-        s[p * a.lds_stride + i] = sbase[p] >= 0 ? v[u] * mul : T(0);
+        s[p * a.lds_stride + (SLOTS ? slot32(i) : i)] = sbase[p] >= 0 ? v[u] * mul : T(0);
       }
     }
   }
 }
 
-template <class T>
+template <class T, bool SLOTS = false>
 __device__ __forceinline__ void unstage(const LineT& a, const int64_t* sbase, T* __restrict__ dst, const T* s,
                                         bool add) {
   const int32_t total = a.P * a.n;
@@ -124,137 +130,315 @@ __device__ __forceinline__ void unstage(const LineT& a, const int64_t* sbase, T*
     const int64_t base = sbase[p];
     if (base < 0) continue;
     const int64_t at = base + i * step;
-    const T v = s[p * a.lds_stride + i];
+    const T v = s[p * a.lds_stride + (SLOTS ? slot32(i) : i)];
     dst[at] = add ? dst[at] + v : v;
   }
 }
 
 // The transposed max (MIN: min) pass of rungs j0 .. j1 - 1. Each output x sends its gradient to the
 // position of its window's extreme, the leftmost where values tie (the window clipped to the line,
-// which holds the same values as the half-sample reflected one). Those positions come from a sparse
-// table of the line's leftmost extremes over runs of 2^k samples (levels k = 1 .. K, all lanes
-// busy): the clipped window [lo, hi] of length m, 2^k <= m < 2^(k+1), is the union of the runs of
-// 2^k from lo and to hi, and the left run's extreme wins ties. The positions do not decrease with x,
-// so the outputs that pick one position are one run of consecutive x: its gradient is the difference
-// of two prefix sums of the output gradient (in double), added by the run's first output.
+// which holds the same values as the half-sample reflected one; -0 and +0 tie).
+//
+// One wave per line, the line's values, gradient and sums staged in LDS (slot i + i / 32, so that
+// lanes eight samples apart fall in distinct banks). The extreme positions come from van Herk /
+// Gil-Werman folds over blocks of L = 2r + 1 samples aligned at 0, carried as (order key, position)
+// pairs in registers: each lane folds a run of 8 consecutive samples, the folds cross lanes by a
+// segmented scan (DPP row shifts, readlane across rows), and the suffix folds H and prefix folds G
+// keep only their positions. Output x picks the better of H(max(0, x - r)) and G(min(n - 1, x + r)).
+// The picks do not decrease with x, so the outputs that pick one position form one run of
+// consecutive x: a segmented sum over the runs (in double; lane runs, then a scan across lanes)
+// leaves each run's gradient at its last output, which adds it at the picked position.
+template <class T>
+struct ExtKey {
+  okey_t<T> k;
+  int32_t i;
+};
+constexpr int kDppRowShl = 0x100, kDppRowShr = 0x110, kDppWaveShl1 = 0x130, kDppWaveShr1 = 0x138;
+template <int CTRL>
+__device__ __forceinline__ int32_t dpp_i32(int32_t old, int32_t v) {
+  return __builtin_amdgcn_update_dpp(old, v, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL, class S>
+__device__ __forceinline__ S dpp_any(S old, S v) {
+  if constexpr (sizeof(S) == 4) {
+    return __builtin_bit_cast(S, dpp_i32<CTRL>(__builtin_bit_cast(int32_t, old), __builtin_bit_cast(int32_t, v)));
+  } else {
+    const uint64_t o = __builtin_bit_cast(uint64_t, old), x = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = static_cast<uint32_t>(dpp_i32<CTRL>(static_cast<int32_t>(o), static_cast<int32_t>(x)));
+    const uint32_t hi = static_cast<uint32_t>(dpp_i32<CTRL>(static_cast<int32_t>(o >> 32), static_cast<int32_t>(x >> 32)));
+    return __builtin_bit_cast(S, (static_cast<uint64_t>(hi) << 32) | lo);
+  }
+}
+template <class S>
+__device__ __forceinline__ S readlane_any(S v, int l) {
+  if constexpr (sizeof(S) == 4) {
+    return __builtin_bit_cast(S, __builtin_amdgcn_readlane(__builtin_bit_cast(int32_t, v), l));
+  } else {
+    const uint64_t x = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int32_t>(x), l));
+    const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int32_t>(x >> 32), l));
+    return __builtin_bit_cast(S, (static_cast<uint64_t>(hi) << 32) | lo);
+  }
+}
+template <int CTRL, class T>
+__device__ __forceinline__ ExtKey<T> dpp_key(ExtKey<T> old, ExtKey<T> v) {
+  return {dpp_any<CTRL>(old.k, v.k), dpp_i32<CTRL>(old.i, v.i)};
+}
+template <class T>
+__device__ __forceinline__ ExtKey<T> readlane_key(ExtKey<T> v, int l) {
+  return {readlane_any(v.k, l), __builtin_amdgcn_readlane(v.i, l)};
+}
+// The better of two pairs: the larger (MIN: smaller) key, the smaller position where keys tie.
 template <class T, bool MIN>
-__global__ void __launch_bounds__(kThreads) k_line_extT(LineT a, const T* __restrict__ vals, const T* __restrict__ gy,
-                                                        T* __restrict__ gx, const int32_t* __restrict__ win,
-                                                        const int32_t* __restrict__ on, int32_t levels) {
+__device__ __forceinline__ ExtKey<T> key_best(ExtKey<T> a, ExtKey<T> b) {
+  const bool ta = MIN ? (a.k < b.k || (a.k == b.k && a.i < b.i)) : (a.k > b.k || (a.k == b.k && a.i < b.i));
+  return ta ? a : b;
+}
+template <class T, bool MIN>
+__device__ __forceinline__ ExtKey<T> key_ident() {
+  using K = okey_t<T>;
+  return {MIN ? std::numeric_limits<K>::max() : std::numeric_limits<K>::lowest(), 0x7fffffff};
+}
+// the order key of v with -0 mapped to +0
+template <class T>
+__device__ __forceinline__ okey_t<T> ext_okey(T v) {
+  return okey(v == T(0) ? T(0) : v);
+}
+__device__ __forceinline__ void wave_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// Inclusive segmented scan across the wave: a lane's (a, f) becomes the fold of a over the lanes
+// from the nearest lane with f set to itself, in lane order (UP) or reverse lane order; depth: the
+// doubling steps inside a 16-lane row, then the rows in order.
+template <bool UP, class V, class Op>
+__device__ __forceinline__ void lane_scan(V& a, int& f, int depth, V id, Op op) {
+  const auto step = [&](auto dc) {
+    constexpr int ctrl = UP ? kDppRowShr + decltype(dc)::value : kDppRowShl + decltype(dc)::value;
+    V a2;
+    if constexpr (std::is_same_v<V, ExtKey<float>> || std::is_same_v<V, ExtKey<double>>) a2 = dpp_key<ctrl>(id, a);
+    else a2 = dpp_any<ctrl>(id, a);
+    const int f2 = dpp_i32<ctrl>(0, f);
+    if (!f) {
+      a = UP ? op(a2, a) : op(a, a2);
+      f = f2;
+    }
+  };
+  if (depth > 0) step(std::integral_constant<int, 1>{});
+  if (depth > 1) step(std::integral_constant<int, 2>{});
+  if (depth > 2) step(std::integral_constant<int, 4>{});
+  if (depth > 3) step(std::integral_constant<int, 8>{});
+  const int row = (threadIdx.x & 63) >> 4;
+#pragma unroll
+  for (int k = 1; k < 4; ++k) {
+    const int rr = UP ? k : 3 - k;
+    const int src = UP ? 16 * rr - 1 : 16 * rr + 16;
+    V as;
+    if constexpr (std::is_same_v<V, ExtKey<float>> || std::is_same_v<V, ExtKey<double>>) as = readlane_key(a, src);
+    else as = readlane_any(a, src);
+    const int fs = __builtin_amdgcn_readlane(f, src);
+    if (row == rr && !f) {
+      a = UP ? op(as, a) : op(a, as);
+      f = fs;
+    }
+  }
+}
+
+constexpr int kExtKT = 8;                  // samples per lane
+constexpr int32_t kExtTile = 64 * kExtKT;  // samples per wave pass
+
+// One line: V the values, Gy the output gradient (slots slot32(i)); Acc (same slots) += the
+// line's input gradient; Hi, Gi (slots slot32(i)) and Pk (plain) the wave's scratch.
+template <class T, bool MIN>
+__device__ void ext_line_vjp(const T* __restrict__ V, const T* __restrict__ Gy, T* __restrict__ Acc,
+                             uint16_t* __restrict__ Hi, uint16_t* __restrict__ Gi, uint16_t* __restrict__ Pk,
+                             int32_t n, int32_t r) {
+  constexpr int KT = kExtKT;
+  using E = ExtKey<T>;
+  const int lane = threadIdx.x & 63;
+  const int32_t L = 2 * r + 1;
+  const float inv = 1.0f / static_cast<float>(L);
+  uint32_t pat = 0;
+  for (int32_t j = 0; j < KT; j += L) pat |= 1u << j;
+  const int32_t chain = (L - 1) / KT;
+  const int depth = chain == 0 ? 0 : chain < 2 ? 1 : chain < 4 ? 2 : chain < 8 ? 3 : 4;
+  const int32_t ntiles = (n + kExtTile - 1) / kExtTile;
+  const E id = key_ident<T, MIN>();
+  const auto best = [](E a, E b) { return key_best<T, MIN>(a, b); };
+  const auto starts_of = [&](int32_t i0) {
+    int32_t m0 = i0 - static_cast<int32_t>(static_cast<float>(i0) * inv) * L;
+    m0 += m0 < 0 ? L : (m0 >= L ? -L : 0);
+    const int32_t k0 = m0 == 0 ? 0 : L - m0;
+    return k0 < KT ? (pat << k0) & ((1u << KT) - 1u) : 0u;
+  };
+  // G: prefix folds inside the blocks, left to right
+  E carry = id;
+  for (int32_t t = 0; t < ntiles; ++t) {
+    const int32_t i0 = t * kExtTile + lane * KT;
+    const uint32_t starts = starts_of(i0);
+    E gv[KT];
+    E g = id;
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+      const int32_t x = i0 + k;
+      const E e = x < n ? E{ext_okey(V[slot32(x)]), x} : id;
+      g = (starts >> k) & 1u ? e : best(g, e);
+      gv[k] = g;
+    }
+    E a = gv[KT - 1];
+    int f = starts != 0;
+    if (chain > 0) lane_scan<true>(a, f, depth, id, best);
+    if (!f) a = best(carry, a);
+    const E prv = dpp_key<kDppWaveShr1>(carry, a);
+    const int kf = starts ? __ffs(static_cast<int>(starts)) - 1 : KT;
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+      const E v = k < kf ? best(prv, gv[k]) : gv[k];
+      if (i0 + k < n) Gi[slot32(i0 + k)] = static_cast<uint16_t>(v.i);
+    }
+    carry = readlane_key(a, 63);
+  }
+  // H: suffix folds inside the blocks, right to left
+  carry = id;
+  for (int32_t t = ntiles - 1; t >= 0; --t) {
+    const int32_t i0 = t * kExtTile + lane * KT;
+    const uint32_t starts = starts_of(i0);
+    int32_t me = i0 + KT - static_cast<int32_t>(static_cast<float>(i0 + KT) * inv) * L;
+    me += me < 0 ? L : (me >= L ? -L : 0);
+    const uint32_t ends = (starts >> 1) | (me == 0 ? 1u << (KT - 1) : 0u);
+    E hv[KT];
+    E h = id;
+#pragma unroll
+    for (int k = KT - 1; k >= 0; --k) {
+      const int32_t x = i0 + k;
+      const E e = x < n ? E{ext_okey(V[slot32(x)]), x} : id;
+      h = (ends >> k) & 1u ? e : best(e, h);
+      hv[k] = h;
+    }
+    E a = hv[0];
+    int f = ends != 0;
+    if (chain > 0) lane_scan<false>(a, f, depth, id, best);
+    if (!f) a = best(a, carry);
+    const E nxt = dpp_key<kDppWaveShl1>(carry, a);
+    const int kl = ends ? 31 - __clz(static_cast<int>(ends)) : -1;
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+      const E v = k > kl ? best(hv[k], nxt) : hv[k];
+      if (i0 + k < n) Hi[slot32(i0 + k)] = static_cast<uint16_t>(v.i);
+    }
+    carry = readlane_key(a, 0);
+  }
+  wave_fence();
+  // the picks
+  const int32_t mlast = (n - 1) - static_cast<int32_t>(static_cast<float>(n - 1) * inv) * L;
+  const int32_t last = (n - 1) - (mlast < 0 ? mlast + L : (mlast >= L ? mlast - L : mlast));  // the last block's start
+  for (int32_t x = lane; x < n; x += 64) {
+    const int32_t lo = x - r > 0 ? x - r : 0;
+    const int32_t hi = x + r < n - 1 ? x + r : n - 1;
+    int32_t p;
+    if (x < r) {
+      p = Gi[slot32(hi)];
+    } else if (x + r > n - 1 && lo >= last) {
+      p = Hi[slot32(lo)];
+    } else {
+      const int32_t ph = Hi[slot32(lo)], pg = Gi[slot32(hi)];
+      const okey_t<T> kh = ext_okey(V[slot32(ph)]), kg = ext_okey(V[slot32(pg)]);
+      p = (MIN ? kh <= kg : kh >= kg) ? ph : pg;  // ph < pg unless both are the whole block's extreme
+    }
+    Pk[x] = static_cast<uint16_t>(p);
+  }
+  wave_fence();
+  // segmented sums of the gradient over the runs of equal picks, left to right
+  double cs = 0.0;    // the open run's sum at the end of the previous tile
+  int32_t cp = -1;    // the previous tile's last pick
+  for (int32_t t = 0; t < ntiles; ++t) {
+    const int32_t i0 = t * kExtTile + lane * KT;
+    int32_t p[KT];
+    double s[KT];
+#pragma unroll
+    for (int k = 0; k < KT; ++k) p[k] = i0 + k < n ? static_cast<int32_t>(Pk[i0 + k]) : 0x10000;
+    const int32_t pl = dpp_i32<kDppWaveShr1>(cp, p[KT - 1]);
+    uint32_t st = 0;
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+      const bool s0 = p[k] != (k ? p[k - 1] : pl);
+      st |= s0 ? 1u << k : 0u;
+      const double gv = i0 + k < n ? static_cast<double>(Gy[slot32(i0 + k)]) : 0.0;
+      acc = s0 ? gv : acc + gv;
+      s[k] = acc;
+    }
+    double a = s[KT - 1];
+    int f = st != 0;
+    lane_scan<true>(a, f, 4, 0.0, [](double u, double v) { return u + v; });
+    if (!f) a = cs + a;
+    const double prv = dpp_any<kDppWaveShr1>(cs, a);
+    const int kf = st ? __ffs(static_cast<int>(st)) - 1 : KT;
+#pragma unroll
+    for (int k = 0; k < KT; ++k)
+      if (k < kf) s[k] = prv + s[k];
+    // the last output of a run adds the run's sum at its pick
+    const int32_t after = i0 + KT < n ? static_cast<int32_t>(Pk[i0 + KT]) : 0x10000;
+    const int32_t pn = dpp_i32<kDppWaveShl1>(after, p[0]);
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+      const int32_t x = i0 + k;
+      const int32_t q = k + 1 < KT ? p[k + 1] : pn;
+      if (x < n && (x == n - 1 || q != p[k])) Acc[slot32(p[k])] += static_cast<T>(s[k]);
+    }
+    cs = readlane_any(a, 63);
+    cp = __builtin_amdgcn_readlane(p[KT - 1], 63);
+  }
+  wave_fence();
+}
+
+// The workgroup stages P lines of the tile (values once when every rung reads the same input, else
+// per rung; the output gradient per rung), and its waves take the lines in turn.
+template <class T, bool MIN>
+__global__ void __launch_bounds__(512) k_line_extT(LineT a, const T* __restrict__ vals, const T* __restrict__ gy,
+                                                   T* __restrict__ gx, const int32_t* __restrict__ win,
+                                                   const int32_t* __restrict__ on) {
   extern __shared__ __align__(16) unsigned char lds[];
   __shared__ int64_t sbase[kLineTileMax];
-  const int32_t S = a.P * a.lds_stride;
-  double* pre = reinterpret_cast<double*>(lds);  // P (n + 1) prefix sums, stride lds_stride + 1
-  T* sv = reinterpret_cast<T*>(pre + a.P * (a.lds_stride + 1));
+  const int32_t st = a.lds_stride;
+  const int32_t S = a.P * st;
+  T* sv = reinterpret_cast<T*>(lds);
   T* sg = sv + S;
   T* sacc = sg + S;
-  uint16_t* lev = reinterpret_cast<uint16_t*>(sacc + S);  // levels 1 .. K, the picks, the run ends
-  const int64_t b = blockIdx.x / a.tiles;
-  const int64_t tile = blockIdx.x - b * a.tiles;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int W = blockDim.x >> 6;
+  const int32_t sst = a.scr_stride;  // uint16 slots of Hi / Gi; Pk: ntiles * 512
+  uint16_t* scr = reinterpret_cast<uint16_t*>(sacc + S) + static_cast<int64_t>(wave) * (2 * sst + a.pk_stride);
+  uint16_t* Hi = scr;
+  uint16_t* Gi = scr + sst;
+  uint16_t* Pk = scr + 2 * sst;
+  const int64_t wg = a.xcd ? xcd_item(blockIdx.x, a.xcd) : static_cast<int64_t>(blockIdx.x);
+  const int64_t b = wg / a.tiles;
+  const int64_t tile = wg - b * a.tiles;
   if (on != nullptr && on[b] == 0) return;
   tile_bases(a, b, tile, sbase);
   const int32_t j0 = a.sum_rungs ? 0 : static_cast<int32_t>(blockIdx.y);
   const int32_t j1 = a.sum_rungs ? a.rungs : j0 + 1;
   const int32_t n = a.n;
-  const int32_t st = a.lds_stride;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const auto better = [&](const T* v, int32_t i, int32_t k2) {  // i left of k2
-    return (MIN ? v[i] <= v[k2] : v[i] >= v[k2]) ? i : k2;
-  };
+  (void)lane;
   for (int32_t k = threadIdx.x; k < S; k += blockDim.x) sacc[k] = T(0);
+  bool have_v = false;
   for (int32_t j = j0; j < j1; ++j) {
     const int32_t w = win[static_cast<int64_t>(j) * a.g.batch + b];
     if (w < 1) continue;
     const int32_t r = min((w - 1) / 2, n - 1);
-    const int32_t wa = 2 * r + 1;
-    int32_t K = 0;  // the levels this window needs: 2^K <= min(wa, n)
-    while (K + 1 < levels && (2 << K) <= min(wa, n)) ++K;
     __syncthreads();
-    stage<T>(a, sbase, vals + j * a.vstride, sv, T(1));
-    stage<T>(a, sbase, gy + j * a.F, sg, T(1));
+    if (!have_v || a.vstride != 0) stage<T, true>(a, sbase, vals + j * a.vstride, sv, T(1));
+    have_v = true;
+    stage<T, true>(a, sbase, gy + j * a.F, sg, T(1));
     __syncthreads();
-    // levels: lev[(k - 1) S + p st + x] = the leftmost extreme of [x, x + 2^k)
-    for (int32_t k = 1; k <= K; ++k) {
-      const int32_t half = 1 << (k - 1);
-      uint16_t* cur = lev + (k - 1) * S;
-      const uint16_t* prev = lev + (k >= 2 ? k - 2 : 0) * S;
-      for (int32_t it = threadIdx.x; it < a.P * n; it += blockDim.x) {
-        const int32_t p = it / n, x = it - p * n;
-        if (x + 2 * half > n) continue;
-        const T* v = sv + p * st;
-        const int32_t l = k == 1 ? x : prev[p * st + x];
-        const int32_t rr = k == 1 ? x + 1 : prev[p * st + x + half];
-        cur[p * st + x] = static_cast<uint16_t>(better(v, l, rr));
-      }
-      __syncthreads();
-    }
-    // the prefix sums of the output gradient, one wave per line (lane chunks, a wave scan)
-    const int32_t chunk = (n + 63) / 64;
-    for (int32_t p = wave; p < a.P; p += blockDim.x / 64) {
-      const T* gl = sg + p * st;
-      double* pl = pre + p * (st + 1);
-      const int32_t i0 = lane * chunk;
-      double tot = 0.0;
-      for (int32_t i = i0; i < min(i0 + chunk, n); ++i) tot += static_cast<double>(gl[i]);
-      double incl = tot;
-#pragma unroll
-      for (int d = 1; d < 64; d <<= 1) {
-        const double y = __shfl_up(incl, d, 64);
-        if (lane >= d) incl += y;
-      }
-      double run = incl - tot;
-      if (lane == 0) pl[0] = 0.0;
-      for (int32_t i = i0; i < min(i0 + chunk, n); ++i) {
-        run += static_cast<double>(gl[i]);
-        pl[i + 1] = run;
-      }
-    }
-    // each output's pick
-    uint16_t* pick = lev + K * S;
-    for (int32_t it = threadIdx.x; it < a.P * n; it += blockDim.x) {
-      const int32_t p = it / n, x = it - p * n;
-      const T* v = sv + p * st;
-      const int32_t lo = max(0, x - r), hi = min(n - 1, x + r);
-      const int32_t m = hi - lo + 1;
-      const int32_t k = min(31 - __clz(m), K);  // 2^k <= m (m < 2^(K+1) unless the window is the line)
-      int32_t best = lo;
-      if (k > 0) {
-        const uint16_t* lk = lev + (k - 1) * S + p * st;
-        const int32_t len = 1 << k;
-        if (m < 2 * len) {
-          best = better(v, lk[lo], lk[hi - len + 1]);
-        } else {  // more levels than the launch holds: the runs in order, then the last one
-          best = lk[lo];
-          int32_t e = lo + len;
-          for (; e + len - 1 <= hi; e += len) best = better(v, best, lk[e]);
-          if (e <= hi) best = better(v, best, lk[hi - len + 1]);
-        }
-      } else {
-        for (int32_t e = lo + 1; e <= hi; ++e) best = better(v, best, e);  // K = 0: m = 1
-      }
-      pick[p * st + x] = static_cast<uint16_t>(best);
-    }
-    __syncthreads();
-    // the run of outputs that picks q: its first output adds the run's gradient sum
-    uint16_t* endx = pick + S;  // endx[q] = the last output that picks q
-    for (int32_t it = threadIdx.x; it < a.P * n; it += blockDim.x) {
-      const int32_t p = it / n, x = it - p * n;
-      const uint16_t* pk = pick + p * st;
-      if (x == n - 1 || pk[x + 1] != pk[x]) endx[p * st + pk[x]] = static_cast<uint16_t>(x);
-    }
-    __syncthreads();
-    for (int32_t it = threadIdx.x; it < a.P * n; it += blockDim.x) {
-      const int32_t p = it / n, x = it - p * n;
-      const uint16_t* pk = pick + p * st;
-      if (x > 0 && pk[x - 1] == pk[x]) continue;
-      const int32_t q = pk[x];
-      const double* pl = pre + p * (st + 1);
-      sacc[p * st + q] += static_cast<T>(pl[endx[p * st + q] + 1] - pl[x]);
-    }
+    for (int32_t p = wave; p < a.P; p += W)
+      if (sbase[p] >= 0) ext_line_vjp<T, MIN>(sv + p * st, sg + p * st, sacc + p * st, Hi, Gi, Pk, n, r);
   }
   __syncthreads();
-  if (a.sum_rungs) unstage<T>(a, sbase, gx, sacc, a.add != 0);
-  else unstage<T>(a, sbase, gx + j0 * a.F, sacc, false);
+  if (a.sum_rungs) unstage<T, true>(a, sbase, gx, sacc, a.add != 0);
+  else unstage<T, true>(a, sbase, gx + j0 * a.F, sacc, false);
 }
 
 // The transposed box pass: y(x) = sum over p in [x - r, x + r] of v(reflect(p)) gives
@@ -359,17 +543,39 @@ LineT line_shape(const Geom& g, int axis, int32_t rungs, int64_t F, bool box) {
   return a;
 }
 
-// LDS of the transposed max / min pass: prefix sums in double, the values, the gradient and the
-// sums, and `levels` + 1 index arrays (levels 1 .. K, the picks, the run ends; K = levels - 1).
-template <class T>
-size_t ext_lds(const LineT& a, int32_t levels) {
-  return static_cast<size_t>(a.P) * (a.lds_stride + 1) * sizeof(double) +
-         static_cast<size_t>(a.P) * a.lds_stride * (3 * sizeof(T) + (levels + 1) * sizeof(uint16_t));
-}
 template <class T>
 size_t box_lds(const LineT& a) {
   return static_cast<size_t>(a.P) * (a.lds_stride + 1) * sizeof(double) +
          static_cast<size_t>(a.P) * a.lds_stride * 2 * sizeof(T);
+}
+
+// The transposed max / min pass's tile: P lines (the values, the gradient and the sums in slots
+// i + i / 32, an odd number of words per line) and per wave the H, G positions and the picks.
+constexpr int32_t host_slot32(int32_t i) { return i + (i >> 5); }
+template <class T>
+size_t ext_lds(const LineT& a, int waves) {
+  return static_cast<size_t>(3) * a.P * a.lds_stride * sizeof(T) +
+         static_cast<size_t>(waves) * (2 * a.scr_stride + a.pk_stride) * sizeof(uint16_t);
+}
+template <class T>
+int ext_shape(LineT& a) {
+  const int32_t span = (a.n + kExtTile - 1) / kExtTile * kExtTile;
+  a.lds_stride = host_slot32(span) | 1;
+  a.scr_stride = (host_slot32(span) + 1) & ~1;
+  a.pk_stride = span;
+  // rows: 4 adjacent lines, one per wave; columns: 16 adjacent columns (64-byte runs), 8 waves
+  int32_t P = a.inner == 1 ? 4 : 16;
+  P = static_cast<int32_t>(std::min<int64_t>(P, a.inner == 1 ? a.outer : a.inner));
+  int waves = a.inner == 1 ? 4 : 8;
+  for (;;) {
+    LineT t = a;
+    t.P = P;
+    if (ext_lds<T>(t, std::min(waves, P)) <= 160 * 1024 || P == 1) break;
+    P /= 2;
+  }
+  a.P = P;
+  a.tiles = a.inner == 1 ? (a.outer + P - 1) / P : a.outer * ((a.inner + P - 1) / P);
+  return std::max(1, std::min(waves, P));
 }
 
 // One transposed pass over all rungs: vals (max / min: the pass's input), gy (the gradient of its
@@ -380,40 +586,25 @@ void line_transpose(hipStream_t s, const Geom& g, int axis, int kind /* 0 max, 1
                     bool scale, const int32_t* on) {
   const int64_t F = g.batch * g.count;
   LineT a = line_shape<T>(g, axis, rungs, F, kind == 3);
-  // max / min: the sparse table's levels for the widest window of the launch (host mirror of the
-  // windows when there is one, else the line), and tiles small enough for 3 workgroups per CU
-  int32_t levels = 1;
-  if (kind != 3) {
-    int32_t wmax = 2 * a.n - 1;
-    const int32_t* hw = static_cast<const int32_t*>(mirror_host(win, static_cast<size_t>(g.batch * rungs) * sizeof(int32_t)));
-    if (hw != nullptr) {
-      wmax = 1;
-      for (int64_t k = 0; k < g.batch * rungs; ++k) wmax = std::max(wmax, hw[k]);
-    }
-    const int32_t span = std::min(2 * std::min((wmax - 1) / 2, a.n - 1) + 1, a.n);
-    while ((2 << (levels - 1)) <= span) ++levels;  // 2^(levels - 1) <= span
-    while (a.P > 1 && ext_lds<T>(a, levels) > 48 * 1024) {
-      a.P /= 2;
-      a.tiles = a.inner == 1 ? (a.outer + a.P - 1) / a.P : a.outer * ((a.inner + a.P - 1) / a.P);
-    }
-  }
+  const int waves = kind == 3 ? kThreads / 64 : ext_shape<T>(a);
   a.vstride = vstride;
   a.sum_rungs = sum_rungs ? 1 : 0;
   a.add = add ? 1 : 0;
   a.scale = scale ? 1 : 0;
   const int64_t wgs = g.batch * a.tiles;
   if (wgs > 0x7fffffff) throw std::invalid_argument("eda: too many workgroups for a transposed pass");
+  a.xcd = kind == 3 || a.inner == 1 ? 0 : xcd_chunk(wgs);
   const dim3 grid(static_cast<unsigned>(wgs), sum_rungs ? 1u : static_cast<unsigned>(rungs));
   const auto go = [&](auto k, size_t lds, auto... args) {
     if (lds > 160 * 1024) throw std::invalid_argument("eda: the line is too long for the backward");
     if (lds > 65536)
       EDA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
                                   static_cast<int>(lds)));
-    EDA_GO(k, grid, dim3(kThreads), lds, s, args...);
+    EDA_GO(k, grid, dim3(64 * waves), lds, s, args...);
   };
   if (kind == 3) go(k_line_boxT<T>, box_lds<T>(a), a, gy, gx, win, on);
-  else if (kind == 0) go(k_line_extT<T, false>, ext_lds<T>(a, levels), a, vals, gy, gx, win, on, levels);
-  else go(k_line_extT<T, true>, ext_lds<T>(a, levels), a, vals, gy, gx, win, on, levels);
+  else if (kind == 0) go(k_line_extT<T, false>, ext_lds<T>(a, waves), a, vals, gy, gx, win, on);
+  else go(k_line_extT<T, true>, ext_lds<T>(a, waves), a, vals, gy, gx, win, on);
   EDA_LAUNCH("k_line_T");
 }
 
