@@ -598,6 +598,7 @@ struct VPass {
   int32_t wrows;    // LDS rows per wave: 1 (max / min of one tile: H in place), 2 (H and G)
   int32_t epad;     // box sum: LDS slots per reflected line (extended position e at e + lead)
   int32_t lead;     // 32 + the largest radius of the launch
+  int32_t rmin;     // elements whose radius is below rmin are skipped (the streaming pass takes them)
 };
 
 // gridDim.y passes (rungs) in one launch: rung j reads win + j B, in + j in_stride and writes
@@ -663,6 +664,7 @@ __global__ void __launch_bounds__(512) k_vhgw(VPass a, VPtrs<T> ptr) {
   const int32_t half = (w - 1) / 2;
   const int32_t r = half < n - 1 ? half : n - 1;
   if (kSum && r > a.lead - 32) return;  // a window above the launch's largest (the host prevents it)
+  if (r < a.rmin) return;               // uniform: the streaming pass of this axis serves the element
   // box sum: windows over 32 samples fold chunk-parallel on padded rows, others walk their segments
   // on plain rows; the offset of extended position e in a staged row
   const bool padded = kSum && r >= 16;
@@ -1345,6 +1347,16 @@ Shape pass_shape(const VPass& a, size_t s, bool sum, int32_t max_w, int layout =
   return sh;
 }
 
+// Whether the staged kernels' LDS holds the pass (pass_shape without the exception).
+bool pass_fits(const VPass& a, size_t s, bool sum, int32_t max_w, int layout = 0) {
+  try {
+    (void)pass_shape(a, s, sum, max_w, layout);
+    return true;
+  } catch (const std::invalid_argument&) {
+    return false;
+  }
+}
+
 template <class T, class S>
 void launch_kt(int kt, dim3 grid, dim3 block, size_t lds, hipStream_t s, const VPass& a, const VPtrs<T>& p) {
   const auto go = [&](auto k) {
@@ -1457,6 +1469,272 @@ void launch_rows(hipStream_t s, const Geom& g, int axis, int nf, bool same_input
     case 8: go_rows<T, S, 8>(s, r.items, r, p); break;
     case 16: go_rows<T, S, 16>(s, r.items, r, p); break;
     default: go_rows<T, S, 32>(s, r.items, r, p); break;
+  }
+}
+
+
+// ---- the streaming pass of a strided axis (windows up to 31 samples) ----------------------------------
+// One wave per (rung, element, slab, group of 64 adjacent lines, piece of R outputs): lane l owns one
+// line of the group and walks it sample by sample in the canonical segment order, reading one row
+// of the group (64 adjacent samples, 256 bytes of float32) per step through a buffer resource whose
+// row offsets live in scalar registers, so that a read or a write costs the lane no address
+// arithmetic. Segments of L = 2r + 1 samples start at o + k L (the box sum's canonical offset; any
+// cut gives max / min the same bits). While segment k + 1 is read, its prefix fold G meets the
+// suffix fold H of segment k kept in registers: the output x = s(k+1) - r + j is
+// op(H_k(s(k) + 1 + j), G(j)) for j < L - 1 and op(H_(k+1)(s(k+1)), G(L-1)) (halved for the box: the
+// window is the segment) for j = L - 1; then H_(k+1) replaces H_k. With L at most 31 a segment is
+// one 32-sample chunk of the canonical order, so G and H are plain sequential folds and the sums
+// are bitwise the staged kernels'. A wave reads the segments of its piece in batches of about 32
+// samples per lane (every read of a batch in flight before the first fold). No LDS and no barrier:
+// the waves of a workgroup are independent. The samples the walk reads lie in [-r - 2L, n + r + L),
+// reflected once into the line (the host asks n >= 128); the pass must not run in place (the
+// pieces of one line read each other's rows).
+constexpr int kStreamMaxL = 31;
+
+struct SPass {
+  Geom g;
+  int32_t n;
+  int32_t lines;    // lanes per group that hold a line (64, fewer in a slab's last group)
+  int64_t inner;    // adjacent lines of a slab
+  int64_t outer;    // slabs per element
+  int32_t cgroups;  // 64-line groups per slab
+  int32_t pieces;   // pieces per line
+  int32_t R;        // outputs per piece
+  int32_t mode;     // 0 max, 1 min, 2 max and min of one input (out0, out1), 3 box sum
+  int32_t scale;    // multiply by fl_T(1 / prod of the clamped sides)
+  int64_t items;    // waves per rung
+};
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, uint32_t bytes) {
+  // raw buffer (stride 0) of `bytes` bytes; dword 3 = the gfx9 data-format word of an untyped dword
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, static_cast<int>(bytes), 0x00020000);
+}
+template <class T>
+__device__ __forceinline__ T buf_ld(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(rs, static_cast<int>(voff), static_cast<int>(soff), 0));
+  } else {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(rs, static_cast<int>(voff), static_cast<int>(soff), 0));
+  }
+}
+template <class T>
+__device__ __forceinline__ void buf_st(T v, __amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff) {
+  if constexpr (sizeof(T) == 4) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int32_t, v), rs, static_cast<int>(voff), static_cast<int>(soff), 0);
+  } else {
+    using V2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(rs, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), rs, static_cast<int>(voff), static_cast<int>(soff), 0);
+  }
+}
+
+// The walk of one piece with a window of L samples (MODE as SPass::mode). src, d0, d1: the element's
+// input and outputs; voff: the lane's byte offset in a row (0 for lanes without a line: they read
+// the group's first line and store nothing); slab: the slab's byte offset; rowb: bytes per row
+// (inner * sizeof(T)); outputs x in [x_lo, x_hi).
+template <class T, int MODE, int L>
+__device__ __forceinline__ void stream_walk(__amdgpu_buffer_rsrc_t src, __amdgpu_buffer_rsrc_t d0,
+                                            __amdgpu_buffer_rsrc_t d1, uint32_t voff, bool live, uint32_t slab,
+                                            uint32_t rowb, int32_t n, int32_t o, int32_t x_lo, int32_t x_hi, T fac,
+                                            bool scale) {
+  constexpr int r = (L - 1) / 2;
+  constexpr int U = L >= 32 ? 1 : 32 / L;  // segments per batch of reads
+  using S = std::conditional_t<MODE == 3, T, okey_t<T>>;
+  constexpr bool kTwo = MODE == 2;
+  // segments k1 - 1 .. k2: k1 holds x_lo + r (the first output's G), k2 holds x_hi - 1 + r
+  const int32_t k1 = static_cast<int32_t>(floor_div(static_cast<int64_t>(x_lo) + r - o, L));
+  const int32_t k2 = static_cast<int32_t>(floor_div(static_cast<int64_t>(x_hi) - 1 + r - o, L));
+  const auto row_of = [&](int32_t e) { return e < 0 ? -1 - e : (e >= n ? 2 * n - 1 - e : e); };
+  const auto load = [&](int32_t e) -> S {
+    const T v = buf_ld<T>(src, voff, slab + static_cast<uint32_t>(row_of(e)) * rowb);
+    if constexpr (MODE == 3) return v;
+    else return okey(v);
+  };
+  const auto emit = [&](__amdgpu_buffer_rsrc_t d, int32_t x, S y) {
+    if (x < x_lo || x >= x_hi) return;  // uniform
+    T v;
+    if constexpr (MODE == 3) v = y;
+    else v = okey_inv(y);
+    if (scale) v = v * fac;
+    if (live) buf_st<T>(v, d, voff, slab + static_cast<uint32_t>(x) * rowb);
+  };
+  S h0[L], h1[kTwo ? L : 1];  // the suffix folds of the previous segment (max; min)
+  for (int32_t k = k1 - 1; k <= k2; k += U) {
+    S v[U * L];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (k + u > k2) break;
+      const int32_t s = o + (k + u) * L;
+#pragma unroll
+      for (int j = 0; j < L; ++j) v[u * L + j] = load(s + j);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int32_t kk = k + u;
+      if (kk > k2) break;
+      const int32_t x0 = o + kk * L - r;  // output of step j: x0 + j
+      const bool out = kk >= k1;
+      const S* sv = v + u * L;
+      // G, and the outputs j < L - 1 from the previous segment's H
+      S g0 = sv[0], g1 = sv[0];
+      if (out) {
+#pragma unroll
+        for (int j = 0; j < L - 1; ++j) {
+          if (j > 0) {
+            if constexpr (MODE == 3) g0 = g0 + sv[j];
+            else if constexpr (MODE == 1) g0 = sv[j] < g0 ? sv[j] : g0;
+            else g0 = sv[j] > g0 ? sv[j] : g0;
+            if constexpr (kTwo) g1 = sv[j] < g1 ? sv[j] : g1;
+          }
+          if constexpr (MODE == 3) {
+            emit(d0, x0 + j, h0[j + 1] + g0);
+          } else if constexpr (MODE == 1) {
+            emit(d0, x0 + j, h0[j + 1] < g0 ? h0[j + 1] : g0);
+          } else {
+            emit(d0, x0 + j, h0[j + 1] > g0 ? h0[j + 1] : g0);
+            if constexpr (kTwo) emit(d1, x0 + j, h1[j + 1] < g1 ? h1[j + 1] : g1);
+          }
+        }
+        if constexpr (L > 1) {
+          if constexpr (MODE == 3) g0 = g0 + sv[L - 1];
+          else if constexpr (MODE == 1) g0 = sv[L - 1] < g0 ? sv[L - 1] : g0;
+          else g0 = sv[L - 1] > g0 ? sv[L - 1] : g0;
+          if constexpr (kTwo) g1 = sv[L - 1] < g1 ? sv[L - 1] : g1;
+        }
+      }
+      // this segment's H
+      h0[L - 1] = sv[L - 1];
+      if constexpr (kTwo) h1[L - 1] = sv[L - 1];
+#pragma unroll
+      for (int j = L - 2; j >= 0; --j) {
+        if constexpr (MODE == 3) h0[j] = h0[j + 1] + sv[j];
+        else if constexpr (MODE == 1) h0[j] = sv[j] < h0[j + 1] ? sv[j] : h0[j + 1];
+        else h0[j] = sv[j] > h0[j + 1] ? sv[j] : h0[j + 1];
+        if constexpr (kTwo) h1[j] = sv[j] < h1[j + 1] ? sv[j] : h1[j + 1];
+      }
+      // the output whose window is this segment
+      if (out) {
+        if constexpr (MODE == 3) {
+          emit(d0, x0 + L - 1, (h0[0] + g0) * T(0.5));
+        } else {
+          emit(d0, x0 + L - 1, g0);
+          if constexpr (kTwo) emit(d1, x0 + L - 1, g1);
+        }
+      }
+    }
+  }
+}
+
+template <class T, int MODE>
+__global__ void __launch_bounds__(256) k_vhgw_stream(SPass a, VPtrs<T> ptr) {
+  const int64_t item = static_cast<int64_t>(blockIdx.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (item >= a.items) return;
+  const int64_t rung = blockIdx.y;
+  // item = ((b * outer + slab) * cgroups + group) * pieces + piece
+  const int64_t per_b = a.outer * a.cgroups * a.pieces;
+  const int64_t b = item / per_b;
+  int64_t rest = item - b * per_b;
+  const int64_t slab = rest / (static_cast<int64_t>(a.cgroups) * a.pieces);
+  rest -= slab * a.cgroups * a.pieces;
+  const int32_t grp = static_cast<int32_t>(rest / a.pieces);
+  const int32_t piece = static_cast<int32_t>(rest - static_cast<int64_t>(grp) * a.pieces);
+  const int32_t w = ptr.win[rung * a.g.batch + b];
+  if (w < 1) return;  // uniform: the element does not run
+  const int32_t n = a.n;
+  const int32_t half = (w - 1) / 2;
+  const int32_t r = half < n - 1 ? half : n - 1;
+  const int32_t L = 2 * r + 1;
+  const int32_t o = static_cast<int32_t>((static_cast<int64_t>(n) * ((L + 1) / 2)) % L);
+  const int32_t x_lo = piece * a.R;
+  const int32_t x_hi = min(n, x_lo + a.R);
+  const int lane = threadIdx.x & 63;
+  const int64_t col = static_cast<int64_t>(grp) * 64 + lane;
+  const bool live = col < a.inner;
+  const uint32_t voff = live ? static_cast<uint32_t>(col * sizeof(T)) : 0u;
+  const uint32_t rowb = static_cast<uint32_t>(a.inner * sizeof(T));
+  const uint32_t slabb = static_cast<uint32_t>(slab * a.inner * n * sizeof(T));
+  const uint32_t bytes = static_cast<uint32_t>(a.g.count * sizeof(T));
+  const int64_t off = b * a.g.count;
+  const __amdgpu_buffer_rsrc_t src = buf_rsrc(ptr.in[0] + rung * ptr.in_stride + off, bytes);
+  const __amdgpu_buffer_rsrc_t d0 = buf_rsrc(ptr.out[0] + rung * ptr.out_stride + off, bytes);
+  const __amdgpu_buffer_rsrc_t d1 = buf_rsrc(ptr.out[1] + rung * ptr.out_stride + off, bytes);
+  T fac = T(1);
+  if (a.scale) {
+    double prod = 1.0;
+    for (int ax = 0; ax < a.g.dim; ++ax) prod *= static_cast<double>(2 * min(half, a.g.n[ax] - 1) + 1);
+    fac = static_cast<T>(1.0 / prod);
+  }
+  const bool sc = a.scale != 0;
+#define EDA_STREAM_CASE(LL) \
+  case LL: stream_walk<T, MODE, LL>(src, d0, d1, voff, live, slabb, rowb, n, o, x_lo, x_hi, fac, sc); break;
+  switch (L) {
+    EDA_STREAM_CASE(1)
+    EDA_STREAM_CASE(3)
+    EDA_STREAM_CASE(5)
+    EDA_STREAM_CASE(7)
+    EDA_STREAM_CASE(9)
+    EDA_STREAM_CASE(11)
+    EDA_STREAM_CASE(13)
+    EDA_STREAM_CASE(15)
+    EDA_STREAM_CASE(17)
+    EDA_STREAM_CASE(19)
+    EDA_STREAM_CASE(21)
+    EDA_STREAM_CASE(23)
+    EDA_STREAM_CASE(25)
+    EDA_STREAM_CASE(27)
+    EDA_STREAM_CASE(29)
+    EDA_STREAM_CASE(31)
+    default: break;  // the host sends only windows of at most 31 samples
+  }
+#undef EDA_STREAM_CASE
+}
+
+// The streaming pass takes the elements of a strided axis whose windows are at most 31 samples when
+// the line has at least 128 samples, the pass does not run in place, one element's field is under
+// 4 GiB (32-bit buffer offsets), and its operators are one of max, min, box or max + min of one
+// input.
+template <class T>
+bool stream_qualifies(const Geom& g, int axis, int nf, bool same_input, const int* ops, const VPtrs<T>& p) {
+  if (axis == g.dim - 1 || g.st[axis] == 1) return false;
+  const int32_t n = g.n[axis];
+  if (n < 128) return false;
+  if (static_cast<double>(g.count) * sizeof(T) >= 4294967295.0) return false;
+  if (nf == 2 && !(same_input && ops[0] == kVMax && ops[1] == kVMin)) return false;
+  if (nf == 1 && ops[0] != kVMax && ops[0] != kVMin && ops[0] != kVSum) return false;
+  // in place: an output field is an input field (their rung strides overlap)
+  for (int f = 0; f < nf; ++f)
+    for (int i = 0; i < nf; ++i)
+      if (p.out[f] == p.in[i]) return false;
+  return true;
+}
+
+template <class T>
+void launch_stream(hipStream_t s, const Geom& g, int axis, int nf, const int* ops, bool scale, const VPtrs<T>& p,
+                   int32_t rungs, int32_t max_w) {
+  SPass a{};
+  a.g = g;
+  a.n = g.n[axis];
+  a.inner = g.st[axis];
+  a.outer = g.count / (static_cast<int64_t>(a.n) * a.inner);
+  a.cgroups = static_cast<int32_t>((a.inner + 63) / 64);
+  a.scale = scale ? 1 : 0;
+  a.mode = nf == 2 ? 2 : (ops[0] == kVSum ? 3 : (ops[0] == kVMin ? 1 : 0));
+  // pieces: enough waves to fill the chip (about 8 per SIMD over the launch), each piece at least
+  // 4 windows long so that the walk's lead-in (one segment and r samples) stays a small part of it
+  const int64_t lines = static_cast<int64_t>(rungs) * g.batch * a.outer * a.cgroups;
+  const int64_t want = static_cast<int64_t>(device_cus()) * 4 * 8;
+  const int32_t Lm = 2 * std::min((max_w - 1) / 2, a.n - 1) + 1;
+  int32_t pieces = 1;
+  while (lines * pieces < want && a.n / (2 * pieces) >= std::max(64, 4 * Lm)) pieces *= 2;
+  a.pieces = pieces;
+  a.R = (a.n + pieces - 1) / pieces;
+  a.items = static_cast<int64_t>(g.batch) * a.outer * a.cgroups * pieces;
+  if ((a.items + 3) / 4 > 0x7fffffff) throw std::invalid_argument("eda: too many waves for one pass");
+  const dim3 grid(static_cast<unsigned>((a.items + 3) / 4), static_cast<unsigned>(rungs));
+  switch (a.mode) {
+    case 0: EDA_GO((k_vhgw_stream<T, 0>), grid, dim3(256), 0, s, a, p); break;
+    case 1: EDA_GO((k_vhgw_stream<T, 1>), grid, dim3(256), 0, s, a, p); break;
+    case 2: EDA_GO((k_vhgw_stream<T, 2>), grid, dim3(256), 0, s, a, p); break;
+    default: EDA_GO((k_vhgw_stream<T, 3>), grid, dim3(256), 0, s, a, p); break;
   }
 }
 
@@ -1946,6 +2224,145 @@ void launch_long(hipStream_t s, const Geom& g, int axis, int nf, const int* ops,
   else go(std::integral_constant<int, kLongMaxMin>{});
 }
 
+// The (rung, element) items of a launch whose clamped radius on lines of n samples is at least 16
+// (longer) or below it, from the host mirror of the window array; -1 when it is not mirrored.
+int64_t count_radius(const int32_t* win, int64_t items, int32_t n, bool longer) {
+  const int32_t* h = static_cast<const int32_t*>(mirror_host(win, static_cast<size_t>(items) * sizeof(int32_t)));
+  if (h == nullptr) return -1;
+  int64_t c = 0;
+  for (int64_t k = 0; k < items; ++k) {
+    const int32_t rw = std::min((h[k] - 1) / 2, n - 1);
+    c += h[k] >= 1 && (rw >= 16) == longer ? 1 : 0;
+  }
+  return c;
+}
+
+// One pass on the staged kernels (a strided axis, a long line or a box sum along the contiguous
+// axis); rmin = 16: the elements with shorter windows are served by the streaming pass and skipped.
+template <class T>
+void enqueue_staged(hipStream_t s, const Geom& g, int axis, int nf, bool same_input, const int* ops, bool scale,
+                    const VPtrs<T>& p, int32_t max_w, int32_t rungs, int32_t rmin, bool timed) {
+  const bool sum = ops[0] == kVSum;
+  VPass a{};
+  a.g = g;
+  a.n = g.n[axis];
+  a.step = g.st[axis];
+  a.inner = axis == g.dim - 1 ? 1 : g.st[axis];
+  a.outer = g.count / (static_cast<int64_t>(a.n) * a.inner);
+  a.nf = nf;
+  a.ops[0] = ops[0];
+  a.ops[1] = nf > 1 ? ops[1] : ops[0];
+  a.scale = scale ? 1 : 0;
+  a.rmin = rmin;
+  // the tile of a shape: lines, rows, workgroups, the 16-byte access test
+  const auto shaped = [&](const Shape& sh) {
+    VPass v = a;
+    v.P = sh.P;
+    v.lpad = sh.lpad;
+    v.span = sh.span;
+    v.flat = sh.flat;
+    v.wrows = sh.wrows;
+    v.epad = sh.epad;
+    v.lead = sh.lead;
+    if (v.inner == 1) {
+      v.groups = (v.outer + v.P - 1) / v.P;
+    } else {
+      const int64_t gpo = (v.inner + v.P - 1) / v.P;
+      v.groups = v.outer * gpo;
+    }
+    const int64_t wgs = g.batch * v.groups;
+    if (wgs > 0x7fffffff) throw std::invalid_argument("eda: too many workgroups for one pass");
+    v.xcd = xcd_chunk(wgs);
+    const auto al16 = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; };
+    constexpr int64_t VW = 16 / static_cast<int64_t>(sizeof(T));
+    // strided axis: P and the column count multiples of VW; contiguous axis: n a multiple of VW
+    const bool shape_ok = v.inner > 1 ? v.P % VW == 0 && v.inner % VW == 0 : v.n % VW == 0;
+    v.vec = shape_ok && al16(p.in[0]) && al16(p.in[1]) && al16(p.out[0]) && al16(p.out[1]) &&
+                    (p.in_stride % VW) == 0 && (p.out_stride % VW) == 0
+                ? 1
+                : 0;
+    return v;
+  };
+  hipEvent_t t0 = nullptr;
+  int tag = kTagStrided;
+  if (timed) timer_note(s, &t0);
+  bool done = false;
+  if (!pass_fits(a, sizeof(T), sum, max_w)) {
+    launch_long<T>(s, g, axis, nf, ops, scale, p, rungs, max_w);
+    EDA_LAUNCH("k_long");
+    done = true;
+  }
+  if (!done && sum && EDA_VHGW_PIPE) {
+    // the pipelined kernel, one launch per row layout: windows up to 32 samples on plain rows
+    // (radius <= 15), longer ones on padded rows
+    const bool shorter = rmin < 16;
+    const bool longer = (max_w - 1) / 2 >= 16 && a.n > 16;
+    const bool pipe_fits = (!shorter || pass_fits(a, sizeof(T), true, std::min(max_w, 31), 1)) &&
+                           (!longer || pass_fits(a, sizeof(T), true, max_w, 2));
+    Shape s1{};
+    VPass a1{};
+    int pf1 = 4;
+    if (pipe_fits && shorter) {
+      s1 = pass_shape(a, sizeof(T), true, std::min(max_w, 31), 1);
+      a1 = shaped(s1);
+      pf1 = box_reads<T>(a1, 64 * s1.W);
+    }
+    Shape s2{};
+    VPass a2{};
+    int pf2 = 4;
+    if (pipe_fits && longer) {
+      s2 = pass_shape(a, sizeof(T), true, max_w, 2);
+      a2 = shaped(s2);
+      pf2 = box_reads<T>(a2, 64 * s2.W);
+    }
+    if (pipe_fits && pf1 && pf2 && (shorter || longer)) {
+      const auto items = [&](bool longer_ones) {
+        const int64_t c = count_radius(p.win, g.batch * rungs, a.n, longer_ones);
+        return static_cast<double>(c < 0 ? g.batch * rungs : c);
+      };
+      if (shorter) {
+        launch_box<T, false>(pf1, s, a1, p, rungs, s1.W, s1.lds);
+        if (timed) {
+          timer_done(s, t0, 2.0 * items(false) * g.count * sizeof(T), kTagBoxShort);
+          t0 = nullptr;
+        }
+      }
+      if (longer) {
+        if (timed && t0 == nullptr) timer_note(s, &t0);
+        launch_box<T, true>(pf2, s, a2, p, rungs, s2.W, s2.lds);
+        if (timed) {
+          timer_done(s, t0, 2.0 * items(true) * g.count * sizeof(T), kTagBoxLong);
+          t0 = nullptr;
+        }
+      }
+      EDA_LAUNCH("k_box");
+      done = true;
+    } else if (!shorter && !longer) {
+      done = true;  // every element streamed
+    }
+  }
+  if (!done) {
+    const Shape sh = pass_shape(a, sizeof(T), sum, max_w);
+    const VPass v = shaped(sh);
+    const int64_t wgs = g.batch * v.groups;
+    const dim3 grid(static_cast<unsigned>(wgs), static_cast<unsigned>(rungs));
+    const dim3 block(static_cast<unsigned>(64 * sh.W));
+    if (sum) launch_kt<T, T>(1, grid, block, sh.lds, s, v, p);
+    else launch_kt<T, okey_t<T>>(kt_for(v.n), grid, block, sh.lds, s, v, p);
+    EDA_LAUNCH("k_vhgw");
+  }
+  if (timed && t0 != nullptr) {
+    int64_t active = mirror_active(p.win, g.batch * rungs);
+    if (active < 0) active = g.batch * rungs;
+    if (rmin > 0) {
+      const int64_t c = count_radius(p.win, g.batch * rungs, a.n, true);
+      if (c >= 0) active = c;
+    }
+    const int fields = (nf == 2 && !same_input ? 2 : 1) + nf;
+    timer_done(s, t0, static_cast<double>(fields) * active * g.count * sizeof(T), tag);
+  }
+}
+
 template <class T>
 void enqueue_vhgw(hipStream_t s, const Geom& g, int axis, int nf, bool same_input, const int* ops, bool scale,
                   const T* const* in, T* const* out, const int32_t* win, int32_t max_w, int32_t rungs,
@@ -1967,158 +2384,101 @@ void enqueue_vhgw(hipStream_t s, const Geom& g, int axis, int nf, bool same_inpu
     p.out[1] = pairs;
     p.out_stride = 2 * out_stride;
   }
-  hipEvent_t t0 = nullptr;
   const bool timed = timer_on();
-  int tag = kTagRows;
   if (rows_qualify(g, axis, sum)) {
+    hipEvent_t t0 = nullptr;
     if (timed) timer_note(s, &t0);
     launch_rows<T>(s, g, axis, nf, same_input, ops, p, rungs, il);
     EDA_LAUNCH("k_vhgw_rows");
-  } else {
+    if (timed) {
+      int64_t active = mirror_active(win, g.batch * rungs);
+      if (active < 0) active = g.batch * rungs;
+      const int fields = (nf == 2 && !same_input ? 2 : 1) + nf;
+      timer_done(s, t0, static_cast<double>(fields) * active * g.count * sizeof(T), kTagRows);
+    }
+    return;
+  }
+  if (stream_qualifies<T>(g, axis, nf, same_input, ops, p)) {
     VPass a{};
     a.g = g;
     a.n = g.n[axis];
     a.step = g.st[axis];
-    a.inner = axis == g.dim - 1 ? 1 : g.st[axis];
+    a.inner = g.st[axis];
     a.outer = g.count / (static_cast<int64_t>(a.n) * a.inner);
     a.nf = nf;
-    a.ops[0] = ops[0];
-    a.ops[1] = nf > 1 ? ops[1] : ops[0];
-    a.scale = scale ? 1 : 0;
-    // the tile of a shape: lines, rows, workgroups, the 16-byte access test
-    const auto shaped = [&](const Shape& sh) {
-      VPass v = a;
-      v.P = sh.P;
-      v.lpad = sh.lpad;
-      v.span = sh.span;
-      v.flat = sh.flat;
-      v.wrows = sh.wrows;
-      v.epad = sh.epad;
-      v.lead = sh.lead;
-      if (v.inner == 1) {
-        v.groups = (v.outer + v.P - 1) / v.P;
-      } else {
-        const int64_t gpo = (v.inner + v.P - 1) / v.P;
-        v.groups = v.outer * gpo;
+    // the staged kernels must hold the launch's longer windows (else the whole pass stays with them)
+    if (max_w <= kStreamMaxL || pass_fits(a, sizeof(T), sum, max_w)) {
+      // elements with windows up to 31 samples stream; the launch's longer windows (if any) run the
+      // staged kernels, which skip the streamed elements
+      hipEvent_t t0 = nullptr;
+      if (timed) timer_note(s, &t0);
+      launch_stream<T>(s, g, axis, nf, ops, scale, p, rungs, std::min(max_w, kStreamMaxL));
+      EDA_LAUNCH("k_vhgw_stream");
+      if (timed) {
+        const int64_t c = count_radius(win, g.batch * rungs, g.n[axis], false);
+        const int fields = (nf == 2 && !same_input ? 2 : 1) + nf;
+        timer_done(s, t0, static_cast<double>(fields) * (c < 0 ? g.batch * rungs : c) * g.count * sizeof(T),
+                   kTagStream);
       }
-      const int64_t wgs = g.batch * v.groups;
-      if (wgs > 0x7fffffff) throw std::invalid_argument("eda: too many workgroups for one pass");
-      v.xcd = xcd_chunk(wgs);
-      const auto al16 = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; };
-      constexpr int64_t VW = 16 / static_cast<int64_t>(sizeof(T));
-      // strided axis: P and the column count multiples of VW; contiguous axis: n a multiple of VW
-      const bool shape_ok = v.inner > 1 ? v.P % VW == 0 && v.inner % VW == 0 : v.n % VW == 0;
-      v.vec = shape_ok && al16(p.in[0]) && al16(p.in[1]) && al16(p.out[0]) && al16(p.out[1]) &&
-                      (p.in_stride % VW) == 0 && (p.out_stride % VW) == 0
-                  ? 1
-                  : 0;
-      return v;
-    };
-    if (timed) timer_note(s, &t0);
-    bool done = false;
-    bool fits = true;
-    try {
-      (void)pass_shape(a, sizeof(T), sum, max_w);
-    } catch (const std::invalid_argument&) {
-      fits = false;
-    }
-    if (!fits) {
-      tag = kTagStrided;
-      launch_long<T>(s, g, axis, nf, ops, scale, p, rungs, max_w);
-      EDA_LAUNCH("k_long");
-      done = true;
-    }
-    if (!done && sum && EDA_VHGW_PIPE) {
-      // the pipelined kernel, one launch per row layout: windows up to 32 samples on plain rows
-      // (radius <= 15), longer ones on padded rows
-      const bool longer = (max_w - 1) / 2 >= 16 && a.n > 16;
-      bool pipe_fits = true;
-      try {
-        (void)pass_shape(a, sizeof(T), true, std::min(max_w, 31), 1);
-        if (longer) (void)pass_shape(a, sizeof(T), true, max_w, 2);
-      } catch (const std::invalid_argument&) {
-        pipe_fits = false;
-      }
-      Shape s1{};
-      VPass a1{};
-      int pf1 = 0;
-      if (pipe_fits) {
-        s1 = pass_shape(a, sizeof(T), true, std::min(max_w, 31), 1);
-        a1 = shaped(s1);
-        pf1 = box_reads<T>(a1, 64 * s1.W);
-      }
-      Shape s2{};
-      VPass a2{};
-      int pf2 = 4;
-      if (pipe_fits && longer) {
-        s2 = pass_shape(a, sizeof(T), true, max_w, 2);
-        a2 = shaped(s2);
-        pf2 = box_reads<T>(a2, 64 * s2.W);
-      }
-      if (pf1 && pf2) {
-        // the timer counts each launch's own items: windows of radius < 16 run in the first
-        const auto items = [&](bool longer_ones) {
-          const int32_t* h = static_cast<const int32_t*>(
-              mirror_host(win, static_cast<size_t>(g.batch * rungs) * sizeof(int32_t)));
-          if (h == nullptr) return static_cast<double>(g.batch * rungs);
-          int64_t c = 0;
-          for (int64_t k = 0; k < g.batch * rungs; ++k) {
-            const int32_t rw = std::min((h[k] - 1) / 2, a.n - 1);
-            c += h[k] >= 1 && (rw >= 16) == longer_ones ? 1 : 0;
-          }
-          return static_cast<double>(c);
-        };
-        launch_box<T, false>(pf1, s, a1, p, rungs, s1.W, s1.lds);
-        if (timed) {
-          timer_done(s, t0, 2.0 * items(false) * g.count * sizeof(T), kTagBoxShort);
-          t0 = nullptr;
-        }
-        if (longer) {
-          if (timed) timer_note(s, &t0);
-          launch_box<T, true>(pf2, s, a2, p, rungs, s2.W, s2.lds);
-          if (timed) {
-            timer_done(s, t0, 2.0 * items(true) * g.count * sizeof(T), kTagBoxLong);
-            t0 = nullptr;
-          }
-        }
-        EDA_LAUNCH("k_box");
-        done = true;
-      }
-    }
-    if (!done) {
-      const Shape sh = pass_shape(a, sizeof(T), sum, max_w);
-      const VPass v = shaped(sh);
-      const int64_t wgs = g.batch * v.groups;
-      const dim3 grid(static_cast<unsigned>(wgs), static_cast<unsigned>(rungs));
-      const dim3 block(static_cast<unsigned>(64 * sh.W));
-      tag = kTagStrided;
-      if (sum) launch_kt<T, T>(1, grid, block, sh.lds, s, v, p);
-      else launch_kt<T, okey_t<T>>(kt_for(v.n), grid, block, sh.lds, s, v, p);
-      EDA_LAUNCH("k_vhgw");
+      if (max_w > kStreamMaxL) enqueue_staged<T>(s, g, axis, nf, same_input, ops, scale, p, max_w, rungs, 16, timed);
+      return;
     }
   }
-  if (timed && t0 != nullptr) {
-    int64_t active = mirror_active(win, g.batch * rungs);
-    if (active < 0) active = g.batch * rungs;
-    const int fields = (nf == 2 && !same_input ? 2 : 1) + nf;
-    timer_done(s, t0, static_cast<double>(fields) * active * g.count * sizeof(T), tag);
-  }
+  enqueue_staged<T>(s, g, axis, nf, same_input, ops, scale, p, max_w, rungs, 0, timed);
 }
 
 // The cube (reference cpp/src/method/common.cpp:65-136): box runs axis 0 first, the last pass
 // scaled; max / min run the last axis first; maxmin reads the input for both outputs. With
 // rungs > 1 every pass runs all rungs in one launch: rung j filters in with the windows win + j B
 // into out0 + j stride (out1 + j stride).
+// A field per (device, stream) that only grows (launches on one stream are ordered): the first pass
+// of an in-place cube writes it, so that the streaming pass of axis 0 does not run in place.
+template <class T>
+T* cube_scratch(hipStream_t s, int64_t elems) {
+  static std::mutex mu;
+  static std::map<std::pair<int, hipStream_t>, std::pair<void*, size_t>> cache;
+  const size_t need = static_cast<size_t>(elems) * sizeof(T);
+  int dev = 0;
+  EDA_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  auto& slot = cache[std::make_pair(dev, s)];
+  if (slot.second < need) {
+    EDA_HIP(hipStreamSynchronize(s));
+    if (slot.first) EDA_HIP(hipFree(slot.first));
+    slot.first = nullptr;
+    slot.second = 0;
+    const hipError_t e = hipMalloc(&slot.first, need);
+    if (e == hipErrorOutOfMemory) {
+      (void)hipGetLastError();
+      slot.first = nullptr;
+      return nullptr;  // the cube then runs in place on the staged kernels
+    }
+    EDA_HIP(e);
+    slot.second = need;
+  }
+  return static_cast<T*>(slot.first);
+}
+
+// The scratch field for the first pass of a one-rung in-place cube whose axis 0 streams; nullptr
+// when the cube keeps its passes in place.
+template <class T>
+T* first_pass_scratch(hipStream_t s, const Geom& g, const T* in, const T* out0, int32_t rungs) {
+  if (g.dim < 2 || rungs != 1 || in != out0 || g.st[0] == 1 || g.n[0] < 128) return nullptr;
+  if (static_cast<double>(g.count) * sizeof(T) >= 4294967295.0) return nullptr;
+  return cube_scratch<T>(s, g.batch * g.count);
+}
+
 template <class T>
 bool enqueue_cube_rungs(hipStream_t s, const Geom& g, CubeOp op, const T* in, T* out0, T* out1, const int32_t* win,
                         int32_t rungs, int64_t stride, int32_t max_w, T* pairs) {
   if (max_w < 1) return false;
   const int D = g.dim;
   if (op == CubeOp::kBox) {
+    T* scr = first_pass_scratch<T>(s, g, in, out0, rungs);
     for (int a = 0; a < D; ++a) {
       const int ops[1] = {kVSum};
-      const T* src[1] = {a == 0 ? in : out0};
-      T* dst[1] = {out0};
+      const T* src[1] = {a == 0 ? in : (a == 1 && scr ? scr : out0)};
+      T* dst[1] = {a == 0 && scr ? scr : out0};
       enqueue_vhgw<T>(s, g, a, 1, true, ops, a == D - 1, src, dst, win, max_w, rungs, a == 0 ? 0 : stride, stride);
     }
     return false;
@@ -2126,8 +2486,11 @@ bool enqueue_cube_rungs(hipStream_t s, const Geom& g, CubeOp op, const T* in, T*
   // The max and min side by side: the streaming row pass writes them, so it runs last (axes 0 ..
   // D - 1; max and min give the same bits in any order of the axes).
   const bool il = pairs != nullptr && op == CubeOp::kMaxMin && D >= 2 && rows_qualify(g, D - 1, false);
+  T* scr = op == CubeOp::kMaxMin ? nullptr : first_pass_scratch<T>(s, g, in, out0, rungs);
   for (int k = 0; k < D; ++k) {
-    const int a = il ? k : D - 1 - k;
+    // axis 0 first: its pass reads the input and writes another field, which the streaming pass
+    // of a strided axis needs (it does not run in place)
+    const int a = k;
     const bool first = k == 0;
     const int64_t ins = first ? 0 : stride;
     if (op == CubeOp::kMaxMin) {
@@ -2138,8 +2501,8 @@ bool enqueue_cube_rungs(hipStream_t s, const Geom& g, CubeOp op, const T* in, T*
                       il && k == D - 1 ? pairs : nullptr);
     } else {
       const int ops[1] = {op == CubeOp::kMax ? kVMax : kVMin};
-      const T* src[1] = {first ? in : out0};
-      T* dst[1] = {out0};
+      const T* src[1] = {first ? in : (k == 1 && scr ? scr : out0)};
+      T* dst[1] = {first && scr ? scr : out0};
       enqueue_vhgw<T>(s, g, a, 1, true, ops, false, src, dst, win, max_w, rungs, ins, stride);
     }
   }
