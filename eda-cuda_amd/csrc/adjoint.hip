@@ -1178,7 +1178,7 @@ class Adjoint {
     int32_t mw = 0;
     for (int j = 0; j < Km; ++j) mw = std::max(mw, tm.rung_max[j]);
     const CubeOp cop = op == 3 ? CubeOp::kBox : (op == 2 ? CubeOp::kMaxMin : CubeOp::kMax);
-    enqueue_cube_rungs<T>(s_, g_, cop, in, r0_, r1_, rw, Km, F_, mw, nullptr);
+    enqueue_cube_rungs<T>(s_, g_, cop, in, r0_, r1_, rw, Km, F_, mw, nullptr, op == 2 ? nullptr : r1_);
     EDA_HIP(hipMemsetAsync(gC0_, 0, sizeof(T) * static_cast<size_t>(Km) * F_, s_));
     if (op == 2) EDA_HIP(hipMemsetAsync(gC1_, 0, sizeof(T) * static_cast<size_t>(Km) * F_, s_));
     T* go = orient_grad_ ? go_ : nullptr;

@@ -402,7 +402,7 @@ struct Uploader {
     pend = a + bytes;
     t_staged = this;
     off = a + bytes;
-    if (timer_state().on.load(std::memory_order_relaxed) != 0) mirror_note(d, host + a, bytes);
+    mirror_note(d, host + a, bytes);  // the launches read the host copy (timer bytes, window classes)
     return reinterpret_cast<V*>(d);
   }
   template <class V>
@@ -1651,7 +1651,7 @@ bool Chunk<T>::mode() {
   }
   rw_ = rw;
   rw_cur_ = rw;
-  if (timer_family(~0) && !rw_cur_.empty()) mirror_note(d_rung_win_, rw_cur_.data(), rw_cur_.size() * sizeof(int32_t));
+  if (!rw_cur_.empty()) mirror_note(d_rung_win_, rw_cur_.data(), rw_cur_.size() * sizeof(int32_t));
   on = upload_on(active);
   on_op_ = on;
   TapeMode* tm = nullptr;
@@ -1745,7 +1745,7 @@ bool Chunk<T>::mode() {
       flush_uploads();
       EDA_HIP(hipMemcpyAsync(d_rung_win_, drw, sizeof(int32_t) * rw.size(), hipMemcpyDeviceToDevice, s_));
       rw_cur_ = rw;
-      if (timer_family(~0) && !rw_cur_.empty()) mirror_note(d_rung_win_, rw_cur_.data(), rw_cur_.size() * sizeof(int32_t));
+      if (!rw_cur_.empty()) mirror_note(d_rung_win_, rw_cur_.data(), rw_cur_.size() * sizeof(int32_t));
     }
     T* plain = X[0];
     T* up = X[1];

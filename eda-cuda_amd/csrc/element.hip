@@ -487,7 +487,7 @@ void enqueue_element_op(hipStream_t s, const Geom& g, int op, const ElemDev* el,
     if (timed) timer_done(s, t0, gather_bytes<T>(g, op, el, on, extra), kTagGather);
   };
   if (op == 3) {
-    enqueue_cube_rungs<T>(s, g, CubeOp::kBox, in, rungs0, nullptr, rung_win, K_max, F, mw);
+    enqueue_cube_rungs<T>(s, g, CubeOp::kBox, in, rungs0, nullptr, rung_win, K_max, F, mw, nullptr, rungs1);
     timed_gather([&] { launch_gather_all<T, 3>(s, g, el, lo, t, o, rungs0, nullptr, out0, nullptr, on, extra); });
   } else if (op == 2) {
     if (enqueue_cube_rungs<T>(s, g, CubeOp::kMaxMin, in, rungs0, rungs1, rung_win, K_max, F, mw, pairs))
@@ -495,7 +495,7 @@ void enqueue_element_op(hipStream_t s, const Geom& g, int op, const ElemDev* el,
     else
       timed_gather([&] { launch_gather_all<T, 2>(s, g, el, lo, t, o, rungs0, rungs1, out0, out1, on, extra); });
   } else {
-    enqueue_cube_rungs<T>(s, g, CubeOp::kMax, in, rungs0, nullptr, rung_win, K_max, F, mw);
+    enqueue_cube_rungs<T>(s, g, CubeOp::kMax, in, rungs0, nullptr, rung_win, K_max, F, mw, nullptr, rungs1);
     timed_gather([&] { launch_gather_all<T, 0>(s, g, el, lo, t, o, rungs0, nullptr, out0, nullptr, on, extra); });
   }
 }

@@ -45,7 +45,7 @@ enum TimerTag : int {
   kTagDct,          // k_dct_fft and the dense contraction
   kTagGather,       // k_gather_all
   kTagFlags,        // k_flags_tile
-  kTagStream,       // k_vhgw_stream (strided axes, windows up to 31 samples)
+  kTagStream,       // k_vhgw_stream (strided axes, windows up to 63 samples)
   kTagCount
 };
 void timer_set(int mask);
@@ -55,8 +55,10 @@ bool timer_family(int family);
 void timer_note(hipStream_t s, hipEvent_t* start);                          // before a launch
 void timer_done(hipStream_t s, hipEvent_t start, double bytes, int tag);    // after it
 void timer_read(double* ms, int64_t* launches, double* bytes);              // synchronizes, sums, clears
-// Host mirrors of device parameter arrays (registered by the driver's uploads while a timer runs):
-// the host copy of [dev, dev + bytes), or nullptr when the range is not mirrored.
+// Host mirrors of device parameter arrays (registered by the driver's uploads, cleared with the
+// upload arena): the host copy of [dev, dev + bytes), or nullptr when the range is not mirrored.
+// The timer counts bytes from them, and the cube passes skip launches of window classes that no
+// element of the launch falls in.
 void mirror_note(const void* dev, const void* host, size_t bytes);
 void mirror_clear();
 const void* mirror_host(const void* dev, size_t bytes);
@@ -78,9 +80,11 @@ void enqueue_axis_pass(hipStream_t s, const Geom& g, int axis, bool min, const T
 // out0 + j stride (out1 + j stride); each rung's outputs are bitwise those of enqueue_cube. With
 // pairs (kMaxMin), the last pass may write the max and min of a voxel side by side instead:
 // pairs + j 2 stride + 2 v (+ 1); returns whether it did (out0 / out1 then hold intermediates).
+// scratch (max, min, box; rungs fields of `stride`): the first pass's output, so that no pass of a
+// 2-D cube runs in place (the streaming passes need that); its contents are then undefined.
 template <class T>
 bool enqueue_cube_rungs(hipStream_t s, const Geom& g, CubeOp op, const T* in, T* out0, T* out1, const int32_t* win,
-                        int32_t rungs, int64_t stride, int32_t max_w, T* pairs = nullptr);
+                        int32_t rungs, int64_t stride, int32_t max_w, T* pairs = nullptr, T* scratch = nullptr);
 
 // ---- reductions (reduce.hip) ---------------------------------------------------------------------
 // Per element: energy[b] = the canonical pairwise sum of squares (double; nullptr: none),

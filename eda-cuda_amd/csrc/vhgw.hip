@@ -950,7 +950,7 @@ __global__ void __launch_bounds__(256) EDA_BOX_ATTR k_box(VPass a, VPtrs<T> ptr,
         const int64_t b = (c - rung * per_rung) / a.groups;
         const int32_t w = ptr.win[rung * a.g.batch + b];
         const int32_t rw = min((w - 1) / 2, n - 1);
-        run = w >= 1 && rw <= a.lead - 32 && (rw >= 16) == PAD;
+        run = w >= 1 && rw <= a.lead - 32 && (rw >= 16) == PAD && rw >= a.rmin;
       }
       const uint64_t m = __ballot(run);
       if (m != 0) return it + static_cast<int64_t>(__ffsll(static_cast<unsigned long long>(m)) - 1) * nloc;
@@ -1473,7 +1473,7 @@ void launch_rows(hipStream_t s, const Geom& g, int axis, int nf, bool same_input
 }
 
 
-// ---- the streaming pass of a strided axis (windows up to 31 samples) ----------------------------------
+// ---- the streaming pass of a strided axis (windows up to 63 samples) ----------------------------------
 // One wave per (rung, element, slab, group of 64 adjacent lines, piece of R outputs): lane l owns one
 // line of the group and walks it sample by sample in the canonical segment order, reading one row
 // of the group (64 adjacent samples, 256 bytes of float32) per step through a buffer resource whose
@@ -1482,14 +1482,17 @@ void launch_rows(hipStream_t s, const Geom& g, int axis, int nf, bool same_input
 // cut gives max / min the same bits). While segment k + 1 is read, its prefix fold G meets the
 // suffix fold H of segment k kept in registers: the output x = s(k+1) - r + j is
 // op(H_k(s(k) + 1 + j), G(j)) for j < L - 1 and op(H_(k+1)(s(k+1)), G(L-1)) (halved for the box: the
-// window is the segment) for j = L - 1; then H_(k+1) replaces H_k. With L at most 31 a segment is
-// one 32-sample chunk of the canonical order, so G and H are plain sequential folds and the sums
-// are bitwise the staged kernels'. A wave reads the segments of its piece in batches of about 32
-// samples per lane (every read of a batch in flight before the first fold). No LDS and no barrier:
-// the waves of a workgroup are independent. The samples the walk reads lie in [-r - 2L, n + r + L),
-// reflected once into the line (the host asks n >= 128); the pass must not run in place (the
-// pieces of one line read each other's rows).
-constexpr int kStreamMaxL = 31;
+// window is the segment) for j = L - 1; then H_(k+1) replaces H_k. G and H keep the canonical
+// 32-sample chunks (L <= 32: one chunk, a plain sequential fold; L <= 63: two), so the sums are
+// bitwise the staged kernels'. L is a template constant (the folds live in registers): one kernel
+// takes L <= 31, another 33 <= L <= 63 (more registers), each skipping the other's elements. A wave
+// reads the segments of its piece in batches of about 32 samples per lane (every read of a batch in
+// flight before the first fold). No LDS and no barrier: the waves of a workgroup are independent.
+// The samples the walk reads lie in [(1 - 3L) / 2, n + r + L), reflected once into the line (the
+// host asks n >= 128); the pass must not run in place (the pieces of one line read each other's
+// rows).
+constexpr int kStreamMaxL = 31;     // the short walk
+constexpr int kStreamBigL = 63;     // the long walk (two canonical chunks per box segment)
 
 struct SPass {
   Geom g;
@@ -1527,6 +1530,81 @@ __device__ __forceinline__ void buf_st(T v, __amdgpu_buffer_rsrc_t rs, uint32_t 
   }
 }
 
+// One segment of a walk (MODE as SPass::mode): sv the segment's L samples (order keys for max /
+// min); h0 (h1) the previous segment's suffix folds of the max or the only op (of the min), replaced
+// by this segment's; out: the segment has outputs, emit(j, y0, y1) for the output x0 + j, j < L
+// (y1: the min of MODE 2). The box sum's folds keep the canonical chunks: a local fold restarted
+// every 32 samples from the segment's start (G) or end (H), plus the carry of the chunks before
+// (for L <= 32 one chunk, a plain sequential fold).
+template <int MODE, int L, class S, class Emit>
+__device__ __forceinline__ void seg_fold(const S* sv, S* h0, S* h1, bool out, Emit emit) {
+  constexpr bool kTwo = MODE == 2;
+  S g0 = sv[0], g1 = sv[0];
+  S loc = sv[0], car = sv[0];  // box: the chunk's local fold and the carry of the chunks before
+  const auto step_g = [&](int j) {
+    if constexpr (MODE == 3 && L <= 32) {
+      g0 = g0 + sv[j];
+    } else if constexpr (MODE == 3) {
+      if (j % 32 == 0) {
+        car = j == 32 ? loc : car + loc;
+        loc = sv[j];
+      } else {
+        loc = loc + sv[j];
+      }
+      g0 = j < 32 ? loc : car + loc;
+    } else {
+      if constexpr (MODE == 1) g0 = sv[j] < g0 ? sv[j] : g0;
+      else g0 = sv[j] > g0 ? sv[j] : g0;
+      if constexpr (kTwo) g1 = sv[j] < g1 ? sv[j] : g1;
+    }
+  };
+  if (out) {
+#pragma unroll
+    for (int j = 0; j < L - 1; ++j) {
+      if (j > 0) step_g(j);
+      if constexpr (MODE == 3) {
+        emit(j, h0[j + 1] + g0, g0);
+      } else if constexpr (MODE == 1) {
+        emit(j, h0[j + 1] < g0 ? h0[j + 1] : g0, g0);
+      } else if constexpr (kTwo) {
+        emit(j, h0[j + 1] > g0 ? h0[j + 1] : g0, h1[j + 1] < g1 ? h1[j + 1] : g1);
+      } else {
+        emit(j, h0[j + 1] > g0 ? h0[j + 1] : g0, g0);
+      }
+    }
+    if constexpr (L > 1) step_g(L - 1);
+  }
+  // this segment's H (d: the distance from the segment's end)
+  h0[L - 1] = sv[L - 1];
+  if constexpr (kTwo) h1[L - 1] = sv[L - 1];
+  S hl = sv[L - 1], hc = sv[L - 1];
+#pragma unroll
+  for (int j = L - 2; j >= 0; --j) {
+    if constexpr (MODE == 3 && L <= 32) {
+      h0[j] = h0[j + 1] + sv[j];
+    } else if constexpr (MODE == 3) {
+      const int d = L - 1 - j;
+      if (d % 32 == 0) {
+        hc = d == 32 ? hl : hc + hl;
+        hl = sv[j];
+      } else {
+        hl = hl + sv[j];
+      }
+      h0[j] = d < 32 ? hl : hc + hl;
+    } else if constexpr (MODE == 1) {
+      h0[j] = sv[j] < h0[j + 1] ? sv[j] : h0[j + 1];
+    } else {
+      h0[j] = sv[j] > h0[j + 1] ? sv[j] : h0[j + 1];
+    }
+    if constexpr (kTwo) h1[j] = sv[j] < h1[j + 1] ? sv[j] : h1[j + 1];
+  }
+  // the output whose window is this segment
+  if (out) {
+    if constexpr (MODE == 3) emit(L - 1, (h0[0] + g0) * decltype(g0)(0.5), g0);
+    else emit(L - 1, g0, g1);
+  }
+}
+
 // The walk of one piece with a window of L samples (MODE as SPass::mode). src, d0, d1: the element's
 // input and outputs; voff: the lane's byte offset in a row (0 for lanes without a line: they read
 // the group's first line and store nothing); slab: the slab's byte offset; rowb: bytes per row
@@ -1539,7 +1617,6 @@ __device__ __forceinline__ void stream_walk(__amdgpu_buffer_rsrc_t src, __amdgpu
   constexpr int r = (L - 1) / 2;
   constexpr int U = L >= 32 ? 1 : 32 / L;  // segments per batch of reads
   using S = std::conditional_t<MODE == 3, T, okey_t<T>>;
-  constexpr bool kTwo = MODE == 2;
   // segments k1 - 1 .. k2: k1 holds x_lo + r (the first output's G), k2 holds x_hi - 1 + r
   const int32_t k1 = static_cast<int32_t>(floor_div(static_cast<int64_t>(x_lo) + r - o, L));
   const int32_t k2 = static_cast<int32_t>(floor_div(static_cast<int64_t>(x_hi) - 1 + r - o, L));
@@ -1549,15 +1626,14 @@ __device__ __forceinline__ void stream_walk(__amdgpu_buffer_rsrc_t src, __amdgpu
     if constexpr (MODE == 3) return v;
     else return okey(v);
   };
-  const auto emit = [&](__amdgpu_buffer_rsrc_t d, int32_t x, S y) {
-    if (x < x_lo || x >= x_hi) return;  // uniform
+  const auto put = [&](__amdgpu_buffer_rsrc_t d, int32_t x, S y) {
     T v;
     if constexpr (MODE == 3) v = y;
     else v = okey_inv(y);
     if (scale) v = v * fac;
     if (live) buf_st<T>(v, d, voff, slab + static_cast<uint32_t>(x) * rowb);
   };
-  S h0[L], h1[kTwo ? L : 1];  // the suffix folds of the previous segment (max; min)
+  S h0[L], h1[MODE == 2 ? L : 1];  // the suffix folds of the previous segment (max; min)
   for (int32_t k = k1 - 1; k <= k2; k += U) {
     S v[U * L];
 #pragma unroll
@@ -1572,59 +1648,17 @@ __device__ __forceinline__ void stream_walk(__amdgpu_buffer_rsrc_t src, __amdgpu
       const int32_t kk = k + u;
       if (kk > k2) break;
       const int32_t x0 = o + kk * L - r;  // output of step j: x0 + j
-      const bool out = kk >= k1;
-      const S* sv = v + u * L;
-      // G, and the outputs j < L - 1 from the previous segment's H
-      S g0 = sv[0], g1 = sv[0];
-      if (out) {
-#pragma unroll
-        for (int j = 0; j < L - 1; ++j) {
-          if (j > 0) {
-            if constexpr (MODE == 3) g0 = g0 + sv[j];
-            else if constexpr (MODE == 1) g0 = sv[j] < g0 ? sv[j] : g0;
-            else g0 = sv[j] > g0 ? sv[j] : g0;
-            if constexpr (kTwo) g1 = sv[j] < g1 ? sv[j] : g1;
-          }
-          if constexpr (MODE == 3) {
-            emit(d0, x0 + j, h0[j + 1] + g0);
-          } else if constexpr (MODE == 1) {
-            emit(d0, x0 + j, h0[j + 1] < g0 ? h0[j + 1] : g0);
-          } else {
-            emit(d0, x0 + j, h0[j + 1] > g0 ? h0[j + 1] : g0);
-            if constexpr (kTwo) emit(d1, x0 + j, h1[j + 1] < g1 ? h1[j + 1] : g1);
-          }
-        }
-        if constexpr (L > 1) {
-          if constexpr (MODE == 3) g0 = g0 + sv[L - 1];
-          else if constexpr (MODE == 1) g0 = sv[L - 1] < g0 ? sv[L - 1] : g0;
-          else g0 = sv[L - 1] > g0 ? sv[L - 1] : g0;
-          if constexpr (kTwo) g1 = sv[L - 1] < g1 ? sv[L - 1] : g1;
-        }
-      }
-      // this segment's H
-      h0[L - 1] = sv[L - 1];
-      if constexpr (kTwo) h1[L - 1] = sv[L - 1];
-#pragma unroll
-      for (int j = L - 2; j >= 0; --j) {
-        if constexpr (MODE == 3) h0[j] = h0[j + 1] + sv[j];
-        else if constexpr (MODE == 1) h0[j] = sv[j] < h0[j + 1] ? sv[j] : h0[j + 1];
-        else h0[j] = sv[j] > h0[j + 1] ? sv[j] : h0[j + 1];
-        if constexpr (kTwo) h1[j] = sv[j] < h1[j + 1] ? sv[j] : h1[j + 1];
-      }
-      // the output whose window is this segment
-      if (out) {
-        if constexpr (MODE == 3) {
-          emit(d0, x0 + L - 1, (h0[0] + g0) * T(0.5));
-        } else {
-          emit(d0, x0 + L - 1, g0);
-          if constexpr (kTwo) emit(d1, x0 + L - 1, g1);
-        }
-      }
+      seg_fold<MODE, L>(v + u * L, h0, h1, kk >= k1, [&](int j, S y0, S y1) {
+        const int32_t x = x0 + j;
+        if (x < x_lo || x >= x_hi) return;  // uniform
+        put(d0, x, y0);
+        if constexpr (MODE == 2) put(d1, x, y1);
+      });
     }
   }
 }
 
-template <class T, int MODE>
+template <class T, int MODE, bool BIG>
 __global__ void __launch_bounds__(256) k_vhgw_stream(SPass a, VPtrs<T> ptr) {
   const int64_t item = static_cast<int64_t>(blockIdx.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (item >= a.items) return;
@@ -1666,29 +1700,52 @@ __global__ void __launch_bounds__(256) k_vhgw_stream(SPass a, VPtrs<T> ptr) {
   const bool sc = a.scale != 0;
 #define EDA_STREAM_CASE(LL) \
   case LL: stream_walk<T, MODE, LL>(src, d0, d1, voff, live, slabb, rowb, n, o, x_lo, x_hi, fac, sc); break;
-  switch (L) {
-    EDA_STREAM_CASE(1)
-    EDA_STREAM_CASE(3)
-    EDA_STREAM_CASE(5)
-    EDA_STREAM_CASE(7)
-    EDA_STREAM_CASE(9)
-    EDA_STREAM_CASE(11)
-    EDA_STREAM_CASE(13)
-    EDA_STREAM_CASE(15)
-    EDA_STREAM_CASE(17)
-    EDA_STREAM_CASE(19)
-    EDA_STREAM_CASE(21)
-    EDA_STREAM_CASE(23)
-    EDA_STREAM_CASE(25)
-    EDA_STREAM_CASE(27)
-    EDA_STREAM_CASE(29)
-    EDA_STREAM_CASE(31)
-    default: break;  // the host sends only windows of at most 31 samples
+  // the short kernel takes windows up to 31 samples, the long one 33 to 63 (each skips the other's)
+  if constexpr (!BIG) {
+    switch (L) {
+      EDA_STREAM_CASE(1)
+      EDA_STREAM_CASE(3)
+      EDA_STREAM_CASE(5)
+      EDA_STREAM_CASE(7)
+      EDA_STREAM_CASE(9)
+      EDA_STREAM_CASE(11)
+      EDA_STREAM_CASE(13)
+      EDA_STREAM_CASE(15)
+      EDA_STREAM_CASE(17)
+      EDA_STREAM_CASE(19)
+      EDA_STREAM_CASE(21)
+      EDA_STREAM_CASE(23)
+      EDA_STREAM_CASE(25)
+      EDA_STREAM_CASE(27)
+      EDA_STREAM_CASE(29)
+      EDA_STREAM_CASE(31)
+      default: break;
+    }
+  } else {
+    switch (L) {
+      EDA_STREAM_CASE(33)
+      EDA_STREAM_CASE(35)
+      EDA_STREAM_CASE(37)
+      EDA_STREAM_CASE(39)
+      EDA_STREAM_CASE(41)
+      EDA_STREAM_CASE(43)
+      EDA_STREAM_CASE(45)
+      EDA_STREAM_CASE(47)
+      EDA_STREAM_CASE(49)
+      EDA_STREAM_CASE(51)
+      EDA_STREAM_CASE(53)
+      EDA_STREAM_CASE(55)
+      EDA_STREAM_CASE(57)
+      EDA_STREAM_CASE(59)
+      EDA_STREAM_CASE(61)
+      EDA_STREAM_CASE(63)
+      default: break;
+    }
   }
 #undef EDA_STREAM_CASE
 }
 
-// The streaming pass takes the elements of a strided axis whose windows are at most 31 samples when
+// The streaming pass takes the elements of a strided axis whose windows are at most 63 samples when
 // the line has at least 128 samples, the pass does not run in place, one element's field is under
 // 4 GiB (32-bit buffer offsets), and its operators are one of max, min, box or max + min of one
 // input.
@@ -1709,7 +1766,7 @@ bool stream_qualifies(const Geom& g, int axis, int nf, bool same_input, const in
 
 template <class T>
 void launch_stream(hipStream_t s, const Geom& g, int axis, int nf, const int* ops, bool scale, const VPtrs<T>& p,
-                   int32_t rungs, int32_t max_w) {
+                   int32_t rungs, int32_t max_w, bool big) {
   SPass a{};
   a.g = g;
   a.n = g.n[axis];
@@ -1730,12 +1787,17 @@ void launch_stream(hipStream_t s, const Geom& g, int axis, int nf, const int* op
   a.items = static_cast<int64_t>(g.batch) * a.outer * a.cgroups * pieces;
   if ((a.items + 3) / 4 > 0x7fffffff) throw std::invalid_argument("eda: too many waves for one pass");
   const dim3 grid(static_cast<unsigned>((a.items + 3) / 4), static_cast<unsigned>(rungs));
-  switch (a.mode) {
-    case 0: EDA_GO((k_vhgw_stream<T, 0>), grid, dim3(256), 0, s, a, p); break;
-    case 1: EDA_GO((k_vhgw_stream<T, 1>), grid, dim3(256), 0, s, a, p); break;
-    case 2: EDA_GO((k_vhgw_stream<T, 2>), grid, dim3(256), 0, s, a, p); break;
-    default: EDA_GO((k_vhgw_stream<T, 3>), grid, dim3(256), 0, s, a, p); break;
-  }
+  const auto go = [&](auto big_tag) {
+    constexpr bool kBig = decltype(big_tag)::value;
+    switch (a.mode) {
+      case 0: EDA_GO((k_vhgw_stream<T, 0, kBig>), grid, dim3(256), 0, s, a, p); break;
+      case 1: EDA_GO((k_vhgw_stream<T, 1, kBig>), grid, dim3(256), 0, s, a, p); break;
+      case 2: EDA_GO((k_vhgw_stream<T, 2, kBig>), grid, dim3(256), 0, s, a, p); break;
+      default: EDA_GO((k_vhgw_stream<T, 3, kBig>), grid, dim3(256), 0, s, a, p); break;
+    }
+  };
+  if (big) go(std::true_type{});
+  else go(std::false_type{});
 }
 
 }  // namespace
@@ -2224,15 +2286,15 @@ void launch_long(hipStream_t s, const Geom& g, int axis, int nf, const int* ops,
   else go(std::integral_constant<int, kLongMaxMin>{});
 }
 
-// The (rung, element) items of a launch whose clamped radius on lines of n samples is at least 16
-// (longer) or below it, from the host mirror of the window array; -1 when it is not mirrored.
-int64_t count_radius(const int32_t* win, int64_t items, int32_t n, bool longer) {
+// The (rung, element) items of a launch whose clamped radius on lines of n samples lies in
+// [rlo, rhi), from the host mirror of the window array; -1 when it is not mirrored.
+int64_t count_radius(const int32_t* win, int64_t items, int32_t n, int32_t rlo, int32_t rhi) {
   const int32_t* h = static_cast<const int32_t*>(mirror_host(win, static_cast<size_t>(items) * sizeof(int32_t)));
   if (h == nullptr) return -1;
   int64_t c = 0;
   for (int64_t k = 0; k < items; ++k) {
     const int32_t rw = std::min((h[k] - 1) / 2, n - 1);
-    c += h[k] >= 1 && (rw >= 16) == longer ? 1 : 0;
+    c += h[k] >= 1 && rw >= rlo && rw < rhi ? 1 : 0;
   }
   return c;
 }
@@ -2295,8 +2357,9 @@ void enqueue_staged(hipStream_t s, const Geom& g, int axis, int nf, bool same_in
   if (!done && sum && EDA_VHGW_PIPE) {
     // the pipelined kernel, one launch per row layout: windows up to 32 samples on plain rows
     // (radius <= 15), longer ones on padded rows
-    const bool shorter = rmin < 16;
-    const bool longer = (max_w - 1) / 2 >= 16 && a.n > 16;
+    const bool shorter = rmin < 16 && count_radius(p.win, g.batch * rungs, a.n, rmin, 16) != 0;
+    const bool longer = (max_w - 1) / 2 >= 16 && a.n > 16 &&
+                        count_radius(p.win, g.batch * rungs, a.n, std::max(16, rmin), a.n) != 0;
     const bool pipe_fits = (!shorter || pass_fits(a, sizeof(T), true, std::min(max_w, 31), 1)) &&
                            (!longer || pass_fits(a, sizeof(T), true, max_w, 2));
     Shape s1{};
@@ -2317,7 +2380,8 @@ void enqueue_staged(hipStream_t s, const Geom& g, int axis, int nf, bool same_in
     }
     if (pipe_fits && pf1 && pf2 && (shorter || longer)) {
       const auto items = [&](bool longer_ones) {
-        const int64_t c = count_radius(p.win, g.batch * rungs, a.n, longer_ones);
+        const int64_t c = count_radius(p.win, g.batch * rungs, a.n, longer_ones ? std::max(16, rmin) : rmin,
+                                       longer_ones ? a.n : 16);
         return static_cast<double>(c < 0 ? g.batch * rungs : c);
       };
       if (shorter) {
@@ -2355,7 +2419,7 @@ void enqueue_staged(hipStream_t s, const Geom& g, int axis, int nf, bool same_in
     int64_t active = mirror_active(p.win, g.batch * rungs);
     if (active < 0) active = g.batch * rungs;
     if (rmin > 0) {
-      const int64_t c = count_radius(p.win, g.batch * rungs, a.n, true);
+      const int64_t c = count_radius(p.win, g.batch * rungs, a.n, rmin, a.n);
       if (c >= 0) active = c;
     }
     const int fields = (nf == 2 && !same_input ? 2 : 1) + nf;
@@ -2407,20 +2471,28 @@ void enqueue_vhgw(hipStream_t s, const Geom& g, int axis, int nf, bool same_inpu
     a.outer = g.count / (static_cast<int64_t>(a.n) * a.inner);
     a.nf = nf;
     // the staged kernels must hold the launch's longer windows (else the whole pass stays with them)
-    if (max_w <= kStreamMaxL || pass_fits(a, sizeof(T), sum, max_w)) {
-      // elements with windows up to 31 samples stream; the launch's longer windows (if any) run the
-      // staged kernels, which skip the streamed elements
-      hipEvent_t t0 = nullptr;
-      if (timed) timer_note(s, &t0);
-      launch_stream<T>(s, g, axis, nf, ops, scale, p, rungs, std::min(max_w, kStreamMaxL));
-      EDA_LAUNCH("k_vhgw_stream");
-      if (timed) {
-        const int64_t c = count_radius(win, g.batch * rungs, g.n[axis], false);
-        const int fields = (nf == 2 && !same_input ? 2 : 1) + nf;
-        timer_done(s, t0, static_cast<double>(fields) * (c < 0 ? g.batch * rungs : c) * g.count * sizeof(T),
-                   kTagStream);
-      }
-      if (max_w > kStreamMaxL) enqueue_staged<T>(s, g, axis, nf, same_input, ops, scale, p, max_w, rungs, 16, timed);
+    if (max_w <= kStreamBigL || pass_fits(a, sizeof(T), sum, max_w)) {
+      // elements with windows up to 31 samples stream in the short walk, up to 63 in the long one;
+      // the launch's longer windows (if any) run the staged kernels, which skip the streamed elements
+      const auto stream_class = [&](bool big) {
+        hipEvent_t t0 = nullptr;
+        if (timed) timer_note(s, &t0);
+        launch_stream<T>(s, g, axis, nf, ops, scale, p, rungs, std::min(max_w, big ? kStreamBigL : kStreamMaxL), big);
+        EDA_LAUNCH("k_vhgw_stream");
+        if (timed) {
+          const int64_t c = count_radius(win, g.batch * rungs, g.n[axis], big ? 16 : 0, big ? 32 : 16);
+          const int fields = (nf == 2 && !same_input ? 2 : 1) + nf;
+          timer_done(s, t0, static_cast<double>(fields) * (c < 0 ? g.batch * rungs : c) * g.count * sizeof(T),
+                     kTagStream);
+        }
+      };
+      // a class none of the launch's elements falls in is not launched (when the window array has
+      // a host copy; else every class is)
+      const int64_t items = g.batch * rungs;
+      if (count_radius(win, items, g.n[axis], 0, 16) != 0) stream_class(false);
+      if (max_w > kStreamMaxL && count_radius(win, items, g.n[axis], 16, 32) != 0) stream_class(true);
+      if (max_w > kStreamBigL && count_radius(win, items, g.n[axis], 32, g.n[axis]) != 0)
+        enqueue_staged<T>(s, g, axis, nf, same_input, ops, scale, p, max_w, rungs, 32, timed);
       return;
     }
   }
@@ -2459,51 +2531,66 @@ T* cube_scratch(hipStream_t s, int64_t elems) {
   return static_cast<T*>(slot.first);
 }
 
-// The scratch field for the first pass of a one-rung in-place cube whose axis 0 streams; nullptr
-// when the cube keeps its passes in place.
+// The fields the passes of a max, min or box cube write, so that no pass reads the field it writes
+// (the streaming passes need that): the last pass writes out0, the others alternate with a scratch
+// field (the caller's, of `rungs` fields; else, for one rung, the stream's own). An in-place 3-D
+// cube keeps its last pass in place. Without a scratch every pass writes out0.
 template <class T>
-T* first_pass_scratch(hipStream_t s, const Geom& g, const T* in, const T* out0, int32_t rungs) {
-  if (g.dim < 2 || rungs != 1 || in != out0 || g.st[0] == 1 || g.n[0] < 128) return nullptr;
-  if (static_cast<double>(g.count) * sizeof(T) >= 4294967295.0) return nullptr;
-  return cube_scratch<T>(s, g.batch * g.count);
+void cube_targets(hipStream_t s, const Geom& g, const T* in, T* out0, int32_t rungs, T* scratch, T** dst) {
+  const int D = g.dim;
+  for (int a = 0; a < D; ++a) dst[a] = out0;
+  if (D < 2) return;
+  T* scr = scratch;
+  if (scr == nullptr && rungs == 1 && static_cast<double>(g.count) * sizeof(T) < 4294967295.0)
+    scr = cube_scratch<T>(s, g.batch * g.count);
+  if (scr == nullptr) return;
+  if (D == 2) {
+    dst[0] = scr;
+  } else if (in != out0) {
+    dst[1] = scr;
+  } else {
+    dst[0] = scr;
+  }
 }
 
 template <class T>
 bool enqueue_cube_rungs(hipStream_t s, const Geom& g, CubeOp op, const T* in, T* out0, T* out1, const int32_t* win,
-                        int32_t rungs, int64_t stride, int32_t max_w, T* pairs) {
+                        int32_t rungs, int64_t stride, int32_t max_w, T* pairs, T* scratch) {
   if (max_w < 1) return false;
   const int D = g.dim;
   if (op == CubeOp::kBox) {
-    T* scr = first_pass_scratch<T>(s, g, in, out0, rungs);
+    T* dst[3];
+    cube_targets<T>(s, g, in, out0, rungs, scratch, dst);
     for (int a = 0; a < D; ++a) {
       const int ops[1] = {kVSum};
-      const T* src[1] = {a == 0 ? in : (a == 1 && scr ? scr : out0)};
-      T* dst[1] = {a == 0 && scr ? scr : out0};
-      enqueue_vhgw<T>(s, g, a, 1, true, ops, a == D - 1, src, dst, win, max_w, rungs, a == 0 ? 0 : stride, stride);
+      const T* src[1] = {a == 0 ? in : dst[a - 1]};
+      T* out[1] = {dst[a]};
+      enqueue_vhgw<T>(s, g, a, 1, true, ops, a == D - 1, src, out, win, max_w, rungs, a == 0 ? 0 : stride, stride);
     }
     return false;
   }
   // The max and min side by side: the streaming row pass writes them, so it runs last (axes 0 ..
   // D - 1; max and min give the same bits in any order of the axes).
   const bool il = pairs != nullptr && op == CubeOp::kMaxMin && D >= 2 && rows_qualify(g, D - 1, false);
-  T* scr = op == CubeOp::kMaxMin ? nullptr : first_pass_scratch<T>(s, g, in, out0, rungs);
+  T* dst[3];
+  if (op != CubeOp::kMaxMin) cube_targets<T>(s, g, in, out0, rungs, scratch, dst);
   for (int k = 0; k < D; ++k) {
-    // axis 0 first: its pass reads the input and writes another field, which the streaming pass
-    // of a strided axis needs (it does not run in place)
+    // axis 0 first: its pass reads the input and writes another field, as the streaming pass of a
+    // strided axis needs
     const int a = k;
     const bool first = k == 0;
     const int64_t ins = first ? 0 : stride;
     if (op == CubeOp::kMaxMin) {
       const int ops[2] = {kVMax, kVMin};
       const T* src[2] = {first ? in : out0, first ? in : out1};
-      T* dst[2] = {out0, out1};
-      enqueue_vhgw<T>(s, g, a, 2, first, ops, false, src, dst, win, max_w, rungs, ins, stride,
+      T* out[2] = {out0, out1};
+      enqueue_vhgw<T>(s, g, a, 2, first, ops, false, src, out, win, max_w, rungs, ins, stride,
                       il && k == D - 1 ? pairs : nullptr);
     } else {
       const int ops[1] = {op == CubeOp::kMax ? kVMax : kVMin};
-      const T* src[1] = {first ? in : (k == 1 && scr ? scr : out0)};
-      T* dst[1] = {first && scr ? scr : out0};
-      enqueue_vhgw<T>(s, g, a, 1, true, ops, false, src, dst, win, max_w, rungs, ins, stride);
+      const T* src[1] = {first ? in : dst[a - 1]};
+      T* out[1] = {dst[a]};
+      enqueue_vhgw<T>(s, g, a, 1, true, ops, false, src, out, win, max_w, rungs, ins, stride);
     }
   }
   return il;
@@ -2512,7 +2599,7 @@ bool enqueue_cube_rungs(hipStream_t s, const Geom& g, CubeOp op, const T* in, T*
 template <class T>
 void enqueue_cube(hipStream_t s, const Geom& g, CubeOp op, const T* in, T* out0, T* out1, const int32_t* win,
                   int32_t max_w) {
-  enqueue_cube_rungs<T>(s, g, op, in, out0, out1, win, 1, 0, max_w);
+  enqueue_cube_rungs<T>(s, g, op, in, out0, out1, win, 1, 0, max_w, nullptr, nullptr);
 }
 
 template <class T>
@@ -2532,8 +2619,8 @@ template void enqueue_cube<float>(hipStream_t, const Geom&, CubeOp, const float*
 template void enqueue_cube<double>(hipStream_t, const Geom&, CubeOp, const double*, double*, double*, const int32_t*,
                                    int32_t);
 template bool enqueue_cube_rungs<float>(hipStream_t, const Geom&, CubeOp, const float*, float*, float*, const int32_t*,
-                                        int32_t, int64_t, int32_t, float*);
+                                        int32_t, int64_t, int32_t, float*, float*);
 template bool enqueue_cube_rungs<double>(hipStream_t, const Geom&, CubeOp, const double*, double*, double*,
-                                         const int32_t*, int32_t, int64_t, int32_t, double*);
+                                         const int32_t*, int32_t, int64_t, int32_t, double*, double*);
 
 }  // namespace eda

@@ -1,10 +1,12 @@
 # SPDX-License-Identifier: Apache-2.0
-"""The streaming pass of a strided axis (windows up to 31 samples, lines of at least 128 samples)
-bitwise against the oracle's cube: every odd window from 1 to 31; launches that mix them with longer
-windows (the staged kernels take those and skip the streamed elements) and with elements that do not
-run (window 0); slabs whose line count is not a multiple of 64; 3-D volumes (axes 0 and 1 stream);
-signed zeros and ties; float32 and float64; and the in-place cube, whose first pass writes a scratch
-field so that axis 0 still streams."""
+"""The streaming pass of strided axes (windows up to 63 samples, lines of at least 128 samples, one
+lane per line, rows read through scalar offsets) bitwise against the oracle's cube: every odd window
+from 1 to 63 (the short walk to 31, the long one with two canonical chunks per box segment above);
+launches that mix them with longer windows (the staged kernels take those and skip
+the streamed elements) and with elements that do not run (window 0); slabs whose line count is not
+a multiple of 64; shapes whose contiguous axis runs the other kernels; 1-D signals; 3-D volumes
+(axes 0 and 1 stream); signed zeros and ties; float32 and float64; the in-place cube (its first
+pass writes a scratch field, so axis 0 still streams)."""
 
 from __future__ import annotations
 
@@ -18,7 +20,7 @@ import oracle
 
 pytestmark = pytest.mark.gpu
 DTYPES = [np.float32, np.float64]
-WINDOWS = list(range(1, 32, 2))
+WINDOWS = list(range(1, 64, 2))
 
 
 def bits_equal(a, b):
@@ -38,7 +40,9 @@ def field(shape, dtype, seed, ties=False):
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
-@pytest.mark.parametrize("shape", [(128, 64), (200, 50), (512, 130), (130, 9, 70)])
+@pytest.mark.parametrize(
+    "shape", [(128, 64), (200, 50), (512, 130), (130, 9, 70), (96, 256), (40, 300), (3000,), (6, 20, 160)]
+)
 @pytest.mark.parametrize("op", ["max", "min", "box", "maxmin"])
 def test_every_short_window_is_bitwise_the_oracle(dtype, shape, op):
     wins = WINDOWS
@@ -55,10 +59,10 @@ def test_every_short_window_is_bitwise_the_oracle(dtype, shape, op):
 @pytest.mark.parametrize("dtype", DTYPES)
 @pytest.mark.parametrize("op", ["max", "box", "maxmin"])
 def test_mixed_launch_and_idle_elements(dtype, op):
-    """One launch whose largest window is 31 streams every element; in one with a 33 the staged
-    kernel takes that element and skips the others; elements with window 0 are not filtered."""
+    """Launches that span the short walk, the long walk and the staged kernels (windows above 63),
+    each skipping the others' elements; elements with window 0 are not filtered."""
     shape = (256, 96)
-    for wins in ([31, 0, 5, 1], [33, 0, 5, 1]):
+    for wins in ([31, 0, 5, 1], [33, 0, 5, 1], [65, 63, 0, 31, 1], [129, 3]):
         x = field((len(wins),) + shape, dtype, zlib.crc32(repr((op, wins)).encode()))
         got = G.cube(x, op, wins)
         for b, w in enumerate(wins):
@@ -74,8 +78,8 @@ def test_mixed_launch_and_idle_elements(dtype, op):
 @pytest.mark.parametrize("dtype", DTYPES)
 @pytest.mark.parametrize("op", ["max", "min", "box"])
 def test_in_place_cube(dtype, op):
-    wins = [3, 15, 31, 45]
-    x = field((4, 512, 512), dtype, 11)
+    wins = [3, 15, 31, 45, 63, 65]
+    x = field((6, 512, 512), dtype, 11)
     got = G.cube(x, op, wins, inplace=True)
     for b, w in enumerate(wins):
         assert bits_equal(got[b], oracle.cube(x[b], op, w)), w
