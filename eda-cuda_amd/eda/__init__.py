@@ -16,6 +16,7 @@ from typing import Any
 from . import _native
 from ._api import DecompositionInfo, Monogenic, Orientation, decompose, monogenic
 from ._autograd import TapedDecomposition, decompose_function, differentiable_decompose
+from ._device import DeviceArray
 from ._errors import CudaError, DecompositionError
 from ._many import decompose_many
 
@@ -25,6 +26,7 @@ __all__ = [
     "CudaError",
     "DecompositionError",
     "DecompositionInfo",
+    "DeviceArray",
     "Monogenic",
     "Orientation",
     "TapedDecomposition",

@@ -4,9 +4,10 @@ libeda_hip.so. The rules, the result layout and the error policy follow the refe
 src/eda/_api.py:184-353 (``decompose``), :433-526 (argument checks), :539-607 (layout, info,
 errors), so that the call is a drop-in.
 
-Host input (NumPy, or anything ``numpy.asarray`` takes) returns NumPy arrays. ROCm device input (a
-PyTorch-ROCm tensor, or any object with ``__dlpack__`` on a ROCm device, which PyTorch imports)
-returns a PyTorch tensor on the input's device, complete on return."""
+Host input (NumPy, or anything ``numpy.asarray`` takes) returns NumPy arrays. ROCm device input is
+read in place: a PyTorch-ROCm tensor returns a PyTorch tensor on its device; any other device array
+(a DLPack producer on a ROCm device, or an object with ``__cuda_array_interface__``) returns an
+``eda.DeviceArray`` (``_device.py``). Results are complete on return."""
 
 from __future__ import annotations
 
@@ -19,7 +20,7 @@ from typing import Any
 
 import numpy as np
 
-from . import _native
+from . import _device, _native
 from ._errors import DecompositionError
 
 ERROR_NON_FINITE = 1
@@ -256,12 +257,12 @@ def _is_device_input(x: Any) -> bool:
     if torch is not None and isinstance(x, torch.Tensor):
         return x.device.type == "cuda"
     dev = getattr(x, "__dlpack_device__", None)
-    if dev is not None:
+    if dev is not None and hasattr(x, "__dlpack__"):
         try:
             return int(dev()[0]) == _KDL_ROCM
         except Exception:
             return False
-    return False
+    return hasattr(x, "__cuda_array_interface__")
 
 
 def _device_tensor(x: Any):
@@ -332,17 +333,45 @@ class _Call:
     B: int
     stream_handle: int
     device: Any = None  # the torch.device of device input
+    wrap: bool = False  # device input that is not a PyTorch tensor: results are DeviceArrays
+
+
+def _wrap(call: _Call, t: Any) -> Any:
+    return _device.DeviceArray(t) if call.wrap else t
+
+
+def _device_layout(x: Any, torch: Any) -> tuple:
+    """(pointer, shape, strides in elements, dtype name, device ordinal, complex, object to keep
+    alive, array-interface dict or None) of a device input: a PyTorch tensor, a DLPack producer on a
+    ROCm device (imported by PyTorch), or an object with ``__cuda_array_interface__``."""
+    if isinstance(x, torch.Tensor) or (hasattr(x, "__dlpack__") and hasattr(x, "__dlpack_device__")):
+        t = _device_tensor(x)
+        dev = t.device.index if t.device.index is not None else torch.cuda.current_device()
+        return (t.data_ptr() if t.numel() else 0, tuple(t.shape), tuple(t.stride()),
+                str(t.dtype).removeprefix("torch."), dev, t.is_complex(), t, None)
+    cai = x.__cuda_array_interface__
+    ptr, shape, strides, np_dt = _device.interface_layout(cai)
+    dev = torch.cuda.current_device()
+    if ptr:
+        got = ctypes.c_int32(-1)
+        _native.core().eda_pointer_device(ctypes.c_void_p(ptr), ctypes.byref(got))
+        if got.value < 0:
+            raise ValueError("eda: the __cuda_array_interface__ pointer is not ROCm device memory")
+        dev = got.value
+    return ptr, shape, strides, np_dt.name, dev, np_dt.kind == "c", x, cai
 
 
 def _prepare(x: Any, axes: Any, dtype: Any, stream: Any) -> _Call:
     """The intake of ``decompose`` and ``monogenic`` (reference src/eda/_api.py:143-181): the dtype
-    rules, the axes, the batch layout, host staging or the device tensor read in place."""
+    rules, the axes, the batch layout, host staging or the device array read in place."""
     on_device = _is_device_input(x)
+    torch = _torch() if on_device else None
     if on_device:
-        data = _device_tensor(x)
-        src = str(data.dtype).removeprefix("torch.")
-        shape = tuple(data.shape)
-        kind_ok = not (data.is_complex())
+        if torch is None:
+            raise TypeError("eda: device input needs PyTorch-ROCm")
+        explicit = None if stream is None else _device.stream_handle(stream)
+        dptr, shape, dstrides, src, dev_index, is_complex, owner, cai = _device_layout(x, torch)
+        kind_ok = not is_complex
     else:
         data = np.asarray(x)
         src = data.dtype.name
@@ -376,20 +405,21 @@ def _prepare(x: Any, axes: Any, dtype: Any, stream: Any) -> _Call:
     keep_alive = None
     stream_handle = 0
     if on_device:
-        torch = _torch()
         if src not in _native.SOURCE_CODES:
             raise TypeError(f"eda: arrays of dtype {src} are not supported (real bool, integer and float arrays only)")
         # read in place through the strides of every axis, cast to the compute dtype on the fly
         # (reference cpp/src/decompose.cpp:25-82, cpp/src/field/convert.cu)
-        tv = data.permute(order) if order != list(range(ndim)) else data
+        pshape = [shape[i] for i in order]
+        pstrides = [dstrides[i] for i in order]
         nb = len(batch_axes)
-        bshape, bstrides = tuple(tv.shape[:nb]), tuple(tv.stride()[:nb])
-        device = tv.device.index if tv.device.index is not None else torch.cuda.current_device()
-        desc.data = tv.data_ptr() if tv.numel() else None
+        bshape, bstrides = tuple(pshape[:nb]), tuple(pstrides[:nb])
+        device = dev_index
+        desc.data = dptr if math.prod(shape) else None
         desc.device = device
         desc.source = 0 if src == target else _native.SOURCE_CODES[src]
         for a in range(len(axes)):
-            desc.strides[a] = tv.stride(nb + a)
+            desc.strides[a] = pstrides[nb + a]
+        arrays = None
         if nb == 1:
             desc.batch_stride = bstrides[0]
         elif nb > 1:
@@ -397,14 +427,14 @@ def _prepare(x: Any, axes: Any, dtype: Any, stream: Any) -> _Call:
             desc.batch_axes = nb
             desc.batch_shape = ctypes.cast(arrays[0], ctypes.POINTER(ctypes.c_int64))
             desc.batch_strides = ctypes.cast(arrays[1], ctypes.POINTER(ctypes.c_int64))
-            tv = (tv, arrays)
-        keep_alive = tv
-        if stream is None:
-            stream_handle = int(torch.cuda.current_stream(device).cuda_stream)
-        elif isinstance(stream, int):
-            stream_handle = stream
+        # the streams the input is read after: an array interface names its own (and stream=);
+        # tensors and DLPack imports are ordered on the current PyTorch stream (or stream=)
+        if cai is not None:
+            stream_handle, joined = _device.join_fences(_device.interface_fences(cai, explicit), device, torch)
         else:
-            stream_handle = int(getattr(stream, "cuda_stream"))
+            joined = None
+            stream_handle = explicit if explicit is not None else int(torch.cuda.current_stream(device).cuda_stream)
+        keep_alive = (owner, arrays, joined)
     else:
         xc = data if data.dtype == np.dtype(target) else data.astype(target)
         xt = np.ascontiguousarray(xc.transpose(order)) if order != list(range(ndim)) else np.ascontiguousarray(xc)
@@ -414,7 +444,8 @@ def _prepare(x: Any, axes: Any, dtype: Any, stream: Any) -> _Call:
         desc.device = -1
 
     return _Call(desc, keep_alive, on_device, target, code, ndim, axes, batch_axes, spatial, batch_shape, B,
-                 stream_handle, data.device if on_device else None)
+                 stream_handle, torch.device("cuda", dev_index) if on_device else None,
+                 on_device and not isinstance(x, torch.Tensor))
 
 
 def decompose(
@@ -449,10 +480,13 @@ def decompose(
     Riesz contraction of the orientation as split-BF16 products on the matrix cores (three bf16
     products per term, float32 accumulation) instead of the exact float32 transforms.
 
-    Host input returns NumPy arrays; a ROCm device array (a PyTorch-ROCm tensor, or a DLPack
-    producer on a ROCm device) returns a PyTorch tensor on the same device. The call waits for the
-    current PyTorch stream of the device (or ``stream``, a ``torch.cuda.Stream`` or a raw HIP
-    stream handle) before it reads device input, and its result is complete on return."""
+    Host input returns NumPy arrays. A PyTorch-ROCm tensor returns a PyTorch tensor on its device;
+    any other ROCm device array (a DLPack producer on a ROCm device, or an object with
+    ``__cuda_array_interface__``) returns an ``eda.DeviceArray``. The call reads device input after
+    the current PyTorch stream of the device, or after ``stream`` (an integer HIP stream handle, an
+    object with ``__cuda_stream__``, a ``torch.cuda.Stream``); an array-interface input is read
+    after the stream it reports (version 3; older versions: the null stream) and ``stream``. The
+    result is complete on return."""
     max_imf = _max_imf(max_imf)
     tie = _tie_tolerance(tie_tolerance)
     sifts = _sifts(sifts)
@@ -474,6 +508,7 @@ def decompose(
                                                        call.batch_shape, call.B)
     stream_handle = call.stream_handle
     tdevice = call.device
+    wrap = _device.DeviceArray if call.wrap else (lambda t: t)
     out_final = _restore_order(ndim, batch_axes, axes)
     del call
     res = _call_native(desc, max_imf, tie, sifts, check_finite, stream_handle, return_orientation, dct_mode)
@@ -501,7 +536,7 @@ def decompose(
             if out.numel():
                 _native.check(lib.eda_collect(ctypes.byref(res), ctypes.c_void_p(out.data_ptr()), int(desc.device),
                                               err, len(err)), err)
-            y = out.reshape(batch_shape + (P,) + spatial).permute(out_final)
+            y = wrap(out.reshape(batch_shape + (P,) + spatial).permute(out_final))
         else:
             out = np.empty((B, P) + spatial, dtype=np.dtype(target))
             if out.size:
@@ -512,7 +547,7 @@ def decompose(
         orient = None
         if return_orientation:
             orient = _orientation(res, batch_shape, spatial, ndim, batch_axes, axes, target,
-                                  tdevice if on_device else None)
+                                  tdevice if on_device else None, wrap)
     finally:
         _native.core().eda_release(ctypes.byref(res))
     del keep_alive
@@ -525,7 +560,8 @@ def decompose(
 
 
 def _orientation(res: _native.Result, batch_shape: tuple[int, ...], spatial: tuple[int, ...], ndim: int,
-                 batch_axes: tuple[int, ...], axes: tuple[int, ...], target: str, device: Any) -> Orientation:
+                 batch_axes: tuple[int, ...], axes: tuple[int, ...], target: str, device: Any,
+                 wrap: Any = None) -> Orientation:
     """The ``Orientation`` of a call from the kept direction (B, K, D, *spatial) and coherence (B,
     K, *spatial), in the public layout (reference src/eda/_api.py:558-577)."""
     D, K = len(axes), int(res.K)
@@ -544,7 +580,7 @@ def _orientation(res: _native.Result, batch_shape: tuple[int, ...], spatial: tup
                                                       ctypes.c_void_p(coh.data_ptr()), int(device.index), err,
                                                       len(err)), err)
         arrays = [a.reshape(sh).permute(pm) for a, sh, pm in zip((dirs, coh), shapes, perms)]
-        return Orientation(*arrays)
+        return Orientation(*(wrap(a) for a in arrays) if wrap is not None else arrays)
     dirs = np.empty((B, K, D) + spatial, dtype=np.dtype(target))
     coh = np.empty((B, K) + spatial, dtype=np.dtype(target))
     if coh.size:
@@ -617,7 +653,7 @@ def monogenic(
         size = n * max(comps, 1)
         part = out[offset:offset + size]
         if call.on_device:
-            parts.append(part.view(computed).permute(perm))
+            parts.append(_wrap(call, part.view(computed).permute(perm)))
         else:
             part = part.reshape(computed)
             parts.append(part.transpose(perm) if perm != list(range(len(perm))) else part)

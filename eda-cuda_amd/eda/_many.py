@@ -36,11 +36,21 @@ from ._api import (
 from ._errors import DecompositionError
 
 
+def _tensor(x: Any) -> Any:
+    """A device field as a PyTorch tensor (no copy): a tensor, a DLPack import, or an array
+    interface wrapped by torch.as_tensor."""
+    import torch  # noqa: PLC0415
+
+    if isinstance(x, torch.Tensor):
+        return x
+    if hasattr(x, "__dlpack__"):
+        return torch.from_dlpack(x)
+    return torch.as_tensor(x, device="cuda")
+
+
 def _key(x: Any) -> tuple:
     if _is_device_input(x):
-        import torch  # noqa: PLC0415
-
-        t = x if isinstance(x, torch.Tensor) else torch.from_dlpack(x)
+        t = _tensor(x)
         return ("device", t.device.index or 0, str(t.dtype), tuple(t.shape))
     a = np.asarray(x)
     return ("host", -1, a.dtype.str, a.shape)
@@ -50,8 +60,7 @@ def _stack(fields: list) -> Any:
     if _is_device_input(fields[0]):
         import torch  # noqa: PLC0415
 
-        ts = [f if isinstance(f, torch.Tensor) else torch.from_dlpack(f) for f in fields]
-        return torch.stack(ts)
+        return torch.stack([_tensor(f) for f in fields])
     return np.stack([np.asarray(f) for f in fields])
 
 
