@@ -380,6 +380,222 @@ __global__ void __launch_bounds__(kThreads) k_gemm_bf16x3(GemmDims d, const floa
       }
 }
 
+
+// ---- the split-BF16 axis contraction (dct = "bf16x3") -------------------------------------------------
+// The same three-term products as k_gemm_bf16x3, laid out for one axis pass of a DCT: the n x n
+// matrix P (P[j][k], output j) is split into bf16 hi / lo once (split_table) and read as such; a
+// workgroup owns 64 lines and up to 256 outputs of them, so the data is read from HBM once and split
+// once per 256 outputs (the generic kernel re-read and re-split both operands for every 64 x 64
+// output tile: 3.4x the algorithmic bytes, 11 % of the matrix cores' cycles busy). K runs in steps of
+// 32 through LDS, rows along k (16-byte operand reads), bf16 pairs hi / lo in separate arrays.
+//   contiguous axis: C[l][j] = sum_k X[l][k] P[j][k]: the MFMA rows are the 64 lines, the columns the
+//     outputs (each wave 64 of them: 4 x 4 tiles of 16 x 16);
+//   strided axis:    C[j][i] = sum_k P[j][k] X[k][i] per (element, slab): rows the outputs (each wave
+//     64), columns 64 adjacent lines i, so that the stores run along i.
+// Every output sums its K terms in one fixed order (k0 ascending, the three products lo.hi, hi.lo,
+// hi.hi per step, as k_gemm_bf16x3), whatever the batch or the grid.
+constexpr int kAxK = 32, kAxRow = kAxK + 8, kAxLines = 64, kAxOut = 256;
+constexpr int kAxLT = kAxLines / 16;  // line tiles of 16 per wave
+
+struct AxisGemm {
+  int64_t lines;     // contiguous: lines per element (count / n); strided: inner (adjacent lines)
+  int64_t slabs;     // strided: slabs per element (outer); contiguous: 1
+  int32_t n;         // K = the outputs = the axis length (a multiple of 32)
+  int64_t count;     // samples per element
+  int64_t groups;    // line groups of kAxLines per slab
+  int32_t oblocks;   // output blocks of 256
+};
+
+template <bool STRIDED>
+__global__ void __launch_bounds__(kThreads) k_axis_bf16x3(AxisGemm a, const __bf16* __restrict__ Ph,
+                                                          const __bf16* __restrict__ Pl,
+                                                          const float* __restrict__ X, float* __restrict__ C,
+                                                          const int32_t* __restrict__ on) {
+  __shared__ __align__(16) __bf16 Dh[kAxLines][kAxRow];  // the data: line i, k
+  __shared__ __align__(16) __bf16 Dl[kAxLines][kAxRow];
+  __shared__ __align__(16) __bf16 Mh[kAxOut][kAxRow];    // the matrix: output j, k
+  __shared__ __align__(16) __bf16 Ml[kAxOut][kAxRow];
+  // blockIdx.x: (slab, line group); blockIdx.y: output block; blockIdx.z: element
+  const int64_t b = blockIdx.z;
+  if (on != nullptr && on[b] == 0) return;
+  const int64_t slab = blockIdx.x / a.groups;
+  const int64_t l0 = (blockIdx.x - slab * a.groups) * kAxLines;
+  const int32_t j0 = static_cast<int32_t>(blockIdx.y) * kAxOut;
+  const int32_t n = a.n;
+  const int32_t nout = min(kAxOut, n - j0);
+  const int64_t base = b * a.count + slab * static_cast<int64_t>(n) * (STRIDED ? a.lines : 1);
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lk = (lane >> 4) * 8;
+  // contiguous: acc[line tile][output tile]; strided: acc[output tile][line tile]
+  mf4 acc[STRIDED ? 4 : kAxLT][STRIDED ? kAxLT : 4];
+#pragma unroll
+  for (int p = 0; p < (STRIDED ? 4 : kAxLT); ++p)
+#pragma unroll
+    for (int q = 0; q < (STRIDED ? kAxLT : 4); ++q) acc[p][q] = mf4{0.f, 0.f, 0.f, 0.f};
+  // the global reads of step k0 go to registers one step ahead (their latency overlaps the
+  // previous step's products); the data tile: kAxLines lines x 32 k (lines past the end: zeros);
+  // the matrix tile: nout outputs x 32 k of the split table (16-byte reads; rows past nout: zeros)
+  constexpr int kXR = kAxLines / 32;  // data runs of 4 per thread
+  float4 px[kXR];
+  bf16x8 pmh[4], pml[4];
+  const auto fetch = [&](int32_t k0) {
+#pragma unroll
+    for (int r = 0; r < kXR; ++r) {
+      const int e = tid + r * kThreads;
+      px[r] = float4{0.f, 0.f, 0.f, 0.f};
+      if constexpr (!STRIDED) {
+        const int ii = e >> 3, kk = (e & 7) * 4;
+        if (l0 + ii < a.lines) px[r] = *reinterpret_cast<const float4*>(X + base + (l0 + ii) * n + k0 + kk);
+      } else {
+        const int kk = e / (kAxLines / 4), ii = (e % (kAxLines / 4)) * 4;
+        if (l0 + ii < a.lines) px[r] = *reinterpret_cast<const float4*>(X + base + (k0 + kk) * a.lines + l0 + ii);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int e = tid + r * kThreads;  // 1024 runs of 8 bf16
+      const int jj = e >> 2, kk = (e & 3) * 8;
+      pmh[r] = bf16x8{};
+      pml[r] = bf16x8{};
+      if (jj < nout) {
+        const int64_t at = static_cast<int64_t>(j0 + jj) * n + k0 + kk;
+        pmh[r] = *reinterpret_cast<const bf16x8*>(Ph + at);
+        pml[r] = *reinterpret_cast<const bf16x8*>(Pl + at);
+      }
+    }
+  };
+  fetch(0);
+  for (int32_t k0 = 0; k0 < n; k0 += kAxK) {
+    // the fetched tiles into LDS, the data split into hi / lo on the way
+#pragma unroll
+    for (int r = 0; r < kXR; ++r) {
+      const int e = tid + r * kThreads;
+      const float4 v = px[r];
+      if constexpr (!STRIDED) {
+        const int ii = e >> 3, kk = (e & 7) * 4;
+        bf16x4 h, l;
+        split4(v, h, l);
+        *reinterpret_cast<bf16x4*>(&Dh[ii][kk]) = h;
+        *reinterpret_cast<bf16x4*>(&Dl[ii][kk]) = l;
+      } else {
+        const int kk = e / (kAxLines / 4), ii = (e % (kAxLines / 4)) * 4;
+        split_bf16(v.x, &Dh[ii][kk], &Dl[ii][kk]);
+        split_bf16(v.y, &Dh[ii + 1][kk], &Dl[ii + 1][kk]);
+        split_bf16(v.z, &Dh[ii + 2][kk], &Dl[ii + 2][kk]);
+        split_bf16(v.w, &Dh[ii + 3][kk], &Dl[ii + 3][kk]);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int e = tid + r * kThreads;
+      const int jj = e >> 2, kk = (e & 3) * 8;
+      *reinterpret_cast<bf16x8*>(&Mh[jj][kk]) = pmh[r];
+      *reinterpret_cast<bf16x8*>(&Ml[jj][kk]) = pml[r];
+    }
+    __syncthreads();
+    if (k0 + kAxK < n) fetch(k0 + kAxK);
+    // the wave's 64 outputs (4 tiles) and the data's kAxLT line tiles: contiguous, A = the data,
+    // B = the outputs; strided, A = the outputs, B = the data
+    bf16x8 dh[kAxLT], dl[kAxLT], mh[4], ml[4];
+#pragma unroll
+    for (int p = 0; p < kAxLT; ++p) {
+      dh[p] = *reinterpret_cast<const bf16x8*>(&Dh[16 * p + lr][lk]);
+      dl[p] = *reinterpret_cast<const bf16x8*>(&Dl[16 * p + lr][lk]);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      mh[q] = *reinterpret_cast<const bf16x8*>(&Mh[64 * wave + 16 * q + lr][lk]);
+      ml[q] = *reinterpret_cast<const bf16x8*>(&Ml[64 * wave + 16 * q + lr][lk]);
+    }
+    const bf16x8* ah = STRIDED ? mh : dh;
+    const bf16x8* al = STRIDED ? ml : dl;
+    const bf16x8* bh = STRIDED ? dh : mh;
+    const bf16x8* bl = STRIDED ? dl : ml;
+#pragma unroll
+    for (int p = 0; p < (STRIDED ? 4 : kAxLT); ++p)
+#pragma unroll
+      for (int q = 0; q < (STRIDED ? kAxLT : 4); ++q) {
+        acc[p][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[p], bh[q], acc[p][q], 0, 0, 0);
+        acc[p][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[p], bl[q], acc[p][q], 0, 0, 0);
+        acc[p][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[p], bh[q], acc[p][q], 0, 0, 0);
+      }
+    __syncthreads();
+  }
+  // D(4 (lane / 16) + v, lane % 16) of tile (p, q)
+#pragma unroll
+  for (int p = 0; p < (STRIDED ? 4 : kAxLT); ++p)
+#pragma unroll
+    for (int q = 0; q < (STRIDED ? kAxLT : 4); ++q)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = 16 * p + 4 * (lane >> 4) + v;
+        const int col = 16 * q + lr;
+        if constexpr (!STRIDED) {
+          const int64_t l = l0 + row;
+          const int32_t j = 64 * wave + col;
+          if (l < a.lines && j < nout) C[base + l * n + j0 + j] = acc[p][q][v];
+        } else {
+          const int32_t j = 64 * wave + row;
+          const int64_t i = l0 + col;
+          if (i < a.lines && j < nout) C[base + static_cast<int64_t>(j0 + j) * a.lines + i] = acc[p][q][v];
+        }
+      }
+}
+
+// The split table of a DCT matrix (n x n, row-major, rows the outputs): hi = bf16(v), lo = bf16(v -
+// hi), built once per matrix on a stream of its own (the matrices live as long as the process).
+struct SplitTab {
+  __bf16* hi = nullptr;
+  __bf16* lo = nullptr;
+};
+const SplitTab& split_table(const float* M, int32_t n) {
+  static std::mutex mu;
+  static std::map<const float*, SplitTab> cache;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find(M);
+  if (it != cache.end()) return it->second;
+  const size_t nn = static_cast<size_t>(n) * n;
+  SplitTab t;
+  EDA_HIP(hipMalloc(&t.hi, nn * sizeof(__bf16)));
+  EDA_HIP(hipMalloc(&t.lo, nn * sizeof(__bf16)));
+  __bf16* hi = t.hi;
+  __bf16* lo = t.lo;
+  hipStream_t bs = nullptr;
+  EDA_HIP(hipStreamCreateWithFlags(&bs, hipStreamNonBlocking));
+  flat(bs, static_cast<int64_t>(nn), [=] __device__(int64_t e) { split_bf16(M[e], hi + e, lo + e); });
+  const hipError_t built = hipStreamSynchronize(bs);
+  (void)hipStreamDestroy(bs);
+  EDA_HIP(built);
+  return cache.emplace(M, t).first->second;
+}
+
+// The contraction takes float32 axes whose length is a multiple of 32 (and, strided, whose line
+// count is a multiple of 4 for the 16-byte reads); others keep the generic split-BF16 GEMM.
+bool axis_bf16_fits(const Geom& g, int a) {
+  const int64_t n = g.n[a];
+  const int64_t inner = g.st[a];
+  return n % kAxK == 0 && (inner == 1 || inner % 4 == 0);
+}
+void axis_bf16x3(hipStream_t s, const Geom& g, int a, const float* M, const float* in, float* out,
+                 const int32_t* on) {
+  const int32_t n = static_cast<int32_t>(g.n[a]);
+  const int64_t inner = g.st[a];
+  const SplitTab& t = split_table(M, n);
+  AxisGemm d{};
+  d.n = n;
+  d.count = g.count;
+  d.lines = inner == 1 ? g.count / n : inner;
+  d.slabs = inner == 1 ? 1 : g.count / (static_cast<int64_t>(n) * inner);
+  d.groups = (d.lines + kAxLines - 1) / kAxLines;
+  d.oblocks = (n + kAxOut - 1) / kAxOut;
+  const dim3 grid(static_cast<unsigned>(d.slabs * d.groups), static_cast<unsigned>(d.oblocks),
+                  static_cast<unsigned>(g.batch));
+  if (inner == 1) EDA_GO(k_axis_bf16x3<false>, grid, dim3(kThreads), 0, s, d, t.hi, t.lo, in, out, on);
+  else EDA_GO(k_axis_bf16x3<true>, grid, dim3(kThreads), 0, s, d, t.hi, t.lo, in, out, on);
+  EDA_LAUNCH("k_axis_bf16x3");
+}
+
 // The DCT contraction mode of the calling thread: 0 the exact float32 forms (the LDS FFT for
 // power-of-two axes, the float32 MFMA GEMM for the others), 1 every axis as a split-BF16 GEMM.
 thread_local int t_dct_mode = 0;
@@ -406,6 +622,12 @@ void gemm(hipStream_t s, const GemmDims& d, const T* A, const T* B, T* C, const 
 // (M row-major (out index, in index), n x n).
 template <class T>
 void axis_gemm(hipStream_t s, const Geom& g, int a, const T* M, const T* in, T* out, const int32_t* on) {
+  if constexpr (std::is_same_v<T, float>) {
+    if (t_dct_mode == 1 && axis_bf16_fits(g, a)) {
+      axis_bf16x3(s, g, a, M, in, out, on);
+      return;
+    }
+  }
   const int64_t n = g.n[a];
   const int64_t inner = g.st[a];
   const int64_t outer = g.count / (n * inner);
