@@ -751,6 +751,70 @@ struct Lerp {
 #pragma unroll
     for (int c = 0; c < (1 << D); ++c) unsafeAtomicAdd(gf + off(c), v * weight(c));
   }
+  // The scatter with the corners of adjacent lanes merged: where lane l - 1's upper corners along
+  // the last axis are lane l's lower ones (the same samples of the same field `tag`: neighbouring
+  // voxels whose points moved alike), lane l adds lane l - 1's share to its own lower corners and
+  // lane l - 1 leaves its upper ones out, so a run of such lanes makes half the atomics. ok: this
+  // lane scatters. Every lane of the wave calls it together (the DPP exchanges read the neighbours'
+  // registers); lanes with ok = false neither give nor take.
+  __device__ __forceinline__ void scatter_merged(T* __restrict__ gf, T v, bool ok, int32_t tag) const {
+    constexpr int DL = D - 1;  // the last axis: corner bit 0
+    const int32_t p_tag = dpp_i32<kDppWaveShr1>(-1, ok ? tag : -1);
+    const int32_t p_hi = dpp_i32<kDppWaveShr1>(-1, off1[DL]);
+    bool same = ok && p_tag == tag && p_hi == off0[DL];
+    T pw0[D], pw1[D];
+#pragma unroll
+    for (int a = 0; a < DL; ++a) {
+      const int32_t q0 = dpp_i32<kDppWaveShr1>(-1, off0[a]);
+      const int32_t q1 = dpp_i32<kDppWaveShr1>(-1, off1[a]);
+      same = same && q0 == off0[a] && q1 == off1[a];
+      pw0[a] = dpp_any<kDppWaveShr1>(T(0), w0[a]);
+      pw1[a] = dpp_any<kDppWaveShr1>(T(0), w1[a]);
+    }
+    const T pu = dpp_any<kDppWaveShr1>(T(0), v * w1[DL]);
+    const bool gave = dpp_i32<kDppWaveShl1>(0, same ? 1 : 0) != 0;
+    if (!ok) return;
+#pragma unroll
+    for (int c = 0; c < (1 << D); ++c) {
+      const bool upper = (c & 1) != 0;
+      if (upper && gave) continue;
+      T val = v * weight(c);
+      if (!upper && same) {
+        T wr = pu;
+#pragma unroll
+        for (int a = 0; a < DL; ++a) wr = wr * (((c >> (D - 1 - a)) & 1) ? pw1[a] : pw0[a]);
+        val = val + wr;
+      }
+      unsafeAtomicAdd(gf + off(c), val);
+    }
+  }
+  // the corner values (one read each), the interpolant and its gradient from them
+  __device__ __forceinline__ void corners(const T* __restrict__ f, T* cv) const {
+#pragma unroll
+    for (int c = 0; c < (1 << D); ++c) cv[c] = f[off(c)];
+  }
+  __device__ __forceinline__ T at_v(const T* cv) const {
+    T acc = T(0);
+#pragma unroll
+    for (int c = 0; c < (1 << D); ++c) acc = acc + cv[c] * weight(c);
+    return acc;
+  }
+  __device__ __forceinline__ void grad_v(const T* cv, T* dy) const {
+#pragma unroll
+    for (int a = 0; a < D; ++a) dy[a] = T(0);
+#pragma unroll
+    for (int c = 0; c < (1 << D); ++c) {
+      const T v = cv[c];
+#pragma unroll
+      for (int a = 0; a < D; ++a) {
+        T w = ((c >> (D - 1 - a)) & 1) ? v : -v;
+#pragma unroll
+        for (int bb = 0; bb < D; ++bb)
+          if (bb != a) w = w * (((c >> (D - 1 - bb)) & 1) ? w1[bb] : w0[bb]);
+        dy[a] = dy[a] + w;
+      }
+    }
+  }
   // d/dy_a of the interpolant
   __device__ __forceinline__ void grad(const T* __restrict__ f, T* dy) const {
 #pragma unroll
@@ -798,18 +862,22 @@ __global__ void __launch_bounds__(kThreads) k_gather_vjp(Geom g, const ElemDev* 
     st[a] = static_cast<int32_t>(g.st[a]);
   }
   const T sgn[4] = {T(0.5), T(-0.5), T(1), T(-1)};
-  for (int64_t i64 = blockIdx.x * static_cast<int64_t>(kThreads) + threadIdx.x; i64 < count;
-       i64 += static_cast<int64_t>(gridDim.x) * kThreads) {
-    const int32_t i = static_cast<int32_t>(i64);
+  const int lane = threadIdx.x & 63;
+  // every lane of a wave runs every step of the loop (past the last voxel with valid = false), so
+  // that the merged scatters' DPP exchanges read live neighbours
+  for (int64_t w0 = blockIdx.x * static_cast<int64_t>(kThreads) + (threadIdx.x & ~63); w0 < count;
+       w0 += static_cast<int64_t>(gridDim.x) * kThreads) {
+    const bool valid = w0 + lane < count;
+    const int32_t i = static_cast<int32_t>(valid ? w0 + lane : w0);
     const int64_t at = base + i;
     const int l = lo[at];
     const T tt = t[at];
     const T wl = T(1) - tt;
-    const bool use[2] = {wl > T(0), tt > T(0)};
+    const bool use[2] = {valid && wl > T(0), valid && tt > T(0)};
     const T wgt[2] = {wl, tt};
     const T gv0 = g0[at];
     const T gv1 = OP == 2 ? g1[at] : T(0);
-    Lerp<T, D> ip[4];
+    Lerp<T, D> ip[4] = {};  // defined in every lane (the merged scatters exchange them)
     const bool any_pts = D >= 2 && ((use[0] && ((pts >> l) & 1u)) || (use[1] && ((pts >> (l + 1)) & 1u)));
     if (any_pts) {
       int32_t x[D];
@@ -849,36 +917,39 @@ __global__ void __launch_bounds__(kThreads) k_gather_vjp(Geom g, const ElemDev* 
     for (int a = 0; a < D; ++a) god[a] = T(0);
 #pragma unroll
     for (int side = 0; side < 2; ++side) {
-      if (!use[side]) continue;
+      const bool us = use[side];
       const int j = l + side;
-      const bool p = D >= 2 && ((pts >> j) & 1u);
+      const bool p = us && D >= 2 && ((pts >> j) & 1u);
       const T* C0 = r0 + j * F + base;
       T* G0 = gr0 + j * F + base;
       const T wj = wgt[side];
       if constexpr (OP == 3) {
-        T acc = C0[i];
-        if (p) {
+        T acc = T(0);
+        T cv[4][1 << D];  // the points' corner values, read once for the value and the slope
+        if (us) {
+          acc = C0[i];
+          if (p) {
 #pragma unroll
-          for (int qq = 0; qq < 4; ++qq) acc = acc + ip[qq].at(C0);
-          acc = acc / T(5);
+            for (int qq = 0; qq < 4; ++qq) {
+              ip[qq].corners(C0, cv[qq]);
+              acc = acc + ip[qq].at_v(cv[qq]);
+            }
+            acc = acc / T(5);
+          }
         }
         accv[side][0] = acc;
         const T gs = wj * gv0;
-        if (p) {
-          const T g5 = gs / T(5);
-          unsafeAtomicAdd(G0 + i, g5);
+        const T g5 = gs / T(5);
+        if (us) unsafeAtomicAdd(G0 + i, p ? g5 : gs);
 #pragma unroll
-          for (int qq = 0; qq < 4; ++qq) {
-            ip[qq].scatter(G0, g5);
-            if (go != nullptr) {
-              T dy[D];
-              ip[qq].grad(C0, dy);
+        for (int qq = 0; qq < 4; ++qq) {
+          ip[qq].scatter_merged(G0, g5, p, j);
+          if (p && go != nullptr) {
+            T dy[D];
+            ip[qq].grad_v(cv[qq], dy);
 #pragma unroll
-              for (int a = 0; a < D; ++a) god[a] = god[a] + g5 * sgn[qq] * dy[a];
-            }
+            for (int a = 0; a < D; ++a) god[a] = god[a] + g5 * sgn[qq] * dy[a];
           }
-        } else {
-          unsafeAtomicAdd(G0 + i, gs);
         }
       } else {
 #pragma unroll
@@ -886,31 +957,41 @@ __global__ void __launch_bounds__(kThreads) k_gather_vjp(Geom g, const ElemDev* 
           const bool mn = part == 1;
           const T* C = mn ? r1 + j * F + base : C0;
           T* G = mn ? gr1 + j * F + base : G0;
-          T best = C[i];
+          T best = T(0);
           int win = -1;
-          if (p) {
+          T wcv[1 << D];  // the winning point's corner values (its slope)
+          if (us) {
+            best = C[i];
+            if (p) {
 #pragma unroll
-            for (int qq = 0; qq < 4; ++qq) {
-              const T v = ip[qq].at(C);
-              const bool keep = mn ? (best < v || (best == v && signbit(best))) : (best > v || (best == v && !signbit(best)));
-              if (!keep) {
-                best = v;
-                win = qq;
+              for (int qq = 0; qq < 4; ++qq) {
+                T cv[1 << D];
+                ip[qq].corners(C, cv);
+                const T v = ip[qq].at_v(cv);
+                const bool keep = mn ? (best < v || (best == v && signbit(best))) : (best > v || (best == v && !signbit(best)));
+                if (!keep) {
+                  best = v;
+                  win = qq;
+#pragma unroll
+                  for (int c = 0; c < (1 << D); ++c) wcv[c] = cv[c];
+                }
               }
             }
           }
           accv[side][part] = best;
           const T gs = wj * (mn ? gv1 : gv0);
-          if (win < 0) {
-            unsafeAtomicAdd(G + i, gs);
-          } else {
-            ip[win].scatter(G, gs);
-            if (go != nullptr) {
-              T dy[D];
-              ip[win].grad(C, dy);
+          if (us && win < 0) unsafeAtomicAdd(G + i, gs);
+          // the winning point (selected without indexing the register array at run time)
+          Lerp<T, D> wp = ip[0];
+          if (win == 1) wp = ip[1];
+          if (win == 2) wp = ip[2];
+          if (win == 3) wp = ip[3];
+          wp.scatter_merged(G, gs, us && win >= 0, 2 * j + part);
+          if (us && win >= 0 && go != nullptr) {
+            T dy[D];
+            wp.grad_v(wcv, dy);
 #pragma unroll
-              for (int a = 0; a < D; ++a) god[a] = god[a] + gs * sgn[win] * dy[a];
-            }
+            for (int a = 0; a < D; ++a) god[a] = god[a] + gs * sgn[win] * dy[a];
           }
         }
       }
