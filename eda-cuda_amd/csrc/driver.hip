@@ -21,6 +21,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <map>
@@ -124,6 +125,99 @@ void timer_read(double* ms, int64_t* launches, double* bytes) {
   *bytes = by;
   t.spans.clear();
 }
+// ---- the autotuner ----
+namespace {
+struct TuneEntry {
+  int choice = 0;
+  std::vector<double> us;
+};
+struct TuneState {
+  std::mutex mu;
+  std::map<std::pair<int, std::string>, TuneEntry> table;
+};
+TuneState& tune_state() {
+  static TuneState* t = new TuneState();  // outlives the static destructors of the process's end
+  return *t;
+}
+bool autotune_on() {
+  static const bool on = [] {
+    const char* v = std::getenv("EDA_AUTOTUNE");
+    return v == nullptr || std::strcmp(v, "0") != 0;
+  }();
+  return on;
+}
+}  // namespace
+
+int autotune(const std::string& key, int ncand, int fallback, hipStream_t s, const std::function<void(int)>& launch) {
+  if (fallback < 0 || fallback >= std::max(ncand, 1)) throw std::logic_error("eda: autotuner fallback outside its list");
+  if (ncand <= 1 || !autotune_on()) {
+    launch(fallback);
+    return fallback;
+  }
+  int dev = 0;
+  EDA_HIP(hipGetDevice(&dev));
+  TuneState& t = tune_state();
+  // the candidate count is part of the key: one shape key can have lists of different lengths
+  // (the window bounds the piece counts), and a choice indexes its own list only
+  const auto k = std::make_pair(dev, key + ":of" + std::to_string(ncand));
+  {
+    std::lock_guard<std::mutex> lock(t.mu);
+    auto it = t.table.find(k);
+    if (it != t.table.end()) {
+      const int c = it->second.choice;
+      if (c < 0 || c >= ncand) throw std::logic_error("eda: autotuner choice outside its candidate list");
+      launch(c);
+      return c;
+    }
+  }
+  // two rounds over the candidates, each launch between two events; the minimum per candidate
+  constexpr int kRounds = 2;
+  std::vector<hipEvent_t> ev(static_cast<size_t>(2 * kRounds * ncand), nullptr);
+  for (auto& e : ev) EDA_HIP(hipEventCreate(&e));
+  for (int r = 0; r < kRounds; ++r)
+    for (int c = 0; c < ncand; ++c) {
+      flush_uploads();
+      EDA_HIP(hipEventRecord(ev[2 * (r * ncand + c)], s));
+      launch(c);
+      EDA_HIP(hipEventRecord(ev[2 * (r * ncand + c) + 1], s));
+    }
+  EDA_HIP(hipEventSynchronize(ev.back()));
+  TuneEntry entry;
+  entry.us.assign(static_cast<size_t>(ncand), 1e30);
+  for (int r = 0; r < kRounds; ++r)
+    for (int c = 0; c < ncand; ++c) {
+      float ms = 0.0f;
+      EDA_HIP(hipEventElapsedTime(&ms, ev[2 * (r * ncand + c)], ev[2 * (r * ncand + c) + 1]));
+      entry.us[c] = std::min(entry.us[c], 1e3 * static_cast<double>(ms));
+    }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  entry.choice = static_cast<int>(std::min_element(entry.us.begin(), entry.us.end()) - entry.us.begin());
+  const int c = entry.choice;
+  std::lock_guard<std::mutex> lock(t.mu);
+  t.table.emplace(k, std::move(entry));
+  return c;
+}
+
+std::string autotune_report() {
+  TuneState& t = tune_state();
+  std::lock_guard<std::mutex> lock(t.mu);
+  std::string js = "{";
+  bool first = true;
+  for (const auto& kv : t.table) {
+    char head[64];
+    std::snprintf(head, sizeof head, "%s\"%d:", first ? "" : ", ", kv.first.first);
+    js += head + kv.first.second + "\": {\"choice\": " + std::to_string(kv.second.choice) + ", \"us\": [";
+    for (size_t i = 0; i < kv.second.us.size(); ++i) {
+      char num[32];
+      std::snprintf(num, sizeof num, "%s%.2f", i ? ", " : "", kv.second.us[i]);
+      js += num;
+    }
+    js += "]}";
+    first = false;
+  }
+  return js + "}";
+}
+
 // the per-kernel totals summed by the timer_read calls since the last report, as JSON; clears them
 std::string timer_report() {
   static const char* const names[kTagCount] = {"k_vhgw_rows", "k_vhgw", "k_box_short", "k_box_long",
@@ -2543,6 +2637,14 @@ int eda_tape_mode(const eda_result* r, int64_t element, int32_t mode, int32_t* i
 int eda_timer_report(char* buf, size_t len, char* err, size_t errlen) {
   return guarded(err, errlen, [&] {
     const std::string js = timer_report();
+    if (buf == nullptr || len <= js.size()) throw std::invalid_argument("eda: the report buffer is too small");
+    std::memcpy(buf, js.c_str(), js.size() + 1);
+  });
+}
+
+int eda_autotune_report(char* buf, size_t len, char* err, size_t errlen) {
+  return guarded(err, errlen, [&] {
+    const std::string js = autotune_report();
     if (buf == nullptr || len <= js.size()) throw std::invalid_argument("eda: the report buffer is too small");
     std::memcpy(buf, js.c_str(), js.size() + 1);
   });

@@ -7,6 +7,7 @@
 #include "common.hpp"
 
 #include <functional>
+#include <string>
 
 #include <cstdint>
 
@@ -48,6 +49,14 @@ enum TimerTag : int {
   kTagStream,       // k_vhgw_stream (strided axes, windows up to 63 samples)
   kTagCount
 };
+// ---- the launch-shape autotuner (driver.hip) -----------------------------------------------------
+// Per device and key (a kernel family and the shape facts its launch shape depends on), the first
+// launch times every candidate shape twice on the launch's stream (HIP events; every candidate
+// writes the same bits, so the output stands whichever ran last) and keeps the fastest; later
+// launches take it. With EDA_AUTOTUNE=0 in the environment, candidate `fallback` always.
+int autotune(const std::string& key, int ncand, int fallback, hipStream_t s, const std::function<void(int)>& launch);
+// The choices so far, as JSON {key: {"choice": c, "us": [min time per candidate]}}.
+std::string autotune_report();
 void timer_set(int mask);
 bool timer_on();                                                            // family 1
 bool timer_gemm_on();                                                       // family 2

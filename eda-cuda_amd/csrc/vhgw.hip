@@ -39,6 +39,8 @@
 #include <mutex>
 #include <atomic>
 #include <limits>
+#include <string>
+#include <vector>
 
 #ifndef EDA_VHGW_SUMU
 #define EDA_VHGW_SUMU 4  // 16-byte reads in flight per thread while a box-sum workgroup stages its lines
@@ -1423,19 +1425,47 @@ int box_reads(const VPass& a, int nthr) {
 template <class T, bool PAD>
 void launch_box(int pf, hipStream_t s, const VPass& a, const VPtrs<T>& p, int32_t rungs, int W, size_t lds) {
   const int64_t items = a.g.batch * a.groups * rungs;
-  const int64_t per_cu = std::max<int64_t>(1, std::min<int64_t>(EDA_BOX_PERCU, static_cast<int64_t>(160 * 1024 / lds)));
-  int64_t wgs = std::min<int64_t>((items + 7) / 8 * 8, device_cus() * per_cu);
-  wgs = std::max<int64_t>(8, wgs / 8 * 8);
-  const auto go = [&](auto k) {
-    if (lds > 65536)
-      EDA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  static_cast<int>(lds)));
-    VPass v = a;
-    v.xcd = xcd_chunk(wgs);
-    EDA_GO(k, dim3(static_cast<unsigned>(wgs)), dim3(64 * W), lds, s, v, p, rungs);
+  // the candidates: 2 to 4 resident workgroups per CU (as the LDS allows); the rule's choice 4
+  const int64_t cap = std::max<int64_t>(1, static_cast<int64_t>(160 * 1024 / lds));
+  std::vector<int64_t> cands;
+  for (int64_t pc = 2; pc <= std::min<int64_t>(4, cap); ++pc) cands.push_back(pc);
+  if (cands.empty()) cands.push_back(cap);
+  const int64_t rule = std::max<int64_t>(1, std::min<int64_t>(EDA_BOX_PERCU, cap));
+  int fallback = static_cast<int>(cands.size()) - 1;
+  for (size_t c = 0; c < cands.size(); ++c)
+    if (cands[c] == rule) fallback = static_cast<int>(c);
+  const auto launch = [&](int c) {
+    int64_t wgs = std::min<int64_t>((items + 7) / 8 * 8, device_cus() * cands[static_cast<size_t>(c)]);
+    wgs = std::max<int64_t>(8, wgs / 8 * 8);
+    const auto go = [&](auto k) {
+      if (lds > 65536)
+        EDA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    static_cast<int>(lds)));
+      VPass v = a;
+      v.xcd = xcd_chunk(wgs);
+      EDA_GO(k, dim3(static_cast<unsigned>(wgs)), dim3(64 * W), lds, s, v, p, rungs);
+    };
+    if (pf == 4) go(k_box<T, 4, PAD>);
+    else go(k_box<T, 8, PAD>);
   };
-  if (pf == 4) go(k_box<T, 4, PAD>);
-  else go(k_box<T, 8, PAD>);
+  // a pass in place is not tuned (a second run would filter its own output)
+  const size_t span = static_cast<size_t>(a.g.batch * a.g.count) * sizeof(T);
+  const auto range = [&](const T* q, int64_t stride) {
+    const char* b0 = reinterpret_cast<const char*>(q);
+    return std::make_pair(b0, b0 + span + static_cast<size_t>(rungs - 1) * static_cast<size_t>(stride) * sizeof(T));
+  };
+  const auto ri = range(p.in[0], p.in_stride);
+  const auto ro = range(p.out[0], p.out_stride);
+  if (ri.first < ro.second && ro.first < ri.second) {
+    launch(fallback);
+    return;
+  }
+  int lb = 0;
+  while ((int64_t{1} << lb) < items) ++lb;
+  const std::string key = std::string("k_box:") + (sizeof(T) == 4 ? "f32" : "f64") + (PAD ? ":long" : ":short") +
+                          ":n" + std::to_string(a.n) + ":inner" + std::to_string(a.inner) + ":P" + std::to_string(a.P) +
+                          ":pf" + std::to_string(pf) + ":items2^" + std::to_string(lb);
+  autotune(key, static_cast<int>(cands.size()), fallback, s, launch);
 }
 
 // The streaming pass serves max / min along a contiguous axis with n <= 2048.
@@ -1780,24 +1810,46 @@ void launch_stream(hipStream_t s, const Geom& g, int axis, int nf, const int* op
   const int64_t lines = static_cast<int64_t>(rungs) * g.batch * a.outer * a.cgroups;
   const int64_t want = static_cast<int64_t>(device_cus()) * 4 * 8;
   const int32_t Lm = 2 * std::min((max_w - 1) / 2, a.n - 1) + 1;
-  int32_t pieces = 1;
-  while (lines * pieces < want && a.n / (2 * pieces) >= std::max(64, 4 * Lm)) pieces *= 2;
-  a.pieces = pieces;
-  a.R = (a.n + pieces - 1) / pieces;
-  a.items = static_cast<int64_t>(g.batch) * a.outer * a.cgroups * pieces;
-  if ((a.items + 3) / 4 > 0x7fffffff) throw std::invalid_argument("eda: too many waves for one pass");
-  const dim3 grid(static_cast<unsigned>((a.items + 3) / 4), static_cast<unsigned>(rungs));
-  const auto go = [&](auto big_tag) {
-    constexpr bool kBig = decltype(big_tag)::value;
-    switch (a.mode) {
-      case 0: EDA_GO((k_vhgw_stream<T, 0, kBig>), grid, dim3(256), 0, s, a, p); break;
-      case 1: EDA_GO((k_vhgw_stream<T, 1, kBig>), grid, dim3(256), 0, s, a, p); break;
-      case 2: EDA_GO((k_vhgw_stream<T, 2, kBig>), grid, dim3(256), 0, s, a, p); break;
-      default: EDA_GO((k_vhgw_stream<T, 3, kBig>), grid, dim3(256), 0, s, a, p); break;
-    }
+  // the candidate piece counts (powers of two, each piece at least 32 samples and 2 windows long),
+  // and the rule's choice: enough waves for about 8 per SIMD, pieces of at least 64 samples and 4
+  // windows
+  std::vector<int32_t> cands;
+  for (int32_t pc = 1; pc <= 32 && a.n / pc >= std::max(32, 2 * Lm); pc *= 2) cands.push_back(pc);
+  int32_t rule = 1;
+  while (lines * rule < want && a.n / (2 * rule) >= std::max(64, 4 * Lm)) rule *= 2;
+  int fallback = -1;
+  for (size_t c = 0; c < cands.size(); ++c)
+    if (cands[c] == rule) fallback = static_cast<int>(c);
+  if (fallback < 0) {  // long lines of few columns: the rule splits them further than the list goes
+    cands.push_back(rule);
+    fallback = static_cast<int>(cands.size()) - 1;
+  }
+  const auto launch = [&](int c) {
+    SPass v = a;
+    v.pieces = cands[static_cast<size_t>(c)];
+    v.R = (v.n + v.pieces - 1) / v.pieces;
+    v.items = static_cast<int64_t>(g.batch) * v.outer * v.cgroups * v.pieces;
+    if ((v.items + 3) / 4 > 0x7fffffff) throw std::invalid_argument("eda: too many waves for one pass");
+    const dim3 grid(static_cast<unsigned>((v.items + 3) / 4), static_cast<unsigned>(rungs));
+    const auto go = [&](auto big_tag) {
+      constexpr bool kBig = decltype(big_tag)::value;
+      switch (v.mode) {
+        case 0: EDA_GO((k_vhgw_stream<T, 0, kBig>), grid, dim3(256), 0, s, v, p); break;
+        case 1: EDA_GO((k_vhgw_stream<T, 1, kBig>), grid, dim3(256), 0, s, v, p); break;
+        case 2: EDA_GO((k_vhgw_stream<T, 2, kBig>), grid, dim3(256), 0, s, v, p); break;
+        default: EDA_GO((k_vhgw_stream<T, 3, kBig>), grid, dim3(256), 0, s, v, p); break;
+      }
+    };
+    if (big) go(std::true_type{});
+    else go(std::false_type{});
   };
-  if (big) go(std::true_type{});
-  else go(std::false_type{});
+  // the key: the kernel, the line and the parallelism (the lines of the launch, to a power of two)
+  int lb = 0;
+  while ((int64_t{1} << lb) < lines) ++lb;
+  const std::string key = std::string("k_vhgw_stream:") + (sizeof(T) == 4 ? "f32" : "f64") + ":mode" +
+                          std::to_string(a.mode) + (big ? ":big" : ":short") + ":n" + std::to_string(a.n) +
+                          ":inner" + std::to_string(a.inner) + ":lines2^" + std::to_string(lb);
+  autotune(key, static_cast<int>(cands.size()), fallback, s, launch);
 }
 
 }  // namespace

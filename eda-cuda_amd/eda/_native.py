@@ -102,6 +102,7 @@ SIGNATURES = {
     "eda_timer_enable": ([_i32], _c.c_int),
     "eda_timer_read": ([_f64p, _i64p, _f64p, *_err], _c.c_int),
     "eda_timer_report": ([_c.c_char_p, _c.c_size_t, *_err], _c.c_int),
+    "eda_autotune_report": ([_c.c_char_p, _c.c_size_t, *_err], _c.c_int),
     "eda_prim_cube": ([_i32, _i32, _i64, _i64p, _i32, _i32p, _vp, _vp, _vp, *_err], _c.c_int),
     "eda_prim_stats": ([_i32, _i32, _i64, _i64p, _vp, _f64p, _f64p, _u64p, *_err], _c.c_int),
     "eda_prim_select": ([_i32, _i32, _i64, _i64p, _vp, _vp, _i32, _i32, _f64, _f64p, _i64p, *_err], _c.c_int),

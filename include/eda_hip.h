@@ -190,6 +190,11 @@ int eda_pointer_device(const void* ptr, int32_t* device);    /* -1 for host memo
 int eda_timer_enable(int32_t on);
 int eda_timer_read(double* ms, int64_t* launches, double* bytes, char* err, size_t errlen);
 int eda_timer_report(char* buf, size_t len, char* err, size_t errlen);
+/* The launch shapes the autotuner chose so far (per device and kernel shape key: the candidate
+ * index and each candidate's best time in us), as a JSON object; the first launch of each key
+ * times its candidates (EDA_AUTOTUNE=0 in the environment turns that off). No counterpart in the
+ * reference (it sizes its launches by fixed rules). */
+int eda_autotune_report(char* buf, size_t len, char* err, size_t errlen);
 
 /* ---- Primitive entry points (device pointers; each call synchronizes before it returns). They
  * expose the method's stages one by one, so that each can be checked against its closed form and
