@@ -10,8 +10,9 @@ stdout (progress goes to stderr):
 
   value      Msamples/s = 64 * 512 * 512 / (seconds per step over the K timed steps)
   roofline   the dominant kernel family, the cube filter's vHGW pass (k_vhgw_rows, k_vhgw_stream,
-             k_vhgw and the two k_box launches): algorithmic bytes per launch (its input field(s) read once,
-             output field(s) written once, over the running elements) / its average duration, both
+             k_vhgw, the two k_box launches and the fused 2-D cube k_cube2): algorithmic bytes per
+             launch (its input field(s) read once, output field(s) written once, over the running
+             elements) / its average duration, both
              measured live with HIP events on the library's stream over two extra, untimed steps
              (the ledger), against 8 TB/s; `kernels` the same figures per kernel (also the rung
              gather, the extrema tiles and the DCT passes); `traffic` the HBM bytes per launch from
@@ -87,13 +88,14 @@ def pmc_summary(fetch_dir, write_dir, out_json):
     """HBM traffic per vHGW-pass launch from two rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE,
     KiB per dispatch) of `bench.py --steps 1 --warmup 0 --no-cpu --copy-probe`, each scaled by what
     the counter reads for the probe's copy kernel (torch.neg over 1 GiB: 1 GiB read, 1 GiB written).
-    The KiB of every k_vhgw_rows, k_vhgw_stream, k_vhgw and k_box dispatch are summed and divided by their number."""
+    The KiB of every k_vhgw_rows, k_vhgw_stream, k_vhgw, k_box and k_cube2 dispatch are summed and divided by their
+    number."""
     def averages(directory, counter):
         total, launches, cp = 0.0, 0, []
         for name, cname, value in _counter_rows(directory):
             if cname != counter:
                 continue
-            if "k_vhgw" in name or "k_box" in name:
+            if "k_vhgw" in name or "k_box" in name or "k_cube2" in name:
                 total += value
                 launches += 1
             elif "neg" in name:
@@ -108,7 +110,7 @@ def pmc_summary(fetch_dir, write_dir, out_json):
     w_scale = COPY_PROBE_BYTES / (w_cp * 1024.0)
     traffic = f_vh * 1024.0 * f_scale + w_vh * 1024.0 * w_scale
     res = {
-        "kernel": "vHGW pass (k_vhgw_rows, k_vhgw, k_vhgw_stream, k_box)",
+        "kernel": "vHGW pass (k_vhgw_rows, k_vhgw, k_vhgw_stream, k_box, k_cube2)",
         "workload": "eda.decompose fwd, 64 x 512 x 512 float32 (one call)",
         "launches": n_f,
         "fetch_kib_per_launch": round(f_vh, 1),
@@ -481,7 +483,7 @@ def kernel_ledger(args, x, lib, torch, eda, _native, step_s):
     (HIP events on the library's stream around each launch; these steps are not the timed ones):
     per kernel its launches per step, average duration, algorithmic bytes per launch, GB/s and
     fraction of the HBM peak, and its share of an un-instrumented step. Also the same figures per
-    family: the vHGW pass (k_vhgw_rows + k_vhgw_stream + k_vhgw + both k_box launches)."""
+    family: the vHGW pass (k_vhgw_rows + k_vhgw_stream + k_vhgw + both k_box launches + k_cube2)."""
     err = _native.errbuf()
     buf = ctypes.create_string_buffer(4096)
     lib.eda_timer_enable(15)
@@ -517,7 +519,8 @@ def kernel_ledger(args, x, lib, torch, eda, _native, step_s):
             log(f"bench: {name:14s} {f['launches_per_step']:7.1f} launches/step {f['us_per_launch']:9.2f} us "
                 f"{f['bytes_per_launch'] / 1e6:9.1f} MB {f['GBs']:8.1f} GB/s ({f['frac']:.3f} of peak), "
                 f"{100 * f['share_of_step']:.1f} % of a step")
-    vh = [raw[k] for k in ("k_vhgw_rows", "k_vhgw", "k_vhgw_stream", "k_box_short", "k_box_long") if k in raw]
+    vh = [raw[k] for k in ("k_vhgw_rows", "k_vhgw", "k_vhgw_stream", "k_box_short", "k_box_long", "k_cube2")
+          if k in raw]
     fam = {"vhgw": figures(sum(v["ms"] for v in vh), sum(v["launches"] for v in vh), sum(v["bytes"] for v in vh))}
     return kernels, fam
 
@@ -626,7 +629,7 @@ def main() -> int:
     vh = fam["vhgw"]
     roofline = {
         "bound": "hbm",
-        "kernel": "vHGW pass (k_vhgw_rows, k_vhgw, k_vhgw_stream, k_box)",
+        "kernel": "vHGW pass (k_vhgw_rows, k_vhgw, k_vhgw_stream, k_box, k_cube2)",
         "achieved": vh["GBs"],
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",

@@ -221,7 +221,8 @@ std::string autotune_report() {
 // the per-kernel totals summed by the timer_read calls since the last report, as JSON; clears them
 std::string timer_report() {
   static const char* const names[kTagCount] = {"k_vhgw_rows", "k_vhgw", "k_box_short", "k_box_long",
-                                                "k_dct_fft", "k_gather_all", "k_flags_tile", "k_vhgw_stream"};
+                                                "k_dct_fft", "k_gather_all", "k_flags_tile", "k_vhgw_stream",
+                                                "k_cube2"};
   double ms = 0.0, by = 0.0;
   int64_t n = 0;
   timer_read(&ms, &n, &by);

@@ -35,7 +35,7 @@ int abi_guard(char* err, size_t errlen, const std::function<void()>& f);
 // stream and its algorithmic bytes are counted (each distinct input field read once, each output
 // written once, over the elements that run; the count of those comes from the host mirror of the
 // per-element arrays, with no device read-back). Families (bits of the mask): 1 the cube filter's
-// passes (k_vhgw_rows, k_vhgw, k_vhgw_stream, k_box), 2 the DCT's axis passes (k_dct_fft, the dense contraction),
+// passes (k_vhgw_rows, k_vhgw, k_vhgw_stream, k_box, k_cube2), 2 the DCT's axis passes (k_dct_fft, the dense contraction),
 // 4 the rung gathers (k_gather_all), 8 the significant-extrema tiles (k_flags_tile).
 enum TimerFamily : int { kTimeVhgw = 1, kTimeDct = 2, kTimeGather = 4, kTimeFlags = 8 };
 enum TimerTag : int {
@@ -47,6 +47,7 @@ enum TimerTag : int {
   kTagGather,       // k_gather_all
   kTagFlags,        // k_flags_tile
   kTagStream,       // k_vhgw_stream (strided axes, windows up to 63 samples)
+  kTagCube2,        // k_cube2 (both passes of a 2-D cube, windows up to 31 samples)
   kTagCount
 };
 // ---- the launch-shape autotuner (driver.hip) -----------------------------------------------------

@@ -1118,6 +1118,7 @@ struct RPass {
   int32_t ops[2];
   int32_t vec;     // n % 4 == 0: 16-byte loads
   int32_t il;      // nf == 2 (max, min): both outputs of sample v side by side at out[0] + 2 v
+  int32_t rmin;    // elements whose radius is below rmin are skipped (the fused cube takes them)
 };
 
 template <class T, int KT>
@@ -1187,7 +1188,7 @@ __global__ void __launch_bounds__(256) k_vhgw_rows(RPass a, VPtrs<T> ptr) {
       }
       const int64_t c = sb + lane * nw;
       sw = c < a.items ? window(c) : 0;
-      mask = __ballot(sw >= 1);
+      mask = __ballot(sw >= 1 && min((sw - 1) / 2, n - 1) >= a.rmin);
     }
     const int l = __ffsll(static_cast<unsigned long long>(mask)) - 1;
     mask &= mask - 1;
@@ -1475,7 +1476,7 @@ bool rows_qualify(const Geom& g, int axis, bool sum) {
 }
 template <class T>
 void launch_rows(hipStream_t s, const Geom& g, int axis, int nf, bool same_input, const int* ops, const VPtrs<T>& p,
-                 int32_t rungs, bool il) {
+                 int32_t rungs, bool il, int32_t rmin) {
   const int32_t n = g.n[axis];
   RPass r{};
   r.g = g;
@@ -1487,6 +1488,7 @@ void launch_rows(hipStream_t s, const Geom& g, int axis, int nf, bool same_input
   r.ops[0] = ops[0];
   r.ops[1] = nf > 1 ? ops[1] : ops[0];
   r.il = il && nf == 2 && r.ops[0] == kVMax && r.ops[1] == kVMin ? 1 : 0;
+  r.rmin = rmin;
   const auto al16 = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; };
   r.vec = n % 4 == 0 && al16(p.in[0]) && al16(p.in[1]) && (p.in_stride * sizeof(T)) % 16 == 0 ? 1 : 0;
   const int32_t per = (n + 63) / 64;
@@ -1533,6 +1535,7 @@ struct SPass {
   int32_t cgroups;  // 64-line groups per slab
   int32_t pieces;   // pieces per line
   int32_t R;        // outputs per piece
+  int32_t rmin;     // elements whose radius is below rmin are skipped (the fused cube takes them)
   int32_t mode;     // 0 max, 1 min, 2 max and min of one input (out0, out1), 3 box sum
   int32_t scale;    // multiply by fl_T(1 / prod of the clamped sides)
   int64_t items;    // waves per rung
@@ -1706,6 +1709,7 @@ __global__ void __launch_bounds__(256) k_vhgw_stream(SPass a, VPtrs<T> ptr) {
   const int32_t n = a.n;
   const int32_t half = (w - 1) / 2;
   const int32_t r = half < n - 1 ? half : n - 1;
+  if (r < a.rmin) return;  // uniform: the fused cube serves the element
   const int32_t L = 2 * r + 1;
   const int32_t o = static_cast<int32_t>((static_cast<int64_t>(n) * ((L + 1) / 2)) % L);
   const int32_t x_lo = piece * a.R;
@@ -1775,6 +1779,314 @@ __global__ void __launch_bounds__(256) k_vhgw_stream(SPass a, VPtrs<T> ptr) {
 #undef EDA_STREAM_CASE
 }
 
+#ifndef EDA_CUBE2_MAXMIN
+#define EDA_CUBE2_MAXMIN 0  // max + min cubes fused too (three staged regions: one workgroup per CU)
+#endif
+// ---- the fused 2-D cube (windows up to 31 samples) --------------------------------------------------
+// Both passes of a 2-D max, min, max + min or box cube in one kernel, for fields of n0 >= 128 rows
+// and 64 <= n1 <= 512 columns, when the cube does not run in place. A workgroup of ceil(n1 / 64)
+// waves owns one (rung, element, piece of R rows). Phase 1 walks the columns in the canonical
+// segment order as the streaming pass does (lane = column, rows read through a buffer resource with
+// scalar row offsets) and writes each finished row of the axis-0 fold into LDS instead of HBM. After
+// every batch of segments (at most 16 rows for windows up to 15 samples, L rows above) the workgroup
+// synchronises and phase 2 walks the staged rows along axis 1: one lane per (row, piece of 32
+// columns), rows fastest across the lanes (lanes of one step read one column of different rows:
+// distinct banks, the row stride being odd), the same canonical walk over the LDS row (reflected in
+// the row) writing the cube's outputs to HBM. The intermediate field never leaves the chip: the
+// cube reads its input once and writes its outputs once. Max / min stay order keys between the
+// passes; the box sum's folds and its final scaling are the two-pass path's, so every output is
+// bitwise the two-pass result.
+struct CPass {
+  Geom g;
+  int32_t n0, n1;  // rows (axis 0) and columns (axis 1, contiguous) of an element
+  int32_t mode;    // 0 max, 1 min, 2 max and min of one input, 3 box sum
+  int32_t pair;    // mode 2: max and min side by side at out[0] + 2 v (else out[0] and out[1])
+  int32_t scale;   // box: multiply by fl_T(1 / window area)
+  int32_t pieces;  // row pieces per element
+  int32_t R;       // rows per piece
+  int32_t rs;      // LDS slots per staged row (odd)
+  int32_t vec;     // 16-byte output stores: n1 a multiple of their samples, outputs 16-byte aligned
+};
+
+// The canonical walk of one line in batches of U segments (segments k1 - 1 .. k2, as the streaming
+// pass walks them): load(e) gives the sample at extended position e, emit(x, y0, y1) takes each
+// output x in [x_lo, x_hi) in order, after(end) runs once per batch with the end of the outputs
+// emitted so far. PREFETCH: the next batch's reads are issued before the batch is folded, so that
+// they are in flight while after() runs. ALL: emit takes every output of the segments walked (it
+// drops those outside [x_lo, x_hi) itself, without a branch).
+template <int MODE, int L, int U, bool PREFETCH = false, bool ALL = false, class S, class Load, class Emit,
+          class After>
+__device__ __forceinline__ void walk_line(int32_t o, int32_t x_lo, int32_t x_hi, Load load, Emit emit, After after) {
+  constexpr int r = (L - 1) / 2;
+  const int32_t k1 = static_cast<int32_t>(floor_div(static_cast<int64_t>(x_lo) + r - o, L));
+  const int32_t k2 = static_cast<int32_t>(floor_div(static_cast<int64_t>(x_hi) - 1 + r - o, L));
+  S h0[L], h1[MODE == 2 ? L : 1];
+  const auto fetch = [&](int32_t k, S (&v)[U * L]) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (k + u > k2) break;
+      const int32_t s = o + (k + u) * L;
+#pragma unroll
+      for (int j = 0; j < L; ++j) v[u * L + j] = load(s + j);
+    }
+  };
+  S v[U * L];
+  if constexpr (PREFETCH) fetch(k1 - 1, v);
+  for (int32_t k = k1 - 1; k <= k2; k += U) {
+    // PREFETCH: the next batch's reads go out before this batch is folded and after() runs
+    S nv[PREFETCH ? U * L : 1];
+    if constexpr (PREFETCH) {
+      if (k + U <= k2) fetch(k + U, nv);
+    } else {
+      fetch(k, v);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int32_t kk = k + u;
+      if (kk > k2) break;
+      const int32_t x0 = o + kk * L - r;
+      seg_fold<MODE, L>(v + u * L, h0, h1, kk >= k1, [&](int j, S y0, S y1) {
+        const int32_t x = x0 + j;
+        if constexpr (ALL) {
+          emit(x, y0, y1);  // every output; emit keeps those in [x_lo, x_hi)
+        } else {
+          if (x < x_lo || x >= x_hi) return;
+          emit(x, y0, y1);
+        }
+      });
+    }
+    const int32_t kl = k + U - 1 < k2 ? k + U - 1 : k2;
+    const int32_t end = o + (kl + 1) * L - r;
+    after(end < x_hi ? end : x_hi);
+    if constexpr (PREFETCH) {
+#pragma unroll
+      for (int j = 0; j < U * L; ++j) v[j] = nv[j];
+    }
+  }
+}
+
+// Rows per batch of the fused cube: about 16 (12 for max + min, which stages three regions), or L.
+template <int MODE, int L>
+constexpr int cube_u() {
+  return L >= (MODE == 2 ? 12 : 16) ? 1 : (MODE == 2 ? 12 : 16) / L;
+}
+template <int MODE, int L>
+constexpr int cube_cap() {
+  return cube_u<MODE, L>() * L;
+}
+
+// Reflected samples staged on each side of a row: the phase-2 walk reads [-(3L - 1) / 2, n1 + (3L - 1) / 2).
+__host__ __device__ constexpr int cube2_halo(int L) { return L <= 15 ? 24 : 48; }
+
+// VS consecutive values of T (VS * sizeof(T) = 4, 8 or 16 bytes) in one buffer store.
+template <class T, int VS>
+__device__ __forceinline__ void buf_st_vec(const T (&v)[VS], __amdgpu_buffer_rsrc_t rs, uint32_t voff) {
+  constexpr int kB = VS * static_cast<int>(sizeof(T));
+  if constexpr (kB == 16) {
+    using V4 = decltype(__builtin_amdgcn_raw_buffer_load_b128(rs, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V4, v), rs, static_cast<int>(voff), 0, 0);
+  } else if constexpr (kB == 8) {
+    using V2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(rs, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), rs, static_cast<int>(voff), 0, 0);
+  } else {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int32_t, v), rs, static_cast<int>(voff), 0, 0);
+  }
+}
+
+// A workgroup barrier that orders LDS only: global reads in flight stay in flight across it (a
+// plain __syncthreads waits for them).
+__device__ __forceinline__ void lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+template <class T, int MODE, int L>
+__device__ __forceinline__ void cube2_run(const CPass& a, const VPtrs<T>& ptr, int64_t b, int64_t rung, int32_t piece,
+                                          unsigned char* smem) {
+  using S = std::conditional_t<MODE == 3, T, okey_t<T>>;
+  constexpr int U = cube_u<MODE, L>();
+  constexpr int CAP = U * L;            // rows staged per batch at most
+  constexpr int PW = CAP <= 16 ? 16 : 32;  // columns per phase-2 piece
+  constexpr int HB = cube2_halo(L);        // reflected samples staged on each side of a row
+  const int32_t n0 = a.n0, n1 = a.n1, rs = a.rs;
+  // LDS: the staged rows B0 (B1: the min of max + min), sample e of a row at HB + e for e in
+  // [-HB, n1 + HB) (the reflection of the row: phase 2 reads no index twice), and the output rows O
+  // (output x at x, slot n1 a junk slot for the outputs a piece drops)
+  S* B0 = reinterpret_cast<S*>(smem);
+  S* B1 = B0 + CAP * rs;
+  S* O = MODE == 2 ? B1 + CAP * rs : B1;
+  const int tid = threadIdx.x;
+  const int nthr = blockDim.x;
+  const bool live = tid < n1;
+  const uint32_t voff = live ? static_cast<uint32_t>(tid) * sizeof(T) : 0u;
+  const uint32_t rowb = static_cast<uint32_t>(n1) * sizeof(T);
+  const int64_t off = b * a.g.count;
+  const int pf = a.pair ? 2 : 1;
+  const __amdgpu_buffer_rsrc_t src =
+      buf_rsrc(ptr.in[0] + rung * ptr.in_stride + off, static_cast<uint32_t>(a.g.count * sizeof(T)));
+  const uint32_t obytes = static_cast<uint32_t>(a.g.count * pf * sizeof(T));
+  const __amdgpu_buffer_rsrc_t d0 = buf_rsrc(ptr.out[0] + rung * ptr.out_stride + off * pf, obytes);
+  const __amdgpu_buffer_rsrc_t d1 = buf_rsrc(ptr.out[1] + rung * ptr.out_stride + off * pf, obytes);
+  T fac = T(1);
+  if (a.scale) fac = static_cast<T>(1.0 / (static_cast<double>(L) * static_cast<double>(L)));
+  const int32_t o0 = static_cast<int32_t>((static_cast<int64_t>(n0) * ((L + 1) / 2)) % L);
+  const int32_t o1 = static_cast<int32_t>((static_cast<int64_t>(n1) * ((L + 1) / 2)) % L);
+  const int32_t x_lo = piece * a.R;
+  const int32_t x_hi = min(n0, x_lo + a.R);
+  // phase 2's lane: row i2 of the batch (fastest), piece p2 of PW columns
+  const int32_t i2 = tid % CAP;
+  const int32_t c_lo = (tid / CAP) * PW;
+  const int32_t c_hi = min(n1, c_lo + PW);
+  // the waves holding the columns whose reflections the halo stages
+  const bool edge_wave = (tid & ~63) < HB || (tid | 63) >= n1 - HB;
+  // the write-out: VS consecutive samples per thread and store (16 bytes where the rows allow)
+  const auto write_out = [&](int32_t xr0, int32_t nr, auto vs_tag) {
+    constexpr int VS = decltype(vs_tag)::value;
+    const int32_t per = n1 / VS;
+    const float inv = 1.0f / static_cast<float>(per);
+    for (int32_t q = tid; q < nr * per; q += nthr) {
+      const int32_t i = div_small(q, per, inv);
+      const int32_t c = (q - i * per) * VS;
+      const S* ro = O + i * rs + c;
+      const uint32_t at = (static_cast<uint32_t>(xr0 + i) * static_cast<uint32_t>(n1) + static_cast<uint32_t>(c)) *
+                          static_cast<uint32_t>(pf) * sizeof(T);
+      if constexpr (MODE == 2) {
+        const S* rm = B0 + i * rs + c;  // the min rows
+        if (a.pair) {
+          if constexpr (2 * VS * sizeof(T) <= 16) {
+            T v[2 * VS];
+#pragma unroll
+            for (int k = 0; k < VS; ++k) {
+              v[2 * k] = okey_inv(ro[k]);
+              v[2 * k + 1] = okey_inv(rm[k]);
+            }
+            buf_st_vec<T, 2 * VS>(v, d0, at);
+          }
+        } else {
+          T v0[VS], v1[VS];
+#pragma unroll
+          for (int k = 0; k < VS; ++k) {
+            v0[k] = okey_inv(ro[k]);
+            v1[k] = okey_inv(rm[k]);
+          }
+          buf_st_vec<T, VS>(v0, d0, at);
+          buf_st_vec<T, VS>(v1, d1, at);
+        }
+      } else {
+        T v[VS];
+#pragma unroll
+        for (int k = 0; k < VS; ++k) {
+          if constexpr (MODE == 3) v[k] = a.scale ? ro[k] * fac : ro[k];
+          else v[k] = okey_inv(ro[k]);
+        }
+        buf_st_vec<T, VS>(v, d0, at);
+      }
+    }
+  };
+  int32_t flushed = x_lo;  // rows before it are written
+  const auto rows = [&](int32_t end) {
+    if (end <= flushed) return;  // uniform
+    lds_sync();
+    const int32_t nr = end - flushed;
+    const bool act = i2 < nr && c_lo < n1;
+    const auto none = [](int32_t) {};
+    if (act) {
+      S* orow = O + i2 * rs;
+      const S* row0 = B0 + i2 * rs + HB;
+      walk_line<MODE == 2 ? 0 : MODE, L, U, false, true, S>(
+          o1, c_lo, c_hi, [&](int32_t e) { return row0[e]; },
+          [&](int32_t x, S y0, S) { orow[x >= c_lo && x < c_hi ? x : n1] = y0; }, none);
+    }
+    if constexpr (MODE == 2) {
+      lds_sync();  // B0 is read; the min rows go over it
+      if (act) {
+        S* orow = B0 + i2 * rs;
+        const S* row1 = B1 + i2 * rs + HB;
+        walk_line<1, L, U, false, true, S>(
+            o1, c_lo, c_hi, [&](int32_t e) { return row1[e]; },
+            [&](int32_t x, S y0, S) { orow[x >= c_lo && x < c_hi ? x : n1] = y0; }, none);
+      }
+    }
+    lds_sync();
+    // 16-byte stores where the rows and the outputs allow them (a.vec, from the host)
+    constexpr int kVec = static_cast<int>(16 / sizeof(T));
+    if (a.vec) {
+      if (a.pair) write_out(flushed, nr, std::integral_constant<int, kVec / 2>{});
+      else write_out(flushed, nr, std::integral_constant<int, kVec>{});
+    } else {
+      write_out(flushed, nr, std::integral_constant<int, 1>{});
+    }
+    lds_sync();  // the batch's rows are out before the next batch overwrites them
+    flushed = end;
+  };
+  const auto row_of0 = [&](int32_t e) { return e < 0 ? -1 - e : (e >= n0 ? 2 * n0 - 1 - e : e); };
+  walk_line<MODE, L, U, true, false, S>(
+      o0, x_lo, x_hi,
+      [&](int32_t e) -> S {
+        const T v = buf_ld<T>(src, voff, static_cast<uint32_t>(row_of0(e)) * rowb);
+        if constexpr (MODE == 3) return v;
+        else return okey(v);
+      },
+      [&](int32_t x, S y0, S y1) {
+        if (!live) return;
+        S* r0 = B0 + (x - flushed) * rs + HB;
+        r0[tid] = y0;
+        if constexpr (MODE == 2) B1[(x - flushed) * rs + HB + tid] = y1;
+        if (edge_wave) {  // the row's reflections: e = -1 - c for c < HB, e = 2 n1 - 1 - c for c >= n1 - HB
+          if (tid < HB) {
+            r0[-1 - tid] = y0;
+            if constexpr (MODE == 2) B1[(x - flushed) * rs + HB - 1 - tid] = y1;
+          }
+          if (tid >= n1 - HB) {
+            r0[2 * n1 - 1 - tid] = y0;
+            if constexpr (MODE == 2) B1[(x - flushed) * rs + HB + 2 * n1 - 1 - tid] = y1;
+          }
+        }
+      },
+      rows);
+}
+
+template <class T, int MODE, bool MID>
+__global__ void __launch_bounds__(512) k_cube2(CPass a, VPtrs<T> ptr) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int64_t b = blockIdx.x / a.pieces;
+  const int32_t piece = static_cast<int32_t>(blockIdx.x - b * a.pieces);
+  const int64_t rung = blockIdx.y;
+  const int32_t w = ptr.win[rung * a.g.batch + b];
+  if (w < 1) return;  // uniform: the element does not run
+  // n0 >= 128 and n1 >= 64: the radius is (w - 1) / 2 on both axes for every window this kernel takes
+  const int32_t L = 2 * ((w - 1) / 2) + 1;
+#define EDA_CUBE2_CASE(LL) \
+  case LL: cube2_run<T, MODE, LL>(a, ptr, b, rung, piece, smem); break;
+  if constexpr (!MID) {
+    switch (L) {
+      EDA_CUBE2_CASE(1)
+      EDA_CUBE2_CASE(3)
+      EDA_CUBE2_CASE(5)
+      EDA_CUBE2_CASE(7)
+      EDA_CUBE2_CASE(9)
+      EDA_CUBE2_CASE(11)
+      EDA_CUBE2_CASE(13)
+      EDA_CUBE2_CASE(15)
+      default: break;  // another launch's element
+    }
+  } else {
+    switch (L) {
+      EDA_CUBE2_CASE(17)
+      EDA_CUBE2_CASE(19)
+      EDA_CUBE2_CASE(21)
+      EDA_CUBE2_CASE(23)
+      EDA_CUBE2_CASE(25)
+      EDA_CUBE2_CASE(27)
+      EDA_CUBE2_CASE(29)
+      EDA_CUBE2_CASE(31)
+      default: break;
+    }
+  }
+#undef EDA_CUBE2_CASE
+}
+
 // The streaming pass takes the elements of a strided axis whose windows are at most 63 samples when
 // the line has at least 128 samples, the pass does not run in place, one element's field is under
 // 4 GiB (32-bit buffer offsets), and its operators are one of max, min, box or max + min of one
@@ -1796,9 +2108,10 @@ bool stream_qualifies(const Geom& g, int axis, int nf, bool same_input, const in
 
 template <class T>
 void launch_stream(hipStream_t s, const Geom& g, int axis, int nf, const int* ops, bool scale, const VPtrs<T>& p,
-                   int32_t rungs, int32_t max_w, bool big) {
+                   int32_t rungs, int32_t max_w, bool big, int32_t rmin) {
   SPass a{};
   a.g = g;
+  a.rmin = rmin;
   a.n = g.n[axis];
   a.inner = g.st[axis];
   a.outer = g.count / (static_cast<int64_t>(a.n) * a.inner);
@@ -1849,6 +2162,104 @@ void launch_stream(hipStream_t s, const Geom& g, int axis, int nf, const int* op
   const std::string key = std::string("k_vhgw_stream:") + (sizeof(T) == 4 ? "f32" : "f64") + ":mode" +
                           std::to_string(a.mode) + (big ? ":big" : ":short") + ":n" + std::to_string(a.n) +
                           ":inner" + std::to_string(a.inner) + ":lines2^" + std::to_string(lb);
+  autotune(key, static_cast<int>(cands.size()), fallback, s, launch);
+}
+
+// ---- the fused 2-D cube's launches ----
+// LDS slots per staged row: the row, its two halos and a junk slot, odd.
+inline int32_t cube2_rs(int32_t n1, bool mid) { return (n1 + 2 * cube2_halo(mid ? 31 : 15) + 1) | 1; }
+
+// LDS of one workgroup: the staged rows of a batch (16, or 31 for windows 17 .. 31), per output field.
+template <class T>
+size_t cube2_lds(int32_t n1, int mode, bool mid) {
+  // the staged rows (two regions for max + min) and the output rows, CAP rows each
+  const int regions = mode == 2 ? 3 : 2;
+  const int cap = mid ? 31 : (mode == 2 ? 15 : 16);
+  return static_cast<size_t>(cap) * static_cast<size_t>(cube2_rs(n1, mid)) * static_cast<size_t>(regions) * sizeof(T);
+}
+
+// The fused cube takes a 2-D launch of n0 >= 128 rows and 64 <= n1 <= 512 columns whose input lies
+// outside every output field, when 32-bit offsets reach an element's outputs and the LDS holds a
+// batch of the longer class.
+template <class T>
+bool cube2_qualifies(const Geom& g, int mode, bool pair, const VPtrs<T>& p, int32_t rungs) {
+  if (g.dim != 2 || g.n[0] < 128 || g.n[1] < 64 || g.n[1] > 512 || g.st[1] != 1) return false;
+  const int pf = pair ? 2 : 1;
+  if (static_cast<double>(g.count) * pf * sizeof(T) >= 4294967295.0) return false;
+  if (cube2_lds<T>(g.n[1], mode, false) > 160 * 1024) return false;
+  const auto range = [&](const T* q, int64_t per, int64_t stride) {
+    const char* b0 = reinterpret_cast<const char*>(q);
+    return std::make_pair(b0, b0 + (static_cast<size_t>(per) + static_cast<size_t>(rungs - 1) * stride) * sizeof(T));
+  };
+  const auto ri = range(p.in[0], g.batch * g.count, p.in_stride);
+  for (int f = 0; f < (mode == 2 && !pair ? 2 : 1); ++f) {
+    const auto ro = range(p.out[f], g.batch * g.count * pf, p.out_stride);
+    if (ri.first < ro.second && ro.first < ri.second) return false;
+  }
+  return true;
+}
+
+template <class T>
+void launch_cube2(hipStream_t s, const Geom& g, int mode, bool pair, bool scale, const VPtrs<T>& p, int32_t rungs,
+                  bool mid) {
+  CPass a{};
+  a.g = g;
+  a.n0 = g.n[0];
+  a.n1 = g.n[1];
+  a.mode = mode;
+  a.pair = pair ? 1 : 0;
+  a.scale = scale ? 1 : 0;
+  a.rs = cube2_rs(a.n1, mid);
+  const size_t lds = cube2_lds<T>(a.n1, mode, mid);
+  {
+    const int64_t vs = 16 / static_cast<int64_t>(sizeof(T)) / (pair ? 2 : 1);
+    const auto al16 = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; };
+    a.vec = a.n1 % vs == 0 && al16(p.out[0]) && al16(p.out[1]) && (p.out_stride * sizeof(T)) % 16 == 0 &&
+                    (g.count * (pair ? 2 : 1) * sizeof(T)) % 16 == 0
+                ? 1
+                : 0;
+  }
+  const int waves = (a.n1 + 63) / 64;
+  // the candidate row pieces per element (powers of two, at least 32 rows each) and the rule's
+  // choice: the fewest that give two workgroups per CU over the launch
+  std::vector<int32_t> cands;
+  for (int32_t pc = 1; pc <= 16 && a.n0 / pc >= 32; pc *= 2) cands.push_back(pc);
+  int32_t rule = 1;
+  const int64_t want = 2 * static_cast<int64_t>(device_cus());
+  while (g.batch * rungs * rule < want && a.n0 / (2 * rule) >= 32 && rule < 16) rule *= 2;
+  int fallback = 0;
+  for (size_t c = 0; c < cands.size(); ++c)
+    if (cands[c] == rule) fallback = static_cast<int>(c);
+  const auto launch = [&](int c) {
+    CPass v = a;
+    v.pieces = cands[static_cast<size_t>(c)];
+    v.R = (v.n0 + v.pieces - 1) / v.pieces;
+    const int64_t wgs = g.batch * v.pieces;
+    if (wgs > 0x7fffffff) throw std::invalid_argument("eda: too many workgroups for one cube");
+    const dim3 grid(static_cast<unsigned>(wgs), static_cast<unsigned>(rungs));
+    const auto go = [&](auto k) {
+      if (lds > 65536)
+        EDA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    static_cast<int>(lds)));
+      EDA_GO(k, grid, dim3(64 * waves), lds, s, v, p);
+    };
+    const auto by_mode = [&](auto mid_tag) {
+      constexpr bool kMid = decltype(mid_tag)::value;
+      switch (mode) {
+        case 0: go(k_cube2<T, 0, kMid>); break;
+        case 1: go(k_cube2<T, 1, kMid>); break;
+        case 2: go(k_cube2<T, 2, kMid>); break;
+        default: go(k_cube2<T, 3, kMid>); break;
+      }
+    };
+    if (mid) by_mode(std::true_type{});
+    else by_mode(std::false_type{});
+  };
+  int lb = 0;
+  while ((int64_t{1} << lb) < g.batch * rungs) ++lb;
+  const std::string key = std::string("k_cube2:") + (sizeof(T) == 4 ? "f32" : "f64") + ":mode" + std::to_string(mode) +
+                          (pair ? ":pair" : "") + (mid ? ":mid" : ":small") + ":n" + std::to_string(a.n0) + "x" +
+                          std::to_string(a.n1) + ":elems2^" + std::to_string(lb);
   autotune(key, static_cast<int>(cands.size()), fallback, s, launch);
 }
 
@@ -2482,7 +2893,7 @@ void enqueue_staged(hipStream_t s, const Geom& g, int axis, int nf, bool same_in
 template <class T>
 void enqueue_vhgw(hipStream_t s, const Geom& g, int axis, int nf, bool same_input, const int* ops, bool scale,
                   const T* const* in, T* const* out, const int32_t* win, int32_t max_w, int32_t rungs,
-                  int64_t in_stride, int64_t out_stride, T* pairs = nullptr) {
+                  int64_t in_stride, int64_t out_stride, T* pairs = nullptr, int32_t rmin0 = 0) {
   if (max_w < 1 || rungs < 1) return;
   if (rungs > 65535) throw std::invalid_argument("eda: too many rungs for one launch");
   VPtrs<T> p{};
@@ -2502,12 +2913,15 @@ void enqueue_vhgw(hipStream_t s, const Geom& g, int axis, int nf, bool same_inpu
   }
   const bool timed = timer_on();
   if (rows_qualify(g, axis, sum)) {
+    // elements below rmin0 are the fused cube's; a launch with none of the others is not made
+    const int64_t above = rmin0 > 0 ? count_radius(win, g.batch * rungs, g.n[axis], rmin0, g.n[axis]) : -1;
+    if (above == 0) return;
     hipEvent_t t0 = nullptr;
     if (timed) timer_note(s, &t0);
-    launch_rows<T>(s, g, axis, nf, same_input, ops, p, rungs, il);
+    launch_rows<T>(s, g, axis, nf, same_input, ops, p, rungs, il, rmin0);
     EDA_LAUNCH("k_vhgw_rows");
     if (timed) {
-      int64_t active = mirror_active(win, g.batch * rungs);
+      int64_t active = above >= 0 ? above : mirror_active(win, g.batch * rungs);
       if (active < 0) active = g.batch * rungs;
       const int fields = (nf == 2 && !same_input ? 2 : 1) + nf;
       timer_done(s, t0, static_cast<double>(fields) * active * g.count * sizeof(T), kTagRows);
@@ -2529,10 +2943,11 @@ void enqueue_vhgw(hipStream_t s, const Geom& g, int axis, int nf, bool same_inpu
       const auto stream_class = [&](bool big) {
         hipEvent_t t0 = nullptr;
         if (timed) timer_note(s, &t0);
-        launch_stream<T>(s, g, axis, nf, ops, scale, p, rungs, std::min(max_w, big ? kStreamBigL : kStreamMaxL), big);
+        launch_stream<T>(s, g, axis, nf, ops, scale, p, rungs, std::min(max_w, big ? kStreamBigL : kStreamMaxL), big,
+                         rmin0);
         EDA_LAUNCH("k_vhgw_stream");
         if (timed) {
-          const int64_t c = count_radius(win, g.batch * rungs, g.n[axis], big ? 16 : 0, big ? 32 : 16);
+          const int64_t c = count_radius(win, g.batch * rungs, g.n[axis], big ? 16 : rmin0, big ? 32 : 16);
           const int fields = (nf == 2 && !same_input ? 2 : 1) + nf;
           timer_done(s, t0, static_cast<double>(fields) * (c < 0 ? g.batch * rungs : c) * g.count * sizeof(T),
                      kTagStream);
@@ -2541,14 +2956,14 @@ void enqueue_vhgw(hipStream_t s, const Geom& g, int axis, int nf, bool same_inpu
       // a class none of the launch's elements falls in is not launched (when the window array has
       // a host copy; else every class is)
       const int64_t items = g.batch * rungs;
-      if (count_radius(win, items, g.n[axis], 0, 16) != 0) stream_class(false);
+      if (rmin0 < 16 && count_radius(win, items, g.n[axis], rmin0, 16) != 0) stream_class(false);
       if (max_w > kStreamMaxL && count_radius(win, items, g.n[axis], 16, 32) != 0) stream_class(true);
       if (max_w > kStreamBigL && count_radius(win, items, g.n[axis], 32, g.n[axis]) != 0)
         enqueue_staged<T>(s, g, axis, nf, same_input, ops, scale, p, max_w, rungs, 32, timed);
       return;
     }
   }
-  enqueue_staged<T>(s, g, axis, nf, same_input, ops, scale, p, max_w, rungs, 0, timed);
+  enqueue_staged<T>(s, g, axis, nf, same_input, ops, scale, p, max_w, rungs, rmin0, timed);
 }
 
 // The cube (reference cpp/src/method/common.cpp:65-136): box runs axis 0 first, the last pass
@@ -2605,11 +3020,63 @@ void cube_targets(hipStream_t s, const Geom& g, const T* in, T* out0, int32_t ru
   }
 }
 
+// The fused 2-D cube for the launch's elements with windows up to 31 samples (two classes, each
+// launched when one of its elements runs). Returns the radius from which the two-pass path takes
+// the elements (0: the fused cube does not serve the launch; 16, or 8 when the longer class does
+// not fit the LDS: it served the shorter windows), or -1 when no element is left to the two-pass
+// path.
+template <class T>
+int32_t enqueue_cube2(hipStream_t s, const Geom& g, int mode, bool pair, bool scale, const VPtrs<T>& p,
+                      int32_t rungs, int32_t max_w) {
+  if (!cube2_qualifies<T>(g, mode, pair, p, rungs)) return 0;
+  const int64_t items = g.batch * rungs;
+  const bool timed = timer_on();
+  const int32_t n1 = g.n[1];  // radii below 16 are the same on both axes (n0 >= 128, n1 >= 64)
+  // windows 17 .. 31 of max + min stage 3 x 31 rows: wide rows leave them to the two-pass path
+  const bool mid_ok = cube2_lds<T>(n1, mode, true) <= 160 * 1024;
+  for (int cls = 0; cls < 2; ++cls) {
+    if (cls == 1 && (max_w < 17 || !mid_ok)) break;
+    const int64_t c = count_radius(p.win, items, n1, cls ? 8 : 0, cls ? 16 : 8);
+    if (c == 0) continue;
+    hipEvent_t t0 = nullptr;
+    if (timed) timer_note(s, &t0);
+    launch_cube2<T>(s, g, mode, pair, scale, p, rungs, cls == 1);
+    EDA_LAUNCH("k_cube2");
+    if (timed) {
+      const int fields = 1 + (mode == 2 ? 2 : 1);
+      timer_done(s, t0, static_cast<double>(fields) * (c < 0 ? items : c) * g.count * sizeof(T), kTagCube2);
+    }
+  }
+  const int32_t from = mid_ok ? 16 : 8;
+  if (max_w < 2 * from + 1) return -1;
+  return count_radius(p.win, items, n1, from, n1) == 0 ? -1 : from;
+}
+
 template <class T>
 bool enqueue_cube_rungs(hipStream_t s, const Geom& g, CubeOp op, const T* in, T* out0, T* out1, const int32_t* win,
                         int32_t rungs, int64_t stride, int32_t max_w, T* pairs, T* scratch) {
   if (max_w < 1) return false;
   const int D = g.dim;
+  // The max and min side by side: the streaming row pass writes them, so it runs last (axes 0 ..
+  // D - 1; max and min give the same bits in any order of the axes).
+  const bool il = pairs != nullptr && op == CubeOp::kMaxMin && D >= 2 && rows_qualify(g, D - 1, false);
+  // 2-D: the fused cube serves the windows up to 31 samples, the passes below the others
+  int32_t rmin0 = 0;
+  if (D == 2) {
+    VPtrs<T> p{};
+    p.in[0] = in;
+    p.in[1] = in;
+    p.out[0] = il ? pairs : out0;
+    p.out[1] = il ? pairs : (out1 != nullptr ? out1 : out0);
+    p.win = win;
+    p.in_stride = 0;
+    p.out_stride = il ? 2 * stride : stride;
+    const int mode = op == CubeOp::kBox ? 3 : op == CubeOp::kMaxMin ? 2 : op == CubeOp::kMin ? 1 : 0;
+    if (mode != 2 || EDA_CUBE2_MAXMIN) {
+      rmin0 = enqueue_cube2<T>(s, g, mode, il, op == CubeOp::kBox, p, rungs, max_w);
+      if (rmin0 < 0) return il;
+    }
+  }
   if (op == CubeOp::kBox) {
     T* dst[3];
     cube_targets<T>(s, g, in, out0, rungs, scratch, dst);
@@ -2617,13 +3084,11 @@ bool enqueue_cube_rungs(hipStream_t s, const Geom& g, CubeOp op, const T* in, T*
       const int ops[1] = {kVSum};
       const T* src[1] = {a == 0 ? in : dst[a - 1]};
       T* out[1] = {dst[a]};
-      enqueue_vhgw<T>(s, g, a, 1, true, ops, a == D - 1, src, out, win, max_w, rungs, a == 0 ? 0 : stride, stride);
+      enqueue_vhgw<T>(s, g, a, 1, true, ops, a == D - 1, src, out, win, max_w, rungs, a == 0 ? 0 : stride, stride,
+                      nullptr, rmin0);
     }
     return false;
   }
-  // The max and min side by side: the streaming row pass writes them, so it runs last (axes 0 ..
-  // D - 1; max and min give the same bits in any order of the axes).
-  const bool il = pairs != nullptr && op == CubeOp::kMaxMin && D >= 2 && rows_qualify(g, D - 1, false);
   T* dst[3];
   if (op != CubeOp::kMaxMin) cube_targets<T>(s, g, in, out0, rungs, scratch, dst);
   for (int k = 0; k < D; ++k) {
@@ -2637,12 +3102,12 @@ bool enqueue_cube_rungs(hipStream_t s, const Geom& g, CubeOp op, const T* in, T*
       const T* src[2] = {first ? in : out0, first ? in : out1};
       T* out[2] = {out0, out1};
       enqueue_vhgw<T>(s, g, a, 2, first, ops, false, src, out, win, max_w, rungs, ins, stride,
-                      il && k == D - 1 ? pairs : nullptr);
+                      il && k == D - 1 ? pairs : nullptr, rmin0);
     } else {
       const int ops[1] = {op == CubeOp::kMax ? kVMax : kVMin};
       const T* src[1] = {first ? in : dst[a - 1]};
       T* out[1] = {dst[a]};
-      enqueue_vhgw<T>(s, g, a, 1, true, ops, false, src, out, win, max_w, rungs, ins, stride);
+      enqueue_vhgw<T>(s, g, a, 1, true, ops, false, src, out, win, max_w, rungs, ins, stride, nullptr, rmin0);
     }
   }
   return il;
