@@ -2017,7 +2017,9 @@ __device__ __forceinline__ void cube2_run(const CPass& a, const VPtrs<T>& ptr, i
     } else {
       write_out(flushed, nr, std::integral_constant<int, 1>{});
     }
-    lds_sync();  // the batch's rows are out before the next batch overwrites them
+    // max + min wrote its min rows over B0, which the next batch stages into; otherwise the next
+    // batch writes B0 (read before the last barrier) and O only after its first barrier
+    if constexpr (MODE == 2) lds_sync();
     flushed = end;
   };
   const auto row_of0 = [&](int32_t e) { return e < 0 ? -1 - e : (e >= n0 ? 2 * n0 - 1 - e : e); };
