@@ -1780,7 +1780,7 @@ __global__ void __launch_bounds__(256) k_vhgw_stream(SPass a, VPtrs<T> ptr) {
 }
 
 #ifndef EDA_CUBE2_MAXMIN
-#define EDA_CUBE2_MAXMIN 0  // max + min cubes fused too (three staged regions: one workgroup per CU)
+#define EDA_CUBE2_MAXMIN 0  // max + min cubes with windows up to 7 samples fused too
 #endif
 // ---- the fused 2-D cube (windows up to 31 samples) --------------------------------------------------
 // Both passes of a 2-D max, min, max + min or box cube in one kernel, for fields of n0 >= 128 rows
@@ -1865,10 +1865,11 @@ __device__ __forceinline__ void walk_line(int32_t o, int32_t x_lo, int32_t x_hi,
   }
 }
 
-// Rows per batch of the fused cube: about 16 (12 for max + min, which stages three regions), or L.
+// Rows per batch of the fused cube: about 16 (8 for max + min, which stages three regions and
+// takes windows up to 7 samples), or L.
 template <int MODE, int L>
 constexpr int cube_u() {
-  return L >= (MODE == 2 ? 12 : 16) ? 1 : (MODE == 2 ? 12 : 16) / L;
+  return L >= (MODE == 2 ? 8 : 16) ? 1 : (MODE == 2 ? 8 : 16) / L;
 }
 template <int MODE, int L>
 constexpr int cube_cap() {
@@ -2061,7 +2062,17 @@ __global__ void __launch_bounds__(512) k_cube2(CPass a, VPtrs<T> ptr) {
   const int32_t L = 2 * ((w - 1) / 2) + 1;
 #define EDA_CUBE2_CASE(LL) \
   case LL: cube2_run<T, MODE, LL>(a, ptr, b, rung, piece, smem); break;
-  if constexpr (!MID) {
+  if constexpr (MODE == 2) {
+    if constexpr (!MID) {
+      switch (L) {  // max + min: windows up to 7 samples (the two-pass path takes the longer ones)
+        EDA_CUBE2_CASE(1)
+        EDA_CUBE2_CASE(3)
+        EDA_CUBE2_CASE(5)
+        EDA_CUBE2_CASE(7)
+        default: break;
+      }
+    }
+  } else if constexpr (!MID) {
     switch (L) {
       EDA_CUBE2_CASE(1)
       EDA_CUBE2_CASE(3)
@@ -2176,7 +2187,7 @@ template <class T>
 size_t cube2_lds(int32_t n1, int mode, bool mid) {
   // the staged rows (two regions for max + min) and the output rows, CAP rows each
   const int regions = mode == 2 ? 3 : 2;
-  const int cap = mid ? 31 : (mode == 2 ? 15 : 16);
+  const int cap = mid ? 31 : (mode == 2 ? 8 : 16);
   return static_cast<size_t>(cap) * static_cast<size_t>(cube2_rs(n1, mid)) * static_cast<size_t>(regions) * sizeof(T);
 }
 
@@ -3034,11 +3045,13 @@ int32_t enqueue_cube2(hipStream_t s, const Geom& g, int mode, bool pair, bool sc
   const int64_t items = g.batch * rungs;
   const bool timed = timer_on();
   const int32_t n1 = g.n[1];  // radii below 16 are the same on both axes (n0 >= 128, n1 >= 64)
-  // windows 17 .. 31 of max + min stage 3 x 31 rows: wide rows leave them to the two-pass path
-  const bool mid_ok = cube2_lds<T>(n1, mode, true) <= 160 * 1024;
+  // max + min: windows up to 7 samples (its three staged regions of 8 rows leave room for two
+  // workgroups per CU); the others: up to 15, and 17 .. 31 where the LDS holds 2 x 31 rows
+  const bool mid_ok = mode != 2 && cube2_lds<T>(n1, mode, true) <= 160 * 1024;
+  const int32_t small_hi = mode == 2 ? 4 : 8;
   for (int cls = 0; cls < 2; ++cls) {
     if (cls == 1 && (max_w < 17 || !mid_ok)) break;
-    const int64_t c = count_radius(p.win, items, n1, cls ? 8 : 0, cls ? 16 : 8);
+    const int64_t c = count_radius(p.win, items, n1, cls ? 8 : 0, cls ? 16 : small_hi);
     if (c == 0) continue;
     hipEvent_t t0 = nullptr;
     if (timed) timer_note(s, &t0);
@@ -3049,7 +3062,7 @@ int32_t enqueue_cube2(hipStream_t s, const Geom& g, int mode, bool pair, bool sc
       timer_done(s, t0, static_cast<double>(fields) * (c < 0 ? items : c) * g.count * sizeof(T), kTagCube2);
     }
   }
-  const int32_t from = mid_ok ? 16 : 8;
+  const int32_t from = mid_ok ? 16 : small_hi;
   if (max_w < 2 * from + 1) return -1;
   return count_radius(p.win, items, n1, from, n1) == 0 ? -1 : from;
 }
