@@ -1549,27 +1549,44 @@ void enqueue_eigen(hipStream_t s, const Geom& g, const T* t, const int32_t* on, 
   else if (g.dim == 3) eigen_pass<T, 3>(s, g, t, on, nvec, coh, nonfinite);
 }
 
-// T = R1 R1^T + R2 R2^T per voxel, written over R2 (pair order), then the eigen pass into n and c.
-template <class T>
-void tensor_eigen(hipStream_t s, const Geom& g, const T* R1, T* R2, const int32_t* on, T* nvec, T* coh,
-                  int32_t* nonfinite) {
-  const int D = g.dim;
-  const int K = D * (D + 1) / 2;
+// Per voxel the structure tensor T = R1 R1^T + R2 R2^T in T's arithmetic (pair order), then its
+// eigen pass into n and c, in one kernel (the tensor stays in registers; nvec may be R1's memory:
+// each voxel reads its R1 before it writes its n).
+template <class T, int D>
+void tensor_eigen_d(hipStream_t s, const Geom& g, const T* R1, const T* R2, const int32_t* on, T* nvec, T* coh,
+                    int32_t* nonfinite) {
+  constexpr int K = D * (D + 1) / 2;
   const int64_t F = g.batch * g.count;
   each(s, g, on, [=] __device__(int64_t b, int64_t i) {
     const int64_t at = b * g.count + i;
-    T r1[kMaxDim], r2[6], out[6];
+    T r1[D], r2[K];
+#pragma unroll
     for (int a = 0; a < D; ++a) r1[a] = R1[a * F + at];
+#pragma unroll
     for (int p = 0; p < K; ++p) r2[p] = R2[p * F + at];
+    double tv[K];
+#pragma unroll
     for (int a = 0; a < D; ++a)
+#pragma unroll
       for (int c = a; c < D; ++c) {
         T v = r1[a] * r1[c];
+#pragma unroll
         for (int q = 0; q < D; ++q) v = v + r2[tensor_pair(D, a, q)] * r2[tensor_pair(D, q, c)];
-        out[tensor_pair(D, a, c)] = v;
+        tv[tensor_pair(D, a, c)] = static_cast<double>(v);
       }
-    for (int p = 0; p < K; ++p) R2[p * F + at] = out[p];
+    double nv[D], cv;
+    const bool ok = eigen_voxel<D>(tv, nv, &cv);
+    if (!ok && nonfinite != nullptr) nonfinite[b] = 1;
+    coh[at] = static_cast<T>(cv);
+#pragma unroll
+    for (int a = 0; a < D; ++a) nvec[a * F + at] = static_cast<T>(nv[a]);
   });
-  enqueue_eigen<T>(s, g, R2, on, nvec, coh, nonfinite);
+}
+template <class T>
+void tensor_eigen(hipStream_t s, const Geom& g, const T* R1, const T* R2, const int32_t* on, T* nvec, T* coh,
+                  int32_t* nonfinite) {
+  if (g.dim == 2) tensor_eigen_d<T, 2>(s, g, R1, R2, on, nvec, coh, nonfinite);
+  else if (g.dim == 3) tensor_eigen_d<T, 3>(s, g, R1, R2, on, nvec, coh, nonfinite);
 }
 
 // Y = T(m X) for a kind of riesz_mult: the multiplier depends on the sample's position only, so
