@@ -1943,6 +1943,68 @@ void enqueue_orientation_vjp(hipStream_t s, int device, const Geom& g, const T* 
   });
 }
 
+// The planes of the monogenic signal per voxel (D a template constant: the small arrays stay in
+// registers): amplitude, phase, frequency, then (D >= 2) the direction with the canonical sign and
+// the coherence.
+template <class T, int D>
+void monogenic_planes(hipStream_t s, const Geom& g, const T* f, const int32_t* kexp, double tiny, const T* R1,
+                      const T* DF, const T* R2, const T* NV, const T* C, T* out, int64_t out_batch, int64_t b0) {
+  const int64_t F = g.batch * g.count;
+  const T zero = std::is_same_v<T, float> ? T(0x1p-14) : T(0x1p-30);
+  const int64_t plane = out_batch * g.count;
+  const T pi_t = static_cast<T>(3.141592653589793);
+  each(s, g, nullptr, [=] __device__(int64_t b, int64_t i) {
+    const int64_t at = b * g.count + i;
+    T nt[D];
+    if constexpr (D == 1) {
+      nt[0] = T(1);
+    } else {
+      int sgn = 0;
+#pragma unroll
+      for (int a = 0; a < D; ++a) {
+        nt[a] = NV[a * F + at];
+        if (sgn == 0 && fabs(nt[a]) > zero) sgn = nt[a] > T(0) ? 1 : -1;
+      }
+#pragma unroll
+      for (int a = 0; a < D; ++a) nt[a] = sgn < 0 ? -nt[a] : nt[a];
+    }
+    double n[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) n[a] = static_cast<double>(nt[a]);
+    const double fv = static_cast<double>(f[at]);
+    double q = 0.0, a2 = fv * fv, dnf = 0.0, dnq = 0.0;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      const double r = static_cast<double>(R1[a * F + at]);
+      q = q + n[a] * r;
+      a2 = a2 + r * r;
+      dnf = dnf + n[a] * static_cast<double>(DF[a * F + at]);
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+      for (int c = a; c < D; ++c) {
+        const double m = a == c ? n[a] * n[a] : 2.0 * (n[a] * n[c]);
+        dnq = dnq + m * static_cast<double>(R2[(D == 1 ? 0 : tensor_pair(D, a, c)) * F + at]);
+      }
+    double phase = 0.0, freq = 0.0;
+    if (a2 > tiny) {
+      phase = atan2(q, fv);
+      freq = (fv * dnq - q * dnf) / a2;
+    }
+    const int64_t ot = (b0 + b) * g.count + i;
+    out[ot] = static_cast<T>(ldexp(sqrt(a2), kexp[b]));
+    const T pt = static_cast<T>(phase);
+    out[plane + ot] = pt == -pi_t ? pi_t : pt;
+    out[2 * plane + ot] = static_cast<T>(freq);
+    if constexpr (D >= 2) {
+#pragma unroll
+      for (int a = 0; a < D; ++a) out[3 * plane + ((b0 + b) * D + a) * g.count + i] = nt[a];
+      out[(3 + D) * plane + ot] = C[at];
+    }
+  });
+}
+
 // The monogenic signal of every element of f (the prescaled fields f 2^-kexp[b]) into the planes
 // of out (reference orientation.cu:420-493,789-831, monogenic.cpp:61-158).
 template <class T>
@@ -1982,53 +2044,9 @@ void enqueue_monogenic(hipStream_t s, int device, const Geom& g, const T* f, con
   // per voxel, in double: q = n . R1, A^2 = f^2 + |R1|^2, D_n f, D_n q; the phase and the frequency
   // where A^2 > tiny (else 0); the amplitude sqrt(A^2) 2^kexp rounded once; a phase that rounds to
   // -pi in T is pi
-  const T zero = std::is_same_v<T, float> ? T(0x1p-14) : T(0x1p-30);
-  const int64_t plane = out_batch * g.count;
-  const T pi_t = static_cast<T>(3.141592653589793);
-  each(s, g, nullptr, [=] __device__(int64_t b, int64_t i) {
-    const int64_t at = b * g.count + i;
-    T nt[kMaxDim];
-    if (D == 1) {
-      nt[0] = T(1);
-    } else {
-      int sgn = 0;
-      for (int a = 0; a < D; ++a) {
-        nt[a] = NV[a * F + at];
-        if (sgn == 0 && fabs(nt[a]) > zero) sgn = nt[a] > T(0) ? 1 : -1;
-      }
-      for (int a = 0; a < D; ++a) nt[a] = sgn < 0 ? -nt[a] : nt[a];
-    }
-    double n[kMaxDim];
-    for (int a = 0; a < D; ++a) n[a] = static_cast<double>(nt[a]);
-    const double fv = static_cast<double>(f[at]);
-    double q = 0.0, a2 = fv * fv, dnf = 0.0, dnq = 0.0;
-    for (int a = 0; a < D; ++a) {
-      const double r = static_cast<double>(R1[a * F + at]);
-      q = q + n[a] * r;
-      a2 = a2 + r * r;
-      dnf = dnf + n[a] * static_cast<double>(DF[a * F + at]);
-    }
-    int p = 0;
-    for (int a = 0; a < D; ++a)
-      for (int c = a; c < D; ++c, ++p) {
-        const double m = a == c ? n[a] * n[a] : 2.0 * (n[a] * n[c]);
-        dnq = dnq + m * static_cast<double>(R2[(D == 1 ? 0 : tensor_pair(D, a, c)) * F + at]);
-      }
-    double phase = 0.0, freq = 0.0;
-    if (a2 > tiny) {
-      phase = atan2(q, fv);
-      freq = (fv * dnq - q * dnf) / a2;
-    }
-    const int64_t ot = (b0 + b) * g.count + i;
-    out[ot] = static_cast<T>(ldexp(sqrt(a2), kexp[b]));
-    const T pt = static_cast<T>(phase);
-    out[plane + ot] = pt == -pi_t ? pi_t : pt;
-    out[2 * plane + ot] = static_cast<T>(freq);
-    if (D >= 2) {
-      for (int a = 0; a < D; ++a) out[3 * plane + ((b0 + b) * D + a) * g.count + i] = nt[a];
-      out[(3 + D) * plane + ot] = C[at];
-    }
-  });
+  if (D == 1) monogenic_planes<T, 1>(s, g, f, kexp, tiny, R1, DF, R2, NV, C, out, out_batch, b0);
+  else if (D == 2) monogenic_planes<T, 2>(s, g, f, kexp, tiny, R1, DF, R2, NV, C, out, out_batch, b0);
+  else monogenic_planes<T, 3>(s, g, f, kexp, tiny, R1, DF, R2, NV, C, out, out_batch, b0);
 }
 
 template const AxisMats<float>& axis_mats<float>(int, int32_t);
