@@ -5,7 +5,7 @@
 // Layout. A call's working field is C-contiguous (B, n_0 .. n_{D-1}) with no padding: element b's
 // samples start at b * count. The reference pads the last axis to a multiple of 32
 // (cpp/src/field/geom.hpp:9-23); the padding changes no result, and wave64 rows of 512 samples are
-// aligned either way, so this build keeps the dense layout and never tests for pad samples.
+// aligned either way, so the library keeps the dense layout and never tests for pad samples.
 //
 // Arithmetic. Every floating-point operation that feeds a result is a single IEEE operation in a
 // fixed order. All HIP code is compiled with -ffp-contract=off, so the compiler never fuses a

@@ -10,10 +10,11 @@
 //   (u_a = k_a / (2 n_a), theta_a = pi k_a (2 x_a + 1) / (2 n_a), c = prod_a c_{k_a}),
 //   T = R1 R1^T + R2 R2^T per voxel, and its Jacobi eigen-decomposition into the top eigenvector n
 //   and the coherence c = (mu1 - mu2) / (mu1 + mu2).
-// The reference computes the transforms with cuFFT through Makhoul's permutation; this build
-// contracts every axis with its dense DCT matrix instead, as batched GEMMs: the DCT of an n-sample
-// axis is an (n x n) matrix, so a transform of a (rows x n) block is one GEMM, with no library and
-// no permutation passes. The matrices are built on the device from cos / sin tables of
+// The reference computes the transforms with cuFFT through Makhoul's permutation. Here a
+// power-of-two axis runs Makhoul's reordering and an LDS Stockham FFT (k_dct_fft); other axes are
+// contracted with their dense DCT matrix as batched GEMMs (the DCT of an n-sample axis is an n x n
+// matrix, so a transform of a rows x n block is one GEMM); dct="bf16x3" contracts in split-BF16 MFMA
+// products. The matrices and twiddles are built on the device from cos / sin tables of
 // pi j / (2n), j < 4n, evaluated in long double on the host.
 #include "common.hpp"
 #include "kernels.hpp"
